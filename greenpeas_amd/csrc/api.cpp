@@ -1,0 +1,612 @@
+// api.cpp -- driver, batch and gradient-plan objects behind the C-ABI
+// declared in include/greenpeas_amd.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cinttypes>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <thread>
+#include <type_traits>
+#include <vector>
+
+#include "../../include/greenpeas_amd.h"
+#include "circuit.hpp"
+#include "engine.hpp"
+#include "stepg.hpp"
+
+using gpa::InvalidArgument;
+using gpa::RuntimeError;
+
+namespace {
+
+thread_local std::string g_lastError;
+
+template <typename F>
+int guarded(F &&f) {
+  try {
+    g_lastError.clear();
+    f();
+    return GP_OK;
+  } catch (const std::invalid_argument &e) {
+    g_lastError = e.what();
+    return GP_INVALID_ARGUMENT;
+  } catch (const std::exception &e) {
+    g_lastError = e.what();
+    return GP_RUNTIME_ERROR;
+  } catch (...) {
+    g_lastError = "unknown error";
+    return GP_RUNTIME_ERROR;
+  }
+}
+
+void checkLevel(int level) {
+  if (level < 0 || level > 2) throw InvalidArgument("Unsupported correlation level.");
+}
+
+void requireText(const char *text) {
+  if (!text) throw InvalidArgument("circuit text is NULL");
+}
+
+// DEM text in stim's format: one "error(p) D.. L.." line per error.
+size_t formatDem(const gpa::VariantResult &r, std::string &out) {
+  out.clear();
+  out.reserve(static_cast<size_t>(r.numErrors) * 40 + r.numTargets * 6);
+  char buf[64];
+  for (uint32_t e = 0; e < r.numErrors; ++e) {
+    int n = std::snprintf(buf, sizeof buf, "error(%.17g)", r.probs[e]);
+    out.append(buf, n);
+    for (uint32_t k = r.offsets[e]; k < r.offsets[e + 1]; ++k) {
+      const uint32_t t = r.targets[k];
+      if (t < r.numDetectors) n = std::snprintf(buf, sizeof buf, " D%u", t);
+      else n = std::snprintf(buf, sizeof buf, " L%u", t - r.numDetectors);
+      out.append(buf, n);
+    }
+    out.push_back('\n');
+  }
+  return out.size();
+}
+
+struct Parsed {
+  gpa::Circuit circuit;
+  gpa::Stepg stepg;
+  gpa::StepgImage image;
+};
+
+void parseInto(const char *text, int level, Parsed &p, bool keepComponents = false,
+               const double *theta = nullptr) {
+  p.circuit = gpa::Circuit::parse(text);
+  gpa::buildStepg(p.circuit, level, p.stepg, theta, keepComponents);
+  gpa::buildImage(p.stepg, p.image);
+}
+
+}  // namespace
+
+struct gp_driver {
+  int level = 2;
+  int device = 0;
+  Parsed parsed;
+  std::unique_ptr<gpa::Pipeline> pipe;
+  bool haveCircuit = false, loaded = false, haveResult = false;
+  gpa::VariantResult result;
+  double wallUs = 0;
+};
+
+struct gp_batch {
+  int level = 2;
+  int device = 0;
+  std::vector<std::unique_ptr<Parsed>> circuits;
+  std::unique_ptr<gpa::Pipeline> pipe;
+  std::unique_ptr<gpa::Pipeline> single;  // one-by-one baseline
+  std::vector<gpa::VariantResult> results;
+  std::vector<std::vector<double>> ownP;
+  std::vector<std::vector<uint32_t>> ownOff, ownTg;
+  std::vector<std::vector<uint64_t>> ownHash;
+  double deviceUs = 0;
+  bool loaded = false;
+};
+
+struct gp_grad_plan {
+  int device = 0;
+  Parsed parsed;
+  std::unique_ptr<gpa::Pipeline> pipe;
+  gpa::VariantResult result;
+  uint32_t numLoc = 0, numComp = 0, numNodes = 0, numErrors = 0;
+  uint32_t *dCompLoc = nullptr, *dNodeStart = nullptr, *dNodeComp = nullptr, *dClassStart = nullptr,
+           *dClassNode = nullptr;
+  uint8_t *dCompKind = nullptr;
+  ~gp_grad_plan() {
+    for (void *p : {static_cast<void *>(dCompLoc), static_cast<void *>(dNodeStart),
+                    static_cast<void *>(dNodeComp), static_cast<void *>(dClassStart),
+                    static_cast<void *>(dClassNode), static_cast<void *>(dCompKind)})
+      if (p) (void)hipFree(p);
+  }
+};
+
+extern "C" {
+
+const char *gp_last_error(void) { return g_lastError.c_str(); }
+
+const char *gp_version(void) { return "greenpeas_amd 0.1.0 (gfx950)"; }
+
+int gp_get_driver(const char *circuit_text, int correlation_level, int device, gp_driver **out) {
+  return guarded([&] {
+    if (!out) throw InvalidArgument("out is NULL");
+    requireText(circuit_text);
+    checkLevel(correlation_level);
+    auto d = std::make_unique<gp_driver>();
+    d->level = correlation_level;
+    d->device = device;
+    parseInto(circuit_text, correlation_level, d->parsed);
+    d->pipe = std::make_unique<gpa::Pipeline>(device, false);
+    d->pipe->reserve({&d->parsed.image});
+    d->haveCircuit = true;
+    d->loaded = true;
+    *out = d.release();
+  });
+}
+
+int gp_driver_destroy(gp_driver *driver) {
+  return guarded([&] { delete driver; });
+}
+
+int gp_driver_parse(gp_driver *driver, const char *circuit_text) {
+  return guarded([&] {
+    if (!driver) throw InvalidArgument("driver is NULL");
+    requireText(circuit_text);
+    parseInto(circuit_text, driver->level, driver->parsed);
+    driver->haveCircuit = true;
+    driver->loaded = false;
+  });
+}
+
+int gp_compile_detector_error_model(gp_driver *driver, const char *circuit_text) {
+  return guarded([&] {
+    if (!driver) throw InvalidArgument("driver is NULL");
+    const auto t0 = std::chrono::steady_clock::now();
+    if (circuit_text) {
+      parseInto(circuit_text, driver->level, driver->parsed);
+      driver->haveCircuit = true;
+      driver->loaded = false;
+    }
+    if (!driver->haveCircuit) throw RuntimeError("no circuit parsed");
+    driver->haveResult = false;
+    // upload the STEPG (Driver.hpp:145-152), run the stages, copy back
+    driver->pipe->load({&driver->parsed.image});
+    driver->loaded = true;
+    driver->pipe->run(true);
+    driver->pipe->download();
+    driver->pipe->result(0, driver->result);
+    driver->haveResult = true;
+    driver->wallUs =
+        std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
+  });
+}
+
+int gp_dem_shape(const gp_driver *driver, uint32_t *num_errors, uint64_t *num_targets,
+                 uint32_t *num_detectors, uint32_t *num_observables) {
+  return guarded([&] {
+    if (!driver || !driver->haveResult) throw RuntimeError("no compiled detector error model");
+    const auto &r = driver->result;
+    if (num_errors) *num_errors = r.numErrors;
+    if (num_targets) *num_targets = r.numTargets;
+    if (num_detectors) *num_detectors = r.numDetectors;
+    if (num_observables) *num_observables = r.numObservables;
+  });
+}
+
+int gp_dem_copy(const gp_driver *driver, double *probs, uint32_t *offsets, uint32_t *targets,
+                uint64_t *hashes) {
+  return guarded([&] {
+    if (!driver || !driver->haveResult) throw RuntimeError("no compiled detector error model");
+    const auto &r = driver->result;
+    if (probs) std::memcpy(probs, r.probs, 8ull * r.numErrors);
+    if (offsets) std::memcpy(offsets, r.offsets, 4ull * (r.numErrors + 1));
+    if (targets) std::memcpy(targets, r.targets, 4ull * r.numTargets);
+    if (hashes) std::memcpy(hashes, r.hashes, 8ull * r.numErrors);
+  });
+}
+
+int gp_dem_text(const gp_driver *driver, char *buf, size_t cap, size_t *len) {
+  return guarded([&] {
+    if (!driver || !driver->haveResult) throw RuntimeError("no compiled detector error model");
+    std::string s;
+    formatDem(driver->result, s);
+    if (len) *len = s.size();
+    if (buf && cap > s.size()) std::memcpy(buf, s.c_str(), s.size() + 1);
+  });
+}
+
+int gp_driver_params(const gp_driver *driver, uint32_t params[7]) {
+  return guarded([&] {
+    if (!driver || !params) throw InvalidArgument("NULL argument");
+    const auto &c = driver->parsed.circuit;
+    params[0] = c.numQubits;
+    params[1] = c.numTicks + 1;
+    params[2] = c.numMeasurements;
+    params[3] = c.numDetectors;
+    params[4] = c.numObservables;
+    params[5] = driver->parsed.stepg.numNodes;
+    params[6] = c.numLocations;
+  });
+}
+
+int gp_driver_timings(const gp_driver *driver, double t[8]) {
+  return guarded([&] {
+    if (!driver || !t) throw InvalidArgument("NULL argument");
+    const auto &s = driver->pipe->times();
+    t[0] = s.upload;
+    t[1] = s.sweep;
+    t[2] = s.classify;
+    t[3] = s.partition;
+    t[4] = s.buckets;
+    t[5] = s.emit;
+    t[6] = s.download;
+    t[7] = driver->wallUs;
+  });
+}
+
+int gp_driver_bench_device(gp_driver *driver, uint32_t iterations, double *mean_us) {
+  return guarded([&] {
+    if (!driver || !mean_us) throw InvalidArgument("NULL argument");
+    if (!driver->haveCircuit) throw RuntimeError("no circuit parsed");
+    if (!driver->loaded) {
+      driver->pipe->load({&driver->parsed.image});
+      driver->loaded = true;
+    }
+    *mean_us = driver->pipe->benchDevice(iterations);
+  });
+}
+
+int gp_debug_pnodes(const gp_driver *driver, uint32_t *num_pnodes, uint32_t *words64) {
+  return guarded([&] {
+    if (!driver) throw InvalidArgument("driver is NULL");
+    const auto &img = driver->parsed.image;
+    if (num_pnodes) *num_pnodes = img.numRanks;
+    if (words64) *words64 = (img.numDetectors + img.numObservables + 63) / 64;
+  });
+}
+
+int gp_debug_sensitivities(const gp_driver *driver, uint32_t *pnode_index, uint64_t *sens) {
+  return guarded([&] {
+    if (!driver || !driver->loaded) throw RuntimeError("no compiled circuit");
+    const auto &img = driver->parsed.image;
+    if (pnode_index) std::memcpy(pnode_index, img.rankNode.data(), 4ull * img.numRanks);
+    if (sens) {
+      std::vector<uint32_t> cols;
+      driver->pipe->debugSensitivities(0, cols);
+      const uint32_t C = img.numCols, W = (img.numDetectors + img.numObservables + 63) / 64;
+      for (uint32_t i = 0; i < img.numRanks; ++i)
+        for (uint32_t w = 0; w < W; ++w) {
+          const uint64_t lo = 2 * w < C ? cols[static_cast<size_t>(i) * C + 2 * w] : 0u;
+          const uint64_t hi = 2 * w + 1 < C ? cols[static_cast<size_t>(i) * C + 2 * w + 1] : 0u;
+          sens[static_cast<size_t>(i) * W + w] = lo | (hi << 32);
+        }
+    }
+  });
+}
+
+int gp_debug_stepg(const gp_driver *driver, uint64_t *adjacency, double *probs) {
+  return guarded([&] {
+    if (!driver) throw InvalidArgument("driver is NULL");
+    const auto &g = driver->parsed.stepg;
+    if (adjacency) std::memcpy(adjacency, g.adjacency.data(), 8ull * g.numNodes);
+    if (probs) std::memcpy(probs, g.probs.data(), 8ull * g.numNodes);
+  });
+}
+
+int gp_build_stepg_host(const char *circuit_text, int correlation_level, uint32_t dims[4],
+                        uint64_t *adjacency, double *probs) {
+  return guarded([&] {
+    requireText(circuit_text);
+    checkLevel(correlation_level);
+    if (!dims) throw InvalidArgument("dims is NULL");
+    gpa::Circuit c = gpa::Circuit::parse(circuit_text);
+    gpa::Stepg g;
+    gpa::buildStepg(c, correlation_level, g);
+    dims[0] = g.numLayers;
+    dims[1] = g.npl;
+    dims[2] = g.numNodes;
+    dims[3] = g.numLocations;
+    if (adjacency) std::memcpy(adjacency, g.adjacency.data(), 8ull * g.numNodes);
+    if (probs) std::memcpy(probs, g.probs.data(), 8ull * g.numNodes);
+  });
+}
+
+// ---------------------------------------------------------------------------
+// batch
+// ---------------------------------------------------------------------------
+
+int gp_batch_create(int correlation_level, int device, gp_batch **out) {
+  return guarded([&] {
+    if (!out) throw InvalidArgument("out is NULL");
+    checkLevel(correlation_level);
+    auto b = std::make_unique<gp_batch>();
+    b->level = correlation_level;
+    b->device = device;
+    b->pipe = std::make_unique<gpa::Pipeline>(device, true);
+    *out = b.release();
+  });
+}
+
+int gp_batch_destroy(gp_batch *batch) {
+  return guarded([&] { delete batch; });
+}
+
+int gp_batch_set_circuits(gp_batch *batch, const char *const *texts, uint32_t count, uint32_t threads) {
+  return guarded([&] {
+    if (!batch || (!texts && count)) throw InvalidArgument("NULL argument");
+    for (uint32_t i = 0; i < count; ++i) requireText(texts[i]);
+    std::vector<std::unique_ptr<Parsed>> cs(count);
+    for (auto &c : cs) c = std::make_unique<Parsed>();
+    uint32_t nt = threads ? threads : std::max(1u, std::thread::hardware_concurrency());
+    nt = std::min<uint32_t>(nt, std::max<uint32_t>(count, 1));
+    std::vector<std::string> errs(count);
+    std::vector<int> kinds(count, 0);
+    std::atomic<uint32_t> next{0};
+    auto work = [&] {
+      for (uint32_t i; (i = next.fetch_add(1)) < count;) {
+        try {
+          parseInto(texts[i], batch->level, *cs[i]);
+        } catch (const std::invalid_argument &e) {
+          errs[i] = e.what();
+          kinds[i] = 1;
+        } catch (const std::exception &e) {
+          errs[i] = e.what();
+          kinds[i] = 2;
+        }
+      }
+    };
+    std::vector<std::thread> pool;
+    for (uint32_t t = 1; t < nt; ++t) pool.emplace_back(work);
+    work();
+    for (auto &t : pool) t.join();
+    for (uint32_t i = 0; i < count; ++i) {
+      if (kinds[i] == 1) throw InvalidArgument("circuit " + std::to_string(i) + ": " + errs[i]);
+      if (kinds[i] == 2) throw RuntimeError("circuit " + std::to_string(i) + ": " + errs[i]);
+    }
+    batch->circuits = std::move(cs);
+    batch->results.clear();
+    batch->loaded = false;
+  });
+}
+
+static void keepResult(gp_batch *b, uint32_t i, const gpa::VariantResult &r) {
+  b->ownP[i].assign(r.probs, r.probs + r.numErrors);
+  b->ownOff[i].assign(r.offsets, r.offsets + r.numErrors + 1);
+  b->ownTg[i].assign(r.targets, r.targets + r.numTargets);
+  b->ownHash[i].assign(r.hashes, r.hashes + r.numErrors);
+  gpa::VariantResult &k = b->results[i];
+  k.numErrors = r.numErrors;
+  k.numTargets = r.numTargets;
+  k.numDetectors = r.numDetectors;
+  k.numObservables = r.numObservables;
+  k.probs = b->ownP[i].data();
+  k.offsets = b->ownOff[i].data();
+  k.targets = b->ownTg[i].data();
+  k.hashes = b->ownHash[i].data();
+}
+
+int gp_batch_compile(gp_batch *batch, int mode) {
+  return guarded([&] {
+    if (!batch) throw InvalidArgument("batch is NULL");
+    const uint32_t n = static_cast<uint32_t>(batch->circuits.size());
+    if (!n) throw RuntimeError("batch has no circuits");
+    batch->results.assign(n, gpa::VariantResult{});
+    batch->ownP.assign(n, {});
+    batch->ownOff.assign(n, {});
+    batch->ownTg.assign(n, {});
+    batch->ownHash.assign(n, {});
+    if (mode == 0) {
+      if (!batch->loaded) {
+        std::vector<const gpa::StepgImage *> imgs;
+        for (auto &c : batch->circuits) imgs.push_back(&c->image);
+        batch->pipe->load(imgs);
+        batch->loaded = true;
+      }
+      batch->deviceUs = batch->pipe->benchDevice(1);
+      batch->pipe->download();
+      for (uint32_t i = 0; i < n; ++i) {
+        gpa::VariantResult r;
+        batch->pipe->result(i, r);
+        keepResult(batch, i, r);
+      }
+    } else if (mode == 1) {
+      if (!batch->single) batch->single = std::make_unique<gpa::Pipeline>(batch->device, true);
+      double total = 0;
+      for (uint32_t i = 0; i < n; ++i) {
+        batch->single->load({&batch->circuits[i]->image});
+        total += batch->single->benchDevice(1);
+        batch->single->download();
+        gpa::VariantResult r;
+        batch->single->result(0, r);
+        keepResult(batch, i, r);
+      }
+      batch->deviceUs = total;
+    } else {
+      throw InvalidArgument("mode must be 0 (batched) or 1 (one by one)");
+    }
+  });
+}
+
+int gp_batch_device_us(const gp_batch *batch, double *us) {
+  return guarded([&] {
+    if (!batch || !us) throw InvalidArgument("NULL argument");
+    *us = batch->deviceUs;
+  });
+}
+
+int gp_batch_dem_shape(const gp_batch *batch, uint32_t index, uint32_t *num_errors,
+                       uint64_t *num_targets, uint32_t *num_detectors, uint32_t *num_observables) {
+  return guarded([&] {
+    if (!batch || index >= batch->results.size()) throw InvalidArgument("index out of range");
+    const auto &r = batch->results[index];
+    if (num_errors) *num_errors = r.numErrors;
+    if (num_targets) *num_targets = r.numTargets;
+    if (num_detectors) *num_detectors = r.numDetectors;
+    if (num_observables) *num_observables = r.numObservables;
+  });
+}
+
+int gp_batch_dem_copy(const gp_batch *batch, uint32_t index, double *probs, uint32_t *offsets,
+                      uint32_t *targets, uint64_t *hashes) {
+  return guarded([&] {
+    if (!batch || index >= batch->results.size()) throw InvalidArgument("index out of range");
+    const auto &r = batch->results[index];
+    if (probs) std::memcpy(probs, r.probs, 8ull * r.numErrors);
+    if (offsets) std::memcpy(offsets, r.offsets, 4ull * (r.numErrors + 1));
+    if (targets) std::memcpy(targets, r.targets, 4ull * r.numTargets);
+    if (hashes) std::memcpy(hashes, r.hashes, 8ull * r.numErrors);
+  });
+}
+
+int gp_batch_device_bytes(const gp_batch *batch, uint64_t *bytes) {
+  return guarded([&] {
+    if (!batch || !bytes) throw InvalidArgument("NULL argument");
+    *bytes = batch->pipe->deviceBytes() + (batch->single ? batch->single->deviceBytes() : 0);
+  });
+}
+
+int gp_estimate_device_bytes(const char *circuit_text, int correlation_level, uint64_t *bytes) {
+  return guarded([&] {
+    requireText(circuit_text);
+    checkLevel(correlation_level);
+    if (!bytes) throw InvalidArgument("bytes is NULL");
+    Parsed p;
+    parseInto(circuit_text, correlation_level, p);
+    *bytes = gpa::imageDeviceBytes(p.image);
+  });
+}
+
+// ---------------------------------------------------------------------------
+// differentiable plan
+// ---------------------------------------------------------------------------
+
+int gp_grad_plan_create(const char *circuit_text, int correlation_level, int device,
+                        gp_grad_plan **out) {
+  return guarded([&] {
+    if (!out) throw InvalidArgument("out is NULL");
+    requireText(circuit_text);
+    checkLevel(correlation_level);
+    auto g = std::make_unique<gp_grad_plan>();
+    g->device = device;
+    // Structure only: every location gets the same positive rate so each
+    // node that carries a noise component is a p>0 node of the plan.
+    gpa::Circuit c = gpa::Circuit::parse(circuit_text);
+    std::vector<double> unit(c.numLocations, 1e-3);
+    g->parsed.circuit = std::move(c);
+    gpa::buildStepg(g->parsed.circuit, correlation_level, g->parsed.stepg, unit.data(), true);
+    gpa::buildImage(g->parsed.stepg, g->parsed.image);
+    g->pipe = std::make_unique<gpa::Pipeline>(device, true);
+    g->pipe->load({&g->parsed.image});
+    g->pipe->run(false);
+    g->pipe->download();
+    g->pipe->result(0, g->result);
+
+    const auto &st = g->parsed.stepg;
+    const auto &img = g->parsed.image;
+    g->numLoc = st.numLocations;
+    g->numComp = static_cast<uint32_t>(st.compNode.size());
+    g->numNodes = img.numRanks;
+    g->numErrors = g->result.numErrors;
+    // node -> rank, components grouped by rank (stable in circuit order)
+    std::vector<uint32_t> rankOf(st.numNodes, gpa::kNone);
+    for (uint32_t i = 0; i < img.numRanks; ++i) rankOf[img.rankNode[i]] = i;
+    std::vector<uint32_t> nodeStart(img.numRanks + 1, 0), nodeComp(g->numComp);
+    for (uint32_t k = 0; k < g->numComp; ++k) {
+      const uint32_t r = rankOf[st.compNode[k]];
+      if (r == gpa::kNone) throw RuntimeError("noise component on a node outside the plan");
+      nodeStart[r + 1]++;
+    }
+    for (uint32_t i = 0; i < img.numRanks; ++i) nodeStart[i + 1] += nodeStart[i];
+    std::vector<uint32_t> cur(nodeStart.begin(), nodeStart.end() - 1);
+    for (uint32_t k = 0; k < g->numComp; ++k) nodeComp[cur[rankOf[st.compNode[k]]]++] = k;
+
+    if (hipSetDevice(device) != hipSuccess) throw RuntimeError("hipSetDevice failed");
+    auto up = [&](auto **dst, const auto *src, size_t count) {
+      using T = std::remove_pointer_t<std::remove_reference_t<decltype(*dst)>>;
+      if (hipMalloc(reinterpret_cast<void **>(dst), std::max<size_t>(count, 1) * sizeof(T)) != hipSuccess)
+        throw RuntimeError("hipMalloc failed");
+      if (count && hipMemcpy(*dst, src, count * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+        throw RuntimeError("hipMemcpy failed");
+    };
+    up(&g->dCompLoc, st.compLoc.data(), g->numComp);
+    up(&g->dCompKind, st.compKind.data(), g->numComp);
+    up(&g->dNodeStart, nodeStart.data(), nodeStart.size());
+    up(&g->dNodeComp, nodeComp.data(), nodeComp.size());
+    // class members: the pipeline's bucket-sorted ranks, grouped by class
+    if (hipMalloc(reinterpret_cast<void **>(&g->dClassStart), 4ull * (g->numErrors + 1)) != hipSuccess ||
+        hipMalloc(reinterpret_cast<void **>(&g->dClassNode), 4ull * std::max<uint32_t>(img.numRanks, 1)) !=
+            hipSuccess)
+      throw RuntimeError("hipMalloc failed");
+    if (hipMemcpy(g->dClassStart, g->pipe->devClassMember(), 4ull * (g->numErrors + 1),
+                  hipMemcpyDeviceToDevice) != hipSuccess ||
+        hipMemcpy(g->dClassNode, g->pipe->devSortedRanks(), 4ull * img.numRanks,
+                  hipMemcpyDeviceToDevice) != hipSuccess)
+      throw RuntimeError("hipMemcpy failed");
+    *out = g.release();
+  });
+}
+
+int gp_grad_plan_destroy(gp_grad_plan *plan) {
+  return guarded([&] { delete plan; });
+}
+
+int gp_grad_plan_sizes(const gp_grad_plan *plan, uint32_t sizes[4]) {
+  return guarded([&] {
+    if (!plan || !sizes) throw InvalidArgument("NULL argument");
+    sizes[0] = plan->numLoc;
+    sizes[1] = plan->numComp;
+    sizes[2] = plan->numNodes;
+    sizes[3] = plan->numErrors;
+  });
+}
+
+int gp_grad_plan_dem_structure(const gp_grad_plan *plan, uint32_t *offsets, uint32_t *targets) {
+  return guarded([&] {
+    if (!plan) throw InvalidArgument("plan is NULL");
+    const auto &r = plan->result;
+    if (offsets) std::memcpy(offsets, r.offsets, 4ull * (r.numErrors + 1));
+    if (targets) std::memcpy(targets, r.targets, 4ull * r.numTargets);
+  });
+}
+
+int gp_grad_plan_device_arrays(const gp_grad_plan *plan, const uint32_t **comp_loc,
+                               const uint8_t **comp_kind, const uint32_t **node_start,
+                               const uint32_t **node_comp, const uint32_t **class_start,
+                               const uint32_t **class_node) {
+  return guarded([&] {
+    if (!plan) throw InvalidArgument("plan is NULL");
+    if (comp_loc) *comp_loc = plan->dCompLoc;
+    if (comp_kind) *comp_kind = plan->dCompKind;
+    if (node_start) *node_start = plan->dNodeStart;
+    if (node_comp) *node_comp = plan->dNodeComp;
+    if (class_start) *class_start = plan->dClassStart;
+    if (class_node) *class_node = plan->dClassNode;
+  });
+}
+
+int gp_xor_merge_forward(const double *values, const uint32_t *index, const uint32_t *seg_start,
+                         uint32_t num_segments, double *out, void *stream) {
+  return guarded([&] {
+    if (num_segments && (!values || !index || !seg_start || !out)) throw InvalidArgument("NULL argument");
+    gpa::xorMergeForward(values, index, seg_start, num_segments, out, stream);
+  });
+}
+
+int gp_xor_merge_backward(const double *values, const uint32_t *index, const uint32_t *seg_start,
+                          uint32_t num_segments, const double *grad_out, double *grad_values,
+                          void *stream) {
+  return guarded([&] {
+    if (num_segments && (!values || !index || !seg_start || !grad_out || !grad_values))
+      throw InvalidArgument("NULL argument");
+    gpa::xorMergeBackward(values, index, seg_start, num_segments, grad_out, grad_values, stream);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,97 @@
+// engine.hpp -- the GPU compile pipeline for one or many circuits.
+//
+// Stages (one HIP stream, all on device; kernels in engine.hip):
+//   sweep      backward sensitivity sweep, one workgroup per (circuit,
+//              32-bit detector column), three-layer LDS ring, per-column
+//              time band (replaces CUDA.hpp:26-52 + CUDAHandlers.hpp:38-62);
+//   classify   per p>0 node: set bits -> FNV-1a-64 over 32 padded slots
+//              (CUDAHandlers.hpp:86-121, Array.hpp:52-65), second hash for
+//              collision checks, bucket histogram;
+//   partition  MSD scatter of (hash, rank) into 2^b buckets per circuit
+//              (first radix digit = top b hash bits);
+//   buckets    one wave64 per bucket: register bitonic sort on (hash, rank)
+//              and segmented XOR-merge scan (replaces the CUB radix sort and
+//              reduce-by-key, CUDA.hpp:275-343); oversized buckets go to a
+//              workgroup fallback;
+//   emit       compact CSR output: probability, offset and targets per class
+//              (replaces the full N x 32 gather + copy, CUDA.hpp:488-497).
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "stepg.hpp"
+
+namespace gpa {
+
+struct VariantResult {
+  uint32_t numErrors = 0;
+  uint64_t numTargets = 0;
+  uint32_t numDetectors = 0, numObservables = 0;
+  const double *probs = nullptr;     // numErrors
+  const uint32_t *offsets = nullptr; // numErrors + 1, relative to targets
+  const uint32_t *targets = nullptr; // numTargets
+  const uint64_t *hashes = nullptr;  // numErrors
+  std::vector<uint32_t> offsetsRel;  // storage for offsets
+};
+
+struct StageTimes {
+  double upload = 0, sweep = 0, classify = 0, partition = 0, buckets = 0, emit = 0, download = 0,
+         total = 0;
+};
+
+class Pipeline {
+ public:
+  // `growable`: buffers grow to fit any load; otherwise load() throws
+  // RuntimeError when a circuit exceeds the capacity fixed by reserve().
+  Pipeline(int device, bool growable);
+  ~Pipeline();
+  Pipeline(const Pipeline &) = delete;
+  Pipeline &operator=(const Pipeline &) = delete;
+
+  // Fixes the capacity to what `imgs` need (non-growable pipelines).
+  void reserve(const std::vector<const StepgImage *> &imgs);
+  // Uploads the images (host -> device) and prepares the launch plan.
+  void load(const std::vector<const StepgImage *> &imgs);
+  // Enqueues the device pipeline; `timed` records per-stage events.
+  void run(bool timed);
+  // Waits for the device, copies the results to host.
+  void download();
+  // run() `iters` times and returns mean device microseconds per run.
+  double benchDevice(uint32_t iters);
+
+  uint32_t numVariants() const { return static_cast<uint32_t>(variants_.size()); }
+  void result(uint32_t v, VariantResult &out) const;
+  const StageTimes &times() const { return times_; }
+  uint64_t deviceBytes() const;
+  int device() const { return device_; }
+  void *stream() const;
+
+  // Parity/debug access (after download or run + sync).
+  void debugSensitivities(uint32_t v, std::vector<uint32_t> &cols32) const;  // ranks x cols
+  // Device pointers for the differentiable path (single variant).
+  const uint32_t *devSortedRanks() const;
+  const uint32_t *devClassMember() const;
+
+  struct Impl;
+
+ private:
+  struct VariantInfo {
+    uint32_t numRanks, numCols, numDetectors, numObservables, bucketBase, numBuckets;
+  };
+  int device_;
+  bool growable_;
+  std::vector<VariantInfo> variants_;
+  StageTimes times_;
+  Impl *impl_;
+};
+
+// Segmented XOR-merge kernels (differentiable path); pointers are device
+// pointers, stream is a hipStream_t.
+void xorMergeForward(const double *values, const uint32_t *index, const uint32_t *segStart,
+                     uint32_t numSegments, double *out, void *stream);
+void xorMergeBackward(const double *values, const uint32_t *index, const uint32_t *segStart,
+                      uint32_t numSegments, const double *gradOut, double *gradValues,
+                      void *stream);
+
+}  // namespace gpa

@@ -1,0 +1,369 @@
+// stepg.cpp -- STEPG builder (see stepg.hpp).
+#include "stepg.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+
+namespace gpa {
+namespace {
+
+// Node slots within a qubit (Mixer.hpp:59-78).
+enum Slot : uint32_t { kX = 0, kZ = 1, kXXZZ = 2, kY = 3, kXZZX = 4, kXYYX = 5, kYY = 6, kYZZY = 7 };
+
+// One correlated two-qubit Pauli component, as data: the auxiliary node sits
+// on qubit `host` (0 = first, 1 = second of the pair) in `slot`, `back`
+// layers before the channel, and flows into two targets given as (qubit of
+// the pair, slot, layer offset from the channel layer: 0 or -1).  The target
+// order is the reference's edge insertion order (Mixer.hpp:305-533), which
+// fixes which child lands in the low/high half of the adjacency word.
+struct PairRule {
+  uint8_t pa, pb;  // Pauli on q0, q1: 1 X, 2 Y, 3 Z
+  uint8_t minLevel;
+  uint8_t host, slot, back;
+  uint8_t t0q, t0slot;
+  int8_t t0dl;
+  uint8_t t1q, t1slot;
+  int8_t t1dl;
+  const char *name;
+};
+
+constexpr PairRule kPairRules[] = {
+    {1, 1, 0, 0, kXXZZ, 1, 0, kX, 0, 1, kX, 0, "XX"},
+    {3, 3, 0, 1, kXXZZ, 1, 0, kZ, 0, 1, kZ, 0, "ZZ"},
+    {1, 3, 1, 0, kXZZX, 1, 0, kX, 0, 1, kZ, 0, "XZ"},
+    {3, 1, 1, 1, kXZZX, 1, 0, kZ, 0, 1, kX, 0, "ZX"},
+    {1, 2, 2, 0, kXYYX, 2, 0, kX, 0, 1, kY, -1, "XY"},
+    {2, 1, 2, 1, kXYYX, 2, 0, kY, -1, 1, kX, 0, "YX"},
+    {2, 2, 2, 0, kYY, 2, 0, kY, -1, 1, kY, -1, "YY"},
+    {2, 3, 2, 1, kYZZY, 2, 0, kY, -1, 1, kZ, 0, "YZ"},
+    {3, 2, 2, 0, kYZZY, 2, 0, kZ, 0, 1, kY, -1, "ZY"},
+};
+
+struct Builder {
+  Stepg &g;
+  bool record;
+
+  uint32_t node(uint32_t q, uint32_t slot, uint32_t l) const { return l * g.npl + q * g.k + slot; }
+
+  // Graph::addEdge / STEPG::addFlow (Graph.hpp:132-156, STEPG.hpp:86-101).
+  void addFlow(uint32_t src, uint32_t sl, uint32_t dst, uint32_t dl) {
+    if (dl <= sl) throw RuntimeError("STEPG: flows must move forward in time.");
+    if (sl >= g.numLayers || dl >= g.numLayers)
+      throw RuntimeError("STEPG: graph.addEdge failed with status 3.");
+    uint64_t &a = g.adjacency[src];
+    const uint32_t lo = static_cast<uint32_t>(a), hi = static_cast<uint32_t>(a >> 32);
+    if (lo == dst || hi == dst) throw RuntimeError("STEPG: graph.addEdge failed with status 1.");
+    if (lo == kNone) a = (a & 0xFFFFFFFF00000000ull) | dst;
+    else if (hi == kNone) a = (a & 0xFFFFFFFFull) | (static_cast<uint64_t>(dst) << 32);
+    else throw RuntimeError("STEPG: graph.addEdge failed with status 2.");
+  }
+
+  // Graph::removeEdge / STEPG::removeFlow (Graph.hpp:162-182, STEPG.hpp:108-123).
+  void removeFlow(uint32_t src, uint32_t sl, uint32_t dst, uint32_t dl) {
+    if (dl <= sl) throw RuntimeError("STEPG: flows must move forward in time.");
+    if (sl >= g.numLayers || dl >= g.numLayers)
+      throw RuntimeError("STEPG: graph.removeEdge failed with status 3.");
+    uint64_t &a = g.adjacency[src];
+    if (static_cast<uint32_t>(a) == dst) a |= 0xFFFFFFFFull;
+    else if (static_cast<uint32_t>(a >> 32) == dst) a |= 0xFFFFFFFF00000000ull;
+    else throw RuntimeError("STEPG: graph.removeEdge failed with status 4.");
+  }
+
+  // An auxiliary node already wired to exactly these two children (the same
+  // Pauli component from another channel in the same layer) is reused: its
+  // probability is merged and no flow is added.  The reference would throw
+  // on the duplicate edge (Graph.hpp:141-143); any other overlap still does.
+  bool wiredTo(uint32_t aux, uint32_t c0, uint32_t c1) const {
+    const uint64_t a = g.adjacency[aux];
+    return static_cast<uint32_t>(a) == c0 && static_cast<uint32_t>(a >> 32) == c1;
+  }
+
+  // STEPG::mergeProbabilities (STEPG.hpp:128-131), same operand order so the
+  // node probabilities are bit-identical to the reference's.
+  void merge(uint32_t n, double p, uint32_t loc, uint8_t kind) {
+    double &old = g.probs[n];
+    old = (1 - p) * old + (1 - old) * p;
+    if (record) {
+      g.compNode.push_back(n);
+      g.compLoc.push_back(loc);
+      g.compKind.push_back(kind);
+    }
+  }
+
+  void single(int pauli, uint32_t q, uint32_t l, double p, uint32_t loc, uint8_t kind) {
+    if (pauli == 1) {
+      merge(node(q, kX, l), p, loc, kind);
+    } else if (pauli == 3) {
+      merge(node(q, kZ, l), p, loc, kind);
+    } else {  // Y (Mixer.hpp:222-235), dropped at L0
+      if (g.level < 1) return;
+      if (l < 1) throw RuntimeError("Mixer: cannot apply Y error in layer l < 1");
+      const uint32_t y = node(q, kY, l - 1);
+      if (!wiredTo(y, node(q, kX, l), node(q, kZ, l))) {
+        addFlow(y, l - 1, node(q, kX, l), l);
+        addFlow(y, l - 1, node(q, kZ, l), l);
+      }
+      merge(y, p, loc, kind);
+    }
+  }
+
+  void pair(int pa, int pb, uint32_t q0, uint32_t q1, uint32_t l, double p, uint32_t loc) {
+    if (pa == 0) return single(pb, q1, l, p, loc, 2);
+    if (pb == 0) return single(pa, q0, l, p, loc, 2);
+    for (const PairRule &r : kPairRules) {
+      if (r.pa != pa || r.pb != pb) continue;
+      if (g.level < r.minLevel) return;
+      if (l < r.back)
+        throw RuntimeError(std::string("Mixer: cannot apply ") + r.name + " error in layer l < " +
+                           std::to_string(r.back));
+      const uint32_t qs[2] = {q0, q1};
+      const uint32_t al = l - r.back;
+      const uint32_t aux = node(qs[r.host], r.slot, al);
+      const uint32_t c0 = node(qs[r.t0q], r.t0slot, l + r.t0dl);
+      const uint32_t c1 = node(qs[r.t1q], r.t1slot, l + r.t1dl);
+      if (!wiredTo(aux, c0, c1)) {
+        addFlow(aux, al, c0, l + r.t0dl);
+        addFlow(aux, al, c1, l + r.t1dl);
+      }
+      merge(aux, p, loc, 2);
+      return;
+    }
+  }
+};
+
+}  // namespace
+
+void buildStepg(const Circuit &c, int level, Stepg &g, const double *theta, bool recordComponents) {
+  g.level = level;
+  g.k = nodesPerQubit(level);
+  g.numLayers = c.numTicks + 1;
+  g.numQubits = c.numQubits;
+  g.npl = g.numQubits * g.k;
+  const uint64_t nn = static_cast<uint64_t>(g.numLayers) * g.npl;
+  if (nn >= (1ull << 31)) throw RuntimeError("circuit too large: more than 2^31 STEPG nodes");
+  if (g.npl >= (1u << 30)) throw RuntimeError("circuit too large: too many nodes per layer");
+  g.numNodes = static_cast<uint32_t>(nn);
+  g.numMeasurements = c.numMeasurements;
+  g.numDetectors = c.numDetectors;
+  g.numObservables = c.numObservables;
+  g.numLocations = c.numLocations;
+  g.numCols = (g.numDetectors + g.numObservables + 31) / 32;
+  g.adjacency.assign(g.numNodes, ~0ull);
+  g.probs.assign(g.numNodes, 0.0);
+  g.keys.assign(g.numMeasurements, 0);
+  g.measBits.assign(static_cast<size_t>(g.numMeasurements) * g.numCols, 0u);
+  g.compNode.clear();
+  g.compLoc.clear();
+  g.compKind.clear();
+
+  Builder b{g, recordComponents};
+  // persistence flows X->X, Z->Z between consecutive layers (Mixer.hpp:97-108)
+  for (uint32_t l = 0; l + 1 < g.numLayers; ++l)
+    for (uint32_t q = 0; q < g.numQubits; ++q) {
+      b.addFlow(b.node(q, kX, l), l, b.node(q, kX, l + 1), l + 1);
+      b.addFlow(b.node(q, kZ, l), l, b.node(q, kZ, l + 1), l + 1);
+    }
+
+  uint32_t layer = 0, meas = 0, det = 0, loc = 0;
+  auto rate = [&](double p) {
+    if (!theta) return p;
+    return theta[loc];
+  };
+  for (const Op &o : c.ops) {
+    const int32_t *t = c.targets.data() + o.t0;
+    switch (o.kind) {
+    case OpKind::Tick:
+      layer++;
+      break;
+    case OpKind::R:  // Mixer.hpp:114-120
+      // Gates in the last layer have no later layer to flow into: no-op
+      // (the reference throws there, Graph.hpp:163-165).
+      if (layer + 1 >= g.numLayers) break;
+      for (uint32_t j = 0; j < o.nt; ++j) {
+        const uint32_t q = t[j];
+        b.removeFlow(b.node(q, kX, layer), layer, b.node(q, kX, layer + 1), layer + 1);
+        b.removeFlow(b.node(q, kZ, layer), layer, b.node(q, kZ, layer + 1), layer + 1);
+      }
+      break;
+    case OpKind::CX:  // Mixer.hpp:127-136
+      if (layer + 1 >= g.numLayers) break;
+      for (uint32_t j = 0; j < o.nt; j += 2) {
+        const uint32_t cq = t[j], tq = t[j + 1];
+        b.addFlow(b.node(cq, kX, layer), layer, b.node(tq, kX, layer + 1), layer + 1);
+        b.addFlow(b.node(tq, kZ, layer), layer, b.node(cq, kZ, layer + 1), layer + 1);
+      }
+      break;
+    case OpKind::H:  // Mixer.hpp:142-150
+      if (layer + 1 >= g.numLayers) break;
+      for (uint32_t j = 0; j < o.nt; ++j) {
+        const uint32_t q = t[j];
+        b.removeFlow(b.node(q, kX, layer), layer, b.node(q, kX, layer + 1), layer + 1);
+        b.removeFlow(b.node(q, kZ, layer), layer, b.node(q, kZ, layer + 1), layer + 1);
+        b.addFlow(b.node(q, kX, layer), layer, b.node(q, kZ, layer + 1), layer + 1);
+        b.addFlow(b.node(q, kZ, layer), layer, b.node(q, kX, layer + 1), layer + 1);
+      }
+      break;
+    case OpKind::M:  // Circuit.hpp:316-319: X error before the measurement + key
+      for (uint32_t j = 0; j < o.nt; ++j) {
+        const double p = rate(o.arg);
+        b.single(1, t[j], layer, p, loc, 0);
+        g.keys[meas++] = b.node(t[j], kX, layer);
+        loc++;
+      }
+      break;
+    case OpKind::XErr:  // Mixer.hpp:209-214
+      for (uint32_t j = 0; j < o.nt; ++j) {
+        b.single(1, t[j], layer, rate(o.arg), loc, 0);
+        loc++;
+      }
+      break;
+    case OpKind::Dep1:  // Mixer.hpp:158-169: X, Z, Y with the per-channel split
+      for (uint32_t j = 0; j < o.nt; ++j) {
+        const double p = rate(o.arg);
+        if (!(p >= 0 && p <= 0.75)) throw RuntimeError("Mixer: invalid DEP1 p > 0.75.");
+        const double s = 0.5 - 0.5 * std::sqrt(1 - (4 * p) / 3);
+        b.single(1, t[j], layer, s, loc, 1);
+        b.single(3, t[j], layer, s, loc, 1);
+        b.single(2, t[j], layer, s, loc, 1);
+        loc++;
+      }
+      break;
+    case OpKind::Dep2:  // Mixer.hpp:178-201: 15 components in IX..ZZ order
+      for (uint32_t j = 0; j < o.nt; j += 2) {
+        const double p = rate(o.arg);
+        if (!(p >= 0 && p <= 0.9375)) throw RuntimeError("Mixer: invalid DEP2 p > 0.9375.");
+        const double s = 0.5 - 0.5 * std::pow(1 - (16 * p) / 15, 0.125);
+        for (int pa = 0; pa < 4; ++pa)
+          for (int pb = 0; pb < 4; ++pb)
+            if (pa || pb) b.pair(pa, pb, t[j], t[j + 1], layer, s, loc);
+        loc++;
+      }
+      break;
+    case OpKind::Detector:
+    case OpKind::Observable: {
+      // Detectors take bits 0..D-1 in order; OBSERVABLE_INCLUDE(k) takes bit
+      // D + k.  Records are XORed (a repeated record cancels).
+      const uint32_t bit =
+          o.kind == OpKind::Detector ? det++ : g.numDetectors + static_cast<uint32_t>(o.arg);
+      for (uint32_t j = 0; j < o.nt; ++j) {
+        const uint32_t row = meas + static_cast<uint32_t>(t[j]);  // t[j] = -k
+        g.measBits[static_cast<size_t>(row) * g.numCols + bit / 32] ^= 1u << (bit % 32);
+      }
+      break;
+    }
+    }
+  }
+}
+
+void buildImage(const Stepg &g, StepgImage &img) {
+  const uint32_t N = g.numNodes, npl = g.npl, L = g.numLayers, C = g.numCols;
+  img.numLayers = L;
+  img.npl = npl;
+  img.numCols = C;
+  img.numDetectors = g.numDetectors;
+  img.numObservables = g.numObservables;
+  img.numLocations = g.numLocations;
+
+  // ranks: p > 0 nodes in node order
+  std::vector<uint32_t> rank(N, kNone);
+  img.rankLayer.clear();
+  img.rankProb.clear();
+  img.rankNode.clear();
+  for (uint32_t l = 0, n = 0; l < L; ++l)
+    for (uint32_t j = 0; j < npl; ++j, ++n)
+      if (g.probs[n] > 0) {
+        rank[n] = static_cast<uint32_t>(img.rankNode.size());
+        img.rankNode.push_back(n);
+        img.rankLayer.push_back(l);
+        img.rankProb.push_back(g.probs[n]);
+      }
+  img.numRanks = static_cast<uint32_t>(img.rankNode.size());
+
+  // active = p > 0 nodes and all their descendants (children are later)
+  std::vector<uint8_t> active(N, 0);
+  for (uint32_t n = 0; n < N; ++n) {
+    if (g.probs[n] > 0) active[n] = 1;
+    if (!active[n]) continue;
+    const uint64_t a = g.adjacency[n];
+    const uint32_t c0 = static_cast<uint32_t>(a), c1 = static_cast<uint32_t>(a >> 32);
+    if (c0 != kNone) active[c0] = 1;
+    if (c1 != kNone) active[c1] = 1;
+  }
+
+  // seeds: XOR of the detector bits of every measurement on an active node
+  std::vector<uint32_t> seedOf(N, kNone);
+  std::vector<uint32_t> seedNode;
+  std::vector<uint32_t> seedRows;  // numSeeds x C, row-major (transposed below)
+  for (uint32_t m = 0; m < g.numMeasurements; ++m) {
+    const uint32_t n = g.keys[m];
+    if (!active[n]) continue;
+    const uint32_t *bits = g.measBits.data() + static_cast<size_t>(m) * C;
+    bool any = false;
+    for (uint32_t c = 0; c < C; ++c) any |= bits[c] != 0;
+    if (!any) continue;
+    if (seedOf[n] == kNone) {
+      seedOf[n] = static_cast<uint32_t>(seedNode.size());
+      seedNode.push_back(n);
+      seedRows.resize(seedRows.size() + C, 0u);
+    }
+    uint32_t *row = seedRows.data() + static_cast<size_t>(seedOf[n]) * C;
+    for (uint32_t c = 0; c < C; ++c) row[c] ^= bits[c];
+  }
+  const uint32_t S = static_cast<uint32_t>(seedNode.size());
+  img.numSeeds = S;
+  img.seedVals.assign(static_cast<size_t>(C) * S, 0u);
+  img.colHi.assign(C, -1);
+  img.colLow.assign(C, -1);
+  for (uint32_t s = 0; s < S; ++s) {
+    const int32_t l = static_cast<int32_t>(seedNode[s] / npl);
+    const uint32_t *row = seedRows.data() + static_cast<size_t>(s) * C;
+    for (uint32_t c = 0; c < C; ++c) {
+      img.seedVals[static_cast<size_t>(c) * S + s] = row[c];
+      if (!row[c]) continue;
+      if (img.colHi[c] < l) img.colHi[c] = l;
+      if (img.colLow[c] < 0 || img.colLow[c] > l) img.colLow[c] = l;
+    }
+  }
+
+  // active node records, layer by layer
+  img.layerAct.assign(L + 1, 0);
+  img.act.clear();
+  img.actRank.clear();
+  img.act.reserve(static_cast<size_t>(N) * 2);
+  auto code = [&](uint32_t child, uint32_t l) -> uint32_t {
+    if (child == kNone) return kNone;
+    const uint32_t lc = child / npl;
+    return ((lc - l) << 30) | (child - lc * npl);
+  };
+  for (uint32_t l = 0, n = 0; l < L; ++l) {
+    img.layerAct[l] = static_cast<uint32_t>(img.actRank.size());
+    for (uint32_t j = 0; j < npl; ++j, ++n) {
+      if (!active[n]) continue;
+      const uint64_t a = g.adjacency[n];
+      img.act.push_back(j);
+      img.act.push_back(code(static_cast<uint32_t>(a), l));
+      img.act.push_back(code(static_cast<uint32_t>(a >> 32), l));
+      img.act.push_back(seedOf[n]);
+      img.actRank.push_back(rank[n]);
+    }
+  }
+  img.layerAct[L] = static_cast<uint32_t>(img.actRank.size());
+  img.numActive = img.layerAct[L];
+}
+
+uint64_t imageDeviceBytes(const StepgImage &img) {
+  const uint64_t R = img.numRanks;
+  uint64_t b = 0;
+  b += (img.numLayers + 1) * 4ull + img.numActive * 20ull + img.seedVals.size() * 4ull;
+  b += img.numCols * 12ull + R * 12ull;  // bands, rank layer + prob
+  b += R * img.numCols * 4ull;            // sensitivity matrix
+  b += R * (8 + 8 + 4 + 4);               // hash, h2, nnz, class-of-rank
+  b += R * (8 + 4) * 2;                   // partitioned keys/ranks + scratch
+  b += R * (4 + 8 + 8 + 4 + 4 + 4);       // runs
+  b += R * (8 + 4 + 8 + 4) + R * 32 * 4;  // outputs (targets sized 32 per class)
+  b += (R / 32 + 64) * 4 * 8;             // bucket arrays
+  return b;
+}
+
+}  // namespace gpa

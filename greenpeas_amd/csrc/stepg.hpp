@@ -1,0 +1,85 @@
+// stepg.hpp -- space-time error-propagation graph (STEPG) builder and the
+// device image the GPU pipeline consumes.
+//
+// STEPG semantics follow the reference (STEPG.hpp:40-132, Mixer.hpp:89-534,
+// Circuit.hpp:290-478): per layer (TICK) and qubit there are 3/5/8 error
+// nodes (L0/L1/L2); gates add "flows" (edges) from layer l to l+1; each node
+// has at most two children; noise components XOR-merge their probability
+// into a node, and correlated components live on auxiliary nodes one or two
+// layers earlier whose flows reproduce the Pauli pair.  A node's detector
+// sensitivity is the XOR of its measurement seed and its children's
+// sensitivities, computed backwards in time on the GPU.
+//
+// The device image re-lays that graph out for the MI355X sweep:
+//  * only "active" nodes are kept (nodes with p > 0 and every node reachable
+//    from one along flows; the others can never influence an output);
+//  * each active node is one 16-byte record {j, child0, child1, seed} with
+//    children encoded as (layer delta << 30) | index-in-layer, so the sweep
+//    resolves them inside a three-layer LDS ring;
+//  * measurement seeds are stored once per seeded node, column-major over
+//    32-bit detector words ("columns");
+//  * per column, the highest and lowest layer holding a seed bound the time
+//    band in which the column can be non-zero;
+//  * the p > 0 nodes ("ranks", ascending node order) are the rows of the
+//    sensitivity matrix and the items that are hashed, sorted and merged.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "circuit.hpp"
+
+namespace gpa {
+
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+
+inline uint32_t nodesPerQubit(int level) {
+  switch (level) {
+  case 0: return 3;
+  case 1: return 5;
+  case 2: return 8;
+  }
+  throw InvalidArgument("Invalid correlation level.");
+}
+
+// Host STEPG in the reference's layout: node = layer * npl + qubit * k + slot.
+struct Stepg {
+  int level = 2;
+  uint32_t k = 8, numLayers = 0, numQubits = 0, npl = 0, numNodes = 0;
+  uint32_t numMeasurements = 0, numDetectors = 0, numObservables = 0, numLocations = 0;
+  uint32_t numCols = 0;  // 32-bit detector words: ceil((D + O) / 32)
+  std::vector<uint64_t> adjacency;  // child0 | child1 << 32, kNone = no child
+  std::vector<double> probs;
+  std::vector<uint32_t> keys;       // measurement -> node
+  std::vector<uint32_t> measBits;   // measurement x numCols (row-major) detector bits
+  // noise components (for the differentiable path)
+  std::vector<uint32_t> compNode, compLoc;
+  std::vector<uint8_t> compKind;    // 0 rate itself, 1 DEPOLARIZE1 split, 2 DEPOLARIZE2 split
+};
+
+// Builds the STEPG of `c` at `level`.  `theta` (if non-null) overrides the
+// rate of every fault location.  Throws RuntimeError on flow-rule violations
+// and invalid rates (the reference's std::runtime_error cases).
+void buildStepg(const Circuit &c, int level, Stepg &g, const double *theta = nullptr,
+                bool recordComponents = false);
+
+// Device-ready arrays of one circuit (see the file comment).
+struct StepgImage {
+  uint32_t numLayers = 0, npl = 0, numCols = 0, numDetectors = 0, numObservables = 0;
+  uint32_t numRanks = 0, numSeeds = 0, numActive = 0, numLocations = 0;
+  std::vector<uint32_t> layerAct;  // numLayers + 1 offsets into act
+  std::vector<uint32_t> act;       // 4 * numActive: j, child0 code, child1 code, seed row
+  std::vector<uint32_t> actRank;   // numActive: rank or kNone
+  std::vector<uint32_t> seedVals;  // numCols x numSeeds (column-major)
+  std::vector<int32_t> colHi, colLow;  // numCols: seed layer range (-1 = no seed)
+  std::vector<uint32_t> rankLayer; // numRanks
+  std::vector<double> rankProb;    // numRanks
+  std::vector<uint32_t> rankNode;  // numRanks (host only: node index of each rank)
+};
+
+void buildImage(const Stepg &g, StepgImage &img);
+
+// Device bytes the pipeline needs for an image (used by admission control).
+uint64_t imageDeviceBytes(const StepgImage &img);
+
+}  // namespace gpa

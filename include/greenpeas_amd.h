@@ -1,0 +1,212 @@
+/*
+ * greenpeas_amd.h -- C-ABI of the MI355X detector-error-model (DEM) compiler.
+ *
+ * Plain C types only (no torch, no C++ types).  Every entry point returns a
+ * status code: GP_OK, GP_INVALID_ARGUMENT (the reference's
+ * std::invalid_argument -> Python ValueError) or GP_RUNTIME_ERROR (the
+ * reference's std::runtime_error -> Python RuntimeError); gp_last_error()
+ * then holds the message for the calling thread.
+ *
+ * The drop-in surface mirrors the reference's pybind module
+ * (source/QEC/ErrorAnalysis/ErrorAnalysis.pybind.cu:47-79) and its C++
+ * Driver (include/GreenPeas/QEC/ErrorAnalysis/Driver.hpp:37-219): a driver is
+ * sized once from a "max" circuit and then compiles any circuit that fits,
+ * reusing its device buffers.
+ */
+#ifndef GREENPEAS_AMD_H
+#define GREENPEAS_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GP_OK 0
+#define GP_INVALID_ARGUMENT 1
+#define GP_RUNTIME_ERROR 2
+
+/* CorrelationLevel (Mixer.hpp:13-20; pybind enum ErrorAnalysis.pybind.cu:48-52) */
+#define GP_L0 0
+#define GP_L1 1
+#define GP_L2 2
+
+typedef struct gp_driver gp_driver;
+typedef struct gp_batch gp_batch;
+typedef struct gp_grad_plan gp_grad_plan;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *gp_last_error(void);
+
+/* Library version string. */
+const char *gp_version(void);
+
+/* ------------------------------------------------------------------------
+ * Drop-in driver
+ * --------------------------------------------------------------------- */
+
+/* get_driver(circuit, correlation_level=L2)
+ * Replaces ErrorAnalysis.pybind.cu:72-78 / getDriver (:26-41) and
+ * Driver::fromStimCircuit (Driver.hpp:216-218): parses `circuit_text` (stim
+ * circuit text), builds its STEPG and allocates device buffers sized for it
+ * on HIP device `device`.  Later compiles must fit within these sizes. */
+int gp_get_driver(const char *circuit_text, int correlation_level, int device,
+                  gp_driver **out);
+
+/* Releases the driver and all of its host and device memory. */
+int gp_driver_destroy(gp_driver *driver);
+
+/* Driver.compile_detector_error_model(circuit)
+ * Replaces ErrorAnalysis.pybind.cu:56-66 and
+ * Driver::compileDetectorErrorModel(const stim::Circuit&) (Driver.hpp:199-204):
+ * parses the circuit into the driver (throws GP_RUNTIME_ERROR if it exceeds
+ * the driver's capacity, like fitto at Vector.hpp:118-123), compiles on the
+ * GPU and keeps the resulting DEM in the driver until the next compile.
+ * With circuit_text == NULL it recompiles the circuit parsed last
+ * (ErrorAnalysis.pybind.cu:67-70, Driver.hpp:208-211). */
+int gp_compile_detector_error_model(gp_driver *driver, const char *circuit_text);
+
+/* Parses `circuit_text` into the driver without compiling
+ * (Circuit::parseFromStimCircuit, Circuit.hpp:471-478). */
+int gp_driver_parse(gp_driver *driver, const char *circuit_text);
+
+/* Shape of the last compiled DEM (Driver::getDetectorErrorModel,
+ * Driver.hpp:166-194): errors in the reference's order (ascending FNV-1a-64
+ * class hash), total number of targets, detector and observable counts. */
+int gp_dem_shape(const gp_driver *driver, uint32_t *num_errors,
+                 uint64_t *num_targets, uint32_t *num_detectors,
+                 uint32_t *num_observables);
+
+/* Copies the last DEM as CSR: probs[num_errors], offsets[num_errors + 1],
+ * targets[num_targets] where a target t < num_detectors is detector t and
+ * otherwise observable t - num_detectors.  hashes (optional, may be NULL)
+ * receives each error's 64-bit class hash. */
+int gp_dem_copy(const gp_driver *driver, double *probs, uint32_t *offsets,
+                uint32_t *targets, uint64_t *hashes);
+
+/* Writes the last DEM as stim DEM text ("error(p) D.. L..\n" per error).
+ * If buf is NULL or too small, only *len (bytes, excluding the NUL) is set
+ * and GP_OK is returned; call again with a buffer of *len + 1 bytes. */
+int gp_dem_text(const gp_driver *driver, char *buf, size_t cap, size_t *len);
+
+/* Circuit parameters of the parsed circuit (CircuitParameters,
+ * Circuit.hpp:20-80): qubits, layers, measurements, detectors, observables,
+ * nodes (at the driver's level), fault locations. */
+int gp_driver_params(const gp_driver *driver, uint32_t params[7]);
+
+/* Per-stage device timings of the last compile, in microseconds, measured
+ * with HIP events on the driver's stream: [0] upload, [1] sensitivity sweep,
+ * [2] classes+hash, [3] partition, [4] bucket sort + segmented reduce,
+ * [5] emit, [6] download, [7] whole compile (host wall clock). */
+int gp_driver_timings(const gp_driver *driver, double timings_us[8]);
+
+/* Compiles the already-uploaded circuit `iterations` times back to back on
+ * the device (no host transfer, no DEM download) and returns the mean
+ * device time per compile in microseconds; the driver keeps the result of the
+ * last iteration on the device.  Used to time the device-resident path. */
+int gp_driver_bench_device(gp_driver *driver, uint32_t iterations,
+                           double *mean_us);
+
+/* Stage outputs for parity tests (small circuits): the dense sensitivity
+ * matrix of the p>0 nodes as computed on the GPU, sens[num_pnodes * words64]
+ * row-major (64-bit words, bit t = detector t or observable t - D), and the
+ * node index of each row.  Call after a compile. */
+int gp_debug_pnodes(const gp_driver *driver, uint32_t *num_pnodes,
+                    uint32_t *words64);
+int gp_debug_sensitivities(const gp_driver *driver, uint32_t *pnode_index,
+                           uint64_t *sens);
+/* The driver's host STEPG: adjacency[nodes] (child0 low 32 bits, child1 high
+ * 32 bits, 0xFFFFFFFF = none) and probs[nodes] (STEPG.hpp:40-132). */
+int gp_debug_stepg(const gp_driver *driver, uint64_t *adjacency, double *probs);
+
+/* Host-only STEPG build (no device is touched): dims[0..3] = layers, nodes
+ * per layer, nodes, fault locations.  Pass NULL arrays to query dims only,
+ * then adjacency[nodes] and probs[nodes] as gp_debug_stepg. */
+int gp_build_stepg_host(const char *circuit_text, int correlation_level,
+                        uint32_t dims[4], uint64_t *adjacency, double *probs);
+
+/* ------------------------------------------------------------------------
+ * Batched compile: many circuits in one set of launches (no counterpart in
+ * the reference, which compiles one circuit per Driver::compile call).
+ * --------------------------------------------------------------------- */
+
+/* Creates a batch context on `device`; buffers grow on demand. */
+int gp_batch_create(int correlation_level, int device, gp_batch **out);
+int gp_batch_destroy(gp_batch *batch);
+/* Parses and builds `count` circuits (host threads: `threads`, 0 = auto). */
+int gp_batch_set_circuits(gp_batch *batch, const char *const *circuit_texts,
+                          uint32_t count, uint32_t threads);
+/* Compiles all circuits of the batch on the GPU.  mode 0: one batched
+ * pipeline for all circuits (captured in a HIP graph and replayed when the
+ * batch is unchanged); mode 1: the circuits one after another through the
+ * same kernels (the one-by-one baseline). */
+int gp_batch_compile(gp_batch *batch, int mode);
+/* Device time of the last gp_batch_compile in microseconds. */
+int gp_batch_device_us(const gp_batch *batch, double *us);
+/* Shape / copy of the DEM of circuit `index` (as gp_dem_shape/gp_dem_copy). */
+int gp_batch_dem_shape(const gp_batch *batch, uint32_t index,
+                       uint32_t *num_errors, uint64_t *num_targets,
+                       uint32_t *num_detectors, uint32_t *num_observables);
+int gp_batch_dem_copy(const gp_batch *batch, uint32_t index, double *probs,
+                      uint32_t *offsets, uint32_t *targets, uint64_t *hashes);
+/* Bytes of device memory the batch holds now. */
+int gp_batch_device_bytes(const gp_batch *batch, uint64_t *bytes);
+/* Estimated device bytes to compile circuit text at a level (admission). */
+int gp_estimate_device_bytes(const char *circuit_text, int correlation_level,
+                             uint64_t *bytes);
+
+/* ------------------------------------------------------------------------
+ * Differentiable probabilities (no counterpart in the reference).
+ *
+ * A plan fixes the structure of a circuit (STEPG and error classes are
+ * independent of the noise rates); probabilities then follow from the
+ * per-location rates theta[num_locations] (one per DEPOLARIZE1 / X_ERROR /
+ * M target and one per DEPOLARIZE2 pair, circuit order) by two segmented
+ * XOR-merges: components -> STEPG nodes -> error classes.  All pointers
+ * below are device pointers; work is queued on `stream` (a hipStream_t,
+ * NULL = default stream).
+ * --------------------------------------------------------------------- */
+
+int gp_grad_plan_create(const char *circuit_text, int correlation_level,
+                        int device, gp_grad_plan **out);
+int gp_grad_plan_destroy(gp_grad_plan *plan);
+/* sizes[0] = locations, [1] = components, [2] = p>0 nodes, [3] = errors */
+int gp_grad_plan_sizes(const gp_grad_plan *plan, uint32_t sizes[4]);
+/* Copies the plan's DEM structure (offsets/targets as gp_dem_copy) to host. */
+int gp_grad_plan_dem_structure(const gp_grad_plan *plan, uint32_t *offsets,
+                               uint32_t *targets);
+/* Device arrays describing the plan, each of the length given by sizes:
+ * comp_loc[components], comp_kind[components] (0 rate, 1 DEPOLARIZE1 split,
+ * 2 DEPOLARIZE2 split), node_start[nodes+1] + node_comp[components]
+ * (components grouped by node), class_start[errors+1] + class_node[nodes]
+ * (nodes grouped by error class). */
+int gp_grad_plan_device_arrays(const gp_grad_plan *plan,
+                               const uint32_t **comp_loc,
+                               const uint8_t **comp_kind,
+                               const uint32_t **node_start,
+                               const uint32_t **node_comp,
+                               const uint32_t **class_start,
+                               const uint32_t **class_node);
+
+/* Segmented XOR-merge, forward: out[s] = XOR-merge of values[index[k]] for k
+ * in [seg_start[s], seg_start[s+1]); the merge p (+) q = p + q - 2pq is the
+ * identity 1 - 2 out = prod(1 - 2 values), evaluated without cancellation.
+ * Replaces the reference's reduce-by-key reducer (CUDAHandlers.hpp:133-138,
+ * CUDA.hpp:315-343) for differentiable use. */
+int gp_xor_merge_forward(const double *values, const uint32_t *index,
+                         const uint32_t *seg_start, uint32_t num_segments,
+                         double *out, void *stream);
+/* Backward: grad_values[index[k]] = grad_out[s] * prod_{j != k in s}
+ * (1 - 2 values[index[j]]).  Each values entry belongs to at most one
+ * segment; entries in no segment are left untouched. */
+int gp_xor_merge_backward(const double *values, const uint32_t *index,
+                          const uint32_t *seg_start, uint32_t num_segments,
+                          const double *grad_out, double *grad_values,
+                          void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GREENPEAS_AMD_H */

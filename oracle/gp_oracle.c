@@ -406,6 +406,16 @@ static void remove_flow(struct gpo_stepg *g, uint32_t sq, uint32_t ss, uint32_t 
   else fail(2, "STEPG: graph.removeEdge failed with status 4.");
 }
 
+/* Adds aux -> (c0, c1) unless aux already has exactly these children (the
+ * same component from another channel in the layer: merge only). */
+static void wire_aux(struct gpo_stepg *g, uint32_t aq, uint32_t as, uint32_t al, uint32_t q0,
+                     uint32_t s0, uint32_t l0, uint32_t q1, uint32_t s1, uint32_t l1) {
+  uint32_t a = nid(g, aq, as, al), c0 = nid(g, q0, s0, l0), c1 = nid(g, q1, s1, l1);
+  if ((uint32_t)g->adj[a] == c0 && (uint32_t)(g->adj[a] >> 32) == c1) return;
+  add_flow(g, aq, as, al, q0, s0, l0);
+  add_flow(g, aq, as, al, q1, s1, l1);
+}
+
 static void add_comp(struct gpo_stepg *g, uint32_t node, uint32_t loc, uint8_t kind) {
   if (g->ncomp == g->capcomp) {
     g->capcomp = g->capcomp ? 2 * g->capcomp : 4096;
@@ -436,8 +446,7 @@ static void err_z(struct gpo_stepg *g, int lv, uint32_t q, uint32_t l, double p,
 static void err_y(struct gpo_stepg *g, int lv, uint32_t q, uint32_t l, double p, uint32_t loc, uint8_t kd) {
   if (lv < 1) return; /* L0 drops Y components */
   if (l < 1) fail(2, "Mixer: cannot apply Y error in layer l < 1");
-  add_flow(g, q, S_Y, l - 1, q, S_X, l);
-  add_flow(g, q, S_Y, l - 1, q, S_Z, l);
+  wire_aux(g, q, S_Y, l - 1, q, S_X, l, q, S_Z, l);
   merge_at(g, nid(g, q, S_Y, l - 1), p, loc, kd);
 }
 
@@ -461,25 +470,21 @@ static void err_pair(struct gpo_stepg *g, int lv, int pa, int pb, uint32_t q0, u
   }
   if (pa == 1 && pb == 1) { /* XX: q0 slot 2, layer l-1 */
     if (l < 1) fail(2, "Mixer: cannot apply XX error in layer l < 1");
-    add_flow(g, q0, S_XXZZ, l - 1, q0, S_X, l);
-    add_flow(g, q0, S_XXZZ, l - 1, q1, S_X, l);
+    wire_aux(g, q0, S_XXZZ, l - 1, q0, S_X, l, q1, S_X, l);
     merge_at(g, nid(g, q0, S_XXZZ, l - 1), p, loc, kd);
   } else if (pa == 3 && pb == 3) { /* ZZ: q1 slot 2, layer l-1 */
     if (l < 1) fail(2, "Mixer: cannot apply ZZ error in layer l < 1");
-    add_flow(g, q1, S_XXZZ, l - 1, q0, S_Z, l);
-    add_flow(g, q1, S_XXZZ, l - 1, q1, S_Z, l);
+    wire_aux(g, q1, S_XXZZ, l - 1, q0, S_Z, l, q1, S_Z, l);
     merge_at(g, nid(g, q1, S_XXZZ, l - 1), p, loc, kd);
   } else if (pa == 1 && pb == 3) { /* XZ: q0 slot 4 (L1+) */
     if (lv < 1) return;
     if (l < 1) fail(2, "Mixer: cannot apply XZ error in layer l < 1");
-    add_flow(g, q0, S_XZZX, l - 1, q0, S_X, l);
-    add_flow(g, q0, S_XZZX, l - 1, q1, S_Z, l);
+    wire_aux(g, q0, S_XZZX, l - 1, q0, S_X, l, q1, S_Z, l);
     merge_at(g, nid(g, q0, S_XZZX, l - 1), p, loc, kd);
   } else if (pa == 3 && pb == 1) { /* ZX: q1 slot 4 (L1+) */
     if (lv < 1) return;
     if (l < 1) fail(2, "Mixer: cannot apply ZX error in layer l < 1");
-    add_flow(g, q1, S_XZZX, l - 1, q0, S_Z, l);
-    add_flow(g, q1, S_XZZX, l - 1, q1, S_X, l);
+    wire_aux(g, q1, S_XZZX, l - 1, q0, S_Z, l, q1, S_X, l);
     merge_at(g, nid(g, q1, S_XZZX, l - 1), p, loc, kd);
   } else {
     if (lv < 2) return; /* pairs involving Y need L2 */
@@ -488,24 +493,19 @@ static void err_pair(struct gpo_stepg *g, int lv, int pa, int pb, uint32_t q0, u
       fail(2, "Mixer: cannot apply %s%s error in layer l < 2", nm[pa], nm[pb]);
     }
     if (pa == 1 && pb == 2) { /* XY: q0 slot 5 -> X0(l), Y1(l-1) */
-      add_flow(g, q0, S_XYYX, l - 2, q0, S_X, l);
-      add_flow(g, q0, S_XYYX, l - 2, q1, S_Y, l - 1);
+      wire_aux(g, q0, S_XYYX, l - 2, q0, S_X, l, q1, S_Y, l - 1);
       merge_at(g, nid(g, q0, S_XYYX, l - 2), p, loc, kd);
     } else if (pa == 2 && pb == 1) { /* YX: q1 slot 5 -> Y0(l-1), X1(l) */
-      add_flow(g, q1, S_XYYX, l - 2, q0, S_Y, l - 1);
-      add_flow(g, q1, S_XYYX, l - 2, q1, S_X, l);
+      wire_aux(g, q1, S_XYYX, l - 2, q0, S_Y, l - 1, q1, S_X, l);
       merge_at(g, nid(g, q1, S_XYYX, l - 2), p, loc, kd);
     } else if (pa == 2 && pb == 2) { /* YY: q0 slot 6 -> Y0(l-1), Y1(l-1) */
-      add_flow(g, q0, S_YY, l - 2, q0, S_Y, l - 1);
-      add_flow(g, q0, S_YY, l - 2, q1, S_Y, l - 1);
+      wire_aux(g, q0, S_YY, l - 2, q0, S_Y, l - 1, q1, S_Y, l - 1);
       merge_at(g, nid(g, q0, S_YY, l - 2), p, loc, kd);
     } else if (pa == 2 && pb == 3) { /* YZ: q1 slot 7 -> Y0(l-1), Z1(l) */
-      add_flow(g, q1, S_YZZY, l - 2, q0, S_Y, l - 1);
-      add_flow(g, q1, S_YZZY, l - 2, q1, S_Z, l);
+      wire_aux(g, q1, S_YZZY, l - 2, q0, S_Y, l - 1, q1, S_Z, l);
       merge_at(g, nid(g, q1, S_YZZY, l - 2), p, loc, kd);
     } else { /* ZY: q0 slot 7 -> Z0(l), Y1(l-1) */
-      add_flow(g, q0, S_YZZY, l - 2, q0, S_Z, l);
-      add_flow(g, q0, S_YZZY, l - 2, q1, S_Y, l - 1);
+      wire_aux(g, q0, S_YZZY, l - 2, q0, S_Z, l, q1, S_Y, l - 1);
       merge_at(g, nid(g, q0, S_YZZY, l - 2), p, loc, kd);
     }
   }
@@ -563,13 +563,15 @@ static void stepg_build(const circ_t *C, int level, const double *theta, uint32_
     const int32_t *t = tg + o->t0;
     switch (o->kind) {
     case K_TICK: layer++; break;
-    case K_R: /* Mixer.hpp:114-120 */
+    case K_R: /* Mixer.hpp:114-120; no later layer after the last: no-op */
+      if (layer + 1 >= g->nl) break;
       for (uint32_t j = 0; j < o->nt; ++j) {
         remove_flow(g, t[j], S_X, layer, t[j], S_X, layer + 1);
         remove_flow(g, t[j], S_Z, layer, t[j], S_Z, layer + 1);
       }
       break;
     case K_CX: /* Mixer.hpp:127-136 */
+      if (layer + 1 >= g->nl) break;
       for (uint32_t j = 0; j < o->nt; j += 2) {
         uint32_t c = t[j], tt = t[j + 1];
         add_flow(g, c, S_X, layer, tt, S_X, layer + 1);
@@ -577,6 +579,7 @@ static void stepg_build(const circ_t *C, int level, const double *theta, uint32_
       }
       break;
     case K_H: /* Mixer.hpp:142-150 */
+      if (layer + 1 >= g->nl) break;
       for (uint32_t j = 0; j < o->nt; ++j) {
         uint32_t q = t[j];
         remove_flow(g, q, S_X, layer, q, S_X, layer + 1);

@@ -32,7 +32,8 @@
  * CUDAHandlers.hpp:57-60); classes are not truncated at 32 targets
  * (CUDAHandlers.hpp:94) -- longer classes hash all targets with no padding;
  * nodes with probability 0 are dropped before grouping (the reference keeps
- * them and could print error(0) lines, Driver.hpp:185).
+ * them and could print error(0) lines, Driver.hpp:185); R/CX/H in the last
+ * layer add or remove no flow (the reference throws, Graph.hpp:163-165).
  */
 #ifndef GP_ORACLE_H
 #define GP_ORACLE_H

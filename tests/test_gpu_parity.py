@@ -1,0 +1,180 @@
+"""GPU parity: the HIP pipeline, called through the C-ABI, against the oracle
+and the reference's golden DEMs.
+
+Bars: detector/observable sets and order bit-exact; probabilities within
+1e-12 relative error (REL below); the sensitivity stage bit-exact.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import golden_dem_path, read_circuit
+from greenpeas_amd import CorrelationLevel, DetectorErrorModel, get_driver
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-12
+
+
+def as_dem(od) -> DetectorErrorModel:
+    return DetectorErrorModel(od.probs, od.offsets, od.targets, od.num_detectors, od.num_observables)
+
+
+@pytest.mark.parametrize("name", ["surface3", "surface5"])
+@pytest.mark.parametrize("level", [CorrelationLevel.L0, CorrelationLevel.L1, CorrelationLevel.L2])
+def test_compile_matches_golden_dem(name, level):
+    """python/tests/test_error_analysis.py:27-45, with the 1e-12 bar."""
+    text = read_circuit(name)
+    driver = get_driver(text, correlation_level=level)
+    dem = driver.compile_detector_error_model(text)
+    gold = DetectorErrorModel.from_file(golden_dem_path(name, int(level)), dem.num_detectors,
+                                        dem.num_observables)
+    assert dem.num_errors == gold.num_errors
+    # same order (ascending FNV-1a-64 of the 32 padded slots) and targets
+    assert np.array_equal(dem.offsets, gold.offsets)
+    assert np.array_equal(dem.targets, gold.targets)
+    assert np.max(np.abs(dem.probs - gold.probs) / gold.probs) <= REL
+
+
+@pytest.mark.parametrize("name", ["surface3", "surface5", "bb6", "bb12"])
+def test_compile_matches_oracles(oracle, name):
+    text = read_circuit(name)
+    dem = get_driver(text).compile_detector_error_model(text)
+    ref = oracle.compile_stepg(text, 2)
+    # identical order and hashes as the STEPG restatement
+    assert np.array_equal(dem.hashes, ref.hashes)
+    assert np.array_equal(dem.offsets, ref.offsets)
+    assert np.array_equal(dem.targets, ref.targets)
+    assert np.max(np.abs(dem.probs - ref.probs) / ref.probs) <= REL
+    # and the same model as the independent forward Pauli-frame compiler
+    assert dem.approx_equals(as_dem(oracle.compile_frames(text)), REL)
+
+
+@pytest.mark.parametrize("name", ["surface3", "bb6", "bb12"])
+@pytest.mark.parametrize("level", [0, 2])
+def test_sensitivity_sweep_bit_exact(oracle, name, level):
+    text = read_circuit(name)
+    driver = get_driver(text, correlation_level=level)
+    driver.compile_detector_error_model()
+    idx, sens = driver.debug_sensitivities()
+    s = oracle.build_stepg(text, level)
+    S = s.sweep()
+    assert np.array_equal(idx, np.nonzero(s.probs > 0)[0])
+    assert np.array_equal(sens, S[idx])
+
+
+@pytest.mark.parametrize("level", [CorrelationLevel.L0, CorrelationLevel.L1, CorrelationLevel.L2])
+def test_driver_reuse(level):
+    """A driver sized for d5 compiles d3 (test_error_analysis.py:48-59)."""
+    driver = get_driver(read_circuit("surface5"), correlation_level=level)
+    dem = driver.compile_detector_error_model(read_circuit("surface3"))
+    gold = DetectorErrorModel.from_file(golden_dem_path("surface3", int(level)), dem.num_detectors,
+                                        dem.num_observables)
+    assert np.array_equal(dem.targets, gold.targets)
+    assert np.max(np.abs(dem.probs - gold.probs) / gold.probs) <= REL
+    # back up to the max circuit: no stale state from the smaller compile
+    big = driver.compile_detector_error_model(read_circuit("surface5"))
+    gold5 = DetectorErrorModel.from_file(golden_dem_path("surface5", int(level)), big.num_detectors,
+                                         big.num_observables)
+    assert np.array_equal(big.targets, gold5.targets)
+
+
+def test_capacity_overflow_raises():
+    driver = get_driver(read_circuit("surface3"))
+    with pytest.raises(RuntimeError, match="capacity"):
+        driver.compile_detector_error_model(read_circuit("bb6"))
+    # the driver still works afterwards
+    dem = driver.compile_detector_error_model(read_circuit("surface3"))
+    assert dem.num_errors == 229
+
+
+def test_recompile_without_argument_is_identical():
+    text = read_circuit("bb6")
+    driver = get_driver(text)
+    a = driver.compile_detector_error_model()
+    b = driver.compile_detector_error_model()
+    c = driver.compile_detector_error_model(text)
+    assert a == b == c
+
+
+def test_dem_text_output():
+    text = read_circuit("surface3")
+    driver = get_driver(text)
+    dem = driver.compile_detector_error_model(text)
+    again = DetectorErrorModel.from_text(driver.dem_text(), dem.num_detectors, dem.num_observables)
+    assert again == dem
+
+
+def test_empty_and_noiseless(oracle):
+    driver = get_driver(read_circuit("surface3"))
+    assert driver.compile_detector_error_model("R 0\nTICK\nM 0\nDETECTOR rec[-1]\n").num_errors == 0
+    assert driver.compile_detector_error_model("").num_errors == 0
+
+
+def test_long_run_takes_fallback_path(oracle):
+    """An idle qubit over 600 layers: 1200 nodes share one class, far beyond a
+    wave's 256-item sort capacity, so the workgroup fallback merges them."""
+    text = "R 0 1\n" + "TICK\nDEPOLARIZE1(0.001) 0 1\n" * 600 + "TICK\nM 0 1\nDETECTOR rec[-2]\nDETECTOR rec[-1]\n"
+    dem = get_driver(text).compile_detector_error_model(text)
+    ref = as_dem(oracle.compile_stepg(text, 2))
+    assert dem.approx_equals(ref, REL)
+    assert dem.approx_equals(as_dem(oracle.compile_frames(text)), REL)
+
+
+def test_long_class_beyond_32_targets(oracle):
+    text = "R 0 1\nTICK\nX_ERROR(0.01) 0 1\nM 0 1\n" + "DETECTOR rec[-1]\n" * 40 + "DETECTOR rec[-2]\n" * 3
+    dem = get_driver(text).compile_detector_error_model(text)
+    ref = oracle.compile_stepg(text, 2)
+    assert np.array_equal(dem.hashes, ref.hashes)
+    assert dem.approx_equals(as_dem(ref), REL)
+    assert max(np.diff(dem.offsets)) == 40
+
+
+def test_errors_are_reference_exception_classes():
+    driver = get_driver(read_circuit("surface3"))
+    with pytest.raises(ValueError, match="Unsupported operation"):
+        driver.compile_detector_error_model("S 0\n")
+    with pytest.raises(RuntimeError):
+        driver.compile_detector_error_model("R 0\nTICK\nDEPOLARIZE1(0.9) 0\nTICK\nM 0\n")
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_random_circuits(oracle, seed):
+    """Random Clifford circuits (R/H/CX/M/noise) against both oracles."""
+    rng = np.random.default_rng(seed)
+    nq = 12
+    lines = [f"R {' '.join(map(str, range(nq)))}"]
+    nmeas = 0
+    for layer in range(40):
+        lines.append("TICK")
+        qs = rng.permutation(nq)
+        lines.append(f"DEPOLARIZE1({rng.uniform(1e-4, 1e-2):.6g}) {' '.join(map(str, range(nq)))}")
+        pairs = qs[: 2 * (nq // 4)].reshape(-1, 2)
+        if layer >= 2:
+            lines.append(f"DEPOLARIZE2({rng.uniform(1e-4, 1e-2):.6g}) " + " ".join(f"{a} {b}" for a, b in pairs))
+        lines.append("CX " + " ".join(f"{a} {b}" for a, b in pairs))
+        hq = [int(q) for q in qs[2 * (nq // 4):] if rng.random() < 0.5]
+        if hq:
+            lines.append("H " + " ".join(map(str, hq)))
+        if layer % 8 == 7:
+            lines.append("TICK")
+            m = [int(q) for q in rng.choice(nq, 4, replace=False)]
+            lines.append(f"X_ERROR(0.001) {' '.join(map(str, m))}")
+            lines.append(f"M(0.002) {' '.join(map(str, m))}")
+            nmeas += 4
+            for k in range(1, 5):
+                lines.append(f"DETECTOR rec[-{k}]")
+            if nmeas > 4:
+                lines.append("DETECTOR rec[-1] rec[-5]")
+            lines.append(f"R {' '.join(map(str, m))}")
+    lines.append("TICK")
+    lines.append(f"M {' '.join(map(str, range(nq)))}")
+    lines.append("OBSERVABLE_INCLUDE(0) rec[-1] rec[-2]")
+    lines.append("DETECTOR rec[-3] rec[-4]")
+    text = "\n".join(lines) + "\n"
+    dem = get_driver(text).compile_detector_error_model(text)
+    ref = oracle.compile_stepg(text, 2)
+    assert np.array_equal(dem.hashes, ref.hashes)
+    assert dem.approx_equals(as_dem(ref), REL)

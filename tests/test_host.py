@@ -1,0 +1,90 @@
+"""Host-side product logic (no GPU): the C-ABI library loads and exports
+every declared symbol, the product's parser/STEPG builder matches the oracle
+bit for bit, the DEM text format round-trips the golden files."""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, golden_dem_path, read_circuit
+from greenpeas_amd import _native
+from greenpeas_amd.dem import DetectorErrorModel
+from greenpeas_amd.error_analysis import build_stepg_host, estimate_device_bytes
+
+
+def declared_symbols() -> list[str]:
+    with open(os.path.join(ROOT, "include", "greenpeas_amd.h")) as f:
+        text = f.read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(gp_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _native.load()
+    decl = declared_symbols()
+    assert len(decl) >= 30
+    for name in decl:
+        assert hasattr(lib, name), name
+    assert set(decl) == set(_native.EXPORTS)
+    assert lib.gp_version().startswith(b"greenpeas_amd")
+
+
+@pytest.mark.parametrize("name", ["surface3", "surface5", "bb6", "bb12"])
+@pytest.mark.parametrize("level", [0, 1, 2])
+def test_product_stepg_matches_oracle(oracle, name, level):
+    text = read_circuit(name)
+    adj, probs, dims = build_stepg_host(text, level)
+    s = oracle.build_stepg(text, level)
+    assert np.array_equal(adj, s.adjacency)
+    assert np.array_equal(probs, s.probs)
+    assert dims[3] == oracle.location_count(text)
+
+
+def test_product_parser_errors():
+    with pytest.raises(ValueError, match="Unsupported operation: I"):
+        build_stepg_host("I 0\n")
+    with pytest.raises(ValueError):
+        build_stepg_host("DETECTOR rec[-1]\n")  # record before any measurement
+    with pytest.raises(RuntimeError, match="DEP1"):
+        build_stepg_host("R 0\nTICK\nDEPOLARIZE1(0.8) 0\n")
+    with pytest.raises(RuntimeError, match="XX error in layer l < 1"):
+        build_stepg_host("DEPOLARIZE2(0.01) 0 1\nTICK\n", 0)
+    with pytest.raises(ValueError):
+        build_stepg_host("R 0\n", 3)
+
+
+def test_product_repeat_and_aliases(oracle):
+    text = "RZ 0 1\nREPEAT 3 {\nTICK\nDEPOLARIZE1(0.01) 0 1\nCNOT 0 1\n}\nTICK\nMR(0.01) 0 1\n"
+    adj, probs, _ = build_stepg_host(text)
+    s = oracle.build_stepg(text, 2)
+    assert np.array_equal(adj, s.adjacency) and np.array_equal(probs, s.probs)
+
+
+@pytest.mark.parametrize("name", ["surface3", "surface5"])
+@pytest.mark.parametrize("level", [0, 1, 2])
+def test_dem_text_round_trip(name, level):
+    path = golden_dem_path(name, level)
+    d = DetectorErrorModel.from_file(path, 24 if name == "surface3" else 120, 1)
+    again = DetectorErrorModel.from_text(str(d), d.num_detectors, d.num_observables)
+    assert again == d
+    assert d.canonical().approx_equals(d)
+
+
+def test_estimate_device_bytes_grows_with_circuit():
+    a = estimate_device_bytes(read_circuit("surface3"))
+    b = estimate_device_bytes(read_circuit("bb12"))
+    assert 0 < a < b
+
+
+def test_product_overlapping_channels(oracle):
+    text = ("R 0 1 2\nTICK\nTICK\nTICK\nDEPOLARIZE1(0.01) 0 1 2\nDEPOLARIZE2(0.02) 0 1\nCX 0 2\n"
+            "TICK\nH 1\nTICK\nM(0.001) 0 1 2\nDETECTOR rec[-1]\n")
+    for level in (0, 1, 2):
+        adj, probs, _ = build_stepg_host(text, level)
+        s = oracle.build_stepg(text, level)
+        assert np.array_equal(adj, s.adjacency) and np.array_equal(probs, s.probs)

@@ -1,0 +1,214 @@
+"""The CPU oracle, pinned against the reference's fixtures and known answers.
+
+Pins (see oracle/gp_oracle.h for what the oracle restates):
+* golden DEMs data/dems/surface{3,5}_L{0,1,2}.dem (reference test
+  python/tests/test_error_analysis.py:27-45): order and target sets exact,
+  probabilities within 1e-12 relative;
+* the reference's own Mixer/STEPG/Graph headers, compiled under oracle/_ref
+  (bit-exact adjacency, node probabilities, measurement keys);
+* known answers from tests/QEC/ErrorAnalysis/MixerTest.cpp:112,144 and
+  STEPGTest.cpp:199-210, and published FNV-1a-64 test vectors;
+* the independent forward Pauli-frame compiler (stim semantics).
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ROOT, golden_dem_path, read_circuit
+from greenpeas_amd.dem import DetectorErrorModel
+
+REL = 1e-12
+
+
+def as_dem(od) -> DetectorErrorModel:
+    return DetectorErrorModel(od.probs, od.offsets, od.targets, od.num_detectors, od.num_observables)
+
+
+@pytest.mark.parametrize("name", ["surface3", "surface5"])
+@pytest.mark.parametrize("level", [0, 1, 2])
+def test_oracle_reproduces_golden_dem(oracle, name, level):
+    d = as_dem(oracle.compile_stepg(read_circuit(name), level))
+    g = DetectorErrorModel.from_file(golden_dem_path(name, level), d.num_detectors, d.num_observables)
+    # same errors in the same (ascending FNV hash) order
+    assert d.num_errors == g.num_errors
+    assert np.array_equal(d.offsets, g.offsets)
+    assert np.array_equal(d.targets, g.targets)
+    rel = np.abs(d.probs - g.probs) / g.probs
+    assert rel.max() <= REL
+    # most doubles are bit-identical; the rest differ by the reduction order
+    assert np.mean(d.probs == g.probs) > 0.3
+
+
+def test_golden_check_discriminates_levels(oracle):
+    """A plausible mistake (wrong correlation level) fails the golden check."""
+    d = as_dem(oracle.compile_stepg(read_circuit("surface3"), 1))
+    g = DetectorErrorModel.from_file(golden_dem_path("surface3", 2), d.num_detectors, d.num_observables)
+    assert not d.approx_equals(g)
+
+
+@pytest.mark.parametrize("name", ["surface3", "surface5", "bb6", "bb12"])
+def test_stepg_oracle_matches_frame_oracle(oracle, name):
+    text = read_circuit(name)
+    a = as_dem(oracle.compile_stepg(text, 2))
+    b = as_dem(oracle.compile_frames(text))
+    assert a.num_errors == b.num_errors
+    assert a.approx_equals(b, REL)
+
+
+def test_oracle_bb12_shape(oracle):
+    d = oracle.compile_stepg(read_circuit("bb12"), 2)
+    assert d.num_detectors == 1728 and d.num_observables == 12
+    assert d.num_errors == 67104
+    assert len(d.targets) == 545578
+    assert oracle.location_count(read_circuit("bb12")) == 43632
+    assert oracle.location_count(read_circuit("surface3")) == 539
+
+
+def test_split_known_answers(oracle):
+    # MixerTest.cpp:112 and :144 (tolerance 1e-15, Macros.hpp:23-28)
+    assert abs(oracle.split_dep1(1e-3) - 0.00033344451858030588) <= 1e-15
+    assert abs(oracle.split_dep2(1e-3) - 6.6697798534409714e-05) <= 1e-15
+    # closed forms: three independent X/Y/Z channels of probability s give
+    # total depolarizing probability p = 3 s (1-s)^2 + s^3 ... checked via the
+    # composition identity 1 - 4p/3 = (1 - 2s)^2
+    for p in (1e-4, 1e-3, 0.1, 0.5, 0.75):
+        s = oracle.split_dep1(p)
+        assert abs((1 - 2 * s) ** 2 - (1 - 4 * p / 3)) < 1e-14
+        s2 = oracle.split_dep2(min(p, 0.9375))
+        assert abs((1 - 2 * s2) ** 8 - (1 - 16 * min(p, 0.9375) / 15)) < 1e-13
+
+
+def test_merge_known_answer(oracle):
+    assert oracle.xor_merge(0.25, 0.75) == 0.625  # STEPGTest.cpp:199-210
+    rng = np.random.default_rng(0)
+    for _ in range(100):
+        p, q = rng.random(2)
+        # XOR of independent events: 1 - 2(p+q-2pq) = (1-2p)(1-2q)
+        assert abs((1 - 2 * oracle.xor_merge(p, q)) - (1 - 2 * p) * (1 - 2 * q)) < 1e-15
+
+
+def test_fnv_published_vectors(oracle):
+    # FNV-1a 64 (one XOR + multiply per slot) equals byte-wise FNV-1a when
+    # every slot holds one byte: published vectors
+    assert oracle.fnv1a64_slots([], pad_to=0) == 0xCBF29CE484222325
+    assert oracle.fnv1a64_slots([ord("a")], pad_to=1) == 0xAF63DC4C8601EC8C
+    assert oracle.fnv1a64_slots([ord(c) for c in "foobar"], pad_to=6) == 0x85944171F73967E8
+
+
+def _ref_lib():
+    path = os.path.join(ROOT, "oracle", "_ref", "libgp_refstepg.so")
+    if not os.path.exists(path):
+        pytest.skip("oracle/_ref not built (needs the reference headers)")
+    L = ctypes.CDLL(path)
+    L.gpr_build_stepg.restype = ctypes.c_void_p
+    L.gpr_build_stepg.argtypes = [ctypes.c_char_p, ctypes.c_int]
+    L.gpr_dims.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    L.gpr_copy.argtypes = [ctypes.c_void_p] * 4
+    L.gpr_free.argtypes = [ctypes.c_void_p]
+    L.gpr_hash64_slots32.argtypes = [ctypes.c_void_p]
+    L.gpr_hash64_slots32.restype = ctypes.c_uint64
+    L.gpr_split_dep1.argtypes = [ctypes.c_double]
+    L.gpr_split_dep1.restype = ctypes.c_double
+    return L
+
+
+@pytest.mark.parametrize("name", ["surface3", "surface5", "bb6"])
+@pytest.mark.parametrize("level", [0, 1, 2])
+def test_oracle_stepg_matches_reference_headers(oracle, name, level):
+    L = _ref_lib()
+    text = read_circuit(name)
+    h = L.gpr_build_stepg(text.encode(), level)
+    assert h
+    dims = np.empty(4, np.uint32)
+    L.gpr_dims(h, dims.ctypes.data)
+    adj = np.empty(dims[2], np.uint64)
+    probs = np.empty(dims[2], np.float64)
+    keys = np.empty(dims[3], np.uint32)
+    L.gpr_copy(h, adj.ctypes.data, probs.ctypes.data, keys.ctypes.data)
+    L.gpr_free(h)
+    s = oracle.build_stepg(text, level)
+    assert np.array_equal(adj, s.adjacency)
+    assert np.array_equal(probs, s.probs)  # bit-identical doubles
+    assert np.array_equal(keys, s.keys)
+
+
+def test_oracle_hash_matches_reference_array(oracle):
+    L = _ref_lib()
+    rng = np.random.default_rng(1)
+    for n in range(0, 33, 4):
+        vals = np.sort(rng.choice(5000, n, replace=False)).astype(np.uint32)
+        slots = np.full(32, 0xFFFFFFFF, np.uint32)
+        slots[:n] = vals
+        assert L.gpr_hash64_slots32(slots.ctypes.data) == oracle.fnv1a64_slots(vals, 32)
+    assert L.gpr_split_dep1(1e-3) == oracle.split_dep1(1e-3)
+
+
+# --- edge cases the reference's tests cover (CircuitTest.cpp, MixerTest.cpp) ---
+
+def test_unsupported_gate_is_invalid_argument(oracle):
+    with pytest.raises(oracle.OracleError) as e:
+        oracle.compile_stepg("I 0\n", 2)
+    assert e.value.code == 1 and "Unsupported operation: I" in str(e.value)
+
+
+def test_bad_probability_is_runtime_error(oracle):
+    with pytest.raises(oracle.OracleError) as e:
+        oracle.compile_stepg("R 0\nTICK\nDEPOLARIZE1(0.8) 0\nM 0\nDETECTOR rec[-1]\n", 2)
+    assert e.value.code == 2
+
+
+def test_correlated_noise_in_first_layer_is_runtime_error(oracle):
+    with pytest.raises(oracle.OracleError) as e:
+        oracle.compile_stepg("DEPOLARIZE2(0.01) 0 1\nTICK\nM 0 1\nDETECTOR rec[-1]\n", 0)
+    assert e.value.code == 2
+
+
+def test_empty_and_noiseless_circuits(oracle):
+    assert oracle.compile_stepg("", 2).num_errors == 0
+    assert oracle.compile_stepg("R 0\nTICK\nM 0\nDETECTOR rec[-1]\n", 2).num_errors == 0
+    assert oracle.compile_frames("R 0\nTICK\nM 0\nDETECTOR rec[-1]\n").num_errors == 0
+
+
+def test_repeat_block_equals_unrolled(oracle):
+    body = "TICK\nDEPOLARIZE1(0.01) 0 1\nCX 0 1\n"
+    head = "R 0 1\n"
+    tail = "TICK\nM(0.01) 0 1\nDETECTOR rec[-1]\nDETECTOR rec[-2]\nOBSERVABLE_INCLUDE(0) rec[-1]\n"
+    a = as_dem(oracle.compile_stepg(head + "REPEAT 5 {\n" + body + "}\n" + tail, 2))
+    b = as_dem(oracle.compile_stepg(head + body * 5 + tail, 2))
+    assert a == b and a.num_errors > 0
+
+
+def test_long_class_beyond_32_targets(oracle):
+    """A fault flipping 40 detectors keeps all 40 (the reference truncates at 32)."""
+    text = "R 0\nTICK\nX_ERROR(0.01) 0\nM 0\n" + "DETECTOR rec[-1]\n" * 40
+    d = as_dem(oracle.compile_stepg(text, 2))
+    f = as_dem(oracle.compile_frames(text))
+    assert d.num_errors == 1 and len(d.targets) == 40
+    assert d.approx_equals(f)
+
+
+def test_observable_index_and_repeated_records(oracle):
+    """OBSERVABLE_INCLUDE(k) maps to L k; a record listed twice cancels."""
+    text = ("R 0 1\nTICK\nX_ERROR(0.1) 0 1\nM 0 1\n"
+            "OBSERVABLE_INCLUDE(1) rec[-2]\nDETECTOR rec[-1] rec[-1]\nDETECTOR rec[-1]\n")
+    d = as_dem(oracle.compile_stepg(text, 2))
+    f = as_dem(oracle.compile_frames(text))
+    assert d.approx_equals(f)
+    assert sorted(d.target_lists()) == [(1,), (2 + 1,)]  # D1, and L1 (= 2 detectors + 1)
+
+
+def test_overlapping_channels_on_one_qubit(oracle):
+    """DEPOLARIZE1 and DEPOLARIZE2 on the same qubit in one layer share the
+    Y component node (the reference throws on the duplicate flow)."""
+    text = ("R 0 1 2\nTICK\nTICK\nTICK\nDEPOLARIZE1(0.01) 0 1 2\nDEPOLARIZE2(0.02) 0 1\nCX 0 2\n"
+            "TICK\nH 1\nTICK\nM(0.001) 0 1 2\nDETECTOR rec[-1]\nDETECTOR rec[-2]\nDETECTOR rec[-3]\n"
+            "OBSERVABLE_INCLUDE(0) rec[-1] rec[-3]\n")
+    a = as_dem(oracle.compile_stepg(text, 2))
+    b = as_dem(oracle.compile_frames(text))
+    assert a.num_errors >= 5
+    assert a.approx_equals(b, REL)
