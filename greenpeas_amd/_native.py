@@ -28,7 +28,8 @@ EXPORTS = (
     "gp_batch_device_us", "gp_batch_dem_shape", "gp_batch_dem_copy", "gp_batch_device_bytes",
     "gp_estimate_device_bytes",
     "gp_grad_plan_create", "gp_grad_plan_destroy", "gp_grad_plan_sizes", "gp_grad_plan_dem_structure",
-    "gp_grad_plan_device_arrays", "gp_xor_merge_forward", "gp_xor_merge_backward",
+    "gp_grad_plan_device_arrays", "gp_grad_plan_copy_arrays", "gp_xor_merge_forward",
+    "gp_xor_merge_backward",
 )
 
 _lib = None
@@ -86,6 +87,7 @@ def load() -> ctypes.CDLL:
     sig("gp_grad_plan_sizes", [P, P])
     sig("gp_grad_plan_dem_structure", [P, P, P])
     sig("gp_grad_plan_device_arrays", [P, PP, PP, PP, PP, PP, PP])
+    sig("gp_grad_plan_copy_arrays", [P, P, P, P, P, P, P])
     sig("gp_xor_merge_forward", [P, P, P, u32, P, P])
     sig("gp_xor_merge_backward", [P, P, P, u32, P, P, P])
     _lib = L

@@ -1,0 +1,145 @@
+"""Differentiable DEM probabilities: gradients with respect to per-location
+noise rates (no counterpart in the reference).
+
+The error classes of a circuit depend only on its structure; the
+probabilities follow from the per-location rates theta by
+
+    r_c  = split(theta[loc(c)])      per noise component c: the rate itself
+                                     (X_ERROR, M), 1/2 - 1/2 sqrt(1 - 4t/3)
+                                     (DEPOLARIZE1, Mixer.hpp:164) or
+                                     1/2 - 1/2 (1 - 16t/15)^(1/8)
+                                     (DEPOLARIZE2, Mixer.hpp:184)
+    q_n  = XOR-merge of r_c over the components placed on STEPG node n
+    p_e  = XOR-merge of q_n over the nodes of error class e
+
+Both merges are the same segmented op, ``XorMerge``, whose forward and
+hand-written backward run as HIP kernels (``gp_xor_merge_forward`` /
+``gp_xor_merge_backward``).  The splits are elementwise torch ops.
+
+Import order: this module imports torch before loading the native library,
+so both share torch's HIP runtime.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _native
+from .dem import DetectorErrorModel
+from .error_analysis import CorrelationLevel, circuit_text
+
+
+def _stream_ptr(device: torch.device) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+class XorMerge(torch.autograd.Function):
+    """out[s] = XOR-merge of values[index[k]], k in [seg_start[s], seg_start[s+1])."""
+
+    @staticmethod
+    def forward(ctx, values: torch.Tensor, index: torch.Tensor, seg_start: torch.Tensor) -> torch.Tensor:
+        if values.dtype != torch.float64 or not values.is_cuda:
+            raise TypeError("XorMerge needs a float64 CUDA tensor")
+        values = values.contiguous()
+        nseg = seg_start.numel() - 1
+        out = torch.empty(nseg, dtype=torch.float64, device=values.device)
+        lib = _native.load()
+        _native.check(lib.gp_xor_merge_forward(values.data_ptr(), index.data_ptr(), seg_start.data_ptr(),
+                                               nseg, out.data_ptr(), _stream_ptr(values.device)))
+        ctx.save_for_backward(values, index, seg_start)
+        return out
+
+    @staticmethod
+    def backward(ctx, grad_out: torch.Tensor):
+        values, index, seg_start = ctx.saved_tensors
+        grad = torch.zeros_like(values)
+        g = grad_out.contiguous().to(torch.float64)
+        lib = _native.load()
+        _native.check(lib.gp_xor_merge_backward(values.data_ptr(), index.data_ptr(), seg_start.data_ptr(),
+                                                seg_start.numel() - 1, g.data_ptr(), grad.data_ptr(),
+                                                _stream_ptr(values.device)))
+        return grad, None, None
+
+
+xor_merge = XorMerge.apply
+
+
+class DifferentiableDem:
+    """Structure of a circuit's DEM, compiled once on the GPU; probabilities
+    (and their gradients) for any per-location rates theta.
+
+    theta has one entry per fault location: each DEPOLARIZE1 / X_ERROR / M
+    target and each DEPOLARIZE2 pair, in circuit order."""
+
+    def __init__(self, circuit, correlation_level: CorrelationLevel = CorrelationLevel.L2,
+                 device: int | torch.device = 0):
+        self.device = torch.device("cuda", device) if isinstance(device, int) else torch.device(device)
+        dev_index = self.device.index or 0
+        lib = _native.load()
+        h = ctypes.c_void_p()
+        _native.check(lib.gp_grad_plan_create(circuit_text(circuit).encode(), int(correlation_level),
+                                               dev_index, ctypes.byref(h)))
+        try:
+            sizes = np.zeros(4, np.uint32)
+            _native.check(lib.gp_grad_plan_sizes(h, sizes.ctypes.data))
+            L, K, R, C = (int(x) for x in sizes)
+            self.num_locations, self.num_components, self.num_nodes, self.num_errors = L, K, R, C
+            comp_loc = np.empty(max(K, 1), np.uint32)
+            comp_kind = np.empty(max(K, 1), np.uint8)
+            node_start = np.empty(R + 1, np.uint32)
+            node_comp = np.empty(max(K, 1), np.uint32)
+            class_start = np.empty(C + 1, np.uint32)
+            class_node = np.empty(max(R, 1), np.uint32)
+            _native.check(lib.gp_grad_plan_copy_arrays(h, comp_loc.ctypes.data, comp_kind.ctypes.data,
+                                                       node_start.ctypes.data, node_comp.ctypes.data,
+                                                       class_start.ctypes.data, class_node.ctypes.data))
+            offsets = np.empty(C + 1, np.uint32)
+            # targets: size from the final offset
+            _native.check(lib.gp_grad_plan_dem_structure(h, offsets.ctypes.data, None))
+            targets = np.empty(max(int(offsets[-1]), 1), np.uint32)
+            _native.check(lib.gp_grad_plan_dem_structure(h, offsets.ctypes.data, targets.ctypes.data))
+        finally:
+            lib.gp_grad_plan_destroy(h)
+        dev = self.device
+        as_i32 = lambda a: torch.from_numpy(a.astype(np.int64)).to(dev)  # noqa: E731
+        self.comp_loc = as_i32(comp_loc[:K])
+        kind = torch.from_numpy(comp_kind[:K].astype(np.int64)).to(dev)
+        self.is_dep1 = kind == 1
+        self.is_dep2 = kind == 2
+        # uint32 index arrays for the kernels (int32 storage, same bits)
+        self.node_comp = torch.from_numpy(node_comp[:K].view(np.int32)).to(dev)
+        self.node_start = torch.from_numpy(node_start.view(np.int32)).to(dev)
+        self.class_node = torch.from_numpy(class_node[:R].view(np.int32)).to(dev)
+        self.class_start = torch.from_numpy(class_start.view(np.int32)).to(dev)
+        self.offsets = offsets
+        self.targets = targets[: int(offsets[-1])]
+        self.num_detectors = None
+        self.num_observables = None
+
+    def component_rates(self, theta: torch.Tensor) -> torch.Tensor:
+        t = theta[self.comp_loc]
+        s1 = 0.5 - 0.5 * torch.sqrt(torch.clamp(1 - 4 * t / 3, min=1e-300))
+        s2 = 0.5 - 0.5 * torch.pow(torch.clamp(1 - 16 * t / 15, min=1e-300), 0.125)
+        return torch.where(self.is_dep1, s1, torch.where(self.is_dep2, s2, t))
+
+    def node_probabilities(self, theta: torch.Tensor) -> torch.Tensor:
+        return xor_merge(self.component_rates(theta), self.node_comp, self.node_start)
+
+    def probabilities(self, theta: torch.Tensor) -> torch.Tensor:
+        """DEM error probabilities p_e(theta), in the DEM's output order."""
+        if theta.shape != (self.num_locations,):
+            raise ValueError(f"theta must have shape ({self.num_locations},)")
+        q = self.node_probabilities(theta.to(self.device, torch.float64))
+        return xor_merge(q, self.class_node, self.class_start)
+
+    def dem(self, theta: torch.Tensor, num_detectors: int, num_observables: int) -> DetectorErrorModel:
+        p = self.probabilities(theta).detach().cpu().numpy()
+        return DetectorErrorModel(p, self.offsets.copy(), self.targets.copy(), num_detectors, num_observables)
+
+
+def logit_loss(p: torch.Tensor, w: torch.Tensor) -> torch.Tensor:
+    """L = sum_e w_e * logit(p_e)."""
+    return torch.sum(w * (torch.log(p) - torch.log1p(-p)))

@@ -591,6 +591,25 @@ int gp_grad_plan_device_arrays(const gp_grad_plan *plan, const uint32_t **comp_l
   });
 }
 
+int gp_grad_plan_copy_arrays(const gp_grad_plan *plan, uint32_t *comp_loc, uint8_t *comp_kind,
+                             uint32_t *node_start, uint32_t *node_comp, uint32_t *class_start,
+                             uint32_t *class_node) {
+  return guarded([&] {
+    if (!plan) throw InvalidArgument("plan is NULL");
+    if (hipSetDevice(plan->device) != hipSuccess) throw RuntimeError("hipSetDevice failed");
+    auto down = [](void *dst, const void *src, size_t bytes) {
+      if (dst && bytes && hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) != hipSuccess)
+        throw RuntimeError("hipMemcpy failed");
+    };
+    down(comp_loc, plan->dCompLoc, 4ull * plan->numComp);
+    down(comp_kind, plan->dCompKind, plan->numComp);
+    down(node_start, plan->dNodeStart, 4ull * (plan->numNodes + 1));
+    down(node_comp, plan->dNodeComp, 4ull * plan->numComp);
+    down(class_start, plan->dClassStart, 4ull * (plan->numErrors + 1));
+    down(class_node, plan->dClassNode, 4ull * plan->numNodes);
+  });
+}
+
 int gp_xor_merge_forward(const double *values, const uint32_t *index, const uint32_t *seg_start,
                          uint32_t num_segments, double *out, void *stream) {
   return guarded([&] {

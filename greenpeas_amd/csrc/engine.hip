@@ -40,9 +40,12 @@ constexpr int kTargetBucketFill = 64;     // mean items per bucket the digit wid
 struct VDesc {
   uint32_t numLayers, npl, numCols, numDetectors;
   uint32_t numObservables, numRanks, numSeeds, bucketBits;
-  uint32_t bucketBase, numBuckets, pad0, pad1;
-  uint64_t layerActOff, actOff, seedOff, colOff, rankOff, sOff;
+  uint32_t bucketBase, numBuckets, maxAct, maskWords;
+  uint64_t layerActOff, actOff, seedOff, colOff, rankOff, sOff, maskOff, pad;
 };
+
+// Counter block zeroed at the start of every run: [flags | bucket counts].
+constexpr uint32_t kFlagWords = 16;  // [0] hash collision, [1] target overflow, [2] oversized buckets
 
 // --------------------------------------------------------------------------
 // wave64 helpers
@@ -245,85 +248,131 @@ __device__ __forceinline__ void scanRuns(const uint64_t (&h)[R], const uint32_t 
 // kernels
 // --------------------------------------------------------------------------
 
+// Zeroes the per-run counters and the per-layer column masks.
+__global__ void zeroKernel(uint32_t *__restrict__ a, uint64_t na, uint32_t *__restrict__ b, uint64_t nb) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < na + nb; i += stride) {
+    if (i < na) a[i] = 0u;
+    else b[i - na] = 0u;
+  }
+}
+
+// LDS-only workgroup barrier: orders LDS accesses but lets global loads
+// (the prefetched records of the next layers) stay in flight.
+__device__ __forceinline__ void ldsBarrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+template <int RPT>
+__device__ __forceinline__ void loadLayer(uint4 (&r)[RPT], const uint4 *__restrict__ ac, uint32_t base,
+                                          uint32_t end) {
+#pragma unroll
+  for (int k = 0; k < RPT; ++k) {
+    const uint32_t a = base + threadIdx.x + k * kSweepBlock;
+    r[k] = a < end ? ac[a] : make_uint4(NONE, NONE, NONE, NONE);
+  }
+}
+
 // Backward sensitivity sweep of one 32-bit column of one circuit.  The
-// sensitivities of layers l, l+1, l+2 live in an LDS ring; each layer is one
-// pass over the active-node records followed by one workgroup barrier, whose
-// OR-reduction also detects when the column has become identically zero so
-// the sweep can stop early (no seeds below and two zero layers above).
+// sensitivities of layers l, l+1, l+2 live in an LDS ring indexed by active
+// position; each layer is one pass over the layer's active-node records (RPT
+// per thread, prefetched two layers ahead in registers) followed by one
+// LDS-only barrier whose ballot vote also detects when the column has become
+// identically zero, so the sweep stops early (no seeds below, two zero
+// layers above).  On exit the column's time band [lo, hi] is recorded, and
+// the column is added to the per-layer column masks of those layers.
+template <int RPT>
 __global__ __launch_bounds__(kSweepBlock) void sweepKernel(
     const VDesc *__restrict__ desc, const uint2 *__restrict__ items,
     const uint32_t *__restrict__ layerAct, const uint4 *__restrict__ act,
-    const uint32_t *__restrict__ actRank, const uint32_t *__restrict__ seedVals,
-    const int32_t *__restrict__ colHi, const int32_t *__restrict__ colLow,
-    int32_t *__restrict__ bandLo, uint32_t *__restrict__ S) {
-  extern __shared__ uint32_t ring[];
+    const uint32_t *__restrict__ seedVals, const int32_t *__restrict__ colHi,
+    const int32_t *__restrict__ colLow, int32_t *__restrict__ bandLo, uint32_t *__restrict__ S,
+    uint32_t *__restrict__ masks, int seedsInLds, int layersInLds) {
+  extern __shared__ uint32_t lds[];
+  __shared__ uint32_t waveAny[2][kSweepBlock / 64];
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint2 it = items[blockIdx.x];
   const VDesc d = desc[it.x];
   const uint32_t c = it.y;
   const int32_t hi = colHi[d.colOff + c];
   if (hi < 0) {
-    if (threadIdx.x == 0) bandLo[d.colOff + c] = 0x7FFFFFFF;
+    if (tid == 0) bandLo[d.colOff + c] = 0x7FFFFFFF;
     return;
   }
   const int32_t lowSeed = colLow[d.colOff + c];
-  const uint32_t npl = d.npl;
-  for (uint32_t i = threadIdx.x; i < 3 * npl; i += kSweepBlock) ring[i] = 0u;
-  __syncthreads();
-
-  const uint32_t *la = layerAct + d.layerActOff;
-  const uint4 *ac = act + d.actOff;
-  const uint32_t *ar = actRank + d.actOff;
+  const uint32_t M = d.maxAct;
+  uint32_t *ring = lds;
+  uint32_t *seedL = lds + 3 * M;
+  uint32_t *laL = seedL + (seedsInLds ? d.numSeeds : 0);
   const uint32_t *sv = seedVals + d.seedOff + static_cast<uint64_t>(c) * d.numSeeds;
+  const uint32_t *laG = layerAct + d.layerActOff;
+  for (uint32_t i = tid; i < 3 * M; i += kSweepBlock) ring[i] = 0u;
+  if (seedsInLds)
+    for (uint32_t i = tid; i < d.numSeeds; i += kSweepBlock) seedL[i] = sv[i];
+  if (layersInLds)
+    for (uint32_t i = tid; i <= d.numLayers; i += kSweepBlock) laL[i] = laG[i];
+  __syncthreads();
+  const uint32_t *la = layersInLds ? laL : laG;
+  const uint32_t *seeds = seedsInLds ? seedL : sv;
+  const uint4 *ac = act + d.actOff;
   uint32_t *Sc = S + d.sOff + static_cast<uint64_t>(c) * d.numRanks;
 
-  uint32_t cur = static_cast<uint32_t>(hi % 3) * npl;
-  uint32_t nb1 = static_cast<uint32_t>((hi + 1) % 3) * npl;
-  uint32_t nb2 = static_cast<uint32_t>((hi + 2) % 3) * npl;
+  // register pipeline: r0 = layer l, r1 = l-1, r2 = l-2
+  uint4 r0[RPT], r1[RPT], r2[RPT];
+  uint32_t b0 = la[hi], e0 = la[hi + 1];
+  uint32_t b1 = hi >= 1 ? la[hi - 1] : 0, e1 = b0;
+  uint32_t b2 = hi >= 2 ? la[hi - 2] : 0, e2 = b1;
+  loadLayer<RPT>(r0, ac, b0, e0);
+  loadLayer<RPT>(r1, ac, b1, hi >= 1 ? e1 : 0);
+  loadLayer<RPT>(r2, ac, b2, hi >= 2 ? e2 : 0);
+
+  uint32_t cur = static_cast<uint32_t>(hi % 3) * M;
+  uint32_t nb1 = static_cast<uint32_t>((hi + 1) % 3) * M;
+  uint32_t nb2 = static_cast<uint32_t>((hi + 2) % 3) * M;
   bool prevZero = true;
   int32_t lo = 0;
-
-  // prefetch of this thread's first record of the next layer
-  uint32_t a0 = la[hi], a1 = la[hi + 1];
-  uint32_t a = a0 + threadIdx.x;
-  uint4 rec = make_uint4(0, NONE, NONE, NONE);
-  uint32_t rrank = NONE, seed = 0;
-  if (a < a1) {
-    rec = ac[a];
-    rrank = ar[a];
-    seed = rec.w != NONE ? sv[rec.w] : 0u;
-  }
+  uint32_t parity = 0;
   for (int32_t l = hi; l >= 0; --l) {
     uint32_t any = 0;
-    if (a < a1) {
-      uint32_t s = seed;
-      if (rec.y != NONE) s ^= ring[((rec.y >> 30) == 1 ? nb1 : nb2) + (rec.y & 0x3FFFFFFFu)];
-      if (rec.z != NONE) s ^= ring[((rec.z >> 30) == 1 ? nb1 : nb2) + (rec.z & 0x3FFFFFFFu)];
-      ring[cur + rec.x] = s;
-      if (rrank != NONE) Sc[rrank] = s;
-      any |= s;
-      for (uint32_t b = a + kSweepBlock; b < a1; b += kSweepBlock) {  // wide layers
-        const uint4 rb = ac[b];
-        uint32_t sb = rb.w != NONE ? sv[rb.w] : 0u;
-        if (rb.y != NONE) sb ^= ring[((rb.y >> 30) == 1 ? nb1 : nb2) + (rb.y & 0x3FFFFFFFu)];
-        if (rb.z != NONE) sb ^= ring[((rb.z >> 30) == 1 ? nb1 : nb2) + (rb.z & 0x3FFFFFFFu)];
-        ring[cur + rb.x] = sb;
-        const uint32_t rr = ar[b];
-        if (rr != NONE) Sc[rr] = sb;
-        any |= sb;
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+      const uint32_t a = b0 + tid + k * kSweepBlock;
+      if (a < e0) {
+        const uint4 r = r0[k];
+        uint32_t s = r.z != NONE ? seeds[r.z] : 0u;
+        if (r.x != NONE) s ^= ring[((r.x >> 30) == 1 ? nb1 : nb2) + (r.x & 0x3FFFFFFFu)];
+        if (r.y != NONE) s ^= ring[((r.y >> 30) == 1 ? nb1 : nb2) + (r.y & 0x3FFFFFFFu)];
+        ring[cur + (a - b0)] = s;
+        if (r.w != NONE) Sc[r.w] = s;
+        any |= s;
       }
     }
-    // prefetch the next layer's first record before the barrier
-    if (l > 0) {
-      a1 = a0;
-      a0 = la[l - 1];
-      a = a0 + threadIdx.x;
-      if (a < a1) {
-        rec = ac[a];
-        rrank = ar[a];
-        seed = rec.w != NONE ? sv[rec.w] : 0u;
-      }
+    // shift the pipeline and prefetch layer l-3
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+      r0[k] = r1[k];
+      r1[k] = r2[k];
     }
-    const int anyL = __syncthreads_or(any != 0);
+    b0 = b1;
+    e0 = e1;
+    b1 = b2;
+    e1 = e2;
+    if (l >= 3) {
+      e2 = b2;
+      b2 = la[l - 3];
+      loadLayer<RPT>(r2, ac, b2, e2);
+    } else {
+      e2 = b2 = 0;
+    }
+    const uint64_t vote = __ballot(any != 0);
+    if (lane == 0) waveAny[parity][wave] = vote != 0ull;
+    ldsBarrier();
+    uint32_t anyL = 0;
+#pragma unroll
+    for (int w = 0; w < kSweepBlock / 64; ++w) anyL |= waveAny[parity][w];
+    parity ^= 1;
     if (!anyL && prevZero && l <= lowSeed) {
       lo = l;
       break;
@@ -334,39 +383,48 @@ __global__ __launch_bounds__(kSweepBlock) void sweepKernel(
     nb1 = cur;
     cur = t;
   }
-  if (threadIdx.x == 0) bandLo[d.colOff + c] = lo;
+  if (tid == 0) bandLo[d.colOff + c] = lo;
+  // add the column to the masks of the layers in its band
+  uint32_t *mk = masks + d.maskOff + (c >> 5);
+  const uint32_t bit = 1u << (c & 31);
+  for (int32_t l = lo + static_cast<int32_t>(tid); l <= hi; l += kSweepBlock)
+    atomicOr(&mk[static_cast<uint64_t>(l) * d.maskWords], bit);
 }
 
 // Class extraction + hashing of every p>0 node: the ascending list of set
-// bits, FNV-1a-64 over it padded with 0xFFFFFFFF to 32 slots (lists longer
-// than 32 hash all entries, unpadded), a second hash, the class size, and
-// the bucket histogram.
+// bits (visiting only the columns whose band covers the node's layer, from
+// the layer's column mask), FNV-1a-64 over it padded with 0xFFFFFFFF to 32
+// slots (lists longer than 32 hash all entries, unpadded), a second hash,
+// the class size, and the node's position within its bucket.
 __global__ __launch_bounds__(kClassBlock) void classifyKernel(
     const VDesc *__restrict__ desc, const uint2 *__restrict__ tiles,
-    const int32_t *__restrict__ colHi, const int32_t *__restrict__ bandLo,
-    const uint32_t *__restrict__ rankLayer, const uint32_t *__restrict__ S,
-    uint64_t *__restrict__ hashOut, uint64_t *__restrict__ h2Out, uint32_t *__restrict__ nnzOut,
-    uint32_t *__restrict__ bucketCount) {
+    const uint32_t *__restrict__ masks, const uint32_t *__restrict__ rankLayer,
+    const uint32_t *__restrict__ S, uint64_t *__restrict__ hashOut, uint64_t *__restrict__ h2Out,
+    uint32_t *__restrict__ nnzOut, uint32_t *__restrict__ bucketCount,
+    uint32_t *__restrict__ bucketPos) {
   const uint2 t = tiles[blockIdx.x];
   const VDesc d = desc[t.x];
   const uint32_t i = t.y + threadIdx.x;
   if (i >= d.numRanks) return;
   const uint64_t gi = d.rankOff + i;
-  const int32_t l = static_cast<int32_t>(rankLayer[gi]);
-  const int32_t *hiC = colHi + d.colOff;
-  const int32_t *loC = bandLo + d.colOff;
+  const uint32_t l = rankLayer[gi];
+  const uint32_t *mk = masks + d.maskOff + static_cast<uint64_t>(l) * d.maskWords;
   const uint32_t *Sv = S + d.sOff + i;
   uint64_t h = kFnvBasis, h2 = 0x84222325cbf29ce4ull;
   uint32_t cnt = 0;
-  for (uint32_t c = 0; c < d.numCols; ++c) {
-    if (l < loC[c] || l > hiC[c]) continue;
-    uint32_t w = Sv[static_cast<uint64_t>(c) * d.numRanks];
-    while (w) {
-      const uint32_t tg = c * 32 + static_cast<uint32_t>(__builtin_ctz(w));
-      h = (h ^ tg) * kFnvPrime;
-      h2 = mix2(h2, tg);
-      cnt++;
-      w &= w - 1;
+  for (uint32_t w = 0; w < d.maskWords; ++w) {
+    uint32_t m = mk[w];
+    while (m) {
+      const uint32_t c = w * 32 + static_cast<uint32_t>(__builtin_ctz(m));
+      m &= m - 1;
+      uint32_t x = Sv[static_cast<uint64_t>(c) * d.numRanks];
+      while (x) {
+        const uint32_t tg = c * 32 + static_cast<uint32_t>(__builtin_ctz(x));
+        h = (h ^ tg) * kFnvPrime;
+        h2 = mix2(h2, tg);
+        cnt++;
+        x &= x - 1;
+      }
     }
   }
   if (cnt && cnt <= 32)
@@ -376,16 +434,18 @@ __global__ __launch_bounds__(kClassBlock) void classifyKernel(
   nnzOut[gi] = cnt;
   if (cnt) {
     const uint32_t b = d.bucketBase + (d.bucketBits ? static_cast<uint32_t>(h >> (64 - d.bucketBits)) : 0u);
-    atomicAdd(&bucketCount[b], 1u);
+    bucketPos[gi] = atomicAdd(&bucketCount[b], 1u);
   }
 }
 
-// MSD partition: scatter (hash, rank) into the buckets.  Order inside a
-// bucket is arbitrary; the bucket sort orders by (hash, rank).
+// MSD partition: scatter (hash, rank) into the buckets (positions fixed by
+// the classify pass; order inside a bucket is arbitrary -- the bucket sort
+// orders by (hash, rank)).
 __global__ __launch_bounds__(kClassBlock) void partitionKernel(
     const VDesc *__restrict__ desc, const uint2 *__restrict__ tiles,
     const uint64_t *__restrict__ hash, const uint32_t *__restrict__ nnz,
-    uint32_t *__restrict__ cursor, uint64_t *__restrict__ pkey, uint32_t *__restrict__ prank) {
+    const uint32_t *__restrict__ bucketStart, const uint32_t *__restrict__ bucketPos,
+    uint64_t *__restrict__ pkey, uint32_t *__restrict__ prank) {
   const uint2 t = tiles[blockIdx.x];
   const VDesc d = desc[t.x];
   const uint32_t i = t.y + threadIdx.x;
@@ -394,7 +454,7 @@ __global__ __launch_bounds__(kClassBlock) void partitionKernel(
   if (!nnz[gi]) return;
   const uint64_t h = hash[gi];
   const uint32_t b = d.bucketBase + (d.bucketBits ? static_cast<uint32_t>(h >> (64 - d.bucketBits)) : 0u);
-  const uint32_t pos = atomicAdd(&cursor[b], 1u);
+  const uint32_t pos = bucketStart[b] + bucketPos[gi];
   pkey[pos] = h;
   prank[pos] = static_cast<uint32_t>(gi);
 }
@@ -427,7 +487,7 @@ __global__ __launch_bounds__(256) void bucketKernel(
     RunOut o, uint32_t *__restrict__ runCount, uint32_t *__restrict__ runNnz,
     uint32_t *__restrict__ overflowList) {
   const uint32_t lane = threadIdx.x & 63;
-  const uint32_t b = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const uint32_t b = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
   if (b >= numBuckets) return;
   const uint32_t bs = bucketStart[b];
   const uint32_t cnt = bucketStart[b + 1] - bs;
@@ -509,7 +569,8 @@ __global__ __launch_bounds__(256) void bucketFallbackKernel(
   }
 }
 
-// Compact output: one wave per bucket, one lane per run.
+// Compact output: one wave per bucket, one lane per run.  Block 0 also
+// records where each circuit's errors and targets start.
 __global__ __launch_bounds__(256) void emitKernel(
     const VDesc *__restrict__ desc, uint32_t numVariants, uint32_t numBuckets,
     const uint32_t *__restrict__ bucketStart, const uint32_t *__restrict__ runCount,
@@ -517,12 +578,19 @@ __global__ __launch_bounds__(256) void emitKernel(
     const uint32_t *__restrict__ runRank, const double *__restrict__ runP,
     const uint64_t *__restrict__ runHash, const uint32_t *__restrict__ runTgtPre,
     const uint32_t *__restrict__ runMember, const uint32_t *__restrict__ rankLayer,
-    const int32_t *__restrict__ colHi, const int32_t *__restrict__ bandLo,
-    const uint32_t *__restrict__ S, double *__restrict__ outP, uint32_t *__restrict__ outOff,
-    uint64_t *__restrict__ outHash, uint32_t *__restrict__ outMember,
-    uint32_t *__restrict__ outTargets, uint64_t capTargets, uint32_t *__restrict__ flags) {
+    const uint32_t *__restrict__ masks, const uint32_t *__restrict__ S, double *__restrict__ outP,
+    uint32_t *__restrict__ outOff, uint64_t *__restrict__ outHash, uint32_t *__restrict__ outMember,
+    uint32_t *__restrict__ outTargets, uint64_t capTargets, uint32_t *__restrict__ flags,
+    uint32_t *__restrict__ vBounds) {
   const uint32_t lane = threadIdx.x & 63;
-  const uint32_t b = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (blockIdx.x == 0 && threadIdx.x < 64) {
+    for (uint32_t v = lane; v <= numVariants; v += 64) {
+      const uint32_t bb = v < numVariants ? desc[v].bucketBase : numBuckets;
+      vBounds[2 * v] = classStart[bb];
+      vBounds[2 * v + 1] = tgtStart[bb];
+    }
+  }
+  const uint32_t b = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
   if (b >= numBuckets) return;
   // circuit owning bucket b
   uint32_t lo = 0, hi = numVariants - 1;
@@ -533,8 +601,6 @@ __global__ __launch_bounds__(256) void emitKernel(
   }
   const VDesc d = desc[lo];
   const uint32_t nR = runCount[b], bs = bucketStart[b], cb = classStart[b], tb = tgtStart[b];
-  const int32_t *hiC = colHi + d.colOff;
-  const int32_t *loC = bandLo + d.colOff;
   for (uint32_t r = lane; r < nR; r += 64) {
     const uint32_t c = cb + r;
     const uint32_t rk = runRank[bs + r];
@@ -544,15 +610,20 @@ __global__ __launch_bounds__(256) void emitKernel(
     outOff[c] = static_cast<uint32_t>(pos);
     outMember[c] = bs + runMember[bs + r];
     const uint32_t i = static_cast<uint32_t>(rk - d.rankOff);
-    const int32_t l = static_cast<int32_t>(rankLayer[rk]);
+    const uint32_t l = rankLayer[rk];
+    const uint32_t *mk = masks + d.maskOff + static_cast<uint64_t>(l) * d.maskWords;
     const uint32_t *Sv = S + d.sOff + i;
-    for (uint32_t col = 0; col < d.numCols; ++col) {
-      if (l < loC[col] || l > hiC[col]) continue;
-      uint32_t w = Sv[static_cast<uint64_t>(col) * d.numRanks];
-      while (w) {
-        if (pos < capTargets) outTargets[pos] = col * 32 + static_cast<uint32_t>(__builtin_ctz(w));
-        pos++;
-        w &= w - 1;
+    for (uint32_t w = 0; w < d.maskWords; ++w) {
+      uint32_t m = mk[w];
+      while (m) {
+        const uint32_t col = w * 32 + static_cast<uint32_t>(__builtin_ctz(m));
+        m &= m - 1;
+        uint32_t x = Sv[static_cast<uint64_t>(col) * d.numRanks];
+        while (x) {
+          if (pos < capTargets) outTargets[pos] = col * 32 + static_cast<uint32_t>(__builtin_ctz(x));
+          pos++;
+          x &= x - 1;
+        }
       }
     }
     if (pos > capTargets) atomicOr(&flags[1], 1u);
@@ -563,87 +634,126 @@ __global__ __launch_bounds__(256) void emitKernel(
   }
 }
 
-// Exclusive scan, tile level: 4096 items per 1024-thread block.
-__global__ __launch_bounds__(kScanBlock) void scanTileKernel(const uint32_t *__restrict__ in, uint32_t n,
-                                                             uint32_t *__restrict__ out,
-                                                             uint32_t *__restrict__ out2,
-                                                             uint32_t *__restrict__ tileSums) {
-  __shared__ uint32_t waveSums[kScanBlock / 64];
+// Exclusive scan of one or two u32 arrays (scanned independently), tile
+// level: 4096 items per 1024-thread block.  With one tile it also writes
+// the totals to out[n]; otherwise the tile totals go to tileSums.
+__global__ __launch_bounds__(kScanBlock) void scanTileKernel(
+    const uint32_t *__restrict__ inA, const uint32_t *__restrict__ inB, uint32_t n,
+    uint32_t *__restrict__ outA, uint32_t *__restrict__ outB, uint32_t *__restrict__ tileSums) {
+  __shared__ uint32_t waveSums[2][kScanBlock / 64];
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * 4;
-  uint32_t v[4];
-  uint32_t tsum = 0;
+  uint32_t va[4], vb[4];
+  uint32_t ta = 0, tb = 0;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    v[k] = base + k < n ? in[base + k] : 0u;
-    tsum += v[k];
+    va[k] = base + k < n ? inA[base + k] : 0u;
+    vb[k] = (inB && base + k < n) ? inB[base + k] : 0u;
+    ta += va[k];
+    tb += vb[k];
   }
-  const uint32_t incl = waveInclusiveSum(tsum, lane);
-  if (lane == 63) waveSums[wave] = incl;
+  const uint32_t ia = waveInclusiveSum(ta, lane);
+  const uint32_t ib = waveInclusiveSum(tb, lane);
+  if (lane == 63) {
+    waveSums[0][wave] = ia;
+    waveSums[1][wave] = ib;
+  }
   __syncthreads();
   if (wave == 0) {
-    uint32_t ws = lane < kScanBlock / 64 ? waveSums[lane] : 0u;
-    const uint32_t wi = waveInclusiveSum(ws, lane);
-    if (lane < kScanBlock / 64) waveSums[lane] = wi - ws;
-    if (lane == kScanBlock / 64 - 1) {
+    constexpr uint32_t W = kScanBlock / 64;
+    const uint32_t xa = lane < W ? waveSums[0][lane] : 0u;
+    const uint32_t xb = lane < W ? waveSums[1][lane] : 0u;
+    const uint32_t ya = waveInclusiveSum(xa, lane);
+    const uint32_t yb = waveInclusiveSum(xb, lane);
+    if (lane < W) {
+      waveSums[0][lane] = ya - xa;
+      waveSums[1][lane] = yb - xb;
+    }
+    if (lane == W - 1) {
       if (gridDim.x == 1) {
-        out[n] = wi;
-        if (out2) out2[n] = wi;
+        outA[n] = ya;
+        if (outB) outB[n] = yb;
       } else {
-        tileSums[blockIdx.x] = wi;
+        tileSums[2 * blockIdx.x] = ya;
+        tileSums[2 * blockIdx.x + 1] = yb;
       }
     }
   }
   __syncthreads();
-  uint32_t run = waveSums[wave] + incl - tsum;
+  uint32_t ra = waveSums[0][wave] + ia - ta;
+  uint32_t rb = waveSums[1][wave] + ib - tb;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     if (base + k < n) {
-      out[base + k] = run;
-      if (out2) out2[base + k] = run;
+      outA[base + k] = ra;
+      if (outB) outB[base + k] = rb;
     }
-    run += v[k];
+    ra += va[k];
+    rb += vb[k];
+  }
+}
+
+// Second level: scans the tile totals (interleaved pairs) in one block.
+__global__ __launch_bounds__(kScanBlock) void scanTileSumsKernel(uint32_t *__restrict__ sums,
+                                                                 uint32_t numTiles) {
+  __shared__ uint32_t carry[2];
+  if (threadIdx.x == 0) carry[0] = carry[1] = 0;
+  __syncthreads();
+  for (uint32_t base = 0; base < numTiles; base += kScanBlock) {
+    const uint32_t t = base + threadIdx.x;
+    const uint32_t a = t < numTiles ? sums[2 * t] : 0u, b = t < numTiles ? sums[2 * t + 1] : 0u;
+    __shared__ uint32_t ws[2][kScanBlock / 64];
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t ia = waveInclusiveSum(a, lane), ib = waveInclusiveSum(b, lane);
+    if (lane == 63) {
+      ws[0][wave] = ia;
+      ws[1][wave] = ib;
+    }
+    __syncthreads();
+    if (wave == 0) {
+      constexpr uint32_t W = kScanBlock / 64;
+      const uint32_t xa = lane < W ? ws[0][lane] : 0u, xb = lane < W ? ws[1][lane] : 0u;
+      const uint32_t ya = waveInclusiveSum(xa, lane), yb = waveInclusiveSum(xb, lane);
+      if (lane < W) {
+        ws[0][lane] = ya - xa;
+        ws[1][lane] = yb - xb;
+      }
+    }
+    __syncthreads();
+    const uint32_t ca = carry[0], cb = carry[1];
+    if (t < numTiles) {
+      sums[2 * t] = ca + ws[0][wave] + ia - a;
+      sums[2 * t + 1] = cb + ws[1][wave] + ib - b;
+    }
+    __syncthreads();
+    if (threadIdx.x == kScanBlock - 1) {
+      carry[0] = ca + ws[0][wave] + ia;
+      carry[1] = cb + ws[1][wave] + ib;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    sums[2 * numTiles] = carry[0];
+    sums[2 * numTiles + 1] = carry[1];
   }
 }
 
 __global__ __launch_bounds__(kScanBlock) void scanAddKernel(uint32_t n, const uint32_t *__restrict__ tileOff,
-                                                            uint32_t numTiles, uint32_t *__restrict__ out,
-                                                            uint32_t *__restrict__ out2) {
+                                                            uint32_t numTiles, uint32_t *__restrict__ outA,
+                                                            uint32_t *__restrict__ outB) {
   const uint32_t tile = blockIdx.x;
-  const uint32_t add = tileOff[tile];
+  const uint32_t addA = tileOff[2 * tile], addB = tileOff[2 * tile + 1];
   const uint32_t base = tile * kScanTile;
   for (uint32_t k = threadIdx.x; k < kScanTile; k += kScanBlock) {
     const uint32_t i = base + k;
     if (i < n) {
-      out[i] += add;
-      if (out2) out2[i] += add;
+      outA[i] += addA;
+      if (outB) outB[i] += addB;
     }
   }
   if (tile == 0 && threadIdx.x == 0) {
-    out[n] = tileOff[numTiles];
-    if (out2) out2[n] = tileOff[numTiles];
-  }
-}
-
-__global__ void zeroKernel(uint32_t *__restrict__ p, uint64_t n) {
-  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
-       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
-    p[i] = 0u;
-}
-
-__global__ void variantBoundsKernel(const VDesc *__restrict__ desc, uint32_t numVariants,
-                                    uint32_t numBuckets, const uint32_t *__restrict__ classStart,
-                                    const uint32_t *__restrict__ tgtStart, uint32_t *__restrict__ out,
-                                    const uint32_t *__restrict__ flags) {
-  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v > numVariants) return;
-  const uint32_t bb = v < numVariants ? desc[v].bucketBase : numBuckets;
-  out[2 * v] = classStart[bb];
-  out[2 * v + 1] = tgtStart[bb];
-  if (v == 0) {
-    out[2 * numVariants + 2] = flags[0];
-    out[2 * numVariants + 3] = flags[1];
-    out[2 * numVariants + 4] = flags[2];
+    outA[n] = tileOff[2 * numTiles];
+    if (outB) outB[n] = tileOff[2 * numTiles + 1];
   }
 }
 
@@ -738,43 +848,45 @@ struct Pipeline::Impl {
   hipEvent_t ev[8] = {};
   bool frozen = false;
   bool timedRun = false;
-  hipEvent_t *evOverride = nullptr;   // per-iteration events of benchDevice
+  hipEvent_t *evOverride = nullptr;  // per-iteration events of benchDevice
   std::vector<hipEvent_t> evPool;
 
   // inputs (one device arena, one pinned staging arena with the same layout)
   DevBuf in;
   HostBuf inHost;
-  size_t oDesc = 0, oItems = 0, oTiles = 0, oLayerAct = 0, oAct = 0, oActRank = 0, oSeed = 0,
-         oColHi = 0, oColLow = 0, oRankLayer = 0, oRankProb = 0, inBytes = 0;
-  uint32_t V = 0, nItems = 0, nTiles = 0, B = 0, R = 0, Ccols = 0, maxNpl = 0;
-  uint64_t Swords = 0;
+  size_t oDesc = 0, oItems = 0, oTiles = 0, oLayerAct = 0, oAct = 0, oSeed = 0, oColHi = 0,
+         oColLow = 0, oRankLayer = 0, oRankProb = 0, inBytes = 0;
+  uint32_t V = 0, nItems = 0, nTiles = 0, B = 0, R = 0, Ccols = 0, rpt = 1;
+  uint64_t Swords = 0, maskWordsTotal = 0;
+  size_t sweepLdsBytes = 0;
+  int seedsInLds = 1, layersInLds = 1;
 
-  DevBuf bandLo, S, hash, h2, nnz, bucketCount, bucketStart, cursor, pkey, prank, runRank, runP,
+  DevBuf bandLo, S, hash, h2, nnz, counters, bucketStart, bucketPos, pkey, prank, runRank, runP,
       runHash, runTgtPre, runMember, runCount, runNnz, classStart, tgtStart, outP, outOff, outHash,
-      outMember, outTargets, scratchKey, scratchRank, flags, overflowList, scanTmp, scanTmp2, vBounds;
+      outMember, outTargets, scratchKey, scratchRank, overflowList, scanTmp, vBounds, masks;
   uint64_t capTargets = 0;
 
   // host results
-  HostBuf hP, hOff, hHash, hTargets, hVB;
+  HostBuf hP, hOff, hHash, hTargets, hVB, hFlags;
   uint32_t totalClasses = 0;
   uint64_t totalTargets = 0;
   std::vector<uint32_t> vClass, vTgt;
   std::vector<VDesc> descs;
 
-  size_t sweepLds() const { return static_cast<size_t>(3) * maxNpl * 4; }
+  uint32_t *flags() const { return counters.as<uint32_t>(); }
+  uint32_t *bucketCount() const { return counters.as<uint32_t>() + kFlagWords; }
 
-  void scan(const uint32_t *inp, uint32_t n, uint32_t *out, uint32_t *out2) {
+  // exclusive scan of n entries of inA (and inB), n+1 outputs each
+  void scan(const uint32_t *inA, const uint32_t *inB, uint32_t n, uint32_t *outA, uint32_t *outB) {
     const uint32_t tiles = (n + kScanTile - 1) / kScanTile;
     if (tiles <= 1) {
-      scanTileKernel<<<1, kScanBlock, 0, stream>>>(inp, n, out, out2, nullptr);
+      scanTileKernel<<<1, kScanBlock, 0, stream>>>(inA, inB, n, outA, outB, nullptr);
       return;
     }
-    if (tiles > kScanTile) throw RuntimeError("scan too large");
     uint32_t *ts = scanTmp.as<uint32_t>();
-    uint32_t *to = scanTmp2.as<uint32_t>();
-    scanTileKernel<<<tiles, kScanBlock, 0, stream>>>(inp, n, out, out2, ts);
-    scanTileKernel<<<1, kScanBlock, 0, stream>>>(ts, tiles, to, nullptr, nullptr);
-    scanAddKernel<<<tiles, kScanBlock, 0, stream>>>(n, to, tiles, out, out2);
+    scanTileKernel<<<tiles, kScanBlock, 0, stream>>>(inA, inB, n, outA, outB, ts);
+    scanTileSumsKernel<<<1, kScanBlock, 0, stream>>>(ts, tiles);
+    scanAddKernel<<<tiles, kScanBlock, 0, stream>>>(n, ts, tiles, outA, outB);
   }
 };
 
@@ -809,6 +921,8 @@ void Pipeline::reserve(const std::vector<const StepgImage *> &imgs) {
   impl_->frozen = !growable_;
 }
 
+static constexpr size_t kLdsBudget = 150 * 1024;  // of gfx950's 160 KB per workgroup
+
 void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   Impl &m = *impl_;
   GPA_HIP(hipSetDevice(device_));
@@ -820,15 +934,12 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   // layout
   m.descs.assign(V, VDesc{});
   variants_.assign(V, VariantInfo{});
-  uint64_t layerActOff = 0, actOff = 0, seedOff = 0, colOff = 0, rankOff = 0, sOff = 0;
-  uint32_t bucketBase = 0, maxNpl = 0;
+  uint64_t layerActOff = 0, actOff = 0, seedOff = 0, colOff = 0, rankOff = 0, sOff = 0, maskOff = 0;
+  uint32_t bucketBase = 0, maxAct = 0, maxSeeds = 0, maxLayers = 0;
   std::vector<uint2> items, tiles;
   std::vector<std::pair<int32_t, uint32_t>> order;  // (band estimate, item index)
   for (uint32_t v = 0; v < V; ++v) {
     const StepgImage &g = *imgs[v];
-    if (3ull * g.npl * 4 > 160 * 1024)
-      throw RuntimeError("circuit has too many STEPG nodes per layer for the LDS sweep (" +
-                         std::to_string(g.npl) + ")");
     VDesc &d = m.descs[v];
     d.numLayers = g.numLayers;
     d.npl = g.npl;
@@ -840,12 +951,15 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
     d.bucketBits = bucketBitsFor(g.numRanks);
     d.numBuckets = 1u << d.bucketBits;
     d.bucketBase = bucketBase;
+    d.maxAct = g.maxActive;
+    d.maskWords = (g.numCols + 31) / 32;
     d.layerActOff = layerActOff;
     d.actOff = actOff;
     d.seedOff = seedOff;
     d.colOff = colOff;
     d.rankOff = rankOff;
     d.sOff = sOff;
+    d.maskOff = maskOff;
     variants_[v] = VariantInfo{g.numRanks, g.numCols, g.numDetectors, g.numObservables, bucketBase,
                                d.numBuckets};
     for (uint32_t c = 0; c < g.numCols; ++c) {
@@ -861,10 +975,27 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
     colOff += g.numCols;
     rankOff += g.numRanks;
     sOff += static_cast<uint64_t>(g.numRanks) * g.numCols;
+    maskOff += static_cast<uint64_t>(g.numLayers) * d.maskWords;
     bucketBase += d.numBuckets;
-    maxNpl = std::max(maxNpl, g.npl);
+    maxAct = std::max(maxAct, g.maxActive);
+    maxSeeds = std::max(maxSeeds, g.numSeeds);
+    maxLayers = std::max(maxLayers, g.numLayers);
   }
   if (rankOff >= 0x7FFFFFFFull) throw RuntimeError("too many error nodes in one load");
+  // sweep launch shape: records per thread and what fits in LDS
+  const uint32_t rpt = (maxAct + kSweepBlock - 1) / kSweepBlock;
+  m.rpt = rpt <= 1 ? 1 : rpt <= 2 ? 2 : rpt <= 4 ? 4 : rpt <= 8 ? 8 : 0;
+  if (!m.rpt)
+    throw RuntimeError("circuit has too many active STEPG nodes per layer for the LDS sweep (" +
+                       std::to_string(maxAct) + " > 8192)");
+  size_t lds = 12ull * maxAct;
+  if (lds > kLdsBudget) throw RuntimeError("circuit layer too wide for the LDS sweep");
+  m.seedsInLds = lds + 4ull * maxSeeds <= kLdsBudget;
+  if (m.seedsInLds) lds += 4ull * maxSeeds;
+  m.layersInLds = lds + 4ull * (maxLayers + 1) <= kLdsBudget;
+  if (m.layersInLds) lds += 4ull * (maxLayers + 1);
+  m.sweepLdsBytes = lds;
+
   // longest bands first (they bound the sweep's critical path)
   std::stable_sort(order.begin(), order.end(),
                    [](const auto &a, const auto &b) { return a.first > b.first; });
@@ -878,7 +1009,7 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   m.R = static_cast<uint32_t>(rankOff);
   m.Ccols = static_cast<uint32_t>(colOff);
   m.Swords = sOff;
-  m.maxNpl = maxNpl;
+  m.maskWordsTotal = maskOff;
 
   // input arena
   size_t off = 0;
@@ -892,7 +1023,6 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   m.oTiles = place(sizeof(uint2) * m.nTiles);
   m.oLayerAct = place(4 * layerActOff);
   m.oAct = place(16 * actOff);
-  m.oActRank = place(4 * actOff);
   m.oSeed = place(4 * seedOff);
   m.oColHi = place(4 * colOff);
   m.oColLow = place(4 * colOff);
@@ -910,7 +1040,6 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
     const VDesc &d = m.descs[v];
     std::memcpy(hb + m.oLayerAct + 4 * d.layerActOff, g.layerAct.data(), 4 * g.layerAct.size());
     std::memcpy(hb + m.oAct + 16 * d.actOff, g.act.data(), 4 * g.act.size());
-    std::memcpy(hb + m.oActRank + 4 * d.actOff, g.actRank.data(), 4 * g.actRank.size());
     std::memcpy(hb + m.oSeed + 4 * d.seedOff, g.seedVals.data(), 4 * g.seedVals.size());
     std::memcpy(hb + m.oColHi + 4 * d.colOff, g.colHi.data(), 4 * g.colHi.size());
     std::memcpy(hb + m.oColLow + 4 * d.colOff, g.colLow.data(), 4 * g.colLow.size());
@@ -921,13 +1050,14 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   // work buffers
   const uint64_t R = std::max<uint64_t>(m.R, 1), B = m.B;
   m.bandLo.ensure(4 * std::max<uint32_t>(m.Ccols, 1), frozen, "columns");
+  m.masks.ensure(4 * std::max<uint64_t>(m.maskWordsTotal, 1), frozen, "layers");
   m.S.ensure(4 * std::max<uint64_t>(m.Swords, 1), frozen, "sensitivity matrix");
   m.hash.ensure(8 * R, frozen, "nodes");
   m.h2.ensure(8 * R, frozen, "nodes");
   m.nnz.ensure(4 * R, frozen, "nodes");
-  m.bucketCount.ensure(4 * (B + 1), frozen, "buckets");
+  m.bucketPos.ensure(4 * R, frozen, "nodes");
+  m.counters.ensure(4 * (kFlagWords + B + 1), frozen, "buckets");
   m.bucketStart.ensure(4 * (B + 1), frozen, "buckets");
-  m.cursor.ensure(4 * (B + 1), frozen, "buckets");
   m.pkey.ensure(8 * R, frozen, "nodes");
   m.prank.ensure(4 * R, frozen, "nodes");
   m.runRank.ensure(4 * R, frozen, "nodes");
@@ -949,17 +1079,21 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   }
   m.scratchKey.ensure(16 * R, frozen, "nodes");
   m.scratchRank.ensure(8 * R, frozen, "nodes");
-  m.flags.ensure(64, frozen, "flags");
   m.overflowList.ensure(4 * (B + 1), frozen, "buckets");
-  m.scanTmp.ensure(4 * (kScanTile + 1), frozen, "scan");
-  m.scanTmp2.ensure(4 * (kScanTile + 1), frozen, "scan");
+  m.scanTmp.ensure(8 * ((B + kScanTile - 1) / kScanTile + 2), frozen, "buckets");
   m.vBounds.ensure(4 * (2 * V + 8), frozen, "circuits");
   m.hVB.ensure(4 * (2 * V + 8));
+  m.hFlags.ensure(4 * kFlagWords);
 
-  if (m.sweepLds() > 64 * 1024)
-    GPA_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(sweepKernel),
-                                hipFuncAttributeMaxDynamicSharedMemorySize,
-                                static_cast<int>(m.sweepLds())));
+  if (m.sweepLdsBytes > 64 * 1024) {
+    const void *fns[] = {reinterpret_cast<const void *>(sweepKernel<1>),
+                         reinterpret_cast<const void *>(sweepKernel<2>),
+                         reinterpret_cast<const void *>(sweepKernel<4>),
+                         reinterpret_cast<const void *>(sweepKernel<8>)};
+    for (const void *f : fns)
+      GPA_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  static_cast<int>(m.sweepLdsBytes)));
+  }
 
   // upload (one copy)
   GPA_HIP(hipEventRecord(m.ev[0], m.stream));
@@ -980,7 +1114,6 @@ void Pipeline::run(bool timed) {
   const uint2 *tiles = reinterpret_cast<const uint2 *>(ib + m.oTiles);
   const uint32_t *layerAct = reinterpret_cast<const uint32_t *>(ib + m.oLayerAct);
   const uint4 *act = reinterpret_cast<const uint4 *>(ib + m.oAct);
-  const uint32_t *actRank = reinterpret_cast<const uint32_t *>(ib + m.oActRank);
   const uint32_t *seed = reinterpret_cast<const uint32_t *>(ib + m.oSeed);
   const int32_t *colHi = reinterpret_cast<const int32_t *>(ib + m.oColHi);
   const int32_t *colLow = reinterpret_cast<const int32_t *>(ib + m.oColLow);
@@ -994,27 +1127,42 @@ void Pipeline::run(bool timed) {
   const uint32_t B = m.B;
   m.timedRun = timed;
 
-  // counters that must start at zero (fill kernels, safe under graph capture)
-  zeroKernel<<<std::max<uint32_t>(1, std::min<uint32_t>((B + 256) / 256, 1024)), 256, 0, st>>>(
-      m.bucketCount.as<uint32_t>(), B + 1);
-  zeroKernel<<<1, 64, 0, st>>>(m.flags.as<uint32_t>(), 16);
+  // per-run counters and masks start at zero (a fill kernel, not a memset,
+  // so the sequence stays safe to capture in a graph)
+  {
+    const uint64_t n = kFlagWords + B + 1 + m.maskWordsTotal;
+    const uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>((n + 255) / 256, 2048));
+    zeroKernel<<<std::max<uint32_t>(blocks, 1), 256, 0, st>>>(m.counters.as<uint32_t>(),
+                                                               kFlagWords + B + 1,
+                                                               m.masks.as<uint32_t>(), m.maskWordsTotal);
+  }
   mark(0);
-  if (m.nItems)
-    sweepKernel<<<m.nItems, kSweepBlock, m.sweepLds(), st>>>(desc, items, layerAct, act, actRank, seed,
-                                                             colHi, colLow, m.bandLo.as<int32_t>(),
-                                                             m.S.as<uint32_t>());
+  if (m.nItems) {
+#define GPA_SWEEP(RPT)                                                                           \
+  sweepKernel<RPT><<<m.nItems, kSweepBlock, m.sweepLdsBytes, st>>>(                              \
+      desc, items, layerAct, act, seed, colHi, colLow, m.bandLo.as<int32_t>(), m.S.as<uint32_t>(), \
+      m.masks.as<uint32_t>(), m.seedsInLds, m.layersInLds)
+    switch (m.rpt) {
+    case 1: GPA_SWEEP(1); break;
+    case 2: GPA_SWEEP(2); break;
+    case 4: GPA_SWEEP(4); break;
+    default: GPA_SWEEP(8); break;
+    }
+#undef GPA_SWEEP
+  }
   mark(1);
   if (m.nTiles)
-    classifyKernel<<<m.nTiles, kClassBlock, 0, st>>>(desc, tiles, colHi, m.bandLo.as<int32_t>(),
-                                                     rankLayer, m.S.as<uint32_t>(),
-                                                     m.hash.as<uint64_t>(), m.h2.as<uint64_t>(),
-                                                     m.nnz.as<uint32_t>(), m.bucketCount.as<uint32_t>());
+    classifyKernel<<<m.nTiles, kClassBlock, 0, st>>>(desc, tiles, m.masks.as<uint32_t>(), rankLayer,
+                                                     m.S.as<uint32_t>(), m.hash.as<uint64_t>(),
+                                                     m.h2.as<uint64_t>(), m.nnz.as<uint32_t>(),
+                                                     m.bucketCount(), m.bucketPos.as<uint32_t>());
   mark(2);
-  m.scan(m.bucketCount.as<uint32_t>(), B, m.bucketStart.as<uint32_t>(), m.cursor.as<uint32_t>());
+  m.scan(m.bucketCount(), nullptr, B, m.bucketStart.as<uint32_t>(), nullptr);
   if (m.nTiles)
     partitionKernel<<<m.nTiles, kClassBlock, 0, st>>>(desc, tiles, m.hash.as<uint64_t>(),
-                                                      m.nnz.as<uint32_t>(), m.cursor.as<uint32_t>(),
-                                                      m.pkey.as<uint64_t>(), m.prank.as<uint32_t>());
+                                                      m.nnz.as<uint32_t>(), m.bucketStart.as<uint32_t>(),
+                                                      m.bucketPos.as<uint32_t>(), m.pkey.as<uint64_t>(),
+                                                      m.prank.as<uint32_t>());
   mark(3);
   RunOut o{rankProb,
            m.h2.as<uint64_t>(),
@@ -1025,7 +1173,7 @@ void Pipeline::run(bool timed) {
            m.runHash.as<uint64_t>(),
            m.runTgtPre.as<uint32_t>(),
            m.runMember.as<uint32_t>(),
-           m.flags.as<uint32_t>()};
+           m.flags()};
   bucketKernel<<<(B + 3) / 4, 256, 0, st>>>(B, m.bucketStart.as<uint32_t>(), m.pkey.as<uint64_t>(), o,
                                             m.runCount.as<uint32_t>(), m.runNnz.as<uint32_t>(),
                                             m.overflowList.as<uint32_t>());
@@ -1034,19 +1182,16 @@ void Pipeline::run(bool timed) {
                                            m.overflowList.as<uint32_t>(), m.scratchKey.as<uint64_t>(),
                                            m.scratchRank.as<uint32_t>());
   mark(4);
-  m.scan(m.runCount.as<uint32_t>(), B, m.classStart.as<uint32_t>(), nullptr);
-  m.scan(m.runNnz.as<uint32_t>(), B, m.tgtStart.as<uint32_t>(), nullptr);
+  m.scan(m.runCount.as<uint32_t>(), m.runNnz.as<uint32_t>(), B, m.classStart.as<uint32_t>(),
+         m.tgtStart.as<uint32_t>());
   emitKernel<<<(B + 3) / 4, 256, 0, st>>>(
       desc, m.V, B, m.bucketStart.as<uint32_t>(), m.runCount.as<uint32_t>(),
       m.classStart.as<uint32_t>(), m.tgtStart.as<uint32_t>(), m.runRank.as<uint32_t>(),
       m.runP.as<double>(), m.runHash.as<uint64_t>(), m.runTgtPre.as<uint32_t>(),
-      m.runMember.as<uint32_t>(), rankLayer, colHi, m.bandLo.as<int32_t>(), m.S.as<uint32_t>(),
+      m.runMember.as<uint32_t>(), rankLayer, m.masks.as<uint32_t>(), m.S.as<uint32_t>(),
       m.outP.as<double>(), m.outOff.as<uint32_t>(), m.outHash.as<uint64_t>(),
-      m.outMember.as<uint32_t>(), m.outTargets.as<uint32_t>(), m.capTargets, m.flags.as<uint32_t>());
-  variantBoundsKernel<<<(m.V + 1 + 63) / 64, 64, 0, st>>>(desc, m.V, B, m.classStart.as<uint32_t>(),
-                                                          m.tgtStart.as<uint32_t>(),
-                                                          m.vBounds.as<uint32_t>(),
-                                                          m.flags.as<uint32_t>());
+      m.outMember.as<uint32_t>(), m.outTargets.as<uint32_t>(), m.capTargets, m.flags(),
+      m.vBounds.as<uint32_t>());
   mark(5);
   GPA_HIP(hipGetLastError());
 }
@@ -1056,7 +1201,8 @@ void Pipeline::download() {
   GPA_HIP(hipSetDevice(device_));
   const uint32_t V = m.V;
   const auto t0 = std::chrono::steady_clock::now();
-  GPA_HIP(hipMemcpyAsync(m.hVB.p, m.vBounds.p, 4 * (2 * V + 5), hipMemcpyDeviceToHost, m.stream));
+  GPA_HIP(hipMemcpyAsync(m.hVB.p, m.vBounds.p, 4 * (2 * V + 2), hipMemcpyDeviceToHost, m.stream));
+  GPA_HIP(hipMemcpyAsync(m.hFlags.p, m.counters.p, 4 * kFlagWords, hipMemcpyDeviceToHost, m.stream));
   GPA_HIP(hipStreamSynchronize(m.stream));
   if (m.timedRun) {
     float ms[5] = {};
@@ -1069,29 +1215,26 @@ void Pipeline::download() {
     m.timedRun = false;
   }
   const uint32_t *vb = m.hVB.as<uint32_t>();
-  if (vb[2 * V + 2]) throw RuntimeError("64-bit class hash collision between different error classes");
+  const uint32_t *fl = m.hFlags.as<uint32_t>();
+  if (fl[0]) throw RuntimeError("64-bit class hash collision between different error classes");
   m.totalClasses = vb[2 * V];
   m.totalTargets = vb[2 * V + 1];
-  if (vb[2 * V + 3]) {
-    // target buffer too small: grow it and re-emit everything
+  if (fl[1]) {
+    // target buffer too small: grow it and re-run everything
     if (m.frozen && 4 * m.totalTargets > m.outTargets.cap)
       throw RuntimeError("circuit exceeds the driver's capacity (targets)");
-    const bool fr = m.frozen;
-    m.frozen = false;
     m.outTargets.ensure(4 * (m.totalTargets + m.totalTargets / 4 + 64), false, "targets");
-    m.frozen = fr;
     m.capTargets = m.outTargets.cap / 4;
     run(false);
     download();
     return;
   }
+  m.vClass.resize(V + 1);
   m.vTgt.resize(V + 1);
-  std::vector<uint32_t> vc(V + 1);
   for (uint32_t v = 0; v <= V; ++v) {
-    vc[v] = vb[2 * v];
+    m.vClass[v] = vb[2 * v];
     m.vTgt[v] = vb[2 * v + 1];
   }
-  m.vClass = vc;
   const uint64_t C = m.totalClasses, T = m.totalTargets;
   m.hP.ensure(8 * C + 8);
   m.hOff.ensure(4 * (C + 1));
@@ -1166,12 +1309,12 @@ double Pipeline::benchDevice(uint32_t iters) {
 
 uint64_t Pipeline::deviceBytes() const {
   const Impl &m = *impl_;
-  const DevBuf *bufs[] = {&m.in, &m.bandLo, &m.S, &m.hash, &m.h2, &m.nnz, &m.bucketCount,
-                          &m.bucketStart, &m.cursor, &m.pkey, &m.prank, &m.runRank, &m.runP,
+  const DevBuf *bufs[] = {&m.in, &m.bandLo, &m.masks, &m.S, &m.hash, &m.h2, &m.nnz, &m.bucketPos,
+                          &m.counters, &m.bucketStart, &m.pkey, &m.prank, &m.runRank, &m.runP,
                           &m.runHash, &m.runTgtPre, &m.runMember, &m.runCount, &m.runNnz,
                           &m.classStart, &m.tgtStart, &m.outP, &m.outOff, &m.outHash, &m.outMember,
-                          &m.outTargets, &m.scratchKey, &m.scratchRank, &m.flags, &m.overflowList,
-                          &m.scanTmp, &m.scanTmp2, &m.vBounds};
+                          &m.outTargets, &m.scratchKey, &m.scratchRank, &m.overflowList, &m.scanTmp,
+                          &m.vBounds};
   uint64_t s = 0;
   for (const DevBuf *b : bufs) s += b->cap;
   return s;
@@ -1194,7 +1337,8 @@ void Pipeline::debugSensitivities(uint32_t v, std::vector<uint32_t> &cols32) con
   for (uint32_t c = 0; c < d.numCols; ++c)
     for (uint32_t i = 0; i < d.numRanks; ++i) {
       const int32_t l = static_cast<int32_t>(layer[i]);
-      if (l >= lo[c] && l <= hi[c]) cols32[static_cast<size_t>(i) * d.numCols + c] = S[static_cast<size_t>(c) * d.numRanks + i];
+      if (l >= lo[c] && l <= hi[c])
+        cols32[static_cast<size_t>(i) * d.numCols + c] = S[static_cast<size_t>(c) * d.numRanks + i];
     }
 }
 

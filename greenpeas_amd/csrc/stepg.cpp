@@ -326,36 +326,46 @@ void buildImage(const Stepg &g, StepgImage &img) {
     }
   }
 
-  // active node records, layer by layer
+  // active index of every active node within its layer
+  std::vector<uint32_t> actIdx(N, kNone);
   img.layerAct.assign(L + 1, 0);
-  img.act.clear();
-  img.actRank.clear();
-  img.act.reserve(static_cast<size_t>(N) * 2);
+  uint32_t total = 0, maxAct = 0;
+  for (uint32_t l = 0, n = 0; l < L; ++l) {
+    img.layerAct[l] = total;
+    uint32_t k = 0;
+    for (uint32_t j = 0; j < npl; ++j, ++n)
+      if (active[n]) actIdx[n] = k++;
+    total += k;
+    maxAct = std::max(maxAct, k);
+  }
+  img.layerAct[L] = total;
+  img.numActive = total;
+  img.maxActive = maxAct;
+
+  // active node records, layer by layer
+  img.act.assign(static_cast<size_t>(total) * 4, kNone);
   auto code = [&](uint32_t child, uint32_t l) -> uint32_t {
     if (child == kNone) return kNone;
     const uint32_t lc = child / npl;
-    return ((lc - l) << 30) | (child - lc * npl);
+    return ((lc - l) << 30) | actIdx[child];
   };
-  for (uint32_t l = 0, n = 0; l < L; ++l) {
-    img.layerAct[l] = static_cast<uint32_t>(img.actRank.size());
+  size_t w = 0;
+  for (uint32_t l = 0, n = 0; l < L; ++l)
     for (uint32_t j = 0; j < npl; ++j, ++n) {
       if (!active[n]) continue;
       const uint64_t a = g.adjacency[n];
-      img.act.push_back(j);
-      img.act.push_back(code(static_cast<uint32_t>(a), l));
-      img.act.push_back(code(static_cast<uint32_t>(a >> 32), l));
-      img.act.push_back(seedOf[n]);
-      img.actRank.push_back(rank[n]);
+      img.act[w++] = code(static_cast<uint32_t>(a), l);
+      img.act[w++] = code(static_cast<uint32_t>(a >> 32), l);
+      img.act[w++] = seedOf[n];
+      img.act[w++] = rank[n];
     }
-  }
-  img.layerAct[L] = static_cast<uint32_t>(img.actRank.size());
-  img.numActive = img.layerAct[L];
 }
 
 uint64_t imageDeviceBytes(const StepgImage &img) {
   const uint64_t R = img.numRanks;
   uint64_t b = 0;
-  b += (img.numLayers + 1) * 4ull + img.numActive * 20ull + img.seedVals.size() * 4ull;
+  b += (img.numLayers + 1) * 4ull + img.numActive * 16ull + img.seedVals.size() * 4ull;
+  b += img.numLayers * ((img.numCols + 31) / 32) * 4ull;  // per-layer column masks
   b += img.numCols * 12ull + R * 12ull;  // bands, rank layer + prob
   b += R * img.numCols * 4ull;            // sensitivity matrix
   b += R * (8 + 8 + 4 + 4);               // hash, h2, nnz, class-of-rank

@@ -13,9 +13,10 @@
 // The device image re-lays that graph out for the MI355X sweep:
 //  * only "active" nodes are kept (nodes with p > 0 and every node reachable
 //    from one along flows; the others can never influence an output);
-//  * each active node is one 16-byte record {j, child0, child1, seed} with
-//    children encoded as (layer delta << 30) | index-in-layer, so the sweep
-//    resolves them inside a three-layer LDS ring;
+//  * each active node is one 16-byte record {child0, child1, seed, rank}
+//    with children encoded as (layer delta << 30) | active index within the
+//    child's layer, so the sweep resolves them inside a three-layer LDS ring
+//    indexed by active position;
 //  * measurement seeds are stored once per seeded node, column-major over
 //    32-bit detector words ("columns");
 //  * per column, the highest and lowest layer holding a seed bound the time
@@ -66,10 +67,9 @@ void buildStepg(const Circuit &c, int level, Stepg &g, const double *theta = nul
 // Device-ready arrays of one circuit (see the file comment).
 struct StepgImage {
   uint32_t numLayers = 0, npl = 0, numCols = 0, numDetectors = 0, numObservables = 0;
-  uint32_t numRanks = 0, numSeeds = 0, numActive = 0, numLocations = 0;
+  uint32_t numRanks = 0, numSeeds = 0, numActive = 0, numLocations = 0, maxActive = 0;
   std::vector<uint32_t> layerAct;  // numLayers + 1 offsets into act
-  std::vector<uint32_t> act;       // 4 * numActive: j, child0 code, child1 code, seed row
-  std::vector<uint32_t> actRank;   // numActive: rank or kNone
+  std::vector<uint32_t> act;       // 4 * numActive: child0 code, child1 code, seed row, rank
   std::vector<uint32_t> seedVals;  // numCols x numSeeds (column-major)
   std::vector<int32_t> colHi, colLow;  // numCols: seed layer range (-1 = no seed)
   std::vector<uint32_t> rankLayer; // numRanks

@@ -189,6 +189,12 @@ int gp_grad_plan_device_arrays(const gp_grad_plan *plan,
                                const uint32_t **class_start,
                                const uint32_t **class_node);
 
+/* Same arrays copied to host memory (any pointer may be NULL). */
+int gp_grad_plan_copy_arrays(const gp_grad_plan *plan, uint32_t *comp_loc,
+                             uint8_t *comp_kind, uint32_t *node_start,
+                             uint32_t *node_comp, uint32_t *class_start,
+                             uint32_t *class_node);
+
 /* Segmented XOR-merge, forward: out[s] = XOR-merge of values[index[k]] for k
  * in [seg_start[s], seg_start[s+1]); the merge p (+) q = p + q - 2pq is the
  * identity 1 - 2 out = prod(1 - 2 values), evaluated without cancellation.

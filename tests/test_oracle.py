@@ -212,3 +212,32 @@ def test_overlapping_channels_on_one_qubit(oracle):
     b = as_dem(oracle.compile_frames(text))
     assert a.num_errors >= 5
     assert a.approx_equals(b, REL)
+
+
+def test_oracle_gradient_matches_central_differences(oracle):
+    """The oracle's FP64 chain-rule gradient of L = sum w_e logit(p_e)
+    against central differences of its own compile (relative step 1e-6)."""
+    text = read_circuit("surface3")
+    n = oracle.location_count(text)
+    rng = np.random.default_rng(7)
+    theta = np.exp(rng.uniform(np.log(1e-4), np.log(5e-3), n))
+    d = oracle.compile_stepg(text, 2, theta)
+    w = rng.standard_normal(d.num_errors)
+
+    def loss(th):
+        p = oracle.compile_stepg(text, 2, th).probs
+        return float(np.sum(w * (np.log(p) - np.log1p(-p))))
+
+    L, g = oracle.logit_loss_grad(text, theta, w, 2)
+    assert abs(L - loss(theta)) <= 1e-12 * abs(L)
+    p = d.probs
+    noise = 8 * np.finfo(float).eps * float(np.sum(np.abs(w * (np.log(p) - np.log1p(-p)))))
+    for k in rng.choice(n, 25, replace=False):
+        h = 1e-6 * theta[k]
+        tp, tm = theta.copy(), theta.copy()
+        tp[k] += h
+        tm[k] -= h
+        fd = (loss(tp) - loss(tm)) / (2 * h)
+        # truncation O(h^2) is far below the roundoff of L divided by h
+        assert abs(fd - g[k]) <= 1e-6 * abs(g[k]) + noise / h, (k, fd, g[k])
+        assert abs(fd - g[k]) <= 1e-3 * abs(g[k]) + 1e-9  # discriminating bound
