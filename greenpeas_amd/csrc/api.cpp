@@ -376,6 +376,25 @@ int gp_batch_set_circuits(gp_batch *batch, const char *const *texts, uint32_t co
   });
 }
 
+// Device microseconds of one replay of the loaded pipeline's graph (the
+// first call after a load captures it; that capture is not timed).
+static double timeGraph(gpa::Pipeline &p) {
+  hipStream_t st = static_cast<hipStream_t>(p.stream());
+  hipEvent_t a, b;
+  if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess)
+    throw RuntimeError("hipEventCreate failed");
+  p.runGraph();  // capture + warm
+  (void)hipEventRecord(a, st);
+  p.runGraph();
+  (void)hipEventRecord(b, st);
+  (void)hipEventSynchronize(b);
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, a, b);
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return ms * 1e3;
+}
+
 static void keepResult(gp_batch *b, uint32_t i, const gpa::VariantResult &r) {
   b->ownP[i].assign(r.probs, r.probs + r.numErrors);
   b->ownOff[i].assign(r.offsets, r.offsets + r.numErrors + 1);
@@ -409,7 +428,8 @@ int gp_batch_compile(gp_batch *batch, int mode) {
         batch->pipe->load(imgs);
         batch->loaded = true;
       }
-      batch->deviceUs = batch->pipe->benchDevice(1);
+      // device time of one replay of the captured pipeline
+      batch->deviceUs = timeGraph(*batch->pipe);
       batch->pipe->download();
       for (uint32_t i = 0; i < n; ++i) {
         gpa::VariantResult r;
@@ -421,7 +441,7 @@ int gp_batch_compile(gp_batch *batch, int mode) {
       double total = 0;
       for (uint32_t i = 0; i < n; ++i) {
         batch->single->load({&batch->circuits[i]->image});
-        total += batch->single->benchDevice(1);
+        total += timeGraph(*batch->single);
         batch->single->download();
         gpa::VariantResult r;
         batch->single->result(0, r);
@@ -607,6 +627,22 @@ int gp_grad_plan_copy_arrays(const gp_grad_plan *plan, uint32_t *comp_loc, uint8
     down(node_comp, plan->dNodeComp, 4ull * plan->numComp);
     down(class_start, plan->dClassStart, 4ull * (plan->numErrors + 1));
     down(class_node, plan->dClassNode, 4ull * plan->numNodes);
+  });
+}
+
+int gp_set_tuning(const char *name, uint32_t value) {
+  return guarded([&] {
+    if (!name) throw InvalidArgument("name is NULL");
+    if (std::string(name) == "bucket_fill") gpa::setBucketFill(value);
+    else throw InvalidArgument(std::string("unknown tuning parameter: ") + name);
+  });
+}
+
+int gp_get_tuning(const char *name, uint32_t *value) {
+  return guarded([&] {
+    if (!name || !value) throw InvalidArgument("NULL argument");
+    if (std::string(name) == "bucket_fill") *value = gpa::bucketFill();
+    else throw InvalidArgument(std::string("unknown tuning parameter: ") + name);
   });
 }
 

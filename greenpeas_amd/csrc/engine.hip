@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstring>
 #include <string>
@@ -34,7 +35,7 @@ constexpr int kClassBlock = 256;
 constexpr int kScanBlock = 1024;
 constexpr int kScanTile = 4 * kScanBlock;
 constexpr int kBucketItemsPerWave = 256;  // wave-register sort capacity (4 per lane)
-constexpr int kTargetBucketFill = 64;     // mean items per bucket the digit width aims at
+std::atomic<uint32_t> gBucketFill{64};    // mean items per bucket the digit width aims at
 
 // Per-circuit descriptor (device and host).
 struct VDesc {
@@ -850,6 +851,16 @@ struct Pipeline::Impl {
   bool timedRun = false;
   hipEvent_t *evOverride = nullptr;  // per-iteration events of benchDevice
   std::vector<hipEvent_t> evPool;
+  // the whole pipeline captured once per load and replayed
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graphExec = nullptr;
+
+  void dropGraph() {
+    if (graphExec) (void)hipGraphExecDestroy(graphExec);
+    if (graph) (void)hipGraphDestroy(graph);
+    graphExec = nullptr;
+    graph = nullptr;
+  }
 
   // inputs (one device arena, one pinned staging arena with the same layout)
   DevBuf in;
@@ -903,6 +914,7 @@ Pipeline::~Pipeline() {
   for (auto &e : impl_->ev)
     if (e) (void)hipEventDestroy(e);
   for (auto &e : impl_->evPool) (void)hipEventDestroy(e);
+  impl_->dropGraph();
   if (impl_->stream) (void)hipStreamDestroy(impl_->stream);
   delete impl_;
 }
@@ -911,7 +923,8 @@ void *Pipeline::stream() const { return impl_->stream; }
 
 static uint32_t bucketBitsFor(uint32_t numRanks) {
   uint32_t b = 0;
-  while (b < 24 && (static_cast<uint64_t>(kTargetBucketFill) << b) < numRanks) b++;
+  const uint64_t fill = gBucketFill.load(std::memory_order_relaxed);
+  while (b < 24 && (fill << b) < numRanks) b++;
   return b;
 }
 
@@ -930,6 +943,7 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   const bool frozen = m.frozen;
   const uint32_t V = static_cast<uint32_t>(imgs.size());
   if (V == 0) throw InvalidArgument("no circuits to compile");
+  m.dropGraph();
 
   // layout
   m.descs.assign(V, VDesc{});
@@ -1269,29 +1283,52 @@ void Pipeline::result(uint32_t v, VariantResult &out) const {
   out.offsets = out.offsetsRel.data();
 }
 
+void Pipeline::runGraph() {
+  Impl &m = *impl_;
+  GPA_HIP(hipSetDevice(device_));
+  if (!m.graphExec) {
+    GPA_HIP(hipStreamBeginCapture(m.stream, hipStreamCaptureModeThreadLocal));
+    try {
+      run(false);
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(m.stream, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    GPA_HIP(hipStreamEndCapture(m.stream, &m.graph));
+    GPA_HIP(hipGraphInstantiate(&m.graphExec, m.graph, nullptr, nullptr, 0));
+  }
+  GPA_HIP(hipGraphLaunch(m.graphExec, m.stream));
+}
+
 double Pipeline::benchDevice(uint32_t iters) {
   Impl &m = *impl_;
   GPA_HIP(hipSetDevice(device_));
   iters = std::max<uint32_t>(iters, 1);
-  run(false);  // warm
-  // six stage events per iteration, read after the last one completes
+  runGraph();  // warm (and capture)
+  // throughput: back-to-back graph replays between two events
+  GPA_HIP(hipEventRecord(m.ev[6], m.stream));
+  for (uint32_t i = 0; i < iters; ++i) runGraph();
+  GPA_HIP(hipEventRecord(m.ev[7], m.stream));
+  GPA_HIP(hipEventSynchronize(m.ev[7]));
+  float ms = 0;
+  GPA_HIP(hipEventElapsedTime(&ms, m.ev[6], m.ev[7]));
+  // per-stage device time: the same iterations launched directly with six
+  // events each, read after the last one completes
   const size_t need = static_cast<size_t>(iters) * 6;
   while (m.evPool.size() < need) {
     hipEvent_t e;
     GPA_HIP(hipEventCreate(&e));
     m.evPool.push_back(e);
   }
-  GPA_HIP(hipEventRecord(m.ev[6], m.stream));
   for (uint32_t i = 0; i < iters; ++i) {
     m.evOverride = m.evPool.data() + static_cast<size_t>(i) * 6;
     run(true);
   }
   m.evOverride = nullptr;
   m.timedRun = false;
-  GPA_HIP(hipEventRecord(m.ev[7], m.stream));
-  GPA_HIP(hipEventSynchronize(m.ev[7]));
-  float ms = 0;
-  GPA_HIP(hipEventElapsedTime(&ms, m.ev[6], m.ev[7]));
+  GPA_HIP(hipStreamSynchronize(m.stream));
   double st[5] = {0, 0, 0, 0, 0};
   for (uint32_t i = 0; i < iters; ++i)
     for (int k = 0; k < 5; ++k) {
@@ -1306,6 +1343,9 @@ double Pipeline::benchDevice(uint32_t iters) {
   times_.emit = st[4] * 1e3 / iters;
   return ms * 1e3 / iters;
 }
+
+void setBucketFill(uint32_t fill) { gBucketFill.store(std::max<uint32_t>(fill, 1)); }
+uint32_t bucketFill() { return gBucketFill.load(); }
 
 uint64_t Pipeline::deviceBytes() const {
   const Impl &m = *impl_;

@@ -55,6 +55,9 @@ class Pipeline {
   void load(const std::vector<const StepgImage *> &imgs);
   // Enqueues the device pipeline; `timed` records per-stage events.
   void run(bool timed);
+  // Enqueues the pipeline as one HIP graph (captured on first use after a
+  // load, replayed afterwards).
+  void runGraph();
   // Waits for the device, copies the results to host.
   void download();
   // run() `iters` times and returns mean device microseconds per run.
@@ -85,6 +88,11 @@ class Pipeline {
   StageTimes times_;
   Impl *impl_;
 };
+
+// Autotuning knob: mean number of items per sort bucket the first radix
+// digit aims at (the digit width is chosen per circuit from it).
+void setBucketFill(uint32_t fill);
+uint32_t bucketFill();
 
 // Segmented XOR-merge kernels (differentiable path); pointers are device
 // pointers, stream is a hipStream_t.

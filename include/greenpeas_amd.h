@@ -156,6 +156,13 @@ int gp_batch_device_bytes(const gp_batch *batch, uint64_t *bytes);
 int gp_estimate_device_bytes(const char *circuit_text, int correlation_level,
                              uint64_t *bytes);
 
+/* Autotuning knobs (process-wide, read at the next load):
+ *   "bucket_fill": mean number of (hash, node) items per sort bucket the
+ *   first radix digit aims at; the digit width per circuit is the smallest b
+ *   with bucket_fill * 2^b >= its p>0 node count (default 64). */
+int gp_set_tuning(const char *name, uint32_t value);
+int gp_get_tuning(const char *name, uint32_t *value);
+
 /* ------------------------------------------------------------------------
  * Differentiable probabilities (no counterpart in the reference).
  *

@@ -1,0 +1,91 @@
+"""Serving loop host logic on the CPU: admission by estimated HBM footprint,
+continuous batching across slots, rejection of oversized requests and error
+delivery, with a fake compiler standing in for the GPU."""
+
+from __future__ import annotations
+
+import threading
+import time
+
+import pytest
+
+from conftest import read_circuit
+from greenpeas_amd import serving
+from greenpeas_amd.dem import DetectorErrorModel
+from greenpeas_amd.error_analysis import estimate_device_bytes
+
+
+class FakeCompiler:
+    """Records batches; 'compiles' by returning the text length."""
+
+    log: list = []
+    lock = threading.Lock()
+    delay = 0.02
+
+    def set_circuits(self, texts, threads=0):
+        self.texts = list(texts)
+
+    def compile(self):
+        with FakeCompiler.lock:
+            FakeCompiler.log.append(list(self.texts))
+        time.sleep(FakeCompiler.delay)
+        if any("FAIL" in t for t in self.texts):
+            raise RuntimeError("planted failure")
+        return [len(t) for t in self.texts]
+
+    def device_us(self):
+        return 1.0
+
+
+def test_estimate_is_an_upper_bound_of_the_real_footprint():
+    for name in ("surface3", "surface5", "bb6", "bb12"):
+        text = read_circuit(name)
+        assert serving.estimate_bytes(text) >= 0.8 * estimate_device_bytes(text)
+
+
+def test_continuous_batching_and_admission():
+    FakeCompiler.log = []
+    texts = [f"R 0\nTICK\nM {i}\n" for i in range(40)]
+    one = serving.estimate_bytes(texts[0])
+    srv = serving.Server(slots=2, max_batch=8, hbm_budget=one * 10, compiler_factory=FakeCompiler)
+    futs = [srv.submit(t) for t in texts]
+    assert [f.result(timeout=10) for f in futs] == [len(t) for t in texts]
+    srv.close()
+    sizes = [len(b) for b in FakeCompiler.log]
+    assert sum(sizes) == 40
+    assert max(sizes) <= 8
+    # the two slots never hold more than the budget together
+    assert all(s <= 10 for s in sizes)
+    st = srv.stats()
+    assert st["requests"] == 40 and st["p99_ms"] >= st["p50_ms"] > 0
+
+
+def test_oversized_request_rejected():
+    srv = serving.Server(slots=1, hbm_budget=1000, compiler_factory=FakeCompiler)
+    with pytest.raises(MemoryError):
+        srv.submit(read_circuit("bb12"))
+    srv.close()
+
+
+def test_errors_reach_every_future_of_the_batch():
+    FakeCompiler.log = []
+    srv = serving.Server(slots=1, max_batch=4, compiler_factory=FakeCompiler)
+    FakeCompiler.delay = 0.2
+    first = srv.submit("R 0\nTICK\nM 0\n")  # occupies the slot
+    time.sleep(0.05)
+    f1 = srv.submit("R 0\nTICK\nM 0\n# FAIL\n")
+    f2 = srv.submit("R 1\nTICK\nM 1\n")
+    assert first.result(timeout=5) > 0
+    with pytest.raises(RuntimeError, match="planted"):
+        f1.result(timeout=5)
+    with pytest.raises(RuntimeError, match="planted"):
+        f2.result(timeout=5)
+    FakeCompiler.delay = 0.02
+    srv.close()
+
+
+def test_request_pool_shapes():
+    pool = serving.request_pool(6, seed=1, surface=(5, 7), bb=(6,), bb_rounds=(6, 7))
+    assert len(pool) == 6 and all(t.startswith("R ") for t in pool)
+    arr = serving.poisson_arrivals(1000.0, 100, 3)
+    assert arr.shape == (100,) and (arr[1:] > arr[:-1]).all()
