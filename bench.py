@@ -82,7 +82,19 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="only run 5 device copies of BYTES bytes (scales rocprof byte counters)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+
+        src = torch.ones(args.calibrate_copy, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": args.calibrate_copy, "copies": 5}), flush=True)
+        return 0
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); the DEM "
               "compile path shards across GPUs by circuit with no communication", file=sys.stderr)

@@ -279,11 +279,73 @@ __device__ __forceinline__ void loadLayer(uint4 (&r)[RPT], const uint4 *__restri
 // Backward sensitivity sweep of one 32-bit column of one circuit.  The
 // sensitivities of layers l, l+1, l+2 live in an LDS ring indexed by active
 // position; each layer is one pass over the layer's active-node records (RPT
-// per thread, prefetched two layers ahead in registers) followed by one
-// LDS-only barrier whose ballot vote also detects when the column has become
-// identically zero, so the sweep stops early (no seeds below, two zero
-// layers above).  On exit the column's time band [lo, hi] is recorded, and
-// the column is added to the per-layer column masks of those layers.
+// per thread) followed by one LDS-only barrier.  Records are prefetched three
+// layers ahead into three register sets that rotate statically (the loop is
+// unrolled by three), so a record load has two full layers to land.  Below
+// the lowest seed of the column, the barrier also carries a vote (one LDS
+// word per layer) that stops the sweep once two consecutive layers are zero.
+// On exit the column's band [lo, hi] is recorded and the column is added to
+// the per-layer column masks of those layers.
+template <int RPT>
+struct SweepState {
+  const uint32_t *la;
+  const uint4 *ac;
+  const uint32_t *seeds;
+  uint32_t *ring;
+  uint32_t *Sc;
+  uint32_t *vote;  // 3 rotating LDS words
+  uint32_t M;
+  int32_t lowSeed;
+  uint32_t cur, nb1, nb2;  // ring slots of layers l, l+1, l+2
+  bool prevZero;
+};
+
+// One layer: consume `r` (records of layer l, bounds [b, e)), then refill it
+// with layer l-3.  Returns true when the column is known to be zero below l.
+template <int RPT>
+__device__ __forceinline__ bool sweepLayer(SweepState<RPT> &st, uint4 (&r)[RPT], uint32_t &b, uint32_t &e,
+                                           int32_t l) {
+  const uint32_t tid = threadIdx.x;
+  uint32_t any = 0;
+#pragma unroll
+  for (int k = 0; k < RPT; ++k) {
+    const uint32_t a = b + tid + k * kSweepBlock;
+    if (a < e) {
+      const uint4 x = r[k];
+      uint32_t s = x.z != NONE ? st.seeds[x.z] : 0u;
+      if (x.x != NONE) s ^= st.ring[((x.x >> 30) == 1 ? st.nb1 : st.nb2) + (x.x & 0x3FFFFFFFu)];
+      if (x.y != NONE) s ^= st.ring[((x.y >> 30) == 1 ? st.nb1 : st.nb2) + (x.y & 0x3FFFFFFFu)];
+      st.ring[st.cur + (a - b)] = s;
+      if (x.w != NONE) st.Sc[x.w] = s;
+      any |= s;
+    }
+  }
+  // refill with layer l-3 (its bounds come from the LDS copy of layerAct)
+  if (l >= 3) {
+    b = st.la[l - 3];
+    e = st.la[l - 2];
+    loadLayer<RPT>(r, st.ac, b, e);
+  } else {
+    b = e = 0;
+  }
+  const bool voting = l <= st.lowSeed + 1;
+  const uint32_t slot = static_cast<uint32_t>(l) % 3;
+  if (voting && __ballot(any != 0) != 0ull && (tid & 63) == 0) atomicOr(&st.vote[slot], 1u);
+  ldsBarrier();
+  bool stop = false;
+  if (voting) {
+    const bool zero = st.vote[slot] == 0u;
+    if (tid == 0) st.vote[(slot + 1) % 3] = 0u;  // reused two layers below
+    stop = zero && st.prevZero && l <= st.lowSeed;
+    st.prevZero = zero;
+  }
+  const uint32_t t = st.nb2;
+  st.nb2 = st.nb1;
+  st.nb1 = st.cur;
+  st.cur = t;
+  return stop;
+}
+
 template <int RPT>
 __global__ __launch_bounds__(kSweepBlock) void sweepKernel(
     const VDesc *__restrict__ desc, const uint2 *__restrict__ items,
@@ -292,8 +354,8 @@ __global__ __launch_bounds__(kSweepBlock) void sweepKernel(
     const int32_t *__restrict__ colLow, int32_t *__restrict__ bandLo, uint32_t *__restrict__ S,
     uint32_t *__restrict__ masks, int seedsInLds, int layersInLds) {
   extern __shared__ uint32_t lds[];
-  __shared__ uint32_t waveAny[2][kSweepBlock / 64];
-  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  __shared__ uint32_t vote[3];
+  const uint32_t tid = threadIdx.x;
   const uint2 it = items[blockIdx.x];
   const VDesc d = desc[it.x];
   const uint32_t c = it.y;
@@ -302,94 +364,68 @@ __global__ __launch_bounds__(kSweepBlock) void sweepKernel(
     if (tid == 0) bandLo[d.colOff + c] = 0x7FFFFFFF;
     return;
   }
-  const int32_t lowSeed = colLow[d.colOff + c];
   const uint32_t M = d.maxAct;
-  uint32_t *ring = lds;
   uint32_t *seedL = lds + 3 * M;
   uint32_t *laL = seedL + (seedsInLds ? d.numSeeds : 0);
   const uint32_t *sv = seedVals + d.seedOff + static_cast<uint64_t>(c) * d.numSeeds;
   const uint32_t *laG = layerAct + d.layerActOff;
-  for (uint32_t i = tid; i < 3 * M; i += kSweepBlock) ring[i] = 0u;
+  for (uint32_t i = tid; i < 3 * M; i += kSweepBlock) lds[i] = 0u;
   if (seedsInLds)
     for (uint32_t i = tid; i < d.numSeeds; i += kSweepBlock) seedL[i] = sv[i];
   if (layersInLds)
     for (uint32_t i = tid; i <= d.numLayers; i += kSweepBlock) laL[i] = laG[i];
+  if (tid < 3) vote[tid] = 0u;
   __syncthreads();
-  const uint32_t *la = layersInLds ? laL : laG;
-  const uint32_t *seeds = seedsInLds ? seedL : sv;
-  const uint4 *ac = act + d.actOff;
-  uint32_t *Sc = S + d.sOff + static_cast<uint64_t>(c) * d.numRanks;
 
-  // register pipeline: r0 = layer l, r1 = l-1, r2 = l-2
-  uint4 r0[RPT], r1[RPT], r2[RPT];
-  uint32_t b0 = la[hi], e0 = la[hi + 1];
-  uint32_t b1 = hi >= 1 ? la[hi - 1] : 0, e1 = b0;
-  uint32_t b2 = hi >= 2 ? la[hi - 2] : 0, e2 = b1;
-  loadLayer<RPT>(r0, ac, b0, e0);
-  loadLayer<RPT>(r1, ac, b1, hi >= 1 ? e1 : 0);
-  loadLayer<RPT>(r2, ac, b2, hi >= 2 ? e2 : 0);
+  SweepState<RPT> st;
+  st.la = layersInLds ? laL : laG;
+  st.ac = act + d.actOff;
+  st.seeds = seedsInLds ? seedL : sv;
+  st.ring = lds;
+  st.Sc = S + d.sOff + static_cast<uint64_t>(c) * d.numRanks;
+  st.vote = vote;
+  st.M = M;
+  st.lowSeed = colLow[d.colOff + c];
+  st.cur = static_cast<uint32_t>(hi % 3) * M;
+  st.nb1 = static_cast<uint32_t>((hi + 1) % 3) * M;
+  st.nb2 = static_cast<uint32_t>((hi + 2) % 3) * M;
+  st.prevZero = true;
 
-  uint32_t cur = static_cast<uint32_t>(hi % 3) * M;
-  uint32_t nb1 = static_cast<uint32_t>((hi + 1) % 3) * M;
-  uint32_t nb2 = static_cast<uint32_t>((hi + 2) % 3) * M;
-  bool prevZero = true;
+  // three register sets: layers hi, hi-1, hi-2
+  uint4 ra[RPT], rb[RPT], rc[RPT];
+  uint32_t ba = st.la[hi], ea = st.la[hi + 1];
+  uint32_t bb = hi >= 1 ? st.la[hi - 1] : 0, eb = hi >= 1 ? ba : 0;
+  uint32_t bc = hi >= 2 ? st.la[hi - 2] : 0, ec = hi >= 2 ? bb : 0;
+  loadLayer<RPT>(ra, st.ac, ba, ea);
+  loadLayer<RPT>(rb, st.ac, bb, eb);
+  loadLayer<RPT>(rc, st.ac, bc, ec);
+
+  int32_t l = hi;
   int32_t lo = 0;
-  uint32_t parity = 0;
-  for (int32_t l = hi; l >= 0; --l) {
-    uint32_t any = 0;
-#pragma unroll
-    for (int k = 0; k < RPT; ++k) {
-      const uint32_t a = b0 + tid + k * kSweepBlock;
-      if (a < e0) {
-        const uint4 r = r0[k];
-        uint32_t s = r.z != NONE ? seeds[r.z] : 0u;
-        if (r.x != NONE) s ^= ring[((r.x >> 30) == 1 ? nb1 : nb2) + (r.x & 0x3FFFFFFFu)];
-        if (r.y != NONE) s ^= ring[((r.y >> 30) == 1 ? nb1 : nb2) + (r.y & 0x3FFFFFFFu)];
-        ring[cur + (a - b0)] = s;
-        if (r.w != NONE) Sc[r.w] = s;
-        any |= s;
-      }
-    }
-    // shift the pipeline and prefetch layer l-3
-#pragma unroll
-    for (int k = 0; k < RPT; ++k) {
-      r0[k] = r1[k];
-      r1[k] = r2[k];
-    }
-    b0 = b1;
-    e0 = e1;
-    b1 = b2;
-    e1 = e2;
-    if (l >= 3) {
-      e2 = b2;
-      b2 = la[l - 3];
-      loadLayer<RPT>(r2, ac, b2, e2);
-    } else {
-      e2 = b2 = 0;
-    }
-    const uint64_t vote = __ballot(any != 0);
-    if (lane == 0) waveAny[parity][wave] = vote != 0ull;
-    ldsBarrier();
-    uint32_t anyL = 0;
-#pragma unroll
-    for (int w = 0; w < kSweepBlock / 64; ++w) anyL |= waveAny[parity][w];
-    parity ^= 1;
-    if (!anyL && prevZero && l <= lowSeed) {
+  for (;;) {
+    if (l < 0) break;
+    if (sweepLayer<RPT>(st, ra, ba, ea, l)) {
       lo = l;
       break;
     }
-    prevZero = !anyL;
-    const uint32_t t = nb2;
-    nb2 = nb1;
-    nb1 = cur;
-    cur = t;
+    if (--l < 0) break;
+    if (sweepLayer<RPT>(st, rb, bb, eb, l)) {
+      lo = l;
+      break;
+    }
+    if (--l < 0) break;
+    if (sweepLayer<RPT>(st, rc, bc, ec, l)) {
+      lo = l;
+      break;
+    }
+    --l;
   }
   if (tid == 0) bandLo[d.colOff + c] = lo;
   // add the column to the masks of the layers in its band
   uint32_t *mk = masks + d.maskOff + (c >> 5);
   const uint32_t bit = 1u << (c & 31);
-  for (int32_t l = lo + static_cast<int32_t>(tid); l <= hi; l += kSweepBlock)
-    atomicOr(&mk[static_cast<uint64_t>(l) * d.maskWords], bit);
+  for (int32_t k = lo + static_cast<int32_t>(tid); k <= hi; k += kSweepBlock)
+    atomicOr(&mk[static_cast<uint64_t>(k) * d.maskWords], bit);
 }
 
 // Class extraction + hashing of every p>0 node: the ascending list of set
