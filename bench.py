@@ -82,6 +82,9 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--workload", default="plain", choices=("plain", "batch", "grad", "serving"),
+                    help="plain: the headline bb12 compile; the others print their own line")
+    ap.add_argument("--variants", type=int, default=256, help="batch workload size")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run 5 device copies of BYTES bytes (scales rocprof byte counters)")
     args = ap.parse_args()
@@ -99,6 +102,13 @@ def main() -> int:
         print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); the DEM "
               "compile path shards across GPUs by circuit with no communication", file=sys.stderr)
         return 2
+
+    if args.workload == "batch":
+        return bench_batch(args)
+    if args.workload == "grad":
+        return bench_grad(args)
+    if args.workload == "serving":
+        return bench_serving(args)
 
     import numpy as np
 
@@ -192,6 +202,101 @@ def main() -> int:
         "speedup_vs_cpu_baseline": None if cpu is None else value / cpu["value"],
     }
     print(json.dumps(out), flush=True)
+    return 0
+
+
+def bench_batch(args) -> int:
+    """Batched compile of seeded uneven variants vs the same variants one by
+    one through the same kernels (device time; inputs resident)."""
+    from greenpeas_amd.batch import BatchCompiler, variant_circuits
+    from greenpeas_amd.codes import count_locations
+
+    t0 = time.perf_counter()
+    texts = variant_circuits(args.variants, seed=1234)
+    locs = sum(count_locations(t) for t in texts)
+    log(f"{len(texts)} variants, {locs} error locations, generated in {time.perf_counter() - t0:.1f} s")
+    bc = BatchCompiler()
+    t0 = time.perf_counter()
+    bc.set_circuits(texts)
+    log(f"host parse + STEPG build: {time.perf_counter() - t0:.2f} s")
+    batched = [bc.run() for _ in range(max(args.warmup, 1) + 3)][-3:]
+    log(f"batched device us: {batched}")
+    one = bc.run(one_by_one=True)
+    log(f"one-by-one device us: {one:.0f}")
+    b = min(batched)
+    out = {
+        "metric": METRIC, "workload": "batch", "variants": len(texts), "error_locations": locs,
+        "batched_device_us": b, "one_by_one_device_us": one, "speedup": one / b,
+        "batched_error_locations_per_s": locs / (b * 1e-6),
+        "device_bytes": bc.device_bytes(),
+        "config": "rotated surface memories, d in {5..15} odd, rounds in [d, 2d], per-instruction "
+                  "rates logU[1e-4, 1e-2], 5% of mid-circuit measurements followed by a "
+                  "reset-conditioned branch, seed 1234",
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def bench_grad(args) -> int:
+    """Forward + backward of L = sum w logit(p(theta)) on bb12."""
+    import numpy as np
+    import torch
+
+    from greenpeas_amd.autograd import DifferentiableDem, logit_loss
+
+    dd = DifferentiableDem(read_circuit("bb12"))
+    rng = np.random.default_rng(2024)
+    theta = torch.tensor(np.exp(rng.uniform(np.log(1e-4), np.log(5e-3), dd.num_locations)),
+                         device="cuda", requires_grad=True)
+    w = torch.tensor(rng.standard_normal(dd.num_errors), device="cuda")
+    for _ in range(max(args.warmup, 1)):
+        theta.grad = None
+        logit_loss(dd.probabilities(theta), w).backward()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(args.steps):
+        theta.grad = None
+        logit_loss(dd.probabilities(theta), w).backward()
+    b.record()
+    torch.cuda.synchronize()
+    us = a.elapsed_time(b) * 1e3 / args.steps
+    out = {"metric": METRIC, "workload": "grad", "circuit": "bb12 L2", "locations": dd.num_locations,
+           "errors": dd.num_errors, "forward_backward_us": us,
+           "gradients_per_s": dd.num_locations / (us * 1e-6)}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def bench_serving(args) -> int:
+    """Poisson request streams against the continuous-batching server."""
+    import threading
+
+    from greenpeas_amd.serving import Server, poisson_arrivals, request_pool
+
+    pool = request_pool(48, seed=77)
+    log(f"request pool of {len(pool)} circuits ready")
+    results = []
+    for rate in (100, 500, 1000, 2000, 5000):
+        n = min(400, max(60, rate // 5))
+        arrivals = poisson_arrivals(rate, n, seed=rate)
+        srv = Server(slots=2, max_batch=64)
+        futs = []
+        t0 = time.perf_counter()
+        for i, at in enumerate(arrivals):
+            now = time.perf_counter() - t0
+            if at > now:
+                time.sleep(at - now)
+            futs.append(srv.submit(pool[i % len(pool)]))
+        for f in futs:
+            f.result(timeout=600)
+        srv.close()
+        st = srv.stats()
+        st["offered_rps"] = rate
+        results.append(st)
+        log(f"rate {rate}: p50 {st['p50_ms']:.1f} ms p99 {st['p99_ms']:.1f} ms, "
+            f"{st['throughput_rps']:.0f} req/s, mean batch {st['mean_batch']:.1f}")
+    print(json.dumps({"metric": METRIC, "workload": "serving", "results": results}), flush=True)
     return 0
 
 

@@ -42,7 +42,8 @@ struct VDesc {
   uint32_t numLayers, npl, numCols, numDetectors;
   uint32_t numObservables, numRanks, numSeeds, bucketBits;
   uint32_t bucketBase, numBuckets, maxAct, maskWords;
-  uint64_t layerActOff, actOff, seedOff, colOff, rankOff, sOff, maskOff, pad;
+  uint64_t layerActOff, actOff, seedOff, colOff, rankOff, sOff, maskOff;
+  uint64_t colStride;  // words per S column: numRanks + a sink slot per sweep thread
 };
 
 // Counter block zeroed at the start of every run: [flags | bucket counts].
@@ -85,7 +86,8 @@ __device__ __forceinline__ uint64_t mix2(uint64_t h, uint32_t t) {
 // e = s * 64 + lane.  Distances below 64 exchange through lane shuffles,
 // larger ones inside a lane's registers.
 template <int R>
-__device__ __forceinline__ void waveBitonicSort(uint64_t (&h)[R], uint32_t (&r)[R], uint32_t lane) {
+__device__ __forceinline__ void waveBitonicSort(uint64_t (&h)[R], uint32_t (&r)[R], uint32_t (&x)[R],
+                                                uint32_t lane) {
   constexpr uint32_t N = 64 * R;
 #pragma unroll
   for (uint32_t k = 2; k <= N; k <<= 1) {
@@ -107,6 +109,9 @@ __device__ __forceinline__ void waveBitonicSort(uint64_t (&h)[R], uint32_t (&r)[
               uint32_t tr = r[s];
               r[s] = r[sp];
               r[sp] = tr;
+              uint32_t tx = x[s];
+              x[s] = x[sp];
+              x[sp] = tx;
             }
           }
         }
@@ -116,6 +121,7 @@ __device__ __forceinline__ void waveBitonicSort(uint64_t (&h)[R], uint32_t (&r)[
           const uint32_t e = static_cast<uint32_t>(s) * 64 + lane;
           const uint64_t ho = __shfl_xor(h[s], static_cast<int>(j));
           const uint32_t ro = __shfl_xor(r[s], static_cast<int>(j));
+          const uint32_t xo = __shfl_xor(x[s], static_cast<int>(j));
           const bool lower = (lane & j) == 0;
           const bool asc = (e & k) == 0;
           const bool otherLess = keyLess(ho, ro, h[s], r[s]);
@@ -124,6 +130,7 @@ __device__ __forceinline__ void waveBitonicSort(uint64_t (&h)[R], uint32_t (&r)[
           if (take) {
             h[s] = ho;
             r[s] = ro;
+            x[s] = xo;
           }
         }
       }
@@ -142,7 +149,7 @@ struct RunCarry {
 };
 
 struct RunOut {
-  const double *prob;      // rank -> node probability
+  const double *prob;      // rank -> node probability (fallback lookups)
   const uint64_t *h2;      // rank -> second hash
   const uint32_t *nnz;     // rank -> class size
   uint32_t *sortedRank;    // bucket-sorted ranks (in place over the partition)
@@ -158,10 +165,20 @@ struct RunOut {
 // positions chunkBase.. within bucket starting at bs) continuing the runs in
 // `cy`.  Runs are maximal sequences of equal hash; each run's probability is
 // the XOR-merge of its members, its representative the first member.
+// Per-element values the merge needs, looked up by key: the bucket's LDS
+// staging (key = position in the bucket) or the rank-indexed global arrays
+// (key = rank).
+struct RunLookup {
+  const double *prob;
+  const uint64_t *h2;
+  const uint32_t *nnz;
+};
+
 template <int R>
-__device__ __forceinline__ void scanRuns(const uint64_t (&h)[R], const uint32_t (&rk)[R], uint32_t cnt,
-                                         uint32_t chunkBase, uint32_t bs, RunCarry &cy, uint32_t lane,
-                                         const RunOut &o) {
+__device__ __forceinline__ void scanRuns(const uint64_t (&h)[R], const uint32_t (&rk)[R],
+                                         const uint32_t (&lk)[R], uint32_t cnt, uint32_t chunkBase,
+                                         uint32_t bs, RunCarry &cy, uint32_t lane, const RunOut &o,
+                                         const RunLookup &lu) {
   bool f[R];
 #pragma unroll
   for (int s = 0; s < R; ++s) {
@@ -180,13 +197,14 @@ __device__ __forceinline__ void scanRuns(const uint64_t (&h)[R], const uint32_t 
     if (static_cast<uint32_t>(s) * 64 >= cnt) break;
     const bool valid = e < cnt;
     const uint32_t r = rk[s];
-    const double own = valid ? o.prob[r] : 0.0;
-    const uint64_t ownH2 = valid ? o.h2[r] : 0ull;
+    const uint32_t key = lk[s];
+    const double own = valid ? lu.prob[key] : 0.0;
+    const uint64_t ownH2 = valid ? lu.h2[key] : 0ull;
     double v = own;
     uint32_t rep = r;
     uint64_t hf = ownH2;
     int fl = f[s] ? 1 : 0;
-    const uint32_t nz = f[s] ? o.nnz[r] : 0u;
+    const uint32_t nz = f[s] ? lu.nnz[key] : 0u;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
       const double pv = __shfl_up(v, d);
@@ -266,13 +284,18 @@ __device__ __forceinline__ void ldsBarrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Loads this thread's records of one layer.  The loads are unconditional
+// (out-of-range slots re-read the layer's last record, or record 0, and are
+// ignored by the consumer), so the compiler can leave them in flight instead
+// of waiting on a select.
 template <int RPT>
 __device__ __forceinline__ void loadLayer(uint4 (&r)[RPT], const uint4 *__restrict__ ac, uint32_t base,
                                           uint32_t end) {
+  const uint32_t last = end > base ? end - 1 : 0;
 #pragma unroll
   for (int k = 0; k < RPT; ++k) {
     const uint32_t a = base + threadIdx.x + k * kSweepBlock;
-    r[k] = a < end ? ac[a] : make_uint4(NONE, NONE, NONE, NONE);
+    r[k] = ac[a < end ? a : last];
   }
 }
 
@@ -288,13 +311,13 @@ __device__ __forceinline__ void loadLayer(uint4 (&r)[RPT], const uint4 *__restri
 // the per-layer column masks of those layers.
 template <int RPT>
 struct SweepState {
-  const uint32_t *la;
+  const uint32_t *la;     // LDS copy of the layer offsets
   const uint4 *ac;
-  const uint32_t *seeds;
-  uint32_t *ring;
-  uint32_t *Sc;
-  uint32_t *vote;  // 3 rotating LDS words
+  const uint32_t *seeds;  // LDS copy (unused when GSEEDS); seeds[seedZero] = 0
+  uint32_t *ring;         // LDS: 3 layer slots, then a zero word and a sink word
+  uint32_t *vote;         // 3 rotating LDS words
   uint32_t M;
+  uint32_t zeroWord, sinkWord, seedZero;
   int32_t lowSeed;
   uint32_t cur, nb1, nb2;  // ring slots of layers l, l+1, l+2
   bool prevZero;
@@ -302,33 +325,47 @@ struct SweepState {
 
 // One layer: consume `r` (records of layer l, bounds [b, e)), then refill it
 // with layer l-3.  Returns true when the column is known to be zero below l.
-template <int RPT>
+template <int RPT, bool GSEEDS>
 __device__ __forceinline__ bool sweepLayer(SweepState<RPT> &st, uint4 (&r)[RPT], uint32_t &b, uint32_t &e,
-                                           int32_t l) {
+                                           int32_t l, const uint32_t *__restrict__ gseeds,
+                                           char *__restrict__ Sc, uint32_t sinkOff) {
   const uint32_t tid = threadIdx.x;
   uint32_t any = 0;
+  // Branch-free record processing: every lane issues the same LDS and
+  // global operations (missing children and seeds read a zero word, invalid
+  // slots write to sink slots), so the compiler can count outstanding memory
+  // operations exactly and leave the prefetched records in flight.
 #pragma unroll
   for (int k = 0; k < RPT; ++k) {
     const uint32_t a = b + tid + k * kSweepBlock;
-    if (a < e) {
-      const uint4 x = r[k];
-      uint32_t s = x.z != NONE ? st.seeds[x.z] : 0u;
-      if (x.x != NONE) s ^= st.ring[((x.x >> 30) == 1 ? st.nb1 : st.nb2) + (x.x & 0x3FFFFFFFu)];
-      if (x.y != NONE) s ^= st.ring[((x.y >> 30) == 1 ? st.nb1 : st.nb2) + (x.y & 0x3FFFFFFFu)];
-      st.ring[st.cur + (a - b)] = s;
-      if (x.w != NONE) st.Sc[x.w] = s;
-      any |= s;
+    const bool valid = a < e;
+    const uint4 x = r[k];
+    uint32_t s;
+    if constexpr (GSEEDS) {
+      s = (valid && x.z != NONE) ? gseeds[x.z] : 0u;
+    } else {
+      s = st.seeds[(valid && x.z != NONE) ? x.z : st.seedZero];
     }
+    const uint32_t i0 = (valid && x.x != NONE) ? ((x.x >> 30) == 1 ? st.nb1 : st.nb2) + (x.x & 0x3FFFFFFFu)
+                                                : st.zeroWord;
+    const uint32_t i1 = (valid && x.y != NONE) ? ((x.y >> 30) == 1 ? st.nb1 : st.nb2) + (x.y & 0x3FFFFFFFu)
+                                                : st.zeroWord;
+    s ^= st.ring[i0] ^ st.ring[i1];
+    st.ring[valid ? st.cur + (a - b) : st.sinkWord] = s;
+    // 32-bit byte offset from the column's (uniform) base: the store takes
+    // the SGPR-base form and needs no 64-bit copy of the in-flight record
+    const uint32_t off = (valid && x.w != NONE) ? (x.w << 2) : sinkOff;
+    *reinterpret_cast<uint32_t *>(Sc + off) = s;
+    any |= s;
   }
-  // refill with layer l-3 (its bounds come from the LDS copy of layerAct)
-  if (l >= 3) {
-    b = st.la[l - 3];
-    e = st.la[l - 2];
-    loadLayer<RPT>(r, st.ac, b, e);
-  } else {
-    b = e = 0;
-  }
-  const bool voting = l <= st.lowSeed + 1;
+  // refill with layer l-3 (bounds from the LDS copy of layerAct); no branch,
+  // so the loads stay in flight: below layer 0 the range is empty
+  const bool more = l >= 3;
+  const uint32_t ln = more ? static_cast<uint32_t>(l - 3) : 0u;
+  b = more ? st.la[ln] : 0u;
+  e = more ? st.la[ln + 1] : 0u;
+  loadLayer<RPT>(r, st.ac, b, e);
+  const bool voting = l >= 0 && l <= st.lowSeed + 1;
   const uint32_t slot = static_cast<uint32_t>(l) % 3;
   if (voting && __ballot(any != 0) != 0ull && (tid & 63) == 0) atomicOr(&st.vote[slot], 1u);
   ldsBarrier();
@@ -346,13 +383,13 @@ __device__ __forceinline__ bool sweepLayer(SweepState<RPT> &st, uint4 (&r)[RPT],
   return stop;
 }
 
-template <int RPT>
+template <int RPT, bool GSEEDS>
 __global__ __launch_bounds__(kSweepBlock) void sweepKernel(
     const VDesc *__restrict__ desc, const uint2 *__restrict__ items,
     const uint32_t *__restrict__ layerAct, const uint4 *__restrict__ act,
     const uint32_t *__restrict__ seedVals, const int32_t *__restrict__ colHi,
     const int32_t *__restrict__ colLow, int32_t *__restrict__ bandLo, uint32_t *__restrict__ S,
-    uint32_t *__restrict__ masks, int seedsInLds, int layersInLds) {
+    uint32_t *__restrict__ masks) {
   extern __shared__ uint32_t lds[];
   __shared__ uint32_t vote[3];
   const uint32_t tid = threadIdx.x;
@@ -365,24 +402,31 @@ __global__ __launch_bounds__(kSweepBlock) void sweepKernel(
     return;
   }
   const uint32_t M = d.maxAct;
-  uint32_t *seedL = lds + 3 * M;
-  uint32_t *laL = seedL + (seedsInLds ? d.numSeeds : 0);
+  // LDS: ring[3M] | zero word | sink word | layer offsets[L+1] | seeds + 0
+  uint32_t *laL = lds + 3 * M + 2;
+  uint32_t *seedL = laL + d.numLayers + 1;
   const uint32_t *sv = seedVals + d.seedOff + static_cast<uint64_t>(c) * d.numSeeds;
   const uint32_t *laG = layerAct + d.layerActOff;
-  for (uint32_t i = tid; i < 3 * M; i += kSweepBlock) lds[i] = 0u;
-  if (seedsInLds)
+  for (uint32_t i = tid; i < 3 * M + 2; i += kSweepBlock) lds[i] = 0u;
+  for (uint32_t i = tid; i <= d.numLayers; i += kSweepBlock) laL[i] = laG[i];
+  if constexpr (!GSEEDS) {
     for (uint32_t i = tid; i < d.numSeeds; i += kSweepBlock) seedL[i] = sv[i];
-  if (layersInLds)
-    for (uint32_t i = tid; i <= d.numLayers; i += kSweepBlock) laL[i] = laG[i];
+    if (tid == 0) seedL[d.numSeeds] = 0u;
+  }
   if (tid < 3) vote[tid] = 0u;
   __syncthreads();
 
+  char *Sc = reinterpret_cast<char *>(S + d.sOff + static_cast<uint64_t>(c) * d.colStride);
+  // each column ends with one sink slot per thread, for writes of invalid slots
+  const uint32_t sinkOff = (d.numRanks + tid) << 2;
   SweepState<RPT> st;
-  st.la = layersInLds ? laL : laG;
+  st.la = laL;
   st.ac = act + d.actOff;
-  st.seeds = seedsInLds ? seedL : sv;
+  st.seeds = seedL;
   st.ring = lds;
-  st.Sc = S + d.sOff + static_cast<uint64_t>(c) * d.numRanks;
+  st.zeroWord = 3 * M;
+  st.sinkWord = 3 * M + 1;
+  st.seedZero = d.numSeeds;
   st.vote = vote;
   st.M = M;
   st.lowSeed = colLow[d.colOff + c];
@@ -400,26 +444,22 @@ __global__ __launch_bounds__(kSweepBlock) void sweepKernel(
   loadLayer<RPT>(rb, st.ac, bb, eb);
   loadLayer<RPT>(rc, st.ac, bc, ec);
 
+  // Three layers per trip and a single exit at the end of the trip (no
+  // exits inside, which would force the compiler to drain the prefetches at
+  // the loop head).  Layers below 0 are empty; the at most two layers after
+  // the stop recompute zeros below the band, which nothing reads.
   int32_t l = hi;
-  int32_t lo = 0;
+  int32_t lo = -1;
   for (;;) {
-    if (l < 0) break;
-    if (sweepLayer<RPT>(st, ra, ba, ea, l)) {
-      lo = l;
-      break;
-    }
-    if (--l < 0) break;
-    if (sweepLayer<RPT>(st, rb, bb, eb, l)) {
-      lo = l;
-      break;
-    }
-    if (--l < 0) break;
-    if (sweepLayer<RPT>(st, rc, bc, ec, l)) {
-      lo = l;
-      break;
-    }
+    if (sweepLayer<RPT, GSEEDS>(st, ra, ba, ea, l, sv, Sc, sinkOff) && lo < 0) lo = l;
     --l;
+    if (sweepLayer<RPT, GSEEDS>(st, rb, bb, eb, l, sv, Sc, sinkOff) && lo < 0) lo = l;
+    --l;
+    if (sweepLayer<RPT, GSEEDS>(st, rc, bc, ec, l, sv, Sc, sinkOff) && lo < 0) lo = l;
+    --l;
+    if (lo >= 0 || l < 0) break;
   }
+  if (lo < 0) lo = 0;
   if (tid == 0) bandLo[d.colOff + c] = lo;
   // add the column to the masks of the layers in its band
   uint32_t *mk = masks + d.maskOff + (c >> 5);
@@ -454,7 +494,7 @@ __global__ __launch_bounds__(kClassBlock) void classifyKernel(
     while (m) {
       const uint32_t c = w * 32 + static_cast<uint32_t>(__builtin_ctz(m));
       m &= m - 1;
-      uint32_t x = Sv[static_cast<uint64_t>(c) * d.numRanks];
+      uint32_t x = Sv[static_cast<uint64_t>(c) * d.colStride];
       while (x) {
         const uint32_t tg = c * 32 + static_cast<uint32_t>(__builtin_ctz(x));
         h = (h ^ tg) * kFnvPrime;
@@ -482,35 +522,50 @@ __global__ __launch_bounds__(kClassBlock) void partitionKernel(
     const VDesc *__restrict__ desc, const uint2 *__restrict__ tiles,
     const uint64_t *__restrict__ hash, const uint32_t *__restrict__ nnz,
     const uint32_t *__restrict__ bucketStart, const uint32_t *__restrict__ bucketPos,
-    uint64_t *__restrict__ pkey, uint32_t *__restrict__ prank) {
+    const double *__restrict__ rankProb, const uint64_t *__restrict__ h2,
+    uint64_t *__restrict__ pkey, uint32_t *__restrict__ prank, double *__restrict__ pprob,
+    uint64_t *__restrict__ ph2, uint32_t *__restrict__ pnnz) {
   const uint2 t = tiles[blockIdx.x];
   const VDesc d = desc[t.x];
   const uint32_t i = t.y + threadIdx.x;
   if (i >= d.numRanks) return;
   const uint64_t gi = d.rankOff + i;
-  if (!nnz[gi]) return;
+  const uint32_t n = nnz[gi];
+  if (!n) return;
   const uint64_t h = hash[gi];
   const uint32_t b = d.bucketBase + (d.bucketBits ? static_cast<uint32_t>(h >> (64 - d.bucketBits)) : 0u);
   const uint32_t pos = bucketStart[b] + bucketPos[gi];
   pkey[pos] = h;
   prank[pos] = static_cast<uint32_t>(gi);
+  pprob[pos] = rankProb[gi];
+  ph2[pos] = h2[gi];
+  pnnz[pos] = n;
 }
 
 template <int R>
 __device__ void bucketWave(uint32_t b, uint32_t bs, uint32_t cnt, uint32_t lane,
-                           const uint64_t *__restrict__ pkey, const RunOut &o,
+                           const uint64_t *__restrict__ pkey, const double *__restrict__ pprob,
+                           const uint64_t *__restrict__ ph2, const uint32_t *__restrict__ pnnz,
+                           const RunOut &o, double *sProb, uint64_t *sH2, uint32_t *sNnz,
                            uint32_t *__restrict__ runCount, uint32_t *__restrict__ runNnz) {
   uint64_t h[R];
-  uint32_t r[R];
+  uint32_t r[R], x[R];
 #pragma unroll
   for (int s = 0; s < R; ++s) {
     const uint32_t e = static_cast<uint32_t>(s) * 64 + lane;
-    h[s] = e < cnt ? pkey[bs + e] : ~0ull;
-    r[s] = e < cnt ? o.sortedRank[bs + e] : NONE;
+    const bool v = e < cnt;
+    h[s] = v ? pkey[bs + e] : ~0ull;
+    r[s] = v ? o.sortedRank[bs + e] : NONE;
+    x[s] = e;
+    if (v) {  // stage what the merge reads, indexed by position in the bucket
+      sProb[e] = pprob[bs + e];
+      sH2[e] = ph2[bs + e];
+      sNnz[e] = pnnz[bs + e];
+    }
   }
-  waveBitonicSort<R>(h, r, lane);
+  waveBitonicSort<R>(h, r, x, lane);
   RunCarry cy{0.0, 0ull, 0ull, 0u, 0u, 0u, 0u};
-  scanRuns<R>(h, r, cnt, 0u, bs, cy, lane, o);
+  scanRuns<R>(h, r, x, cnt, 0u, bs, cy, lane, o, RunLookup{sProb, sH2, sNnz});
   if (lane == 0) {
     runCount[b] = cy.runs;
     runNnz[b] = cy.nnzSum;
@@ -521,9 +576,16 @@ __device__ void bucketWave(uint32_t b, uint32_t bs, uint32_t cnt, uint32_t lane,
 // wave capacity are queued for bucketFallbackKernel.
 __global__ __launch_bounds__(256) void bucketKernel(
     uint32_t numBuckets, const uint32_t *__restrict__ bucketStart, const uint64_t *__restrict__ pkey,
+    const double *__restrict__ pprob, const uint64_t *__restrict__ ph2, const uint32_t *__restrict__ pnnz,
     RunOut o, uint32_t *__restrict__ runCount, uint32_t *__restrict__ runNnz,
     uint32_t *__restrict__ overflowList) {
+  // per-wave LDS staging of the values the merge reads (wave-private: the
+  // wave's own lanes write and read it, in order, so no barrier is needed)
+  __shared__ double sProb[4][kBucketItemsPerWave];
+  __shared__ uint64_t sH2[4][kBucketItemsPerWave];
+  __shared__ uint32_t sNnz[4][kBucketItemsPerWave];
   const uint32_t lane = threadIdx.x & 63;
+  const uint32_t wv = threadIdx.x >> 6;
   const uint32_t b = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
   if (b >= numBuckets) return;
   const uint32_t bs = bucketStart[b];
@@ -534,11 +596,11 @@ __global__ __launch_bounds__(256) void bucketKernel(
       runNnz[b] = 0;
     }
   } else if (cnt <= 64) {
-    bucketWave<1>(b, bs, cnt, lane, pkey, o, runCount, runNnz);
+    bucketWave<1>(b, bs, cnt, lane, pkey, pprob, ph2, pnnz, o, sProb[wv], sH2[wv], sNnz[wv], runCount, runNnz);
   } else if (cnt <= 128) {
-    bucketWave<2>(b, bs, cnt, lane, pkey, o, runCount, runNnz);
+    bucketWave<2>(b, bs, cnt, lane, pkey, pprob, ph2, pnnz, o, sProb[wv], sH2[wv], sNnz[wv], runCount, runNnz);
   } else if (cnt <= kBucketItemsPerWave) {
-    bucketWave<4>(b, bs, cnt, lane, pkey, o, runCount, runNnz);
+    bucketWave<4>(b, bs, cnt, lane, pkey, pprob, ph2, pnnz, o, sProb[wv], sH2[wv], sNnz[wv], runCount, runNnz);
   } else if (lane == 0) {
     const uint32_t idx = atomicAdd(&o.flags[2], 1u);
     overflowList[idx] = b;
@@ -590,12 +652,13 @@ __global__ __launch_bounds__(256) void bucketFallbackKernel(
       RunCarry cy{0.0, 0ull, 0ull, 0u, 0u, 0u, 0u};
       for (uint32_t c0 = 0; c0 < cnt; c0 += 64) {
         uint64_t h[1];
-        uint32_t r[1];
+        uint32_t r[1], k[1];
         const uint32_t e = c0 + lane;
         h[0] = e < cnt ? K[e] : ~0ull;
         r[0] = e < cnt ? Rk[e] : NONE;
+        k[0] = r[0];
         const uint32_t n = cnt - c0 < 64 ? cnt - c0 : 64;
-        scanRuns<1>(h, r, n, c0, bs, cy, lane, o);
+        scanRuns<1>(h, r, k, n, c0, bs, cy, lane, o, RunLookup{o.prob, o.h2, o.nnz});
       }
       if (lane == 0) {
         runCount[b] = cy.runs;
@@ -655,7 +718,7 @@ __global__ __launch_bounds__(256) void emitKernel(
       while (m) {
         const uint32_t col = w * 32 + static_cast<uint32_t>(__builtin_ctz(m));
         m &= m - 1;
-        uint32_t x = Sv[static_cast<uint64_t>(col) * d.numRanks];
+        uint32_t x = Sv[static_cast<uint64_t>(col) * d.colStride];
         while (x) {
           if (pos < capTargets) outTargets[pos] = col * 32 + static_cast<uint32_t>(__builtin_ctz(x));
           pos++;
@@ -906,9 +969,10 @@ struct Pipeline::Impl {
   uint32_t V = 0, nItems = 0, nTiles = 0, B = 0, R = 0, Ccols = 0, rpt = 1;
   uint64_t Swords = 0, maskWordsTotal = 0;
   size_t sweepLdsBytes = 0;
-  int seedsInLds = 1, layersInLds = 1;
+  int seedsInLds = 1;
 
-  DevBuf bandLo, S, hash, h2, nnz, counters, bucketStart, bucketPos, pkey, prank, runRank, runP,
+  DevBuf bandLo, S, hash, h2, nnz, counters, bucketStart, bucketPos, pkey, prank, pprob, ph2, pnnz,
+      runRank, runP,
       runHash, runTgtPre, runMember, runCount, runNnz, classStart, tgtStart, outP, outOff, outHash,
       outMember, outTargets, scratchKey, scratchRank, overflowList, scanTmp, vBounds, masks;
   uint64_t capTargets = 0;
@@ -1010,6 +1074,7 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
     d.rankOff = rankOff;
     d.sOff = sOff;
     d.maskOff = maskOff;
+    d.colStride = static_cast<uint64_t>(g.numRanks) + kSweepBlock;
     variants_[v] = VariantInfo{g.numRanks, g.numCols, g.numDetectors, g.numObservables, bucketBase,
                                d.numBuckets};
     for (uint32_t c = 0; c < g.numCols; ++c) {
@@ -1024,7 +1089,7 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
     seedOff += static_cast<uint64_t>(g.numCols) * g.numSeeds;
     colOff += g.numCols;
     rankOff += g.numRanks;
-    sOff += static_cast<uint64_t>(g.numRanks) * g.numCols;
+    sOff += d.colStride * g.numCols;
     maskOff += static_cast<uint64_t>(g.numLayers) * d.maskWords;
     bucketBase += d.numBuckets;
     maxAct = std::max(maxAct, g.maxActive);
@@ -1038,12 +1103,13 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   if (!m.rpt)
     throw RuntimeError("circuit has too many active STEPG nodes per layer for the LDS sweep (" +
                        std::to_string(maxAct) + " > 8192)");
-  size_t lds = 12ull * maxAct;
-  if (lds > kLdsBudget) throw RuntimeError("circuit layer too wide for the LDS sweep");
+  // LDS: ring (3 layers) + layer offsets, plus the column's seeds if they fit
+  size_t lds = 12ull * maxAct + 4ull * (maxLayers + 1);
+  if (lds > kLdsBudget)
+    throw RuntimeError("circuit too wide or too deep for the LDS sweep (" + std::to_string(maxAct) +
+                       " active nodes per layer, " + std::to_string(maxLayers) + " layers)");
   m.seedsInLds = lds + 4ull * maxSeeds <= kLdsBudget;
   if (m.seedsInLds) lds += 4ull * maxSeeds;
-  m.layersInLds = lds + 4ull * (maxLayers + 1) <= kLdsBudget;
-  if (m.layersInLds) lds += 4ull * (maxLayers + 1);
   m.sweepLdsBytes = lds;
 
   // longest bands first (they bound the sweep's critical path)
@@ -1078,7 +1144,7 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   m.oColLow = place(4 * colOff);
   m.oRankLayer = place(4 * rankOff);
   m.oRankProb = place(8 * rankOff);
-  m.inBytes = off;
+  m.inBytes = off + 256;  // slack: the sweep's clamped record loads may touch record 0
   m.in.ensure(m.inBytes, frozen, "inputs");
   m.inHost.ensure(m.inBytes);
   char *hb = m.inHost.as<char>();
@@ -1110,6 +1176,9 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   m.bucketStart.ensure(4 * (B + 1), frozen, "buckets");
   m.pkey.ensure(8 * R, frozen, "nodes");
   m.prank.ensure(4 * R, frozen, "nodes");
+  m.pprob.ensure(8 * R, frozen, "nodes");
+  m.ph2.ensure(8 * R, frozen, "nodes");
+  m.pnnz.ensure(4 * R, frozen, "nodes");
   m.runRank.ensure(4 * R, frozen, "nodes");
   m.runP.ensure(8 * R, frozen, "nodes");
   m.runHash.ensure(8 * R, frozen, "nodes");
@@ -1136,10 +1205,14 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   m.hFlags.ensure(4 * kFlagWords);
 
   if (m.sweepLdsBytes > 64 * 1024) {
-    const void *fns[] = {reinterpret_cast<const void *>(sweepKernel<1>),
-                         reinterpret_cast<const void *>(sweepKernel<2>),
-                         reinterpret_cast<const void *>(sweepKernel<4>),
-                         reinterpret_cast<const void *>(sweepKernel<8>)};
+    const void *fns[] = {reinterpret_cast<const void *>(sweepKernel<1, false>),
+                         reinterpret_cast<const void *>(sweepKernel<2, false>),
+                         reinterpret_cast<const void *>(sweepKernel<4, false>),
+                         reinterpret_cast<const void *>(sweepKernel<8, false>),
+                         reinterpret_cast<const void *>(sweepKernel<1, true>),
+                         reinterpret_cast<const void *>(sweepKernel<2, true>),
+                         reinterpret_cast<const void *>(sweepKernel<4, true>),
+                         reinterpret_cast<const void *>(sweepKernel<8, true>)};
     for (const void *f : fns)
       GPA_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   static_cast<int>(m.sweepLdsBytes)));
@@ -1188,15 +1261,24 @@ void Pipeline::run(bool timed) {
   }
   mark(0);
   if (m.nItems) {
-#define GPA_SWEEP(RPT)                                                                           \
-  sweepKernel<RPT><<<m.nItems, kSweepBlock, m.sweepLdsBytes, st>>>(                              \
+#define GPA_SWEEP(RPT, G)                                                                        \
+  sweepKernel<RPT, G><<<m.nItems, kSweepBlock, m.sweepLdsBytes, st>>>(                           \
       desc, items, layerAct, act, seed, colHi, colLow, m.bandLo.as<int32_t>(), m.S.as<uint32_t>(), \
-      m.masks.as<uint32_t>(), m.seedsInLds, m.layersInLds)
-    switch (m.rpt) {
-    case 1: GPA_SWEEP(1); break;
-    case 2: GPA_SWEEP(2); break;
-    case 4: GPA_SWEEP(4); break;
-    default: GPA_SWEEP(8); break;
+      m.masks.as<uint32_t>())
+    if (m.seedsInLds) {
+      switch (m.rpt) {
+      case 1: GPA_SWEEP(1, false); break;
+      case 2: GPA_SWEEP(2, false); break;
+      case 4: GPA_SWEEP(4, false); break;
+      default: GPA_SWEEP(8, false); break;
+      }
+    } else {
+      switch (m.rpt) {
+      case 1: GPA_SWEEP(1, true); break;
+      case 2: GPA_SWEEP(2, true); break;
+      case 4: GPA_SWEEP(4, true); break;
+      default: GPA_SWEEP(8, true); break;
+      }
     }
 #undef GPA_SWEEP
   }
@@ -1209,10 +1291,10 @@ void Pipeline::run(bool timed) {
   mark(2);
   m.scan(m.bucketCount(), nullptr, B, m.bucketStart.as<uint32_t>(), nullptr);
   if (m.nTiles)
-    partitionKernel<<<m.nTiles, kClassBlock, 0, st>>>(desc, tiles, m.hash.as<uint64_t>(),
-                                                      m.nnz.as<uint32_t>(), m.bucketStart.as<uint32_t>(),
-                                                      m.bucketPos.as<uint32_t>(), m.pkey.as<uint64_t>(),
-                                                      m.prank.as<uint32_t>());
+    partitionKernel<<<m.nTiles, kClassBlock, 0, st>>>(
+        desc, tiles, m.hash.as<uint64_t>(), m.nnz.as<uint32_t>(), m.bucketStart.as<uint32_t>(),
+        m.bucketPos.as<uint32_t>(), rankProb, m.h2.as<uint64_t>(), m.pkey.as<uint64_t>(),
+        m.prank.as<uint32_t>(), m.pprob.as<double>(), m.ph2.as<uint64_t>(), m.pnnz.as<uint32_t>());
   mark(3);
   RunOut o{rankProb,
            m.h2.as<uint64_t>(),
@@ -1224,7 +1306,8 @@ void Pipeline::run(bool timed) {
            m.runTgtPre.as<uint32_t>(),
            m.runMember.as<uint32_t>(),
            m.flags()};
-  bucketKernel<<<(B + 3) / 4, 256, 0, st>>>(B, m.bucketStart.as<uint32_t>(), m.pkey.as<uint64_t>(), o,
+  bucketKernel<<<(B + 3) / 4, 256, 0, st>>>(B, m.bucketStart.as<uint32_t>(), m.pkey.as<uint64_t>(),
+                                            m.pprob.as<double>(), m.ph2.as<uint64_t>(), m.pnnz.as<uint32_t>(), o,
                                             m.runCount.as<uint32_t>(), m.runNnz.as<uint32_t>(),
                                             m.overflowList.as<uint32_t>());
   bucketFallbackKernel<<<64, 256, 0, st>>>(m.bucketStart.as<uint32_t>(), m.pkey.as<uint64_t>(), o,
@@ -1386,7 +1469,8 @@ uint32_t bucketFill() { return gBucketFill.load(); }
 uint64_t Pipeline::deviceBytes() const {
   const Impl &m = *impl_;
   const DevBuf *bufs[] = {&m.in, &m.bandLo, &m.masks, &m.S, &m.hash, &m.h2, &m.nnz, &m.bucketPos,
-                          &m.counters, &m.bucketStart, &m.pkey, &m.prank, &m.runRank, &m.runP,
+                          &m.counters, &m.bucketStart, &m.pkey, &m.prank, &m.pprob, &m.ph2,
+                          &m.pnnz, &m.runRank, &m.runP,
                           &m.runHash, &m.runTgtPre, &m.runMember, &m.runCount, &m.runNnz,
                           &m.classStart, &m.tgtStart, &m.outP, &m.outOff, &m.outHash, &m.outMember,
                           &m.outTargets, &m.scratchKey, &m.scratchRank, &m.overflowList, &m.scanTmp,
@@ -1401,7 +1485,7 @@ void Pipeline::debugSensitivities(uint32_t v, std::vector<uint32_t> &cols32) con
   GPA_HIP(hipSetDevice(device_));
   GPA_HIP(hipStreamSynchronize(m.stream));
   const VDesc &d = m.descs.at(v);
-  std::vector<uint32_t> S(static_cast<size_t>(d.numRanks) * d.numCols);
+  std::vector<uint32_t> S(static_cast<size_t>(d.colStride) * d.numCols);
   std::vector<int32_t> lo(d.numCols), hi(d.numCols);
   if (!S.empty())
     GPA_HIP(hipMemcpy(S.data(), m.S.as<uint32_t>() + d.sOff, 4 * S.size(), hipMemcpyDeviceToHost));
@@ -1414,7 +1498,7 @@ void Pipeline::debugSensitivities(uint32_t v, std::vector<uint32_t> &cols32) con
     for (uint32_t i = 0; i < d.numRanks; ++i) {
       const int32_t l = static_cast<int32_t>(layer[i]);
       if (l >= lo[c] && l <= hi[c])
-        cols32[static_cast<size_t>(i) * d.numCols + c] = S[static_cast<size_t>(c) * d.numRanks + i];
+        cols32[static_cast<size_t>(i) * d.numCols + c] = S[static_cast<size_t>(c) * d.colStride + i];
     }
 }
 

@@ -367,7 +367,7 @@ uint64_t imageDeviceBytes(const StepgImage &img) {
   b += (img.numLayers + 1) * 4ull + img.numActive * 16ull + img.seedVals.size() * 4ull;
   b += img.numLayers * ((img.numCols + 31) / 32) * 4ull;  // per-layer column masks
   b += img.numCols * 12ull + R * 12ull;  // bands, rank layer + prob
-  b += R * img.numCols * 4ull;            // sensitivity matrix
+  b += (R + 1024) * img.numCols * 4ull;   // sensitivity matrix (+ per-column sink slots)
   b += R * (8 + 8 + 4 + 4);               // hash, h2, nnz, class-of-rank
   b += R * (8 + 4) * 2;                   // partitioned keys/ranks + scratch
   b += R * (4 + 8 + 8 + 4 + 4 + 4);       // runs
