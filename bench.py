@@ -85,6 +85,8 @@ def main() -> int:
     ap.add_argument("--workload", default="plain", choices=("plain", "batch", "grad", "serving"),
                     help="plain: the headline bb12 compile; the others print their own line")
     ap.add_argument("--variants", type=int, default=256, help="batch workload size")
+    ap.add_argument("--sweep-rpt", type=int, default=0,
+                    help="tuning: fewest sweep records per thread (1, 2, 4, 8; 0 = library default)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run 5 device copies of BYTES bytes (scales rocprof byte counters)")
     args = ap.parse_args()
@@ -103,6 +105,10 @@ def main() -> int:
               "compile path shards across GPUs by circuit with no communication", file=sys.stderr)
         return 2
 
+    if args.sweep_rpt:
+        from greenpeas_amd import autotune
+
+        autotune.set_tuning("sweep_rpt", args.sweep_rpt)
     if args.workload == "batch":
         return bench_batch(args)
     if args.workload == "grad":

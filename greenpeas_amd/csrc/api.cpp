@@ -634,6 +634,7 @@ int gp_set_tuning(const char *name, uint32_t value) {
   return guarded([&] {
     if (!name) throw InvalidArgument("name is NULL");
     if (std::string(name) == "bucket_fill") gpa::setBucketFill(value);
+    else if (std::string(name) == "sweep_rpt") gpa::setSweepMinRpt(value);
     else throw InvalidArgument(std::string("unknown tuning parameter: ") + name);
   });
 }
@@ -642,6 +643,7 @@ int gp_get_tuning(const char *name, uint32_t *value) {
   return guarded([&] {
     if (!name || !value) throw InvalidArgument("NULL argument");
     if (std::string(name) == "bucket_fill") *value = gpa::bucketFill();
+    else if (std::string(name) == "sweep_rpt") *value = gpa::sweepMinRpt();
     else throw InvalidArgument(std::string("unknown tuning parameter: ") + name);
   });
 }

@@ -30,7 +30,6 @@ namespace {
 constexpr uint32_t NONE = 0xFFFFFFFFu;
 constexpr uint64_t kFnvBasis = 0xcbf29ce484222325ull;  // Array.hpp:54
 constexpr uint64_t kFnvPrime = 0x100000001b3ull;       // Array.hpp:56
-constexpr int kSweepBlock = 1024;
 constexpr int kClassBlock = 256;
 constexpr int kScanBlock = 1024;
 constexpr int kScanTile = 4 * kScanBlock;
@@ -41,9 +40,10 @@ std::atomic<uint32_t> gBucketFill{64};    // mean items per bucket the digit wid
 struct VDesc {
   uint32_t numLayers, npl, numCols, numDetectors;
   uint32_t numObservables, numRanks, numSeeds, bucketBits;
-  uint32_t bucketBase, numBuckets, maxAct, maskWords;
-  uint64_t layerActOff, actOff, seedOff, colOff, rankOff, sOff, maskOff;
-  uint64_t colStride;  // words per S column: numRanks + a sink slot per sweep thread
+  uint32_t bucketBase, numBuckets, maskWords, layerStride;
+  uint32_t sweepThreads, sweepRpt, pad0, pad1;
+  uint64_t actOff, seedOff, colOff, rankOff, sOff, maskOff;
+  uint64_t colStride;  // words per S column: numRanks + kSinkSlots sink words
 };
 
 // Counter block zeroed at the start of every run: [flags | bucket counts].
@@ -86,8 +86,7 @@ __device__ __forceinline__ uint64_t mix2(uint64_t h, uint32_t t) {
 // e = s * 64 + lane.  Distances below 64 exchange through lane shuffles,
 // larger ones inside a lane's registers.
 template <int R>
-__device__ __forceinline__ void waveBitonicSort(uint64_t (&h)[R], uint32_t (&r)[R], uint32_t (&x)[R],
-                                                uint32_t lane) {
+__device__ __forceinline__ void waveBitonicSort(uint64_t (&h)[R], uint32_t (&r)[R], uint32_t lane) {
   constexpr uint32_t N = 64 * R;
 #pragma unroll
   for (uint32_t k = 2; k <= N; k <<= 1) {
@@ -109,9 +108,6 @@ __device__ __forceinline__ void waveBitonicSort(uint64_t (&h)[R], uint32_t (&r)[
               uint32_t tr = r[s];
               r[s] = r[sp];
               r[sp] = tr;
-              uint32_t tx = x[s];
-              x[s] = x[sp];
-              x[sp] = tx;
             }
           }
         }
@@ -121,7 +117,6 @@ __device__ __forceinline__ void waveBitonicSort(uint64_t (&h)[R], uint32_t (&r)[
           const uint32_t e = static_cast<uint32_t>(s) * 64 + lane;
           const uint64_t ho = __shfl_xor(h[s], static_cast<int>(j));
           const uint32_t ro = __shfl_xor(r[s], static_cast<int>(j));
-          const uint32_t xo = __shfl_xor(x[s], static_cast<int>(j));
           const bool lower = (lane & j) == 0;
           const bool asc = (e & k) == 0;
           const bool otherLess = keyLess(ho, ro, h[s], r[s]);
@@ -130,7 +125,6 @@ __device__ __forceinline__ void waveBitonicSort(uint64_t (&h)[R], uint32_t (&r)[
           if (take) {
             h[s] = ho;
             r[s] = ro;
-            x[s] = xo;
           }
         }
       }
@@ -288,183 +282,222 @@ __device__ __forceinline__ void ldsBarrier() {
 // (out-of-range slots re-read the layer's last record, or record 0, and are
 // ignored by the consumer), so the compiler can leave them in flight instead
 // of waiting on a select.
+// The sweep addresses LDS by raw byte address (its kernel has no static LDS,
+// so its dynamic LDS starts at address 0): the addresses stored in the
+// records feed the LDS instructions with no base added.
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+__device__ __forceinline__ lds_u32 *ldsAt(uint32_t byteAddr) {
+  return reinterpret_cast<lds_u32 *>(static_cast<uintptr_t>(byteAddr));
+}
+
+// Loads this thread's records of one layer: a buffer load with the layer's
+// byte offset in the scalar offset, so the address costs no vector work.
 template <int RPT>
-__device__ __forceinline__ void loadLayer(uint4 (&r)[RPT], const uint4 *__restrict__ ac, uint32_t base,
-                                          uint32_t end) {
-  const uint32_t last = end > base ? end - 1 : 0;
+__device__ __forceinline__ void loadLayer(uint4 (&r)[RPT], __amdgpu_buffer_rsrc_t rs, uint32_t layerOff,
+                                          const uint32_t (&off)[RPT]) {
+  layerOff = __builtin_amdgcn_readfirstlane(layerOff);
 #pragma unroll
   for (int k = 0; k < RPT; ++k) {
-    const uint32_t a = base + threadIdx.x + k * kSweepBlock;
-    r[k] = ac[a < end ? a : last];
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(rs, off[k], layerOff, 0);
+    r[k] = make_uint4(v[0], v[1], v[2], v[3]);
   }
 }
 
 // Backward sensitivity sweep of one 32-bit column of one circuit.  The
-// sensitivities of layers l, l+1, l+2 live in an LDS ring indexed by active
-// position; each layer is one pass over the layer's active-node records (RPT
-// per thread) followed by one LDS-only barrier.  Records are prefetched three
-// layers ahead into three register sets that rotate statically (the loop is
-// unrolled by three), so a record load has two full layers to land.  Below
-// the lowest seed of the column, the barrier also carries a vote (one LDS
-// word per layer) that stops the sweep once two consecutive layers are zero.
-// On exit the column's band [lo, hi] is recorded and the column is added to
-// the per-layer column masks of those layers.
+// sensitivities of layers l, l+1, l+2 live in an LDS ring (slot l mod 3);
+// each layer is one pass over the layer's fixed-stride block of records
+// (sweepRpt per thread) followed by one LDS-only barrier.  A record carries
+// the LDS byte addresses of its children and seed and the byte offset of
+// its S entry, so a record costs two (three if seeded) LDS reads, two XORs,
+// one LDS write and one global store.  The loop runs trips of three layers,
+// unrolled, so each position in a trip has a fixed ring slot: the ring
+// addresses a thread writes are per-column constants, and the record
+// prefetch (three layers ahead, into three rotating register sets) advances
+// one scalar offset.  Per-layer scalar work is what bounds a layer here
+// (one scalar instruction per SIMD per four cycles, shared by its waves), so
+// the early-termination vote runs once per trip: a trip whose sensitivities
+// are all zero, at or below the column's lowest seed, ends the band (the ring
+// then holds zeros and no seed lies below).  Each trip ORs its values into
+// one of two LDS words (by trip parity) before its last barrier; the next
+// trip reads the word in its first layer, behind that layer's own LDS reads,
+// and clears it after its first barrier.  Waves beyond the circuit's
+// sweepThreads (a batch launches with its widest circuit's shape) only take
+// part in the barriers and votes.  On exit the column's band [lo, hi] is
+// recorded and the column is added to the per-layer column masks of those
+// layers.
+constexpr int kSweepDepth = 3;  // prefetch distance = layers per trip
+
 template <int RPT>
 struct SweepState {
-  const uint32_t *la;     // LDS copy of the layer offsets
-  const uint4 *ac;
-  const uint32_t *seeds;  // LDS copy (unused when GSEEDS); seeds[seedZero] = 0
-  uint32_t *ring;         // LDS: 3 layer slots, then a zero word and a sink word
-  uint32_t *vote;         // 3 rotating LDS words
-  uint32_t M;
-  uint32_t zeroWord, sinkWord, seedZero;
-  int32_t lowSeed;
-  uint32_t cur, nb1, nb2;  // ring slots of layers l, l+1, l+2
-  bool prevZero;
+  __amdgpu_buffer_rsrc_t ac;  // the circuit's record blocks
+  const uint32_t *gseeds;  // the column's seeds in global memory (GSEEDS)
+  uint32_t layerBytes;     // 16 * layerStride
+  uint32_t zeroB, seedB;   // byte addresses of the zero word and the first seed
+  uint32_t off[RPT];       // byte offsets of this thread's records in a block
 };
 
-// One layer: consume `r` (records of layer l, bounds [b, e)), then refill it
-// with layer l-3.  Returns true when the column is known to be zero below l.
+__device__ __forceinline__ uint32_t sweepSlot(int32_t l) { return static_cast<uint32_t>(l + 6) % 3; }  // l >= -6
+
+// One layer: consume `r` (this thread's records of the layer), write the
+// values to the ring at `dst` and to S, OR them into `any`, then refill `r`
+// with the block kSweepDepth layers below (`loadOff`, advanced here).
 template <int RPT, bool GSEEDS>
-__device__ __forceinline__ bool sweepLayer(SweepState<RPT> &st, uint4 (&r)[RPT], uint32_t &b, uint32_t &e,
-                                           int32_t l, const uint32_t *__restrict__ gseeds,
-                                           char *__restrict__ Sc, uint32_t sinkOff) {
-  const uint32_t tid = threadIdx.x;
-  uint32_t any = 0;
-  // Branch-free record processing: every lane issues the same LDS and
-  // global operations (missing children and seeds read a zero word, invalid
-  // slots write to sink slots), so the compiler can count outstanding memory
-  // operations exactly and leave the prefetched records in flight.
+__device__ __forceinline__ void sweepRecords(const SweepState<RPT> &st, uint4 (&r)[RPT],
+                                             const uint32_t (&dst)[RPT], uint32_t &loadOff,
+                                             uint32_t &any, char *__restrict__ Sc) {
+  // all of the wave's child reads first, so the records' round trips
+  // overlap; seeds are read only by waves holding a seeded record (seeded
+  // nodes are rare: measurement nodes)
+  uint32_t v[RPT];
+#pragma unroll
+  for (int k = 0; k < RPT; ++k) v[k] = *ldsAt(r[k].x) ^ *ldsAt(r[k].y);
+  if constexpr (GSEEDS) {
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+      const bool hs = r[k].z != st.zeroB;
+      const uint32_t g = st.gseeds[hs ? (r[k].z - st.seedB) >> 2 : 0u];
+      v[k] ^= hs ? g : 0u;
+    }
+  } else {
+    bool seeded = false;
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) seeded |= r[k].z != st.zeroB;
+    if (__ballot(seeded) != 0ull) {
+#pragma unroll
+      for (int k = 0; k < RPT; ++k) v[k] ^= *ldsAt(r[k].z);
+    }
+  }
 #pragma unroll
   for (int k = 0; k < RPT; ++k) {
-    const uint32_t a = b + tid + k * kSweepBlock;
-    const bool valid = a < e;
-    const uint4 x = r[k];
-    uint32_t s;
-    if constexpr (GSEEDS) {
-      s = (valid && x.z != NONE) ? gseeds[x.z] : 0u;
-    } else {
-      s = st.seeds[(valid && x.z != NONE) ? x.z : st.seedZero];
-    }
-    const uint32_t i0 = (valid && x.x != NONE) ? ((x.x >> 30) == 1 ? st.nb1 : st.nb2) + (x.x & 0x3FFFFFFFu)
-                                                : st.zeroWord;
-    const uint32_t i1 = (valid && x.y != NONE) ? ((x.y >> 30) == 1 ? st.nb1 : st.nb2) + (x.y & 0x3FFFFFFFu)
-                                                : st.zeroWord;
-    s ^= st.ring[i0] ^ st.ring[i1];
-    st.ring[valid ? st.cur + (a - b) : st.sinkWord] = s;
-    // 32-bit byte offset from the column's (uniform) base: the store takes
-    // the SGPR-base form and needs no 64-bit copy of the in-flight record
-    const uint32_t off = (valid && x.w != NONE) ? (x.w << 2) : sinkOff;
-    *reinterpret_cast<uint32_t *>(Sc + off) = s;
-    any |= s;
+    *ldsAt(dst[k]) = v[k];
+    *reinterpret_cast<uint32_t *>(Sc + r[k].w) = v[k];
+    any |= v[k];
   }
-  // refill with layer l-3 (bounds from the LDS copy of layerAct); no branch,
-  // so the loads stay in flight: below layer 0 the range is empty
-  const bool more = l >= 3;
-  const uint32_t ln = more ? static_cast<uint32_t>(l - 3) : 0u;
-  b = more ? st.la[ln] : 0u;
-  e = more ? st.la[ln + 1] : 0u;
-  loadLayer<RPT>(r, st.ac, b, e);
-  const bool voting = l >= 0 && l <= st.lowSeed + 1;
-  const uint32_t slot = static_cast<uint32_t>(l) % 3;
-  if (voting && __ballot(any != 0) != 0ull && (tid & 63) == 0) atomicOr(&st.vote[slot], 1u);
-  ldsBarrier();
-  bool stop = false;
-  if (voting) {
-    const bool zero = st.vote[slot] == 0u;
-    if (tid == 0) st.vote[(slot + 1) % 3] = 0u;  // reused two layers below
-    stop = zero && st.prevZero && l <= st.lowSeed;
-    st.prevZero = zero;
-  }
-  const uint32_t t = st.nb2;
-  st.nb2 = st.nb1;
-  st.nb1 = st.cur;
-  st.cur = t;
-  return stop;
+  loadOff -= st.layerBytes;
+  loadLayer<RPT>(r, st.ac, loadOff, st.off);
 }
 
 template <int RPT, bool GSEEDS>
-__global__ __launch_bounds__(kSweepBlock) void sweepKernel(
-    const VDesc *__restrict__ desc, const uint2 *__restrict__ items,
-    const uint32_t *__restrict__ layerAct, const uint4 *__restrict__ act,
+__global__ __launch_bounds__(kSweepMaxThreads) void sweepKernel(
+    const VDesc *__restrict__ desc, const uint2 *__restrict__ items, const uint4 *__restrict__ act,
     const uint32_t *__restrict__ seedVals, const int32_t *__restrict__ colHi,
     const int32_t *__restrict__ colLow, int32_t *__restrict__ bandLo, uint32_t *__restrict__ S,
     uint32_t *__restrict__ masks) {
-  extern __shared__ uint32_t lds[];
-  __shared__ uint32_t vote[3];
+  // dynamic LDS only, addressed from 0:
+  // ring[3P] | zero word | seeds (unless GSEEDS) | vote[2]
   const uint32_t tid = threadIdx.x;
   const uint2 it = items[blockIdx.x];
   const VDesc d = desc[it.x];
   const uint32_t c = it.y;
-  const int32_t hi = colHi[d.colOff + c];
+  const int32_t hi = __builtin_amdgcn_readfirstlane(colHi[d.colOff + c]);
   if (hi < 0) {
     if (tid == 0) bandLo[d.colOff + c] = 0x7FFFFFFF;
     return;
   }
-  const uint32_t M = d.maxAct;
-  // LDS: ring[3M] | zero word | sink word | layer offsets[L+1] | seeds + 0
-  uint32_t *laL = lds + 3 * M + 2;
-  uint32_t *seedL = laL + d.numLayers + 1;
+  const int32_t lowSeed = __builtin_amdgcn_readfirstlane(colLow[d.colOff + c]);
+  const uint32_t P = d.layerStride, T = d.sweepThreads;
   const uint32_t *sv = seedVals + d.seedOff + static_cast<uint64_t>(c) * d.numSeeds;
-  const uint32_t *laG = layerAct + d.layerActOff;
-  for (uint32_t i = tid; i < 3 * M + 2; i += kSweepBlock) lds[i] = 0u;
-  for (uint32_t i = tid; i <= d.numLayers; i += kSweepBlock) laL[i] = laG[i];
+  for (uint32_t i = tid; i <= 3 * P; i += blockDim.x) *ldsAt(4 * i) = 0u;  // ring + zero word
+  uint32_t voteB = 4 * (3 * P + 1);
   if constexpr (!GSEEDS) {
-    for (uint32_t i = tid; i < d.numSeeds; i += kSweepBlock) seedL[i] = sv[i];
-    if (tid == 0) seedL[d.numSeeds] = 0u;
+    for (uint32_t i = tid; i < d.numSeeds; i += blockDim.x) *ldsAt(4 * (3 * P + 1 + i)) = sv[i];
+    voteB += 4 * d.numSeeds;
   }
-  if (tid < 3) vote[tid] = 0u;
+  if (tid < 2) *ldsAt(voteB + 4 * tid) = 0u;
   __syncthreads();
 
-  char *Sc = reinterpret_cast<char *>(S + d.sOff + static_cast<uint64_t>(c) * d.colStride);
-  // each column ends with one sink slot per thread, for writes of invalid slots
-  const uint32_t sinkOff = (d.numRanks + tid) << 2;
-  SweepState<RPT> st;
-  st.la = laL;
-  st.ac = act + d.actOff;
-  st.seeds = seedL;
-  st.ring = lds;
-  st.zeroWord = 3 * M;
-  st.sinkWord = 3 * M + 1;
-  st.seedZero = d.numSeeds;
-  st.vote = vote;
-  st.M = M;
-  st.lowSeed = colLow[d.colOff + c];
-  st.cur = static_cast<uint32_t>(hi % 3) * M;
-  st.nb1 = static_cast<uint32_t>((hi + 1) % 3) * M;
-  st.nb2 = static_cast<uint32_t>((hi + 2) % 3) * M;
-  st.prevZero = true;
-
-  // three register sets: layers hi, hi-1, hi-2
-  uint4 ra[RPT], rb[RPT], rc[RPT];
-  uint32_t ba = st.la[hi], ea = st.la[hi + 1];
-  uint32_t bb = hi >= 1 ? st.la[hi - 1] : 0, eb = hi >= 1 ? ba : 0;
-  uint32_t bc = hi >= 2 ? st.la[hi - 2] : 0, ec = hi >= 2 ? bb : 0;
-  loadLayer<RPT>(ra, st.ac, ba, ea);
-  loadLayer<RPT>(rb, st.ac, bb, eb);
-  loadLayer<RPT>(rc, st.ac, bc, ec);
-
-  // Three layers per trip and a single exit at the end of the trip (no
-  // exits inside, which would force the compiler to drain the prefetches at
-  // the loop head).  Layers below 0 are empty; the at most two layers after
-  // the stop recompute zeros below the band, which nothing reads.
-  int32_t l = hi;
-  int32_t lo = -1;
-  for (;;) {
-    if (sweepLayer<RPT, GSEEDS>(st, ra, ba, ea, l, sv, Sc, sinkOff) && lo < 0) lo = l;
-    --l;
-    if (sweepLayer<RPT, GSEEDS>(st, rb, bb, eb, l, sv, Sc, sinkOff) && lo < 0) lo = l;
-    --l;
-    if (sweepLayer<RPT, GSEEDS>(st, rc, bc, ec, l, sv, Sc, sinkOff) && lo < 0) lo = l;
-    --l;
-    if (lo >= 0 || l < 0) break;
+  // Both loops below run three-layer trips with the vote steps at the same
+  // points and decide on leaving at the end of a trip, from the same vote
+  // words, so all waves meet the same number of barriers.  A single exit
+  // per trip keeps the prefetches in flight (an exit inside would make the
+  // compiler drain them at the loop head).  Layers below 0 process padding;
+  // the trip after a zero trip recomputes zeros below the band, which
+  // nothing reads.
+  int32_t l = hi;     // top layer of the current trip
+  int32_t lo = -1;    // lowest layer of the band, once known
+  uint32_t trip = 0;  // vote word parity
+  // whole waves: sweepThreads is a multiple of 64
+  if (__builtin_amdgcn_readfirstlane(tid >> 6) < (T >> 6)) {
+    SweepState<RPT> st;
+    st.ac = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint4 *>(act + d.actOff), 0,
+                                              16 * P * (kSweepLeadPad + d.numLayers), 0x00020000);
+    st.gseeds = sv;
+    st.layerBytes = 16 * P;
+    st.zeroB = 12 * P;
+    st.seedB = 12 * P + 4;
+    char *Sc = reinterpret_cast<char *>(S + d.sOff + static_cast<uint64_t>(c) * d.colStride);
+    // ring addresses of this thread's records at each trip position
+    uint32_t dst[kSweepDepth][RPT];
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+      // circuits with fewer records per thread repeat their last record
+      // (same value to the same addresses)
+      const uint32_t j = static_cast<uint32_t>(k) < d.sweepRpt ? static_cast<uint32_t>(k) : d.sweepRpt - 1;
+      const uint32_t pos = j * T + tid;
+      st.off[k] = pos * 16;
+#pragma unroll
+      for (int u = 0; u < kSweepDepth; ++u) dst[u][k] = (sweepSlot(hi - u) * P + pos) * 4;
+    }
+    // register sets for layers hi, hi-1, hi-2 (block kSweepLeadPad + layer).
+    // Each set but the last is followed by RPT stores to sink words, as each
+    // refill is followed by a layer's RPT stores in the loop: the compiler's
+    // count of memory operations outstanding at the loop head is then the
+    // same on entry and on the back edge, and its waits keep the full
+    // prefetch distance.  One store instruction per record: the sink words of
+    // one store are contiguous across lanes, those of different stores 32
+    // words apart (mod kSinkSlots; at most 16 stores, so a lane's addresses
+    // differ).
+    uint32_t *sink = reinterpret_cast<uint32_t *>(Sc) + d.numRanks;
+    uint4 rs[kSweepDepth][RPT];
+    uint32_t loadOff = static_cast<uint32_t>(hi + static_cast<int32_t>(kSweepLeadPad)) * st.layerBytes;
+#pragma unroll
+    for (int u = 0; u < kSweepDepth; ++u) {
+      loadLayer<RPT>(rs[u], st.ac, loadOff, st.off);
+      __builtin_amdgcn_sched_barrier(0);
+      if (u + 1 < kSweepDepth) {
+        loadOff -= st.layerBytes;
+#pragma unroll
+        for (int k = 0; k < RPT; ++k) sink[(tid + 32 * (u * RPT + k)) & (kSinkSlots - 1)] = 0u;
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+    for (;;) {
+      uint32_t any = 0;
+      const uint32_t myWord = voteB + 4 * (trip & 1), prevWord = voteB + 4 * ((trip + 1) & 1);
+      // first layer: also read the previous trip's vote
+      sweepRecords<RPT, GSEEDS>(st, rs[0], dst[0], loadOff, any, Sc);
+      const uint32_t prev = trip ? *ldsAt(prevWord) : 1u;
+      if (prev == 0u && l + 1 <= lowSeed && lo < 0) lo = l + 1;
+      ldsBarrier();
+      if (tid == 0) *ldsAt(prevWord) = 0u;
+      sweepRecords<RPT, GSEEDS>(st, rs[1], dst[1], loadOff, any, Sc);
+      ldsBarrier();
+      sweepRecords<RPT, GSEEDS>(st, rs[2], dst[2], loadOff, any, Sc);
+      if (__ballot(any != 0) != 0ull && (tid & 63) == 0) *ldsAt(myWord) = 1u;
+      ldsBarrier();
+      l -= kSweepDepth;
+      ++trip;
+      if (lo >= 0 || l < 0) break;
+    }
+  } else {
+    for (;;) {
+      const uint32_t prevWord = voteB + 4 * ((trip + 1) & 1);
+      const uint32_t prev = trip ? *ldsAt(prevWord) : 1u;
+      if (prev == 0u && l + 1 <= lowSeed && lo < 0) lo = l + 1;
+      ldsBarrier();
+      ldsBarrier();
+      ldsBarrier();
+      l -= kSweepDepth;
+      ++trip;
+      if (lo >= 0 || l < 0) break;
+    }
   }
   if (lo < 0) lo = 0;
   if (tid == 0) bandLo[d.colOff + c] = lo;
   // add the column to the masks of the layers in its band
   uint32_t *mk = masks + d.maskOff + (c >> 5);
   const uint32_t bit = 1u << (c & 31);
-  for (int32_t k = lo + static_cast<int32_t>(tid); k <= hi; k += kSweepBlock)
+  for (int32_t k = lo + static_cast<int32_t>(tid); k <= hi; k += static_cast<int32_t>(blockDim.x))
     atomicOr(&mk[static_cast<uint64_t>(k) * d.maskWords], bit);
 }
 
@@ -522,50 +555,35 @@ __global__ __launch_bounds__(kClassBlock) void partitionKernel(
     const VDesc *__restrict__ desc, const uint2 *__restrict__ tiles,
     const uint64_t *__restrict__ hash, const uint32_t *__restrict__ nnz,
     const uint32_t *__restrict__ bucketStart, const uint32_t *__restrict__ bucketPos,
-    const double *__restrict__ rankProb, const uint64_t *__restrict__ h2,
-    uint64_t *__restrict__ pkey, uint32_t *__restrict__ prank, double *__restrict__ pprob,
-    uint64_t *__restrict__ ph2, uint32_t *__restrict__ pnnz) {
+    uint64_t *__restrict__ pkey, uint32_t *__restrict__ prank) {
   const uint2 t = tiles[blockIdx.x];
   const VDesc d = desc[t.x];
   const uint32_t i = t.y + threadIdx.x;
   if (i >= d.numRanks) return;
   const uint64_t gi = d.rankOff + i;
-  const uint32_t n = nnz[gi];
-  if (!n) return;
+  if (!nnz[gi]) return;
   const uint64_t h = hash[gi];
   const uint32_t b = d.bucketBase + (d.bucketBits ? static_cast<uint32_t>(h >> (64 - d.bucketBits)) : 0u);
   const uint32_t pos = bucketStart[b] + bucketPos[gi];
   pkey[pos] = h;
   prank[pos] = static_cast<uint32_t>(gi);
-  pprob[pos] = rankProb[gi];
-  ph2[pos] = h2[gi];
-  pnnz[pos] = n;
 }
 
 template <int R>
 __device__ void bucketWave(uint32_t b, uint32_t bs, uint32_t cnt, uint32_t lane,
-                           const uint64_t *__restrict__ pkey, const double *__restrict__ pprob,
-                           const uint64_t *__restrict__ ph2, const uint32_t *__restrict__ pnnz,
-                           const RunOut &o, double *sProb, uint64_t *sH2, uint32_t *sNnz,
+                           const uint64_t *__restrict__ pkey, const RunOut &o,
                            uint32_t *__restrict__ runCount, uint32_t *__restrict__ runNnz) {
   uint64_t h[R];
-  uint32_t r[R], x[R];
+  uint32_t r[R];
 #pragma unroll
   for (int s = 0; s < R; ++s) {
     const uint32_t e = static_cast<uint32_t>(s) * 64 + lane;
-    const bool v = e < cnt;
-    h[s] = v ? pkey[bs + e] : ~0ull;
-    r[s] = v ? o.sortedRank[bs + e] : NONE;
-    x[s] = e;
-    if (v) {  // stage what the merge reads, indexed by position in the bucket
-      sProb[e] = pprob[bs + e];
-      sH2[e] = ph2[bs + e];
-      sNnz[e] = pnnz[bs + e];
-    }
+    h[s] = e < cnt ? pkey[bs + e] : ~0ull;
+    r[s] = e < cnt ? o.sortedRank[bs + e] : NONE;
   }
-  waveBitonicSort<R>(h, r, x, lane);
+  waveBitonicSort<R>(h, r, lane);
   RunCarry cy{0.0, 0ull, 0ull, 0u, 0u, 0u, 0u};
-  scanRuns<R>(h, r, x, cnt, 0u, bs, cy, lane, o, RunLookup{sProb, sH2, sNnz});
+  scanRuns<R>(h, r, r, cnt, 0u, bs, cy, lane, o, RunLookup{o.prob, o.h2, o.nnz});
   if (lane == 0) {
     runCount[b] = cy.runs;
     runNnz[b] = cy.nnzSum;
@@ -576,16 +594,9 @@ __device__ void bucketWave(uint32_t b, uint32_t bs, uint32_t cnt, uint32_t lane,
 // wave capacity are queued for bucketFallbackKernel.
 __global__ __launch_bounds__(256) void bucketKernel(
     uint32_t numBuckets, const uint32_t *__restrict__ bucketStart, const uint64_t *__restrict__ pkey,
-    const double *__restrict__ pprob, const uint64_t *__restrict__ ph2, const uint32_t *__restrict__ pnnz,
     RunOut o, uint32_t *__restrict__ runCount, uint32_t *__restrict__ runNnz,
     uint32_t *__restrict__ overflowList) {
-  // per-wave LDS staging of the values the merge reads (wave-private: the
-  // wave's own lanes write and read it, in order, so no barrier is needed)
-  __shared__ double sProb[4][kBucketItemsPerWave];
-  __shared__ uint64_t sH2[4][kBucketItemsPerWave];
-  __shared__ uint32_t sNnz[4][kBucketItemsPerWave];
   const uint32_t lane = threadIdx.x & 63;
-  const uint32_t wv = threadIdx.x >> 6;
   const uint32_t b = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
   if (b >= numBuckets) return;
   const uint32_t bs = bucketStart[b];
@@ -596,11 +607,11 @@ __global__ __launch_bounds__(256) void bucketKernel(
       runNnz[b] = 0;
     }
   } else if (cnt <= 64) {
-    bucketWave<1>(b, bs, cnt, lane, pkey, pprob, ph2, pnnz, o, sProb[wv], sH2[wv], sNnz[wv], runCount, runNnz);
+    bucketWave<1>(b, bs, cnt, lane, pkey, o, runCount, runNnz);
   } else if (cnt <= 128) {
-    bucketWave<2>(b, bs, cnt, lane, pkey, pprob, ph2, pnnz, o, sProb[wv], sH2[wv], sNnz[wv], runCount, runNnz);
+    bucketWave<2>(b, bs, cnt, lane, pkey, o, runCount, runNnz);
   } else if (cnt <= kBucketItemsPerWave) {
-    bucketWave<4>(b, bs, cnt, lane, pkey, pprob, ph2, pnnz, o, sProb[wv], sH2[wv], sNnz[wv], runCount, runNnz);
+    bucketWave<4>(b, bs, cnt, lane, pkey, o, runCount, runNnz);
   } else if (lane == 0) {
     const uint32_t idx = atomicAdd(&o.flags[2], 1u);
     overflowList[idx] = b;
@@ -964,15 +975,15 @@ struct Pipeline::Impl {
   // inputs (one device arena, one pinned staging arena with the same layout)
   DevBuf in;
   HostBuf inHost;
-  size_t oDesc = 0, oItems = 0, oTiles = 0, oLayerAct = 0, oAct = 0, oSeed = 0, oColHi = 0,
+  size_t oDesc = 0, oItems = 0, oTiles = 0, oAct = 0, oSeed = 0, oColHi = 0,
          oColLow = 0, oRankLayer = 0, oRankProb = 0, inBytes = 0;
   uint32_t V = 0, nItems = 0, nTiles = 0, B = 0, R = 0, Ccols = 0, rpt = 1;
   uint64_t Swords = 0, maskWordsTotal = 0;
   size_t sweepLdsBytes = 0;
   int seedsInLds = 1;
+  uint32_t sweepThreads = 64;
 
-  DevBuf bandLo, S, hash, h2, nnz, counters, bucketStart, bucketPos, pkey, prank, pprob, ph2, pnnz,
-      runRank, runP,
+  DevBuf bandLo, S, hash, h2, nnz, counters, bucketStart, bucketPos, pkey, prank, runRank, runP,
       runHash, runTgtPre, runMember, runCount, runNnz, classStart, tgtStart, outP, outOff, outHash,
       outMember, outTargets, scratchKey, scratchRank, overflowList, scanTmp, vBounds, masks;
   uint64_t capTargets = 0;
@@ -1048,8 +1059,9 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   // layout
   m.descs.assign(V, VDesc{});
   variants_.assign(V, VariantInfo{});
-  uint64_t layerActOff = 0, actOff = 0, seedOff = 0, colOff = 0, rankOff = 0, sOff = 0, maskOff = 0;
-  uint32_t bucketBase = 0, maxAct = 0, maxSeeds = 0, maxLayers = 0;
+  uint64_t actOff = 0, seedOff = 0, colOff = 0, rankOff = 0, sOff = 0, maskOff = 0;
+  uint32_t bucketBase = 0, maxRpt = 1, maxThreads = 64;
+  size_t ldsRing = 0, ldsAll = 0;  // largest ring (+ zero word), largest ring + seeds
   std::vector<uint2> items, tiles;
   std::vector<std::pair<int32_t, uint32_t>> order;  // (band estimate, item index)
   for (uint32_t v = 0; v < V; ++v) {
@@ -1065,16 +1077,17 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
     d.bucketBits = bucketBitsFor(g.numRanks);
     d.numBuckets = 1u << d.bucketBits;
     d.bucketBase = bucketBase;
-    d.maxAct = g.maxActive;
     d.maskWords = (g.numCols + 31) / 32;
-    d.layerActOff = layerActOff;
+    d.layerStride = g.layerStride;
+    d.sweepThreads = g.sweepThreads;
+    d.sweepRpt = g.sweepRpt;
     d.actOff = actOff;
     d.seedOff = seedOff;
     d.colOff = colOff;
     d.rankOff = rankOff;
     d.sOff = sOff;
     d.maskOff = maskOff;
-    d.colStride = static_cast<uint64_t>(g.numRanks) + kSweepBlock;
+    d.colStride = static_cast<uint64_t>(g.numRanks) + kSinkSlots;
     variants_[v] = VariantInfo{g.numRanks, g.numCols, g.numDetectors, g.numObservables, bucketBase,
                                d.numBuckets};
     for (uint32_t c = 0; c < g.numCols; ++c) {
@@ -1084,33 +1097,30 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
       items.push_back(make_uint2(v, c));
     }
     for (uint32_t t = 0; t < g.numRanks; t += kClassBlock) tiles.push_back(make_uint2(v, t));
-    layerActOff += g.numLayers + 1;
-    actOff += g.numActive;
+    actOff += static_cast<uint64_t>(kSweepLeadPad + g.numLayers) * g.layerStride;
     seedOff += static_cast<uint64_t>(g.numCols) * g.numSeeds;
     colOff += g.numCols;
     rankOff += g.numRanks;
     sOff += d.colStride * g.numCols;
     maskOff += static_cast<uint64_t>(g.numLayers) * d.maskWords;
     bucketBase += d.numBuckets;
-    maxAct = std::max(maxAct, g.maxActive);
-    maxSeeds = std::max(maxSeeds, g.numSeeds);
-    maxLayers = std::max(maxLayers, g.numLayers);
+    maxRpt = std::max(maxRpt, g.sweepRpt);
+    maxThreads = std::max(maxThreads, g.sweepThreads);
+    ldsRing = std::max<size_t>(ldsRing, 4ull * (3ull * g.layerStride + 1 + 2));
+    ldsAll = std::max<size_t>(ldsAll, 4ull * (3ull * g.layerStride + 1 + g.numSeeds + 2));
+    if (16ull * g.layerStride * (kSweepLeadPad + g.numLayers) >= (1ull << 32))
+      throw RuntimeError("circuit too large for the sweep: sweep records exceed 4 GB");
   }
   if (rankOff >= 0x7FFFFFFFull) throw RuntimeError("too many error nodes in one load");
-  // sweep launch shape: records per thread and what fits in LDS
-  const uint32_t rpt = (maxAct + kSweepBlock - 1) / kSweepBlock;
-  m.rpt = rpt <= 1 ? 1 : rpt <= 2 ? 2 : rpt <= 4 ? 4 : rpt <= 8 ? 8 : 0;
-  if (!m.rpt)
-    throw RuntimeError("circuit has too many active STEPG nodes per layer for the LDS sweep (" +
-                       std::to_string(maxAct) + " > 8192)");
-  // LDS: ring (3 layers) + layer offsets, plus the column's seeds if they fit
-  size_t lds = 12ull * maxAct + 4ull * (maxLayers + 1);
-  if (lds > kLdsBudget)
-    throw RuntimeError("circuit too wide or too deep for the LDS sweep (" + std::to_string(maxAct) +
-                       " active nodes per layer, " + std::to_string(maxLayers) + " layers)");
-  m.seedsInLds = lds + 4ull * maxSeeds <= kLdsBudget;
-  if (m.seedsInLds) lds += 4ull * maxSeeds;
-  m.sweepLdsBytes = lds;
+  // sweep launch shape (the widest circuit's) and LDS: the ring and zero
+  // word, plus the column's seeds if they fit
+  m.rpt = maxRpt;
+  m.sweepThreads = maxThreads;
+  if (ldsRing > kLdsBudget)
+    throw RuntimeError("circuit too wide for the LDS sweep (" + std::to_string(ldsRing) +
+                       " bytes of LDS per layer ring)");
+  m.seedsInLds = ldsAll <= kLdsBudget;
+  m.sweepLdsBytes = m.seedsInLds ? ldsAll : ldsRing;
 
   // longest bands first (they bound the sweep's critical path)
   std::stable_sort(order.begin(), order.end(),
@@ -1137,14 +1147,13 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   m.oDesc = place(sizeof(VDesc) * V);
   m.oItems = place(sizeof(uint2) * m.nItems);
   m.oTiles = place(sizeof(uint2) * m.nTiles);
-  m.oLayerAct = place(4 * layerActOff);
   m.oAct = place(16 * actOff);
   m.oSeed = place(4 * seedOff);
   m.oColHi = place(4 * colOff);
   m.oColLow = place(4 * colOff);
   m.oRankLayer = place(4 * rankOff);
   m.oRankProb = place(8 * rankOff);
-  m.inBytes = off + 256;  // slack: the sweep's clamped record loads may touch record 0
+  m.inBytes = off;
   m.in.ensure(m.inBytes, frozen, "inputs");
   m.inHost.ensure(m.inBytes);
   char *hb = m.inHost.as<char>();
@@ -1154,7 +1163,6 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   for (uint32_t v = 0; v < V; ++v) {
     const StepgImage &g = *imgs[v];
     const VDesc &d = m.descs[v];
-    std::memcpy(hb + m.oLayerAct + 4 * d.layerActOff, g.layerAct.data(), 4 * g.layerAct.size());
     std::memcpy(hb + m.oAct + 16 * d.actOff, g.act.data(), 4 * g.act.size());
     std::memcpy(hb + m.oSeed + 4 * d.seedOff, g.seedVals.data(), 4 * g.seedVals.size());
     std::memcpy(hb + m.oColHi + 4 * d.colOff, g.colHi.data(), 4 * g.colHi.size());
@@ -1176,9 +1184,6 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   m.bucketStart.ensure(4 * (B + 1), frozen, "buckets");
   m.pkey.ensure(8 * R, frozen, "nodes");
   m.prank.ensure(4 * R, frozen, "nodes");
-  m.pprob.ensure(8 * R, frozen, "nodes");
-  m.ph2.ensure(8 * R, frozen, "nodes");
-  m.pnnz.ensure(4 * R, frozen, "nodes");
   m.runRank.ensure(4 * R, frozen, "nodes");
   m.runP.ensure(8 * R, frozen, "nodes");
   m.runHash.ensure(8 * R, frozen, "nodes");
@@ -1235,7 +1240,6 @@ void Pipeline::run(bool timed) {
   const VDesc *desc = reinterpret_cast<const VDesc *>(ib + m.oDesc);
   const uint2 *items = reinterpret_cast<const uint2 *>(ib + m.oItems);
   const uint2 *tiles = reinterpret_cast<const uint2 *>(ib + m.oTiles);
-  const uint32_t *layerAct = reinterpret_cast<const uint32_t *>(ib + m.oLayerAct);
   const uint4 *act = reinterpret_cast<const uint4 *>(ib + m.oAct);
   const uint32_t *seed = reinterpret_cast<const uint32_t *>(ib + m.oSeed);
   const int32_t *colHi = reinterpret_cast<const int32_t *>(ib + m.oColHi);
@@ -1262,8 +1266,8 @@ void Pipeline::run(bool timed) {
   mark(0);
   if (m.nItems) {
 #define GPA_SWEEP(RPT, G)                                                                        \
-  sweepKernel<RPT, G><<<m.nItems, kSweepBlock, m.sweepLdsBytes, st>>>(                           \
-      desc, items, layerAct, act, seed, colHi, colLow, m.bandLo.as<int32_t>(), m.S.as<uint32_t>(), \
+  sweepKernel<RPT, G><<<m.nItems, m.sweepThreads, m.sweepLdsBytes, st>>>(                       \
+      desc, items, act, seed, colHi, colLow, m.bandLo.as<int32_t>(), m.S.as<uint32_t>(), \
       m.masks.as<uint32_t>())
     if (m.seedsInLds) {
       switch (m.rpt) {
@@ -1293,8 +1297,7 @@ void Pipeline::run(bool timed) {
   if (m.nTiles)
     partitionKernel<<<m.nTiles, kClassBlock, 0, st>>>(
         desc, tiles, m.hash.as<uint64_t>(), m.nnz.as<uint32_t>(), m.bucketStart.as<uint32_t>(),
-        m.bucketPos.as<uint32_t>(), rankProb, m.h2.as<uint64_t>(), m.pkey.as<uint64_t>(),
-        m.prank.as<uint32_t>(), m.pprob.as<double>(), m.ph2.as<uint64_t>(), m.pnnz.as<uint32_t>());
+        m.bucketPos.as<uint32_t>(), m.pkey.as<uint64_t>(), m.prank.as<uint32_t>());
   mark(3);
   RunOut o{rankProb,
            m.h2.as<uint64_t>(),
@@ -1306,8 +1309,7 @@ void Pipeline::run(bool timed) {
            m.runTgtPre.as<uint32_t>(),
            m.runMember.as<uint32_t>(),
            m.flags()};
-  bucketKernel<<<(B + 3) / 4, 256, 0, st>>>(B, m.bucketStart.as<uint32_t>(), m.pkey.as<uint64_t>(),
-                                            m.pprob.as<double>(), m.ph2.as<uint64_t>(), m.pnnz.as<uint32_t>(), o,
+  bucketKernel<<<(B + 3) / 4, 256, 0, st>>>(B, m.bucketStart.as<uint32_t>(), m.pkey.as<uint64_t>(), o,
                                             m.runCount.as<uint32_t>(), m.runNnz.as<uint32_t>(),
                                             m.overflowList.as<uint32_t>());
   bucketFallbackKernel<<<64, 256, 0, st>>>(m.bucketStart.as<uint32_t>(), m.pkey.as<uint64_t>(), o,
@@ -1469,8 +1471,8 @@ uint32_t bucketFill() { return gBucketFill.load(); }
 uint64_t Pipeline::deviceBytes() const {
   const Impl &m = *impl_;
   const DevBuf *bufs[] = {&m.in, &m.bandLo, &m.masks, &m.S, &m.hash, &m.h2, &m.nnz, &m.bucketPos,
-                          &m.counters, &m.bucketStart, &m.pkey, &m.prank, &m.pprob, &m.ph2,
-                          &m.pnnz, &m.runRank, &m.runP,
+                          &m.counters, &m.bucketStart, &m.pkey, &m.prank,
+                          &m.runRank, &m.runP,
                           &m.runHash, &m.runTgtPre, &m.runMember, &m.runCount, &m.runNnz,
                           &m.classStart, &m.tgtStart, &m.outP, &m.outOff, &m.outHash, &m.outMember,
                           &m.outTargets, &m.scratchKey, &m.scratchRank, &m.overflowList, &m.scanTmp,

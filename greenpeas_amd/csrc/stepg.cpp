@@ -2,6 +2,7 @@
 #include "stepg.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <string>
 
@@ -256,6 +257,17 @@ void buildStepg(const Circuit &c, int level, Stepg &g, const double *theta, bool
   }
 }
 
+namespace {
+std::atomic<uint32_t> gSweepMinRpt{1};
+}
+
+void setSweepMinRpt(uint32_t rpt) {
+  if (rpt != 1 && rpt != 2 && rpt != 4 && rpt != 8)
+    throw InvalidArgument("sweep_rpt must be 1, 2, 4 or 8");
+  gSweepMinRpt.store(rpt);
+}
+uint32_t sweepMinRpt() { return gSweepMinRpt.load(); }
+
 void buildImage(const Stepg &g, StepgImage &img) {
   const uint32_t N = g.numNodes, npl = g.npl, L = g.numLayers, C = g.numCols;
   img.numLayers = L;
@@ -279,6 +291,8 @@ void buildImage(const Stepg &g, StepgImage &img) {
         img.rankProb.push_back(g.probs[n]);
       }
   img.numRanks = static_cast<uint32_t>(img.rankNode.size());
+  // sweep records address a column of S in bytes with 32 bits
+  if (img.numRanks >= (1u << 30) - kSinkSlots) throw RuntimeError("circuit too large: more than 2^30 error nodes");
 
   // active = p > 0 nodes and all their descendants (children are later)
   std::vector<uint8_t> active(N, 0);
@@ -342,32 +356,56 @@ void buildImage(const Stepg &g, StepgImage &img) {
   img.numActive = total;
   img.maxActive = maxAct;
 
-  // active node records, layer by layer
-  img.act.assign(static_cast<size_t>(total) * 4, kNone);
-  auto code = [&](uint32_t child, uint32_t l) -> uint32_t {
-    if (child == kNone) return kNone;
-    const uint32_t lc = child / npl;
-    return ((lc - l) << 30) | actIdx[child];
+  // sweep shape: the fewest records per thread (a power of two) that fit
+  // the layer in one workgroup, then the fewest 64-lane waves
+  uint32_t rpt = 1;
+  while (rpt < sweepMinRpt()) rpt <<= 1;
+  while (rpt < 8 && static_cast<uint64_t>(rpt) * kSweepMaxThreads < maxAct) rpt <<= 1;
+  if (static_cast<uint64_t>(rpt) * kSweepMaxThreads < maxAct)
+    throw RuntimeError("circuit has too many active STEPG nodes per layer for the LDS sweep (" +
+                       std::to_string(maxAct) + " > " + std::to_string(8 * kSweepMaxThreads) + ")");
+  const uint32_t threads = std::max<uint32_t>(64, ((maxAct + rpt - 1) / rpt + 63) / 64 * 64);
+  const uint32_t P = threads * rpt;
+  img.sweepThreads = threads;
+  img.sweepRpt = rpt;
+  img.layerStride = P;
+
+  // records, layer by layer, with final LDS / S addresses
+  const uint32_t zeroB = 3u * P * 4u;        // the zero word after the ring
+  const uint32_t seedB = (3u * P + 1u) * 4u;  // the column's first seed word
+  img.act.resize(static_cast<size_t>(kSweepLeadPad + L) * P * 4);
+  auto child = [&](uint32_t ch) -> uint32_t {
+    if (ch == kNone) return zeroB;
+    return ((ch / npl) % 3u * P + actIdx[ch]) * 4u;
   };
-  size_t w = 0;
-  for (uint32_t l = 0, n = 0; l < L; ++l)
-    for (uint32_t j = 0; j < npl; ++j, ++n) {
+  for (uint32_t blk = 0, n = 0; blk < kSweepLeadPad + L; ++blk) {
+    const bool pad = blk < kSweepLeadPad;  // block kSweepLeadPad + l holds layer l
+    uint32_t *rec = img.act.data() + static_cast<size_t>(blk) * P * 4;
+    uint32_t pos = 0;
+    for (uint32_t j = 0; !pad && j < npl; ++j, ++n) {
       if (!active[n]) continue;
       const uint64_t a = g.adjacency[n];
-      img.act[w++] = code(static_cast<uint32_t>(a), l);
-      img.act[w++] = code(static_cast<uint32_t>(a >> 32), l);
-      img.act[w++] = seedOf[n];
-      img.act[w++] = rank[n];
+      rec[0] = child(static_cast<uint32_t>(a));
+      rec[1] = child(static_cast<uint32_t>(a >> 32));
+      rec[2] = seedOf[n] == kNone ? zeroB : seedB + 4u * seedOf[n];
+      rec[3] = 4u * (rank[n] != kNone ? rank[n] : img.numRanks + (pos & (kSinkSlots - 1)));
+      rec += 4;
+      ++pos;
     }
+    for (; pos < P; ++pos, rec += 4) {
+      rec[0] = rec[1] = rec[2] = zeroB;
+      rec[3] = 4u * (img.numRanks + (pos & (kSinkSlots - 1)));
+    }
+  }
 }
 
 uint64_t imageDeviceBytes(const StepgImage &img) {
   const uint64_t R = img.numRanks;
   uint64_t b = 0;
-  b += (img.numLayers + 1) * 4ull + img.numActive * 16ull + img.seedVals.size() * 4ull;
+  b += static_cast<uint64_t>(kSweepLeadPad + img.numLayers) * img.layerStride * 16ull + img.seedVals.size() * 4ull;
   b += img.numLayers * ((img.numCols + 31) / 32) * 4ull;  // per-layer column masks
   b += img.numCols * 12ull + R * 12ull;  // bands, rank layer + prob
-  b += (R + 1024) * img.numCols * 4ull;   // sensitivity matrix (+ per-column sink slots)
+  b += (R + kSinkSlots) * img.numCols * 4ull;  // sensitivity matrix (+ per-column sink slots)
   b += R * (8 + 8 + 4 + 4);               // hash, h2, nnz, class-of-rank
   b += R * (8 + 4) * 2;                   // partitioned keys/ranks + scratch
   b += R * (4 + 8 + 8 + 4 + 4 + 4);       // runs

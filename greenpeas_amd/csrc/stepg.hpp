@@ -13,10 +13,16 @@
 // The device image re-lays that graph out for the MI355X sweep:
 //  * only "active" nodes are kept (nodes with p > 0 and every node reachable
 //    from one along flows; the others can never influence an output);
-//  * each active node is one 16-byte record {child0, child1, seed, rank}
-//    with children encoded as (layer delta << 30) | active index within the
-//    child's layer, so the sweep resolves them inside a three-layer LDS ring
-//    indexed by active position;
+//  * each layer is a fixed-stride block of `layerStride` 16-byte records,
+//    the layer's active nodes in node order, then padding; a record holds
+//    the final addresses the sweep uses, so it does no index arithmetic:
+//    {LDS byte address of child 0's sensitivity, of child 1's, of the seed
+//    word, byte offset of the node's sensitivity in its column of S}.  The
+//    sweep keeps layers l, l+1, l+2 in an LDS ring whose slot for layer l is
+//    l mod 3 (slot size `layerStride` words), followed by a zero word (what
+//    absent children, absent seeds and padding read) and the column's seed
+//    words.  Nodes without a rank (and padding) write to one of
+//    kSinkSlots sink words past the column's ranks;
 //  * measurement seeds are stored once per seeded node, column-major over
 //    32-bit detector words ("columns");
 //  * per column, the highest and lowest layer holding a seed bound the time
@@ -33,6 +39,14 @@
 namespace gpa {
 
 constexpr uint32_t kNone = 0xFFFFFFFFu;
+// sink words per column of S (targets of record stores that keep no value)
+constexpr uint32_t kSinkSlots = 1024;
+// largest sweep workgroup; a layer is split over sweepThreads <= this many
+// threads with sweepRpt records each
+constexpr uint32_t kSweepMaxThreads = 1024;
+// all-padding record blocks in front of layer 0 (what the sweep's
+// prefetches load for the layers below 0 that its trips run into)
+constexpr uint32_t kSweepLeadPad = 5;
 
 inline uint32_t nodesPerQubit(int level) {
   switch (level) {
@@ -68,8 +82,12 @@ void buildStepg(const Circuit &c, int level, Stepg &g, const double *theta = nul
 struct StepgImage {
   uint32_t numLayers = 0, npl = 0, numCols = 0, numDetectors = 0, numObservables = 0;
   uint32_t numRanks = 0, numSeeds = 0, numActive = 0, numLocations = 0, maxActive = 0;
-  std::vector<uint32_t> layerAct;  // numLayers + 1 offsets into act
-  std::vector<uint32_t> act;       // 4 * numActive: child0 code, child1 code, seed row, rank
+  // sweep shape: record j*sweepThreads + t of a layer belongs to thread t
+  // (j < sweepRpt); layerStride = sweepThreads * sweepRpt >= maxActive
+  uint32_t sweepThreads = 64, sweepRpt = 1, layerStride = 64;
+  std::vector<uint32_t> layerAct;  // numLayers + 1 offsets of each layer's first active node
+  std::vector<uint32_t> act;       // 4 * (kSweepLeadPad + numLayers) * layerStride: padding
+                                   // blocks, then one block per layer (see the file comment)
   std::vector<uint32_t> seedVals;  // numCols x numSeeds (column-major)
   std::vector<int32_t> colHi, colLow;  // numCols: seed layer range (-1 = no seed)
   std::vector<uint32_t> rankLayer; // numRanks
@@ -78,6 +96,12 @@ struct StepgImage {
 };
 
 void buildImage(const Stepg &g, StepgImage &img);
+
+// Tuning knob: the fewest sweep records per thread an image is built with
+// (1, 2, 4 or 8; more when a layer does not fit 1024 threads).  Fewer waves
+// with more records each trade barrier participants for per-wave work.
+void setSweepMinRpt(uint32_t rpt);
+uint32_t sweepMinRpt();
 
 // Device bytes the pipeline needs for an image (used by admission control).
 uint64_t imageDeviceBytes(const StepgImage &img);

@@ -156,10 +156,14 @@ int gp_batch_device_bytes(const gp_batch *batch, uint64_t *bytes);
 int gp_estimate_device_bytes(const char *circuit_text, int correlation_level,
                              uint64_t *bytes);
 
-/* Autotuning knobs (process-wide, read at the next load):
+/* Autotuning knobs (process-wide):
  *   "bucket_fill": mean number of (hash, node) items per sort bucket the
  *   first radix digit aims at; the digit width per circuit is the smallest b
- *   with bucket_fill * 2^b >= its p>0 node count (default 64). */
+ *   with bucket_fill * 2^b >= its p>0 node count (default 64; read at the
+ *   next load).
+ *   "sweep_rpt": fewest STEPG records per thread of the sensitivity sweep
+ *   (1, 2, 4 or 8; default 1, raised as needed to fit a layer in 1024
+ *   threads; read when a circuit is parsed). */
 int gp_set_tuning(const char *name, uint32_t value);
 int gp_get_tuning(const char *name, uint32_t *value);
 

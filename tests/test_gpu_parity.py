@@ -178,3 +178,24 @@ def test_random_circuits(oracle, seed):
     ref = oracle.compile_stepg(text, 2)
     assert np.array_equal(dem.hashes, ref.hashes)
     assert dem.approx_equals(as_dem(ref), REL)
+
+
+@pytest.mark.parametrize("rpt", [2, 4, 8])
+def test_sweep_records_per_thread_knob(oracle, rpt):
+    """The sweep's launch shape (records per thread) changes nothing in the
+    result: bit-identical DEMs and sensitivities to the default shape."""
+    from greenpeas_amd import autotune
+
+    texts = [read_circuit(n) for n in ("surface5", "bb6", "bb12")]
+    base = [get_driver(t) for t in texts]
+    ref = [(d.compile_detector_error_model(), d.debug_sensitivities()) for d in base]
+    old = autotune.get_tuning("sweep_rpt")
+    try:
+        autotune.set_tuning("sweep_rpt", rpt)
+        for t, (dem0, (idx0, s0)) in zip(texts, ref):
+            d = get_driver(t)
+            assert d.compile_detector_error_model() == dem0
+            idx, s = d.debug_sensitivities()
+            assert np.array_equal(idx, idx0) and np.array_equal(s, s0)
+    finally:
+        autotune.set_tuning("sweep_rpt", old)

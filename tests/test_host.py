@@ -88,3 +88,24 @@ def test_product_overlapping_channels(oracle):
         adj, probs, _ = build_stepg_host(text, level)
         s = oracle.build_stepg(text, level)
         assert np.array_equal(adj, s.adjacency) and np.array_equal(probs, s.probs)
+
+
+def test_tuning_knobs_roundtrip():
+    """Both knobs set and read back through the C-ABI (no device work); bad
+    names and sweep_rpt values are rejected with ValueError."""
+    from greenpeas_amd import autotune
+
+    old_fill, old_rpt = autotune.get_tuning("bucket_fill"), autotune.get_tuning("sweep_rpt")
+    try:
+        autotune.set_tuning("bucket_fill", 48)
+        assert autotune.get_tuning("bucket_fill") == 48
+        for r in (1, 2, 4, 8):
+            autotune.set_tuning("sweep_rpt", r)
+            assert autotune.get_tuning("sweep_rpt") == r
+        with pytest.raises(ValueError):
+            autotune.set_tuning("sweep_rpt", 3)
+        with pytest.raises(ValueError):
+            autotune.set_tuning("no_such_knob", 1)
+    finally:
+        autotune.set_tuning("bucket_fill", old_fill)
+        autotune.set_tuning("sweep_rpt", old_rpt)
