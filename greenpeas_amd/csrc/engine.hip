@@ -42,12 +42,14 @@ struct VDesc {
   uint32_t numObservables, numRanks, numSeeds, bucketBits;
   uint32_t bucketBase, numBuckets, maskWords, layerStride;
   uint32_t sweepThreads, sweepRpt, pad0, pad1;
-  uint64_t actOff, seedOff, colOff, rankOff, sOff, maskOff;
+  uint64_t actOff, seedOff, colOff, rankOff, sOff;
   uint64_t colStride;  // words per S column: numRanks + kSinkSlots sink words
 };
 
 // Counter block zeroed at the start of every run: [flags | bucket counts].
-constexpr uint32_t kFlagWords = 16;  // [0] hash collision, [1] target overflow, [2] oversized buckets
+// Per-run flag words, zeroed by the sweep kernel: [0] hash collision,
+// [1] target overflow, [3] overflow-list length, [5] oversized-bucket arena fill
+constexpr uint32_t kFlagWords = 16;
 
 // --------------------------------------------------------------------------
 // wave64 helpers
@@ -146,7 +148,7 @@ struct RunOut {
   const double *prob;      // rank -> node probability (fallback lookups)
   const uint64_t *h2;      // rank -> second hash
   const uint32_t *nnz;     // rank -> class size
-  uint32_t *sortedRank;    // bucket-sorted ranks (in place over the partition)
+  uint32_t *sortedRank;    // bucket-sorted ranks (in place over the bucket slots)
   uint32_t *runRank;       // per run: representative (smallest) rank
   double *runP;            // per run: merged probability
   uint64_t *runHash;       // per run: FNV hash
@@ -261,15 +263,6 @@ __device__ __forceinline__ void scanRuns(const uint64_t (&h)[R], const uint32_t 
 // kernels
 // --------------------------------------------------------------------------
 
-// Zeroes the per-run counters and the per-layer column masks.
-__global__ void zeroKernel(uint32_t *__restrict__ a, uint64_t na, uint32_t *__restrict__ b, uint64_t nb) {
-  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < na + nb; i += stride) {
-    if (i < na) a[i] = 0u;
-    else b[i - na] = 0u;
-  }
-}
-
 // LDS-only workgroup barrier: orders LDS accesses but lets global loads
 // (the prefetched records of the next layers) stay in flight.
 __device__ __forceinline__ void ldsBarrier() {
@@ -278,10 +271,6 @@ __device__ __forceinline__ void ldsBarrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-// Loads this thread's records of one layer.  The loads are unconditional
-// (out-of-range slots re-read the layer's last record, or record 0, and are
-// ignored by the consumer), so the compiler can leave them in flight instead
-// of waiting on a select.
 // The sweep addresses LDS by raw byte address (its kernel has no static LDS,
 // so its dynamic LDS starts at address 0): the addresses stored in the
 // records feed the LDS instructions with no base added.
@@ -309,11 +298,11 @@ __device__ __forceinline__ void loadLayer(uint4 (&r)[RPT], __amdgpu_buffer_rsrc_
 // (sweepRpt per thread) followed by one LDS-only barrier.  A record carries
 // the LDS byte addresses of its children and seed and the byte offset of
 // its S entry, so a record costs two (three if seeded) LDS reads, two XORs,
-// one LDS write and one global store.  The loop runs trips of three layers,
-// unrolled, so each position in a trip has a fixed ring slot: the ring
-// addresses a thread writes are per-column constants, and the record
-// prefetch (three layers ahead, into three rotating register sets) advances
-// one scalar offset.  Per-layer scalar work is what bounds a layer here
+// one LDS write and one global store.  The loop runs trips of D layers (D a
+// multiple of 3; see sweepTrip), unrolled, so each position in a trip has a
+// fixed ring slot: the ring addresses a thread writes are per-column
+// constants, and the record prefetch (D layers ahead, into D rotating
+// register sets) advances one scalar offset.  Per-layer scalar work bounds a layer here
 // (one scalar instruction per SIMD per four cycles, shared by its waves), so
 // the early-termination vote runs once per trip: a trip whose sensitivities
 // are all zero, at or below the column's lowest seed, ends the band (the ring
@@ -323,9 +312,18 @@ __device__ __forceinline__ void loadLayer(uint4 (&r)[RPT], __amdgpu_buffer_rsrc_
 // and clears it after its first barrier.  Waves beyond the circuit's
 // sweepThreads (a batch launches with its widest circuit's shape) only take
 // part in the barriers and votes.  On exit the column's band [lo, hi] is
-// recorded and the column is added to the per-layer column masks of those
-// layers.
-constexpr int kSweepDepth = 3;  // prefetch distance = layers per trip
+// recorded ({lo, hi}; {INT_MAX, -1} for a column without seeds).  Block 0
+// also zeroes the run's flag words (read and written only by the later
+// stages); the grid has at least one block.
+// Layers per trip = prefetch distance (a multiple of the ring's three
+// slots, so each trip position keeps its slot): six while the register sets
+// fit (up to two records per thread), else three.
+#ifndef GPA_SWEEP_TRIP
+#define GPA_SWEEP_TRIP 6
+#endif
+__host__ __device__ constexpr int sweepTrip(int rpt) { return rpt <= 2 ? GPA_SWEEP_TRIP : 3; }
+static_assert(GPA_SWEEP_TRIP % 3 == 0 && 2 * GPA_SWEEP_TRIP - 1 <= static_cast<int>(kSweepLeadPad),
+              "trip length: a multiple of 3 whose prefetches stay within the padding blocks");
 
 template <int RPT>
 struct SweepState {
@@ -338,12 +336,27 @@ struct SweepState {
 
 __device__ __forceinline__ uint32_t sweepSlot(int32_t l) { return static_cast<uint32_t>(l + 6) % 3; }  // l >= -6
 
-// One layer: consume `r` (this thread's records of the layer), write the
-// values to the ring at `dst` and to S, OR them into `any`, then refill `r`
-// with the block kSweepDepth layers below (`loadOff`, advanced here).
-template <int RPT, bool GSEEDS>
+// Ring byte addresses of a thread's records at one trip position: kept in
+// registers for up to two records per thread, else formed per layer from
+// the position's slot base (fewer registers).
+template <int RPT, int D>
+struct SweepDst {
+  static constexpr bool kStored = RPT <= 2;
+  uint32_t addr[kStored ? D : 1][RPT];
+  uint32_t slotB[D];
+  uint32_t pos4[RPT];
+  __device__ __forceinline__ uint32_t at(int u, int k) const {
+    if constexpr (kStored) return addr[u][k];
+    else return slotB[u] + pos4[k];
+  }
+};
+
+// One layer (trip position u): consume `r` (this thread's records of the
+// layer), write the values to the ring and to S, OR them into `any`, then
+// refill `r` with the block one trip below (`loadOff`, advanced here).
+template <int RPT, bool GSEEDS, int D>
 __device__ __forceinline__ void sweepRecords(const SweepState<RPT> &st, uint4 (&r)[RPT],
-                                             const uint32_t (&dst)[RPT], uint32_t &loadOff,
+                                             const SweepDst<RPT, D> &dst, int u, uint32_t &loadOff,
                                              uint32_t &any, char *__restrict__ Sc) {
   // all of the wave's child reads first, so the records' round trips
   // overlap; seeds are read only by waves holding a seeded record (seeded
@@ -369,7 +382,7 @@ __device__ __forceinline__ void sweepRecords(const SweepState<RPT> &st, uint4 (&
   }
 #pragma unroll
   for (int k = 0; k < RPT; ++k) {
-    *ldsAt(dst[k]) = v[k];
+    *ldsAt(dst.at(u, k)) = v[k];
     *reinterpret_cast<uint32_t *>(Sc + r[k].w) = v[k];
     any |= v[k];
   }
@@ -381,17 +394,19 @@ template <int RPT, bool GSEEDS>
 __global__ __launch_bounds__(kSweepMaxThreads) void sweepKernel(
     const VDesc *__restrict__ desc, const uint2 *__restrict__ items, const uint4 *__restrict__ act,
     const uint32_t *__restrict__ seedVals, const int32_t *__restrict__ colHi,
-    const int32_t *__restrict__ colLow, int32_t *__restrict__ bandLo, uint32_t *__restrict__ S,
-    uint32_t *__restrict__ masks) {
+    const int32_t *__restrict__ colLow, int2 *__restrict__ band, uint32_t *__restrict__ S,
+    uint32_t numItems, uint32_t *__restrict__ flags) {
   // dynamic LDS only, addressed from 0:
   // ring[3P] | zero word | seeds (unless GSEEDS) | vote[2]
   const uint32_t tid = threadIdx.x;
+  if (blockIdx.x == 0 && tid < kFlagWords) flags[tid] = 0u;
+  if (blockIdx.x >= numItems) return;
   const uint2 it = items[blockIdx.x];
   const VDesc d = desc[it.x];
   const uint32_t c = it.y;
   const int32_t hi = __builtin_amdgcn_readfirstlane(colHi[d.colOff + c]);
   if (hi < 0) {
-    if (tid == 0) bandLo[d.colOff + c] = 0x7FFFFFFF;
+    if (tid == 0) band[d.colOff + c] = make_int2(0x7FFFFFFF, -1);
     return;
   }
   const int32_t lowSeed = __builtin_amdgcn_readfirstlane(colLow[d.colOff + c]);
@@ -406,7 +421,7 @@ __global__ __launch_bounds__(kSweepMaxThreads) void sweepKernel(
   if (tid < 2) *ldsAt(voteB + 4 * tid) = 0u;
   __syncthreads();
 
-  // Both loops below run three-layer trips with the vote steps at the same
+  // Both loops below run D-layer trips with the vote steps at the same
   // points and decide on leaving at the end of a trip, from the same vote
   // words, so all waves meet the same number of barriers.  A single exit
   // per trip keeps the prefetches in flight (an exit inside would make the
@@ -427,7 +442,10 @@ __global__ __launch_bounds__(kSweepMaxThreads) void sweepKernel(
     st.seedB = 12 * P + 4;
     char *Sc = reinterpret_cast<char *>(S + d.sOff + static_cast<uint64_t>(c) * d.colStride);
     // ring addresses of this thread's records at each trip position
-    uint32_t dst[kSweepDepth][RPT];
+    constexpr int D = sweepTrip(RPT);
+    SweepDst<RPT, D> dst;
+#pragma unroll
+    for (int u = 0; u < D; ++u) dst.slotB[u] = sweepSlot(hi - u) * P * 4;
 #pragma unroll
     for (int k = 0; k < RPT; ++k) {
       // circuits with fewer records per thread repeat their last record
@@ -435,10 +453,13 @@ __global__ __launch_bounds__(kSweepMaxThreads) void sweepKernel(
       const uint32_t j = static_cast<uint32_t>(k) < d.sweepRpt ? static_cast<uint32_t>(k) : d.sweepRpt - 1;
       const uint32_t pos = j * T + tid;
       st.off[k] = pos * 16;
+      dst.pos4[k] = pos * 4;
+      if constexpr (SweepDst<RPT, D>::kStored) {
 #pragma unroll
-      for (int u = 0; u < kSweepDepth; ++u) dst[u][k] = (sweepSlot(hi - u) * P + pos) * 4;
+        for (int u = 0; u < D; ++u) dst.addr[u][k] = dst.slotB[u] + pos * 4;
+      }
     }
-    // register sets for layers hi, hi-1, hi-2 (block kSweepLeadPad + layer).
+    // register sets for layers hi, hi-1, ..., hi-D+1 (block kSweepLeadPad + layer).
     // Each set but the last is followed by RPT stores to sink words, as each
     // refill is followed by a layer's RPT stores in the loop: the compiler's
     // count of memory operations outstanding at the loop head is then the
@@ -448,13 +469,13 @@ __global__ __launch_bounds__(kSweepMaxThreads) void sweepKernel(
     // words apart (mod kSinkSlots; at most 16 stores, so a lane's addresses
     // differ).
     uint32_t *sink = reinterpret_cast<uint32_t *>(Sc) + d.numRanks;
-    uint4 rs[kSweepDepth][RPT];
+    uint4 rs[D][RPT];
     uint32_t loadOff = static_cast<uint32_t>(hi + static_cast<int32_t>(kSweepLeadPad)) * st.layerBytes;
 #pragma unroll
-    for (int u = 0; u < kSweepDepth; ++u) {
+    for (int u = 0; u < D; ++u) {
       loadLayer<RPT>(rs[u], st.ac, loadOff, st.off);
       __builtin_amdgcn_sched_barrier(0);
-      if (u + 1 < kSweepDepth) {
+      if (u + 1 < D) {
         loadOff -= st.layerBytes;
 #pragma unroll
         for (int k = 0; k < RPT; ++k) sink[(tid + 32 * (u * RPT + k)) & (kSinkSlots - 1)] = 0u;
@@ -465,17 +486,18 @@ __global__ __launch_bounds__(kSweepMaxThreads) void sweepKernel(
       uint32_t any = 0;
       const uint32_t myWord = voteB + 4 * (trip & 1), prevWord = voteB + 4 * ((trip + 1) & 1);
       // first layer: also read the previous trip's vote
-      sweepRecords<RPT, GSEEDS>(st, rs[0], dst[0], loadOff, any, Sc);
+      sweepRecords<RPT, GSEEDS, D>(st, rs[0], dst, 0, loadOff, any, Sc);
       const uint32_t prev = trip ? *ldsAt(prevWord) : 1u;
       if (prev == 0u && l + 1 <= lowSeed && lo < 0) lo = l + 1;
       ldsBarrier();
       if (tid == 0) *ldsAt(prevWord) = 0u;
-      sweepRecords<RPT, GSEEDS>(st, rs[1], dst[1], loadOff, any, Sc);
-      ldsBarrier();
-      sweepRecords<RPT, GSEEDS>(st, rs[2], dst[2], loadOff, any, Sc);
-      if (__ballot(any != 0) != 0ull && (tid & 63) == 0) *ldsAt(myWord) = 1u;
-      ldsBarrier();
-      l -= kSweepDepth;
+#pragma unroll
+      for (int u = 1; u < D; ++u) {
+        sweepRecords<RPT, GSEEDS, D>(st, rs[u], dst, u, loadOff, any, Sc);
+        if (u == D - 1 && __ballot(any != 0) != 0ull && (tid & 63) == 0) *ldsAt(myWord) = 1u;
+        ldsBarrier();
+      }
+      l -= D;
       ++trip;
       if (lo >= 0 || l < 0) break;
     }
@@ -484,89 +506,191 @@ __global__ __launch_bounds__(kSweepMaxThreads) void sweepKernel(
       const uint32_t prevWord = voteB + 4 * ((trip + 1) & 1);
       const uint32_t prev = trip ? *ldsAt(prevWord) : 1u;
       if (prev == 0u && l + 1 <= lowSeed && lo < 0) lo = l + 1;
-      ldsBarrier();
-      ldsBarrier();
-      ldsBarrier();
-      l -= kSweepDepth;
+      constexpr int D = sweepTrip(RPT);
+#pragma unroll
+      for (int u = 0; u < D; ++u) ldsBarrier();
+      l -= D;
       ++trip;
       if (lo >= 0 || l < 0) break;
     }
   }
   if (lo < 0) lo = 0;
-  if (tid == 0) bandLo[d.colOff + c] = lo;
-  // add the column to the masks of the layers in its band
-  uint32_t *mk = masks + d.maskOff + (c >> 5);
-  const uint32_t bit = 1u << (c & 31);
-  for (int32_t k = lo + static_cast<int32_t>(tid); k <= hi; k += static_cast<int32_t>(blockDim.x))
-    atomicOr(&mk[static_cast<uint64_t>(k) * d.maskWords], bit);
+  if (tid == 0) band[d.colOff + c] = make_int2(lo, hi);
+}
+
+// Column masks of the layers a workgroup's nodes lie in.  Bit c of word w
+// of a layer's mask is set when column 32w + c's band {lo, hi} covers the
+// layer; a node's targets live only in those columns.  The table covers
+// layers [l0, l0 + numLayers) in LDS (kMaskTableWords words); without it a
+// word is computed from the bands directly (band loads issued eight at a
+// time).
+constexpr uint32_t kMaskTableWords = 1024;
+
+struct LayerMasks {
+  const uint32_t *tab;  // LDS table, or nullptr
+  const int2 *band;     // the circuit's columns
+  uint32_t l0, numCols, maskWords;
+
+  __device__ __forceinline__ uint32_t word(uint32_t l, uint32_t w) const {
+    if (tab) return tab[(l - l0) * maskWords + w];
+    const int32_t li = static_cast<int32_t>(l);
+    uint32_t m = 0;
+    for (uint32_t k0 = 0; k0 < 32; k0 += 8) {
+      int2 bd[8];
+#pragma unroll
+      for (uint32_t k = 0; k < 8; ++k) {
+        const uint32_t c = w * 32 + k0 + k;
+        bd[k] = band[c < numCols ? c : 0u];
+      }
+#pragma unroll
+      for (uint32_t k = 0; k < 8; ++k)
+        m |= (w * 32 + k0 + k < numCols && bd[k].x <= li && li <= bd[k].y) ? 1u << (k0 + k) : 0u;
+    }
+    return m;
+  }
+};
+
+// Fills the LDS mask table for layers [l0, l0 + nl) (the whole workgroup
+// calls this; it ends with a barrier).
+__device__ __forceinline__ void buildMaskTable(uint32_t *tab, uint32_t l0, uint32_t nl, uint32_t numCols,
+                                               uint32_t maskWords, const int2 *__restrict__ band) {
+  for (uint32_t i = threadIdx.x; i < nl * maskWords; i += blockDim.x) tab[i] = 0u;
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < nl * numCols; i += blockDim.x) {
+    const uint32_t li = i / numCols, c = i % numCols;
+    const int32_t l = static_cast<int32_t>(l0 + li);
+    const int2 bd = band[c];
+    if (bd.x <= l && l <= bd.y) atomicOr(&tab[li * maskWords + c / 32], 1u << (c % 32));
+  }
+  __syncthreads();
+}
+
+// Calls f(target) for every set bit of one node's sensitivity row, in
+// ascending target order: the row's words are those of the columns set in
+// the node's layer mask, at stride colStride from Sv.  Columns are taken
+// eight at a time and their eight words loaded before any is used (absent
+// columns load column 0's word, which is always valid), so a node waits for
+// one memory round trip per eight columns, not per column.
+template <typename F>
+__device__ __forceinline__ void forEachTarget(const LayerMasks &lm, uint32_t l,
+                                              const uint32_t *__restrict__ Sv, uint64_t colStride, F &&f) {
+  constexpr int kBatch = 8;
+  uint32_t w = 0;
+  uint32_t m = lm.maskWords ? lm.word(l, 0) : 0u;
+  for (;;) {
+    uint32_t col[kBatch], x[kBatch];
+#pragma unroll
+    for (int j = 0; j < kBatch; ++j) {
+      while (m == 0u && w + 1 < lm.maskWords) m = lm.word(l, ++w);
+      col[j] = m ? w * 32 + static_cast<uint32_t>(__builtin_ctz(m)) : NONE;
+      m &= m - 1;
+    }
+#pragma unroll
+    for (int j = 0; j < kBatch; ++j) x[j] = Sv[static_cast<uint64_t>(col[j] != NONE ? col[j] : 0u) * colStride];
+#pragma unroll
+    for (int j = 0; j < kBatch; ++j) {
+      if (col[j] == NONE) continue;
+      uint32_t v = x[j];
+      while (v) {
+        f(col[j] * 32 + static_cast<uint32_t>(__builtin_ctz(v)));
+        v &= v - 1;
+      }
+    }
+    if (col[kBatch - 1] == NONE) break;
+  }
 }
 
 // Class extraction + hashing of every p>0 node: the ascending list of set
-// bits (visiting only the columns whose band covers the node's layer, from
-// the layer's column mask), FNV-1a-64 over it padded with 0xFFFFFFFF to 32
-// slots (lists longer than 32 hash all entries, unpadded), a second hash,
-// the class size, and the node's position within its bucket.
+// bits, FNV-1a-64 over it padded with 0xFFFFFFFF to 32 slots (lists longer
+// than 32 hash all entries, unpadded), a second hash and the class size.
+// The node then takes a slot of its sort bucket (first radix digit = top
+// bucketBits hash bits): slot pos of bucket b is pkey/prank[b * kBucketCap
+// + pos], pos from an atomic counter per bucket; nodes beyond a bucket's
+// kBucketCap slots go to the overflow list.  The first kStashSlots targets
+// of every node are also stored as one row per node (`stash`), so the emit
+// stage copies a class's targets from its representative's row instead of
+// gathering them from S one column at a time: each wave assembles its 64
+// rows in LDS and writes them out contiguously.
+constexpr uint32_t kStashSlots = 16;
+constexpr uint32_t kBucketCap = kBucketItemsPerWave;
+
+struct ClassifyOut {
+  uint64_t *hash, *h2;
+  uint32_t *nnz;
+  uint32_t *bucketCount;  // bucket b's counter at b * countStride
+  uint32_t countStride;
+  uint64_t *pkey;         // bucket slots
+  uint32_t *prank;
+  uint64_t *ovKey;        // overflow list (count in flags[3])
+  uint32_t *ovRank, *ovBucket;
+  uint32_t *flags;
+  uint32_t *stash;
+};
+
 __global__ __launch_bounds__(kClassBlock) void classifyKernel(
-    const VDesc *__restrict__ desc, const uint2 *__restrict__ tiles,
-    const uint32_t *__restrict__ masks, const uint32_t *__restrict__ rankLayer,
-    const uint32_t *__restrict__ S, uint64_t *__restrict__ hashOut, uint64_t *__restrict__ h2Out,
-    uint32_t *__restrict__ nnzOut, uint32_t *__restrict__ bucketCount,
-    uint32_t *__restrict__ bucketPos) {
+    const VDesc *__restrict__ desc, const uint2 *__restrict__ tiles, const uint32_t *__restrict__ rankLayer,
+    const int2 *__restrict__ band, const uint32_t *__restrict__ S,
+    ClassifyOut o) {
+  // per wave: 64 rows of kStashSlots words, padded to kStashSlots + 1 so
+  // that a lane's writes and the row-major read-out avoid bank conflicts
+  __shared__ uint32_t rows[kClassBlock / 64][64 * (kStashSlots + 1)];
+  __shared__ uint32_t maskTab[kMaskTableWords];
   const uint2 t = tiles[blockIdx.x];
   const VDesc d = desc[t.x];
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t i = t.y + threadIdx.x;
-  if (i >= d.numRanks) return;
-  const uint64_t gi = d.rankOff + i;
-  const uint32_t l = rankLayer[gi];
-  const uint32_t *mk = masks + d.maskOff + static_cast<uint64_t>(l) * d.maskWords;
-  const uint32_t *Sv = S + d.sOff + i;
-  uint64_t h = kFnvBasis, h2 = 0x84222325cbf29ce4ull;
-  uint32_t cnt = 0;
-  for (uint32_t w = 0; w < d.maskWords; ++w) {
-    uint32_t m = mk[w];
-    while (m) {
-      const uint32_t c = w * 32 + static_cast<uint32_t>(__builtin_ctz(m));
-      m &= m - 1;
-      uint32_t x = Sv[static_cast<uint64_t>(c) * d.colStride];
-      while (x) {
-        const uint32_t tg = c * 32 + static_cast<uint32_t>(__builtin_ctz(x));
-        h = (h ^ tg) * kFnvPrime;
-        h2 = mix2(h2, tg);
-        cnt++;
-        x &= x - 1;
+  const uint32_t last = (t.y + kClassBlock < d.numRanks ? t.y + kClassBlock : d.numRanks) - 1;
+  const uint32_t l0 = rankLayer[d.rankOff + t.y], l1 = rankLayer[d.rankOff + last];
+  LayerMasks lm{nullptr, band + d.colOff, l0, d.numCols, d.maskWords};
+  if ((l1 - l0 + 1) * d.maskWords <= kMaskTableWords) {
+    buildMaskTable(maskTab, l0, l1 - l0 + 1, d.numCols, d.maskWords, lm.band);
+    lm.tab = maskTab;
+  }
+  uint32_t *row = rows[wave] + lane * (kStashSlots + 1);
+  if (i < d.numRanks) {
+    const uint64_t gi = d.rankOff + i;
+    const uint32_t *Sv = S + d.sOff + i;
+    uint64_t h = kFnvBasis, h2 = 0x84222325cbf29ce4ull;
+    uint32_t cnt = 0;
+    forEachTarget(lm, rankLayer[gi], Sv, d.colStride, [&](uint32_t tg) {
+      h = (h ^ tg) * kFnvPrime;
+      h2 = mix2(h2, tg);
+      if (cnt < kStashSlots) row[cnt] = tg;
+      cnt++;
+    });
+    if (cnt && cnt <= 32)
+      for (uint32_t k = cnt; k < 32; ++k) h = (h ^ 0xFFFFFFFFull) * kFnvPrime;
+    h2 = mix2(h2, cnt);
+    o.hash[gi] = h;
+    o.h2[gi] = h2;
+    o.nnz[gi] = cnt;
+    if (cnt) {
+      const uint32_t b = d.bucketBase + (d.bucketBits ? static_cast<uint32_t>(h >> (64 - d.bucketBits)) : 0u);
+      const uint32_t pos = atomicAdd(&o.bucketCount[static_cast<uint64_t>(b) * o.countStride], 1u);
+      if (pos < kBucketCap) {
+        o.pkey[static_cast<uint64_t>(b) * kBucketCap + pos] = h;
+        o.prank[static_cast<uint64_t>(b) * kBucketCap + pos] = static_cast<uint32_t>(gi);
+      } else {
+        const uint32_t k = atomicAdd(&o.flags[3], 1u);
+        o.ovKey[k] = h;
+        o.ovRank[k] = static_cast<uint32_t>(gi);
+        o.ovBucket[k] = b;
       }
     }
   }
-  if (cnt && cnt <= 32)
-    for (uint32_t k = cnt; k < 32; ++k) h = (h ^ 0xFFFFFFFFull) * kFnvPrime;
-  hashOut[gi] = h;
-  h2Out[gi] = mix2(h2, cnt);
-  nnzOut[gi] = cnt;
-  if (cnt) {
-    const uint32_t b = d.bucketBase + (d.bucketBits ? static_cast<uint32_t>(h >> (64 - d.bucketBits)) : 0u);
-    bucketPos[gi] = atomicAdd(&bucketCount[b], 1u);
+  // the wave's rows, row-major and contiguous from its first rank (a wave's
+  // LDS accesses are in order, so no barrier is needed)
+  const uint32_t waveFirst = t.y + wave * 64;
+  if (waveFirst < d.numRanks) {
+    const uint32_t nRows = d.numRanks - waveFirst < 64 ? d.numRanks - waveFirst : 64;
+    uint32_t *dst = o.stash + (d.rankOff + waveFirst) * kStashSlots;
+#pragma unroll
+    for (uint32_t k = 0; k < kStashSlots; ++k) {
+      const uint32_t e = k * 64 + lane;  // element of the 64 x kStashSlots block
+      const uint32_t r = e / kStashSlots, c = e % kStashSlots;
+      if (r < nRows) dst[e] = rows[wave][r * (kStashSlots + 1) + c];
+    }
   }
-}
-
-// MSD partition: scatter (hash, rank) into the buckets (positions fixed by
-// the classify pass; order inside a bucket is arbitrary -- the bucket sort
-// orders by (hash, rank)).
-__global__ __launch_bounds__(kClassBlock) void partitionKernel(
-    const VDesc *__restrict__ desc, const uint2 *__restrict__ tiles,
-    const uint64_t *__restrict__ hash, const uint32_t *__restrict__ nnz,
-    const uint32_t *__restrict__ bucketStart, const uint32_t *__restrict__ bucketPos,
-    uint64_t *__restrict__ pkey, uint32_t *__restrict__ prank) {
-  const uint2 t = tiles[blockIdx.x];
-  const VDesc d = desc[t.x];
-  const uint32_t i = t.y + threadIdx.x;
-  if (i >= d.numRanks) return;
-  const uint64_t gi = d.rankOff + i;
-  if (!nnz[gi]) return;
-  const uint64_t h = hash[gi];
-  const uint32_t b = d.bucketBase + (d.bucketBits ? static_cast<uint32_t>(h >> (64 - d.bucketBits)) : 0u);
-  const uint32_t pos = bucketStart[b] + bucketPos[gi];
-  pkey[pos] = h;
-  prank[pos] = static_cast<uint32_t>(gi);
 }
 
 template <int R>
@@ -590,281 +714,347 @@ __device__ void bucketWave(uint32_t b, uint32_t bs, uint32_t cnt, uint32_t lane,
   }
 }
 
-// One wave64 per bucket: sort in registers, merge runs.  Buckets above the
-// wave capacity are queued for bucketFallbackKernel.
-__global__ __launch_bounds__(256) void bucketKernel(
-    uint32_t numBuckets, const uint32_t *__restrict__ bucketStart, const uint64_t *__restrict__ pkey,
-    RunOut o, uint32_t *__restrict__ runCount, uint32_t *__restrict__ runNnz,
-    uint32_t *__restrict__ overflowList) {
-  const uint32_t lane = threadIdx.x & 63;
-  const uint32_t b = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
-  if (b >= numBuckets) return;
-  const uint32_t bs = bucketStart[b];
-  const uint32_t cnt = bucketStart[b + 1] - bs;
-  if (cnt == 0) {
-    if (lane == 0) {
-      runCount[b] = 0;
-      runNnz[b] = 0;
-    }
-  } else if (cnt <= 64) {
-    bucketWave<1>(b, bs, cnt, lane, pkey, o, runCount, runNnz);
-  } else if (cnt <= 128) {
-    bucketWave<2>(b, bs, cnt, lane, pkey, o, runCount, runNnz);
-  } else if (cnt <= kBucketItemsPerWave) {
-    bucketWave<4>(b, bs, cnt, lane, pkey, o, runCount, runNnz);
-  } else if (lane == 0) {
-    const uint32_t idx = atomicAdd(&o.flags[2], 1u);
-    overflowList[idx] = b;
-  }
-}
+struct BucketIo {
+  uint32_t *bucketCount;  // read and reset to zero for the next run
+  uint32_t countStride;
+  uint64_t *pkey;         // slots (sorted in place)
+  const uint64_t *ovKey;  // overflow list (count in flags[3])
+  const uint32_t *ovRank, *ovBucket;
+  uint32_t *runCount, *runNnz, *itemCount;  // per bucket
+  uint32_t *slotBase;     // per bucket: where its sorted items and runs start
+  uint64_t *scratchKey;   // sort workspace for oversized buckets (2 * arenaSize)
+  uint32_t *scratchRank;
+  uint32_t arenaBase;     // numBuckets * kBucketCap: oversized buckets' items and runs follow
+};
 
-// Oversized buckets: workgroup bitonic sort in global scratch, then one
-// wave merges the runs chunk by chunk.
-__global__ __launch_bounds__(256) void bucketFallbackKernel(
-    const uint32_t *__restrict__ bucketStart, const uint64_t *__restrict__ pkey, RunOut o,
-    uint32_t *__restrict__ runCount, uint32_t *__restrict__ runNnz,
-    const uint32_t *__restrict__ overflowList, uint64_t *__restrict__ scratchKey,
-    uint32_t *__restrict__ scratchRank) {
-  const uint32_t numOver = o.flags[2];
+// An oversized bucket (more than kBucketCap items), by the whole workgroup:
+// its items (slots, then its entries of the overflow list) are copied to a
+// workspace, bitonic-sorted there, and merged chunk by chunk by wave 0; the
+// sorted items and runs go to an arena range taken with one atomic.
+__device__ void bigBucket(uint32_t b, uint32_t cnt, const BucketIo &io, const RunOut &o) {
+  __shared__ uint32_t shOff, shFill;
   const uint32_t lane = threadIdx.x & 63;
-  for (uint32_t idx = blockIdx.x; idx < numOver; idx += gridDim.x) {
-    const uint32_t b = overflowList[idx];
-    const uint32_t bs = bucketStart[b];
-    const uint32_t cnt = bucketStart[b + 1] - bs;
-    uint32_t P = 1;
-    while (P < cnt) P <<= 1;
-    uint64_t *K = scratchKey + 2ull * bs;
-    uint32_t *Rk = scratchRank + 2ull * bs;
-    for (uint32_t i = threadIdx.x; i < P; i += blockDim.x) {
-      K[i] = i < cnt ? pkey[bs + i] : ~0ull;
-      Rk[i] = i < cnt ? o.sortedRank[bs + i] : NONE;
+  if (threadIdx.x == 0) {
+    shOff = atomicAdd(&o.flags[5], cnt);  // total items never exceed the arena
+    shFill = kBucketCap;
+  }
+  __syncthreads();
+  const uint32_t off = shOff;
+  uint32_t P = 1;
+  while (P < cnt) P <<= 1;
+  uint64_t *K = io.scratchKey + 2ull * off;  // P < 2 * cnt
+  uint32_t *Rk = io.scratchRank + 2ull * off;
+  const uint64_t slot0 = static_cast<uint64_t>(b) * kBucketCap;
+  for (uint32_t i = threadIdx.x; i < kBucketCap; i += blockDim.x) {
+    K[i] = io.pkey[slot0 + i];
+    Rk[i] = o.sortedRank[slot0 + i];
+  }
+  const uint32_t nov = o.flags[3];
+  for (uint32_t k = threadIdx.x; k < nov; k += blockDim.x)
+    if (io.ovBucket[k] == b) {
+      const uint32_t at = atomicAdd(&shFill, 1u);
+      K[at] = io.ovKey[k];
+      Rk[at] = io.ovRank[k];
     }
-    __syncthreads();
-    for (uint32_t k = 2; k <= P; k <<= 1)
-      for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-        for (uint32_t i = threadIdx.x; i < P; i += blockDim.x) {
-          const uint32_t ixj = i ^ j;
-          if (ixj > i) {
-            const bool asc = (i & k) == 0;
-            const uint64_t ka = K[i], kb = K[ixj];
-            const uint32_t ra = Rk[i], rb = Rk[ixj];
-            const bool sw = asc ? keyLess(kb, rb, ka, ra) : keyLess(ka, ra, kb, rb);
-            if (sw) {
-              K[i] = kb;
-              K[ixj] = ka;
-              Rk[i] = rb;
-              Rk[ixj] = ra;
-            }
+  for (uint32_t i = cnt + threadIdx.x; i < P; i += blockDim.x) {
+    K[i] = ~0ull;
+    Rk[i] = NONE;
+  }
+  __syncthreads();
+  for (uint32_t k = 2; k <= P; k <<= 1)
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      for (uint32_t i = threadIdx.x; i < P; i += blockDim.x) {
+        const uint32_t ixj = i ^ j;
+        if (ixj > i) {
+          const bool asc = (i & k) == 0;
+          const uint64_t ka = K[i], kb = K[ixj];
+          const uint32_t ra = Rk[i], rb = Rk[ixj];
+          const bool sw = asc ? keyLess(kb, rb, ka, ra) : keyLess(ka, ra, kb, rb);
+          if (sw) {
+            K[i] = kb;
+            K[ixj] = ka;
+            Rk[i] = rb;
+            Rk[ixj] = ra;
           }
         }
-        __syncthreads();
       }
-    if (threadIdx.x < 64) {
-      RunCarry cy{0.0, 0ull, 0ull, 0u, 0u, 0u, 0u};
-      for (uint32_t c0 = 0; c0 < cnt; c0 += 64) {
-        uint64_t h[1];
-        uint32_t r[1], k[1];
-        const uint32_t e = c0 + lane;
-        h[0] = e < cnt ? K[e] : ~0ull;
-        r[0] = e < cnt ? Rk[e] : NONE;
-        k[0] = r[0];
-        const uint32_t n = cnt - c0 < 64 ? cnt - c0 : 64;
-        scanRuns<1>(h, r, k, n, c0, bs, cy, lane, o, RunLookup{o.prob, o.h2, o.nnz});
-      }
-      if (lane == 0) {
-        runCount[b] = cy.runs;
-        runNnz[b] = cy.nnzSum;
-      }
+      __syncthreads();
     }
-    __syncthreads();
+  const uint32_t base = io.arenaBase + off;
+  if (threadIdx.x < 64) {
+    RunCarry cy{0.0, 0ull, 0ull, 0u, 0u, 0u, 0u};
+    for (uint32_t c0 = 0; c0 < cnt; c0 += 64) {
+      uint64_t h[1];
+      uint32_t r[1];
+      const uint32_t e = c0 + lane;
+      h[0] = e < cnt ? K[e] : ~0ull;
+      r[0] = e < cnt ? Rk[e] : NONE;
+      const uint32_t n = cnt - c0 < 64 ? cnt - c0 : 64;
+      scanRuns<1>(h, r, r, n, c0, base, cy, lane, o, RunLookup{o.prob, o.h2, o.nnz});
+    }
+    if (lane == 0) {
+      io.runCount[b] = cy.runs;
+      io.runNnz[b] = cy.nnzSum;
+      io.slotBase[b] = base;
+    }
   }
+  __syncthreads();
 }
 
-// Compact output: one wave per bucket, one lane per run.  Block 0 also
-// records where each circuit's errors and targets start.
-__global__ __launch_bounds__(256) void emitKernel(
-    const VDesc *__restrict__ desc, uint32_t numVariants, uint32_t numBuckets,
-    const uint32_t *__restrict__ bucketStart, const uint32_t *__restrict__ runCount,
-    const uint32_t *__restrict__ classStart, const uint32_t *__restrict__ tgtStart,
-    const uint32_t *__restrict__ runRank, const double *__restrict__ runP,
-    const uint64_t *__restrict__ runHash, const uint32_t *__restrict__ runTgtPre,
-    const uint32_t *__restrict__ runMember, const uint32_t *__restrict__ rankLayer,
-    const uint32_t *__restrict__ masks, const uint32_t *__restrict__ S, double *__restrict__ outP,
-    uint32_t *__restrict__ outOff, uint64_t *__restrict__ outHash, uint32_t *__restrict__ outMember,
-    uint32_t *__restrict__ outTargets, uint64_t capTargets, uint32_t *__restrict__ flags,
-    uint32_t *__restrict__ vBounds) {
+// One wave64 per bucket: sort in registers, merge runs, reading the
+// bucket's slots.  Buckets above kBucketCap items are then taken by their
+// workgroup (bigBucket).  Each bucket's counter is reset for the next run.
+__global__ __launch_bounds__(256) void bucketKernel(uint32_t numBuckets, BucketIo io, RunOut o) {
+  __shared__ uint32_t bigB[4], bigCnt[4], nBig;
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t b = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+  if (threadIdx.x == 0) nBig = 0;
+  __syncthreads();
+  if (b < numBuckets) {
+    uint32_t *ctr = io.bucketCount + static_cast<uint64_t>(b) * io.countStride;
+    const uint32_t cnt = __builtin_amdgcn_readfirstlane(*ctr);
+    const uint32_t bs = b * kBucketCap;
+    if (lane == 0) {
+      *ctr = 0u;
+      io.itemCount[b] = cnt;
+    }
+    if (cnt <= kBucketCap) {
+      if (lane == 0) io.slotBase[b] = bs;
+      if (cnt == 0) {
+        if (lane == 0) {
+          io.runCount[b] = 0;
+          io.runNnz[b] = 0;
+        }
+      } else if (cnt <= 64) {
+        bucketWave<1>(b, bs, cnt, lane, io.pkey, o, io.runCount, io.runNnz);
+      } else if (cnt <= 128) {
+        bucketWave<2>(b, bs, cnt, lane, io.pkey, o, io.runCount, io.runNnz);
+      } else {
+        bucketWave<4>(b, bs, cnt, lane, io.pkey, o, io.runCount, io.runNnz);
+      }
+    } else if (lane == 0) {
+      const uint32_t k = atomicAdd(&nBig, 1u);
+      bigB[k] = b;
+      bigCnt[k] = cnt;
+    }
+  }
+  __syncthreads();
+  const uint32_t n = nBig;
+  for (uint32_t k = 0; k < n; ++k) bigBucket(bigB[k], bigCnt[k], io, o);
+}
+
+// Writes the compact output: per class (run) its probability, hash, target
+// offset and first member, and its targets (from the representative's
+// stashed row, or from S for classes above kStashSlots targets).  One wave
+// per bucket, one lane per run; block 0 also writes each circuit's class and
+// target bounds.
+struct EmitIn {
+  uint32_t numVariants, numBuckets;
+  const uint32_t *slotBase, *runCount, *classStart, *tgtStart, *memberStart;
+  const uint32_t *runRank, *runTgtPre, *runMember;
+  const double *runP;
+  const uint64_t *runHash;
+  const uint32_t *rankLayer, *S, *nnz, *stash;
+  const int2 *band;
+};
+struct EmitOut {
+  double *p;
+  uint32_t *off;
+  uint64_t *hash;
+  uint32_t *member;
+  uint32_t *targets;
+  uint64_t capTargets;
+  uint32_t *flags, *vBounds;
+};
+
+__global__ __launch_bounds__(256) void emitKernel(const VDesc *__restrict__ desc, EmitIn in, EmitOut out) {
   const uint32_t lane = threadIdx.x & 63;
   if (blockIdx.x == 0 && threadIdx.x < 64) {
-    for (uint32_t v = lane; v <= numVariants; v += 64) {
-      const uint32_t bb = v < numVariants ? desc[v].bucketBase : numBuckets;
-      vBounds[2 * v] = classStart[bb];
-      vBounds[2 * v + 1] = tgtStart[bb];
+    for (uint32_t v = lane; v <= in.numVariants; v += 64) {
+      const uint32_t bb = v < in.numVariants ? desc[v].bucketBase : in.numBuckets;
+      out.vBounds[2 * v] = in.classStart[bb];
+      out.vBounds[2 * v + 1] = in.tgtStart[bb];
     }
   }
   const uint32_t b = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
-  if (b >= numBuckets) return;
+  if (b >= in.numBuckets) return;
   // circuit owning bucket b
-  uint32_t lo = 0, hi = numVariants - 1;
+  uint32_t lo = 0, hi = in.numVariants - 1;
   while (lo < hi) {
     const uint32_t mid = (lo + hi + 1) >> 1;
     if (desc[mid].bucketBase <= b) lo = mid;
     else hi = mid - 1;
   }
   const VDesc d = desc[lo];
-  const uint32_t nR = runCount[b], bs = bucketStart[b], cb = classStart[b], tb = tgtStart[b];
+  const uint32_t nR = in.runCount[b], bs = in.slotBase[b], cb = in.classStart[b], tb = in.tgtStart[b];
+  const uint32_t mb = in.memberStart[b];
   for (uint32_t r = lane; r < nR; r += 64) {
     const uint32_t c = cb + r;
-    const uint32_t rk = runRank[bs + r];
-    uint64_t pos = tb + runTgtPre[bs + r];
-    outP[c] = runP[bs + r];
-    outHash[c] = runHash[bs + r];
-    outOff[c] = static_cast<uint32_t>(pos);
-    outMember[c] = bs + runMember[bs + r];
-    const uint32_t i = static_cast<uint32_t>(rk - d.rankOff);
-    const uint32_t l = rankLayer[rk];
-    const uint32_t *mk = masks + d.maskOff + static_cast<uint64_t>(l) * d.maskWords;
-    const uint32_t *Sv = S + d.sOff + i;
-    for (uint32_t w = 0; w < d.maskWords; ++w) {
-      uint32_t m = mk[w];
-      while (m) {
-        const uint32_t col = w * 32 + static_cast<uint32_t>(__builtin_ctz(m));
-        m &= m - 1;
-        uint32_t x = Sv[static_cast<uint64_t>(col) * d.colStride];
-        while (x) {
-          if (pos < capTargets) outTargets[pos] = col * 32 + static_cast<uint32_t>(__builtin_ctz(x));
-          pos++;
-          x &= x - 1;
-        }
+    const uint32_t rk = in.runRank[bs + r];
+    uint64_t pos = tb + in.runTgtPre[bs + r];
+    out.p[c] = in.runP[bs + r];
+    out.hash[c] = in.runHash[bs + r];
+    out.off[c] = static_cast<uint32_t>(pos);
+    out.member[c] = mb + in.runMember[bs + r];
+    const uint32_t n = in.nnz[rk];
+    if (n <= kStashSlots) {
+      // the representative's stashed row (one 64-byte line)
+      const uint4 *src = reinterpret_cast<const uint4 *>(in.stash + static_cast<uint64_t>(rk) * kStashSlots);
+      uint32_t tg[kStashSlots];
+#pragma unroll
+      for (uint32_t q = 0; q < kStashSlots / 4; ++q) {
+        const uint4 x = src[q];
+        tg[4 * q] = x.x;
+        tg[4 * q + 1] = x.y;
+        tg[4 * q + 2] = x.z;
+        tg[4 * q + 3] = x.w;
       }
+#pragma unroll
+      for (uint32_t j = 0; j < kStashSlots; ++j)
+        if (j < n && pos + j < out.capTargets) out.targets[pos + j] = tg[j];
+      pos += n;
+    } else {
+      const uint32_t i = static_cast<uint32_t>(rk - d.rankOff);
+      const LayerMasks lm{nullptr, in.band + d.colOff, 0u, d.numCols, d.maskWords};
+      forEachTarget(lm, in.rankLayer[rk], in.S + d.sOff + i, d.colStride, [&](uint32_t tg) {
+        if (pos < out.capTargets) out.targets[pos] = tg;
+        pos++;
+      });
     }
-    if (pos > capTargets) atomicOr(&flags[1], 1u);
+    if (pos > out.capTargets) atomicOr(&out.flags[1], 1u);
   }
-  if (b == numBuckets - 1 && lane == 0) {
-    outOff[classStart[numBuckets]] = tgtStart[numBuckets];
-    outMember[classStart[numBuckets]] = bucketStart[numBuckets];
+  if (b == in.numBuckets - 1 && lane == 0) {
+    out.off[in.classStart[in.numBuckets]] = in.tgtStart[in.numBuckets];
+    out.member[in.classStart[in.numBuckets]] = in.memberStart[in.numBuckets];
   }
 }
 
-// Exclusive scan of one or two u32 arrays (scanned independently), tile
-// level: 4096 items per 1024-thread block.  With one tile it also writes
-// the totals to out[n]; otherwise the tile totals go to tileSums.
-__global__ __launch_bounds__(kScanBlock) void scanTileKernel(
-    const uint32_t *__restrict__ inA, const uint32_t *__restrict__ inB, uint32_t n,
-    uint32_t *__restrict__ outA, uint32_t *__restrict__ outB, uint32_t *__restrict__ tileSums) {
-  __shared__ uint32_t waveSums[2][kScanBlock / 64];
+// Compact copy of the bucket-sorted members (for the differentiable path):
+// bucket b's itemCount[b] sorted ranks, from its slots, to memberStart[b].
+__global__ __launch_bounds__(256) void compactMembersKernel(uint32_t numBuckets, const uint32_t *__restrict__ slotBase,
+                                                            const uint32_t *__restrict__ itemCount,
+                                                            const uint32_t *__restrict__ memberStart,
+                                                            const uint32_t *__restrict__ sortedRank,
+                                                            uint32_t *__restrict__ dst) {
+  const uint32_t b = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+  if (b >= numBuckets) return;
+  const uint32_t lane = threadIdx.x & 63, n = itemCount[b], s = slotBase[b], m = memberStart[b];
+  for (uint32_t k = lane; k < n; k += 64) dst[m + k] = sortedRank[s + k];
+}
+
+// Exclusive scan of three u32 arrays (scanned independently), tile level:
+// 4096 items per 1024-thread block.  With one tile it also writes the totals
+// to out[n]; otherwise the tile totals go to tileSums (interleaved triples).
+constexpr int kScanArrays = 3;
+struct ScanArrays {
+  const uint32_t *in[kScanArrays];
+  uint32_t *out[kScanArrays];
+};
+
+__global__ __launch_bounds__(kScanBlock) void scanTileKernel(ScanArrays a, uint32_t n,
+                                                             uint32_t *__restrict__ tileSums) {
+  __shared__ uint32_t waveSums[kScanArrays][kScanBlock / 64];
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * 4;
-  uint32_t va[4], vb[4];
-  uint32_t ta = 0, tb = 0;
+  uint32_t v[kScanArrays][4], t[kScanArrays], incl[kScanArrays];
+  // every load first (one memory round trip), then the arithmetic
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    va[k] = base + k < n ? inA[base + k] : 0u;
-    vb[k] = (inB && base + k < n) ? inB[base + k] : 0u;
-    ta += va[k];
-    tb += vb[k];
-  }
-  const uint32_t ia = waveInclusiveSum(ta, lane);
-  const uint32_t ib = waveInclusiveSum(tb, lane);
-  if (lane == 63) {
-    waveSums[0][wave] = ia;
-    waveSums[1][wave] = ib;
+  for (int q = 0; q < kScanArrays; ++q)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[q][k] = a.in[q][base + k < n ? base + k : 0u];
+#pragma unroll
+  for (int q = 0; q < kScanArrays; ++q) {
+    t[q] = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (base + k >= n) v[q][k] = 0u;
+      t[q] += v[q][k];
+    }
+    incl[q] = waveInclusiveSum(t[q], lane);
+    if (lane == 63) waveSums[q][wave] = incl[q];
   }
   __syncthreads();
   if (wave == 0) {
     constexpr uint32_t W = kScanBlock / 64;
-    const uint32_t xa = lane < W ? waveSums[0][lane] : 0u;
-    const uint32_t xb = lane < W ? waveSums[1][lane] : 0u;
-    const uint32_t ya = waveInclusiveSum(xa, lane);
-    const uint32_t yb = waveInclusiveSum(xb, lane);
-    if (lane < W) {
-      waveSums[0][lane] = ya - xa;
-      waveSums[1][lane] = yb - xb;
-    }
-    if (lane == W - 1) {
-      if (gridDim.x == 1) {
-        outA[n] = ya;
-        if (outB) outB[n] = yb;
-      } else {
-        tileSums[2 * blockIdx.x] = ya;
-        tileSums[2 * blockIdx.x + 1] = yb;
+#pragma unroll
+    for (int q = 0; q < kScanArrays; ++q) {
+      const uint32_t x = lane < W ? waveSums[q][lane] : 0u;
+      const uint32_t y = waveInclusiveSum(x, lane);
+      if (lane < W) waveSums[q][lane] = y - x;
+      if (lane == W - 1) {
+        if (gridDim.x == 1) a.out[q][n] = y;
+        else tileSums[kScanArrays * blockIdx.x + q] = y;
       }
     }
   }
   __syncthreads();
-  uint32_t ra = waveSums[0][wave] + ia - ta;
-  uint32_t rb = waveSums[1][wave] + ib - tb;
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    if (base + k < n) {
-      outA[base + k] = ra;
-      if (outB) outB[base + k] = rb;
+  for (int q = 0; q < kScanArrays; ++q) {
+    uint32_t r = waveSums[q][wave] + incl[q] - t[q];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (base + k < n) a.out[q][base + k] = r;
+      r += v[q][k];
     }
-    ra += va[k];
-    rb += vb[k];
   }
 }
 
-// Second level: scans the tile totals (interleaved pairs) in one block.
+// Second level: scans the tile totals in one block; sums[3 * numTiles + q]
+// gets the grand totals.
 __global__ __launch_bounds__(kScanBlock) void scanTileSumsKernel(uint32_t *__restrict__ sums,
                                                                  uint32_t numTiles) {
-  __shared__ uint32_t carry[2];
-  if (threadIdx.x == 0) carry[0] = carry[1] = 0;
+  __shared__ uint32_t carry[kScanArrays];
+  __shared__ uint32_t ws[kScanArrays][kScanBlock / 64];
+  if (threadIdx.x < kScanArrays) carry[threadIdx.x] = 0;
   __syncthreads();
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (uint32_t base = 0; base < numTiles; base += kScanBlock) {
     const uint32_t t = base + threadIdx.x;
-    const uint32_t a = t < numTiles ? sums[2 * t] : 0u, b = t < numTiles ? sums[2 * t + 1] : 0u;
-    __shared__ uint32_t ws[2][kScanBlock / 64];
-    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const uint32_t ia = waveInclusiveSum(a, lane), ib = waveInclusiveSum(b, lane);
-    if (lane == 63) {
-      ws[0][wave] = ia;
-      ws[1][wave] = ib;
+    uint32_t x[kScanArrays], incl[kScanArrays];
+#pragma unroll
+    for (int q = 0; q < kScanArrays; ++q) {
+      x[q] = t < numTiles ? sums[kScanArrays * t + q] : 0u;
+      incl[q] = waveInclusiveSum(x[q], lane);
+      if (lane == 63) ws[q][wave] = incl[q];
     }
     __syncthreads();
     if (wave == 0) {
       constexpr uint32_t W = kScanBlock / 64;
-      const uint32_t xa = lane < W ? ws[0][lane] : 0u, xb = lane < W ? ws[1][lane] : 0u;
-      const uint32_t ya = waveInclusiveSum(xa, lane), yb = waveInclusiveSum(xb, lane);
-      if (lane < W) {
-        ws[0][lane] = ya - xa;
-        ws[1][lane] = yb - xb;
+#pragma unroll
+      for (int q = 0; q < kScanArrays; ++q) {
+        const uint32_t y0 = lane < W ? ws[q][lane] : 0u;
+        const uint32_t y = waveInclusiveSum(y0, lane);
+        if (lane < W) ws[q][lane] = y - y0;
       }
     }
     __syncthreads();
-    const uint32_t ca = carry[0], cb = carry[1];
-    if (t < numTiles) {
-      sums[2 * t] = ca + ws[0][wave] + ia - a;
-      sums[2 * t + 1] = cb + ws[1][wave] + ib - b;
+    uint32_t c[kScanArrays];
+#pragma unroll
+    for (int q = 0; q < kScanArrays; ++q) {
+      c[q] = carry[q];
+      if (t < numTiles) sums[kScanArrays * t + q] = c[q] + ws[q][wave] + incl[q] - x[q];
     }
     __syncthreads();
     if (threadIdx.x == kScanBlock - 1) {
-      carry[0] = ca + ws[0][wave] + ia;
-      carry[1] = cb + ws[1][wave] + ib;
+#pragma unroll
+      for (int q = 0; q < kScanArrays; ++q) carry[q] = c[q] + ws[q][wave] + incl[q];
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    sums[2 * numTiles] = carry[0];
-    sums[2 * numTiles + 1] = carry[1];
-  }
+  if (threadIdx.x < kScanArrays) sums[kScanArrays * numTiles + threadIdx.x] = carry[threadIdx.x];
 }
 
-__global__ __launch_bounds__(kScanBlock) void scanAddKernel(uint32_t n, const uint32_t *__restrict__ tileOff,
-                                                            uint32_t numTiles, uint32_t *__restrict__ outA,
-                                                            uint32_t *__restrict__ outB) {
+__global__ __launch_bounds__(kScanBlock) void scanAddKernel(ScanArrays a, uint32_t n,
+                                                            const uint32_t *__restrict__ tileOff,
+                                                            uint32_t numTiles) {
   const uint32_t tile = blockIdx.x;
-  const uint32_t addA = tileOff[2 * tile], addB = tileOff[2 * tile + 1];
   const uint32_t base = tile * kScanTile;
-  for (uint32_t k = threadIdx.x; k < kScanTile; k += kScanBlock) {
-    const uint32_t i = base + k;
-    if (i < n) {
-      outA[i] += addA;
-      if (outB) outB[i] += addB;
-    }
-  }
-  if (tile == 0 && threadIdx.x == 0) {
-    outA[n] = tileOff[2 * numTiles];
-    if (outB) outB[n] = tileOff[2 * numTiles + 1];
+#pragma unroll
+  for (int q = 0; q < kScanArrays; ++q) {
+    const uint32_t add = tileOff[kScanArrays * tile + q];
+    for (uint32_t k = threadIdx.x; k < kScanTile; k += kScanBlock)
+      if (base + k < n) a.out[q][base + k] += add;
+    if (tile == 0 && threadIdx.x == 0) a.out[q][n] = tileOff[kScanArrays * numTiles + q];
   }
 }
 
@@ -978,14 +1168,23 @@ struct Pipeline::Impl {
   size_t oDesc = 0, oItems = 0, oTiles = 0, oAct = 0, oSeed = 0, oColHi = 0,
          oColLow = 0, oRankLayer = 0, oRankProb = 0, inBytes = 0;
   uint32_t V = 0, nItems = 0, nTiles = 0, B = 0, R = 0, Ccols = 0, rpt = 1;
-  uint64_t Swords = 0, maskWordsTotal = 0;
+  uint64_t Swords = 0;
   size_t sweepLdsBytes = 0;
   int seedsInLds = 1;
   uint32_t sweepThreads = 64;
+  // bucket counters sit countStride words apart (one per 128-byte line when
+  // there are few buckets): atomics on one line serialize, and classify
+  // issues one per node
+  uint32_t countStride = 1;
 
-  DevBuf bandLo, S, hash, h2, nnz, counters, bucketStart, bucketPos, pkey, prank, runRank, runP,
-      runHash, runTgtPre, runMember, runCount, runNnz, classStart, tgtStart, outP, outOff, outHash,
-      outMember, outTargets, scratchKey, scratchRank, overflowList, scanTmp, vBounds, masks;
+  // per node: stash, hash, h2, nnz, overflow list (ovKey/ovRank/ovBucket);
+  // per bucket slot (numBuckets * kBucketCap, plus a numRanks arena for
+  // oversized buckets): pkey, prank (sorted in place) and the run arrays;
+  // per bucket: counters, slotBase, item/run/target counts and their scans
+  DevBuf band, S, stash, hash, h2, nnz, counters, bucketCnt, ovKey, ovRank, ovBucket, pkey, prank,
+      runRank, runP, runHash, runTgtPre, runMember, runCount, runNnz, itemCount, slotBase, classStart,
+      tgtStart, memberStart, outP, outOff, outHash, outMember, outTargets, scratchKey, scratchRank,
+      scanTmp, vBounds, members;
   uint64_t capTargets = 0;
 
   // host results
@@ -996,19 +1195,19 @@ struct Pipeline::Impl {
   std::vector<VDesc> descs;
 
   uint32_t *flags() const { return counters.as<uint32_t>(); }
-  uint32_t *bucketCount() const { return counters.as<uint32_t>() + kFlagWords; }
+  uint32_t *bucketCount() const { return bucketCnt.as<uint32_t>(); }
 
-  // exclusive scan of n entries of inA (and inB), n+1 outputs each
-  void scan(const uint32_t *inA, const uint32_t *inB, uint32_t n, uint32_t *outA, uint32_t *outB) {
+  // exclusive scans of n entries of three arrays, n + 1 outputs each
+  void scan(const ScanArrays &a, uint32_t n) {
     const uint32_t tiles = (n + kScanTile - 1) / kScanTile;
     if (tiles <= 1) {
-      scanTileKernel<<<1, kScanBlock, 0, stream>>>(inA, inB, n, outA, outB, nullptr);
+      scanTileKernel<<<1, kScanBlock, 0, stream>>>(a, n, nullptr);
       return;
     }
     uint32_t *ts = scanTmp.as<uint32_t>();
-    scanTileKernel<<<tiles, kScanBlock, 0, stream>>>(inA, inB, n, outA, outB, ts);
+    scanTileKernel<<<tiles, kScanBlock, 0, stream>>>(a, n, ts);
     scanTileSumsKernel<<<1, kScanBlock, 0, stream>>>(ts, tiles);
-    scanAddKernel<<<tiles, kScanBlock, 0, stream>>>(n, ts, tiles, outA, outB);
+    scanAddKernel<<<tiles, kScanBlock, 0, stream>>>(a, n, ts, tiles);
   }
 };
 
@@ -1059,7 +1258,7 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   // layout
   m.descs.assign(V, VDesc{});
   variants_.assign(V, VariantInfo{});
-  uint64_t actOff = 0, seedOff = 0, colOff = 0, rankOff = 0, sOff = 0, maskOff = 0;
+  uint64_t actOff = 0, seedOff = 0, colOff = 0, rankOff = 0, sOff = 0;
   uint32_t bucketBase = 0, maxRpt = 1, maxThreads = 64;
   size_t ldsRing = 0, ldsAll = 0;  // largest ring (+ zero word), largest ring + seeds
   std::vector<uint2> items, tiles;
@@ -1086,7 +1285,6 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
     d.colOff = colOff;
     d.rankOff = rankOff;
     d.sOff = sOff;
-    d.maskOff = maskOff;
     d.colStride = static_cast<uint64_t>(g.numRanks) + kSinkSlots;
     variants_[v] = VariantInfo{g.numRanks, g.numCols, g.numDetectors, g.numObservables, bucketBase,
                                d.numBuckets};
@@ -1102,7 +1300,6 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
     colOff += g.numCols;
     rankOff += g.numRanks;
     sOff += d.colStride * g.numCols;
-    maskOff += static_cast<uint64_t>(g.numLayers) * d.maskWords;
     bucketBase += d.numBuckets;
     maxRpt = std::max(maxRpt, g.sweepRpt);
     maxThreads = std::max(maxThreads, g.sweepThreads);
@@ -1135,7 +1332,6 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   m.R = static_cast<uint32_t>(rankOff);
   m.Ccols = static_cast<uint32_t>(colOff);
   m.Swords = sOff;
-  m.maskWordsTotal = maskOff;
 
   // input arena
   size_t off = 0;
@@ -1173,26 +1369,39 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
 
   // work buffers
   const uint64_t R = std::max<uint64_t>(m.R, 1), B = m.B;
-  m.bandLo.ensure(4 * std::max<uint32_t>(m.Ccols, 1), frozen, "columns");
-  m.masks.ensure(4 * std::max<uint64_t>(m.maskWordsTotal, 1), frozen, "layers");
+  m.band.ensure(8 * std::max<uint32_t>(m.Ccols, 1), frozen, "columns");
   m.S.ensure(4 * std::max<uint64_t>(m.Swords, 1), frozen, "sensitivity matrix");
   m.hash.ensure(8 * R, frozen, "nodes");
+  m.stash.ensure(4ull * kStashSlots * R, frozen, "nodes");
   m.h2.ensure(8 * R, frozen, "nodes");
   m.nnz.ensure(4 * R, frozen, "nodes");
-  m.bucketPos.ensure(4 * R, frozen, "nodes");
-  m.counters.ensure(4 * (kFlagWords + B + 1), frozen, "buckets");
-  m.bucketStart.ensure(4 * (B + 1), frozen, "buckets");
-  m.pkey.ensure(8 * R, frozen, "nodes");
-  m.prank.ensure(4 * R, frozen, "nodes");
-  m.runRank.ensure(4 * R, frozen, "nodes");
-  m.runP.ensure(8 * R, frozen, "nodes");
-  m.runHash.ensure(8 * R, frozen, "nodes");
-  m.runTgtPre.ensure(4 * R, frozen, "nodes");
-  m.runMember.ensure(4 * R, frozen, "nodes");
+  m.ovKey.ensure(8 * R, frozen, "nodes");
+  m.ovRank.ensure(4 * R, frozen, "nodes");
+  m.ovBucket.ensure(4 * R, frozen, "nodes");
+  m.counters.ensure(4 * kFlagWords, frozen, "buckets");
+  m.countStride = 32;
+  while (m.countStride > 1 && static_cast<uint64_t>(B) * m.countStride > (1u << 17)) m.countStride >>= 1;
+  // (B + 1) * countStride <= 2^17 + 32 words whatever B is, so a reserved
+  // pipeline fits every later load)
+  m.bucketCnt.ensure(4 * std::max<uint64_t>((static_cast<uint64_t>(B) + 1) * m.countStride, (1u << 17) + 32),
+                     frozen, "buckets");
+  // bucket slots, then an arena of numRanks entries for oversized buckets
+  const uint64_t slots = B * static_cast<uint64_t>(kBucketCap) + R;
+  if (slots >= 0xFFFFFFFFull) throw RuntimeError("too many sort buckets in one load");
+  m.pkey.ensure(8 * slots, frozen, "nodes");
+  m.prank.ensure(4 * slots, frozen, "nodes");
+  m.runRank.ensure(4 * slots, frozen, "nodes");
+  m.runP.ensure(8 * slots, frozen, "nodes");
+  m.runHash.ensure(8 * slots, frozen, "nodes");
+  m.runTgtPre.ensure(4 * slots, frozen, "nodes");
+  m.runMember.ensure(4 * slots, frozen, "nodes");
   m.runCount.ensure(4 * (B + 1), frozen, "buckets");
   m.runNnz.ensure(4 * (B + 1), frozen, "buckets");
+  m.itemCount.ensure(4 * (B + 1), frozen, "buckets");
+  m.slotBase.ensure(4 * (B + 1), frozen, "buckets");
   m.classStart.ensure(4 * (B + 1), frozen, "buckets");
   m.tgtStart.ensure(4 * (B + 1), frozen, "buckets");
+  m.memberStart.ensure(4 * (B + 1), frozen, "buckets");
   m.outP.ensure(8 * R, frozen, "nodes");
   m.outOff.ensure(4 * (R + 1), frozen, "nodes");
   m.outHash.ensure(8 * R, frozen, "nodes");
@@ -1203,8 +1412,7 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   }
   m.scratchKey.ensure(16 * R, frozen, "nodes");
   m.scratchRank.ensure(8 * R, frozen, "nodes");
-  m.overflowList.ensure(4 * (B + 1), frozen, "buckets");
-  m.scanTmp.ensure(8 * ((B + kScanTile - 1) / kScanTile + 2), frozen, "buckets");
+  m.scanTmp.ensure(4 * kScanArrays * ((B + kScanTile - 1) / kScanTile + 2), frozen, "buckets");
   m.vBounds.ensure(4 * (2 * V + 8), frozen, "circuits");
   m.hVB.ensure(4 * (2 * V + 8));
   m.hFlags.ensure(4 * kFlagWords);
@@ -1222,6 +1430,9 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
       GPA_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   static_cast<int>(m.sweepLdsBytes)));
   }
+
+  // bucket counters start at zero; every run's bucket stage leaves them zeroed
+  GPA_HIP(hipMemsetAsync(m.bucketCnt.p, 0, m.bucketCnt.cap, m.stream));
 
   // upload (one copy)
   GPA_HIP(hipEventRecord(m.ev[0], m.stream));
@@ -1254,21 +1465,16 @@ void Pipeline::run(bool timed) {
   const uint32_t B = m.B;
   m.timedRun = timed;
 
-  // per-run counters and masks start at zero (a fill kernel, not a memset,
-  // so the sequence stays safe to capture in a graph)
-  {
-    const uint64_t n = kFlagWords + B + 1 + m.maskWordsTotal;
-    const uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>((n + 255) / 256, 2048));
-    zeroKernel<<<std::max<uint32_t>(blocks, 1), 256, 0, st>>>(m.counters.as<uint32_t>(),
-                                                               kFlagWords + B + 1,
-                                                               m.masks.as<uint32_t>(), m.maskWordsTotal);
-  }
+  // Five launches: sweep (block 0 also zeroes the flags), classify (hash,
+  // stash, bucket slots), buckets (sort + merge; counters reset for the next
+  // run), the three scans, emit.  No memset inside, so the sequence can be
+  // captured in a graph.
   mark(0);
-  if (m.nItems) {
+  {
 #define GPA_SWEEP(RPT, G)                                                                        \
-  sweepKernel<RPT, G><<<m.nItems, m.sweepThreads, m.sweepLdsBytes, st>>>(                       \
-      desc, items, act, seed, colHi, colLow, m.bandLo.as<int32_t>(), m.S.as<uint32_t>(), \
-      m.masks.as<uint32_t>())
+  sweepKernel<RPT, G><<<std::max<uint32_t>(m.nItems, 1), m.sweepThreads, m.sweepLdsBytes, st>>>(  \
+      desc, items, act, seed, colHi, colLow, m.band.as<int2>(), m.S.as<uint32_t>(), m.nItems,      \
+      m.flags())
     if (m.seedsInLds) {
       switch (m.rpt) {
       case 1: GPA_SWEEP(1, false); break;
@@ -1287,17 +1493,14 @@ void Pipeline::run(bool timed) {
 #undef GPA_SWEEP
   }
   mark(1);
-  if (m.nTiles)
-    classifyKernel<<<m.nTiles, kClassBlock, 0, st>>>(desc, tiles, m.masks.as<uint32_t>(), rankLayer,
-                                                     m.S.as<uint32_t>(), m.hash.as<uint64_t>(),
-                                                     m.h2.as<uint64_t>(), m.nnz.as<uint32_t>(),
-                                                     m.bucketCount(), m.bucketPos.as<uint32_t>());
+  if (m.nTiles) {
+    ClassifyOut co{m.hash.as<uint64_t>(), m.h2.as<uint64_t>(), m.nnz.as<uint32_t>(), m.bucketCount(),
+                   m.countStride, m.pkey.as<uint64_t>(), m.prank.as<uint32_t>(), m.ovKey.as<uint64_t>(),
+                   m.ovRank.as<uint32_t>(), m.ovBucket.as<uint32_t>(), m.flags(), m.stash.as<uint32_t>()};
+    classifyKernel<<<m.nTiles, kClassBlock, 0, st>>>(desc, tiles, rankLayer, m.band.as<int2>(),
+                                                     m.S.as<uint32_t>(), co);
+  }
   mark(2);
-  m.scan(m.bucketCount(), nullptr, B, m.bucketStart.as<uint32_t>(), nullptr);
-  if (m.nTiles)
-    partitionKernel<<<m.nTiles, kClassBlock, 0, st>>>(
-        desc, tiles, m.hash.as<uint64_t>(), m.nnz.as<uint32_t>(), m.bucketStart.as<uint32_t>(),
-        m.bucketPos.as<uint32_t>(), m.pkey.as<uint64_t>(), m.prank.as<uint32_t>());
   mark(3);
   RunOut o{rankProb,
            m.h2.as<uint64_t>(),
@@ -1309,24 +1512,23 @@ void Pipeline::run(bool timed) {
            m.runTgtPre.as<uint32_t>(),
            m.runMember.as<uint32_t>(),
            m.flags()};
-  bucketKernel<<<(B + 3) / 4, 256, 0, st>>>(B, m.bucketStart.as<uint32_t>(), m.pkey.as<uint64_t>(), o,
-                                            m.runCount.as<uint32_t>(), m.runNnz.as<uint32_t>(),
-                                            m.overflowList.as<uint32_t>());
-  bucketFallbackKernel<<<64, 256, 0, st>>>(m.bucketStart.as<uint32_t>(), m.pkey.as<uint64_t>(), o,
-                                           m.runCount.as<uint32_t>(), m.runNnz.as<uint32_t>(),
-                                           m.overflowList.as<uint32_t>(), m.scratchKey.as<uint64_t>(),
-                                           m.scratchRank.as<uint32_t>());
+  BucketIo bio{m.bucketCount(), m.countStride, m.pkey.as<uint64_t>(), m.ovKey.as<uint64_t>(),
+               m.ovRank.as<uint32_t>(), m.ovBucket.as<uint32_t>(), m.runCount.as<uint32_t>(),
+               m.runNnz.as<uint32_t>(), m.itemCount.as<uint32_t>(), m.slotBase.as<uint32_t>(),
+               m.scratchKey.as<uint64_t>(), m.scratchRank.as<uint32_t>(), B * kBucketCap};
+  if (B) bucketKernel<<<(B + 3) / 4, 256, 0, st>>>(B, bio, o);
   mark(4);
-  m.scan(m.runCount.as<uint32_t>(), m.runNnz.as<uint32_t>(), B, m.classStart.as<uint32_t>(),
-         m.tgtStart.as<uint32_t>());
-  emitKernel<<<(B + 3) / 4, 256, 0, st>>>(
-      desc, m.V, B, m.bucketStart.as<uint32_t>(), m.runCount.as<uint32_t>(),
-      m.classStart.as<uint32_t>(), m.tgtStart.as<uint32_t>(), m.runRank.as<uint32_t>(),
-      m.runP.as<double>(), m.runHash.as<uint64_t>(), m.runTgtPre.as<uint32_t>(),
-      m.runMember.as<uint32_t>(), rankLayer, m.masks.as<uint32_t>(), m.S.as<uint32_t>(),
-      m.outP.as<double>(), m.outOff.as<uint32_t>(), m.outHash.as<uint64_t>(),
-      m.outMember.as<uint32_t>(), m.outTargets.as<uint32_t>(), m.capTargets, m.flags(),
-      m.vBounds.as<uint32_t>());
+  ScanArrays sa{{m.runCount.as<uint32_t>(), m.runNnz.as<uint32_t>(), m.itemCount.as<uint32_t>()},
+                {m.classStart.as<uint32_t>(), m.tgtStart.as<uint32_t>(), m.memberStart.as<uint32_t>()}};
+  m.scan(sa, B);
+  EmitIn ein{m.V, B, m.slotBase.as<uint32_t>(), m.runCount.as<uint32_t>(), m.classStart.as<uint32_t>(),
+             m.tgtStart.as<uint32_t>(), m.memberStart.as<uint32_t>(), m.runRank.as<uint32_t>(),
+             m.runTgtPre.as<uint32_t>(), m.runMember.as<uint32_t>(), m.runP.as<double>(),
+             m.runHash.as<uint64_t>(), rankLayer, m.S.as<uint32_t>(), m.nnz.as<uint32_t>(),
+             m.stash.as<uint32_t>(), m.band.as<int2>()};
+  EmitOut eout{m.outP.as<double>(), m.outOff.as<uint32_t>(), m.outHash.as<uint64_t>(), m.outMember.as<uint32_t>(),
+               m.outTargets.as<uint32_t>(), m.capTargets, m.flags(), m.vBounds.as<uint32_t>()};
+  emitKernel<<<std::max<uint32_t>((B + 3) / 4, 1), 256, 0, st>>>(desc, ein, eout);
   mark(5);
   GPA_HIP(hipGetLastError());
 }
@@ -1470,13 +1672,14 @@ uint32_t bucketFill() { return gBucketFill.load(); }
 
 uint64_t Pipeline::deviceBytes() const {
   const Impl &m = *impl_;
-  const DevBuf *bufs[] = {&m.in, &m.bandLo, &m.masks, &m.S, &m.hash, &m.h2, &m.nnz, &m.bucketPos,
-                          &m.counters, &m.bucketStart, &m.pkey, &m.prank,
-                          &m.runRank, &m.runP,
-                          &m.runHash, &m.runTgtPre, &m.runMember, &m.runCount, &m.runNnz,
-                          &m.classStart, &m.tgtStart, &m.outP, &m.outOff, &m.outHash, &m.outMember,
-                          &m.outTargets, &m.scratchKey, &m.scratchRank, &m.overflowList, &m.scanTmp,
-                          &m.vBounds};
+  const DevBuf *bufs[] = {&m.in,        &m.band,     &m.S,          &m.stash,     &m.hash,
+                          &m.h2,        &m.nnz,        &m.counters,   &m.bucketCnt, &m.ovKey,
+                          &m.ovRank,    &m.ovBucket,   &m.pkey,       &m.prank,     &m.runRank,
+                          &m.runP,      &m.runHash,    &m.runTgtPre,  &m.runMember, &m.runCount,
+                          &m.runNnz,    &m.itemCount,  &m.slotBase,   &m.classStart, &m.tgtStart,
+                          &m.memberStart, &m.outP,     &m.outOff,     &m.outHash,   &m.outMember,
+                          &m.outTargets, &m.scratchKey, &m.scratchRank, &m.scanTmp, &m.vBounds,
+                          &m.members};
   uint64_t s = 0;
   for (const DevBuf *b : bufs) s += b->cap;
   return s;
@@ -1488,23 +1691,35 @@ void Pipeline::debugSensitivities(uint32_t v, std::vector<uint32_t> &cols32) con
   GPA_HIP(hipStreamSynchronize(m.stream));
   const VDesc &d = m.descs.at(v);
   std::vector<uint32_t> S(static_cast<size_t>(d.colStride) * d.numCols);
-  std::vector<int32_t> lo(d.numCols), hi(d.numCols);
+  std::vector<int2> band(d.numCols);
   if (!S.empty())
     GPA_HIP(hipMemcpy(S.data(), m.S.as<uint32_t>() + d.sOff, 4 * S.size(), hipMemcpyDeviceToHost));
-  GPA_HIP(hipMemcpy(lo.data(), m.bandLo.as<int32_t>() + d.colOff, 4 * d.numCols, hipMemcpyDeviceToHost));
-  std::memcpy(hi.data(), m.inHost.as<char>() + m.oColHi + 4 * d.colOff, 4 * d.numCols);
+  if (d.numCols)
+    GPA_HIP(hipMemcpy(band.data(), m.band.as<int2>() + d.colOff, 8 * d.numCols, hipMemcpyDeviceToHost));
   std::vector<uint32_t> layer(d.numRanks);
   std::memcpy(layer.data(), m.inHost.as<char>() + m.oRankLayer + 4 * d.rankOff, 4 * d.numRanks);
   cols32.assign(static_cast<size_t>(d.numRanks) * d.numCols, 0u);
   for (uint32_t c = 0; c < d.numCols; ++c)
     for (uint32_t i = 0; i < d.numRanks; ++i) {
       const int32_t l = static_cast<int32_t>(layer[i]);
-      if (l >= lo[c] && l <= hi[c])
+      if (l >= band[c].x && l <= band[c].y)
         cols32[static_cast<size_t>(i) * d.numCols + c] = S[static_cast<size_t>(c) * d.colStride + i];
     }
 }
 
-const uint32_t *Pipeline::devSortedRanks() const { return impl_->prank.as<uint32_t>(); }
+const uint32_t *Pipeline::devSortedRanks() {
+  Impl &m = *impl_;
+  GPA_HIP(hipSetDevice(device_));
+  m.members.ensure(4 * std::max<uint64_t>(m.R, 1), false, "nodes");
+  if (m.B)
+    compactMembersKernel<<<(m.B + 3) / 4, 256, 0, m.stream>>>(m.B, m.slotBase.as<uint32_t>(),
+                                                              m.itemCount.as<uint32_t>(),
+                                                              m.memberStart.as<uint32_t>(),
+                                                              m.prank.as<uint32_t>(), m.members.as<uint32_t>());
+  GPA_HIP(hipGetLastError());
+  GPA_HIP(hipStreamSynchronize(m.stream));
+  return m.members.as<uint32_t>();
+}
 const uint32_t *Pipeline::devClassMember() const { return impl_->outMember.as<uint32_t>(); }
 
 void xorMergeForward(const double *values, const uint32_t *index, const uint32_t *segStart,

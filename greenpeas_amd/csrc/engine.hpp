@@ -1,18 +1,19 @@
 // engine.hpp -- the GPU compile pipeline for one or many circuits.
 //
-// Stages (one HIP stream, all on device; kernels in engine.hip):
+// Stages (one HIP stream, five launches, all on device; kernels in
+// engine.hip):
 //   sweep      backward sensitivity sweep, one workgroup per (circuit,
 //              32-bit detector column), three-layer LDS ring, per-column
 //              time band (replaces CUDA.hpp:26-52 + CUDAHandlers.hpp:38-62);
 //   classify   per p>0 node: set bits -> FNV-1a-64 over 32 padded slots
 //              (CUDAHandlers.hpp:86-121, Array.hpp:52-65), second hash for
-//              collision checks, bucket histogram;
-//   partition  MSD scatter of (hash, rank) into 2^b buckets per circuit
-//              (first radix digit = top b hash bits);
+//              collision checks, a row of its first 16 targets, and a slot
+//              in its sort bucket (first radix digit = top b hash bits);
 //   buckets    one wave64 per bucket: register bitonic sort on (hash, rank)
 //              and segmented XOR-merge scan (replaces the CUB radix sort and
-//              reduce-by-key, CUDA.hpp:275-343); oversized buckets go to a
-//              workgroup fallback;
+//              reduce-by-key, CUDA.hpp:275-343); oversized buckets are
+//              sorted by their workgroup;
+//   scan       per-bucket class, target and member offsets;
 //   emit       compact CSR output: probability, offset and targets per class
 //              (replaces the full N x 32 gather + copy, CUDA.hpp:488-497).
 #pragma once
@@ -72,8 +73,10 @@ class Pipeline {
 
   // Parity/debug access (after download or run + sync).
   void debugSensitivities(uint32_t v, std::vector<uint32_t> &cols32) const;  // ranks x cols
-  // Device pointers for the differentiable path (single variant).
-  const uint32_t *devSortedRanks() const;
+  // Device pointers for the differentiable path (single variant, after a
+  // run): the p>0 nodes of non-empty classes, grouped by class in output
+  // order (compacted on each call), and each class's first member in it.
+  const uint32_t *devSortedRanks();
   const uint32_t *devClassMember() const;
 
   struct Impl;

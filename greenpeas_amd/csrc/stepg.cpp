@@ -407,6 +407,7 @@ uint64_t imageDeviceBytes(const StepgImage &img) {
   b += img.numCols * 12ull + R * 12ull;  // bands, rank layer + prob
   b += (R + kSinkSlots) * img.numCols * 4ull;  // sensitivity matrix (+ per-column sink slots)
   b += R * (8 + 8 + 4 + 4);               // hash, h2, nnz, class-of-rank
+  b += R * 64;                            // stashed target rows (16 per node)
   b += R * (8 + 4) * 2;                   // partitioned keys/ranks + scratch
   b += R * (4 + 8 + 8 + 4 + 4 + 4);       // runs
   b += R * (8 + 4 + 8 + 4) + R * 32 * 4;  // outputs (targets sized 32 per class)

@@ -46,7 +46,7 @@ constexpr uint32_t kSinkSlots = 1024;
 constexpr uint32_t kSweepMaxThreads = 1024;
 // all-padding record blocks in front of layer 0 (what the sweep's
 // prefetches load for the layers below 0 that its trips run into)
-constexpr uint32_t kSweepLeadPad = 5;
+constexpr uint32_t kSweepLeadPad = 11;
 
 inline uint32_t nodesPerQubit(int level) {
   switch (level) {

@@ -58,7 +58,7 @@ def estimate_bytes(text: str, level: int = 2) -> int:
     nodes = (ticks + 1) * (maxq + 1) * k
     ranks = nodes * 3 // 5 + 64
     cols = (dets + obs + 31) // 32
-    return int(ranks * (cols * 4 + 180) + nodes * 24 + (ticks + 1) * cols * 4 + 65536)
+    return int(ranks * (cols * 4 + 244) + nodes * 24 + (ticks + 1) * cols * 4 + 65536)
 
 
 @dataclass
