@@ -53,6 +53,29 @@ def algorithmic_bytes(N: int, Wd: int, Nprime: int, C: int, nnz_bar: float) -> d
     }
 
 
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "bb12_plain_compile_hbm_traffic.md")
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (FETCH_SIZE and WRITE_SIZE passes over this workload, each scaled
+    by the counter's reading on a copy of known size); None without one."""
+    try:
+        with open(TRAFFIC_SUMMARY) as f:
+            text = f.read()
+    except OSError:
+        return None
+    start = text.find("```json")
+    if start < 0:
+        return None
+    end = text.find("```", start + 7)
+    try:
+        data = json.loads(text[start + 7:end])
+        return data["kernels"][kernel]["hbm_bytes_per_launch"]
+    except (ValueError, KeyError):
+        return None
+
+
 def cpu_baseline(text: str, locations: int, budget_s: float = 8.0) -> dict:
     from oracle import oracle as O
 
@@ -154,13 +177,17 @@ def main() -> int:
     idx, sens = driver.debug_sensitivities()
     Nprime = int(np.count_nonzero(sens.any(axis=1)))
     model = algorithmic_bytes(N, Wd, Nprime, C, nnz_bar)
-    stage_us = {k: stages[k] for k in ("sweep", "classify", "partition", "buckets", "emit")}
+    # per-stage device time (HIP events on the pipeline's stream): sweep,
+    # classify (+ scatter into the sort buckets), bucket sort + merge, offset
+    # scans + emit
+    stage_us = {k: stages[k] for k in ("sweep", "classify", "buckets", "emit")}
     dom = max(stage_us, key=stage_us.get)
     dom_bytes = {"sweep": model["sweep"], "classify": model["classify"],
-                 "partition": model["sort"] / 2, "buckets": model["sort"] / 2,
-                 "emit": model["reduce_emit"]}[dom]
+                 "buckets": model["sort"], "emit": model["reduce_emit"]}[dom]
+    kernel_name = {"sweep": "sweepKernel", "classify": "classifyKernel", "buckets": "bucketKernel",
+                   "emit": "emitKernel"}[dom]
     achieved = dom_bytes / (stage_us[dom] * 1e-6) / 1e9
-    sortmerge_us = stage_us["partition"] + stage_us["buckets"] + stage_us["emit"]
+    sortmerge_us = stage_us["buckets"] + stage_us["emit"]
     sortmerge_gbs = (model["sort"] + model["reduce_emit"]) / (sortmerge_us * 1e-6) / 1e9
 
     cpu = None if args.no_cpu else cpu_baseline(text, locations)
@@ -190,13 +217,14 @@ def main() -> int:
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": dom,
+            "kernel": kernel_name,
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(kernel_name),
             "algorithmic_bytes": dom_bytes,
+            "kernel_us": stage_us[dom],
         },
         "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
         "stage_us": stage_us,

@@ -5,7 +5,7 @@ The reference compiles one circuit per ``Driver::compile`` call
 (Driver.hpp:138-162) and recompiles per branch in its adaptive workload
 (Cli.hpp:408-416).  Here all circuits of a batch share every launch: the
 sweep has one workgroup per (circuit, detector column), longest time bands
-first; classify/partition run over the concatenated p>0 nodes; each circuit
+first; classify runs over the concatenated p>0 nodes; each circuit
 owns its own range of sort buckets, so the output stays grouped by circuit
 and ordered by class hash within each.
 """

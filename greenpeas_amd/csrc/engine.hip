@@ -1501,7 +1501,6 @@ void Pipeline::run(bool timed) {
                                                      m.S.as<uint32_t>(), co);
   }
   mark(2);
-  mark(3);
   RunOut o{rankProb,
            m.h2.as<uint64_t>(),
            m.nnz.as<uint32_t>(),
@@ -1517,7 +1516,7 @@ void Pipeline::run(bool timed) {
                m.runNnz.as<uint32_t>(), m.itemCount.as<uint32_t>(), m.slotBase.as<uint32_t>(),
                m.scratchKey.as<uint64_t>(), m.scratchRank.as<uint32_t>(), B * kBucketCap};
   if (B) bucketKernel<<<(B + 3) / 4, 256, 0, st>>>(B, bio, o);
-  mark(4);
+  mark(3);
   ScanArrays sa{{m.runCount.as<uint32_t>(), m.runNnz.as<uint32_t>(), m.itemCount.as<uint32_t>()},
                 {m.classStart.as<uint32_t>(), m.tgtStart.as<uint32_t>(), m.memberStart.as<uint32_t>()}};
   m.scan(sa, B);
@@ -1529,7 +1528,7 @@ void Pipeline::run(bool timed) {
   EmitOut eout{m.outP.as<double>(), m.outOff.as<uint32_t>(), m.outHash.as<uint64_t>(), m.outMember.as<uint32_t>(),
                m.outTargets.as<uint32_t>(), m.capTargets, m.flags(), m.vBounds.as<uint32_t>()};
   emitKernel<<<std::max<uint32_t>((B + 3) / 4, 1), 256, 0, st>>>(desc, ein, eout);
-  mark(5);
+  mark(4);
   GPA_HIP(hipGetLastError());
 }
 
@@ -1542,13 +1541,13 @@ void Pipeline::download() {
   GPA_HIP(hipMemcpyAsync(m.hFlags.p, m.counters.p, 4 * kFlagWords, hipMemcpyDeviceToHost, m.stream));
   GPA_HIP(hipStreamSynchronize(m.stream));
   if (m.timedRun) {
-    float ms[5] = {};
-    for (int i = 0; i < 5; ++i) GPA_HIP(hipEventElapsedTime(&ms[i], m.ev[i], m.ev[i + 1]));
+    float ms[4] = {};
+    for (int i = 0; i < 4; ++i) GPA_HIP(hipEventElapsedTime(&ms[i], m.ev[i], m.ev[i + 1]));
     times_.sweep = ms[0] * 1e3;
     times_.classify = ms[1] * 1e3;
-    times_.partition = ms[2] * 1e3;
-    times_.buckets = ms[3] * 1e3;
-    times_.emit = ms[4] * 1e3;
+    times_.partition = 0;  // the bucket slots are filled by classify
+    times_.buckets = ms[2] * 1e3;
+    times_.emit = ms[3] * 1e3;
     m.timedRun = false;
   }
   const uint32_t *vb = m.hVB.as<uint32_t>();
@@ -1637,33 +1636,35 @@ double Pipeline::benchDevice(uint32_t iters) {
   GPA_HIP(hipEventSynchronize(m.ev[7]));
   float ms = 0;
   GPA_HIP(hipEventElapsedTime(&ms, m.ev[6], m.ev[7]));
-  // per-stage device time: the same iterations launched directly with six
-  // events each, read after the last one completes
-  const size_t need = static_cast<size_t>(iters) * 6;
+  // per-stage device time: the same iterations launched directly with five
+  // events each (before the sweep and after the sweep, classify, bucket and
+  // scan + emit stages), read after the last one completes
+  constexpr int kEv = 5;
+  const size_t need = static_cast<size_t>(iters) * kEv;
   while (m.evPool.size() < need) {
     hipEvent_t e;
     GPA_HIP(hipEventCreate(&e));
     m.evPool.push_back(e);
   }
   for (uint32_t i = 0; i < iters; ++i) {
-    m.evOverride = m.evPool.data() + static_cast<size_t>(i) * 6;
+    m.evOverride = m.evPool.data() + static_cast<size_t>(i) * kEv;
     run(true);
   }
   m.evOverride = nullptr;
   m.timedRun = false;
   GPA_HIP(hipStreamSynchronize(m.stream));
-  double st[5] = {0, 0, 0, 0, 0};
+  double st[kEv - 1] = {};
   for (uint32_t i = 0; i < iters; ++i)
-    for (int k = 0; k < 5; ++k) {
+    for (int k = 0; k + 1 < kEv; ++k) {
       float t = 0;
-      GPA_HIP(hipEventElapsedTime(&t, m.evPool[i * 6 + k], m.evPool[i * 6 + k + 1]));
+      GPA_HIP(hipEventElapsedTime(&t, m.evPool[i * kEv + k], m.evPool[i * kEv + k + 1]));
       st[k] += t;
     }
   times_.sweep = st[0] * 1e3 / iters;
   times_.classify = st[1] * 1e3 / iters;
-  times_.partition = st[2] * 1e3 / iters;
-  times_.buckets = st[3] * 1e3 / iters;
-  times_.emit = st[4] * 1e3 / iters;
+  times_.partition = 0;
+  times_.buckets = st[2] * 1e3 / iters;
+  times_.emit = st[3] * 1e3 / iters;
   return ms * 1e3 / iters;
 }
 

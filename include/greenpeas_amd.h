@@ -97,8 +97,10 @@ int gp_driver_params(const gp_driver *driver, uint32_t params[7]);
 
 /* Per-stage device timings of the last compile, in microseconds, measured
  * with HIP events on the driver's stream: [0] upload, [1] sensitivity sweep,
- * [2] classes+hash, [3] partition, [4] bucket sort + segmented reduce,
- * [5] emit, [6] download, [7] whole compile (host wall clock). */
+ * [2] classes + hash + scatter into sort buckets, [3] unused (0; the
+ * partition is part of [2]), [4] bucket sort + segmented reduce,
+ * [5] offset scans + emit, [6] download, [7] whole compile (host wall
+ * clock). */
 int gp_driver_timings(const gp_driver *driver, double timings_us[8]);
 
 /* Compiles the already-uploaded circuit `iterations` times back to back on
