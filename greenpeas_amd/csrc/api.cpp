@@ -635,6 +635,7 @@ int gp_set_tuning(const char *name, uint32_t value) {
     if (!name) throw InvalidArgument("name is NULL");
     if (std::string(name) == "bucket_fill") gpa::setBucketFill(value);
     else if (std::string(name) == "sweep_rpt") gpa::setSweepMinRpt(value);
+    else if (std::string(name) == "stash") gpa::setStash(value != 0);
     else throw InvalidArgument(std::string("unknown tuning parameter: ") + name);
   });
 }
@@ -644,6 +645,7 @@ int gp_get_tuning(const char *name, uint32_t *value) {
     if (!name || !value) throw InvalidArgument("NULL argument");
     if (std::string(name) == "bucket_fill") *value = gpa::bucketFill();
     else if (std::string(name) == "sweep_rpt") *value = gpa::sweepMinRpt();
+    else if (std::string(name) == "stash") *value = gpa::stash() ? 1u : 0u;
     else throw InvalidArgument(std::string("unknown tuning parameter: ") + name);
   });
 }

@@ -35,6 +35,7 @@ constexpr int kScanBlock = 1024;
 constexpr int kScanTile = 4 * kScanBlock;
 constexpr int kBucketItemsPerWave = 256;  // wave-register sort capacity (4 per lane)
 std::atomic<uint32_t> gBucketFill{64};    // mean items per bucket the digit width aims at
+std::atomic<uint32_t> gStash{1};          // target rows on (1) or off (0)
 
 // Per-circuit descriptor (device and host).
 struct VDesc {
@@ -134,20 +135,30 @@ __device__ __forceinline__ void waveBitonicSort(uint64_t (&h)[R], uint32_t (&r)[
   }
 }
 
+// State carried from one 64-element chunk of a bucket to the next: the open
+// run's merged probability and second hash, the last key, run and target
+// counts so far.
 struct RunCarry {
   double p;
-  uint64_t h2;
   uint64_t prevHash;
-  uint32_t rep;
+  uint32_t h2;
   uint32_t runs;
   uint32_t nnzSum;
   uint32_t havePrev;
 };
 
+// Per-node record written by classify and read by the merge, one 16-byte
+// load per node: {probability (2 words), 32-bit second hash, class size}.
+__device__ __forceinline__ uint4 nodeInfo(double p, uint32_t h2, uint32_t nnz) {
+  const uint64_t b = __double_as_longlong(p);
+  return make_uint4(static_cast<uint32_t>(b), static_cast<uint32_t>(b >> 32), h2, nnz);
+}
+__device__ __forceinline__ double infoProb(const uint4 &x) {
+  return __longlong_as_double(static_cast<long long>((static_cast<uint64_t>(x.y) << 32) | x.x));
+}
+
 struct RunOut {
-  const double *prob;      // rank -> node probability (fallback lookups)
-  const uint64_t *h2;      // rank -> second hash
-  const uint32_t *nnz;     // rank -> class size
+  const uint4 *info;       // rank -> nodeInfo
   uint32_t *sortedRank;    // bucket-sorted ranks (in place over the bucket slots)
   uint32_t *runRank;       // per run: representative (smallest) rank
   double *runP;            // per run: merged probability
@@ -160,21 +171,12 @@ struct RunOut {
 // Segmented XOR-merge of one chunk of sorted keys (cnt valid elements,
 // positions chunkBase.. within bucket starting at bs) continuing the runs in
 // `cy`.  Runs are maximal sequences of equal hash; each run's probability is
-// the XOR-merge of its members, its representative the first member.
-// Per-element values the merge needs, looked up by key: the bucket's LDS
-// staging (key = position in the bucket) or the rank-indexed global arrays
-// (key = rank).
-struct RunLookup {
-  const double *prob;
-  const uint64_t *h2;
-  const uint32_t *nnz;
-};
-
+// the XOR-merge of its members, its representative the first member.  The
+// values the merge needs come from each member's node record (rank-indexed).
 template <int R>
-__device__ __forceinline__ void scanRuns(const uint64_t (&h)[R], const uint32_t (&rk)[R],
-                                         const uint32_t (&lk)[R], uint32_t cnt, uint32_t chunkBase,
-                                         uint32_t bs, RunCarry &cy, uint32_t lane, const RunOut &o,
-                                         const RunLookup &lu) {
+__device__ __forceinline__ void scanRuns(const uint64_t (&h)[R], const uint32_t (&rk)[R], uint32_t cnt,
+                                         uint32_t chunkBase, uint32_t bs, RunCarry &cy, uint32_t lane,
+                                         const RunOut &o) {
   bool f[R];
 #pragma unroll
   for (int s = 0; s < R; ++s) {
@@ -193,24 +195,20 @@ __device__ __forceinline__ void scanRuns(const uint64_t (&h)[R], const uint32_t 
     if (static_cast<uint32_t>(s) * 64 >= cnt) break;
     const bool valid = e < cnt;
     const uint32_t r = rk[s];
-    const uint32_t key = lk[s];
-    const double own = valid ? lu.prob[key] : 0.0;
-    const uint64_t ownH2 = valid ? lu.h2[key] : 0ull;
-    double v = own;
-    uint32_t rep = r;
-    uint64_t hf = ownH2;
+    const uint4 inf = valid ? o.info[r] : make_uint4(0u, 0u, 0u, 0u);
+    const uint32_t ownH2 = inf.z;
+    double v = infoProb(inf);
+    uint32_t hf = ownH2;
     int fl = f[s] ? 1 : 0;
-    const uint32_t nz = f[s] ? lu.nnz[key] : 0u;
+    const uint32_t nz = f[s] ? inf.w : 0u;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
       const double pv = __shfl_up(v, d);
-      const uint32_t pr = __shfl_up(rep, d);
-      const uint64_t ph2 = __shfl_up(hf, d);
+      const uint32_t ph2 = __shfl_up(hf, d);
       const int pf = __shfl_up(fl, d);
       if (lane >= static_cast<uint32_t>(d)) {
         if (!fl) {
           v = xmerge(pv, v);
-          rep = pr;
           hf = ph2;
         }
         fl |= pf;
@@ -218,7 +216,6 @@ __device__ __forceinline__ void scanRuns(const uint64_t (&h)[R], const uint32_t 
     }
     if (!fl) {  // the run began before this slot
       v = xmerge(cy.p, v);
-      rep = cy.rep;
       hf = cy.h2;
     }
     const uint64_t bal = __ballot(f[s]);
@@ -241,7 +238,6 @@ __device__ __forceinline__ void scanRuns(const uint64_t (&h)[R], const uint32_t 
       if (e + 1 == cnt || nf) o.runP[bs + runIdx] = v;
     }
     cy.p = __shfl(v, 63);
-    cy.rep = __shfl(rep, 63);
     cy.h2 = __shfl(hf, 63);
     cy.runs += __popcll(bal);
     cy.nnzSum += __shfl(incl, 63);
@@ -606,17 +602,20 @@ __device__ __forceinline__ void forEachTarget(const LayerMasks &lm, uint32_t l,
 // The node then takes a slot of its sort bucket (first radix digit = top
 // bucketBits hash bits): slot pos of bucket b is pkey/prank[b * kBucketCap
 // + pos], pos from an atomic counter per bucket; nodes beyond a bucket's
-// kBucketCap slots go to the overflow list.  The first kStashSlots targets
-// of every node are also stored as one row per node (`stash`), so the emit
-// stage copies a class's targets from its representative's row instead of
-// gathering them from S one column at a time: each wave assembles its 64
-// rows in LDS and writes them out contiguously.
-constexpr uint32_t kStashSlots = 16;
+// kBucketCap slots go to the overflow list.  In circuits whose targets fit
+// 16 bits (at most kStashMaxCols columns), the first kStashTargets targets
+// of every node are also stored as one 64-byte row per node (`stash`, 16-bit
+// entries), so the emit stage copies a class's targets from its
+// representative's row instead of gathering them from S one column at a
+// time: each wave assembles its 64 rows in LDS and writes them out
+// contiguously.
+constexpr uint32_t kStashWords = 16;                  // 32-bit words per row
+constexpr uint32_t kStashTargets = 2 * kStashWords;   // 16-bit targets per row
+constexpr uint32_t kStashMaxCols = 65536 / 32;
 constexpr uint32_t kBucketCap = kBucketItemsPerWave;
 
 struct ClassifyOut {
-  uint64_t *hash, *h2;
-  uint32_t *nnz;
+  uint4 *info;            // per node: nodeInfo(probability, second hash, class size)
   uint32_t *bucketCount;  // bucket b's counter at b * countStride
   uint32_t countStride;
   uint64_t *pkey;         // bucket slots
@@ -625,15 +624,16 @@ struct ClassifyOut {
   uint32_t *ovRank, *ovBucket;
   uint32_t *flags;
   uint32_t *stash;
+  uint32_t stashMaxCols;  // circuits with more columns get no rows (0: none do)
 };
 
 __global__ __launch_bounds__(kClassBlock) void classifyKernel(
     const VDesc *__restrict__ desc, const uint2 *__restrict__ tiles, const uint32_t *__restrict__ rankLayer,
-    const int2 *__restrict__ band, const uint32_t *__restrict__ S,
+    const int2 *__restrict__ band, const uint32_t *__restrict__ S, const double *__restrict__ rankProb,
     ClassifyOut o) {
-  // per wave: 64 rows of kStashSlots words, padded to kStashSlots + 1 so
+  // per wave: 64 rows of kStashWords words, padded to kStashWords + 1 so
   // that a lane's writes and the row-major read-out avoid bank conflicts
-  __shared__ uint32_t rows[kClassBlock / 64][64 * (kStashSlots + 1)];
+  __shared__ uint32_t rows[kClassBlock / 64][64 * (kStashWords + 1)];
   __shared__ uint32_t maskTab[kMaskTableWords];
   const uint2 t = tiles[blockIdx.x];
   const VDesc d = desc[t.x];
@@ -646,7 +646,8 @@ __global__ __launch_bounds__(kClassBlock) void classifyKernel(
     buildMaskTable(maskTab, l0, l1 - l0 + 1, d.numCols, d.maskWords, lm.band);
     lm.tab = maskTab;
   }
-  uint32_t *row = rows[wave] + lane * (kStashSlots + 1);
+  uint16_t *row = reinterpret_cast<uint16_t *>(rows[wave] + lane * (kStashWords + 1));
+  const bool stashed = d.numCols <= o.stashMaxCols;
   if (i < d.numRanks) {
     const uint64_t gi = d.rankOff + i;
     const uint32_t *Sv = S + d.sOff + i;
@@ -655,15 +656,13 @@ __global__ __launch_bounds__(kClassBlock) void classifyKernel(
     forEachTarget(lm, rankLayer[gi], Sv, d.colStride, [&](uint32_t tg) {
       h = (h ^ tg) * kFnvPrime;
       h2 = mix2(h2, tg);
-      if (cnt < kStashSlots) row[cnt] = tg;
+      if (cnt < kStashTargets) row[cnt] = static_cast<uint16_t>(tg);
       cnt++;
     });
     if (cnt && cnt <= 32)
       for (uint32_t k = cnt; k < 32; ++k) h = (h ^ 0xFFFFFFFFull) * kFnvPrime;
     h2 = mix2(h2, cnt);
-    o.hash[gi] = h;
-    o.h2[gi] = h2;
-    o.nnz[gi] = cnt;
+    o.info[gi] = nodeInfo(rankProb[gi], static_cast<uint32_t>(h2 ^ (h2 >> 32)), cnt);
     if (cnt) {
       const uint32_t b = d.bucketBase + (d.bucketBits ? static_cast<uint32_t>(h >> (64 - d.bucketBits)) : 0u);
       const uint32_t pos = atomicAdd(&o.bucketCount[static_cast<uint64_t>(b) * o.countStride], 1u);
@@ -681,14 +680,14 @@ __global__ __launch_bounds__(kClassBlock) void classifyKernel(
   // the wave's rows, row-major and contiguous from its first rank (a wave's
   // LDS accesses are in order, so no barrier is needed)
   const uint32_t waveFirst = t.y + wave * 64;
-  if (waveFirst < d.numRanks) {
+  if (stashed && waveFirst < d.numRanks) {
     const uint32_t nRows = d.numRanks - waveFirst < 64 ? d.numRanks - waveFirst : 64;
-    uint32_t *dst = o.stash + (d.rankOff + waveFirst) * kStashSlots;
+    uint32_t *dst = o.stash + (d.rankOff + waveFirst) * kStashWords;
 #pragma unroll
-    for (uint32_t k = 0; k < kStashSlots; ++k) {
-      const uint32_t e = k * 64 + lane;  // element of the 64 x kStashSlots block
-      const uint32_t r = e / kStashSlots, c = e % kStashSlots;
-      if (r < nRows) dst[e] = rows[wave][r * (kStashSlots + 1) + c];
+    for (uint32_t k = 0; k < kStashWords; ++k) {
+      const uint32_t e = k * 64 + lane;  // element of the 64 x kStashWords block
+      const uint32_t r = e / kStashWords, c = e % kStashWords;
+      if (r < nRows) dst[e] = rows[wave][r * (kStashWords + 1) + c];
     }
   }
 }
@@ -706,8 +705,8 @@ __device__ void bucketWave(uint32_t b, uint32_t bs, uint32_t cnt, uint32_t lane,
     r[s] = e < cnt ? o.sortedRank[bs + e] : NONE;
   }
   waveBitonicSort<R>(h, r, lane);
-  RunCarry cy{0.0, 0ull, 0ull, 0u, 0u, 0u, 0u};
-  scanRuns<R>(h, r, r, cnt, 0u, bs, cy, lane, o, RunLookup{o.prob, o.h2, o.nnz});
+  RunCarry cy{0.0, 0ull, 0u, 0u, 0u, 0u};
+  scanRuns<R>(h, r, cnt, 0u, bs, cy, lane, o);
   if (lane == 0) {
     runCount[b] = cy.runs;
     runNnz[b] = cy.nnzSum;
@@ -722,6 +721,7 @@ struct BucketIo {
   const uint32_t *ovRank, *ovBucket;
   uint32_t *runCount, *runNnz, *itemCount;  // per bucket
   uint32_t *slotBase;     // per bucket: where its sorted items and runs start
+  uint32_t *wgSum;        // per workgroup: its buckets' runs, targets, items
   uint64_t *scratchKey;   // sort workspace for oversized buckets (2 * arenaSize)
   uint32_t *scratchRank;
   uint32_t arenaBase;     // numBuckets * kBucketCap: oversized buckets' items and runs follow
@@ -782,7 +782,7 @@ __device__ void bigBucket(uint32_t b, uint32_t cnt, const BucketIo &io, const Ru
     }
   const uint32_t base = io.arenaBase + off;
   if (threadIdx.x < 64) {
-    RunCarry cy{0.0, 0ull, 0ull, 0u, 0u, 0u, 0u};
+    RunCarry cy{0.0, 0ull, 0u, 0u, 0u, 0u};
     for (uint32_t c0 = 0; c0 < cnt; c0 += 64) {
       uint64_t h[1];
       uint32_t r[1];
@@ -790,7 +790,7 @@ __device__ void bigBucket(uint32_t b, uint32_t cnt, const BucketIo &io, const Ru
       h[0] = e < cnt ? K[e] : ~0ull;
       r[0] = e < cnt ? Rk[e] : NONE;
       const uint32_t n = cnt - c0 < 64 ? cnt - c0 : 64;
-      scanRuns<1>(h, r, r, n, c0, base, cy, lane, o, RunLookup{o.prob, o.h2, o.nnz});
+      scanRuns<1>(h, r, n, c0, base, cy, lane, o);
     }
     if (lane == 0) {
       io.runCount[b] = cy.runs;
@@ -841,20 +841,40 @@ __global__ __launch_bounds__(256) void bucketKernel(uint32_t numBuckets, BucketI
   __syncthreads();
   const uint32_t n = nBig;
   for (uint32_t k = 0; k < n; ++k) bigBucket(bigB[k], bigCnt[k], io, o);
+  // the workgroup's totals, from which the emit stage forms the offsets of
+  // its buckets (the counts were written by this workgroup's waves)
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const uint32_t *a = threadIdx.x == 0 ? io.runCount : threadIdx.x == 1 ? io.runNnz : io.itemCount;
+    uint32_t t = 0;
+    for (uint32_t w = 0; w < 4; ++w)
+      if (blockIdx.x * 4 + w < numBuckets) t += a[blockIdx.x * 4 + w];
+    io.wgSum[3 * blockIdx.x + threadIdx.x] = t;
+  }
 }
 
 // Writes the compact output: per class (run) its probability, hash, target
 // offset and first member, and its targets (from the representative's
-// stashed row, or from S for classes above kStashSlots targets).  One wave
-// per bucket, one lane per run; block 0 also writes each circuit's class and
-// target bounds.
+// stashed row, or from S for classes above kStashTargets targets).  One wave
+// per bucket, one lane per run, four buckets per workgroup (the bucket
+// kernel's grouping).  The buckets' class, target and member offsets come
+// either from the scans (classStart, tgtStart, memberStart) or, when wgSum
+// is set, from the bucket kernel's per-workgroup totals: workgroup g adds up
+// those of workgroups 0..g-1 (fine for up to kEmitPrefixGroups workgroups)
+// and writes memberStart for its buckets.  Each circuit's class and target
+// bounds go to vBounds from the workgroup holding its first bucket.
+constexpr uint32_t kEmitPrefixGroups = 2048;
+
 struct EmitIn {
   uint32_t numVariants, numBuckets;
-  const uint32_t *slotBase, *runCount, *classStart, *tgtStart, *memberStart;
+  const uint32_t *wgSum;  // or nullptr: offsets from the scan arrays
+  const uint32_t *slotBase, *runCount, *itemCount, *classStart, *tgtStart;
+  uint32_t *memberStart;
   const uint32_t *runRank, *runTgtPre, *runMember;
   const double *runP;
   const uint64_t *runHash;
-  const uint32_t *rankLayer, *S, *nnz, *stash;
+  const uint32_t *runNnz, *rankLayer, *S, *stash;
+  uint32_t stashMaxCols;
   const int2 *band;
 };
 struct EmitOut {
@@ -868,16 +888,51 @@ struct EmitOut {
 };
 
 __global__ __launch_bounds__(256) void emitKernel(const VDesc *__restrict__ desc, EmitIn in, EmitOut out) {
-  const uint32_t lane = threadIdx.x & 63;
-  if (blockIdx.x == 0 && threadIdx.x < 64) {
-    for (uint32_t v = lane; v <= in.numVariants; v += 64) {
-      const uint32_t bb = v < in.numVariants ? desc[v].bucketBase : in.numBuckets;
-      out.vBounds[2 * v] = in.classStart[bb];
-      out.vBounds[2 * v + 1] = in.tgtStart[bb];
+  __shared__ uint32_t red[3][4];
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t g = blockIdx.x;
+  // offsets (classes, targets, members) of this workgroup's first bucket
+  uint32_t base[3] = {0u, 0u, 0u};
+  if (in.wgSum) {
+    uint32_t part[3] = {0u, 0u, 0u};
+    for (uint32_t i = threadIdx.x; i < g; i += blockDim.x)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) part[q] += in.wgSum[3 * i + q];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const uint32_t t = waveInclusiveSum(part[q], lane);
+      if (lane == 63) red[q][wave] = t;
     }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 3; ++q) base[q] = red[q][0] + red[q][1] + red[q][2] + red[q][3];
   }
-  const uint32_t b = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+  const uint32_t b = __builtin_amdgcn_readfirstlane(g * 4 + wave);
+  if (in.numBuckets == 0) {
+    if (g == 0 && threadIdx.x == 0) {
+      for (uint32_t v = 0; v <= in.numVariants; ++v) out.vBounds[2 * v] = out.vBounds[2 * v + 1] = 0u;
+      out.off[0] = 0u;
+      out.member[0] = 0u;
+    }
+    return;
+  }
   if (b >= in.numBuckets) return;
+  uint32_t cb, tb, mb;
+  if (in.wgSum) {
+    cb = base[0];
+    tb = base[1];
+    mb = base[2];
+    for (uint32_t w = 0; w < wave; ++w) {
+      cb += in.runCount[g * 4 + w];
+      tb += in.runNnz[g * 4 + w];
+      mb += in.itemCount[g * 4 + w];
+    }
+    if (lane == 0) in.memberStart[b] = mb;
+  } else {
+    cb = in.classStart[b];
+    tb = in.tgtStart[b];
+    mb = in.memberStart[b];
+  }
   // circuit owning bucket b
   uint32_t lo = 0, hi = in.numVariants - 1;
   while (lo < hi) {
@@ -886,8 +941,20 @@ __global__ __launch_bounds__(256) void emitKernel(const VDesc *__restrict__ desc
     else hi = mid - 1;
   }
   const VDesc d = desc[lo];
-  const uint32_t nR = in.runCount[b], bs = in.slotBase[b], cb = in.classStart[b], tb = in.tgtStart[b];
-  const uint32_t mb = in.memberStart[b];
+  const uint32_t nR = in.runCount[b], bs = in.slotBase[b];
+  if (lane == 0) {
+    if (d.bucketBase == b) {  // the circuit's first bucket: its bounds
+      out.vBounds[2 * lo] = cb;
+      out.vBounds[2 * lo + 1] = tb;
+    }
+    if (b + 1 == in.numBuckets) {  // totals
+      const uint32_t C = cb + nR, T = tb + in.runNnz[b], M = mb + in.itemCount[b];
+      out.vBounds[2 * in.numVariants] = C;
+      out.vBounds[2 * in.numVariants + 1] = T;
+      out.off[C] = T;
+      out.member[C] = M;
+    }
+  }
   for (uint32_t r = lane; r < nR; r += 64) {
     const uint32_t c = cb + r;
     const uint32_t rk = in.runRank[bs + r];
@@ -896,22 +963,23 @@ __global__ __launch_bounds__(256) void emitKernel(const VDesc *__restrict__ desc
     out.hash[c] = in.runHash[bs + r];
     out.off[c] = static_cast<uint32_t>(pos);
     out.member[c] = mb + in.runMember[bs + r];
-    const uint32_t n = in.nnz[rk];
-    if (n <= kStashSlots) {
-      // the representative's stashed row (one 64-byte line)
-      const uint4 *src = reinterpret_cast<const uint4 *>(in.stash + static_cast<uint64_t>(rk) * kStashSlots);
-      uint32_t tg[kStashSlots];
+    // class size: the gap to the next run's target offset
+    const uint32_t n = (r + 1 < nR ? in.runTgtPre[bs + r + 1] : in.runNnz[b]) - in.runTgtPre[bs + r];
+    if (n <= kStashTargets && d.numCols <= in.stashMaxCols) {
+      // the representative's stashed row (one 64-byte line, 16-bit targets)
+      const uint4 *src = reinterpret_cast<const uint4 *>(in.stash + static_cast<uint64_t>(rk) * kStashWords);
+      uint32_t w[kStashWords];
 #pragma unroll
-      for (uint32_t q = 0; q < kStashSlots / 4; ++q) {
+      for (uint32_t q = 0; q < kStashWords / 4; ++q) {
         const uint4 x = src[q];
-        tg[4 * q] = x.x;
-        tg[4 * q + 1] = x.y;
-        tg[4 * q + 2] = x.z;
-        tg[4 * q + 3] = x.w;
+        w[4 * q] = x.x;
+        w[4 * q + 1] = x.y;
+        w[4 * q + 2] = x.z;
+        w[4 * q + 3] = x.w;
       }
 #pragma unroll
-      for (uint32_t j = 0; j < kStashSlots; ++j)
-        if (j < n && pos + j < out.capTargets) out.targets[pos + j] = tg[j];
+      for (uint32_t j = 0; j < kStashTargets; ++j)
+        if (j < n && pos + j < out.capTargets) out.targets[pos + j] = (w[j / 2] >> (16 * (j % 2))) & 0xFFFFu;
       pos += n;
     } else {
       const uint32_t i = static_cast<uint32_t>(rk - d.rankOff);
@@ -922,10 +990,6 @@ __global__ __launch_bounds__(256) void emitKernel(const VDesc *__restrict__ desc
       });
     }
     if (pos > out.capTargets) atomicOr(&out.flags[1], 1u);
-  }
-  if (b == in.numBuckets - 1 && lane == 0) {
-    out.off[in.classStart[in.numBuckets]] = in.tgtStart[in.numBuckets];
-    out.member[in.classStart[in.numBuckets]] = in.memberStart[in.numBuckets];
   }
 }
 
@@ -1177,12 +1241,13 @@ struct Pipeline::Impl {
   // issues one per node
   uint32_t countStride = 1;
 
-  // per node: stash, hash, h2, nnz, overflow list (ovKey/ovRank/ovBucket);
+  // per node: stash, info (probability, second hash, class size), overflow
+  // list (ovKey/ovRank/ovBucket);
   // per bucket slot (numBuckets * kBucketCap, plus a numRanks arena for
   // oversized buckets): pkey, prank (sorted in place) and the run arrays;
   // per bucket: counters, slotBase, item/run/target counts and their scans
-  DevBuf band, S, stash, hash, h2, nnz, counters, bucketCnt, ovKey, ovRank, ovBucket, pkey, prank,
-      runRank, runP, runHash, runTgtPre, runMember, runCount, runNnz, itemCount, slotBase, classStart,
+  DevBuf band, S, stash, info, counters, bucketCnt, ovKey, ovRank, ovBucket, pkey, prank,
+      runRank, runP, runHash, runTgtPre, runMember, runCount, runNnz, itemCount, slotBase, wgSum, classStart,
       tgtStart, memberStart, outP, outOff, outHash, outMember, outTargets, scratchKey, scratchRank,
       scanTmp, vBounds, members;
   uint64_t capTargets = 0;
@@ -1371,10 +1436,8 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   const uint64_t R = std::max<uint64_t>(m.R, 1), B = m.B;
   m.band.ensure(8 * std::max<uint32_t>(m.Ccols, 1), frozen, "columns");
   m.S.ensure(4 * std::max<uint64_t>(m.Swords, 1), frozen, "sensitivity matrix");
-  m.hash.ensure(8 * R, frozen, "nodes");
-  m.stash.ensure(4ull * kStashSlots * R, frozen, "nodes");
-  m.h2.ensure(8 * R, frozen, "nodes");
-  m.nnz.ensure(4 * R, frozen, "nodes");
+  m.info.ensure(16 * R, frozen, "nodes");
+  m.stash.ensure(4ull * kStashWords * R, frozen, "nodes");
   m.ovKey.ensure(8 * R, frozen, "nodes");
   m.ovRank.ensure(4 * R, frozen, "nodes");
   m.ovBucket.ensure(4 * R, frozen, "nodes");
@@ -1399,6 +1462,7 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   m.runNnz.ensure(4 * (B + 1), frozen, "buckets");
   m.itemCount.ensure(4 * (B + 1), frozen, "buckets");
   m.slotBase.ensure(4 * (B + 1), frozen, "buckets");
+  m.wgSum.ensure(4 * 3 * ((B + 3) / 4 + 1), frozen, "buckets");
   m.classStart.ensure(4 * (B + 1), frozen, "buckets");
   m.tgtStart.ensure(4 * (B + 1), frozen, "buckets");
   m.memberStart.ensure(4 * (B + 1), frozen, "buckets");
@@ -1464,6 +1528,7 @@ void Pipeline::run(bool timed) {
   };
   const uint32_t B = m.B;
   m.timedRun = timed;
+  const uint32_t stashMaxCols = gStash.load() ? kStashMaxCols : 0u;
 
   // Five launches: sweep (block 0 also zeroes the flags), classify (hash,
   // stash, bucket slots), buckets (sort + merge; counters reset for the next
@@ -1494,16 +1559,15 @@ void Pipeline::run(bool timed) {
   }
   mark(1);
   if (m.nTiles) {
-    ClassifyOut co{m.hash.as<uint64_t>(), m.h2.as<uint64_t>(), m.nnz.as<uint32_t>(), m.bucketCount(),
+    ClassifyOut co{m.info.as<uint4>(), m.bucketCount(),
                    m.countStride, m.pkey.as<uint64_t>(), m.prank.as<uint32_t>(), m.ovKey.as<uint64_t>(),
-                   m.ovRank.as<uint32_t>(), m.ovBucket.as<uint32_t>(), m.flags(), m.stash.as<uint32_t>()};
+                   m.ovRank.as<uint32_t>(), m.ovBucket.as<uint32_t>(), m.flags(), m.stash.as<uint32_t>(),
+                   stashMaxCols};
     classifyKernel<<<m.nTiles, kClassBlock, 0, st>>>(desc, tiles, rankLayer, m.band.as<int2>(),
-                                                     m.S.as<uint32_t>(), co);
+                                                     m.S.as<uint32_t>(), rankProb, co);
   }
   mark(2);
-  RunOut o{rankProb,
-           m.h2.as<uint64_t>(),
-           m.nnz.as<uint32_t>(),
+  RunOut o{m.info.as<uint4>(),
            m.prank.as<uint32_t>(),
            m.runRank.as<uint32_t>(),
            m.runP.as<double>(),
@@ -1514,17 +1578,24 @@ void Pipeline::run(bool timed) {
   BucketIo bio{m.bucketCount(), m.countStride, m.pkey.as<uint64_t>(), m.ovKey.as<uint64_t>(),
                m.ovRank.as<uint32_t>(), m.ovBucket.as<uint32_t>(), m.runCount.as<uint32_t>(),
                m.runNnz.as<uint32_t>(), m.itemCount.as<uint32_t>(), m.slotBase.as<uint32_t>(),
-               m.scratchKey.as<uint64_t>(), m.scratchRank.as<uint32_t>(), B * kBucketCap};
+               m.wgSum.as<uint32_t>(), m.scratchKey.as<uint64_t>(), m.scratchRank.as<uint32_t>(),
+               B * kBucketCap};
   if (B) bucketKernel<<<(B + 3) / 4, 256, 0, st>>>(B, bio, o);
   mark(3);
-  ScanArrays sa{{m.runCount.as<uint32_t>(), m.runNnz.as<uint32_t>(), m.itemCount.as<uint32_t>()},
-                {m.classStart.as<uint32_t>(), m.tgtStart.as<uint32_t>(), m.memberStart.as<uint32_t>()}};
-  m.scan(sa, B);
-  EmitIn ein{m.V, B, m.slotBase.as<uint32_t>(), m.runCount.as<uint32_t>(), m.classStart.as<uint32_t>(),
+  // bucket offsets: inside emit from the workgroup totals, or by the scans
+  // when there are too many workgroups for that
+  const bool prefixInEmit = (B + 3) / 4 <= kEmitPrefixGroups;
+  if (!prefixInEmit) {
+    ScanArrays sa{{m.runCount.as<uint32_t>(), m.runNnz.as<uint32_t>(), m.itemCount.as<uint32_t>()},
+                  {m.classStart.as<uint32_t>(), m.tgtStart.as<uint32_t>(), m.memberStart.as<uint32_t>()}};
+    m.scan(sa, B);
+  }
+  EmitIn ein{m.V, B, prefixInEmit ? m.wgSum.as<uint32_t>() : nullptr, m.slotBase.as<uint32_t>(),
+             m.runCount.as<uint32_t>(), m.itemCount.as<uint32_t>(), m.classStart.as<uint32_t>(),
              m.tgtStart.as<uint32_t>(), m.memberStart.as<uint32_t>(), m.runRank.as<uint32_t>(),
              m.runTgtPre.as<uint32_t>(), m.runMember.as<uint32_t>(), m.runP.as<double>(),
-             m.runHash.as<uint64_t>(), rankLayer, m.S.as<uint32_t>(), m.nnz.as<uint32_t>(),
-             m.stash.as<uint32_t>(), m.band.as<int2>()};
+             m.runHash.as<uint64_t>(), m.runNnz.as<uint32_t>(), rankLayer, m.S.as<uint32_t>(),
+             m.stash.as<uint32_t>(), stashMaxCols, m.band.as<int2>()};
   EmitOut eout{m.outP.as<double>(), m.outOff.as<uint32_t>(), m.outHash.as<uint64_t>(), m.outMember.as<uint32_t>(),
                m.outTargets.as<uint32_t>(), m.capTargets, m.flags(), m.vBounds.as<uint32_t>()};
   emitKernel<<<std::max<uint32_t>((B + 3) / 4, 1), 256, 0, st>>>(desc, ein, eout);
@@ -1669,15 +1740,17 @@ double Pipeline::benchDevice(uint32_t iters) {
 }
 
 void setBucketFill(uint32_t fill) { gBucketFill.store(std::max<uint32_t>(fill, 1)); }
+void setStash(bool on) { gStash.store(on ? 1u : 0u); }
+bool stash() { return gStash.load() != 0; }
 uint32_t bucketFill() { return gBucketFill.load(); }
 
 uint64_t Pipeline::deviceBytes() const {
   const Impl &m = *impl_;
-  const DevBuf *bufs[] = {&m.in,        &m.band,     &m.S,          &m.stash,     &m.hash,
-                          &m.h2,        &m.nnz,        &m.counters,   &m.bucketCnt, &m.ovKey,
+  const DevBuf *bufs[] = {&m.in,        &m.band,     &m.S,          &m.stash,     &m.info,
+                          &m.counters,   &m.bucketCnt, &m.ovKey,
                           &m.ovRank,    &m.ovBucket,   &m.pkey,       &m.prank,     &m.runRank,
                           &m.runP,      &m.runHash,    &m.runTgtPre,  &m.runMember, &m.runCount,
-                          &m.runNnz,    &m.itemCount,  &m.slotBase,   &m.classStart, &m.tgtStart,
+                          &m.runNnz,    &m.itemCount,  &m.slotBase, &m.wgSum,   &m.classStart, &m.tgtStart,
                           &m.memberStart, &m.outP,     &m.outOff,     &m.outHash,   &m.outMember,
                           &m.outTargets, &m.scratchKey, &m.scratchRank, &m.scanTmp, &m.vBounds,
                           &m.members};

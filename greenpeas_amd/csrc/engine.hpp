@@ -96,6 +96,11 @@ class Pipeline {
 // digit aims at (the digit width is chosen per circuit from it).
 void setBucketFill(uint32_t fill);
 uint32_t bucketFill();
+// Whether classify stores each node's first targets for emit (on by
+// default; off, emit gathers every class's targets from S -- the path taken
+// for circuits with more than 65536 targets).  Read at each run.
+void setStash(bool on);
+bool stash();
 
 // Segmented XOR-merge kernels (differentiable path); pointers are device
 // pointers, stream is a hipStream_t.

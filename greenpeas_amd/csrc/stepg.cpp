@@ -401,17 +401,19 @@ void buildImage(const Stepg &g, StepgImage &img) {
 
 uint64_t imageDeviceBytes(const StepgImage &img) {
   const uint64_t R = img.numRanks;
+  // sort buckets at the default first-digit fill (64 nodes per bucket),
+  // 256 slots each, plus an arena of R slots for oversized buckets
+  uint64_t B = 1;
+  while (B * 64 < R) B <<= 1;
+  const uint64_t slots = B * 256 + R;
   uint64_t b = 0;
   b += static_cast<uint64_t>(kSweepLeadPad + img.numLayers) * img.layerStride * 16ull + img.seedVals.size() * 4ull;
-  b += img.numLayers * ((img.numCols + 31) / 32) * 4ull;  // per-layer column masks
-  b += img.numCols * 12ull + R * 12ull;  // bands, rank layer + prob
-  b += (R + kSinkSlots) * img.numCols * 4ull;  // sensitivity matrix (+ per-column sink slots)
-  b += R * (8 + 8 + 4 + 4);               // hash, h2, nnz, class-of-rank
-  b += R * 64;                            // stashed target rows (16 per node)
-  b += R * (8 + 4) * 2;                   // partitioned keys/ranks + scratch
-  b += R * (4 + 8 + 8 + 4 + 4 + 4);       // runs
-  b += R * (8 + 4 + 8 + 4) + R * 32 * 4;  // outputs (targets sized 32 per class)
-  b += (R / 32 + 64) * 4 * 8;             // bucket arrays
+  b += img.numCols * 16ull + R * 12ull;         // column bands, rank layer + prob
+  b += (R + kSinkSlots) * img.numCols * 4ull;   // sensitivity matrix (+ per-column sink slots)
+  b += R * (16 + 64 + 16 + 24);                 // node records, target rows, overflow list, sort workspace
+  b += slots * (8 + 4 + 4 + 8 + 8 + 4 + 4);     // bucket slots: keys, ranks, run arrays
+  b += (B + 1) * 4 * 9;                         // per-bucket counters, counts and offsets
+  b += R * (8 + 4 + 8 + 4) + R * 16 * 4;        // outputs (targets sized 16 per class at first)
   return b;
 }
 

@@ -58,7 +58,14 @@ def estimate_bytes(text: str, level: int = 2) -> int:
     nodes = (ticks + 1) * (maxq + 1) * k
     ranks = nodes * 3 // 5 + 64
     cols = (dets + obs + 31) // 32
-    return int(ranks * (cols * 4 + 244) + nodes * 24 + (ticks + 1) * cols * 4 + 65536)
+    # sort buckets (64 nodes each at the default fill, 256 slots per bucket,
+    # plus an arena of one slot per node), 40 bytes per slot
+    buckets = 1
+    while buckets * 64 < ranks:
+        buckets <<= 1
+    slots = buckets * 256 + ranks
+    return int(ranks * (cols * 4 + 232) + slots * 40 + cols * 4096 + nodes * 24
+               + (ticks + 1) * cols * 4 + 65536)
 
 
 @dataclass

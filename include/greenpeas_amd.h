@@ -165,7 +165,11 @@ int gp_estimate_device_bytes(const char *circuit_text, int correlation_level,
  *   next load).
  *   "sweep_rpt": fewest STEPG records per thread of the sensitivity sweep
  *   (1, 2, 4 or 8; default 1, raised as needed to fit a layer in 1024
- *   threads; read when a circuit is parsed). */
+ *   threads; read when a circuit is parsed).
+ *   "stash": 1 (default) stores each node's first 32 targets as a 16-bit
+ *   row for the emit stage; 0 makes emit gather every class's targets from
+ *   the sensitivity matrix (what circuits above 65536 targets always do);
+ *   read at each compile. */
 int gp_set_tuning(const char *name, uint32_t value);
 int gp_get_tuning(const char *name, uint32_t *value);
 

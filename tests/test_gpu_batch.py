@@ -72,3 +72,17 @@ def test_graph_replays_are_identical():
         _ = solo(read_circuit("surface3"))  # eager work on another stream
         again = bc.compile()
         assert again == first
+
+
+def test_batch_with_many_buckets_takes_the_scan_path():
+    """Three bb12 circuits (12288 sort buckets): the bucket offsets come from
+    the scan kernels instead of inside the emit stage; every result is still
+    bit-identical to its solo compile (which takes the in-emit path)."""
+    text = read_circuit("bb12")
+    variants = [text, SurfaceCode(5).memory_text(5, noise=Noise(rng=np.random.default_rng(9))), text, text]
+    bc = BatchCompiler()
+    bc.set_circuits(variants)
+    dems = bc.compile()
+    ref12 = solo(text)
+    assert dems[0] == ref12 and dems[2] == ref12 and dems[3] == ref12
+    assert dems[1] == solo(variants[1])

@@ -199,3 +199,19 @@ def test_sweep_records_per_thread_knob(oracle, rpt):
             assert np.array_equal(idx, idx0) and np.array_equal(s, s0)
     finally:
         autotune.set_tuning("sweep_rpt", old)
+
+
+def test_emit_without_target_rows(oracle):
+    """With the stored target rows switched off, emit gathers every class's
+    targets from the sensitivity matrix (the path of circuits above 65536
+    targets): same DEMs, bit for bit."""
+    from greenpeas_amd import autotune
+
+    texts = [read_circuit(n) for n in ("surface5", "bb12")]
+    ref = [get_driver(t).compile_detector_error_model() for t in texts]
+    try:
+        autotune.set_tuning("stash", 0)
+        for t, dem0 in zip(texts, ref):
+            assert get_driver(t).compile_detector_error_model() == dem0
+    finally:
+        autotune.set_tuning("stash", 1)

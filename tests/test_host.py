@@ -102,6 +102,10 @@ def test_tuning_knobs_roundtrip():
         for r in (1, 2, 4, 8):
             autotune.set_tuning("sweep_rpt", r)
             assert autotune.get_tuning("sweep_rpt") == r
+        autotune.set_tuning("stash", 0)
+        assert autotune.get_tuning("stash") == 0
+        autotune.set_tuning("stash", 1)
+        assert autotune.get_tuning("stash") == 1
         with pytest.raises(ValueError):
             autotune.set_tuning("sweep_rpt", 3)
         with pytest.raises(ValueError):
