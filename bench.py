@@ -161,6 +161,7 @@ def main() -> int:
     compile_us.sort()
     trimmed = compile_us[1:-1]
     log(f"compile (upload+device+download): {statistics.mean(trimmed):.1f} us")
+    h2h = driver.timings()  # the last of those compiles, by stage
 
     # device-resident throughput
     driver.bench_device(max(args.warmup, 1))
@@ -231,6 +232,7 @@ def main() -> int:
         "sort_merge": {"us": sortmerge_us, "algorithmic_GBs": sortmerge_gbs,
                        "frac_hbm": sortmerge_gbs / HBM_PEAK_GBS},
         "compile_us_host_to_host": statistics.mean(trimmed),
+        "host_to_host_last_us": {k: h2h[k] for k in ("upload", "download", "total")},
         "reference_compile_us_bb12_L2": REFERENCE_BB12_L2_US,
         "speedup_vs_reference_compile": REFERENCE_BB12_L2_US / statistics.mean(trimmed),
         "speedup_vs_cpu_baseline": None if cpu is None else value / cpu["value"],

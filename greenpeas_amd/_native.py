@@ -26,7 +26,7 @@ EXPORTS = (
     "gp_build_stepg_host",
     "gp_batch_create", "gp_batch_destroy", "gp_batch_set_circuits", "gp_batch_compile",
     "gp_batch_device_us", "gp_batch_dem_shape", "gp_batch_dem_copy", "gp_batch_device_bytes",
-    "gp_estimate_device_bytes", "gp_set_tuning", "gp_get_tuning",
+    "gp_estimate_device_bytes", "gp_circuit_summary", "gp_set_tuning", "gp_get_tuning",
     "gp_grad_plan_create", "gp_grad_plan_destroy", "gp_grad_plan_sizes", "gp_grad_plan_dem_structure",
     "gp_grad_plan_device_arrays", "gp_grad_plan_copy_arrays", "gp_xor_merge_forward",
     "gp_xor_merge_backward",
@@ -82,6 +82,7 @@ def load() -> ctypes.CDLL:
     sig("gp_batch_dem_copy", [P, u32, P, P, P, P])
     sig("gp_batch_device_bytes", [P, pu64])
     sig("gp_estimate_device_bytes", [cs, i32, pu64])
+    sig("gp_circuit_summary", [cs, P])
     sig("gp_set_tuning", [cs, u32])
     sig("gp_get_tuning", [cs, pu32])
     sig("gp_grad_plan_create", [cs, i32, i32, PP])

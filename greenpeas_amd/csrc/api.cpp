@@ -492,6 +492,18 @@ int gp_batch_device_bytes(const gp_batch *batch, uint64_t *bytes) {
   });
 }
 
+int gp_circuit_summary(const char *circuit_text, uint64_t summary[4]) {
+  return guarded([&] {
+    requireText(circuit_text);
+    if (!summary) throw InvalidArgument("summary is NULL");
+    const gpa::CircuitSummary s = gpa::summarize(circuit_text);
+    summary[0] = s.layers;
+    summary[1] = s.qubits;
+    summary[2] = s.detectors;
+    summary[3] = s.observables;
+  });
+}
+
 int gp_estimate_device_bytes(const char *circuit_text, int correlation_level, uint64_t *bytes) {
   return guarded([&] {
     requireText(circuit_text);

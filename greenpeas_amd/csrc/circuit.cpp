@@ -244,4 +244,101 @@ Circuit Circuit::parse(std::string_view text) {
   return c;
 }
 
+namespace {
+
+bool startsWith(std::string_view s, std::string_view p) { return s.substr(0, p.size()) == p; }
+
+// Largest decimal number in a target token (qubit targets: "5", "!5",
+// "X5*Y6" style products); record and sweep targets are skipped.
+void maxQubitOf(std::string_view tok, uint64_t &maxq, bool &any) {
+  if (startsWith(tok, "rec[") || startsWith(tok, "sweep[")) return;
+  uint64_t v = 0;
+  bool in = false;
+  for (size_t i = 0; i <= tok.size(); ++i) {
+    const char c = i < tok.size() ? tok[i] : ' ';
+    if (c >= '0' && c <= '9') {
+      v = v * 10 + static_cast<uint64_t>(c - '0');
+      in = true;
+    } else {
+      if (in && (!any || v > maxq)) {
+        maxq = v;
+        any = true;
+      }
+      v = 0;
+      in = false;
+    }
+  }
+}
+
+}  // namespace
+
+CircuitSummary summarize(std::string_view text) {
+  CircuitSummary r;
+  std::vector<uint64_t> mult{1};
+  uint64_t maxq = 0, ticks = 0;
+  bool anyQ = false;
+  size_t pos = 0;
+  while (pos < text.size()) {
+    size_t end = text.find('\n', pos);
+    if (end == std::string_view::npos) end = text.size();
+    std::string_view line = text.substr(pos, end - pos);
+    pos = end + 1;
+    const size_t hash = line.find('#');
+    if (hash != std::string_view::npos) line = line.substr(0, hash);
+    size_t b = 0;
+    while (b < line.size() && (line[b] == ' ' || line[b] == '\t' || line[b] == '\r')) ++b;
+    line = line.substr(b);
+    if (line.empty()) continue;
+    const uint64_t m = mult.back();
+    if (line[0] == '}') {
+      if (mult.size() > 1) mult.pop_back();
+      continue;
+    }
+    size_t n = 0;
+    while (n < line.size() && (std::isalnum(static_cast<unsigned char>(line[n])) || line[n] == '_')) ++n;
+    const std::string_view name = line.substr(0, n);
+    std::string_view rest = line.substr(n);
+    if (name == "REPEAT") {
+      uint64_t k = 0;
+      for (char c : rest)
+        if (c >= '0' && c <= '9') k = k * 10 + static_cast<uint64_t>(c - '0');
+      mult.push_back(m * k);
+      continue;
+    }
+    if (name == "TICK") {
+      ticks += m;
+      continue;
+    }
+    if (name == "DETECTOR") {
+      r.detectors += m;
+      continue;
+    }
+    if (name == "OBSERVABLE_INCLUDE") {
+      uint64_t k = 0;
+      const size_t open = rest.find('('), close = rest.find(')');
+      if (open != std::string_view::npos && close != std::string_view::npos)
+        for (size_t i = open + 1; i < close; ++i)
+          if (rest[i] >= '0' && rest[i] <= '9') k = k * 10 + static_cast<uint64_t>(rest[i] - '0');
+      r.observables = std::max(r.observables, k + 1);
+      continue;
+    }
+    if (name == "QUBIT_COORDS" || name == "SHIFT_COORDS") continue;
+    if (!rest.empty() && rest[0] == '(') {
+      const size_t close = rest.find(')');
+      rest = close == std::string_view::npos ? std::string_view() : rest.substr(close + 1);
+    }
+    size_t i = 0;
+    while (i < rest.size()) {
+      while (i < rest.size() && (rest[i] == ' ' || rest[i] == '\t' || rest[i] == '\r')) ++i;
+      size_t j = i;
+      while (j < rest.size() && rest[j] != ' ' && rest[j] != '\t' && rest[j] != '\r') ++j;
+      if (j > i) maxQubitOf(rest.substr(i, j - i), maxq, anyQ);
+      i = j;
+    }
+  }
+  r.layers = ticks + 1;
+  r.qubits = anyQ ? maxq + 1 : 0;
+  return r;
+}
+
 }  // namespace gpa

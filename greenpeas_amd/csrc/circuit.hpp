@@ -53,4 +53,13 @@ struct Circuit {
   static Circuit parse(std::string_view text);
 };
 
+// Sizes read off circuit text in one pass, without parsing it (for memory
+// admission): layers (TICK count + 1, REPEAT bodies counted per
+// repetition), qubits (highest qubit target + 1), detectors (per
+// repetition) and observables (highest OBSERVABLE_INCLUDE index + 1).
+struct CircuitSummary {
+  uint64_t layers = 1, qubits = 0, detectors = 0, observables = 0;
+};
+CircuitSummary summarize(std::string_view text);
+
 }  // namespace gpa

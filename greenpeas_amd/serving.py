@@ -23,7 +23,6 @@ merge order within a circuit does not depend on the batch.
 from __future__ import annotations
 
 import collections
-import re
 import threading
 import time
 from concurrent.futures import Future
@@ -31,31 +30,24 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
+from . import _native
 from .error_analysis import CorrelationLevel, circuit_text
 
-_TICK = re.compile(r"^\s*TICK\b", re.M)
-_DET = re.compile(r"^\s*DETECTOR\b", re.M)
-_OBS = re.compile(r"^\s*OBSERVABLE_INCLUDE\((\d+)\)", re.M)
-_QUBITS = re.compile(r"(?<![\[\-\w])(\d+)(?![\]\w\.])")
+
+def circuit_summary(text: str) -> tuple[int, int, int, int]:
+    """(layers, qubits, detectors, observables) read off circuit text in one
+    native pass (nothing is parsed or built)."""
+    out = np.zeros(4, np.uint64)
+    _native.check(_native.load().gp_circuit_summary(text.encode(), out.ctypes.data))
+    return tuple(int(x) for x in out)
 
 
 def estimate_bytes(text: str, level: int = 2) -> int:
     """Cheap upper estimate of the device bytes one circuit needs in the
-    batched pipeline (scans the text, builds nothing)."""
-    ticks = len(_TICK.findall(text))
-    dets = len(_DET.findall(text))
-    obs = max((int(x) + 1 for x in _OBS.findall(text)), default=0)
-    # largest qubit index on gate lines
-    maxq = 0
-    for line in text.splitlines():
-        if not line or line.startswith(("DETECTOR", "OBSERVABLE", "#", "QUBIT_COORDS", "SHIFT_COORDS")):
-            continue
-        for m in _QUBITS.findall(line.split(")")[-1]):
-            v = int(m)
-            if v > maxq:
-                maxq = v
+    batched pipeline (from circuit_summary; builds nothing)."""
+    layers, qubits, dets, obs = circuit_summary(text)
     k = (3, 5, 8)[level]
-    nodes = (ticks + 1) * (maxq + 1) * k
+    nodes = layers * qubits * k
     ranks = nodes * 3 // 5 + 64
     cols = (dets + obs + 31) // 32
     # sort buckets (64 nodes each at the default fill, 256 slots per bucket,
@@ -65,7 +57,7 @@ def estimate_bytes(text: str, level: int = 2) -> int:
         buckets <<= 1
     slots = buckets * 256 + ranks
     return int(ranks * (cols * 4 + 232) + slots * 40 + cols * 4096 + nodes * 24
-               + (ticks + 1) * cols * 4 + 65536)
+               + layers * cols * 4 + 65536)
 
 
 @dataclass

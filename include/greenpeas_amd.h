@@ -154,6 +154,11 @@ int gp_batch_dem_copy(const gp_batch *batch, uint32_t index, double *probs,
                       uint32_t *offsets, uint32_t *targets, uint64_t *hashes);
 /* Bytes of device memory the batch holds now. */
 int gp_batch_device_bytes(const gp_batch *batch, uint64_t *bytes);
+/* Sizes read off circuit text in one pass, without parsing or building
+ * anything (no counterpart in the reference; for memory admission of
+ * queued requests): summary = {layers (TICKs + 1, REPEAT bodies counted per
+ * repetition), qubits (highest qubit target + 1), detectors, observables}. */
+int gp_circuit_summary(const char *circuit_text, uint64_t summary[4]);
 /* Estimated device bytes to compile circuit text at a level (admission). */
 int gp_estimate_device_bytes(const char *circuit_text, int correlation_level,
                              uint64_t *bytes);

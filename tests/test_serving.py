@@ -89,3 +89,21 @@ def test_request_pool_shapes():
     assert len(pool) == 6 and all(t.startswith("R ") for t in pool)
     arr = serving.poisson_arrivals(1000.0, 100, 3)
     assert arr.shape == (100,) and (arr[1:] > arr[:-1]).all()
+
+
+def test_circuit_summary_matches_the_parser():
+    """The one-pass summary used for admission agrees with the full parse
+    (layers, qubits via nodes per layer, detectors, observables), REPEAT
+    bodies included."""
+    from greenpeas_amd.error_analysis import build_stepg_host
+
+    rep = "R 0 1 2\nTICK\nREPEAT 5 {\n  CX 0 1\n  TICK\n  M 1\n  DETECTOR rec[-1]\n}\nM 0 2\nOBSERVABLE_INCLUDE(3) rec[-1]\n"
+    for text in [read_circuit(n) for n in ("surface3", "surface5", "bb6", "bb12")] + [rep]:
+        layers, qubits, dets, obs = serving.circuit_summary(text)
+        _, _, dims = build_stepg_host(text)
+        assert layers == int(dims[0])
+        assert qubits * 8 == int(dims[1])
+        assert dets == text.count("DETECTOR") if "REPEAT" not in text else dets == 5
+        lines = [ln for ln in text.splitlines() if ln.startswith("OBSERVABLE_INCLUDE(")]
+        assert obs == max(int(ln.split("(")[1].split(")")[0]) for ln in lines) + 1
+    assert serving.circuit_summary("X_ERROR(0.1) 7\nMPP X2*Z9\n# 100\n")[1] == 10
