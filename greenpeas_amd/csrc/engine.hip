@@ -42,9 +42,9 @@ struct VDesc {
   uint32_t numLayers, npl, numCols, numDetectors;
   uint32_t numObservables, numRanks, numSeeds, bucketBits;
   uint32_t bucketBase, numBuckets, maskWords, layerStride;
-  uint32_t sweepThreads, sweepRpt, pad0, pad1;
+  uint32_t sweepThreads, sweepRpt, rowSpan, pad1;  // rowSpan: see StepgImage
   uint64_t actOff, seedOff, colOff, rankOff, sOff;
-  uint64_t colStride;  // words per S column: numRanks + kSinkSlots sink words
+  uint64_t colStride;  // words per S column: numLayers * layerStride frame rows + kSinkSlots sinks
 };
 
 // Counter block zeroed at the start of every run: [flags | bucket counts].
@@ -464,7 +464,7 @@ __global__ __launch_bounds__(kSweepMaxThreads) void sweepKernel(
     // one store are contiguous across lanes, those of different stores 32
     // words apart (mod kSinkSlots; at most 16 stores, so a lane's addresses
     // differ).
-    uint32_t *sink = reinterpret_cast<uint32_t *>(Sc) + d.numRanks;
+    uint32_t *sink = reinterpret_cast<uint32_t *>(Sc) + d.numLayers * P;
     uint4 rs[D][RPT];
     uint32_t loadOff = static_cast<uint32_t>(hi + static_cast<int32_t>(kSweepLeadPad)) * st.layerBytes;
 #pragma unroll
@@ -521,6 +521,7 @@ __global__ __launch_bounds__(kSweepMaxThreads) void sweepKernel(
 // word is computed from the bands directly (band loads issued eight at a
 // time).
 constexpr uint32_t kMaskTableWords = 1024;
+constexpr uint32_t kBandTableCols = 1024;  // column bands kept in LDS by classify
 
 struct LayerMasks {
   const uint32_t *tab;  // LDS table, or nullptr
@@ -562,31 +563,59 @@ __device__ __forceinline__ void buildMaskTable(uint32_t *tab, uint32_t l0, uint3
 }
 
 // Calls f(target) for every set bit of one node's sensitivity row, in
-// ascending target order: the row's words are those of the columns set in
-// the node's layer mask, at stride colStride from Sv.  Columns are taken
-// eight at a time and their eight words loaded before any is used (absent
-// columns load column 0's word, which is always valid), so a node waits for
-// one memory round trip per eight columns, not per column.
+// ascending target order.  The row is the XOR of the node's frame rows
+// (`code`: up to four row codes, layer << kRowPosBits | position,
+// kNone-padded at the end): word c = XOR of S[c][layer * P + position] over
+// the rows whose layer lies in column c's band (outside the band a frame
+// sensitivity is zero and its S word is not written).  The columns visited
+// are those set in the masks of the rows' layers.  Columns are taken four at
+// a time and all their words loaded before any is used, so a node waits for
+// one memory round trip per four columns.
 template <typename F>
-__device__ __forceinline__ void forEachTarget(const LayerMasks &lm, uint32_t l,
-                                              const uint32_t *__restrict__ Sv, uint64_t colStride, F &&f) {
-  constexpr int kBatch = 8;
+__device__ __forceinline__ void forEachTarget(const LayerMasks &lm, const int2 *band, const uint4 code,
+                                              uint32_t P, const uint32_t *__restrict__ S0, uint64_t colStride,
+                                              F &&f) {
+  constexpr uint32_t kPosMask = (1u << kRowPosBits) - 1;
+  const uint32_t cd[4] = {code.x, code.y, code.z, code.w};
+  bool ok[4];
+  uint32_t lay[4], idx[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    ok[q] = cd[q] != NONE;
+    lay[q] = ok[q] ? cd[q] >> kRowPosBits : 0u;
+    idx[q] = ok[q] ? lay[q] * P + (cd[q] & kPosMask) : 0u;
+  }
+  auto maskWord = [&](uint32_t w) {
+    uint32_t m = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (ok[q]) m |= lm.word(lay[q], w);
+    return m;
+  };
+  constexpr int kBatch = 4;
   uint32_t w = 0;
-  uint32_t m = lm.maskWords ? lm.word(l, 0) : 0u;
+  uint32_t m = (lm.maskWords && ok[0]) ? maskWord(0) : 0u;
   for (;;) {
-    uint32_t col[kBatch], x[kBatch];
+    uint32_t col[kBatch], x[kBatch][4];
 #pragma unroll
     for (int j = 0; j < kBatch; ++j) {
-      while (m == 0u && w + 1 < lm.maskWords) m = lm.word(l, ++w);
+      while (m == 0u && ok[0] && w + 1 < lm.maskWords) m = maskWord(++w);
       col[j] = m ? w * 32 + static_cast<uint32_t>(__builtin_ctz(m)) : NONE;
       m &= m - 1;
     }
 #pragma unroll
-    for (int j = 0; j < kBatch; ++j) x[j] = Sv[static_cast<uint64_t>(col[j] != NONE ? col[j] : 0u) * colStride];
+    for (int j = 0; j < kBatch; ++j)
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        x[j][q] = (col[j] != NONE && ok[q]) ? S0[static_cast<uint64_t>(col[j]) * colStride + idx[q]] : 0u;
 #pragma unroll
     for (int j = 0; j < kBatch; ++j) {
       if (col[j] == NONE) continue;
-      uint32_t v = x[j];
+      const int2 bd = band[col[j]];
+      uint32_t v = 0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if (ok[q] && bd.x <= static_cast<int32_t>(lay[q]) && static_cast<int32_t>(lay[q]) <= bd.y) v ^= x[j][q];
       while (v) {
         f(col[j] * 32 + static_cast<uint32_t>(__builtin_ctz(v)));
         v &= v - 1;
@@ -630,30 +659,42 @@ struct ClassifyOut {
 __global__ __launch_bounds__(kClassBlock) void classifyKernel(
     const VDesc *__restrict__ desc, const uint2 *__restrict__ tiles, const uint32_t *__restrict__ rankLayer,
     const int2 *__restrict__ band, const uint32_t *__restrict__ S, const double *__restrict__ rankProb,
-    ClassifyOut o) {
+    const uint4 *__restrict__ rankRows, ClassifyOut o) {
   // per wave: 64 rows of kStashWords words, padded to kStashWords + 1 so
   // that a lane's writes and the row-major read-out avoid bank conflicts
   __shared__ uint32_t rows[kClassBlock / 64][64 * (kStashWords + 1)];
   __shared__ uint32_t maskTab[kMaskTableWords];
+  __shared__ int2 bandTab[kBandTableCols];
   const uint2 t = tiles[blockIdx.x];
   const VDesc d = desc[t.x];
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t i = t.y + threadIdx.x;
+  // this node's record and frame rows, loaded before the tables are built
+  const uint64_t gi = d.rankOff + (i < d.numRanks ? i : t.y);
+  const uint4 code = rankRows[gi];
+  const double prob = rankProb[gi];
   const uint32_t last = (t.y + kClassBlock < d.numRanks ? t.y + kClassBlock : d.numRanks) - 1;
-  const uint32_t l0 = rankLayer[d.rankOff + t.y], l1 = rankLayer[d.rankOff + last];
+  // the rows of a node at layer l lie in layers l .. l + rowSpan
+  const uint32_t l0 = rankLayer[d.rankOff + t.y];
+  const uint32_t l1 = min(rankLayer[d.rankOff + last] + d.rowSpan, d.numLayers - 1);
+  const int2 *bandC = band + d.colOff;
+  if (d.numCols <= kBandTableCols) {
+    for (uint32_t c = threadIdx.x; c < d.numCols; c += blockDim.x) bandTab[c] = bandC[c];
+    bandC = bandTab;  // filled by the barriers of buildMaskTable, or the one below
+  }
   LayerMasks lm{nullptr, band + d.colOff, l0, d.numCols, d.maskWords};
   if ((l1 - l0 + 1) * d.maskWords <= kMaskTableWords) {
     buildMaskTable(maskTab, l0, l1 - l0 + 1, d.numCols, d.maskWords, lm.band);
     lm.tab = maskTab;
+  } else {
+    __syncthreads();
   }
   uint16_t *row = reinterpret_cast<uint16_t *>(rows[wave] + lane * (kStashWords + 1));
   const bool stashed = d.numCols <= o.stashMaxCols;
   if (i < d.numRanks) {
-    const uint64_t gi = d.rankOff + i;
-    const uint32_t *Sv = S + d.sOff + i;
     uint64_t h = kFnvBasis, h2 = 0x84222325cbf29ce4ull;
     uint32_t cnt = 0;
-    forEachTarget(lm, rankLayer[gi], Sv, d.colStride, [&](uint32_t tg) {
+    forEachTarget(lm, bandC, code, d.layerStride, S + d.sOff, d.colStride, [&](uint32_t tg) {
       h = (h ^ tg) * kFnvPrime;
       h2 = mix2(h2, tg);
       if (cnt < kStashTargets) row[cnt] = static_cast<uint16_t>(tg);
@@ -662,7 +703,7 @@ __global__ __launch_bounds__(kClassBlock) void classifyKernel(
     if (cnt && cnt <= 32)
       for (uint32_t k = cnt; k < 32; ++k) h = (h ^ 0xFFFFFFFFull) * kFnvPrime;
     h2 = mix2(h2, cnt);
-    o.info[gi] = nodeInfo(rankProb[gi], static_cast<uint32_t>(h2 ^ (h2 >> 32)), cnt);
+    o.info[gi] = nodeInfo(prob, static_cast<uint32_t>(h2 ^ (h2 >> 32)), cnt);
     if (cnt) {
       const uint32_t b = d.bucketBase + (d.bucketBits ? static_cast<uint32_t>(h >> (64 - d.bucketBits)) : 0u);
       const uint32_t pos = atomicAdd(&o.bucketCount[static_cast<uint64_t>(b) * o.countStride], 1u);
@@ -873,7 +914,8 @@ struct EmitIn {
   const uint32_t *runRank, *runTgtPre, *runMember;
   const double *runP;
   const uint64_t *runHash;
-  const uint32_t *runNnz, *rankLayer, *S, *stash;
+  const uint32_t *runNnz, *S, *stash;
+  const uint4 *rankRows;
   uint32_t stashMaxCols;
   const int2 *band;
 };
@@ -982,9 +1024,9 @@ __global__ __launch_bounds__(256) void emitKernel(const VDesc *__restrict__ desc
         if (j < n && pos + j < out.capTargets) out.targets[pos + j] = (w[j / 2] >> (16 * (j % 2))) & 0xFFFFu;
       pos += n;
     } else {
-      const uint32_t i = static_cast<uint32_t>(rk - d.rankOff);
       const LayerMasks lm{nullptr, in.band + d.colOff, 0u, d.numCols, d.maskWords};
-      forEachTarget(lm, in.rankLayer[rk], in.S + d.sOff + i, d.colStride, [&](uint32_t tg) {
+      forEachTarget(lm, in.band + d.colOff, in.rankRows[rk], d.layerStride, in.S + d.sOff, d.colStride,
+                    [&](uint32_t tg) {
         if (pos < out.capTargets) out.targets[pos] = tg;
         pos++;
       });
@@ -1230,7 +1272,7 @@ struct Pipeline::Impl {
   DevBuf in;
   HostBuf inHost;
   size_t oDesc = 0, oItems = 0, oTiles = 0, oAct = 0, oSeed = 0, oColHi = 0,
-         oColLow = 0, oRankLayer = 0, oRankProb = 0, inBytes = 0;
+         oColLow = 0, oRankLayer = 0, oRankProb = 0, oRankRows = 0, inBytes = 0;
   uint32_t V = 0, nItems = 0, nTiles = 0, B = 0, R = 0, Ccols = 0, rpt = 1;
   uint64_t Swords = 0;
   size_t sweepLdsBytes = 0;
@@ -1345,12 +1387,13 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
     d.layerStride = g.layerStride;
     d.sweepThreads = g.sweepThreads;
     d.sweepRpt = g.sweepRpt;
+    d.rowSpan = g.rowSpan;
     d.actOff = actOff;
     d.seedOff = seedOff;
     d.colOff = colOff;
     d.rankOff = rankOff;
     d.sOff = sOff;
-    d.colStride = static_cast<uint64_t>(g.numRanks) + kSinkSlots;
+    d.colStride = static_cast<uint64_t>(g.numLayers) * g.layerStride + kSinkSlots;
     variants_[v] = VariantInfo{g.numRanks, g.numCols, g.numDetectors, g.numObservables, bucketBase,
                                d.numBuckets};
     for (uint32_t c = 0; c < g.numCols; ++c) {
@@ -1414,6 +1457,7 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   m.oColLow = place(4 * colOff);
   m.oRankLayer = place(4 * rankOff);
   m.oRankProb = place(8 * rankOff);
+  m.oRankRows = place(16 * rankOff);
   m.inBytes = off;
   m.in.ensure(m.inBytes, frozen, "inputs");
   m.inHost.ensure(m.inBytes);
@@ -1430,6 +1474,7 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
     std::memcpy(hb + m.oColLow + 4 * d.colOff, g.colLow.data(), 4 * g.colLow.size());
     std::memcpy(hb + m.oRankLayer + 4 * d.rankOff, g.rankLayer.data(), 4 * g.rankLayer.size());
     std::memcpy(hb + m.oRankProb + 8 * d.rankOff, g.rankProb.data(), 8 * g.rankProb.size());
+    std::memcpy(hb + m.oRankRows + 16 * d.rankOff, g.rankRows.data(), 4 * g.rankRows.size());
   }
 
   // work buffers
@@ -1521,6 +1566,7 @@ void Pipeline::run(bool timed) {
   const int32_t *colLow = reinterpret_cast<const int32_t *>(ib + m.oColLow);
   const uint32_t *rankLayer = reinterpret_cast<const uint32_t *>(ib + m.oRankLayer);
   const double *rankProb = reinterpret_cast<const double *>(ib + m.oRankProb);
+  const uint4 *rankRows = reinterpret_cast<const uint4 *>(ib + m.oRankRows);
   hipStream_t st = m.stream;
   hipEvent_t *evs = m.evOverride ? m.evOverride : m.ev;
   auto mark = [&](int i) {
@@ -1564,7 +1610,7 @@ void Pipeline::run(bool timed) {
                    m.ovRank.as<uint32_t>(), m.ovBucket.as<uint32_t>(), m.flags(), m.stash.as<uint32_t>(),
                    stashMaxCols};
     classifyKernel<<<m.nTiles, kClassBlock, 0, st>>>(desc, tiles, rankLayer, m.band.as<int2>(),
-                                                     m.S.as<uint32_t>(), rankProb, co);
+                                                     m.S.as<uint32_t>(), rankProb, rankRows, co);
   }
   mark(2);
   RunOut o{m.info.as<uint4>(),
@@ -1594,8 +1640,8 @@ void Pipeline::run(bool timed) {
              m.runCount.as<uint32_t>(), m.itemCount.as<uint32_t>(), m.classStart.as<uint32_t>(),
              m.tgtStart.as<uint32_t>(), m.memberStart.as<uint32_t>(), m.runRank.as<uint32_t>(),
              m.runTgtPre.as<uint32_t>(), m.runMember.as<uint32_t>(), m.runP.as<double>(),
-             m.runHash.as<uint64_t>(), m.runNnz.as<uint32_t>(), rankLayer, m.S.as<uint32_t>(),
-             m.stash.as<uint32_t>(), stashMaxCols, m.band.as<int2>()};
+             m.runHash.as<uint64_t>(), m.runNnz.as<uint32_t>(), m.S.as<uint32_t>(),
+             m.stash.as<uint32_t>(), rankRows, stashMaxCols, m.band.as<int2>()};
   EmitOut eout{m.outP.as<double>(), m.outOff.as<uint32_t>(), m.outHash.as<uint64_t>(), m.outMember.as<uint32_t>(),
                m.outTargets.as<uint32_t>(), m.capTargets, m.flags(), m.vBounds.as<uint32_t>()};
   emitKernel<<<std::max<uint32_t>((B + 3) / 4, 1), 256, 0, st>>>(desc, ein, eout);
@@ -1770,14 +1816,22 @@ void Pipeline::debugSensitivities(uint32_t v, std::vector<uint32_t> &cols32) con
     GPA_HIP(hipMemcpy(S.data(), m.S.as<uint32_t>() + d.sOff, 4 * S.size(), hipMemcpyDeviceToHost));
   if (d.numCols)
     GPA_HIP(hipMemcpy(band.data(), m.band.as<int2>() + d.colOff, 8 * d.numCols, hipMemcpyDeviceToHost));
-  std::vector<uint32_t> layer(d.numRanks);
-  std::memcpy(layer.data(), m.inHost.as<char>() + m.oRankLayer + 4 * d.rankOff, 4 * d.numRanks);
+  // each rank's word = XOR of its in-band frame rows
+  std::vector<uint32_t> code(4ull * d.numRanks);
+  std::memcpy(code.data(), m.inHost.as<char>() + m.oRankRows + 16 * d.rankOff, 16ull * d.numRanks);
   cols32.assign(static_cast<size_t>(d.numRanks) * d.numCols, 0u);
   for (uint32_t c = 0; c < d.numCols; ++c)
     for (uint32_t i = 0; i < d.numRanks; ++i) {
-      const int32_t l = static_cast<int32_t>(layer[i]);
-      if (l >= band[c].x && l <= band[c].y)
-        cols32[static_cast<size_t>(i) * d.numCols + c] = S[static_cast<size_t>(c) * d.colStride + i];
+      uint32_t w = 0;
+      for (int q = 0; q < 4; ++q) {
+        const uint32_t cd = code[4ull * i + q];
+        if (cd == NONE) continue;
+        const int32_t l = static_cast<int32_t>(cd >> kRowPosBits);
+        if (l < band[c].x || l > band[c].y) continue;
+        const uint64_t row = static_cast<uint64_t>(l) * d.layerStride + (cd & ((1u << kRowPosBits) - 1));
+        w ^= S[static_cast<size_t>(c) * d.colStride + row];
+      }
+      cols32[static_cast<size_t>(i) * d.numCols + c] = w;
     }
 }
 

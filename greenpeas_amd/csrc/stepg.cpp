@@ -291,8 +291,8 @@ void buildImage(const Stepg &g, StepgImage &img) {
         img.rankProb.push_back(g.probs[n]);
       }
   img.numRanks = static_cast<uint32_t>(img.rankNode.size());
-  // sweep records address a column of S in bytes with 32 bits
-  if (img.numRanks >= (1u << 30) - kSinkSlots) throw RuntimeError("circuit too large: more than 2^30 error nodes");
+  const uint32_t k = nodesPerQubit(g.level);
+  auto isFrame = [&](uint32_t n) { return (n % npl) % k < 2; };
 
   // active = p > 0 nodes and all their descendants (children are later)
   std::vector<uint8_t> active(N, 0);
@@ -340,17 +340,28 @@ void buildImage(const Stepg &g, StepgImage &img) {
     }
   }
 
-  // active index of every active node within its layer
+  // The sweep runs over the active frame nodes only (see the file comment);
+  // check the structure it relies on.
+  for (uint32_t n = 0; n < N; ++n) {
+    if (!active[n] || !isFrame(n)) continue;
+    const uint64_t a = g.adjacency[n];
+    for (uint32_t ch : {static_cast<uint32_t>(a), static_cast<uint32_t>(a >> 32)})
+      if (ch != kNone && !isFrame(ch)) throw RuntimeError("STEPG: a frame node flows into an auxiliary node");
+  }
+  for (uint32_t s2 = 0; s2 < S; ++s2)
+    if (!isFrame(seedNode[s2])) throw RuntimeError("STEPG: a measurement seeds an auxiliary node");
+
+  // position of every active frame node within its layer
   std::vector<uint32_t> actIdx(N, kNone);
   img.layerAct.assign(L + 1, 0);
   uint32_t total = 0, maxAct = 0;
   for (uint32_t l = 0, n = 0; l < L; ++l) {
     img.layerAct[l] = total;
-    uint32_t k = 0;
+    uint32_t cnt = 0;
     for (uint32_t j = 0; j < npl; ++j, ++n)
-      if (active[n]) actIdx[n] = k++;
-    total += k;
-    maxAct = std::max(maxAct, k);
+      if (active[n] && isFrame(n)) actIdx[n] = cnt++;
+    total += cnt;
+    maxAct = std::max(maxAct, cnt);
   }
   img.layerAct[L] = total;
   img.numActive = total;
@@ -369,32 +380,76 @@ void buildImage(const Stepg &g, StepgImage &img) {
   img.sweepThreads = threads;
   img.sweepRpt = rpt;
   img.layerStride = P;
+  // frame rows of a column of S, addressed in bytes with 32 bits; row codes
+  // hold the position in kRowPosBits bits
+  if ((static_cast<uint64_t>(L) * P + kSinkSlots) * 4 >= (1ull << 32) || P > (1u << kRowPosBits) ||
+      L >= (1u << (32 - kRowPosBits)))
+    throw RuntimeError("circuit too large for the sweep: too many layers x frame nodes");
 
   // records, layer by layer, with final LDS / S addresses
   const uint32_t zeroB = 3u * P * 4u;        // the zero word after the ring
   const uint32_t seedB = (3u * P + 1u) * 4u;  // the column's first seed word
+  const uint32_t sinkRow = L * P;             // sink words after the frame rows
   img.act.resize(static_cast<size_t>(kSweepLeadPad + L) * P * 4);
   auto child = [&](uint32_t ch) -> uint32_t {
     if (ch == kNone) return zeroB;
     return ((ch / npl) % 3u * P + actIdx[ch]) * 4u;
   };
-  for (uint32_t blk = 0, n = 0; blk < kSweepLeadPad + L; ++blk) {
+  for (uint32_t blk = 0; blk < kSweepLeadPad + L; ++blk) {
     const bool pad = blk < kSweepLeadPad;  // block kSweepLeadPad + l holds layer l
+    const uint32_t l = blk - kSweepLeadPad;
     uint32_t *rec = img.act.data() + static_cast<size_t>(blk) * P * 4;
     uint32_t pos = 0;
-    for (uint32_t j = 0; !pad && j < npl; ++j, ++n) {
-      if (!active[n]) continue;
+    for (uint32_t j = 0, n = l * npl; !pad && j < npl; ++j, ++n) {
+      if (actIdx[n] == kNone) continue;
       const uint64_t a = g.adjacency[n];
       rec[0] = child(static_cast<uint32_t>(a));
       rec[1] = child(static_cast<uint32_t>(a >> 32));
       rec[2] = seedOf[n] == kNone ? zeroB : seedB + 4u * seedOf[n];
-      rec[3] = 4u * (rank[n] != kNone ? rank[n] : img.numRanks + (pos & (kSinkSlots - 1)));
+      rec[3] = 4u * (l * P + pos);
       rec += 4;
       ++pos;
     }
     for (; pos < P; ++pos, rec += 4) {
       rec[0] = rec[1] = rec[2] = zeroB;
-      rec[3] = 4u * (img.numRanks + (pos & (kSinkSlots - 1)));
+      rec[3] = 4u * (pad ? sinkRow + (pos & (kSinkSlots - 1)) : l * P + pos);
+    }
+  }
+
+  // each rank's frame rows: itself, or the frame nodes its flows reach
+  // (through at most one auxiliary node), each counted modulo 2
+  img.rankRows.assign(static_cast<size_t>(img.numRanks) * 4, kNone);
+  for (uint32_t r = 0; r < img.numRanks; ++r) {
+    uint32_t frames[8];
+    uint32_t nf = 0;
+    auto toggle = [&](uint32_t f) {
+      for (uint32_t i = 0; i < nf; ++i)
+        if (frames[i] == f) {
+          frames[i] = frames[--nf];
+          return;
+        }
+      if (nf == 8) throw RuntimeError("STEPG: an auxiliary node reaches too many frame nodes");
+      frames[nf++] = f;
+    };
+    auto expand = [&](auto &self, uint32_t n, int depth) -> void {
+      if (isFrame(n)) {
+        toggle(n);
+        return;
+      }
+      if (depth == 2) throw RuntimeError("STEPG: auxiliary nodes nested more than two deep");
+      const uint64_t a = g.adjacency[n];
+      for (uint32_t ch : {static_cast<uint32_t>(a), static_cast<uint32_t>(a >> 32)})
+        if (ch != kNone) self(self, ch, depth + 1);
+    };
+    expand(expand, img.rankNode[r], 0);
+    if (nf > 4) throw RuntimeError("STEPG: an auxiliary node reaches more than four frame nodes");
+    std::sort(frames, frames + nf);
+    const uint32_t rl = img.rankNode[r] / npl;
+    for (uint32_t i = 0; i < nf; ++i) {
+      const uint32_t fl = frames[i] / npl;
+      if (fl < rl) throw RuntimeError("STEPG: a flow runs backwards in time");
+      img.rowSpan = std::max(img.rowSpan, fl - rl);
+      img.rankRows[4 * r + i] = (fl << kRowPosBits) | actIdx[frames[i]];
     }
   }
 }
@@ -408,8 +463,8 @@ uint64_t imageDeviceBytes(const StepgImage &img) {
   const uint64_t slots = B * 256 + R;
   uint64_t b = 0;
   b += static_cast<uint64_t>(kSweepLeadPad + img.numLayers) * img.layerStride * 16ull + img.seedVals.size() * 4ull;
-  b += img.numCols * 16ull + R * 12ull;         // column bands, rank layer + prob
-  b += (R + kSinkSlots) * img.numCols * 4ull;   // sensitivity matrix (+ per-column sink slots)
+  b += img.numCols * 16ull + R * 28ull;         // column bands, rank layer + prob + frame rows
+  b += (static_cast<uint64_t>(img.numLayers) * img.layerStride + kSinkSlots) * img.numCols * 4ull;  // frame sensitivities
   b += R * (16 + 64 + 16 + 24);                 // node records, target rows, overflow list, sort workspace
   b += slots * (8 + 4 + 4 + 8 + 8 + 4 + 4);     // bucket slots: keys, ranks, run arrays
   b += (B + 1) * 4 * 9;                         // per-bucket counters, counts and offsets

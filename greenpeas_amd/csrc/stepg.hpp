@@ -13,22 +13,31 @@
 // The device image re-lays that graph out for the MI355X sweep:
 //  * only "active" nodes are kept (nodes with p > 0 and every node reachable
 //    from one along flows; the others can never influence an output);
+//  * only the frame nodes (slots X and Z of each qubit) are swept: gates
+//    flow frame nodes into frame nodes, measurements seed frame nodes, and
+//    the auxiliary nodes of correlated components flow into frame nodes (or
+//    into an auxiliary Y node that does) but never receive a flow from one.
+//    An auxiliary node's sensitivity is therefore the XOR of at most four
+//    frame nodes' sensitivities, which the later stages form on the fly from
+//    its `rankRows`; the sequential sweep touches frame nodes only;
 //  * each layer is a fixed-stride block of `layerStride` 16-byte records,
 //    the layer's active nodes in node order, then padding; a record holds
 //    the final addresses the sweep uses, so it does no index arithmetic:
 //    {LDS byte address of child 0's sensitivity, of child 1's, of the seed
-//    word, byte offset of the node's sensitivity in its column of S}.  The
+//    word, byte offset of the node's sensitivity in its column of S (frame
+//    row l * layerStride + position)}.  The
 //    sweep keeps layers l, l+1, l+2 in an LDS ring whose slot for layer l is
 //    l mod 3 (slot size `layerStride` words), followed by a zero word (what
 //    absent children, absent seeds and padding read) and the column's seed
-//    words.  Nodes without a rank (and padding) write to one of
-//    kSinkSlots sink words past the column's ranks;
+//    words.  Padding in front of layer 0 writes to one of kSinkSlots sink
+//    words past the column's frame rows;
 //  * measurement seeds are stored once per seeded node, column-major over
 //    32-bit detector words ("columns");
 //  * per column, the highest and lowest layer holding a seed bound the time
 //    band in which the column can be non-zero;
-//  * the p > 0 nodes ("ranks", ascending node order) are the rows of the
-//    sensitivity matrix and the items that are hashed, sorted and merged.
+//  * the p > 0 nodes ("ranks", ascending node order) are the items that are
+//    hashed, sorted and merged; each lists the frame rows whose XOR is its
+//    sensitivity (rankRows).
 #pragma once
 
 #include <cstdint>
@@ -47,6 +56,8 @@ constexpr uint32_t kSweepMaxThreads = 1024;
 // all-padding record blocks in front of layer 0 (what the sweep's
 // prefetches load for the layers below 0 that its trips run into)
 constexpr uint32_t kSweepLeadPad = 11;
+// frame row codes in rankRows: layer << kRowPosBits | position in the layer
+constexpr uint32_t kRowPosBits = 14;
 
 inline uint32_t nodesPerQubit(int level) {
   switch (level) {
@@ -85,6 +96,8 @@ struct StepgImage {
   // sweep shape: record j*sweepThreads + t of a layer belongs to thread t
   // (j < sweepRpt); layerStride = sweepThreads * sweepRpt >= maxActive
   uint32_t sweepThreads = 64, sweepRpt = 1, layerStride = 64;
+  // largest (frame row layer - rank layer) over all rankRows entries
+  uint32_t rowSpan = 0;
   std::vector<uint32_t> layerAct;  // numLayers + 1 offsets of each layer's first active node
   std::vector<uint32_t> act;       // 4 * (kSweepLeadPad + numLayers) * layerStride: padding
                                    // blocks, then one block per layer (see the file comment)
@@ -93,6 +106,7 @@ struct StepgImage {
   std::vector<uint32_t> rankLayer; // numRanks
   std::vector<double> rankProb;    // numRanks
   std::vector<uint32_t> rankNode;  // numRanks (host only: node index of each rank)
+  std::vector<uint32_t> rankRows;  // 4 * numRanks: frame row codes, kNone-padded
 };
 
 void buildImage(const Stepg &g, StepgImage &img);
