@@ -648,6 +648,7 @@ int gp_set_tuning(const char *name, uint32_t value) {
     if (std::string(name) == "bucket_fill") gpa::setBucketFill(value);
     else if (std::string(name) == "sweep_rpt") gpa::setSweepMinRpt(value);
     else if (std::string(name) == "stash") gpa::setStash(value != 0);
+    else if (std::string(name) == "mask_table") gpa::setMaskTable(value != 0);
     else throw InvalidArgument(std::string("unknown tuning parameter: ") + name);
   });
 }
@@ -658,6 +659,7 @@ int gp_get_tuning(const char *name, uint32_t *value) {
     if (std::string(name) == "bucket_fill") *value = gpa::bucketFill();
     else if (std::string(name) == "sweep_rpt") *value = gpa::sweepMinRpt();
     else if (std::string(name) == "stash") *value = gpa::stash() ? 1u : 0u;
+    else if (std::string(name) == "mask_table") *value = gpa::maskTable() ? 1u : 0u;
     else throw InvalidArgument(std::string("unknown tuning parameter: ") + name);
   });
 }

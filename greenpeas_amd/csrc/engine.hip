@@ -36,6 +36,7 @@ constexpr int kScanTile = 4 * kScanBlock;
 constexpr int kBucketItemsPerWave = 256;  // wave-register sort capacity (4 per lane)
 std::atomic<uint32_t> gBucketFill{64};    // mean items per bucket the digit width aims at
 std::atomic<uint32_t> gStash{1};          // target rows on (1) or off (0)
+std::atomic<uint32_t> gMaskTable{1};      // classify's LDS mask tables allowed (1) or not (0)
 
 // Per-circuit descriptor (device and host).
 struct VDesc {
@@ -514,92 +515,93 @@ __global__ __launch_bounds__(kSweepMaxThreads) void sweepKernel(
   if (tid == 0) band[d.colOff + c] = make_int2(lo, hi);
 }
 
-// Column masks of the layers a workgroup's nodes lie in.  Bit c of word w
-// of a layer's mask is set when column 32w + c's band {lo, hi} covers the
-// layer; a node's targets live only in those columns.  The table covers
-// layers [l0, l0 + numLayers) in LDS (kMaskTableWords words); without it a
-// word is computed from the bands directly (band loads issued eight at a
-// time).
+// One node's frame rows (its sensitivity is their XOR), decoded from its
+// rankRows code: up to four codes layer << kRowPosBits | position, in
+// ascending layer order, kNone-padded at the end.  idx = the row's word in a
+// column of S (layer * P + position); the rows span layers [lo, lo + span].
+struct FrameRows {
+  uint32_t lay[4], idx[4];
+  bool ok[4];
+  uint32_t lo, span;
+};
+
+__device__ __forceinline__ FrameRows decodeRows(const uint4 code, uint32_t P) {
+  constexpr uint32_t kPosMask = (1u << kRowPosBits) - 1;
+  const uint32_t cd[4] = {code.x, code.y, code.z, code.w};
+  FrameRows r;
+  uint32_t hi = 0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    r.ok[q] = cd[q] != NONE;
+    r.lay[q] = r.ok[q] ? cd[q] >> kRowPosBits : 0u;
+    r.idx[q] = r.ok[q] ? r.lay[q] * P + (cd[q] & kPosMask) : 0u;
+    if (r.ok[q]) hi = r.lay[q];
+  }
+  r.lo = r.lay[0];
+  r.span = hi - r.lo;
+  return r;
+}
+
+// Word of column c for one node: XOR of its rows' S words over the rows
+// whose layer lies in the column's band (outside the band a frame
+// sensitivity is zero and its S word is not written).
+__device__ __forceinline__ uint32_t bandXor(const FrameRows &r, const int2 bd, const uint32_t (&x)[4]) {
+  uint32_t v = 0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    if (r.ok[q] && bd.x <= static_cast<int32_t>(r.lay[q]) && static_cast<int32_t>(r.lay[q]) <= bd.y) v ^= x[q];
+  return v;
+}
+
+// Column masks of the layer windows a workgroup's nodes' rows lie in, in
+// LDS.  Bit c of word w of window (s, lo) is set when column 32w + c's band
+// meets layers [lo, lo + s]; a node whose rows span [lo, lo + s] has its
+// targets in those columns only (the exact words then come from bandXor).
+// Windows s = 0..span for lo in [l0, l0 + nl), s-major.  Each wave turns 64
+// columns' bands into two mask words per window with one ballot.
 constexpr uint32_t kMaskTableWords = 1024;
 constexpr uint32_t kBandTableCols = 1024;  // column bands kept in LDS by classify
 
-struct LayerMasks {
-  const uint32_t *tab;  // LDS table, or nullptr
-  const int2 *band;     // the circuit's columns
-  uint32_t l0, numCols, maskWords;
+__host__ __device__ constexpr uint32_t maskTableWords(uint32_t nl, uint32_t span, uint32_t maskWords) {
+  return (span + 1) * nl * maskWords;
+}
 
-  __device__ __forceinline__ uint32_t word(uint32_t l, uint32_t w) const {
-    if (tab) return tab[(l - l0) * maskWords + w];
-    const int32_t li = static_cast<int32_t>(l);
-    uint32_t m = 0;
-    for (uint32_t k0 = 0; k0 < 32; k0 += 8) {
-      int2 bd[8];
-#pragma unroll
-      for (uint32_t k = 0; k < 8; ++k) {
-        const uint32_t c = w * 32 + k0 + k;
-        bd[k] = band[c < numCols ? c : 0u];
+__device__ __forceinline__ void buildMaskTable(uint32_t *tab, uint32_t l0, uint32_t nl, uint32_t span,
+                                               uint32_t numCols, uint32_t maskWords,
+                                               const int2 *__restrict__ band) {
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6, waves = blockDim.x >> 6;
+  for (uint32_t ch = wave; 2 * ch < maskWords; ch += waves) {
+    const uint32_t c = ch * 64 + lane;
+    const int2 bd = c < numCols ? band[c] : make_int2(1, 0);
+    const uint32_t w = 2 * ch + (lane & 1);
+    for (uint32_t s = 0; s <= span; ++s)
+      for (uint32_t li = 0; li < nl; ++li) {
+        const int32_t lo = static_cast<int32_t>(l0 + li), hi = lo + static_cast<int32_t>(s);
+        const uint64_t bits = __ballot(bd.x <= hi && lo <= bd.y);
+        if (lane < 2 && w < maskWords)
+          tab[(s * nl + li) * maskWords + w] = static_cast<uint32_t>(lane ? bits >> 32 : bits);
       }
-#pragma unroll
-      for (uint32_t k = 0; k < 8; ++k)
-        m |= (w * 32 + k0 + k < numCols && bd[k].x <= li && li <= bd[k].y) ? 1u << (k0 + k) : 0u;
-    }
-    return m;
-  }
-};
-
-// Fills the LDS mask table for layers [l0, l0 + nl) (the whole workgroup
-// calls this; it ends with a barrier).
-__device__ __forceinline__ void buildMaskTable(uint32_t *tab, uint32_t l0, uint32_t nl, uint32_t numCols,
-                                               uint32_t maskWords, const int2 *__restrict__ band) {
-  for (uint32_t i = threadIdx.x; i < nl * maskWords; i += blockDim.x) tab[i] = 0u;
-  __syncthreads();
-  for (uint32_t i = threadIdx.x; i < nl * numCols; i += blockDim.x) {
-    const uint32_t li = i / numCols, c = i % numCols;
-    const int32_t l = static_cast<int32_t>(l0 + li);
-    const int2 bd = band[c];
-    if (bd.x <= l && l <= bd.y) atomicOr(&tab[li * maskWords + c / 32], 1u << (c % 32));
   }
   __syncthreads();
 }
 
 // Calls f(target) for every set bit of one node's sensitivity row, in
-// ascending target order.  The row is the XOR of the node's frame rows
-// (`code`: up to four row codes, layer << kRowPosBits | position,
-// kNone-padded at the end): word c = XOR of S[c][layer * P + position] over
-// the rows whose layer lies in column c's band (outside the band a frame
-// sensitivity is zero and its S word is not written).  The columns visited
-// are those set in the masks of the rows' layers.  Columns are taken four at
-// a time and all their words loaded before any is used, so a node waits for
-// one memory round trip per four columns.
+// ascending target order, visiting the columns set in the node's window
+// mask (`mrow`, maskWords words).  Columns are taken four at a time and all
+// their row words loaded before any is used, so a node waits for one memory
+// round trip per four columns.
 template <typename F>
-__device__ __forceinline__ void forEachTarget(const LayerMasks &lm, const int2 *band, const uint4 code,
-                                              uint32_t P, const uint32_t *__restrict__ S0, uint64_t colStride,
-                                              F &&f) {
-  constexpr uint32_t kPosMask = (1u << kRowPosBits) - 1;
-  const uint32_t cd[4] = {code.x, code.y, code.z, code.w};
-  bool ok[4];
-  uint32_t lay[4], idx[4];
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    ok[q] = cd[q] != NONE;
-    lay[q] = ok[q] ? cd[q] >> kRowPosBits : 0u;
-    idx[q] = ok[q] ? lay[q] * P + (cd[q] & kPosMask) : 0u;
-  }
-  auto maskWord = [&](uint32_t w) {
-    uint32_t m = 0;
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-      if (ok[q]) m |= lm.word(lay[q], w);
-    return m;
-  };
+__device__ __forceinline__ void forEachTargetMasked(const uint32_t *mrow, uint32_t maskWords, const int2 *band,
+                                                    const FrameRows &r, const uint32_t *__restrict__ S0,
+                                                    uint64_t colStride, F &&f) {
+  if (!r.ok[0] || !maskWords) return;
   constexpr int kBatch = 4;
-  uint32_t w = 0;
-  uint32_t m = (lm.maskWords && ok[0]) ? maskWord(0) : 0u;
+  uint32_t w = 0, m = mrow[0];
   for (;;) {
     uint32_t col[kBatch], x[kBatch][4];
 #pragma unroll
     for (int j = 0; j < kBatch; ++j) {
-      while (m == 0u && ok[0] && w + 1 < lm.maskWords) m = maskWord(++w);
+      while (m == 0u && w + 1 < maskWords) m = mrow[++w];
       col[j] = m ? w * 32 + static_cast<uint32_t>(__builtin_ctz(m)) : NONE;
       m &= m - 1;
     }
@@ -607,21 +609,39 @@ __device__ __forceinline__ void forEachTarget(const LayerMasks &lm, const int2 *
     for (int j = 0; j < kBatch; ++j)
 #pragma unroll
       for (int q = 0; q < 4; ++q)
-        x[j][q] = (col[j] != NONE && ok[q]) ? S0[static_cast<uint64_t>(col[j]) * colStride + idx[q]] : 0u;
+        x[j][q] = (col[j] != NONE && r.ok[q]) ? S0[static_cast<uint64_t>(col[j]) * colStride + r.idx[q]] : 0u;
 #pragma unroll
     for (int j = 0; j < kBatch; ++j) {
       if (col[j] == NONE) continue;
-      const int2 bd = band[col[j]];
-      uint32_t v = 0;
-#pragma unroll
-      for (int q = 0; q < 4; ++q)
-        if (ok[q] && bd.x <= static_cast<int32_t>(lay[q]) && static_cast<int32_t>(lay[q]) <= bd.y) v ^= x[j][q];
+      uint32_t v = bandXor(r, band[col[j]], x[j]);
       while (v) {
         f(col[j] * 32 + static_cast<uint32_t>(__builtin_ctz(v)));
         v &= v - 1;
       }
     }
     if (col[kBatch - 1] == NONE) break;
+  }
+}
+
+// The same walk without a mask table: every column's band is tested (one
+// memory round trip per column holding a row).  Used by the emit stage for
+// the few nodes without a stashed target row, and by classify when a
+// workgroup's mask table would not fit in LDS.
+template <typename F>
+__device__ __forceinline__ void forEachTargetScan(uint32_t numCols, const int2 *band, const FrameRows &r,
+                                                  const uint32_t *__restrict__ S0, uint64_t colStride, F &&f) {
+  if (!r.ok[0]) return;
+  for (uint32_t c = 0; c < numCols; ++c) {
+    const int2 bd = band[c];
+    if (bd.y < static_cast<int32_t>(r.lo) || bd.x > static_cast<int32_t>(r.lo + r.span)) continue;
+    uint32_t x[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) x[q] = r.ok[q] ? S0[static_cast<uint64_t>(c) * colStride + r.idx[q]] : 0u;
+    uint32_t v = bandXor(r, bd, x);
+    while (v) {
+      f(c * 32 + static_cast<uint32_t>(__builtin_ctz(v)));
+      v &= v - 1;
+    }
   }
 }
 
@@ -656,6 +676,7 @@ struct ClassifyOut {
   uint32_t stashMaxCols;  // circuits with more columns get no rows (0: none do)
 };
 
+template <bool TABLE>
 __global__ __launch_bounds__(kClassBlock) void classifyKernel(
     const VDesc *__restrict__ desc, const uint2 *__restrict__ tiles, const uint32_t *__restrict__ rankLayer,
     const int2 *__restrict__ band, const uint32_t *__restrict__ S, const double *__restrict__ rankProb,
@@ -663,7 +684,7 @@ __global__ __launch_bounds__(kClassBlock) void classifyKernel(
   // per wave: 64 rows of kStashWords words, padded to kStashWords + 1 so
   // that a lane's writes and the row-major read-out avoid bank conflicts
   __shared__ uint32_t rows[kClassBlock / 64][64 * (kStashWords + 1)];
-  __shared__ uint32_t maskTab[kMaskTableWords];
+  __shared__ uint32_t maskTab[TABLE ? kMaskTableWords : 1];
   __shared__ int2 bandTab[kBandTableCols];
   const uint2 t = tiles[blockIdx.x];
   const VDesc d = desc[t.x];
@@ -674,32 +695,36 @@ __global__ __launch_bounds__(kClassBlock) void classifyKernel(
   const uint4 code = rankRows[gi];
   const double prob = rankProb[gi];
   const uint32_t last = (t.y + kClassBlock < d.numRanks ? t.y + kClassBlock : d.numRanks) - 1;
-  // the rows of a node at layer l lie in layers l .. l + rowSpan
+  // the rows of a node at layer l lie in layers l .. l + rowSpan (the host
+  // picks TABLE only when every workgroup's table fits kMaskTableWords)
   const uint32_t l0 = rankLayer[d.rankOff + t.y];
-  const uint32_t l1 = min(rankLayer[d.rankOff + last] + d.rowSpan, d.numLayers - 1);
+  const uint32_t nl = min(rankLayer[d.rankOff + last] + d.rowSpan, d.numLayers - 1) - l0 + 1;
   const int2 *bandC = band + d.colOff;
   if (d.numCols <= kBandTableCols) {
     for (uint32_t c = threadIdx.x; c < d.numCols; c += blockDim.x) bandTab[c] = bandC[c];
-    bandC = bandTab;  // filled by the barriers of buildMaskTable, or the one below
+    bandC = bandTab;  // filled by the barrier below
   }
-  LayerMasks lm{nullptr, band + d.colOff, l0, d.numCols, d.maskWords};
-  if ((l1 - l0 + 1) * d.maskWords <= kMaskTableWords) {
-    buildMaskTable(maskTab, l0, l1 - l0 + 1, d.numCols, d.maskWords, lm.band);
-    lm.tab = maskTab;
-  } else {
+  if constexpr (TABLE)
+    buildMaskTable(maskTab, l0, nl, d.rowSpan, d.numCols, d.maskWords, band + d.colOff);
+  else
     __syncthreads();
-  }
   uint16_t *row = reinterpret_cast<uint16_t *>(rows[wave] + lane * (kStashWords + 1));
   const bool stashed = d.numCols <= o.stashMaxCols;
   if (i < d.numRanks) {
     uint64_t h = kFnvBasis, h2 = 0x84222325cbf29ce4ull;
     uint32_t cnt = 0;
-    forEachTarget(lm, bandC, code, d.layerStride, S + d.sOff, d.colStride, [&](uint32_t tg) {
+    const FrameRows fr = decodeRows(code, d.layerStride);
+    auto take = [&](uint32_t tg) {
       h = (h ^ tg) * kFnvPrime;
       h2 = mix2(h2, tg);
       if (cnt < kStashTargets) row[cnt] = static_cast<uint16_t>(tg);
       cnt++;
-    });
+    };
+    if constexpr (TABLE)
+      forEachTargetMasked(maskTab + (fr.span * nl + (fr.lo - l0)) * d.maskWords, d.maskWords, bandC, fr,
+                          S + d.sOff, d.colStride, take);
+    else
+      forEachTargetScan(d.numCols, bandC, fr, S + d.sOff, d.colStride, take);
     if (cnt && cnt <= 32)
       for (uint32_t k = cnt; k < 32; ++k) h = (h ^ 0xFFFFFFFFull) * kFnvPrime;
     h2 = mix2(h2, cnt);
@@ -1024,9 +1049,8 @@ __global__ __launch_bounds__(256) void emitKernel(const VDesc *__restrict__ desc
         if (j < n && pos + j < out.capTargets) out.targets[pos + j] = (w[j / 2] >> (16 * (j % 2))) & 0xFFFFu;
       pos += n;
     } else {
-      const LayerMasks lm{nullptr, in.band + d.colOff, 0u, d.numCols, d.maskWords};
-      forEachTarget(lm, in.band + d.colOff, in.rankRows[rk], d.layerStride, in.S + d.sOff, d.colStride,
-                    [&](uint32_t tg) {
+      forEachTargetScan(d.numCols, in.band + d.colOff, decodeRows(in.rankRows[rk], d.layerStride), in.S + d.sOff,
+                        d.colStride, [&](uint32_t tg) {
         if (pos < out.capTargets) out.targets[pos] = tg;
         pos++;
       });
@@ -1274,6 +1298,7 @@ struct Pipeline::Impl {
   size_t oDesc = 0, oItems = 0, oTiles = 0, oAct = 0, oSeed = 0, oColHi = 0,
          oColLow = 0, oRankLayer = 0, oRankProb = 0, oRankRows = 0, inBytes = 0;
   uint32_t V = 0, nItems = 0, nTiles = 0, B = 0, R = 0, Ccols = 0, rpt = 1;
+  bool classTable = true;  // every classify workgroup's mask table fits in LDS
   uint64_t Swords = 0;
   size_t sweepLdsBytes = 0;
   int seedsInLds = 1;
@@ -1369,6 +1394,7 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   uint32_t bucketBase = 0, maxRpt = 1, maxThreads = 64;
   size_t ldsRing = 0, ldsAll = 0;  // largest ring (+ zero word), largest ring + seeds
   std::vector<uint2> items, tiles;
+  bool classTable = true;
   std::vector<std::pair<int32_t, uint32_t>> order;  // (band estimate, item index)
   for (uint32_t v = 0; v < V; ++v) {
     const StepgImage &g = *imgs[v];
@@ -1402,7 +1428,12 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
       order.push_back({est, static_cast<uint32_t>(items.size())});
       items.push_back(make_uint2(v, c));
     }
-    for (uint32_t t = 0; t < g.numRanks; t += kClassBlock) tiles.push_back(make_uint2(v, t));
+    for (uint32_t t = 0; t < g.numRanks; t += kClassBlock) {
+      tiles.push_back(make_uint2(v, t));
+      const uint32_t last = std::min(t + kClassBlock, g.numRanks) - 1;
+      const uint32_t nl = std::min(g.rankLayer[last] + g.rowSpan, g.numLayers - 1) - g.rankLayer[t] + 1;
+      if (maskTableWords(nl, g.rowSpan, (g.numCols + 31) / 32) > kMaskTableWords) classTable = false;
+    }
     actOff += static_cast<uint64_t>(kSweepLeadPad + g.numLayers) * g.layerStride;
     seedOff += static_cast<uint64_t>(g.numCols) * g.numSeeds;
     colOff += g.numCols;
@@ -1436,6 +1467,7 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   m.V = V;
   m.nItems = static_cast<uint32_t>(sortedItems.size());
   m.nTiles = static_cast<uint32_t>(tiles.size());
+  m.classTable = classTable;
   m.B = bucketBase;
   m.R = static_cast<uint32_t>(rankOff);
   m.Ccols = static_cast<uint32_t>(colOff);
@@ -1609,8 +1641,12 @@ void Pipeline::run(bool timed) {
                    m.countStride, m.pkey.as<uint64_t>(), m.prank.as<uint32_t>(), m.ovKey.as<uint64_t>(),
                    m.ovRank.as<uint32_t>(), m.ovBucket.as<uint32_t>(), m.flags(), m.stash.as<uint32_t>(),
                    stashMaxCols};
-    classifyKernel<<<m.nTiles, kClassBlock, 0, st>>>(desc, tiles, rankLayer, m.band.as<int2>(),
-                                                     m.S.as<uint32_t>(), rankProb, rankRows, co);
+    if (m.classTable && gMaskTable.load())
+      classifyKernel<true><<<m.nTiles, kClassBlock, 0, st>>>(desc, tiles, rankLayer, m.band.as<int2>(),
+                                                             m.S.as<uint32_t>(), rankProb, rankRows, co);
+    else
+      classifyKernel<false><<<m.nTiles, kClassBlock, 0, st>>>(desc, tiles, rankLayer, m.band.as<int2>(),
+                                                              m.S.as<uint32_t>(), rankProb, rankRows, co);
   }
   mark(2);
   RunOut o{m.info.as<uint4>(),
@@ -1788,6 +1824,8 @@ double Pipeline::benchDevice(uint32_t iters) {
 void setBucketFill(uint32_t fill) { gBucketFill.store(std::max<uint32_t>(fill, 1)); }
 void setStash(bool on) { gStash.store(on ? 1u : 0u); }
 bool stash() { return gStash.load() != 0; }
+void setMaskTable(bool on) { gMaskTable.store(on ? 1u : 0u); }
+bool maskTable() { return gMaskTable.load() != 0; }
 uint32_t bucketFill() { return gBucketFill.load(); }
 
 uint64_t Pipeline::deviceBytes() const {

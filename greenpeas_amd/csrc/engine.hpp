@@ -101,6 +101,11 @@ uint32_t bucketFill();
 // for circuits with more than 65536 targets).  Read at each run.
 void setStash(bool on);
 bool stash();
+// Whether classify may build per-workgroup column-mask tables in LDS (on
+// by default; off, every node tests every column's band -- the path of
+// workgroups whose table would not fit).  Read at each run.
+void setMaskTable(bool on);
+bool maskTable();
 
 // Segmented XOR-merge kernels (differentiable path); pointers are device
 // pointers, stream is a hipStream_t.

@@ -174,7 +174,10 @@ int gp_estimate_device_bytes(const char *circuit_text, int correlation_level,
  *   "stash": 1 (default) stores each node's first 32 targets as a 16-bit
  *   row for the emit stage; 0 makes emit gather every class's targets from
  *   the sensitivity matrix (what circuits above 65536 targets always do);
- *   read at each compile. */
+ *   read at each compile.
+ *   "mask_table": 1 (default) lets the class stage keep per-workgroup column
+ *   masks in LDS; 0 makes every node test every column's time band (what
+ *   circuits whose masks do not fit in LDS always do); read at each compile. */
 int gp_set_tuning(const char *name, uint32_t value);
 int gp_get_tuning(const char *name, uint32_t *value);
 

@@ -215,3 +215,27 @@ def test_emit_without_target_rows(oracle):
             assert get_driver(t).compile_detector_error_model() == dem0
     finally:
         autotune.set_tuning("stash", 1)
+
+
+@pytest.mark.parametrize("stash", [1, 0])
+def test_class_stage_without_mask_tables(oracle, stash):
+    """With the class stage's LDS column masks switched off, every node tests
+    every column's time band (the path of circuits whose masks do not fit in
+    LDS): identical hashes, order, targets and probabilities to the STEPG
+    restatement, with and without the stored target rows."""
+    from greenpeas_amd import autotune
+
+    try:
+        autotune.set_tuning("mask_table", 0)
+        autotune.set_tuning("stash", stash)
+        for name, level in (("surface3", 0), ("surface5", 2), ("bb6", 1), ("bb12", 2)):
+            text = read_circuit(name)
+            dem = get_driver(text, correlation_level=level).compile_detector_error_model()
+            ref = oracle.compile_stepg(text, level)
+            assert np.array_equal(dem.hashes, ref.hashes)
+            assert np.array_equal(dem.offsets, ref.offsets)
+            assert np.array_equal(dem.targets, ref.targets)
+            assert np.max(np.abs(dem.probs - ref.probs) / ref.probs) <= REL
+    finally:
+        autotune.set_tuning("mask_table", 1)
+        autotune.set_tuning("stash", 1)
