@@ -21,7 +21,8 @@ GP_RUNTIME_ERROR = 2
 EXPORTS = (
     "gp_last_error", "gp_version",
     "gp_get_driver", "gp_driver_destroy", "gp_compile_detector_error_model", "gp_driver_parse",
-    "gp_dem_shape", "gp_dem_copy", "gp_dem_text", "gp_driver_params", "gp_driver_timings",
+    "gp_dem_shape", "gp_dem_copy", "gp_dem_acquire", "gp_dem_release", "gp_dem_text", "gp_driver_params",
+    "gp_driver_timings",
     "gp_driver_bench_device", "gp_debug_pnodes", "gp_debug_sensitivities", "gp_debug_stepg",
     "gp_build_stepg_host",
     "gp_batch_create", "gp_batch_destroy", "gp_batch_set_circuits", "gp_batch_compile",
@@ -33,6 +34,22 @@ EXPORTS = (
 )
 
 _lib = None
+
+
+class DemView(ctypes.Structure):
+    """gp_dem_view: a DEM's arrays in the driver's pinned result memory."""
+
+    _fields_ = [
+        ("probs", ctypes.c_void_p),
+        ("offsets", ctypes.c_void_p),
+        ("targets", ctypes.c_void_p),
+        ("hashes", ctypes.c_void_p),
+        ("num_errors", ctypes.c_uint32),
+        ("num_detectors", ctypes.c_uint32),
+        ("num_observables", ctypes.c_uint32),
+        ("num_targets", ctypes.c_uint64),
+        ("token", ctypes.c_void_p),
+    ]
 
 
 def load() -> ctypes.CDLL:
@@ -65,6 +82,8 @@ def load() -> ctypes.CDLL:
     sig("gp_driver_parse", [P, cs])
     sig("gp_dem_shape", [P, pu32, pu64, pu32, pu32])
     sig("gp_dem_copy", [P, P, P, P, P])
+    sig("gp_dem_acquire", [P, ctypes.POINTER(DemView)])
+    sig("gp_dem_release", [P])
     sig("gp_dem_text", [P, P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)])
     sig("gp_driver_params", [P, P])
     sig("gp_driver_timings", [P, P])

@@ -176,9 +176,14 @@ int gp_compile_detector_error_model(gp_driver *driver, const char *circuit_text)
     }
     if (!driver->haveCircuit) throw RuntimeError("no circuit parsed");
     driver->haveResult = false;
-    // upload the STEPG (Driver.hpp:145-152), run the stages, copy back
-    driver->pipe->load({&driver->parsed.image});
-    driver->loaded = true;
+    // upload the STEPG (Driver.hpp:145-152), run the stages, copy back; the
+    // packed input arena of an unchanged circuit is sent again as it is
+    if (driver->loaded) {
+      driver->pipe->upload();
+    } else {
+      driver->pipe->load({&driver->parsed.image});
+      driver->loaded = true;
+    }
     driver->pipe->run(true);
     driver->pipe->download();
     driver->pipe->result(0, driver->result);
@@ -209,6 +214,33 @@ int gp_dem_copy(const gp_driver *driver, double *probs, uint32_t *offsets, uint3
     if (offsets) std::memcpy(offsets, r.offsets, 4ull * (r.numErrors + 1));
     if (targets) std::memcpy(targets, r.targets, 4ull * r.numTargets);
     if (hashes) std::memcpy(hashes, r.hashes, 8ull * r.numErrors);
+  });
+}
+
+int gp_dem_acquire(const gp_driver *driver, gp_dem_view *view) {
+  return guarded([&] {
+    if (!driver || !view) throw InvalidArgument("NULL argument");
+    if (!driver->haveResult) throw RuntimeError("no compiled detector error model");
+    const auto &r = driver->result;
+    // a driver compiles one circuit, whose offsets are the block's own
+    if (!r.offsetsRel.empty()) throw RuntimeError("result offsets are not in the result block");
+    auto *h = new gpa::ResultHandle(driver->pipe->resultBlock());
+    view->probs = r.probs;
+    view->offsets = r.offsets;
+    view->targets = r.targets;
+    view->hashes = r.hashes;
+    view->num_errors = r.numErrors;
+    view->num_detectors = r.numDetectors;
+    view->num_observables = r.numObservables;
+    view->num_targets = r.numTargets;
+    view->token = h;
+  });
+}
+
+int gp_dem_release(void *token) {
+  return guarded([&] {
+    if (!token) throw InvalidArgument("token is NULL");
+    delete static_cast<gpa::ResultHandle *>(token);
   });
 }
 
@@ -649,6 +681,7 @@ int gp_set_tuning(const char *name, uint32_t value) {
     else if (std::string(name) == "sweep_rpt") gpa::setSweepMinRpt(value);
     else if (std::string(name) == "stash") gpa::setStash(value != 0);
     else if (std::string(name) == "mask_table") gpa::setMaskTable(value != 0);
+    else if (std::string(name) == "prob_table") gpa::setProbTable(value != 0);
     else throw InvalidArgument(std::string("unknown tuning parameter: ") + name);
   });
 }
@@ -660,6 +693,7 @@ int gp_get_tuning(const char *name, uint32_t *value) {
     else if (std::string(name) == "sweep_rpt") *value = gpa::sweepMinRpt();
     else if (std::string(name) == "stash") *value = gpa::stash() ? 1u : 0u;
     else if (std::string(name) == "mask_table") *value = gpa::maskTable() ? 1u : 0u;
+    else if (std::string(name) == "prob_table") *value = gpa::probTable() ? 1u : 0u;
     else throw InvalidArgument(std::string("unknown tuning parameter: ") + name);
   });
 }

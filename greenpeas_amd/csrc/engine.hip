@@ -10,6 +10,8 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <memory>
+#include <mutex>
 #include <cstring>
 #include <string>
 
@@ -43,10 +45,30 @@ struct VDesc {
   uint32_t numLayers, npl, numCols, numDetectors;
   uint32_t numObservables, numRanks, numSeeds, bucketBits;
   uint32_t bucketBase, numBuckets, maskWords, layerStride;
-  uint32_t sweepThreads, sweepRpt, rowSpan, pad1;  // rowSpan: see StepgImage
+  uint32_t sweepThreads, sweepRpt, rowSpan, probDict;  // rowSpan: see StepgImage
   uint64_t actOff, seedOff, colOff, rankOff, sOff;
   uint64_t colStride;  // words per S column: numLayers * layerStride frame rows + kSinkSlots sinks
+  uint64_t spillOff;   // first spill entry of the circuit's ranks
+  uint64_t probOff;    // first probability (table entry if probDict, else the ranks' own)
 };
+
+// A rank's four rankRows codes from its compact code (the format is given
+// with kSpillBase); `spill` = the circuit's spill entries.
+__host__ __device__ inline uint4 expandRankCode(const uint2 c, const uint4 *spill) {
+  if ((c.x >> 16) == kSpillBase) return spill[(static_cast<uint64_t>(c.x & 0xFFFFu) << 32) | c.y];
+  const uint32_t base = c.x >> 16;
+  auto full = [base](uint32_t v) {
+    return v == kRowNone16 ? 0xFFFFFFFFu
+                           : ((base + (v >> kRowPosBits)) << kRowPosBits) | (v & ((1u << kRowPosBits) - 1));
+  };
+  return make_uint4(full(c.x & 0xFFFFu), full(c.y & 0xFFFFu), full(c.y >> 16), 0xFFFFFFFFu);
+}
+
+// A rank's probability: table entry or its own value.
+__device__ __forceinline__ double rankProbability(const VDesc &d, const double *probVal, const uint16_t *probIdx,
+                                                  uint64_t gi) {
+  return probVal[d.probOff + (d.probDict ? probIdx[gi] : gi - d.rankOff)];
+}
 
 // Counter block zeroed at the start of every run: [flags | bucket counts].
 // Per-run flag words, zeroed by the sweep kernel: [0] hash collision,
@@ -676,29 +698,35 @@ struct ClassifyOut {
   uint32_t stashMaxCols;  // circuits with more columns get no rows (0: none do)
 };
 
+// Ranks' inputs: compact frame rows, spill entries, probabilities.
+struct RankIn {
+  const uint2 *code;
+  const uint4 *spill;
+  const double *probVal;
+  const uint16_t *probIdx;
+};
+
 template <bool TABLE>
 __global__ __launch_bounds__(kClassBlock) void classifyKernel(
-    const VDesc *__restrict__ desc, const uint2 *__restrict__ tiles, const uint32_t *__restrict__ rankLayer,
-    const int2 *__restrict__ band, const uint32_t *__restrict__ S, const double *__restrict__ rankProb,
-    const uint4 *__restrict__ rankRows, ClassifyOut o) {
+    const VDesc *__restrict__ desc, const uint4 *__restrict__ tiles, const int2 *__restrict__ band,
+    const uint32_t *__restrict__ S, RankIn rin, ClassifyOut o) {
   // per wave: 64 rows of kStashWords words, padded to kStashWords + 1 so
   // that a lane's writes and the row-major read-out avoid bank conflicts
   __shared__ uint32_t rows[kClassBlock / 64][64 * (kStashWords + 1)];
   __shared__ uint32_t maskTab[TABLE ? kMaskTableWords : 1];
   __shared__ int2 bandTab[kBandTableCols];
-  const uint2 t = tiles[blockIdx.x];
+  // tile: {circuit, first rank, lowest layer of its ranks' rows, layers
+  // spanned by them}; the host picks TABLE only when every workgroup's
+  // table fits kMaskTableWords
+  const uint4 t = tiles[blockIdx.x];
   const VDesc d = desc[t.x];
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t i = t.y + threadIdx.x;
-  // this node's record and frame rows, loaded before the tables are built
+  // this node's frame rows and probability, loaded before the tables are built
   const uint64_t gi = d.rankOff + (i < d.numRanks ? i : t.y);
-  const uint4 code = rankRows[gi];
-  const double prob = rankProb[gi];
-  const uint32_t last = (t.y + kClassBlock < d.numRanks ? t.y + kClassBlock : d.numRanks) - 1;
-  // the rows of a node at layer l lie in layers l .. l + rowSpan (the host
-  // picks TABLE only when every workgroup's table fits kMaskTableWords)
-  const uint32_t l0 = rankLayer[d.rankOff + t.y];
-  const uint32_t nl = min(rankLayer[d.rankOff + last] + d.rowSpan, d.numLayers - 1) - l0 + 1;
+  const uint2 ccode = rin.code[gi];
+  const double prob = rankProbability(d, rin.probVal, rin.probIdx, gi);
+  const uint32_t l0 = t.z, nl = t.w;
   const int2 *bandC = band + d.colOff;
   if (d.numCols <= kBandTableCols) {
     for (uint32_t c = threadIdx.x; c < d.numCols; c += blockDim.x) bandTab[c] = bandC[c];
@@ -713,7 +741,7 @@ __global__ __launch_bounds__(kClassBlock) void classifyKernel(
   if (i < d.numRanks) {
     uint64_t h = kFnvBasis, h2 = 0x84222325cbf29ce4ull;
     uint32_t cnt = 0;
-    const FrameRows fr = decodeRows(code, d.layerStride);
+    const FrameRows fr = decodeRows(expandRankCode(ccode, rin.spill + d.spillOff), d.layerStride);
     auto take = [&](uint32_t tg) {
       h = (h ^ tg) * kFnvPrime;
       h2 = mix2(h2, tg);
@@ -940,7 +968,8 @@ struct EmitIn {
   const double *runP;
   const uint64_t *runHash;
   const uint32_t *runNnz, *S, *stash;
-  const uint4 *rankRows;
+  const uint2 *rankCode;
+  const uint4 *spill;
   uint32_t stashMaxCols;
   const int2 *band;
 };
@@ -1049,8 +1078,8 @@ __global__ __launch_bounds__(256) void emitKernel(const VDesc *__restrict__ desc
         if (j < n && pos + j < out.capTargets) out.targets[pos + j] = (w[j / 2] >> (16 * (j % 2))) & 0xFFFFu;
       pos += n;
     } else {
-      forEachTargetScan(d.numCols, in.band + d.colOff, decodeRows(in.rankRows[rk], d.layerStride), in.S + d.sOff,
-                        d.colStride, [&](uint32_t tg) {
+      const FrameRows fr = decodeRows(expandRankCode(in.rankCode[rk], in.spill + d.spillOff), d.layerStride);
+      forEachTargetScan(d.numCols, in.band + d.colOff, fr, in.S + d.sOff, d.colStride, [&](uint32_t tg) {
         if (pos < out.capTargets) out.targets[pos] = tg;
         pos++;
       });
@@ -1249,6 +1278,9 @@ struct DevBuf {
 };
 
 struct HostBuf {
+  HostBuf() = default;
+  HostBuf(const HostBuf &) = delete;
+  HostBuf &operator=(const HostBuf &) = delete;
   void *p = nullptr;
   size_t cap = 0;
   ~HostBuf() {
@@ -1269,6 +1301,43 @@ struct HostBuf {
   }
 };
 
+}  // namespace
+
+struct ResultBlock {
+  HostBuf p, off, hash, targets;
+};
+
+namespace {
+
+// Result blocks whose holders let go, kept for the next downloads.
+struct ResultPool {
+  std::mutex mu;
+  std::vector<std::unique_ptr<ResultBlock>> free;
+};
+
+// A block from the pool (or a new one); when its last handle goes, it
+// returns to the pool, or is freed once the pool is gone.
+std::shared_ptr<ResultBlock> takeBlock(const std::shared_ptr<ResultPool> &pool) {
+  std::unique_ptr<ResultBlock> b;
+  {
+    std::lock_guard<std::mutex> lk(pool->mu);
+    if (!pool->free.empty()) {
+      b = std::move(pool->free.back());
+      pool->free.pop_back();
+    }
+  }
+  if (!b) b = std::make_unique<ResultBlock>();
+  std::weak_ptr<ResultPool> wp = pool;
+  return std::shared_ptr<ResultBlock>(b.release(), [wp](ResultBlock *r) {
+    if (auto p = wp.lock()) {
+      std::lock_guard<std::mutex> lk(p->mu);
+      p->free.emplace_back(r);
+    } else {
+      delete r;
+    }
+  });
+}
+
 size_t align256(size_t x) { return (x + 255) & ~static_cast<size_t>(255); }
 
 }  // namespace
@@ -1276,7 +1345,8 @@ size_t align256(size_t x) { return (x + 255) & ~static_cast<size_t>(255); }
 struct Pipeline::Impl {
   int device;
   hipStream_t stream = nullptr;
-  hipEvent_t ev[8] = {};
+  hipEvent_t ev[8] = {};  // [0, 5): stage marks of a timed run; 5, 6: upload
+  bool uploadPending = false;  // upload events recorded, not yet read
   bool frozen = false;
   bool timedRun = false;
   hipEvent_t *evOverride = nullptr;  // per-iteration events of benchDevice
@@ -1296,7 +1366,7 @@ struct Pipeline::Impl {
   DevBuf in;
   HostBuf inHost;
   size_t oDesc = 0, oItems = 0, oTiles = 0, oAct = 0, oSeed = 0, oColHi = 0,
-         oColLow = 0, oRankLayer = 0, oRankProb = 0, oRankRows = 0, inBytes = 0;
+         oColLow = 0, oRankCode = 0, oSpill = 0, oProbIdx = 0, oProbVal = 0, inBytes = 0;
   uint32_t V = 0, nItems = 0, nTiles = 0, B = 0, R = 0, Ccols = 0, rpt = 1;
   bool classTable = true;  // every classify workgroup's mask table fits in LDS
   uint64_t Swords = 0;
@@ -1319,8 +1389,11 @@ struct Pipeline::Impl {
       scanTmp, vBounds, members;
   uint64_t capTargets = 0;
 
-  // host results
-  HostBuf hP, hOff, hHash, hTargets, hVB, hFlags;
+  // host results: the block of the last download, blocks released by
+  // their holders for reuse
+  std::shared_ptr<ResultBlock> res;
+  std::shared_ptr<ResultPool> pool = std::make_shared<ResultPool>();
+  HostBuf hVB, hFlags;
   uint32_t totalClasses = 0;
   uint64_t totalTargets = 0;
   std::vector<uint32_t> vClass, vTgt;
@@ -1390,10 +1463,11 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   // layout
   m.descs.assign(V, VDesc{});
   variants_.assign(V, VariantInfo{});
-  uint64_t actOff = 0, seedOff = 0, colOff = 0, rankOff = 0, sOff = 0;
+  uint64_t actOff = 0, seedOff = 0, colOff = 0, rankOff = 0, sOff = 0, spillOff = 0, probOff = 0;
   uint32_t bucketBase = 0, maxRpt = 1, maxThreads = 64;
   size_t ldsRing = 0, ldsAll = 0;  // largest ring (+ zero word), largest ring + seeds
-  std::vector<uint2> items, tiles;
+  std::vector<uint2> items;
+  std::vector<uint4> tiles;
   bool classTable = true;
   std::vector<std::pair<int32_t, uint32_t>> order;  // (band estimate, item index)
   for (uint32_t v = 0; v < V; ++v) {
@@ -1420,6 +1494,9 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
     d.rankOff = rankOff;
     d.sOff = sOff;
     d.colStride = static_cast<uint64_t>(g.numLayers) * g.layerStride + kSinkSlots;
+    d.spillOff = spillOff;
+    d.probOff = probOff;
+    d.probDict = g.probTab.empty() ? 0u : 1u;
     variants_[v] = VariantInfo{g.numRanks, g.numCols, g.numDetectors, g.numObservables, bucketBase,
                                d.numBuckets};
     for (uint32_t c = 0; c < g.numCols; ++c) {
@@ -1429,15 +1506,18 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
       items.push_back(make_uint2(v, c));
     }
     for (uint32_t t = 0; t < g.numRanks; t += kClassBlock) {
-      tiles.push_back(make_uint2(v, t));
+      // the rows of a rank at layer l lie in layers l .. l + rowSpan
       const uint32_t last = std::min(t + kClassBlock, g.numRanks) - 1;
       const uint32_t nl = std::min(g.rankLayer[last] + g.rowSpan, g.numLayers - 1) - g.rankLayer[t] + 1;
+      tiles.push_back(make_uint4(v, t, g.rankLayer[t], nl));
       if (maskTableWords(nl, g.rowSpan, (g.numCols + 31) / 32) > kMaskTableWords) classTable = false;
     }
     actOff += static_cast<uint64_t>(kSweepLeadPad + g.numLayers) * g.layerStride;
     seedOff += static_cast<uint64_t>(g.numCols) * g.numSeeds;
     colOff += g.numCols;
     rankOff += g.numRanks;
+    spillOff += g.rankSpill.size() / 4;
+    probOff += d.probDict ? g.probTab.size() : g.numRanks;
     sOff += d.colStride * g.numCols;
     bucketBase += d.numBuckets;
     maxRpt = std::max(maxRpt, g.sweepRpt);
@@ -1482,21 +1562,22 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   };
   m.oDesc = place(sizeof(VDesc) * V);
   m.oItems = place(sizeof(uint2) * m.nItems);
-  m.oTiles = place(sizeof(uint2) * m.nTiles);
+  m.oTiles = place(sizeof(uint4) * m.nTiles);
   m.oAct = place(16 * actOff);
   m.oSeed = place(4 * seedOff);
   m.oColHi = place(4 * colOff);
   m.oColLow = place(4 * colOff);
-  m.oRankLayer = place(4 * rankOff);
-  m.oRankProb = place(8 * rankOff);
-  m.oRankRows = place(16 * rankOff);
+  m.oRankCode = place(8 * rankOff);
+  m.oSpill = place(16 * spillOff);
+  m.oProbIdx = place(2 * rankOff);
+  m.oProbVal = place(8 * probOff);
   m.inBytes = off;
   m.in.ensure(m.inBytes, frozen, "inputs");
   m.inHost.ensure(m.inBytes);
   char *hb = m.inHost.as<char>();
   std::memcpy(hb + m.oDesc, m.descs.data(), sizeof(VDesc) * V);
   std::memcpy(hb + m.oItems, sortedItems.data(), sizeof(uint2) * m.nItems);
-  std::memcpy(hb + m.oTiles, tiles.data(), sizeof(uint2) * m.nTiles);
+  std::memcpy(hb + m.oTiles, tiles.data(), sizeof(uint4) * m.nTiles);
   for (uint32_t v = 0; v < V; ++v) {
     const StepgImage &g = *imgs[v];
     const VDesc &d = m.descs[v];
@@ -1504,9 +1585,15 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
     std::memcpy(hb + m.oSeed + 4 * d.seedOff, g.seedVals.data(), 4 * g.seedVals.size());
     std::memcpy(hb + m.oColHi + 4 * d.colOff, g.colHi.data(), 4 * g.colHi.size());
     std::memcpy(hb + m.oColLow + 4 * d.colOff, g.colLow.data(), 4 * g.colLow.size());
-    std::memcpy(hb + m.oRankLayer + 4 * d.rankOff, g.rankLayer.data(), 4 * g.rankLayer.size());
-    std::memcpy(hb + m.oRankProb + 8 * d.rankOff, g.rankProb.data(), 8 * g.rankProb.size());
-    std::memcpy(hb + m.oRankRows + 16 * d.rankOff, g.rankRows.data(), 4 * g.rankRows.size());
+    std::memcpy(hb + m.oRankCode + 8 * d.rankOff, g.rankCode.data(), 4 * g.rankCode.size());
+    std::memcpy(hb + m.oSpill + 16 * d.spillOff, g.rankSpill.data(), 4 * g.rankSpill.size());
+    if (d.probDict) {
+      std::memcpy(hb + m.oProbIdx + 2 * d.rankOff, g.probIdx.data(), 2 * g.probIdx.size());
+      std::memcpy(hb + m.oProbVal + 8 * d.probOff, g.probTab.data(), 8 * g.probTab.size());
+    } else {
+      std::memset(hb + m.oProbIdx + 2 * d.rankOff, 0, 2ull * g.numRanks);
+      std::memcpy(hb + m.oProbVal + 8 * d.probOff, g.rankProb.data(), 8 * g.rankProb.size());
+    }
   }
 
   // work buffers
@@ -1574,15 +1661,19 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
 
   // bucket counters start at zero; every run's bucket stage leaves them zeroed
   GPA_HIP(hipMemsetAsync(m.bucketCnt.p, 0, m.bucketCnt.cap, m.stream));
+  upload();
+}
 
-  // upload (one copy)
-  GPA_HIP(hipEventRecord(m.ev[0], m.stream));
+void Pipeline::upload() {
+  Impl &m = *impl_;
+  if (!m.inBytes) throw RuntimeError("nothing loaded");
+  GPA_HIP(hipSetDevice(device_));
+  // one copy of the packed inputs, stream-ordered before the next run (its
+  // duration is read once the stream has drained, in download())
+  GPA_HIP(hipEventRecord(m.ev[5], m.stream));
   GPA_HIP(hipMemcpyAsync(m.in.p, m.inHost.p, m.inBytes, hipMemcpyHostToDevice, m.stream));
-  GPA_HIP(hipEventRecord(m.ev[1], m.stream));
-  GPA_HIP(hipStreamSynchronize(m.stream));
-  float ms = 0;
-  GPA_HIP(hipEventElapsedTime(&ms, m.ev[0], m.ev[1]));
-  times_.upload = ms * 1e3;
+  GPA_HIP(hipEventRecord(m.ev[6], m.stream));
+  m.uploadPending = true;
 }
 
 void Pipeline::run(bool timed) {
@@ -1591,14 +1682,14 @@ void Pipeline::run(bool timed) {
   char *ib = m.in.as<char>();
   const VDesc *desc = reinterpret_cast<const VDesc *>(ib + m.oDesc);
   const uint2 *items = reinterpret_cast<const uint2 *>(ib + m.oItems);
-  const uint2 *tiles = reinterpret_cast<const uint2 *>(ib + m.oTiles);
+  const uint4 *tiles = reinterpret_cast<const uint4 *>(ib + m.oTiles);
   const uint4 *act = reinterpret_cast<const uint4 *>(ib + m.oAct);
   const uint32_t *seed = reinterpret_cast<const uint32_t *>(ib + m.oSeed);
   const int32_t *colHi = reinterpret_cast<const int32_t *>(ib + m.oColHi);
   const int32_t *colLow = reinterpret_cast<const int32_t *>(ib + m.oColLow);
-  const uint32_t *rankLayer = reinterpret_cast<const uint32_t *>(ib + m.oRankLayer);
-  const double *rankProb = reinterpret_cast<const double *>(ib + m.oRankProb);
-  const uint4 *rankRows = reinterpret_cast<const uint4 *>(ib + m.oRankRows);
+  const RankIn rin{reinterpret_cast<const uint2 *>(ib + m.oRankCode), reinterpret_cast<const uint4 *>(ib + m.oSpill),
+                   reinterpret_cast<const double *>(ib + m.oProbVal),
+                   reinterpret_cast<const uint16_t *>(ib + m.oProbIdx)};
   hipStream_t st = m.stream;
   hipEvent_t *evs = m.evOverride ? m.evOverride : m.ev;
   auto mark = [&](int i) {
@@ -1642,11 +1733,11 @@ void Pipeline::run(bool timed) {
                    m.ovRank.as<uint32_t>(), m.ovBucket.as<uint32_t>(), m.flags(), m.stash.as<uint32_t>(),
                    stashMaxCols};
     if (m.classTable && gMaskTable.load())
-      classifyKernel<true><<<m.nTiles, kClassBlock, 0, st>>>(desc, tiles, rankLayer, m.band.as<int2>(),
-                                                             m.S.as<uint32_t>(), rankProb, rankRows, co);
+      classifyKernel<true><<<m.nTiles, kClassBlock, 0, st>>>(desc, tiles, m.band.as<int2>(), m.S.as<uint32_t>(),
+                                                             rin, co);
     else
-      classifyKernel<false><<<m.nTiles, kClassBlock, 0, st>>>(desc, tiles, rankLayer, m.band.as<int2>(),
-                                                              m.S.as<uint32_t>(), rankProb, rankRows, co);
+      classifyKernel<false><<<m.nTiles, kClassBlock, 0, st>>>(desc, tiles, m.band.as<int2>(), m.S.as<uint32_t>(),
+                                                              rin, co);
   }
   mark(2);
   RunOut o{m.info.as<uint4>(),
@@ -1677,7 +1768,7 @@ void Pipeline::run(bool timed) {
              m.tgtStart.as<uint32_t>(), m.memberStart.as<uint32_t>(), m.runRank.as<uint32_t>(),
              m.runTgtPre.as<uint32_t>(), m.runMember.as<uint32_t>(), m.runP.as<double>(),
              m.runHash.as<uint64_t>(), m.runNnz.as<uint32_t>(), m.S.as<uint32_t>(),
-             m.stash.as<uint32_t>(), rankRows, stashMaxCols, m.band.as<int2>()};
+             m.stash.as<uint32_t>(), rin.code, rin.spill, stashMaxCols, m.band.as<int2>()};
   EmitOut eout{m.outP.as<double>(), m.outOff.as<uint32_t>(), m.outHash.as<uint64_t>(), m.outMember.as<uint32_t>(),
                m.outTargets.as<uint32_t>(), m.capTargets, m.flags(), m.vBounds.as<uint32_t>()};
   emitKernel<<<std::max<uint32_t>((B + 3) / 4, 1), 256, 0, st>>>(desc, ein, eout);
@@ -1693,6 +1784,12 @@ void Pipeline::download() {
   GPA_HIP(hipMemcpyAsync(m.hVB.p, m.vBounds.p, 4 * (2 * V + 2), hipMemcpyDeviceToHost, m.stream));
   GPA_HIP(hipMemcpyAsync(m.hFlags.p, m.counters.p, 4 * kFlagWords, hipMemcpyDeviceToHost, m.stream));
   GPA_HIP(hipStreamSynchronize(m.stream));
+  if (m.uploadPending) {
+    float ms = 0;
+    GPA_HIP(hipEventElapsedTime(&ms, m.ev[5], m.ev[6]));
+    times_.upload = ms * 1e3;
+    m.uploadPending = false;
+  }
   if (m.timedRun) {
     float ms[4] = {};
     for (int i = 0; i < 4; ++i) GPA_HIP(hipEventElapsedTime(&ms[i], m.ev[i], m.ev[i + 1]));
@@ -1725,16 +1822,19 @@ void Pipeline::download() {
     m.vTgt[v] = vb[2 * v + 1];
   }
   const uint64_t C = m.totalClasses, T = m.totalTargets;
-  m.hP.ensure(8 * C + 8);
-  m.hOff.ensure(4 * (C + 1));
-  m.hHash.ensure(8 * C + 8);
-  m.hTargets.ensure(4 * T + 4);
+  // a block still held by a caller keeps its contents; take another
+  if (!m.res || m.res.use_count() > 1) m.res = takeBlock(m.pool);
+  ResultBlock &rb = *m.res;
+  rb.p.ensure(8 * C + 8);
+  rb.off.ensure(4 * (C + 1));
+  rb.hash.ensure(8 * C + 8);
+  rb.targets.ensure(4 * T + 4);
   if (C) {
-    GPA_HIP(hipMemcpyAsync(m.hP.p, m.outP.p, 8 * C, hipMemcpyDeviceToHost, m.stream));
-    GPA_HIP(hipMemcpyAsync(m.hHash.p, m.outHash.p, 8 * C, hipMemcpyDeviceToHost, m.stream));
+    GPA_HIP(hipMemcpyAsync(rb.p.p, m.outP.p, 8 * C, hipMemcpyDeviceToHost, m.stream));
+    GPA_HIP(hipMemcpyAsync(rb.hash.p, m.outHash.p, 8 * C, hipMemcpyDeviceToHost, m.stream));
   }
-  GPA_HIP(hipMemcpyAsync(m.hOff.p, m.outOff.p, 4 * (C + 1), hipMemcpyDeviceToHost, m.stream));
-  if (T) GPA_HIP(hipMemcpyAsync(m.hTargets.p, m.outTargets.p, 4 * T, hipMemcpyDeviceToHost, m.stream));
+  GPA_HIP(hipMemcpyAsync(rb.off.p, m.outOff.p, 4 * (C + 1), hipMemcpyDeviceToHost, m.stream));
+  if (T) GPA_HIP(hipMemcpyAsync(rb.targets.p, m.outTargets.p, 4 * T, hipMemcpyDeviceToHost, m.stream));
   GPA_HIP(hipStreamSynchronize(m.stream));
   times_.download =
       std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
@@ -1749,14 +1849,24 @@ void Pipeline::result(uint32_t v, VariantResult &out) const {
   out.numTargets = m.vTgt[v + 1] - t0;
   out.numDetectors = variants_[v].numDetectors;
   out.numObservables = variants_[v].numObservables;
-  out.probs = m.hP.as<double>() + c0;
-  out.hashes = m.hHash.as<uint64_t>() + c0;
-  out.targets = m.hTargets.as<uint32_t>() + t0;
-  out.offsetsRel.resize(out.numErrors + 1);
-  const uint32_t *off = m.hOff.as<uint32_t>();
-  for (uint32_t i = 0; i <= out.numErrors; ++i) out.offsetsRel[i] = off[c0 + i] - t0;
-  out.offsets = out.offsetsRel.data();
+  if (!m.res) throw RuntimeError("no results downloaded");
+  const ResultBlock &rb = *m.res;
+  out.probs = rb.p.as<double>() + c0;
+  out.hashes = rb.hash.as<uint64_t>() + c0;
+  out.targets = rb.targets.as<uint32_t>() + t0;
+  const uint32_t *off = rb.off.as<uint32_t>();
+  if (t0 == 0) {
+    // the first circuit's offsets are its own
+    out.offsetsRel.clear();
+    out.offsets = off + c0;
+  } else {
+    out.offsetsRel.resize(out.numErrors + 1);
+    for (uint32_t i = 0; i <= out.numErrors; ++i) out.offsetsRel[i] = off[c0 + i] - t0;
+    out.offsets = out.offsetsRel.data();
+  }
 }
+
+ResultHandle Pipeline::resultBlock() const { return impl_->res; }
 
 void Pipeline::runGraph() {
   Impl &m = *impl_;
@@ -1856,7 +1966,15 @@ void Pipeline::debugSensitivities(uint32_t v, std::vector<uint32_t> &cols32) con
     GPA_HIP(hipMemcpy(band.data(), m.band.as<int2>() + d.colOff, 8 * d.numCols, hipMemcpyDeviceToHost));
   // each rank's word = XOR of its in-band frame rows
   std::vector<uint32_t> code(4ull * d.numRanks);
-  std::memcpy(code.data(), m.inHost.as<char>() + m.oRankRows + 16 * d.rankOff, 16ull * d.numRanks);
+  const uint2 *cc = reinterpret_cast<const uint2 *>(m.inHost.as<char>() + m.oRankCode) + d.rankOff;
+  const uint4 *spill = reinterpret_cast<const uint4 *>(m.inHost.as<char>() + m.oSpill) + d.spillOff;
+  for (uint32_t i = 0; i < d.numRanks; ++i) {
+    const uint4 q = expandRankCode(cc[i], spill);
+    code[4ull * i] = q.x;
+    code[4ull * i + 1] = q.y;
+    code[4ull * i + 2] = q.z;
+    code[4ull * i + 3] = q.w;
+  }
   cols32.assign(static_cast<size_t>(d.numRanks) * d.numCols, 0u);
   for (uint32_t c = 0; c < d.numCols; ++c)
     for (uint32_t i = 0; i < d.numRanks; ++i) {

@@ -19,6 +19,7 @@
 #pragma once
 
 #include <cstdint>
+#include <memory>
 #include <vector>
 
 #include "stepg.hpp"
@@ -36,6 +37,12 @@ struct VariantResult {
   std::vector<uint32_t> offsetsRel;  // storage for offsets
 };
 
+// Pinned host memory holding one download's results (defined in
+// engine.hip).  A holder of a handle keeps the block's arrays valid; the
+// pipeline downloads into another block while one is held.
+struct ResultBlock;
+using ResultHandle = std::shared_ptr<const ResultBlock>;
+
 struct StageTimes {
   double upload = 0, sweep = 0, classify = 0, partition = 0, buckets = 0, emit = 0, download = 0,
          total = 0;
@@ -52,8 +59,12 @@ class Pipeline {
 
   // Fixes the capacity to what `imgs` need (non-growable pipelines).
   void reserve(const std::vector<const StepgImage *> &imgs);
-  // Uploads the images (host -> device) and prepares the launch plan.
+  // Packs the images into the pinned input arena, prepares the launch plan
+  // and uploads the arena (host -> device, stream-ordered).
   void load(const std::vector<const StepgImage *> &imgs);
+  // Uploads the packed arena of the last load again (what each compile of
+  // an unchanged circuit does).
+  void upload();
   // Enqueues the device pipeline; `timed` records per-stage events.
   void run(bool timed);
   // Enqueues the pipeline as one HIP graph (captured on first use after a
@@ -66,6 +77,8 @@ class Pipeline {
 
   uint32_t numVariants() const { return static_cast<uint32_t>(variants_.size()); }
   void result(uint32_t v, VariantResult &out) const;
+  // The block the last download wrote (the arrays result() points into).
+  ResultHandle resultBlock() const;
   const StageTimes &times() const { return times_; }
   uint64_t deviceBytes() const;
   int device() const { return device_; }

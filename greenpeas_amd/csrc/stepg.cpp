@@ -3,6 +3,8 @@
 
 #include <algorithm>
 #include <atomic>
+#include <cstring>
+#include <unordered_map>
 #include <cmath>
 #include <string>
 
@@ -268,6 +270,10 @@ void setSweepMinRpt(uint32_t rpt) {
 }
 uint32_t sweepMinRpt() { return gSweepMinRpt.load(); }
 
+std::atomic<uint32_t> gProbTable{1};
+void setProbTable(bool on) { gProbTable.store(on ? 1u : 0u); }
+bool probTable() { return gProbTable.load() != 0; }
+
 void buildImage(const Stepg &g, StepgImage &img) {
   const uint32_t N = g.numNodes, npl = g.npl, L = g.numLayers, C = g.numCols;
   img.numLayers = L;
@@ -452,6 +458,65 @@ void buildImage(const Stepg &g, StepgImage &img) {
       img.rankRows[4 * r + i] = (fl << kRowPosBits) | actIdx[frames[i]];
     }
   }
+
+  // compact codes (see kSpillBase), spilling what does not fit
+  img.rankCode.assign(2ull * img.numRanks, 0u);
+  img.rankSpill.clear();
+  for (uint32_t r = 0; r < img.numRanks; ++r) {
+    const uint32_t *rows = img.rankRows.data() + 4ull * r;
+    const uint32_t base = rows[0] == kNone ? 0u : rows[0] >> kRowPosBits;
+    uint32_t c16[3] = {kRowNone16, kRowNone16, kRowNone16};
+    bool fits = rows[3] == kNone && base < kSpillBase;
+    for (int i = 0; i < 3 && fits; ++i) {
+      if (rows[i] == kNone) break;
+      const uint32_t dl = (rows[i] >> kRowPosBits) - base;
+      const uint32_t v = (dl << kRowPosBits) | (rows[i] & ((1u << kRowPosBits) - 1));
+      if (dl >= (1u << (16 - kRowPosBits)) || v == kRowNone16) fits = false;
+      c16[i] = v;
+    }
+    if (fits) {
+      img.rankCode[2ull * r] = (base << 16) | c16[0];
+      img.rankCode[2ull * r + 1] = (c16[2] << 16) | c16[1];
+    } else {
+      const uint64_t idx = img.rankSpill.size() / 4;
+      img.rankSpill.insert(img.rankSpill.end(), rows, rows + 4);
+      img.rankCode[2ull * r] = (kSpillBase << 16) | static_cast<uint32_t>(idx >> 32);
+      img.rankCode[2ull * r + 1] = static_cast<uint32_t>(idx);
+    }
+  }
+
+  // probability table when the distinct values are few (exact: the table
+  // holds the doubles themselves)
+  img.probTab.clear();
+  img.probIdx.assign(img.numRanks, 0);
+  if (!probTable()) {
+    img.probIdx.clear();
+  } else {
+    std::unordered_map<uint64_t, uint32_t> seen;
+    uint64_t lastBits = 0;
+    uint32_t lastIdx = kNone;  // the previous rank's value (runs are common)
+    for (uint32_t r = 0; r < img.numRanks; ++r) {
+      uint64_t bits;
+      std::memcpy(&bits, &img.rankProb[r], 8);
+      if (lastIdx != kNone && bits == lastBits) {
+        img.probIdx[r] = static_cast<uint16_t>(lastIdx);
+        continue;
+      }
+      auto it = seen.find(bits);
+      if (it == seen.end()) {
+        if (seen.size() == kProbDictMax) {
+          img.probTab.clear();
+          img.probIdx.clear();
+          break;
+        }
+        it = seen.emplace(bits, static_cast<uint32_t>(img.probTab.size())).first;
+        img.probTab.push_back(img.rankProb[r]);
+      }
+      img.probIdx[r] = static_cast<uint16_t>(it->second);
+      lastBits = bits;
+      lastIdx = it->second;
+    }
+  }
 }
 
 uint64_t imageDeviceBytes(const StepgImage &img) {
@@ -463,7 +528,7 @@ uint64_t imageDeviceBytes(const StepgImage &img) {
   const uint64_t slots = B * 256 + R;
   uint64_t b = 0;
   b += static_cast<uint64_t>(kSweepLeadPad + img.numLayers) * img.layerStride * 16ull + img.seedVals.size() * 4ull;
-  b += img.numCols * 16ull + R * 28ull;         // column bands, rank layer + prob + frame rows
+  b += img.numCols * 16ull + R * 16ull + img.rankSpill.size() * 4ull + img.probTab.size() * 8ull;  // column bands, probabilities, frame rows
   b += (static_cast<uint64_t>(img.numLayers) * img.layerStride + kSinkSlots) * img.numCols * 4ull;  // frame sensitivities
   b += R * (16 + 64 + 16 + 24);                 // node records, target rows, overflow list, sort workspace
   b += slots * (8 + 4 + 4 + 8 + 8 + 4 + 4);     // bucket slots: keys, ranks, run arrays

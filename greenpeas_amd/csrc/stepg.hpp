@@ -58,6 +58,17 @@ constexpr uint32_t kSweepMaxThreads = 1024;
 constexpr uint32_t kSweepLeadPad = 11;
 // frame row codes in rankRows: layer << kRowPosBits | position in the layer
 constexpr uint32_t kRowPosBits = 14;
+// Compact frame rows (rankCode, two words per rank, what the device reads):
+// x = base << 16 | row 0, y = row 2 << 16 | row 1, each row (layer - base)
+// << kRowPosBits | position, kRowNone16 = none, base = the first row's
+// layer.  A rank with four rows, a base >= kSpillBase or a row that would
+// read as kRowNone16 is spilled instead: x = kSpillBase << 16 | (index >>
+// 32), y = the low 32 bits of index, and rankSpill[4 index .. 4 index + 3]
+// holds its rankRows codes.
+constexpr uint32_t kRowNone16 = 0xFFFFu, kSpillBase = 0xFFFFu;
+// probabilities are sent as 16-bit indices into a per-circuit table when a
+// circuit has at most this many distinct values (else as doubles)
+constexpr uint32_t kProbDictMax = 65536;
 
 inline uint32_t nodesPerQubit(int level) {
   switch (level) {
@@ -103,10 +114,14 @@ struct StepgImage {
                                    // blocks, then one block per layer (see the file comment)
   std::vector<uint32_t> seedVals;  // numCols x numSeeds (column-major)
   std::vector<int32_t> colHi, colLow;  // numCols: seed layer range (-1 = no seed)
-  std::vector<uint32_t> rankLayer; // numRanks
-  std::vector<double> rankProb;    // numRanks
+  std::vector<uint32_t> rankLayer; // numRanks (host only)
+  std::vector<double> rankProb;    // numRanks (sent when probTab is empty)
   std::vector<uint32_t> rankNode;  // numRanks (host only: node index of each rank)
-  std::vector<uint32_t> rankRows;  // 4 * numRanks: frame row codes, kNone-padded
+  std::vector<uint32_t> rankRows;  // 4 * numRanks: frame row codes, kNone-padded (host only)
+  std::vector<uint32_t> rankCode;  // 2 * numRanks: compact frame rows
+  std::vector<uint32_t> rankSpill; // 4 per spilled rank: its rankRows codes
+  std::vector<double> probTab;     // distinct probabilities (empty: too many)
+  std::vector<uint16_t> probIdx;   // numRanks indices into probTab
 };
 
 void buildImage(const Stepg &g, StepgImage &img);
@@ -116,6 +131,11 @@ void buildImage(const Stepg &g, StepgImage &img);
 // with more records each trade barrier participants for per-wave work.
 void setSweepMinRpt(uint32_t rpt);
 uint32_t sweepMinRpt();
+// Tuning knob: whether images may send probabilities as table indices (on
+// by default; off, every rank's double is sent).  Read when a circuit is
+// parsed.
+void setProbTable(bool on);
+bool probTable();
 
 // Device bytes the pipeline needs for an image (used by admission control).
 uint64_t imageDeviceBytes(const StepgImage &img);

@@ -21,6 +21,7 @@ from __future__ import annotations
 
 import ctypes
 import enum
+import sys
 import os
 
 import numpy as np
@@ -64,6 +65,42 @@ def _read_dem(lib, shape_fn, copy_fn, *handle) -> DetectorErrorModel:
     return DetectorErrorModel(probs, offsets, targets[:t], nd.value, no.value, hashes[:n])
 
 
+class _PinnedBlock:
+    """Holds a gp_dem_view token; lets go of it when the last array over
+    its memory is gone."""
+
+    __slots__ = ("_lib", "_token")
+
+    def __init__(self, lib, token):
+        self._lib = lib
+        self._token = token
+
+    def __del__(self):
+        if self._token and not sys.is_finalizing():
+            self._lib.gp_dem_release(self._token)
+            self._token = None
+
+
+def _pinned_array(addr: int, n: int, ctype, dtype, owner: _PinnedBlock) -> np.ndarray:
+    if n == 0 or not addr:
+        return np.empty(0, dtype)
+    buf = (ctype * n).from_address(addr)
+    buf._owner = owner  # the array's base holds the block
+    return np.frombuffer(buf, dtype)
+
+
+def _view_dem(lib, handle) -> DetectorErrorModel:
+    v = _native.DemView()
+    _native.check(lib.gp_dem_acquire(handle, ctypes.byref(v)))
+    owner = _PinnedBlock(lib, v.token)
+    n, t = v.num_errors, v.num_targets
+    probs = _pinned_array(v.probs, n, ctypes.c_double, np.float64, owner)
+    offsets = _pinned_array(v.offsets, n + 1, ctypes.c_uint32, np.uint32, owner)
+    targets = _pinned_array(v.targets, t, ctypes.c_uint32, np.uint32, owner)
+    hashes = _pinned_array(v.hashes, n, ctypes.c_uint64, np.uint64, owner)
+    return DetectorErrorModel(probs, offsets, targets, v.num_detectors, v.num_observables, hashes)
+
+
 class Driver:
     """A DEM compiler whose GPU buffers are sized once and reused
     (Driver.hpp:37-219)."""
@@ -101,6 +138,12 @@ class Driver:
         return self.last_dem()
 
     def last_dem(self) -> DetectorErrorModel:
+        """The last compile's DEM; its arrays are the pinned host memory the
+        driver downloaded into (no copy), kept alive by the arrays."""
+        return _view_dem(self._lib, self._h)
+
+    def last_dem_copy(self) -> DetectorErrorModel:
+        """The last compile's DEM copied into fresh numpy arrays."""
         return _read_dem(self._lib, self._lib.gp_dem_shape, self._lib.gp_dem_copy, self._h)
 
     def dem_text(self) -> str:

@@ -85,6 +85,25 @@ int gp_dem_shape(const gp_driver *driver, uint32_t *num_errors,
 int gp_dem_copy(const gp_driver *driver, double *probs, uint32_t *offsets,
                 uint32_t *targets, uint64_t *hashes);
 
+/* The last DEM's arrays in place, without a copy (the pinned host memory
+ * the driver downloaded them into; no counterpart in the reference, whose
+ * DEM is returned by value).  The arrays stay valid until
+ * gp_dem_release(view->token), also across later compiles and after the
+ * driver is destroyed; the driver downloads later DEMs elsewhere while a
+ * view is held.  Layout as gp_dem_copy. */
+typedef struct gp_dem_view {
+  const double *probs;       /* num_errors */
+  const uint32_t *offsets;   /* num_errors + 1 */
+  const uint32_t *targets;   /* num_targets */
+  const uint64_t *hashes;    /* num_errors */
+  uint32_t num_errors, num_detectors, num_observables;
+  uint64_t num_targets;
+  void *token;
+} gp_dem_view;
+int gp_dem_acquire(const gp_driver *driver, gp_dem_view *view);
+/* Lets go of a view's arrays (token from gp_dem_acquire; thread-safe). */
+int gp_dem_release(void *token);
+
 /* Writes the last DEM as stim DEM text ("error(p) D.. L..\n" per error).
  * If buf is NULL or too small, only *len (bytes, excluding the NUL) is set
  * and GP_OK is returned; call again with a buffer of *len + 1 bytes. */
@@ -177,7 +196,11 @@ int gp_estimate_device_bytes(const char *circuit_text, int correlation_level,
  *   read at each compile.
  *   "mask_table": 1 (default) lets the class stage keep per-workgroup column
  *   masks in LDS; 0 makes every node test every column's time band (what
- *   circuits whose masks do not fit in LDS always do); read at each compile. */
+ *   circuits whose masks do not fit in LDS always do); read at each compile.
+ *   "prob_table": 1 (default) sends a circuit's probabilities as 16-bit
+ *   indices into its table of distinct values when it has at most 65536 of
+ *   them; 0 always sends one double per error node; read when a circuit is
+ *   parsed. */
 int gp_set_tuning(const char *name, uint32_t value);
 int gp_get_tuning(const char *name, uint32_t *value);
 

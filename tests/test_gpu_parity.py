@@ -239,3 +239,41 @@ def test_class_stage_without_mask_tables(oracle, stash):
     finally:
         autotune.set_tuning("mask_table", 1)
         autotune.set_tuning("stash", 1)
+
+
+def test_probabilities_sent_as_doubles(oracle):
+    """With the probability table switched off every error node's double is
+    sent (the path of circuits with more than 65536 distinct rates): the same
+    DEMs as the STEPG restatement, bit for bit in order and targets."""
+    from greenpeas_amd import autotune
+
+    try:
+        autotune.set_tuning("prob_table", 0)
+        for name, level in (("surface3", 2), ("bb6", 2)):
+            text = read_circuit(name)
+            dem = get_driver(text, correlation_level=level).compile_detector_error_model()
+            ref = oracle.compile_stepg(text, level)
+            assert np.array_equal(dem.hashes, ref.hashes)
+            assert np.array_equal(dem.offsets, ref.offsets)
+            assert np.array_equal(dem.targets, ref.targets)
+            assert np.max(np.abs(dem.probs - ref.probs) / ref.probs) <= REL
+    finally:
+        autotune.set_tuning("prob_table", 1)
+
+
+def test_dem_views_outlive_later_compiles_and_the_driver():
+    """A DEM's arrays are views of the driver's pinned result memory: they
+    keep their values through later compiles of other circuits and after the
+    driver is closed (gp_dem_acquire / gp_dem_release)."""
+    small, big = read_circuit("surface3"), read_circuit("surface5")
+    d = get_driver(big)
+    dem_small = d.compile_detector_error_model(small)
+    copy_small = d.last_dem_copy()
+    dem_big = d.compile_detector_error_model(big)
+    dem_small2 = d.compile_detector_error_model(small)
+    d.close()
+    assert dem_small == copy_small and dem_small2 == copy_small
+    assert np.array_equal(dem_small.offsets, copy_small.offsets)
+    assert np.array_equal(dem_small.hashes, copy_small.hashes)
+    assert dem_big == get_driver(big).compile_detector_error_model()
+    del dem_small, dem_small2, dem_big  # releases the blocks after the driver

@@ -110,6 +110,10 @@ def test_tuning_knobs_roundtrip():
         assert autotune.get_tuning("mask_table") == 0
         autotune.set_tuning("mask_table", 1)
         assert autotune.get_tuning("mask_table") == 1
+        autotune.set_tuning("prob_table", 0)
+        assert autotune.get_tuning("prob_table") == 0
+        autotune.set_tuning("prob_table", 1)
+        assert autotune.get_tuning("prob_table") == 1
         with pytest.raises(ValueError):
             autotune.set_tuning("sweep_rpt", 3)
         with pytest.raises(ValueError):
