@@ -46,8 +46,9 @@ struct VDesc {
   uint32_t numObservables, numRanks, numSeeds, bucketBits;
   uint32_t bucketBase, numBuckets, maskWords, layerStride;
   uint32_t sweepThreads, sweepRpt, rowSpan, probDict;  // rowSpan: see StepgImage
+  uint32_t numPairs, pad0, pad1, pad2;
   uint64_t actOff, seedOff, colOff, rankOff, sOff;
-  uint64_t colStride;  // words per S column: numLayers * layerStride frame rows + kSinkSlots sinks
+  uint64_t colStride;  // words per S column: 2 (numPairs + kSweepLeadPad) rows of layerStride
   uint64_t spillOff;   // first spill entry of the circuit's ranks
   uint64_t probOff;    // first probability (table entry if probDict, else the ranks' own)
 };
@@ -298,115 +299,114 @@ __device__ __forceinline__ lds_u32 *ldsAt(uint32_t byteAddr) {
   return reinterpret_cast<lds_u32 *>(static_cast<uintptr_t>(byteAddr));
 }
 
-// Loads this thread's records of one layer: a buffer load with the layer's
-// byte offset in the scalar offset, so the address costs no vector work.
+// Loads this thread's records of one pair block and the block's header
+// word: buffer loads with the block's byte offset in the scalar offset, so
+// the address costs no vector work.
 template <int RPT>
-__device__ __forceinline__ void loadLayer(uint4 (&r)[RPT], __amdgpu_buffer_rsrc_t rs, uint32_t layerOff,
-                                          const uint32_t (&off)[RPT]) {
-  layerOff = __builtin_amdgcn_readfirstlane(layerOff);
+__device__ __forceinline__ void loadBlock(uint4 (&r)[RPT], uint32_t &head, __amdgpu_buffer_rsrc_t rs,
+                                          uint32_t blockOff, const uint32_t (&off)[RPT], uint32_t headOff) {
+  blockOff = __builtin_amdgcn_readfirstlane(blockOff);
 #pragma unroll
   for (int k = 0; k < RPT; ++k) {
-    const auto v = __builtin_amdgcn_raw_buffer_load_b128(rs, off[k], layerOff, 0);
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(rs, off[k], blockOff, 0);
     r[k] = make_uint4(v[0], v[1], v[2], v[3]);
   }
+  head = __builtin_amdgcn_raw_buffer_load_b32(rs, headOff, blockOff, 0);
 }
 
-// Backward sensitivity sweep of one 32-bit column of one circuit.  The
-// sensitivities of layers l, l+1, l+2 live in an LDS ring (slot l mod 3);
-// each layer is one pass over the layer's fixed-stride block of records
-// (sweepRpt per thread) followed by one LDS-only barrier.  A record carries
-// the LDS byte addresses of its children and seed and the byte offset of
-// its S entry, so a record costs two (three if seeded) LDS reads, two XORs,
-// one LDS write and one global store.  The loop runs trips of D layers (D a
-// multiple of 3; see sweepTrip), unrolled, so each position in a trip has a
+// Backward sensitivity sweep of one 32-bit column of one circuit, two
+// layers (2m + 1, 2m) per step and one LDS-only barrier per step.  Layer
+// 2m + 2 is held in an LDS ring of two slots (slot (m + 1) mod 2); a record
+// (one position of a pair, see StepgImage) gives the odd node's value as
+// the XOR of two ring words and the even node's as the XOR of four, each
+// plus a seed word; the even value goes to the ring, both to S.  Rows of S
+// are addressed by layer (rows of the padding layers below 0 come after the
+// circuit's rows), so a record needs no S address.  The loop runs trips of D
+// steps (D even, see sweepTrip), unrolled, so each position in a trip has a
 // fixed ring slot: the ring addresses a thread writes are per-column
-// constants, and the record prefetch (D layers ahead, into D rotating
-// register sets) advances one scalar offset.  Per-layer scalar work bounds a layer here
-// (one scalar instruction per SIMD per four cycles, shared by its waves), so
-// the early-termination vote runs once per trip: a trip whose sensitivities
-// are all zero, at or below the column's lowest seed, ends the band (the ring
-// then holds zeros and no seed lies below).  Each trip ORs its values into
-// one of two LDS words (by trip parity) before its last barrier; the next
-// trip reads the word in its first layer, behind that layer's own LDS reads,
-// and clears it after its first barrier.  Waves beyond the circuit's
-// sweepThreads (a batch launches with its widest circuit's shape) only take
-// part in the barriers and votes.  On exit the column's band [lo, hi] is
-// recorded ({lo, hi}; {INT_MAX, -1} for a column without seeds).  Block 0
-// also zeroes the run's flag words (read and written only by the later
-// stages); the grid has at least one block.
-// Layers per trip = prefetch distance (a multiple of the ring's three
-// slots, so each trip position keeps its slot): six while the register sets
-// fit (up to two records per thread), else three.
-#ifndef GPA_SWEEP_TRIP
-#define GPA_SWEEP_TRIP 6
-#endif
-__host__ __device__ constexpr int sweepTrip(int rpt) { return rpt <= 2 ? GPA_SWEEP_TRIP : 3; }
-static_assert(GPA_SWEEP_TRIP % 3 == 0 && 2 * GPA_SWEEP_TRIP - 1 <= static_cast<int>(kSweepLeadPad),
-              "trip length: a multiple of 3 whose prefetches stay within the padding blocks");
+// constants, and the record prefetch (D steps ahead, into D rotating
+// register sets) advances one scalar offset.  The early-termination vote
+// runs once per trip: a trip whose sensitivities are all zero, at or below
+// the column's lowest seed, ends the band (the ring then holds zeros and no
+// seed lies below).  Each trip ORs its values into one of two LDS words (by
+// trip parity) before its last barrier; the next trip reads the word in its
+// first step, behind that step's own LDS reads, and clears it after its
+// first barrier.  Waves beyond the circuit's sweepThreads (a batch launches
+// with its widest circuit's shape) only take part in the barriers and
+// votes.  On exit the column's band [lo, hi] is recorded ({lo, hi};
+// {INT_MAX, -1} for a column without seeds).  Block 0 also zeroes the run's
+// flag words (read and written only by the later stages); the grid has at
+// least one block.
+// Steps per trip = prefetch distance: four while the register sets fit (up
+// to two records per thread), else two.
+__host__ __device__ constexpr int sweepTrip(int rpt) { return rpt <= 2 ? 4 : 2; }
+static_assert(2 * sweepTrip(1) - 1 <= static_cast<int>(kSweepLeadPad),
+              "trip length: prefetches must stay within the padding blocks");
 
 template <int RPT>
 struct SweepState {
   __amdgpu_buffer_rsrc_t ac;  // the circuit's record blocks
-  const uint32_t *gseeds;  // the column's seeds in global memory (GSEEDS)
-  uint32_t layerBytes;     // 16 * layerStride
-  uint32_t zeroB, seedB;   // byte addresses of the zero word and the first seed
-  uint32_t off[RPT];       // byte offsets of this thread's records in a block
+  __amdgpu_buffer_rsrc_t sc;  // the column of S
+  const uint32_t *gseeds;     // the column's seeds in global memory (GSEEDS)
+  uint32_t blockBytes;        // 16 * (layerStride + 1)
+  uint32_t headOff;           // 16 * layerStride: the header within a block
+  uint32_t zeroB, seedB;      // LDS byte addresses of the zero word and the first seed
+  uint32_t rowBytes;          // 4 * layerStride
+  uint32_t rows;              // rows per column (layers, padding layers included)
+  uint32_t off[RPT];          // byte offsets of this thread's records in a block
+  uint32_t pos4[RPT];         // byte offsets of this thread's positions in a row or slot
 };
 
-__device__ __forceinline__ uint32_t sweepSlot(int32_t l) { return static_cast<uint32_t>(l + 6) % 3; }  // l >= -6
+// LDS byte address of the ring word named by a 16-bit index.
+__device__ __forceinline__ uint32_t ringLo(uint32_t x) { return (x & 0xFFFFu) << 2; }
+__device__ __forceinline__ uint32_t ringHi(uint32_t x) { return (x >> 14) & 0x3FFFCu; }
 
-// Ring byte addresses of a thread's records at one trip position: kept in
-// registers for up to two records per thread, else formed per layer from
-// the position's slot base (fewer registers).
-template <int RPT, int D>
-struct SweepDst {
-  static constexpr bool kStored = RPT <= 2;
-  uint32_t addr[kStored ? D : 1][RPT];
-  uint32_t slotB[D];
-  uint32_t pos4[RPT];
-  __device__ __forceinline__ uint32_t at(int u, int k) const {
-    if constexpr (kStored) return addr[u][k];
-    else return slotB[u] + pos4[k];
-  }
-};
-
-// One layer (trip position u): consume `r` (this thread's records of the
-// layer), write the values to the ring and to S, OR them into `any`, then
-// refill `r` with the block one trip below (`loadOff`, advanced here).
-template <int RPT, bool GSEEDS, int D>
-__device__ __forceinline__ void sweepRecords(const SweepState<RPT> &st, uint4 (&r)[RPT],
-                                             const SweepDst<RPT, D> &dst, int u, uint32_t &loadOff,
-                                             uint32_t &any, char *__restrict__ Sc) {
-  // all of the wave's child reads first, so the records' round trips
-  // overlap; seeds are read only by waves holding a seeded record (seeded
-  // nodes are rare: measurement nodes)
-  uint32_t v[RPT];
-#pragma unroll
-  for (int k = 0; k < RPT; ++k) v[k] = *ldsAt(r[k].x) ^ *ldsAt(r[k].y);
-  if constexpr (GSEEDS) {
-#pragma unroll
-    for (int k = 0; k < RPT; ++k) {
-      const bool hs = r[k].z != st.zeroB;
-      const uint32_t g = st.gseeds[hs ? (r[k].z - st.seedB) >> 2 : 0u];
-      v[k] ^= hs ? g : 0u;
-    }
-  } else {
-    bool seeded = false;
-#pragma unroll
-    for (int k = 0; k < RPT; ++k) seeded |= r[k].z != st.zeroB;
-    if (__ballot(seeded) != 0ull) {
-#pragma unroll
-      for (int k = 0; k < RPT; ++k) v[k] ^= *ldsAt(r[k].z);
-    }
-  }
+// One step (pair m, trip position u): consume `r` (this thread's records),
+// write the even values to ring slot `slotB` and both values to S, OR them
+// into `any`, then refill `r` and `head` with the block one trip below
+// (`loadOff`, advanced here).
+template <int RPT, bool GSEEDS>
+__device__ __forceinline__ void sweepStep(const SweepState<RPT> &st, uint4 (&r)[RPT], uint32_t &head,
+                                          uint32_t slotB, int32_t m, uint32_t &loadOff, uint32_t &any) {
+  // all of the wave's ring reads first, so the records' round trips overlap;
+  // seeds are read only by waves holding a seeded record (seeded nodes are
+  // rare: measurement nodes and their parents)
+  uint32_t a[RPT], b[RPT];
 #pragma unroll
   for (int k = 0; k < RPT; ++k) {
-    *ldsAt(dst.at(u, k)) = v[k];
-    *reinterpret_cast<uint32_t *>(Sc + r[k].w) = v[k];
-    any |= v[k];
+    a[k] = *ldsAt(ringLo(r[k].x)) ^ *ldsAt(ringHi(r[k].x));
+    b[k] = (*ldsAt(ringLo(r[k].y)) ^ *ldsAt(ringHi(r[k].y))) ^ (*ldsAt(ringLo(r[k].z)) ^ *ldsAt(ringHi(r[k].z)));
   }
-  loadOff -= st.layerBytes;
-  loadLayer<RPT>(r, st.ac, loadOff, st.off);
+  bool seeded = false;
+#pragma unroll
+  for (int k = 0; k < RPT; ++k) seeded |= r[k].w != 0xFFFFFFFFu;
+  if (__ballot(seeded) != 0ull) {
+    const uint32_t first = __builtin_amdgcn_readfirstlane(head);
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+      const uint32_t so = r[k].w & 0xFFFFu, se = r[k].w >> 16;
+      if constexpr (GSEEDS) {
+        a[k] ^= so != 0xFFFFu ? st.gseeds[first + so] : 0u;
+        b[k] ^= se != 0xFFFFu ? st.gseeds[first + se] : 0u;
+      } else {
+        a[k] ^= *ldsAt(so != 0xFFFFu ? st.seedB + 4 * (first + so) : st.zeroB);
+        b[k] ^= *ldsAt(se != 0xFFFFu ? st.seedB + 4 * (first + se) : st.zeroB);
+      }
+    }
+  }
+  // rows of layers 2m (even) and 2m + 1 (odd); padding layers below 0 wrap
+  // to the rows after the circuit's
+  const int32_t evenRow = 2 * m < 0 ? 2 * m + static_cast<int32_t>(st.rows) : 2 * m;
+  const uint32_t evenB = static_cast<uint32_t>(evenRow) * st.rowBytes, oddB = evenB + st.rowBytes;
+#pragma unroll
+  for (int k = 0; k < RPT; ++k) {
+    *ldsAt(slotB + st.pos4[k]) = b[k];
+    __builtin_amdgcn_raw_buffer_store_b32(a[k], st.sc, st.pos4[k], oddB, 0);
+    __builtin_amdgcn_raw_buffer_store_b32(b[k], st.sc, st.pos4[k], evenB, 0);
+    any |= a[k] | b[k];
+  }
+  loadOff -= st.blockBytes;
+  loadBlock<RPT>(r, head, st.ac, loadOff, st.off, st.headOff);
 }
 
 template <int RPT, bool GSEEDS>
@@ -416,7 +416,7 @@ __global__ __launch_bounds__(kSweepMaxThreads) void sweepKernel(
     const int32_t *__restrict__ colLow, int2 *__restrict__ band, uint32_t *__restrict__ S,
     uint32_t numItems, uint32_t *__restrict__ flags) {
   // dynamic LDS only, addressed from 0:
-  // ring[3P] | zero word | seeds (unless GSEEDS) | vote[2]
+  // ring[2P] | zero word | seeds (unless GSEEDS) | vote[2]
   const uint32_t tid = threadIdx.x;
   if (blockIdx.x == 0 && tid < kFlagWords) flags[tid] = 0u;
   if (blockIdx.x >= numItems) return;
@@ -431,40 +431,45 @@ __global__ __launch_bounds__(kSweepMaxThreads) void sweepKernel(
   const int32_t lowSeed = __builtin_amdgcn_readfirstlane(colLow[d.colOff + c]);
   const uint32_t P = d.layerStride, T = d.sweepThreads;
   const uint32_t *sv = seedVals + d.seedOff + static_cast<uint64_t>(c) * d.numSeeds;
-  for (uint32_t i = tid; i <= 3 * P; i += blockDim.x) *ldsAt(4 * i) = 0u;  // ring + zero word
-  uint32_t voteB = 4 * (3 * P + 1);
+  for (uint32_t i = tid; i <= 2 * P; i += blockDim.x) *ldsAt(4 * i) = 0u;  // ring + zero word
+  uint32_t voteB = 4 * (2 * P + 1);
   if constexpr (!GSEEDS) {
-    for (uint32_t i = tid; i < d.numSeeds; i += blockDim.x) *ldsAt(4 * (3 * P + 1 + i)) = sv[i];
+    for (uint32_t i = tid; i < d.numSeeds; i += blockDim.x) *ldsAt(4 * (2 * P + 1 + i)) = sv[i];
     voteB += 4 * d.numSeeds;
   }
   if (tid < 2) *ldsAt(voteB + 4 * tid) = 0u;
   __syncthreads();
 
-  // Both loops below run D-layer trips with the vote steps at the same
+  // Both loops below run D-step trips with the vote steps at the same
   // points and decide on leaving at the end of a trip, from the same vote
   // words, so all waves meet the same number of barriers.  A single exit
   // per trip keeps the prefetches in flight (an exit inside would make the
-  // compiler drain them at the loop head).  Layers below 0 process padding;
+  // compiler drain them at the loop head).  Pairs below 0 process padding;
   // the trip after a zero trip recomputes zeros below the band, which
   // nothing reads.
-  int32_t l = hi;     // top layer of the current trip
-  int32_t lo = -1;    // lowest layer of the band, once known
-  uint32_t trip = 0;  // vote word parity
+  constexpr int D = sweepTrip(RPT);
+  int32_t mt = hi / 2;  // top pair of the current trip
+  int32_t lo = -1;      // lowest layer of the band, once known
+  uint32_t trip = 0;    // vote word parity
   // whole waves: sweepThreads is a multiple of 64
   if (__builtin_amdgcn_readfirstlane(tid >> 6) < (T >> 6)) {
     SweepState<RPT> st;
     st.ac = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint4 *>(act + d.actOff), 0,
-                                              16 * P * (kSweepLeadPad + d.numLayers), 0x00020000);
+                                              16 * (P + 1) * (kSweepLeadPad + d.numPairs), 0x00020000);
+    const uint32_t rows = static_cast<uint32_t>(d.colStride / P);
+    st.sc = __builtin_amdgcn_make_buffer_rsrc(S + d.sOff + static_cast<uint64_t>(c) * d.colStride, 0,
+                                              static_cast<uint32_t>(4 * d.colStride), 0x00020000);
     st.gseeds = sv;
-    st.layerBytes = 16 * P;
-    st.zeroB = 12 * P;
-    st.seedB = 12 * P + 4;
-    char *Sc = reinterpret_cast<char *>(S + d.sOff + static_cast<uint64_t>(c) * d.colStride);
-    // ring addresses of this thread's records at each trip position
-    constexpr int D = sweepTrip(RPT);
-    SweepDst<RPT, D> dst;
+    st.blockBytes = 16 * (P + 1);
+    st.headOff = 16 * P;
+    st.zeroB = 8 * P;
+    st.seedB = 8 * P + 4;
+    st.rowBytes = 4 * P;
+    st.rows = rows;
+    // ring slot of each trip position (pair mt - u; D is even)
+    uint32_t slotB[D];
 #pragma unroll
-    for (int u = 0; u < D; ++u) dst.slotB[u] = sweepSlot(hi - u) * P * 4;
+    for (int u = 0; u < D; ++u) slotB[u] = static_cast<uint32_t>((mt - u) & 1) * P * 4;
 #pragma unroll
     for (int k = 0; k < RPT; ++k) {
       // circuits with fewer records per thread repeat their last record
@@ -472,65 +477,62 @@ __global__ __launch_bounds__(kSweepMaxThreads) void sweepKernel(
       const uint32_t j = static_cast<uint32_t>(k) < d.sweepRpt ? static_cast<uint32_t>(k) : d.sweepRpt - 1;
       const uint32_t pos = j * T + tid;
       st.off[k] = pos * 16;
-      dst.pos4[k] = pos * 4;
-      if constexpr (SweepDst<RPT, D>::kStored) {
-#pragma unroll
-        for (int u = 0; u < D; ++u) dst.addr[u][k] = dst.slotB[u] + pos * 4;
-      }
+      st.pos4[k] = pos * 4;
     }
-    // register sets for layers hi, hi-1, ..., hi-D+1 (block kSweepLeadPad + layer).
-    // Each set but the last is followed by RPT stores to sink words, as each
-    // refill is followed by a layer's RPT stores in the loop: the compiler's
+    // register sets for pairs mt, mt-1, ..., mt-D+1 (block kSweepLeadPad +
+    // pair).  Each set but the last is followed by the stores of one step
+    // (to the rows of the lowest padding layers, which nothing reads), as
+    // each refill is followed by a step's stores in the loop: the compiler's
     // count of memory operations outstanding at the loop head is then the
     // same on entry and on the back edge, and its waits keep the full
-    // prefetch distance.  One store instruction per record: the sink words of
-    // one store are contiguous across lanes, those of different stores 32
-    // words apart (mod kSinkSlots; at most 16 stores, so a lane's addresses
-    // differ).
-    uint32_t *sink = reinterpret_cast<uint32_t *>(Sc) + d.numLayers * P;
+    // prefetch distance.
     uint4 rs[D][RPT];
-    uint32_t loadOff = static_cast<uint32_t>(hi + static_cast<int32_t>(kSweepLeadPad)) * st.layerBytes;
+    uint32_t head[D];
+    uint32_t loadOff = static_cast<uint32_t>(mt + static_cast<int32_t>(kSweepLeadPad)) * st.blockBytes;
+    const uint32_t padB = (rows - 2 * kSweepLeadPad) * st.rowBytes;
 #pragma unroll
     for (int u = 0; u < D; ++u) {
-      loadLayer<RPT>(rs[u], st.ac, loadOff, st.off);
+      loadBlock<RPT>(rs[u], head[u], st.ac, loadOff, st.off, st.headOff);
       __builtin_amdgcn_sched_barrier(0);
       if (u + 1 < D) {
-        loadOff -= st.layerBytes;
+        loadOff -= st.blockBytes;
 #pragma unroll
-        for (int k = 0; k < RPT; ++k) sink[(tid + 32 * (u * RPT + k)) & (kSinkSlots - 1)] = 0u;
+        for (int k = 0; k < RPT; ++k) {
+          __builtin_amdgcn_raw_buffer_store_b32(0u, st.sc, st.pos4[k], padB, 0);
+          __builtin_amdgcn_raw_buffer_store_b32(0u, st.sc, st.pos4[k], padB + st.rowBytes, 0);
+        }
         __builtin_amdgcn_sched_barrier(0);
       }
     }
     for (;;) {
       uint32_t any = 0;
       const uint32_t myWord = voteB + 4 * (trip & 1), prevWord = voteB + 4 * ((trip + 1) & 1);
-      // first layer: also read the previous trip's vote
-      sweepRecords<RPT, GSEEDS, D>(st, rs[0], dst, 0, loadOff, any, Sc);
+      // first step: also read the previous trip's vote
+      sweepStep<RPT, GSEEDS>(st, rs[0], head[0], slotB[0], mt, loadOff, any);
       const uint32_t prev = trip ? *ldsAt(prevWord) : 1u;
-      if (prev == 0u && l + 1 <= lowSeed && lo < 0) lo = l + 1;
+      if (prev == 0u && 2 * mt + 2 <= lowSeed && lo < 0) lo = 2 * mt + 2;
       ldsBarrier();
       if (tid == 0) *ldsAt(prevWord) = 0u;
 #pragma unroll
       for (int u = 1; u < D; ++u) {
-        sweepRecords<RPT, GSEEDS, D>(st, rs[u], dst, u, loadOff, any, Sc);
+        sweepStep<RPT, GSEEDS>(st, rs[u], head[u], slotB[u], mt - u, loadOff, any);
         if (u == D - 1 && __ballot(any != 0) != 0ull && (tid & 63) == 0) *ldsAt(myWord) = 1u;
         ldsBarrier();
       }
-      l -= D;
+      mt -= D;
       ++trip;
-      if (lo >= 0 || l < 0) break;
+      if (lo >= 0 || mt < 0) break;
     }
   } else {
     for (;;) {
       const uint32_t prevWord = voteB + 4 * ((trip + 1) & 1);
       const uint32_t prev = trip ? *ldsAt(prevWord) : 1u;
-      if (prev == 0u && l + 1 <= lowSeed && lo < 0) lo = l + 1;
-      constexpr int D = sweepTrip(RPT);
+      if (prev == 0u && 2 * mt + 2 <= lowSeed && lo < 0) lo = 2 * mt + 2;
 #pragma unroll
       for (int u = 0; u < D; ++u) ldsBarrier();
-      l -= D;
+      mt -= D;
       ++trip;
-      if (lo >= 0 || l < 0) break;
+      if (lo >= 0 || mt < 0) break;
     }
   }
   if (lo < 0) lo = 0;
@@ -1493,7 +1495,8 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
     d.colOff = colOff;
     d.rankOff = rankOff;
     d.sOff = sOff;
-    d.colStride = static_cast<uint64_t>(g.numLayers) * g.layerStride + kSinkSlots;
+    d.numPairs = g.numPairs;
+    d.colStride = 2ull * (g.numPairs + kSweepLeadPad) * g.layerStride;
     d.spillOff = spillOff;
     d.probOff = probOff;
     d.probDict = g.probTab.empty() ? 0u : 1u;
@@ -1512,7 +1515,7 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
       tiles.push_back(make_uint4(v, t, g.rankLayer[t], nl));
       if (maskTableWords(nl, g.rowSpan, (g.numCols + 31) / 32) > kMaskTableWords) classTable = false;
     }
-    actOff += static_cast<uint64_t>(kSweepLeadPad + g.numLayers) * g.layerStride;
+    actOff += static_cast<uint64_t>(kSweepLeadPad + g.numPairs) * (g.layerStride + 1);
     seedOff += static_cast<uint64_t>(g.numCols) * g.numSeeds;
     colOff += g.numCols;
     rankOff += g.numRanks;
@@ -1522,9 +1525,9 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
     bucketBase += d.numBuckets;
     maxRpt = std::max(maxRpt, g.sweepRpt);
     maxThreads = std::max(maxThreads, g.sweepThreads);
-    ldsRing = std::max<size_t>(ldsRing, 4ull * (3ull * g.layerStride + 1 + 2));
-    ldsAll = std::max<size_t>(ldsAll, 4ull * (3ull * g.layerStride + 1 + g.numSeeds + 2));
-    if (16ull * g.layerStride * (kSweepLeadPad + g.numLayers) >= (1ull << 32))
+    ldsRing = std::max<size_t>(ldsRing, 4ull * (2ull * g.layerStride + 1 + 2));
+    ldsAll = std::max<size_t>(ldsAll, 4ull * (2ull * g.layerStride + 1 + g.numSeeds + 2));
+    if (16ull * (g.layerStride + 1) * (kSweepLeadPad + g.numPairs) >= (1ull << 32))
       throw RuntimeError("circuit too large for the sweep: sweep records exceed 4 GB");
   }
   if (rankOff >= 0x7FFFFFFFull) throw RuntimeError("too many error nodes in one load");

@@ -386,39 +386,132 @@ void buildImage(const Stepg &g, StepgImage &img) {
   img.sweepThreads = threads;
   img.sweepRpt = rpt;
   img.layerStride = P;
-  // frame rows of a column of S, addressed in bytes with 32 bits; row codes
-  // hold the position in kRowPosBits bits
-  if ((static_cast<uint64_t>(L) * P + kSinkSlots) * 4 >= (1ull << 32) || P > (1u << kRowPosBits) ||
-      L >= (1u << (32 - kRowPosBits)))
+  // The sweep takes layers in pairs (2m + 1, 2m).  A column of S holds a
+  // row of P words per layer for layers 0 .. 2 numPairs - 1, then the rows
+  // of the padding layers below 0 (what the lead blocks write); byte
+  // offsets within a column and row codes (position in kRowPosBits bits)
+  // must fit 32 bits.
+  const uint32_t numPairs = (L + 1) / 2;
+  img.numPairs = numPairs;
+  const uint64_t rowsPerCol = 2ull * (numPairs + kSweepLeadPad);
+  if (rowsPerCol * P * 4 >= (1ull << 32) || P > (1u << kRowPosBits) || L >= (1u << (32 - kRowPosBits)))
     throw RuntimeError("circuit too large for the sweep: too many layers x frame nodes");
+  for (uint32_t n = 0; n < N; ++n) {
+    if (!active[n] || !isFrame(n)) continue;
+    const uint64_t a = g.adjacency[n];
+    for (uint32_t ch : {static_cast<uint32_t>(a), static_cast<uint32_t>(a >> 32)})
+      if (ch != kNone && ch / npl != n / npl + 1) throw RuntimeError("STEPG: a frame flow skips a layer");
+  }
 
-  // records, layer by layer, with final LDS / S addresses
-  const uint32_t zeroB = 3u * P * 4u;        // the zero word after the ring
-  const uint32_t seedB = (3u * P + 1u) * 4u;  // the column's first seed word
-  const uint32_t sinkRow = L * P;             // sink words after the frame rows
-  img.act.resize(static_cast<size_t>(kSweepLeadPad + L) * P * 4);
-  auto child = [&](uint32_t ch) -> uint32_t {
-    if (ch == kNone) return zeroB;
-    return ((ch / npl) % 3u * P + actIdx[ch]) * 4u;
-  };
-  for (uint32_t blk = 0; blk < kSweepLeadPad + L; ++blk) {
-    const bool pad = blk < kSweepLeadPad;  // block kSweepLeadPad + l holds layer l
-    const uint32_t l = blk - kSweepLeadPad;
-    uint32_t *rec = img.act.data() + static_cast<size_t>(blk) * P * 4;
-    uint32_t pos = 0;
-    for (uint32_t j = 0, n = l * npl; !pad && j < npl; ++j, ++n) {
+  // Seeds, grouped by pair: pair m's list holds the seeds of layers 2m and
+  // 2m + 1, then one combined seed per even-layer node whose own seed and
+  // its children's seeds are two or more (their XOR, column by column);
+  // records name a seed by its index within the pair's list.
+  std::vector<uint32_t> pairFirst(numPairs + 1, 0);
+  std::vector<uint32_t> local(N, 0xFFFFu);      // a seeded node's index in its pair's list
+  std::vector<uint32_t> newRows;                // seeds' column words, row-major
+  std::vector<uint32_t> evenSeed(N, 0xFFFFu);   // even-layer node -> its (combined) seed
+  std::vector<uint32_t> bySeedNode(N, kNone);   // node -> original seed
+  for (uint32_t s2 = 0; s2 < S; ++s2) bySeedNode[seedNode[s2]] = s2;
+  for (uint32_t m = 0; m < numPairs; ++m) {
+    const uint32_t first = static_cast<uint32_t>(newRows.size() / std::max<uint32_t>(C, 1));
+    pairFirst[m] = first;
+    uint32_t cnt = 0;
+    for (uint32_t l = 2 * m; l <= 2 * m + 1 && l < L; ++l)
+      for (uint32_t n = l * npl; n < (l + 1) * npl; ++n) {
+        if (actIdx[n] == kNone || bySeedNode[n] == kNone) continue;
+        local[n] = cnt++;
+        const uint32_t *row = seedRows.data() + static_cast<size_t>(bySeedNode[n]) * C;
+        newRows.insert(newRows.end(), row, row + C);
+      }
+    for (uint32_t n = 2 * m * npl; n < (2 * m + 1) * npl; ++n) {
       if (actIdx[n] == kNone) continue;
+      uint32_t parts[3], np = 0;
+      auto toggle = [&](uint32_t x) {
+        for (uint32_t i = 0; i < np; ++i)
+          if (parts[i] == x) {
+            parts[i] = parts[--np];
+            return;
+          }
+        parts[np++] = x;
+      };
+      if (local[n] != 0xFFFFu) toggle(n);
       const uint64_t a = g.adjacency[n];
-      rec[0] = child(static_cast<uint32_t>(a));
-      rec[1] = child(static_cast<uint32_t>(a >> 32));
-      rec[2] = seedOf[n] == kNone ? zeroB : seedB + 4u * seedOf[n];
-      rec[3] = 4u * (l * P + pos);
-      rec += 4;
-      ++pos;
+      for (uint32_t ch : {static_cast<uint32_t>(a), static_cast<uint32_t>(a >> 32)})
+        if (ch != kNone && local[ch] != 0xFFFFu) toggle(ch);
+      if (np == 1) {
+        evenSeed[n] = local[parts[0]];
+      } else if (np > 1) {
+        evenSeed[n] = cnt++;
+        newRows.resize(newRows.size() + C, 0u);
+        uint32_t *row = newRows.data() + newRows.size() - C;
+        for (uint32_t i = 0; i < np; ++i) {
+          const uint32_t *src = seedRows.data() + static_cast<size_t>(bySeedNode[parts[i]]) * C;
+          for (uint32_t c = 0; c < C; ++c) row[c] ^= src[c];
+        }
+      }
     }
-    for (; pos < P; ++pos, rec += 4) {
-      rec[0] = rec[1] = rec[2] = zeroB;
-      rec[3] = 4u * (pad ? sinkRow + (pos & (kSinkSlots - 1)) : l * P + pos);
+    if (cnt >= 0xFFFFu) throw RuntimeError("circuit too large for the sweep: too many seeds in two layers");
+  }
+  const uint32_t S2 = C ? static_cast<uint32_t>(newRows.size() / C) : 0u;
+  pairFirst[numPairs] = S2;
+  img.numSeeds = S2;
+  img.seedVals.assign(static_cast<size_t>(C) * S2, 0u);
+  for (uint32_t s2 = 0; s2 < S2; ++s2)
+    for (uint32_t c = 0; c < C; ++c) img.seedVals[static_cast<size_t>(c) * S2 + s2] = newRows[static_cast<size_t>(s2) * C + c];
+
+  // Record blocks: kSweepLeadPad padding blocks, then one per pair; a block
+  // is P records + one header {first seed of the pair, 0, 0, 0}.  Record
+  // position j of pair m: {odd node's two children, even node's grandchildren
+  // 0-1, grandchildren 2-3 (16-bit LDS word indices), odd seed | even seed
+  // << 16 (indices in the pair's seed list, 0xFFFF = none)}.  Layer 2m + 2
+  // lives in ring slot (m + 1) mod 2 (words [slot P, slot P + P)); absent
+  // children read the zero word 2P.
+  const uint32_t zeroW = 2u * P;
+  img.act.assign(static_cast<size_t>(kSweepLeadPad + numPairs) * (P + 1) * 4, 0u);
+  for (uint32_t blk = 0; blk < kSweepLeadPad + numPairs; ++blk) {
+    uint32_t *rec = img.act.data() + static_cast<size_t>(blk) * (P + 1) * 4;
+    for (uint32_t j = 0; j < P; ++j) {
+      rec[4 * j] = rec[4 * j + 1] = rec[4 * j + 2] = zeroW | (zeroW << 16);
+      rec[4 * j + 3] = 0xFFFFFFFFu;
+    }
+    if (blk < kSweepLeadPad) continue;
+    const uint32_t m = blk - kSweepLeadPad;
+    rec[4 * P] = pairFirst[m];
+    auto word = [&](uint32_t ch) { return ch == kNone ? zeroW : ((m + 1) & 1u) * P + actIdx[ch]; };
+    if (2 * m + 1 < L)
+      for (uint32_t n = (2 * m + 1) * npl; n < (2 * m + 2) * npl; ++n) {
+        if (actIdx[n] == kNone) continue;
+        uint32_t *r = rec + 4 * actIdx[n];
+        const uint64_t a = g.adjacency[n];
+        r[0] = word(static_cast<uint32_t>(a)) | (word(static_cast<uint32_t>(a >> 32)) << 16);
+        r[3] = (r[3] & 0xFFFF0000u) | local[n];
+      }
+    for (uint32_t n = 2 * m * npl; n < (2 * m + 1) * npl; ++n) {
+      if (actIdx[n] == kNone) continue;
+      uint32_t gc[4], ng = 0;
+      auto toggle = [&](uint32_t x) {
+        for (uint32_t i = 0; i < ng; ++i)
+          if (gc[i] == x) {
+            gc[i] = gc[--ng];
+            return;
+          }
+        if (ng == 4) throw RuntimeError("STEPG: a node reaches more than four nodes two layers on");
+        gc[ng++] = x;
+      };
+      const uint64_t a = g.adjacency[n];
+      for (uint32_t ch : {static_cast<uint32_t>(a), static_cast<uint32_t>(a >> 32)}) {
+        if (ch == kNone) continue;
+        const uint64_t b2 = g.adjacency[ch];
+        for (uint32_t gch : {static_cast<uint32_t>(b2), static_cast<uint32_t>(b2 >> 32)})
+          if (gch != kNone) toggle(gch);
+      }
+      uint32_t w[4] = {zeroW, zeroW, zeroW, zeroW};
+      for (uint32_t i = 0; i < ng; ++i) w[i] = word(gc[i]);
+      uint32_t *r = rec + 4 * actIdx[n];
+      r[1] = w[0] | (w[1] << 16);
+      r[2] = w[2] | (w[3] << 16);
+      r[3] = (r[3] & 0x0000FFFFu) | (evenSeed[n] << 16);
     }
   }
 
@@ -527,9 +620,9 @@ uint64_t imageDeviceBytes(const StepgImage &img) {
   while (B * 64 < R) B <<= 1;
   const uint64_t slots = B * 256 + R;
   uint64_t b = 0;
-  b += static_cast<uint64_t>(kSweepLeadPad + img.numLayers) * img.layerStride * 16ull + img.seedVals.size() * 4ull;
+  b += static_cast<uint64_t>(kSweepLeadPad + img.numPairs) * (img.layerStride + 1) * 16ull + img.seedVals.size() * 4ull;
   b += img.numCols * 16ull + R * 16ull + img.rankSpill.size() * 4ull + img.probTab.size() * 8ull;  // column bands, probabilities, frame rows
-  b += (static_cast<uint64_t>(img.numLayers) * img.layerStride + kSinkSlots) * img.numCols * 4ull;  // frame sensitivities
+  b += 2ull * (img.numPairs + kSweepLeadPad) * img.layerStride * img.numCols * 4ull;  // frame sensitivities
   b += R * (16 + 64 + 16 + 24);                 // node records, target rows, overflow list, sort workspace
   b += slots * (8 + 4 + 4 + 8 + 8 + 4 + 4);     // bucket slots: keys, ranks, run arrays
   b += (B + 1) * 4 * 9;                         // per-bucket counters, counts and offsets

@@ -20,19 +20,26 @@
 //    An auxiliary node's sensitivity is therefore the XOR of at most four
 //    frame nodes' sensitivities, which the later stages form on the fly from
 //    its `rankRows`; the sequential sweep touches frame nodes only;
-//  * each layer is a fixed-stride block of `layerStride` 16-byte records,
-//    the layer's active nodes in node order, then padding; a record holds
-//    the final addresses the sweep uses, so it does no index arithmetic:
-//    {LDS byte address of child 0's sensitivity, of child 1's, of the seed
-//    word, byte offset of the node's sensitivity in its column of S (frame
-//    row l * layerStride + position)}.  The
-//    sweep keeps layers l, l+1, l+2 in an LDS ring whose slot for layer l is
-//    l mod 3 (slot size `layerStride` words), followed by a zero word (what
-//    absent children, absent seeds and padding read) and the column's seed
-//    words.  Padding in front of layer 0 writes to one of kSinkSlots sink
-//    words past the column's frame rows;
-//  * measurement seeds are stored once per seeded node, column-major over
-//    32-bit detector words ("columns");
+//  * the sweep takes layers in pairs (2m + 1, 2m), one step per pair: frame
+//    flows always go from layer l to l + 1, so a node of the odd layer 2m + 1
+//    is the XOR of its seed and its two children in layer 2m + 2, and a node
+//    of the even layer 2m is the XOR of its own and its children's seeds and
+//    of its grandchildren in layer 2m + 2 (at most four after cancellation).
+//    Only even layers are read back: the sweep keeps layer 2m + 2 in an LDS
+//    ring of two slots (slot (m + 1) mod 2, `layerStride` words each),
+//    followed by a zero word (what absent nodes and padding read) and the
+//    column's seed words;
+//  * each pair is a fixed-stride block of `layerStride` 16-byte records plus
+//    a header, position j holding the odd layer's j-th and the even layer's
+//    j-th active node (node order): {odd node's children, even node's
+//    grandchildren 0-1, grandchildren 2-3 (16-bit LDS word indices), odd
+//    seed | even seed << 16 (indices in the pair's seed list, 0xFFFF =
+//    none)}; the header holds the pair's first seed.  A node's sensitivity
+//    goes to row l * layerStride + position of its column of S;
+//  * measurement seeds are stored per pair (the seeds of the pair's two
+//    layers, then the XOR of the seeds each even node combines, where it
+//    combines more than one), column-major over 32-bit detector words
+//    ("columns");
 //  * per column, the highest and lowest layer holding a seed bound the time
 //    band in which the column can be non-zero;
 //  * the p > 0 nodes ("ranks", ascending node order) are the items that are
@@ -48,14 +55,13 @@
 namespace gpa {
 
 constexpr uint32_t kNone = 0xFFFFFFFFu;
-// sink words per column of S (targets of record stores that keep no value)
-constexpr uint32_t kSinkSlots = 1024;
 // largest sweep workgroup; a layer is split over sweepThreads <= this many
 // threads with sweepRpt records each
 constexpr uint32_t kSweepMaxThreads = 1024;
-// all-padding record blocks in front of layer 0 (what the sweep's
-// prefetches load for the layers below 0 that its trips run into)
-constexpr uint32_t kSweepLeadPad = 11;
+// all-padding record blocks (pairs) in front of pair 0: what the sweep's
+// prefetches load for the pairs below 0 that its trips run into; a column of
+// S has rows for their layers after its 2 numPairs layers
+constexpr uint32_t kSweepLeadPad = 7;
 // frame row codes in rankRows: layer << kRowPosBits | position in the layer
 constexpr uint32_t kRowPosBits = 14;
 // Compact frame rows (rankCode, two words per rank, what the device reads):
@@ -109,10 +115,11 @@ struct StepgImage {
   uint32_t sweepThreads = 64, sweepRpt = 1, layerStride = 64;
   // largest (frame row layer - rank layer) over all rankRows entries
   uint32_t rowSpan = 0;
+  uint32_t numPairs = 0;           // (numLayers + 1) / 2 sweep steps
   std::vector<uint32_t> layerAct;  // numLayers + 1 offsets of each layer's first active node
-  std::vector<uint32_t> act;       // 4 * (kSweepLeadPad + numLayers) * layerStride: padding
-                                   // blocks, then one block per layer (see the file comment)
-  std::vector<uint32_t> seedVals;  // numCols x numSeeds (column-major)
+  std::vector<uint32_t> act;       // 4 * (kSweepLeadPad + numPairs) * (layerStride + 1): padding
+                                   // blocks, then one block per pair (see the file comment)
+  std::vector<uint32_t> seedVals;  // numCols x numSeeds (column-major; grouped by pair)
   std::vector<int32_t> colHi, colLow;  // numCols: seed layer range (-1 = no seed)
   std::vector<uint32_t> rankLayer; // numRanks (host only)
   std::vector<double> rankProb;    // numRanks (sent when probTab is empty)
