@@ -1375,9 +1375,10 @@ struct Pipeline::Impl {
   size_t sweepLdsBytes = 0;
   int seedsInLds = 1;
   uint32_t sweepThreads = 64;
-  // bucket counters sit countStride words apart (one per 128-byte line when
-  // there are few buckets): atomics on one line serialize, and classify
-  // issues one per node
+  // bucket counters sit countStride words apart (16 bytes when there are
+  // few buckets): classify issues one atomic per node; measured on bb12,
+  // 16-byte spacing beats both packed counters (same-line atomics
+  // serialize) and one counter per 128-byte line (four times the lines)
   uint32_t countStride = 1;
 
   // per node: stash, info (probability, second hash, class size), overflow
@@ -1609,7 +1610,7 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   m.ovRank.ensure(4 * R, frozen, "nodes");
   m.ovBucket.ensure(4 * R, frozen, "nodes");
   m.counters.ensure(4 * kFlagWords, frozen, "buckets");
-  m.countStride = 32;
+  m.countStride = 4;
   while (m.countStride > 1 && static_cast<uint64_t>(B) * m.countStride > (1u << 17)) m.countStride >>= 1;
   // (B + 1) * countStride <= 2^17 + 32 words whatever B is, so a reserved
   // pipeline fits every later load)
