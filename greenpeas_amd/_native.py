@@ -47,6 +47,7 @@ class DemView(ctypes.Structure):
         ("num_errors", ctypes.c_uint32),
         ("num_detectors", ctypes.c_uint32),
         ("num_observables", ctypes.c_uint32),
+        ("target_bytes", ctypes.c_uint32),
         ("num_targets", ctypes.c_uint64),
         ("token", ctypes.c_void_p),
     ]

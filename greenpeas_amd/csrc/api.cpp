@@ -62,7 +62,7 @@ size_t formatDem(const gpa::VariantResult &r, std::string &out) {
     int n = std::snprintf(buf, sizeof buf, "error(%.17g)", r.probs[e]);
     out.append(buf, n);
     for (uint32_t k = r.offsets[e]; k < r.offsets[e + 1]; ++k) {
-      const uint32_t t = r.targets[k];
+      const uint32_t t = r.target(k);
       if (t < r.numDetectors) n = std::snprintf(buf, sizeof buf, " D%u", t);
       else n = std::snprintf(buf, sizeof buf, " L%u", t - r.numDetectors);
       out.append(buf, n);
@@ -144,6 +144,10 @@ int gp_get_driver(const char *circuit_text, int correlation_level, int device, g
     d->device = device;
     parseInto(circuit_text, correlation_level, d->parsed);
     d->pipe = std::make_unique<gpa::Pipeline>(device, false);
+    // the DEM is the reference's (no hashes; they are read on request),
+    // with 16-bit targets where they fit
+    d->pipe->setDownloadHashes(false);
+    d->pipe->setNarrowTargets(true);
     d->pipe->reserve({&d->parsed.image});
     d->haveCircuit = true;
     d->loaded = true;
@@ -212,8 +216,11 @@ int gp_dem_copy(const gp_driver *driver, double *probs, uint32_t *offsets, uint3
     const auto &r = driver->result;
     if (probs) std::memcpy(probs, r.probs, 8ull * r.numErrors);
     if (offsets) std::memcpy(offsets, r.offsets, 4ull * (r.numErrors + 1));
-    if (targets) std::memcpy(targets, r.targets, 4ull * r.numTargets);
-    if (hashes) std::memcpy(hashes, r.hashes, 8ull * r.numErrors);
+    if (targets) r.copyTargets(targets);
+    if (hashes) {
+      if (r.hashes) std::memcpy(hashes, r.hashes, 8ull * r.numErrors);
+      else driver->pipe->copyHashes(0, hashes);
+    }
   });
 }
 
@@ -227,7 +234,8 @@ int gp_dem_acquire(const gp_driver *driver, gp_dem_view *view) {
     auto *h = new gpa::ResultHandle(driver->pipe->resultBlock());
     view->probs = r.probs;
     view->offsets = r.offsets;
-    view->targets = r.targets;
+    view->targets = r.targets16 ? static_cast<const void *>(r.targets16) : static_cast<const void *>(r.targets);
+    view->target_bytes = r.targets16 ? 2u : 4u;
     view->hashes = r.hashes;
     view->num_errors = r.numErrors;
     view->num_detectors = r.numDetectors;
@@ -430,7 +438,8 @@ static double timeGraph(gpa::Pipeline &p) {
 static void keepResult(gp_batch *b, uint32_t i, const gpa::VariantResult &r) {
   b->ownP[i].assign(r.probs, r.probs + r.numErrors);
   b->ownOff[i].assign(r.offsets, r.offsets + r.numErrors + 1);
-  b->ownTg[i].assign(r.targets, r.targets + r.numTargets);
+  b->ownTg[i].resize(r.numTargets);
+  r.copyTargets(b->ownTg[i].data());
   b->ownHash[i].assign(r.hashes, r.hashes + r.numErrors);
   gpa::VariantResult &k = b->results[i];
   k.numErrors = r.numErrors;
@@ -512,7 +521,7 @@ int gp_batch_dem_copy(const gp_batch *batch, uint32_t index, double *probs, uint
     const auto &r = batch->results[index];
     if (probs) std::memcpy(probs, r.probs, 8ull * r.numErrors);
     if (offsets) std::memcpy(offsets, r.offsets, 4ull * (r.numErrors + 1));
-    if (targets) std::memcpy(targets, r.targets, 4ull * r.numTargets);
+    if (targets) r.copyTargets(targets);
     if (hashes) std::memcpy(hashes, r.hashes, 8ull * r.numErrors);
   });
 }
@@ -636,7 +645,7 @@ int gp_grad_plan_dem_structure(const gp_grad_plan *plan, uint32_t *offsets, uint
     if (!plan) throw InvalidArgument("plan is NULL");
     const auto &r = plan->result;
     if (offsets) std::memcpy(offsets, r.offsets, 4ull * (r.numErrors + 1));
-    if (targets) std::memcpy(targets, r.targets, 4ull * r.numTargets);
+    if (targets) r.copyTargets(targets);
   });
 }
 

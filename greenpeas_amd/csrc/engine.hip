@@ -981,6 +981,7 @@ struct EmitOut {
   uint64_t *hash;
   uint32_t *member;
   uint32_t *targets;
+  uint16_t *targets16;  // non-null: targets are written here, 16 bits each
   uint64_t capTargets;
   uint32_t *flags, *vBounds;
 };
@@ -1077,12 +1078,19 @@ __global__ __launch_bounds__(256) void emitKernel(const VDesc *__restrict__ desc
       }
 #pragma unroll
       for (uint32_t j = 0; j < kStashTargets; ++j)
-        if (j < n && pos + j < out.capTargets) out.targets[pos + j] = (w[j / 2] >> (16 * (j % 2))) & 0xFFFFu;
+        if (j < n && pos + j < out.capTargets) {
+          const uint32_t t = (w[j / 2] >> (16 * (j % 2))) & 0xFFFFu;
+          if (out.targets16) out.targets16[pos + j] = static_cast<uint16_t>(t);
+          else out.targets[pos + j] = t;
+        }
       pos += n;
     } else {
       const FrameRows fr = decodeRows(expandRankCode(in.rankCode[rk], in.spill + d.spillOff), d.layerStride);
       forEachTargetScan(d.numCols, in.band + d.colOff, fr, in.S + d.sOff, d.colStride, [&](uint32_t tg) {
-        if (pos < out.capTargets) out.targets[pos] = tg;
+        if (pos < out.capTargets) {
+          if (out.targets16) out.targets16[pos] = static_cast<uint16_t>(tg);
+          else out.targets[pos] = tg;
+        }
         pos++;
       });
     }
@@ -1347,8 +1355,20 @@ size_t align256(size_t x) { return (x + 255) & ~static_cast<size_t>(255); }
 struct Pipeline::Impl {
   int device;
   hipStream_t stream = nullptr;
-  hipEvent_t ev[8] = {};  // [0, 5): stage marks of a timed run; 5, 6: upload
+  hipEvent_t ev[8] = {};  // [0, 5): stage marks of a timed run; 6, 7: benchDevice
+  // upload: the sweep's inputs on `stream`, the error nodes' inputs on
+  // `side` (overlapping the sweep; classify waits for evSide)
+  hipStream_t side = nullptr;
+  hipEvent_t evUp0 = nullptr, evUp1 = nullptr, evPartA = nullptr, evSide = nullptr;
   bool uploadPending = false;  // upload events recorded, not yet read
+  bool sidePending = false;    // the main stream has not yet waited for evSide
+  // the main stream waits for the side upload (before classify, or before
+  // a graph launch: the graph holds no cross-stream wait)
+  void joinSide() {
+    if (!sidePending) return;
+    GPA_HIP(hipStreamWaitEvent(stream, evSide, 0));
+    sidePending = false;
+  }
   bool frozen = false;
   bool timedRun = false;
   hipEvent_t *evOverride = nullptr;  // per-iteration events of benchDevice
@@ -1395,6 +1415,11 @@ struct Pipeline::Impl {
   // host results: the block of the last download, blocks released by
   // their holders for reuse
   std::shared_ptr<ResultBlock> res;
+  bool wantHashes = true;  // download the class hashes with the rest
+  bool narrowTargets = false;  // 16-bit targets allowed (setNarrowTargets)
+  bool targets16 = false;      // this load's targets are 16-bit
+  bool sizesKnown = false;  // lastC / lastT hold the DEM size of this load's circuits
+  uint64_t lastC = 0, lastT = 0;
   std::shared_ptr<ResultPool> pool = std::make_shared<ResultPool>();
   HostBuf hVB, hFlags;
   uint32_t totalClasses = 0;
@@ -1423,14 +1448,23 @@ Pipeline::Pipeline(int device, bool growable) : device_(device), growable_(growa
   impl_->device = device;
   GPA_HIP(hipSetDevice(device));
   GPA_HIP(hipStreamCreateWithFlags(&impl_->stream, hipStreamNonBlocking));
+  GPA_HIP(hipStreamCreateWithFlags(&impl_->side, hipStreamNonBlocking));
   for (auto &e : impl_->ev) GPA_HIP(hipEventCreate(&e));
+  GPA_HIP(hipEventCreate(&impl_->evUp0));
+  GPA_HIP(hipEventCreate(&impl_->evUp1));
+  GPA_HIP(hipEventCreateWithFlags(&impl_->evPartA, hipEventDisableTiming));
+  GPA_HIP(hipEventCreateWithFlags(&impl_->evSide, hipEventDisableTiming));
 }
 
 Pipeline::~Pipeline() {
   (void)hipSetDevice(device_);
   if (impl_->stream) (void)hipStreamSynchronize(impl_->stream);
+  if (impl_->side) (void)hipStreamSynchronize(impl_->side);
   for (auto &e : impl_->ev)
     if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : {impl_->evUp0, impl_->evUp1, impl_->evPartA, impl_->evSide})
+    if (e) (void)hipEventDestroy(e);
+  if (impl_->side) (void)hipStreamDestroy(impl_->side);
   for (auto &e : impl_->evPool) (void)hipEventDestroy(e);
   impl_->dropGraph();
   if (impl_->stream) (void)hipStreamDestroy(impl_->stream);
@@ -1457,7 +1491,10 @@ static constexpr size_t kLdsBudget = 150 * 1024;  // of gfx950's 160 KB per work
 void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   Impl &m = *impl_;
   GPA_HIP(hipSetDevice(device_));
+  m.joinSide();
   GPA_HIP(hipStreamSynchronize(m.stream));
+  GPA_HIP(hipStreamSynchronize(m.side));
+  m.sizesKnown = false;
   const bool frozen = m.frozen;
   const uint32_t V = static_cast<uint32_t>(imgs.size());
   if (V == 0) throw InvalidArgument("no circuits to compile");
@@ -1552,6 +1589,9 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   m.nItems = static_cast<uint32_t>(sortedItems.size());
   m.nTiles = static_cast<uint32_t>(tiles.size());
   m.classTable = classTable;
+  m.targets16 = m.narrowTargets;
+  for (const StepgImage *g : imgs)
+    if (g->numDetectors + static_cast<uint64_t>(g->numObservables) > 65536) m.targets16 = false;
   m.B = bucketBase;
   m.R = static_cast<uint32_t>(rankOff);
   m.Ccols = static_cast<uint32_t>(colOff);
@@ -1672,11 +1712,26 @@ void Pipeline::upload() {
   Impl &m = *impl_;
   if (!m.inBytes) throw RuntimeError("nothing loaded");
   GPA_HIP(hipSetDevice(device_));
-  // one copy of the packed inputs, stream-ordered before the next run (its
-  // duration is read once the stream has drained, in download())
-  GPA_HIP(hipEventRecord(m.ev[5], m.stream));
-  GPA_HIP(hipMemcpyAsync(m.in.p, m.inHost.p, m.inBytes, hipMemcpyHostToDevice, m.stream));
-  GPA_HIP(hipEventRecord(m.ev[6], m.stream));
+  // the arena's head (descriptors, sweep records, seeds, bands: what the
+  // sweep reads) on the pipeline's stream; then its tail (the error nodes'
+  // frame rows and probabilities, read from classify on) on the side
+  // stream, overlapping the sweep.  Stream-ordered before the next run; the
+  // duration is read once the streams have drained, in download().
+  m.joinSide();
+  const size_t headBytes = m.oRankCode;
+  GPA_HIP(hipEventRecord(m.evUp0, m.stream));
+  GPA_HIP(hipMemcpyAsync(m.in.p, m.inHost.p, headBytes, hipMemcpyHostToDevice, m.stream));
+  if (m.inBytes > headBytes) {
+    GPA_HIP(hipEventRecord(m.evPartA, m.stream));
+    GPA_HIP(hipStreamWaitEvent(m.side, m.evPartA, 0));
+    GPA_HIP(hipMemcpyAsync(m.in.as<char>() + headBytes, m.inHost.as<char>() + headBytes, m.inBytes - headBytes,
+                           hipMemcpyHostToDevice, m.side));
+    GPA_HIP(hipEventRecord(m.evUp1, m.side));
+    GPA_HIP(hipEventRecord(m.evSide, m.side));
+    m.sidePending = true;
+  } else {
+    GPA_HIP(hipEventRecord(m.evUp1, m.stream));
+  }
   m.uploadPending = true;
 }
 
@@ -1731,6 +1786,7 @@ void Pipeline::run(bool timed) {
 #undef GPA_SWEEP
   }
   mark(1);
+  m.joinSide();  // classify reads the error nodes' inputs
   if (m.nTiles) {
     ClassifyOut co{m.info.as<uint4>(), m.bucketCount(),
                    m.countStride, m.pkey.as<uint64_t>(), m.prank.as<uint32_t>(), m.ovKey.as<uint64_t>(),
@@ -1774,7 +1830,8 @@ void Pipeline::run(bool timed) {
              m.runHash.as<uint64_t>(), m.runNnz.as<uint32_t>(), m.S.as<uint32_t>(),
              m.stash.as<uint32_t>(), rin.code, rin.spill, stashMaxCols, m.band.as<int2>()};
   EmitOut eout{m.outP.as<double>(), m.outOff.as<uint32_t>(), m.outHash.as<uint64_t>(), m.outMember.as<uint32_t>(),
-               m.outTargets.as<uint32_t>(), m.capTargets, m.flags(), m.vBounds.as<uint32_t>()};
+               m.outTargets.as<uint32_t>(), m.targets16 ? m.outTargets.as<uint16_t>() : nullptr, m.capTargets,
+               m.flags(), m.vBounds.as<uint32_t>()};
   emitKernel<<<std::max<uint32_t>((B + 3) / 4, 1), 256, 0, st>>>(desc, ein, eout);
   mark(4);
   GPA_HIP(hipGetLastError());
@@ -1785,12 +1842,35 @@ void Pipeline::download() {
   GPA_HIP(hipSetDevice(device_));
   const uint32_t V = m.V;
   const auto t0 = std::chrono::steady_clock::now();
+  // a block still held by a caller keeps its contents; take another
+  if (!m.res || m.res.use_count() > 1) m.res = takeBlock(m.pool);
+  ResultBlock &rb = *m.res;
+  auto copyArrays = [&](uint64_t C, uint64_t T) {
+    rb.p.ensure(8 * C + 8);
+    rb.off.ensure(4 * (C + 1));
+    rb.hash.ensure(8 * C + 8);
+    rb.targets.ensure(4 * T + 4);
+    const uint64_t tb = m.targets16 ? 2 : 4;
+    if (C) {
+      GPA_HIP(hipMemcpyAsync(rb.p.p, m.outP.p, 8 * C, hipMemcpyDeviceToHost, m.stream));
+      if (m.wantHashes) GPA_HIP(hipMemcpyAsync(rb.hash.p, m.outHash.p, 8 * C, hipMemcpyDeviceToHost, m.stream));
+    }
+    GPA_HIP(hipMemcpyAsync(rb.off.p, m.outOff.p, 4 * (C + 1), hipMemcpyDeviceToHost, m.stream));
+    if (T) GPA_HIP(hipMemcpyAsync(rb.targets.p, m.outTargets.p, tb * T, hipMemcpyDeviceToHost, m.stream));
+  };
   GPA_HIP(hipMemcpyAsync(m.hVB.p, m.vBounds.p, 4 * (2 * V + 2), hipMemcpyDeviceToHost, m.stream));
   GPA_HIP(hipMemcpyAsync(m.hFlags.p, m.counters.p, 4 * kFlagWords, hipMemcpyDeviceToHost, m.stream));
+  // The DEM of a loaded circuit has the same size on every run: once one
+  // run has told it, the arrays are copied with the sizes, behind one wait
+  // (and again if the sizes differ).
+  const bool guessed = m.sizesKnown;
+  const uint64_t gC = m.lastC, gT = m.lastT;
+  if (guessed) copyArrays(gC, gT);
   GPA_HIP(hipStreamSynchronize(m.stream));
   if (m.uploadPending) {
     float ms = 0;
-    GPA_HIP(hipEventElapsedTime(&ms, m.ev[5], m.ev[6]));
+    GPA_HIP(hipEventSynchronize(m.evUp1));
+    GPA_HIP(hipEventElapsedTime(&ms, m.evUp0, m.evUp1));
     times_.upload = ms * 1e3;
     m.uploadPending = false;
   }
@@ -1815,6 +1895,7 @@ void Pipeline::download() {
       throw RuntimeError("circuit exceeds the driver's capacity (targets)");
     m.outTargets.ensure(4 * (m.totalTargets + m.totalTargets / 4 + 64), false, "targets");
     m.capTargets = m.outTargets.cap / 4;
+    m.sizesKnown = false;
     run(false);
     download();
     return;
@@ -1826,20 +1907,13 @@ void Pipeline::download() {
     m.vTgt[v] = vb[2 * v + 1];
   }
   const uint64_t C = m.totalClasses, T = m.totalTargets;
-  // a block still held by a caller keeps its contents; take another
-  if (!m.res || m.res.use_count() > 1) m.res = takeBlock(m.pool);
-  ResultBlock &rb = *m.res;
-  rb.p.ensure(8 * C + 8);
-  rb.off.ensure(4 * (C + 1));
-  rb.hash.ensure(8 * C + 8);
-  rb.targets.ensure(4 * T + 4);
-  if (C) {
-    GPA_HIP(hipMemcpyAsync(rb.p.p, m.outP.p, 8 * C, hipMemcpyDeviceToHost, m.stream));
-    GPA_HIP(hipMemcpyAsync(rb.hash.p, m.outHash.p, 8 * C, hipMemcpyDeviceToHost, m.stream));
+  if (!guessed || C != gC || T != gT) {
+    copyArrays(C, T);
+    GPA_HIP(hipStreamSynchronize(m.stream));
   }
-  GPA_HIP(hipMemcpyAsync(rb.off.p, m.outOff.p, 4 * (C + 1), hipMemcpyDeviceToHost, m.stream));
-  if (T) GPA_HIP(hipMemcpyAsync(rb.targets.p, m.outTargets.p, 4 * T, hipMemcpyDeviceToHost, m.stream));
-  GPA_HIP(hipStreamSynchronize(m.stream));
+  m.lastC = C;
+  m.lastT = T;
+  m.sizesKnown = true;
   times_.download =
       std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
 }
@@ -1856,8 +1930,9 @@ void Pipeline::result(uint32_t v, VariantResult &out) const {
   if (!m.res) throw RuntimeError("no results downloaded");
   const ResultBlock &rb = *m.res;
   out.probs = rb.p.as<double>() + c0;
-  out.hashes = rb.hash.as<uint64_t>() + c0;
-  out.targets = rb.targets.as<uint32_t>() + t0;
+  out.hashes = m.wantHashes ? rb.hash.as<uint64_t>() + c0 : nullptr;
+  out.targets = m.targets16 ? nullptr : rb.targets.as<uint32_t>() + t0;
+  out.targets16 = m.targets16 ? rb.targets.as<uint16_t>() + t0 : nullptr;
   const uint32_t *off = rb.off.as<uint32_t>();
   if (t0 == 0) {
     // the first circuit's offsets are its own
@@ -1872,9 +1947,22 @@ void Pipeline::result(uint32_t v, VariantResult &out) const {
 
 ResultHandle Pipeline::resultBlock() const { return impl_->res; }
 
+void Pipeline::setDownloadHashes(bool on) { impl_->wantHashes = on; }
+void Pipeline::setNarrowTargets(bool on) { impl_->narrowTargets = on; }
+
+void Pipeline::copyHashes(uint32_t v, uint64_t *dst) const {
+  const Impl &m = *impl_;
+  if (v >= m.V || m.vClass.size() != m.V + 1) throw RuntimeError("no results downloaded");
+  const uint32_t c0 = m.vClass[v], n = m.vClass[v + 1] - c0;
+  GPA_HIP(hipSetDevice(device_));
+  GPA_HIP(hipStreamSynchronize(m.stream));
+  if (n) GPA_HIP(hipMemcpy(dst, m.outHash.as<uint64_t>() + c0, 8ull * n, hipMemcpyDeviceToHost));
+}
+
 void Pipeline::runGraph() {
   Impl &m = *impl_;
   GPA_HIP(hipSetDevice(device_));
+  m.joinSide();
   if (!m.graphExec) {
     GPA_HIP(hipStreamBeginCapture(m.stream, hipStreamCaptureModeThreadLocal));
     try {

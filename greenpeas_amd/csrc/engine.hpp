@@ -18,6 +18,7 @@
 //              (replaces the full N x 32 gather + copy, CUDA.hpp:488-497).
 #pragma once
 
+#include <algorithm>
 #include <cstdint>
 #include <memory>
 #include <vector>
@@ -32,9 +33,15 @@ struct VariantResult {
   uint32_t numDetectors = 0, numObservables = 0;
   const double *probs = nullptr;     // numErrors
   const uint32_t *offsets = nullptr; // numErrors + 1, relative to targets
-  const uint32_t *targets = nullptr; // numTargets
-  const uint64_t *hashes = nullptr;  // numErrors
+  const uint32_t *targets = nullptr; // numTargets (nullptr when targets16 holds them)
+  const uint16_t *targets16 = nullptr;  // numTargets, 16 bits each (setNarrowTargets)
+  const uint64_t *hashes = nullptr;  // numErrors (nullptr when not downloaded)
   std::vector<uint32_t> offsetsRel;  // storage for offsets
+  uint32_t target(uint64_t k) const { return targets16 ? targets16[k] : targets[k]; }
+  void copyTargets(uint32_t *dst) const {
+    if (targets16) for (uint64_t k = 0; k < numTargets; ++k) dst[k] = targets16[k];
+    else if (numTargets) std::copy(targets, targets + numTargets, dst);
+  }
 };
 
 // Pinned host memory holding one download's results (defined in
@@ -79,6 +86,15 @@ class Pipeline {
   void result(uint32_t v, VariantResult &out) const;
   // The block the last download wrote (the arrays result() points into).
   ResultHandle resultBlock() const;
+  // Whether download() copies the class hashes too (default on; off,
+  // result() gives no hashes and copyHashes() reads them from the device).
+  void setDownloadHashes(bool on);
+  // Whether circuits of at most 65536 targets get 16-bit target arrays
+  // (default off; halves the target download).  Read at the next load.
+  void setNarrowTargets(bool on);
+  // Circuit v's class hashes of the last run, from the device (valid until
+  // the next run).
+  void copyHashes(uint32_t v, uint64_t *dst) const;
   const StageTimes &times() const { return times_; }
   uint64_t deviceBytes() const;
   int device() const { return device_; }

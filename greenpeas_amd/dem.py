@@ -20,7 +20,9 @@ _ERROR_RE = re.compile(r"^\s*error\(([^)]*)\)(.*)$")
 @dataclass
 class DetectorErrorModel:
     """Errors in output order: probability, and targets (detector index t <
-    num_detectors, else observable t - num_detectors) as CSR."""
+    num_detectors, else observable t - num_detectors) as CSR.  Targets are
+    unsigned integers: 16-bit in a driver's DEM when every target fits, else
+    32-bit."""
 
     probs: np.ndarray
     offsets: np.ndarray

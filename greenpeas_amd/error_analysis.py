@@ -96,8 +96,11 @@ def _view_dem(lib, handle) -> DetectorErrorModel:
     n, t = v.num_errors, v.num_targets
     probs = _pinned_array(v.probs, n, ctypes.c_double, np.float64, owner)
     offsets = _pinned_array(v.offsets, n + 1, ctypes.c_uint32, np.uint32, owner)
-    targets = _pinned_array(v.targets, t, ctypes.c_uint32, np.uint32, owner)
-    hashes = _pinned_array(v.hashes, n, ctypes.c_uint64, np.uint64, owner)
+    if v.target_bytes == 2:
+        targets = _pinned_array(v.targets, t, ctypes.c_uint16, np.uint16, owner)
+    else:
+        targets = _pinned_array(v.targets, t, ctypes.c_uint32, np.uint32, owner)
+    hashes = _pinned_array(v.hashes, n, ctypes.c_uint64, np.uint64, owner) if v.hashes else None
     return DetectorErrorModel(probs, offsets, targets, v.num_detectors, v.num_observables, hashes)
 
 
@@ -141,6 +144,16 @@ class Driver:
         """The last compile's DEM; its arrays are the pinned host memory the
         driver downloaded into (no copy), kept alive by the arrays."""
         return _view_dem(self._lib, self._h)
+
+    def last_hashes(self) -> np.ndarray:
+        """The 64-bit class hash of each error of the last compile, in output
+        order (the sort key), read from the device."""
+        ne, nt, nd, no = ctypes.c_uint32(), ctypes.c_uint64(), ctypes.c_uint32(), ctypes.c_uint32()
+        _native.check(self._lib.gp_dem_shape(self._h, ctypes.byref(ne), ctypes.byref(nt), ctypes.byref(nd),
+                                             ctypes.byref(no)))
+        h = np.empty(max(ne.value, 1), np.uint64)
+        _native.check(self._lib.gp_dem_copy(self._h, None, None, None, h.ctypes.data))
+        return h[: ne.value]
 
     def last_dem_copy(self) -> DetectorErrorModel:
         """The last compile's DEM copied into fresh numpy arrays."""

@@ -81,7 +81,8 @@ int gp_dem_shape(const gp_driver *driver, uint32_t *num_errors,
 /* Copies the last DEM as CSR: probs[num_errors], offsets[num_errors + 1],
  * targets[num_targets] where a target t < num_detectors is detector t and
  * otherwise observable t - num_detectors.  hashes (optional, may be NULL)
- * receives each error's 64-bit class hash. */
+ * receives each error's 64-bit class hash (read from the device: a
+ * compile downloads only the DEM itself). */
 int gp_dem_copy(const gp_driver *driver, double *probs, uint32_t *offsets,
                 uint32_t *targets, uint64_t *hashes);
 
@@ -90,13 +91,15 @@ int gp_dem_copy(const gp_driver *driver, double *probs, uint32_t *offsets,
  * DEM is returned by value).  The arrays stay valid until
  * gp_dem_release(view->token), also across later compiles and after the
  * driver is destroyed; the driver downloads later DEMs elsewhere while a
- * view is held.  Layout as gp_dem_copy. */
+ * view is held.  Layout as gp_dem_copy, except that targets are 16-bit
+ * (target_bytes = 2) when a circuit has at most 65536 detectors and
+ * observables. */
 typedef struct gp_dem_view {
   const double *probs;       /* num_errors */
   const uint32_t *offsets;   /* num_errors + 1 */
-  const uint32_t *targets;   /* num_targets */
-  const uint64_t *hashes;    /* num_errors */
-  uint32_t num_errors, num_detectors, num_observables;
+  const void *targets;       /* num_targets, target_bytes (2 or 4) each */
+  const uint64_t *hashes;    /* NULL: not downloaded (gp_dem_copy reads them) */
+  uint32_t num_errors, num_detectors, num_observables, target_bytes;
   uint64_t num_targets;
   void *token;
 } gp_dem_view;
