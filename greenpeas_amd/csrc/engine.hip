@@ -673,12 +673,15 @@ __device__ __forceinline__ void forEachTargetScan(uint32_t numCols, const int2 *
 // bits, FNV-1a-64 over it padded with 0xFFFFFFFF to 32 slots (lists longer
 // than 32 hash all entries, unpadded), a second hash and the class size.
 // The node then takes a slot of its sort bucket (first radix digit = top
-// bucketBits hash bits): slot pos of bucket b is pkey/prank[b * kBucketCap
-// + pos], pos from an atomic counter per bucket; nodes beyond a bucket's
-// kBucketCap slots go to the overflow list.  In circuits whose targets fit
-// 16 bits (at most kStashMaxCols columns), the first kStashTargets targets
-// of every node are also stored as one 64-byte row per node (`stash`, 16-bit
-// entries), so the emit stage copies a class's targets from its
+// bucketBits hash bits): slot pos of bucket b is pslot[b * kBucketCap +
+// pos], one 16-byte {hash, rank} record (one write per node), pos from an
+// atomic counter per bucket; nodes beyond a bucket's kBucketCap slots go to
+// the overflow list.  In circuits whose targets fit 16 bits (at most
+// kStashMaxCols columns), the first kStashTargets targets of every node are
+// also stored as one 64-byte row per node (`stash`, 16-bit entries; 32
+// targets cover every class of the reference's circuits, 16 would leave
+// 1.1 % of bb12 L2's to the much slower gather from S), so the emit stage
+// copies a class's targets from its
 // representative's row instead of gathering them from S one column at a
 // time: each wave assembles its 64 rows in LDS and writes them out
 // contiguously.
@@ -691,8 +694,7 @@ struct ClassifyOut {
   uint4 *info;            // per node: nodeInfo(probability, second hash, class size)
   uint32_t *bucketCount;  // bucket b's counter at b * countStride
   uint32_t countStride;
-  uint64_t *pkey;         // bucket slots
-  uint32_t *prank;
+  uint4 *pslot;           // bucket slots: {hash low, hash high, rank, 0}
   uint64_t *ovKey;        // overflow list (count in flags[3])
   uint32_t *ovRank, *ovBucket;
   uint32_t *flags;
@@ -763,8 +765,8 @@ __global__ __launch_bounds__(kClassBlock) void classifyKernel(
       const uint32_t b = d.bucketBase + (d.bucketBits ? static_cast<uint32_t>(h >> (64 - d.bucketBits)) : 0u);
       const uint32_t pos = atomicAdd(&o.bucketCount[static_cast<uint64_t>(b) * o.countStride], 1u);
       if (pos < kBucketCap) {
-        o.pkey[static_cast<uint64_t>(b) * kBucketCap + pos] = h;
-        o.prank[static_cast<uint64_t>(b) * kBucketCap + pos] = static_cast<uint32_t>(gi);
+        o.pslot[static_cast<uint64_t>(b) * kBucketCap + pos] =
+            make_uint4(static_cast<uint32_t>(h), static_cast<uint32_t>(h >> 32), static_cast<uint32_t>(gi), 0u);
       } else {
         const uint32_t k = atomicAdd(&o.flags[3], 1u);
         o.ovKey[k] = h;
@@ -790,15 +792,16 @@ __global__ __launch_bounds__(kClassBlock) void classifyKernel(
 
 template <int R>
 __device__ void bucketWave(uint32_t b, uint32_t bs, uint32_t cnt, uint32_t lane,
-                           const uint64_t *__restrict__ pkey, const RunOut &o,
+                           const uint4 *__restrict__ pslot, const RunOut &o,
                            uint32_t *__restrict__ runCount, uint32_t *__restrict__ runNnz) {
   uint64_t h[R];
   uint32_t r[R];
 #pragma unroll
   for (int s = 0; s < R; ++s) {
     const uint32_t e = static_cast<uint32_t>(s) * 64 + lane;
-    h[s] = e < cnt ? pkey[bs + e] : ~0ull;
-    r[s] = e < cnt ? o.sortedRank[bs + e] : NONE;
+    const uint4 x = e < cnt ? pslot[bs + e] : make_uint4(~0u, ~0u, NONE, 0u);
+    h[s] = (static_cast<uint64_t>(x.y) << 32) | x.x;
+    r[s] = x.z;
   }
   waveBitonicSort<R>(h, r, lane);
   RunCarry cy{0.0, 0ull, 0u, 0u, 0u, 0u};
@@ -812,7 +815,7 @@ __device__ void bucketWave(uint32_t b, uint32_t bs, uint32_t cnt, uint32_t lane,
 struct BucketIo {
   uint32_t *bucketCount;  // read and reset to zero for the next run
   uint32_t countStride;
-  uint64_t *pkey;         // slots (sorted in place)
+  const uint4 *pslot;     // slots written by classify
   const uint64_t *ovKey;  // overflow list (count in flags[3])
   const uint32_t *ovRank, *ovBucket;
   uint32_t *runCount, *runNnz, *itemCount;  // per bucket
@@ -842,8 +845,9 @@ __device__ void bigBucket(uint32_t b, uint32_t cnt, const BucketIo &io, const Ru
   uint32_t *Rk = io.scratchRank + 2ull * off;
   const uint64_t slot0 = static_cast<uint64_t>(b) * kBucketCap;
   for (uint32_t i = threadIdx.x; i < kBucketCap; i += blockDim.x) {
-    K[i] = io.pkey[slot0 + i];
-    Rk[i] = o.sortedRank[slot0 + i];
+    const uint4 x = io.pslot[slot0 + i];
+    K[i] = (static_cast<uint64_t>(x.y) << 32) | x.x;
+    Rk[i] = x.z;
   }
   const uint32_t nov = o.flags[3];
   for (uint32_t k = threadIdx.x; k < nov; k += blockDim.x)
@@ -922,11 +926,11 @@ __global__ __launch_bounds__(256) void bucketKernel(uint32_t numBuckets, BucketI
           io.runNnz[b] = 0;
         }
       } else if (cnt <= 64) {
-        bucketWave<1>(b, bs, cnt, lane, io.pkey, o, io.runCount, io.runNnz);
+        bucketWave<1>(b, bs, cnt, lane, io.pslot, o, io.runCount, io.runNnz);
       } else if (cnt <= 128) {
-        bucketWave<2>(b, bs, cnt, lane, io.pkey, o, io.runCount, io.runNnz);
+        bucketWave<2>(b, bs, cnt, lane, io.pslot, o, io.runCount, io.runNnz);
       } else {
-        bucketWave<4>(b, bs, cnt, lane, io.pkey, o, io.runCount, io.runNnz);
+        bucketWave<4>(b, bs, cnt, lane, io.pslot, o, io.runCount, io.runNnz);
       }
     } else if (lane == 0) {
       const uint32_t k = atomicAdd(&nBig, 1u);
@@ -1404,9 +1408,10 @@ struct Pipeline::Impl {
   // per node: stash, info (probability, second hash, class size), overflow
   // list (ovKey/ovRank/ovBucket);
   // per bucket slot (numBuckets * kBucketCap, plus a numRanks arena for
-  // oversized buckets): pkey, prank (sorted in place) and the run arrays;
+  // oversized buckets): pslot (classify's records), prank (the sorted
+  // ranks) and the run arrays;
   // per bucket: counters, slotBase, item/run/target counts and their scans
-  DevBuf band, S, stash, info, counters, bucketCnt, ovKey, ovRank, ovBucket, pkey, prank,
+  DevBuf band, S, stash, info, counters, bucketCnt, ovKey, ovRank, ovBucket, pslot, prank,
       runRank, runP, runHash, runTgtPre, runMember, runCount, runNnz, itemCount, slotBase, wgSum, classStart,
       tgtStart, memberStart, outP, outOff, outHash, outMember, outTargets, scratchKey, scratchRank,
       scanTmp, vBounds, members;
@@ -1659,7 +1664,7 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   // bucket slots, then an arena of numRanks entries for oversized buckets
   const uint64_t slots = B * static_cast<uint64_t>(kBucketCap) + R;
   if (slots >= 0xFFFFFFFFull) throw RuntimeError("too many sort buckets in one load");
-  m.pkey.ensure(8 * slots, frozen, "nodes");
+  m.pslot.ensure(16 * slots, frozen, "nodes");
   m.prank.ensure(4 * slots, frozen, "nodes");
   m.runRank.ensure(4 * slots, frozen, "nodes");
   m.runP.ensure(8 * slots, frozen, "nodes");
@@ -1789,7 +1794,7 @@ void Pipeline::run(bool timed) {
   m.joinSide();  // classify reads the error nodes' inputs
   if (m.nTiles) {
     ClassifyOut co{m.info.as<uint4>(), m.bucketCount(),
-                   m.countStride, m.pkey.as<uint64_t>(), m.prank.as<uint32_t>(), m.ovKey.as<uint64_t>(),
+                   m.countStride, m.pslot.as<uint4>(), m.ovKey.as<uint64_t>(),
                    m.ovRank.as<uint32_t>(), m.ovBucket.as<uint32_t>(), m.flags(), m.stash.as<uint32_t>(),
                    stashMaxCols};
     if (m.classTable && gMaskTable.load())
@@ -1808,7 +1813,7 @@ void Pipeline::run(bool timed) {
            m.runTgtPre.as<uint32_t>(),
            m.runMember.as<uint32_t>(),
            m.flags()};
-  BucketIo bio{m.bucketCount(), m.countStride, m.pkey.as<uint64_t>(), m.ovKey.as<uint64_t>(),
+  BucketIo bio{m.bucketCount(), m.countStride, m.pslot.as<uint4>(), m.ovKey.as<uint64_t>(),
                m.ovRank.as<uint32_t>(), m.ovBucket.as<uint32_t>(), m.runCount.as<uint32_t>(),
                m.runNnz.as<uint32_t>(), m.itemCount.as<uint32_t>(), m.slotBase.as<uint32_t>(),
                m.wgSum.as<uint32_t>(), m.scratchKey.as<uint64_t>(), m.scratchRank.as<uint32_t>(),
@@ -2034,7 +2039,7 @@ uint64_t Pipeline::deviceBytes() const {
   const Impl &m = *impl_;
   const DevBuf *bufs[] = {&m.in,        &m.band,     &m.S,          &m.stash,     &m.info,
                           &m.counters,   &m.bucketCnt, &m.ovKey,
-                          &m.ovRank,    &m.ovBucket,   &m.pkey,       &m.prank,     &m.runRank,
+                          &m.ovRank,    &m.ovBucket,   &m.pslot,      &m.prank,     &m.runRank,
                           &m.runP,      &m.runHash,    &m.runTgtPre,  &m.runMember, &m.runCount,
                           &m.runNnz,    &m.itemCount,  &m.slotBase, &m.wgSum,   &m.classStart, &m.tgtStart,
                           &m.memberStart, &m.outP,     &m.outOff,     &m.outHash,   &m.outMember,
