@@ -53,7 +53,7 @@ struct VDesc {
   uint64_t probOff;    // first probability (table entry if probDict, else the ranks' own)
 };
 
-// A rank's four rankRows codes from its compact code (the format is given
+// A rank's four frame row codes from its compact code (the format is given
 // with kSpillBase); `spill` = the circuit's spill entries.
 __host__ __device__ inline uint4 expandRankCode(const uint2 c, const uint4 *spill) {
   if ((c.x >> 16) == kSpillBase) return spill[(static_cast<uint64_t>(c.x & 0xFFFFu) << 32) | c.y];
@@ -540,7 +540,7 @@ __global__ __launch_bounds__(kSweepMaxThreads) void sweepKernel(
 }
 
 // One node's frame rows (its sensitivity is their XOR), decoded from its
-// rankRows code: up to four codes layer << kRowPosBits | position, in
+// compact code: up to four codes layer << kRowPosBits | position, in
 // ascending layer order, kNone-padded at the end.  idx = the row's word in a
 // column of S (layer * P + position); the rows span layers [lo, lo + span].
 struct FrameRows {

@@ -276,45 +276,50 @@ bool probTable() { return gProbTable.load() != 0; }
 
 void buildImage(const Stepg &g, StepgImage &img) {
   const uint32_t N = g.numNodes, npl = g.npl, L = g.numLayers, C = g.numCols;
+  const uint32_t k = nodesPerQubit(g.level);
   img.numLayers = L;
   img.npl = npl;
   img.numCols = C;
   img.numDetectors = g.numDetectors;
   img.numObservables = g.numObservables;
   img.numLocations = g.numLocations;
+  img.rowSpan = 0;
+  // slot j of a layer is a frame node (X or Z of its qubit)
+  std::vector<uint8_t> frameSlot(npl);
+  for (uint32_t j = 0; j < npl; ++j) frameSlot[j] = (j % k) < 2;
 
-  // ranks: p > 0 nodes in node order
-  std::vector<uint32_t> rank(N, kNone);
-  img.rankLayer.clear();
-  img.rankProb.clear();
-  img.rankNode.clear();
-  for (uint32_t l = 0, n = 0; l < L; ++l)
-    for (uint32_t j = 0; j < npl; ++j, ++n)
-      if (g.probs[n] > 0) {
-        rank[n] = static_cast<uint32_t>(img.rankNode.size());
-        img.rankNode.push_back(n);
-        img.rankLayer.push_back(l);
-        img.rankProb.push_back(g.probs[n]);
-      }
-  img.numRanks = static_cast<uint32_t>(img.rankNode.size());
-  const uint32_t k = nodesPerQubit(g.level);
-  auto isFrame = [&](uint32_t n) { return (n % npl) % k < 2; };
-
-  // active = p > 0 nodes and all their descendants (children are later)
+  // ranks (p > 0 nodes, node order) and active nodes (ranks and everything
+  // their flows reach; children lie in later layers)
+  uint32_t R = 0;
+  for (uint32_t n = 0; n < N; ++n) R += g.probs[n] > 0;
+  img.numRanks = R;
+  img.rankNode.resize(R);
+  img.rankLayer.resize(R);
+  img.rankProb.resize(R);
   std::vector<uint8_t> active(N, 0);
-  for (uint32_t n = 0; n < N; ++n) {
-    if (g.probs[n] > 0) active[n] = 1;
-    if (!active[n]) continue;
-    const uint64_t a = g.adjacency[n];
-    const uint32_t c0 = static_cast<uint32_t>(a), c1 = static_cast<uint32_t>(a >> 32);
-    if (c0 != kNone) active[c0] = 1;
-    if (c1 != kNone) active[c1] = 1;
-  }
+  for (uint32_t l = 0, n = 0, r = 0; l < L; ++l)
+    for (uint32_t j = 0; j < npl; ++j, ++n) {
+      const double p = g.probs[n];
+      if (p > 0) {
+        img.rankNode[r] = n;
+        img.rankLayer[r] = l;
+        img.rankProb[r] = p;
+        ++r;
+        active[n] = 1;
+      }
+      if (!active[n]) continue;
+      const uint64_t a = g.adjacency[n];
+      const uint32_t c0 = static_cast<uint32_t>(a), c1 = static_cast<uint32_t>(a >> 32);
+      if (c0 != kNone) active[c0] = 1;
+      if (c1 != kNone) active[c1] = 1;
+    }
 
   // seeds: XOR of the detector bits of every measurement on an active node
   std::vector<uint32_t> seedOf(N, kNone);
   std::vector<uint32_t> seedNode;
-  std::vector<uint32_t> seedRows;  // numSeeds x C, row-major (transposed below)
+  std::vector<uint32_t> seedRows;  // numSeeds x C, row-major
+  seedNode.reserve(g.numMeasurements);
+  seedRows.reserve(static_cast<size_t>(g.numMeasurements) * C);
   for (uint32_t m = 0; m < g.numMeasurements; ++m) {
     const uint32_t n = g.keys[m];
     if (!active[n]) continue;
@@ -323,6 +328,7 @@ void buildImage(const Stepg &g, StepgImage &img) {
     for (uint32_t c = 0; c < C; ++c) any |= bits[c] != 0;
     if (!any) continue;
     if (seedOf[n] == kNone) {
+      if (!frameSlot[n % npl]) throw RuntimeError("STEPG: a measurement seeds an auxiliary node");
       seedOf[n] = static_cast<uint32_t>(seedNode.size());
       seedNode.push_back(n);
       seedRows.resize(seedRows.size() + C, 0u);
@@ -331,41 +337,38 @@ void buildImage(const Stepg &g, StepgImage &img) {
     for (uint32_t c = 0; c < C; ++c) row[c] ^= bits[c];
   }
   const uint32_t S = static_cast<uint32_t>(seedNode.size());
-  img.numSeeds = S;
-  img.seedVals.assign(static_cast<size_t>(C) * S, 0u);
   img.colHi.assign(C, -1);
   img.colLow.assign(C, -1);
   for (uint32_t s = 0; s < S; ++s) {
     const int32_t l = static_cast<int32_t>(seedNode[s] / npl);
     const uint32_t *row = seedRows.data() + static_cast<size_t>(s) * C;
     for (uint32_t c = 0; c < C; ++c) {
-      img.seedVals[static_cast<size_t>(c) * S + s] = row[c];
       if (!row[c]) continue;
       if (img.colHi[c] < l) img.colHi[c] = l;
       if (img.colLow[c] < 0 || img.colLow[c] > l) img.colLow[c] = l;
     }
   }
 
-  // The sweep runs over the active frame nodes only (see the file comment);
-  // check the structure it relies on.
-  for (uint32_t n = 0; n < N; ++n) {
-    if (!active[n] || !isFrame(n)) continue;
-    const uint64_t a = g.adjacency[n];
-    for (uint32_t ch : {static_cast<uint32_t>(a), static_cast<uint32_t>(a >> 32)})
-      if (ch != kNone && !isFrame(ch)) throw RuntimeError("STEPG: a frame node flows into an auxiliary node");
-  }
-  for (uint32_t s2 = 0; s2 < S; ++s2)
-    if (!isFrame(seedNode[s2])) throw RuntimeError("STEPG: a measurement seeds an auxiliary node");
-
-  // position of every active frame node within its layer
+  // positions of the active frame nodes in their layers, and the structure
+  // the sweep relies on: frame nodes flow only into frame nodes of the next
+  // layer
   std::vector<uint32_t> actIdx(N, kNone);
   img.layerAct.assign(L + 1, 0);
   uint32_t total = 0, maxAct = 0;
   for (uint32_t l = 0, n = 0; l < L; ++l) {
     img.layerAct[l] = total;
     uint32_t cnt = 0;
-    for (uint32_t j = 0; j < npl; ++j, ++n)
-      if (active[n] && isFrame(n)) actIdx[n] = cnt++;
+    const uint32_t next0 = (l + 1) * npl;
+    for (uint32_t j = 0; j < npl; ++j, ++n) {
+      if (!active[n] || !frameSlot[j]) continue;
+      actIdx[n] = cnt++;
+      const uint64_t a = g.adjacency[n];
+      for (uint32_t ch : {static_cast<uint32_t>(a), static_cast<uint32_t>(a >> 32)}) {
+        if (ch == kNone) continue;
+        if (ch < next0 || ch >= next0 + npl) throw RuntimeError("STEPG: a frame flow skips a layer");
+        if (!frameSlot[ch - next0]) throw RuntimeError("STEPG: a frame node flows into an auxiliary node");
+      }
+    }
     total += cnt;
     maxAct = std::max(maxAct, cnt);
   }
@@ -386,141 +389,143 @@ void buildImage(const Stepg &g, StepgImage &img) {
   img.sweepThreads = threads;
   img.sweepRpt = rpt;
   img.layerStride = P;
-  // The sweep takes layers in pairs (2m + 1, 2m).  A column of S holds a
-  // row of P words per layer for layers 0 .. 2 numPairs - 1, then the rows
-  // of the padding layers below 0 (what the lead blocks write); byte
-  // offsets within a column and row codes (position in kRowPosBits bits)
-  // must fit 32 bits.
   const uint32_t numPairs = (L + 1) / 2;
   img.numPairs = numPairs;
   const uint64_t rowsPerCol = 2ull * (numPairs + kSweepLeadPad);
   if (rowsPerCol * P * 4 >= (1ull << 32) || P > (1u << kRowPosBits) || L >= (1u << (32 - kRowPosBits)))
     throw RuntimeError("circuit too large for the sweep: too many layers x frame nodes");
-  for (uint32_t n = 0; n < N; ++n) {
-    if (!active[n] || !isFrame(n)) continue;
-    const uint64_t a = g.adjacency[n];
-    for (uint32_t ch : {static_cast<uint32_t>(a), static_cast<uint32_t>(a >> 32)})
-      if (ch != kNone && ch / npl != n / npl + 1) throw RuntimeError("STEPG: a frame flow skips a layer");
-  }
 
-  // Seeds, grouped by pair: pair m's list holds the seeds of layers 2m and
-  // 2m + 1, then one combined seed per even-layer node whose own seed and
-  // its children's seeds are two or more (their XOR, column by column);
-  // records name a seed by its index within the pair's list.
-  std::vector<uint32_t> pairFirst(numPairs + 1, 0);
-  std::vector<uint32_t> local(N, 0xFFFFu);      // a seeded node's index in its pair's list
-  std::vector<uint32_t> newRows;                // seeds' column words, row-major
-  std::vector<uint32_t> evenSeed(N, 0xFFFFu);   // even-layer node -> its (combined) seed
-  std::vector<uint32_t> bySeedNode(N, kNone);   // node -> original seed
-  for (uint32_t s2 = 0; s2 < S; ++s2) bySeedNode[seedNode[s2]] = s2;
-  for (uint32_t m = 0; m < numPairs; ++m) {
-    const uint32_t first = static_cast<uint32_t>(newRows.size() / std::max<uint32_t>(C, 1));
-    pairFirst[m] = first;
-    uint32_t cnt = 0;
-    for (uint32_t l = 2 * m; l <= 2 * m + 1 && l < L; ++l)
-      for (uint32_t n = l * npl; n < (l + 1) * npl; ++n) {
-        if (actIdx[n] == kNone || bySeedNode[n] == kNone) continue;
-        local[n] = cnt++;
-        const uint32_t *row = seedRows.data() + static_cast<size_t>(bySeedNode[n]) * C;
-        newRows.insert(newRows.end(), row, row + C);
-      }
-    for (uint32_t n = 2 * m * npl; n < (2 * m + 1) * npl; ++n) {
-      if (actIdx[n] == kNone) continue;
-      uint32_t parts[3], np = 0;
-      auto toggle = [&](uint32_t x) {
-        for (uint32_t i = 0; i < np; ++i)
-          if (parts[i] == x) {
-            parts[i] = parts[--np];
-            return;
-          }
-        parts[np++] = x;
-      };
-      if (local[n] != 0xFFFFu) toggle(n);
-      const uint64_t a = g.adjacency[n];
-      for (uint32_t ch : {static_cast<uint32_t>(a), static_cast<uint32_t>(a >> 32)})
-        if (ch != kNone && local[ch] != 0xFFFFu) toggle(ch);
-      if (np == 1) {
-        evenSeed[n] = local[parts[0]];
-      } else if (np > 1) {
-        evenSeed[n] = cnt++;
-        newRows.resize(newRows.size() + C, 0u);
-        uint32_t *row = newRows.data() + newRows.size() - C;
-        for (uint32_t i = 0; i < np; ++i) {
-          const uint32_t *src = seedRows.data() + static_cast<size_t>(bySeedNode[parts[i]]) * C;
-          for (uint32_t c = 0; c < C; ++c) row[c] ^= src[c];
-        }
-      }
-    }
-    if (cnt >= 0xFFFFu) throw RuntimeError("circuit too large for the sweep: too many seeds in two layers");
-  }
-  const uint32_t S2 = C ? static_cast<uint32_t>(newRows.size() / C) : 0u;
-  pairFirst[numPairs] = S2;
-  img.numSeeds = S2;
-  img.seedVals.assign(static_cast<size_t>(C) * S2, 0u);
-  for (uint32_t s2 = 0; s2 < S2; ++s2)
-    for (uint32_t c = 0; c < C; ++c) img.seedVals[static_cast<size_t>(c) * S2 + s2] = newRows[static_cast<size_t>(s2) * C + c];
-
-  // Record blocks: kSweepLeadPad padding blocks, then one per pair; a block
-  // is P records + one header {first seed of the pair, 0, 0, 0}.  Record
-  // position j of pair m: {odd node's two children, even node's grandchildren
-  // 0-1, grandchildren 2-3 (16-bit LDS word indices), odd seed | even seed
-  // << 16 (indices in the pair's seed list, 0xFFFF = none)}.  Layer 2m + 2
-  // lives in ring slot (m + 1) mod 2 (words [slot P, slot P + P)); absent
-  // children read the zero word 2P.
+  // records and seeds, pair by pair
   const uint32_t zeroW = 2u * P;
-  img.act.assign(static_cast<size_t>(kSweepLeadPad + numPairs) * (P + 1) * 4, 0u);
-  for (uint32_t blk = 0; blk < kSweepLeadPad + numPairs; ++blk) {
-    uint32_t *rec = img.act.data() + static_cast<size_t>(blk) * (P + 1) * 4;
-    for (uint32_t j = 0; j < P; ++j) {
-      rec[4 * j] = rec[4 * j + 1] = rec[4 * j + 2] = zeroW | (zeroW << 16);
-      rec[4 * j + 3] = 0xFFFFFFFFu;
+  img.act.resize(static_cast<size_t>(kSweepLeadPad + numPairs) * (P + 1) * 4);
+  {
+    uint32_t *rec = img.act.data();
+    for (size_t j = 0, e = static_cast<size_t>(kSweepLeadPad + numPairs) * (P + 1); j < e; ++j, rec += 4) {
+      rec[0] = rec[1] = rec[2] = zeroW | (zeroW << 16);
+      rec[3] = 0xFFFFFFFFu;
     }
-    if (blk < kSweepLeadPad) continue;
-    const uint32_t m = blk - kSweepLeadPad;
-    rec[4 * P] = pairFirst[m];
-    auto word = [&](uint32_t ch) { return ch == kNone ? zeroW : ((m + 1) & 1u) * P + actIdx[ch]; };
-    if (2 * m + 1 < L)
-      for (uint32_t n = (2 * m + 1) * npl; n < (2 * m + 2) * npl; ++n) {
+    for (uint32_t blk = 0; blk < kSweepLeadPad; ++blk) {
+      uint32_t *h = img.act.data() + (static_cast<size_t>(blk) * (P + 1) + P) * 4;
+      h[0] = h[1] = h[2] = h[3] = 0u;
+    }
+  }
+  std::vector<uint32_t> newRows;  // seeds' column words, row-major, grouped by pair
+  newRows.reserve(seedRows.size() + static_cast<size_t>(C) * 64);
+  std::vector<uint32_t> localOdd(npl), localEven(npl);  // a seeded node's index in its pair's list
+  for (uint32_t m = 0; m < numPairs; ++m) {
+    const uint32_t first = C ? static_cast<uint32_t>(newRows.size() / C) : 0u;
+    const uint32_t le = 2 * m, lo = 2 * m + 1;
+    const uint32_t e0 = le * npl, o0 = lo * npl, n0 = (lo + 1) * npl;  // first nodes of layers 2m, 2m+1, 2m+2
+    uint32_t cnt = 0;
+    // the seeds of the two layers, in node order
+    for (uint32_t j = 0; j < npl; ++j) {
+      const uint32_t n = e0 + j;
+      localEven[j] = 0xFFFFu;
+      if (actIdx[n] == kNone || seedOf[n] == kNone) continue;
+      localEven[j] = cnt++;
+      const uint32_t *row = seedRows.data() + static_cast<size_t>(seedOf[n]) * C;
+      newRows.insert(newRows.end(), row, row + C);
+    }
+    for (uint32_t j = 0; j < npl; ++j) {
+      localOdd[j] = 0xFFFFu;
+      if (lo >= L) continue;
+      const uint32_t n = o0 + j;
+      if (actIdx[n] == kNone || seedOf[n] == kNone) continue;
+      localOdd[j] = cnt++;
+      const uint32_t *row = seedRows.data() + static_cast<size_t>(seedOf[n]) * C;
+      newRows.insert(newRows.end(), row, row + C);
+    }
+    uint32_t *rec = img.act.data() + static_cast<size_t>(kSweepLeadPad + m) * (P + 1) * 4;
+    rec[4 * P] = first;
+    rec[4 * P + 1] = rec[4 * P + 2] = rec[4 * P + 3] = 0u;
+    const uint32_t slotBase = ((m + 1) & 1u) * P;
+    auto word = [&](uint32_t ch) { return ch == kNone ? zeroW : slotBase + actIdx[ch]; };
+    // odd layer: children in layer 2m + 2
+    if (lo < L)
+      for (uint32_t j = 0; j < npl; ++j) {
+        const uint32_t n = o0 + j;
         if (actIdx[n] == kNone) continue;
         uint32_t *r = rec + 4 * actIdx[n];
         const uint64_t a = g.adjacency[n];
         r[0] = word(static_cast<uint32_t>(a)) | (word(static_cast<uint32_t>(a >> 32)) << 16);
-        r[3] = (r[3] & 0xFFFF0000u) | local[n];
+        r[3] = (r[3] & 0xFFFF0000u) | localOdd[j];
       }
-    for (uint32_t n = 2 * m * npl; n < (2 * m + 1) * npl; ++n) {
+    // even layer: grandchildren in layer 2m + 2 and the combined seed
+    for (uint32_t j = 0; j < npl; ++j) {
+      const uint32_t n = e0 + j;
       if (actIdx[n] == kNone) continue;
-      uint32_t gc[4], ng = 0;
-      auto toggle = [&](uint32_t x) {
-        for (uint32_t i = 0; i < ng; ++i)
-          if (gc[i] == x) {
-            gc[i] = gc[--ng];
-            return;
-          }
-        if (ng == 4) throw RuntimeError("STEPG: a node reaches more than four nodes two layers on");
-        gc[ng++] = x;
-      };
       const uint64_t a = g.adjacency[n];
-      for (uint32_t ch : {static_cast<uint32_t>(a), static_cast<uint32_t>(a >> 32)}) {
-        if (ch == kNone) continue;
-        const uint64_t b2 = g.adjacency[ch];
+      const uint32_t ch[2] = {static_cast<uint32_t>(a), static_cast<uint32_t>(a >> 32)};
+      uint32_t gc[4], ng = 0;
+      uint32_t parts[3], np = 0;  // seeded nodes among n and its children (counted mod 2)
+      auto toggle = [](uint32_t *set, uint32_t &cnt, uint32_t x, uint32_t cap) {
+        for (uint32_t i = 0; i < cnt; ++i)
+          if (set[i] == x) {
+            set[i] = set[--cnt];
+            return true;
+          }
+        if (cnt == cap) return false;
+        set[cnt++] = x;
+        return true;
+      };
+      if (localEven[j] != 0xFFFFu) toggle(parts, np, n, 3);
+      for (uint32_t c : ch) {
+        if (c == kNone) continue;
+        if (localOdd[c - o0] != 0xFFFFu) toggle(parts, np, c, 3);
+        const uint64_t b2 = g.adjacency[c];
         for (uint32_t gch : {static_cast<uint32_t>(b2), static_cast<uint32_t>(b2 >> 32)})
-          if (gch != kNone) toggle(gch);
+          if (gch != kNone && !toggle(gc, ng, gch, 4))
+            throw RuntimeError("STEPG: a node reaches more than four nodes two layers on");
       }
+      (void)n0;
       uint32_t w[4] = {zeroW, zeroW, zeroW, zeroW};
       for (uint32_t i = 0; i < ng; ++i) w[i] = word(gc[i]);
       uint32_t *r = rec + 4 * actIdx[n];
       r[1] = w[0] | (w[1] << 16);
       r[2] = w[2] | (w[3] << 16);
-      r[3] = (r[3] & 0x0000FFFFu) | (evenSeed[n] << 16);
+      uint32_t es = 0xFFFFu;
+      if (np == 1) {
+        es = parts[0] < o0 ? localEven[parts[0] - e0] : localOdd[parts[0] - o0];
+      } else if (np > 1) {
+        es = cnt++;
+        newRows.resize(newRows.size() + C, 0u);
+        uint32_t *row = newRows.data() + newRows.size() - C;
+        for (uint32_t i = 0; i < np; ++i) {
+          const uint32_t *src = seedRows.data() + static_cast<size_t>(seedOf[parts[i]]) * C;
+          for (uint32_t c = 0; c < C; ++c) row[c] ^= src[c];
+        }
+      }
+      r[3] = (r[3] & 0x0000FFFFu) | (es << 16);
     }
+    if (cnt >= 0xFFFFu) throw RuntimeError("circuit too large for the sweep: too many seeds in two layers");
   }
+  const uint32_t S2 = C ? static_cast<uint32_t>(newRows.size() / C) : 0u;
+  img.numSeeds = S2;
+  img.seedVals.resize(static_cast<size_t>(C) * S2);
+  for (uint32_t s2 = 0; s2 < S2; ++s2)
+    for (uint32_t c = 0; c < C; ++c)
+      img.seedVals[static_cast<size_t>(c) * S2 + s2] = newRows[static_cast<size_t>(s2) * C + c];
 
-  // each rank's frame rows: itself, or the frame nodes its flows reach
-  // (through at most one auxiliary node), each counted modulo 2
-  img.rankRows.assign(static_cast<size_t>(img.numRanks) * 4, kNone);
-  for (uint32_t r = 0; r < img.numRanks; ++r) {
-    uint32_t frames[8];
-    uint32_t nf = 0;
+  // each rank's frame rows (itself, or the frame nodes its flows reach
+  // through at most one auxiliary node, each counted modulo 2), encoded
+  // compactly (see kSpillBase); and the probability table
+  img.rankCode.resize(2ull * R);
+  img.rankSpill.clear();
+  img.probTab.clear();
+  img.probIdx.clear();
+  const bool useTable = probTable();
+  if (useTable) img.probIdx.resize(R);
+  std::unordered_map<uint64_t, uint32_t> seen;
+  // a direct-mapped cache of recent values in front of the map (circuits
+  // use few distinct rates, in repeating patterns)
+  constexpr uint32_t kCache = 64;
+  uint64_t cacheBits[kCache];
+  uint32_t cacheIdx[kCache];
+  for (uint32_t i = 0; i < kCache; ++i) cacheIdx[i] = kNone;
+  bool tableOk = useTable;
+  for (uint32_t r = 0; r < R; ++r) {
+    const uint32_t n = img.rankNode[r], rl = img.rankLayer[r];
+    uint32_t frames[8], nf = 0;
     auto toggle = [&](uint32_t f) {
       for (uint32_t i = 0; i < nf; ++i)
         if (frames[i] == f) {
@@ -530,33 +535,35 @@ void buildImage(const Stepg &g, StepgImage &img) {
       if (nf == 8) throw RuntimeError("STEPG: an auxiliary node reaches too many frame nodes");
       frames[nf++] = f;
     };
-    auto expand = [&](auto &self, uint32_t n, int depth) -> void {
-      if (isFrame(n)) {
-        toggle(n);
-        return;
-      }
-      if (depth == 2) throw RuntimeError("STEPG: auxiliary nodes nested more than two deep");
+    if (frameSlot[n - rl * npl]) {
+      frames[nf++] = n;
+    } else {
       const uint64_t a = g.adjacency[n];
-      for (uint32_t ch : {static_cast<uint32_t>(a), static_cast<uint32_t>(a >> 32)})
-        if (ch != kNone) self(self, ch, depth + 1);
-    };
-    expand(expand, img.rankNode[r], 0);
+      for (uint32_t c : {static_cast<uint32_t>(a), static_cast<uint32_t>(a >> 32)}) {
+        if (c == kNone) continue;
+        if (frameSlot[c % npl]) {
+          toggle(c);
+          continue;
+        }
+        const uint64_t b2 = g.adjacency[c];
+        for (uint32_t gch : {static_cast<uint32_t>(b2), static_cast<uint32_t>(b2 >> 32)}) {
+          if (gch == kNone) continue;
+          if (!frameSlot[gch % npl]) throw RuntimeError("STEPG: auxiliary nodes nested more than two deep");
+          toggle(gch);
+        }
+      }
+    }
     if (nf > 4) throw RuntimeError("STEPG: an auxiliary node reaches more than four frame nodes");
-    std::sort(frames, frames + nf);
-    const uint32_t rl = img.rankNode[r] / npl;
+    for (uint32_t i = 1; i < nf; ++i)  // insertion sort (at most four)
+      for (uint32_t q = i; q > 0 && frames[q - 1] > frames[q]; --q) std::swap(frames[q - 1], frames[q]);
+    uint32_t rows[4] = {kNone, kNone, kNone, kNone};
     for (uint32_t i = 0; i < nf; ++i) {
-      const uint32_t fl = frames[i] / npl;
+      const uint32_t fl = frames[i] == n ? rl : frames[i] / npl;
       if (fl < rl) throw RuntimeError("STEPG: a flow runs backwards in time");
       img.rowSpan = std::max(img.rowSpan, fl - rl);
-      img.rankRows[4 * r + i] = (fl << kRowPosBits) | actIdx[frames[i]];
+      rows[i] = (fl << kRowPosBits) | actIdx[frames[i]];
     }
-  }
-
-  // compact codes (see kSpillBase), spilling what does not fit
-  img.rankCode.assign(2ull * img.numRanks, 0u);
-  img.rankSpill.clear();
-  for (uint32_t r = 0; r < img.numRanks; ++r) {
-    const uint32_t *rows = img.rankRows.data() + 4ull * r;
+    // compact code, or a spill entry
     const uint32_t base = rows[0] == kNone ? 0u : rows[0] >> kRowPosBits;
     uint32_t c16[3] = {kRowNone16, kRowNone16, kRowNone16};
     bool fits = rows[3] == kNone && base < kSpillBase;
@@ -576,39 +583,34 @@ void buildImage(const Stepg &g, StepgImage &img) {
       img.rankCode[2ull * r] = (kSpillBase << 16) | static_cast<uint32_t>(idx >> 32);
       img.rankCode[2ull * r + 1] = static_cast<uint32_t>(idx);
     }
-  }
-
-  // probability table when the distinct values are few (exact: the table
-  // holds the doubles themselves)
-  img.probTab.clear();
-  img.probIdx.assign(img.numRanks, 0);
-  if (!probTable()) {
-    img.probIdx.clear();
-  } else {
-    std::unordered_map<uint64_t, uint32_t> seen;
-    uint64_t lastBits = 0;
-    uint32_t lastIdx = kNone;  // the previous rank's value (runs are common)
-    for (uint32_t r = 0; r < img.numRanks; ++r) {
+    // probability table (exact: it holds the doubles themselves)
+    if (tableOk) {
       uint64_t bits;
       std::memcpy(&bits, &img.rankProb[r], 8);
-      if (lastIdx != kNone && bits == lastBits) {
-        img.probIdx[r] = static_cast<uint16_t>(lastIdx);
-        continue;
-      }
-      auto it = seen.find(bits);
-      if (it == seen.end()) {
-        if (seen.size() == kProbDictMax) {
-          img.probTab.clear();
-          img.probIdx.clear();
-          break;
+      const uint32_t slot = static_cast<uint32_t>((bits * 0x9E3779B97F4A7C15ull) >> 58);
+      if (cacheIdx[slot] != kNone && cacheBits[slot] == bits) {
+        img.probIdx[r] = static_cast<uint16_t>(cacheIdx[slot]);
+      } else {
+        auto it = seen.find(bits);
+        if (it == seen.end()) {
+          if (seen.size() == kProbDictMax) {
+            tableOk = false;
+          } else {
+            it = seen.emplace(bits, static_cast<uint32_t>(img.probTab.size())).first;
+            img.probTab.push_back(img.rankProb[r]);
+          }
         }
-        it = seen.emplace(bits, static_cast<uint32_t>(img.probTab.size())).first;
-        img.probTab.push_back(img.rankProb[r]);
+        if (tableOk) {
+          img.probIdx[r] = static_cast<uint16_t>(it->second);
+          cacheBits[slot] = bits;
+          cacheIdx[slot] = it->second;
+        }
       }
-      img.probIdx[r] = static_cast<uint16_t>(it->second);
-      lastBits = bits;
-      lastIdx = it->second;
     }
+  }
+  if (!tableOk) {
+    img.probTab.clear();
+    img.probIdx.clear();
   }
 }
 

@@ -19,7 +19,7 @@
 //    into an auxiliary Y node that does) but never receive a flow from one.
 //    An auxiliary node's sensitivity is therefore the XOR of at most four
 //    frame nodes' sensitivities, which the later stages form on the fly from
-//    its `rankRows`; the sequential sweep touches frame nodes only;
+//    its frame row codes; the sequential sweep touches frame nodes only;
 //  * the sweep takes layers in pairs (2m + 1, 2m), one step per pair: frame
 //    flows always go from layer l to l + 1, so a node of the odd layer 2m + 1
 //    is the XOR of its seed and its two children in layer 2m + 2, and a node
@@ -44,7 +44,7 @@
 //    band in which the column can be non-zero;
 //  * the p > 0 nodes ("ranks", ascending node order) are the items that are
 //    hashed, sorted and merged; each lists the frame rows whose XOR is its
-//    sensitivity (rankRows).
+//    sensitivity (rankCode).
 #pragma once
 
 #include <cstdint>
@@ -62,7 +62,7 @@ constexpr uint32_t kSweepMaxThreads = 1024;
 // prefetches load for the pairs below 0 that its trips run into; a column of
 // S has rows for their layers after its 2 numPairs layers
 constexpr uint32_t kSweepLeadPad = 7;
-// frame row codes in rankRows: layer << kRowPosBits | position in the layer
+// frame row codes: layer << kRowPosBits | position in the layer
 constexpr uint32_t kRowPosBits = 14;
 // Compact frame rows (rankCode, two words per rank, what the device reads):
 // x = base << 16 | row 0, y = row 2 << 16 | row 1, each row (layer - base)
@@ -70,7 +70,7 @@ constexpr uint32_t kRowPosBits = 14;
 // layer.  A rank with four rows, a base >= kSpillBase or a row that would
 // read as kRowNone16 is spilled instead: x = kSpillBase << 16 | (index >>
 // 32), y = the low 32 bits of index, and rankSpill[4 index .. 4 index + 3]
-// holds its rankRows codes.
+// holds its four frame row codes (kNone-padded).
 constexpr uint32_t kRowNone16 = 0xFFFFu, kSpillBase = 0xFFFFu;
 // probabilities are sent as 16-bit indices into a per-circuit table when a
 // circuit has at most this many distinct values (else as doubles)
@@ -113,7 +113,7 @@ struct StepgImage {
   // sweep shape: record j*sweepThreads + t of a layer belongs to thread t
   // (j < sweepRpt); layerStride = sweepThreads * sweepRpt >= maxActive
   uint32_t sweepThreads = 64, sweepRpt = 1, layerStride = 64;
-  // largest (frame row layer - rank layer) over all rankRows entries
+  // largest (frame row layer - rank layer) over all ranks' frame rows
   uint32_t rowSpan = 0;
   uint32_t numPairs = 0;           // (numLayers + 1) / 2 sweep steps
   std::vector<uint32_t> layerAct;  // numLayers + 1 offsets of each layer's first active node
@@ -124,9 +124,8 @@ struct StepgImage {
   std::vector<uint32_t> rankLayer; // numRanks (host only)
   std::vector<double> rankProb;    // numRanks (sent when probTab is empty)
   std::vector<uint32_t> rankNode;  // numRanks (host only: node index of each rank)
-  std::vector<uint32_t> rankRows;  // 4 * numRanks: frame row codes, kNone-padded (host only)
   std::vector<uint32_t> rankCode;  // 2 * numRanks: compact frame rows
-  std::vector<uint32_t> rankSpill; // 4 per spilled rank: its rankRows codes
+  std::vector<uint32_t> rankSpill; // 4 per spilled rank: its frame row codes
   std::vector<double> probTab;     // distinct probabilities (empty: too many)
   std::vector<uint16_t> probIdx;   // numRanks indices into probTab
 };
