@@ -43,6 +43,26 @@ constexpr PairRule kPairRules[] = {
     {3, 2, 2, 0, kYZZY, 2, 0, kZ, 0, 1, kY, -1, "ZY"},
 };
 
+// kPairRules index of each (pa, pb) with both non-identity, at 4 * pa + pb.
+struct PairRuleIndex {
+  uint8_t at[16] = {};
+  constexpr PairRuleIndex() {
+    for (uint8_t i = 0; i < sizeof(kPairRules) / sizeof(kPairRules[0]); ++i)
+      at[4 * kPairRules[i].pa + kPairRules[i].pb] = i;
+  }
+};
+constexpr PairRuleIndex kPairRuleIndex{};
+constexpr uint32_t pairRuleOf(int pa, int pb) { return kPairRuleIndex.at[4 * pa + pb]; }
+constexpr bool pairRuleIndexComplete() {
+  for (int pa = 1; pa < 4; ++pa)
+    for (int pb = 1; pb < 4; ++pb) {
+      const PairRule &r = kPairRules[pairRuleOf(pa, pb)];
+      if (r.pa != pa || r.pb != pb) return false;
+    }
+  return true;
+}
+static_assert(pairRuleIndexComplete(), "a two-qubit Pauli without its rule");
+
 struct Builder {
   Stepg &g;
   bool record;
@@ -114,8 +134,8 @@ struct Builder {
   void pair(int pa, int pb, uint32_t q0, uint32_t q1, uint32_t l, double p, uint32_t loc) {
     if (pa == 0) return single(pb, q1, l, p, loc, 2);
     if (pb == 0) return single(pa, q0, l, p, loc, 2);
-    for (const PairRule &r : kPairRules) {
-      if (r.pa != pa || r.pb != pb) continue;
+    {
+      const PairRule &r = kPairRules[pairRuleOf(pa, pb)];
       if (g.level < r.minLevel) return;
       if (l < r.back)
         throw RuntimeError(std::string("Mixer: cannot apply ") + r.name + " error in layer l < " +
@@ -169,6 +189,23 @@ void buildStepg(const Circuit &c, int level, Stepg &g, const double *theta, bool
     }
 
   uint32_t layer = 0, meas = 0, det = 0, loc = 0;
+  // per-component rates of DEPOLARIZE1/2 (Mixer.hpp:164, 184), memoized on
+  // the channel rate (circuits repeat few rates)
+  double last1 = -1, last1s = 0, last2 = -1, last2s = 0;
+  auto split1 = [&](double p) {
+    if (p != last1) {
+      last1 = p;
+      last1s = 0.5 - 0.5 * std::sqrt(1 - (4 * p) / 3);
+    }
+    return last1s;
+  };
+  auto split2 = [&](double p) {
+    if (p != last2) {
+      last2 = p;
+      last2s = 0.5 - 0.5 * std::pow(1 - (16 * p) / 15, 0.125);
+    }
+    return last2s;
+  };
   auto rate = [&](double p) {
     if (!theta) return p;
     return theta[loc];
@@ -225,7 +262,7 @@ void buildStepg(const Circuit &c, int level, Stepg &g, const double *theta, bool
       for (uint32_t j = 0; j < o.nt; ++j) {
         const double p = rate(o.arg);
         if (!(p >= 0 && p <= 0.75)) throw RuntimeError("Mixer: invalid DEP1 p > 0.75.");
-        const double s = 0.5 - 0.5 * std::sqrt(1 - (4 * p) / 3);
+        const double s = split1(p);
         b.single(1, t[j], layer, s, loc, 1);
         b.single(3, t[j], layer, s, loc, 1);
         b.single(2, t[j], layer, s, loc, 1);
@@ -236,7 +273,7 @@ void buildStepg(const Circuit &c, int level, Stepg &g, const double *theta, bool
       for (uint32_t j = 0; j < o.nt; j += 2) {
         const double p = rate(o.arg);
         if (!(p >= 0 && p <= 0.9375)) throw RuntimeError("Mixer: invalid DEP2 p > 0.9375.");
-        const double s = 0.5 - 0.5 * std::pow(1 - (16 * p) / 15, 0.125);
+        const double s = split2(p);
         for (int pa = 0; pa < 4; ++pa)
           for (int pb = 0; pb < 4; ++pb)
             if (pa || pb) b.pair(pa, pb, t[j], t[j + 1], layer, s, loc);
