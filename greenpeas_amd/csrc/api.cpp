@@ -482,6 +482,10 @@ int gp_batch_compile(gp_batch *batch, int mode) {
       double total = 0;
       for (uint32_t i = 0; i < n; ++i) {
         batch->single->load({&batch->circuits[i]->image});
+        // one untimed compile first, as the batched timing follows earlier
+        // runs of its load (a load's first run sizes the target rows)
+        batch->single->run(false);
+        batch->single->download();
         total += timeGraph(*batch->single);
         batch->single->download();
         gpa::VariantResult r;

@@ -685,8 +685,8 @@ __device__ __forceinline__ void forEachTargetScan(uint32_t numCols, const int2 *
 // representative's row instead of gathering them from S one column at a
 // time: each wave assembles its 64 rows in LDS and writes them out
 // contiguously.
-constexpr uint32_t kStashWords = 16;                  // 32-bit words per row
-constexpr uint32_t kStashTargets = 2 * kStashWords;   // 16-bit targets per row
+constexpr uint32_t kStashWords = 16;                  // 32-bit words per row, at most
+constexpr uint32_t kStashTargets = 2 * kStashWords;   // 16-bit targets per row, at most
 constexpr uint32_t kStashMaxCols = 65536 / 32;
 constexpr uint32_t kBucketCap = kBucketItemsPerWave;
 
@@ -700,6 +700,7 @@ struct ClassifyOut {
   uint32_t *flags;
   uint32_t *stash;
   uint32_t stashMaxCols;  // circuits with more columns get no rows (0: none do)
+  uint32_t stashShift;    // a row holds 1 << stashShift words (2..4)
 };
 
 // Ranks' inputs: compact frame rows, spill entries, probabilities.
@@ -740,7 +741,8 @@ __global__ __launch_bounds__(kClassBlock) void classifyKernel(
     buildMaskTable(maskTab, l0, nl, d.rowSpan, d.numCols, d.maskWords, band + d.colOff);
   else
     __syncthreads();
-  uint16_t *row = reinterpret_cast<uint16_t *>(rows[wave] + lane * (kStashWords + 1));
+  const uint32_t W = 1u << o.stashShift;  // words per row (the LDS rows are padded to W + 1)
+  uint16_t *row = reinterpret_cast<uint16_t *>(rows[wave] + lane * (W + 1));
   const bool stashed = d.numCols <= o.stashMaxCols;
   if (i < d.numRanks) {
     uint64_t h = kFnvBasis, h2 = 0x84222325cbf29ce4ull;
@@ -749,7 +751,7 @@ __global__ __launch_bounds__(kClassBlock) void classifyKernel(
     auto take = [&](uint32_t tg) {
       h = (h ^ tg) * kFnvPrime;
       h2 = mix2(h2, tg);
-      if (cnt < kStashTargets) row[cnt] = static_cast<uint16_t>(tg);
+      if (cnt < 2 * W) row[cnt] = static_cast<uint16_t>(tg);
       cnt++;
     };
     if constexpr (TABLE)
@@ -780,12 +782,13 @@ __global__ __launch_bounds__(kClassBlock) void classifyKernel(
   const uint32_t waveFirst = t.y + wave * 64;
   if (stashed && waveFirst < d.numRanks) {
     const uint32_t nRows = d.numRanks - waveFirst < 64 ? d.numRanks - waveFirst : 64;
-    uint32_t *dst = o.stash + (d.rankOff + waveFirst) * kStashWords;
+    uint32_t *dst = o.stash + (d.rankOff + waveFirst) * W;
 #pragma unroll
     for (uint32_t k = 0; k < kStashWords; ++k) {
-      const uint32_t e = k * 64 + lane;  // element of the 64 x kStashWords block
-      const uint32_t r = e / kStashWords, c = e % kStashWords;
-      if (r < nRows) dst[e] = rows[wave][r * (kStashWords + 1) + c];
+      if (k >= W) break;
+      const uint32_t e = k * 64 + lane;  // element of the 64 x W block
+      const uint32_t r = e >> o.stashShift, c = e & (W - 1);
+      if (r < nRows) dst[e] = rows[wave][r * (W + 1) + c];
     }
   }
 }
@@ -977,6 +980,7 @@ struct EmitIn {
   const uint2 *rankCode;
   const uint4 *spill;
   uint32_t stashMaxCols;
+  uint32_t stashShift;  // row words = 1 << stashShift
   const int2 *band;
 };
 struct EmitOut {
@@ -1068,12 +1072,14 @@ __global__ __launch_bounds__(256) void emitKernel(const VDesc *__restrict__ desc
     out.member[c] = mb + in.runMember[bs + r];
     // class size: the gap to the next run's target offset
     const uint32_t n = (r + 1 < nR ? in.runTgtPre[bs + r + 1] : in.runNnz[b]) - in.runTgtPre[bs + r];
-    if (n <= kStashTargets && d.numCols <= in.stashMaxCols) {
-      // the representative's stashed row (one 64-byte line, 16-bit targets)
-      const uint4 *src = reinterpret_cast<const uint4 *>(in.stash + static_cast<uint64_t>(rk) * kStashWords);
-      uint32_t w[kStashWords];
+    const uint32_t W = 1u << in.stashShift;
+    if (n <= 2 * W && d.numCols <= in.stashMaxCols) {
+      // the representative's stashed row (4 W bytes, 16-bit targets)
+      const uint4 *src = reinterpret_cast<const uint4 *>(in.stash + static_cast<uint64_t>(rk) * W);
+      uint32_t w[kStashWords] = {};
 #pragma unroll
       for (uint32_t q = 0; q < kStashWords / 4; ++q) {
+        if (4 * q >= W) break;
         const uint4 x = src[q];
         w[4 * q] = x.x;
         w[4 * q + 1] = x.y;
@@ -1424,6 +1430,9 @@ struct Pipeline::Impl {
   bool narrowTargets = false;  // 16-bit targets allowed (setNarrowTargets)
   bool targets16 = false;      // this load's targets are 16-bit
   bool sizesKnown = false;  // lastC / lastT hold the DEM size of this load's circuits
+  // target rows hold 1 << stashShift words: the widest until a run of this
+  // load has shown its largest class, then the narrowest that holds it
+  uint32_t stashShift = 4;
   uint64_t lastC = 0, lastT = 0;
   std::shared_ptr<ResultPool> pool = std::make_shared<ResultPool>();
   HostBuf hVB, hFlags;
@@ -1500,6 +1509,7 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   GPA_HIP(hipStreamSynchronize(m.stream));
   GPA_HIP(hipStreamSynchronize(m.side));
   m.sizesKnown = false;
+  m.stashShift = 4;
   const bool frozen = m.frozen;
   const uint32_t V = static_cast<uint32_t>(imgs.size());
   if (V == 0) throw InvalidArgument("no circuits to compile");
@@ -1796,7 +1806,7 @@ void Pipeline::run(bool timed) {
     ClassifyOut co{m.info.as<uint4>(), m.bucketCount(),
                    m.countStride, m.pslot.as<uint4>(), m.ovKey.as<uint64_t>(),
                    m.ovRank.as<uint32_t>(), m.ovBucket.as<uint32_t>(), m.flags(), m.stash.as<uint32_t>(),
-                   stashMaxCols};
+                   stashMaxCols, m.stashShift};
     if (m.classTable && gMaskTable.load())
       classifyKernel<true><<<m.nTiles, kClassBlock, 0, st>>>(desc, tiles, m.band.as<int2>(), m.S.as<uint32_t>(),
                                                              rin, co);
@@ -1833,7 +1843,7 @@ void Pipeline::run(bool timed) {
              m.tgtStart.as<uint32_t>(), m.memberStart.as<uint32_t>(), m.runRank.as<uint32_t>(),
              m.runTgtPre.as<uint32_t>(), m.runMember.as<uint32_t>(), m.runP.as<double>(),
              m.runHash.as<uint64_t>(), m.runNnz.as<uint32_t>(), m.S.as<uint32_t>(),
-             m.stash.as<uint32_t>(), rin.code, rin.spill, stashMaxCols, m.band.as<int2>()};
+             m.stash.as<uint32_t>(), rin.code, rin.spill, stashMaxCols, m.stashShift, m.band.as<int2>()};
   EmitOut eout{m.outP.as<double>(), m.outOff.as<uint32_t>(), m.outHash.as<uint64_t>(), m.outMember.as<uint32_t>(),
                m.outTargets.as<uint32_t>(), m.targets16 ? m.outTargets.as<uint16_t>() : nullptr, m.capTargets,
                m.flags(), m.vBounds.as<uint32_t>()};
@@ -1915,6 +1925,18 @@ void Pipeline::download() {
   if (!guessed || C != gC || T != gT) {
     copyArrays(C, T);
     GPA_HIP(hipStreamSynchronize(m.stream));
+  }
+  if (!m.sizesKnown) {
+    // the load's largest class (the same on every run) sets the row width
+    const uint32_t *off = m.res->off.as<uint32_t>();
+    uint32_t most = 0;
+    for (uint64_t c = 0; c < C; ++c) most = std::max(most, off[c + 1] - off[c]);
+    uint32_t shift = 2;
+    while (shift < 4 && (2u << shift) < most) ++shift;
+    if (shift != m.stashShift) {
+      m.stashShift = shift;
+      m.dropGraph();
+    }
   }
   m.lastC = C;
   m.lastT = T;

@@ -284,3 +284,23 @@ def test_dem_views_outlive_later_compiles_and_the_driver():
     assert np.array_equal(hashes_small, copy_small.hashes)
     assert dem_big == get_driver(big).compile_detector_error_model()
     del dem_small, dem_small2, dem_big  # releases the blocks after the driver
+
+
+@pytest.mark.parametrize("name", ["surface5", "bb12"])
+def test_repeated_compiles_with_narrowed_target_rows(oracle, name):
+    """After a load's first compile the stored target rows shrink to the
+    narrowest width that holds its largest class (surface5: 4 words;
+    bb12's 18-target classes keep 16); every compile gives the same DEM and
+    order as the STEPG restatement."""
+    text = read_circuit(name)
+    ref = oracle.compile_stepg(text, 2)
+    drv = get_driver(text)
+    for _ in range(3):
+        dem = drv.compile_detector_error_model()
+        assert np.array_equal(drv.last_hashes(), ref.hashes)
+        assert np.array_equal(dem.offsets, ref.offsets)
+        assert np.array_equal(dem.targets, ref.targets)
+        assert np.max(np.abs(dem.probs - ref.probs) / ref.probs) <= REL
+    drv.bench_device(3)  # graph replays with the narrowed rows
+    again = drv.compile_detector_error_model()
+    assert np.array_equal(again.targets, ref.targets)
