@@ -257,6 +257,17 @@ def bench_batch(args) -> int:
     log(f"host parse + STEPG build: {time.perf_counter() - t0:.2f} s")
     batched = [bc.run() for _ in range(max(args.warmup, 1) + 3)][-3:]
     log(f"batched device us: {batched}")
+    # stage times of the batched pipeline and the sort/merge stages' share of
+    # HBM peak at this size (byte model as for the plain workload)
+    runs = [bc.stage_us() for _ in range(3)]
+    stages, items = min(runs, key=lambda r: r[0]["buckets"] + r[0]["emit"])
+    shapes = [bc.shape(i) for i in range(len(texts))]
+    C = sum(s[0] for s in shapes)
+    nnz_bar = sum(s[1] for s in shapes) / C if C else 0.0
+    model = algorithmic_bytes(0, 0, items, C, nnz_bar)
+    sm_us = stages["buckets"] + stages["emit"]
+    sm_gbs = (model["sort"] + model["reduce_emit"]) / (sm_us * 1e-6) / 1e9
+    log(f"batched stages us: {stages}; sort/merge {sm_gbs:.0f} GB/s over {items} items")
     one = bc.run(one_by_one=True)
     log(f"one-by-one device us: {one:.0f}")
     b = min(batched)
@@ -265,6 +276,10 @@ def bench_batch(args) -> int:
         "batched_device_us": b, "one_by_one_device_us": one, "speedup": one / b,
         "batched_error_locations_per_s": locs / (b * 1e-6),
         "device_bytes": bc.device_bytes(),
+        "stage_us": stages,
+        "sort_merge": {"us": sm_us, "items": items, "dem_errors": C,
+                       "algorithmic_bytes": model["sort"] + model["reduce_emit"],
+                       "algorithmic_GBs": sm_gbs, "frac_hbm": sm_gbs / HBM_PEAK_GBS},
         "config": "rotated surface memories, d in {5..15} odd, rounds in [d, 2d], per-instruction "
                   "rates logU[1e-4, 1e-2], 5% of mid-circuit measurements followed by a "
                   "reset-conditioned branch, seed 1234",

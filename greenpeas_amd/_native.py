@@ -26,7 +26,7 @@ EXPORTS = (
     "gp_driver_bench_device", "gp_debug_pnodes", "gp_debug_sensitivities", "gp_debug_stepg",
     "gp_build_stepg_host",
     "gp_batch_create", "gp_batch_destroy", "gp_batch_set_circuits", "gp_batch_compile",
-    "gp_batch_device_us", "gp_batch_dem_shape", "gp_batch_dem_copy", "gp_batch_device_bytes",
+    "gp_batch_stage_us", "gp_batch_device_us", "gp_batch_dem_shape", "gp_batch_dem_copy", "gp_batch_device_bytes",
     "gp_estimate_device_bytes", "gp_circuit_summary", "gp_set_tuning", "gp_get_tuning",
     "gp_grad_plan_create", "gp_grad_plan_destroy", "gp_grad_plan_sizes", "gp_grad_plan_dem_structure",
     "gp_grad_plan_device_arrays", "gp_grad_plan_copy_arrays", "gp_xor_merge_forward",
@@ -97,6 +97,7 @@ def load() -> ctypes.CDLL:
     sig("gp_batch_destroy", [P])
     sig("gp_batch_set_circuits", [P, ctypes.POINTER(cs), u32, u32])
     sig("gp_batch_compile", [P, i32])
+    sig("gp_batch_stage_us", [P, P, pu64])
     sig("gp_batch_device_us", [P, pf64])
     sig("gp_batch_dem_shape", [P, u32, pu32, pu64, pu32, pu32])
     sig("gp_batch_dem_copy", [P, u32, P, P, P, P])

@@ -66,6 +66,22 @@ class BatchCompiler:
         return _read_dem(self._lib, self._lib.gp_batch_dem_shape, self._lib.gp_batch_dem_copy, self._h,
                          ctypes.c_uint32(index))
 
+    def shape(self, index: int) -> tuple[int, int]:
+        """(errors, targets) of circuit `index`'s DEM without copying it."""
+        ne, nt = ctypes.c_uint32(), ctypes.c_uint64()
+        _native.check(self._lib.gp_batch_dem_shape(self._h, ctypes.c_uint32(index), ctypes.byref(ne),
+                                                   ctypes.byref(nt), None, None))
+        return ne.value, nt.value
+
+    def stage_us(self) -> tuple[dict, int]:
+        """Runs the compiled batch once more with events between the stages:
+        ({"sweep", "classify", "buckets", "emit"} device microseconds, the
+        number of items the sort stages handled).  Call after compile()/run()."""
+        st = (ctypes.c_double * 4)()
+        items = ctypes.c_uint64()
+        _native.check(self._lib.gp_batch_stage_us(self._h, st, ctypes.byref(items)))
+        return dict(zip(("sweep", "classify", "buckets", "emit"), list(st))), int(items.value)
+
     def device_us(self) -> float:
         us = ctypes.c_double()
         _native.check(self._lib.gp_batch_device_us(self._h, ctypes.byref(us)))

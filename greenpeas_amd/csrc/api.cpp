@@ -499,6 +499,21 @@ int gp_batch_compile(gp_batch *batch, int mode) {
   });
 }
 
+int gp_batch_stage_us(gp_batch *batch, double stage_us[4], uint64_t *sorted_items) {
+  return guarded([&] {
+    if (!batch || !stage_us) throw InvalidArgument("NULL argument");
+    if (!batch->loaded) throw RuntimeError("no batched compile to time (gp_batch_compile mode 0 first)");
+    batch->pipe->run(true);
+    batch->pipe->download();
+    const gpa::StageTimes &t = batch->pipe->times();
+    stage_us[0] = t.sweep;
+    stage_us[1] = t.classify;
+    stage_us[2] = t.buckets;
+    stage_us[3] = t.emit;
+    if (sorted_items) *sorted_items = batch->pipe->sortedItems();
+  });
+}
+
 int gp_batch_device_us(const gp_batch *batch, double *us) {
   return guarded([&] {
     if (!batch || !us) throw InvalidArgument("NULL argument");

@@ -2072,6 +2072,17 @@ uint64_t Pipeline::deviceBytes() const {
   return s;
 }
 
+uint64_t Pipeline::sortedItems() const {
+  const Impl &m = *impl_;
+  GPA_HIP(hipSetDevice(device_));
+  GPA_HIP(hipStreamSynchronize(m.stream));
+  std::vector<uint32_t> cnt(m.B);
+  if (m.B) GPA_HIP(hipMemcpy(cnt.data(), m.itemCount.as<uint32_t>(), 4ull * m.B, hipMemcpyDeviceToHost));
+  uint64_t t = 0;
+  for (uint32_t c : cnt) t += c;
+  return t;
+}
+
 void Pipeline::debugSensitivities(uint32_t v, std::vector<uint32_t> &cols32) const {
   const Impl &m = *impl_;
   GPA_HIP(hipSetDevice(device_));

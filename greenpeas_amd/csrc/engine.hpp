@@ -96,6 +96,9 @@ class Pipeline {
   // the next run).
   void copyHashes(uint32_t v, uint64_t *dst) const;
   const StageTimes &times() const { return times_; }
+  // Items the last run sorted (p > 0 nodes with a non-empty class, over all
+  // circuits; synchronizes).  For measurement.
+  uint64_t sortedItems() const;
   uint64_t deviceBytes() const;
   int device() const { return device_; }
   void *stream() const;

@@ -166,6 +166,12 @@ int gp_batch_set_circuits(gp_batch *batch, const char *const *circuit_texts,
  * batch is unchanged); mode 1: the circuits one after another through the
  * same kernels (the one-by-one baseline). */
 int gp_batch_compile(gp_batch *batch, int mode);
+/* Runs the loaded batch once more with HIP events between the stages and
+ * writes their device times in microseconds: [0] sensitivity sweep,
+ * [1] classes + scatter into sort buckets, [2] bucket sort + segmented
+ * reduce, [3] emit; *sorted_items (may be NULL) = the items that run sorted.
+ * Needs a gp_batch_compile in mode 0 before; the DEMs are unchanged. */
+int gp_batch_stage_us(gp_batch *batch, double stage_us[4], uint64_t *sorted_items);
 /* Device time of the last gp_batch_compile in microseconds. */
 int gp_batch_device_us(const gp_batch *batch, double *us);
 /* Shape / copy of the DEM of circuit `index` (as gp_dem_shape/gp_dem_copy). */

@@ -86,3 +86,25 @@ def test_batch_with_many_buckets_takes_the_scan_path():
     ref12 = solo(text)
     assert dems[0] == ref12 and dems[2] == ref12 and dems[3] == ref12
     assert dems[1] == solo(variants[1])
+
+
+def test_stage_timing_counts_the_sorted_items():
+    """The timed pass of a batch reports positive stage times, leaves the
+    DEMs as they were, and counts as sorted items exactly the p>0 nodes with
+    a non-empty sensitivity (what the sort/merge byte model is priced on)."""
+    texts = [read_circuit("surface3"), read_circuit("bb6"), read_circuit("surface5")]
+    bc = BatchCompiler()
+    bc.set_circuits(texts)
+    before = bc.compile()
+    stages, items = bc.stage_us()
+    assert set(stages) == {"sweep", "classify", "buckets", "emit"}
+    assert all(v > 0 for v in stages.values())
+    expect = 0
+    for t in texts:
+        drv = get_driver(t)
+        drv.compile_detector_error_model(t)
+        _, sens = drv.debug_sensitivities()
+        expect += int(np.count_nonzero(sens.any(axis=1)))
+    assert items == expect
+    assert [bc.dem(i) for i in range(len(texts))] == before
+    assert [bc.shape(i) for i in range(len(texts))] == [(d.num_errors, len(d.targets)) for d in before]
