@@ -23,7 +23,7 @@ EXPORTS = (
     "gp_get_driver", "gp_driver_destroy", "gp_compile_detector_error_model", "gp_driver_parse",
     "gp_dem_shape", "gp_dem_copy", "gp_dem_acquire", "gp_dem_release", "gp_dem_text", "gp_driver_params",
     "gp_driver_timings",
-    "gp_driver_bench_device", "gp_debug_pnodes", "gp_debug_sensitivities", "gp_debug_stepg",
+    "gp_driver_bench_device", "gp_debug_pnodes", "gp_debug_sensitivities", "gp_debug_stepg", "gp_debug_sweep_plan",
     "gp_build_stepg_host",
     "gp_batch_create", "gp_batch_destroy", "gp_batch_set_circuits", "gp_batch_compile",
     "gp_batch_stage_us", "gp_batch_device_us", "gp_batch_dem_shape", "gp_batch_dem_copy", "gp_batch_device_bytes",
@@ -92,6 +92,7 @@ def load() -> ctypes.CDLL:
     sig("gp_debug_pnodes", [P, pu32, pu32])
     sig("gp_debug_sensitivities", [P, P, P])
     sig("gp_debug_stepg", [P, P, P])
+    sig("gp_debug_sweep_plan", [P, P])
     sig("gp_build_stepg_host", [cs, i32, P, P, P])
     sig("gp_batch_create", [i32, i32, PP])
     sig("gp_batch_destroy", [P])

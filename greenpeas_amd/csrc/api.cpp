@@ -340,6 +340,14 @@ int gp_debug_stepg(const gp_driver *driver, uint64_t *adjacency, double *probs) 
   });
 }
 
+int gp_debug_sweep_plan(const gp_driver *driver, uint32_t plan[3]) {
+  return guarded([&] {
+    if (!driver || !plan) throw InvalidArgument("NULL argument");
+    if (!driver->pipe) throw RuntimeError("driver has not compiled a circuit");
+    driver->pipe->sweepPlan(plan);
+  });
+}
+
 int gp_build_stepg_host(const char *circuit_text, int correlation_level, uint32_t dims[4],
                         uint64_t *adjacency, double *probs) {
   return guarded([&] {
@@ -710,6 +718,10 @@ int gp_set_tuning(const char *name, uint32_t value) {
     else if (std::string(name) == "stash") gpa::setStash(value != 0);
     else if (std::string(name) == "mask_table") gpa::setMaskTable(value != 0);
     else if (std::string(name) == "prob_table") gpa::setProbTable(value != 0);
+    else if (std::string(name) == "sweep_split") {
+      if (value > 2) throw InvalidArgument("sweep_split must be 0, 1 or 2");
+      gpa::setSweepSplit(value);
+    }
     else throw InvalidArgument(std::string("unknown tuning parameter: ") + name);
   });
 }
@@ -722,6 +734,7 @@ int gp_get_tuning(const char *name, uint32_t *value) {
     else if (std::string(name) == "stash") *value = gpa::stash() ? 1u : 0u;
     else if (std::string(name) == "mask_table") *value = gpa::maskTable() ? 1u : 0u;
     else if (std::string(name) == "prob_table") *value = gpa::probTable() ? 1u : 0u;
+    else if (std::string(name) == "sweep_split") *value = gpa::sweepSplit();
     else throw InvalidArgument(std::string("unknown tuning parameter: ") + name);
   });
 }

@@ -39,6 +39,7 @@ constexpr int kBucketItemsPerWave = 256;  // wave-register sort capacity (4 per 
 std::atomic<uint32_t> gBucketFill{64};    // mean items per bucket the digit width aims at
 std::atomic<uint32_t> gStash{1};          // target rows on (1) or off (0)
 std::atomic<uint32_t> gMaskTable{1};      // classify's LDS mask tables allowed (1) or not (0)
+std::atomic<uint32_t> gSweepSplit{1};     // long columns swept in time chunks (1; 2: always) or whole (0)
 
 // Per-circuit descriptor (device and host).
 struct VDesc {
@@ -334,9 +335,11 @@ __device__ __forceinline__ void loadBlock(uint4 (&r)[RPT], uint32_t &head, __amd
 // first barrier.  Waves beyond the circuit's sweepThreads (a batch launches
 // with its widest circuit's shape) only take part in the barriers and
 // votes.  On exit the column's band [lo, hi] is recorded ({lo, hi};
-// {INT_MAX, -1} for a column without seeds).  Block 0 also zeroes the run's
-// flag words (read and written only by the later stages); the grid has at
-// least one block.
+// {INT_MAX, -1} for a column without seeds; {0, hi} for a split column).
+// Block 0 also zeroes the run's flag words (read and written only by the
+// later stages; a second, split launch zeroes them again); the grid has at
+// least one block.  Split columns (SplitDesc) run chunks of whole trips with
+// no early end; see there.
 // Steps per trip = prefetch distance: four while the register sets fit (up
 // to two records per thread), else two.
 __host__ __device__ constexpr int sweepTrip(int rpt) { return rpt <= 2 ? 4 : 2; }
@@ -409,36 +412,197 @@ __device__ __forceinline__ void sweepStep(const SweepState<RPT> &st, uint4 (&r)[
   loadBlock<RPT>(r, head, st.ac, loadOff, st.off, st.headOff);
 }
 
+// Time-chunked sweep of a long column.  The sweep is linear over GF(2): the
+// values of a run of pairs are a linear function of the ring state entering
+// it (the even layer above) plus the contribution of its own seeds.  A
+// column split into chunks 0..K-1 (chunk k = pairs bound[k] .. bound[k+1]-1)
+// is swept in two launches:
+//   launch 1  the top chunk, from the column's top seed (its state out goes
+//             to the column's state area), and for each middle chunk k
+//             (1 <= k <= K-2) its transfer map T_k -- ceil(P/32) sweeps with
+//             one 32-bit group of unit vectors injected as the state, no
+//             seeds, no S stores -- and its particular part p_k (zero state,
+//             its seeds, no S stores);
+//   launch 2  chunks 0..K-2, each from its entering state, formed as
+//             s = T_j s ^ p_j for j = K-2 .. k+1 starting from the top
+//             chunk's state out, storing S as the plain sweep does.
+// The chunks' lengths are multiples of 4 pairs (whole trips).  State area
+// of a split column (words, P = layerStride, G = ceil(P/32)): top state out
+// [P], then per middle chunk T_k [G][P] (word g*P + i: bit b = bit of
+// output node i for input node 32g + b) and p_k [P].
+constexpr int kSplitMaxChunks = 6;
+struct SplitDesc {
+  uint32_t v, c, K, stateOff;
+  int32_t bound[kSplitMaxChunks + 1];  // first pair of chunk k; bound[K] = top pair + 1
+  uint32_t pad;
+};
+enum : uint32_t { kSplitNone = 0, kSplitTop = 1, kSplitBasis = 2, kSplitPart = 3, kSplitResume = 4 };
+struct SplitIo {
+  const uint4 *items;  // after the launch's plain items: {split, mode | chunk << 8 | group << 16, circuit, column}
+  const SplitDesc *descs;
+  uint32_t *state;
+  uint32_t numItems;
+};
+// A plain sweep item is {circuit, column, top seed layer, lowest seed
+// layer}; a split column's top chunk is {circuit, column | kItemSplit, top
+// seed layer, split}.
+constexpr uint32_t kItemSplit = 0x80000000u;
+__host__ __device__ inline uint64_t splitTOff(const SplitDesc &s, uint32_t P, uint32_t k) {
+  return s.stateOff + P + static_cast<uint64_t>(k - 1) * ((P + 31) / 32 + 1) * P;
+}
+__host__ __device__ inline uint64_t splitPOff(const SplitDesc &s, uint32_t P, uint32_t k) {
+  return splitTOff(s, P, k) + static_cast<uint64_t>((P + 31) / 32) * P;
+}
+__host__ __device__ inline uint64_t splitStateWords(uint32_t P, uint32_t K) {
+  return P + static_cast<uint64_t>(K - 2) * ((P + 31) / 32 + 1) * P;
+}
+// LDS words a resumed chunk's entering-state chain adds to the sweep's own,
+// with n matvecs: the state, the n transfer maps and particular parts it
+// applies (copied in at once), a scratch vector and a 4-bit lookup table
+// (16 entries per 4 input nodes, for a multiple of 32 words of T's rows),
+// plus alignment.
+__host__ __device__ inline uint32_t splitTableGroups(uint32_t P) { return ((P + 31) / 32 + 7) / 8 * 8; }
+__host__ __device__ inline uint32_t splitChainWords(uint32_t P, uint32_t n) {
+  const uint32_t G = (P + 31) / 32;
+  return 2 * P + 128 * splitTableGroups(P) + n * (G + 1) * P + 4;
+}
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) u32x4 lds_u4;
+__device__ __forceinline__ lds_u4 *lds4At(uint32_t byteAddr) {
+  return reinterpret_cast<lds_u4 *>(static_cast<uintptr_t>(byteAddr));
+}
+
+// s = T s ^ p over GF(2), all in LDS (s at sB, written back there; T at TB
+// as [G][P] words, p at pB; scratch at wB, table at tabB): the table holds,
+// for each group of 4 input nodes, the XOR of every subset of their words;
+// output node i then XORs one table entry per 4 bits of its row of T.
+// Branch-free: the table covers whole groups of 8 words of T's rows (its
+// entries past the P input nodes are 0) and a row word past G reads as 0.
+__device__ __forceinline__ void splitMatvec(uint32_t P, uint32_t sB, uint32_t wB, uint32_t tabB, uint32_t TB,
+                                            uint32_t pB) {
+  const uint32_t G = (P + 31) / 32, G8 = splitTableGroups(P), tid = threadIdx.x;
+  for (uint32_t e = tid; e < 128 * G8; e += blockDim.x) {
+    const uint32_t q = e >> 4, x = e & 15;
+    uint32_t v = 0;
+#pragma unroll
+    for (uint32_t b = 0; b < 4; ++b) {
+      const uint32_t j = 4 * q + b;
+      const uint32_t w = *ldsAt(sB + 4 * (j < P ? j : 0u));
+      v ^= (x >> b & 1u) && j < P ? w : 0u;
+    }
+    *ldsAt(tabB + 4 * e) = v;
+  }
+  __syncthreads();
+  for (uint32_t i = tid; i < P; i += blockDim.x) {
+    uint32_t v = *ldsAt(pB + 4 * i);
+    // eight words of the row, then their 64 lookups: independent reads, so
+    // the LDS round trips overlap
+    for (uint32_t g0 = 0; g0 < G8; g0 += 8) {
+      uint32_t t[8];
+#pragma unroll
+      for (uint32_t u = 0; u < 8; ++u) {
+        const uint32_t g = g0 + u;
+        const uint32_t w = *ldsAt(TB + 4 * ((g < G ? g : 0u) * P + i));
+        t[u] = g < G ? w : 0u;
+      }
+      uint32_t a[8];
+#pragma unroll
+      for (uint32_t u = 0; u < 8; ++u) {
+        a[u] = 0u;
+#pragma unroll
+        for (uint32_t n = 0; n < 8; ++n)
+          a[u] ^= *ldsAt(tabB + 4 * (128 * (g0 + u) + 16 * n + ((t[u] >> (4 * n)) & 15u)));
+      }
+      v ^= ((a[0] ^ a[1]) ^ (a[2] ^ a[3])) ^ ((a[4] ^ a[5]) ^ (a[6] ^ a[7]));
+    }
+    *ldsAt(wB + 4 * i) = v;
+  }
+  __syncthreads();
+  for (uint32_t i = tid; i < P; i += blockDim.x) *ldsAt(sB + 4 * i) = *ldsAt(wB + 4 * i);
+  __syncthreads();
+}
+
+// Copies a resumed chunk's inputs to LDS in one pass of wide loads (U per
+// thread in flight): the top chunk's state out (P words) to sB, then the
+// transfer maps and particular parts of chunks k+1 .. K-2 (contiguous in
+// the state area) right after it.  Every count and offset is a multiple of
+// 4 words.
+template <uint32_t U>
+__device__ void splitStage(const SplitDesc &sd, uint32_t P, uint32_t k, const uint32_t *state, uint32_t sB) {
+  const uint32_t q0 = P / 4, nq = q0 + (sd.K - 2 - k) * ((P + 31) / 32 + 1) * P / 4;
+  const u32x4 *src0 = reinterpret_cast<const u32x4 *>(state + sd.stateOff);
+  const u32x4 *src1 = reinterpret_cast<const u32x4 *>(state + splitTOff(sd, P, k + 1));
+  for (uint32_t base = threadIdx.x; base < nq; base += U * blockDim.x) {
+    u32x4 x[U];
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u) {
+      const uint32_t q = base + u * blockDim.x;
+      if (q < nq) x[u] = q < q0 ? src0[q] : src1[q - q0];
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u) {
+      const uint32_t q = base + u * blockDim.x;
+      if (q < nq) *lds4At(sB + 16 * q) = x[u];
+    }
+  }
+  __syncthreads();
+}
+
 template <int RPT, bool GSEEDS>
 __global__ __launch_bounds__(kSweepMaxThreads) void sweepKernel(
-    const VDesc *__restrict__ desc, const uint2 *__restrict__ items, const uint4 *__restrict__ act,
-    const uint32_t *__restrict__ seedVals, const int32_t *__restrict__ colHi,
-    const int32_t *__restrict__ colLow, int2 *__restrict__ band, uint32_t *__restrict__ S,
-    uint32_t numItems, uint32_t *__restrict__ flags) {
+    const VDesc *__restrict__ desc, const uint4 *__restrict__ items, const uint4 *__restrict__ act,
+    const uint32_t *__restrict__ seedVals, int2 *__restrict__ band, uint32_t *__restrict__ S,
+    uint32_t numItems, uint32_t *__restrict__ flags, SplitIo sp) {
   // dynamic LDS only, addressed from 0:
-  // ring[2P] | zero word | seeds (unless GSEEDS) | vote[2]
+  // ring[2P] | zero word | seeds (unless GSEEDS) | vote[2] | (resumed
+  // chunks: the entering-state chain's words)
   const uint32_t tid = threadIdx.x;
   if (blockIdx.x == 0 && tid < kFlagWords) flags[tid] = 0u;
-  if (blockIdx.x >= numItems) return;
-  const uint2 it = items[blockIdx.x];
-  const VDesc d = desc[it.x];
-  const uint32_t c = it.y;
-  const int32_t hi = __builtin_amdgcn_readfirstlane(colHi[d.colOff + c]);
-  if (hi < 0) {
+  if (blockIdx.x >= numItems + sp.numItems) return;
+  // the item, then the circuit's descriptor (and the split's): two round
+  // trips before the seed, state and record loads
+  const bool plain = blockIdx.x < numItems;
+  const uint4 it = plain ? items[blockIdx.x] : sp.items[blockIdx.x - numItems];
+  uint32_t v, c, mode, chunk = 0, group = 0, si;
+  int32_t hi = 0, lowSeed = INT32_MIN;  // split chunks run their whole range (no early end)
+  if (plain) {
+    v = __builtin_amdgcn_readfirstlane(it.x);
+    c = __builtin_amdgcn_readfirstlane(it.y & ~kItemSplit);
+    hi = static_cast<int32_t>(__builtin_amdgcn_readfirstlane(it.z));
+    mode = (it.y & kItemSplit) ? kSplitTop : kSplitNone;
+    si = __builtin_amdgcn_readfirstlane(it.w);
+    if (mode == kSplitNone) lowSeed = static_cast<int32_t>(si);
+  } else {
+    si = __builtin_amdgcn_readfirstlane(it.x);
+    mode = __builtin_amdgcn_readfirstlane(it.y & 0xFFu);
+    chunk = __builtin_amdgcn_readfirstlane((it.y >> 8) & 0xFFu);
+    group = __builtin_amdgcn_readfirstlane(it.y >> 16);
+    v = __builtin_amdgcn_readfirstlane(it.z);
+    c = __builtin_amdgcn_readfirstlane(it.w);
+  }
+  mode = __builtin_amdgcn_readfirstlane(mode);
+  const VDesc d = desc[v];
+  const SplitDesc *sd = sp.descs + si;  // read only in split modes
+  if (mode == kSplitNone && hi < 0) {
     if (tid == 0) band[d.colOff + c] = make_int2(0x7FFFFFFF, -1);
     return;
   }
-  const int32_t lowSeed = __builtin_amdgcn_readfirstlane(colLow[d.colOff + c]);
   const uint32_t P = d.layerStride, T = d.sweepThreads;
-  const uint32_t *sv = seedVals + d.seedOff + static_cast<uint64_t>(c) * d.numSeeds;
-  for (uint32_t i = tid; i <= 2 * P; i += blockDim.x) *ldsAt(4 * i) = 0u;  // ring + zero word
-  uint32_t voteB = 4 * (2 * P + 1);
-  if constexpr (!GSEEDS) {
-    for (uint32_t i = tid; i < d.numSeeds; i += blockDim.x) *ldsAt(4 * (2 * P + 1 + i)) = sv[i];
-    voteB += 4 * d.numSeeds;
+  int32_t start = hi / 2, stop = 0;
+  if (mode != kSplitNone) {
+    const uint32_t K = __builtin_amdgcn_readfirstlane(sd->K);
+    if (mode == kSplitTop) {
+      stop = __builtin_amdgcn_readfirstlane(sd->bound[K - 1]);
+    } else {
+      start = __builtin_amdgcn_readfirstlane(sd->bound[chunk + 1]) - 1;
+      stop = __builtin_amdgcn_readfirstlane(sd->bound[chunk]);
+    }
   }
-  if (tid < 2) *ldsAt(voteB + 4 * tid) = 0u;
-  __syncthreads();
+  const bool seeds = mode != kSplitBasis, stores = mode != kSplitBasis && mode != kSplitPart;
+  const uint32_t *sv = seedVals + d.seedOff + static_cast<uint64_t>(c) * d.numSeeds;
+  uint32_t voteB = 4 * (2 * P + 1);
+  if constexpr (!GSEEDS) voteB += 4 * d.numSeeds;
 
   // Both loops below run D-step trips with the vote steps at the same
   // points and decide on leaving at the end of a trip, from the same vote
@@ -448,17 +612,71 @@ __global__ __launch_bounds__(kSweepMaxThreads) void sweepKernel(
   // the trip after a zero trip recomputes zeros below the band, which
   // nothing reads.
   constexpr int D = sweepTrip(RPT);
-  int32_t mt = hi / 2;  // top pair of the current trip
+  int32_t mt = start;   // top pair of the current trip
   int32_t lo = -1;      // lowest layer of the band, once known
   uint32_t trip = 0;    // vote word parity
   // whole waves: sweepThreads is a multiple of 64
-  if (__builtin_amdgcn_readfirstlane(tid >> 6) < (T >> 6)) {
-    SweepState<RPT> st;
+  const bool active = __builtin_amdgcn_readfirstlane(tid >> 6) < (T >> 6);
+  SweepState<RPT> st;
+  uint32_t slotB[D];
+  uint4 rs[D][RPT];
+  uint32_t head[D];
+  uint32_t loadOff = 0;
+  // seeds to LDS, several loads in flight per thread (fewer beside many
+  // prefetched records)
+  constexpr uint32_t U = RPT >= 4 ? 2 : 8;
+  constexpr uint32_t US = RPT >= 4 ? 2 : 16;  // wide loads of a resumed chunk's state inputs
+  if constexpr (!GSEEDS) {
+    const uint32_t n = d.numSeeds;
+    for (uint32_t base = tid; base < n; base += U * blockDim.x) {
+      uint32_t x[U];
+#pragma unroll
+      for (uint32_t u = 0; u < U; ++u) {
+        const uint32_t i = base + u * blockDim.x;
+        x[u] = i < n && seeds ? sv[i] : 0u;
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < U; ++u) {
+        const uint32_t i = base + u * blockDim.x;
+        if (i < n) *ldsAt(4 * (2 * P + 1 + i)) = x[u];
+      }
+    }
+  }
+  // a resumed chunk's entering state (chain from the top chunk's state
+  // out): LDS state | maps and parts of chunks chunk+1 .. K-2 | scratch |
+  // table (16-byte aligned)
+  const uint32_t sB = (voteB + 8 + 15) & ~15u;
+  const uint32_t injSlot = mode == kSplitNone || mode == kSplitTop ? 0u : static_cast<uint32_t>((start + 1) & 1) * P;
+  if (mode == kSplitResume) {
+    const uint32_t K = __builtin_amdgcn_readfirstlane(sd->K), G = (P + 31) / 32;
+    const uint32_t mB = sB + 4 * P, wB = mB + 4 * (K - 2 - chunk) * (G + 1) * P, tabB = wB + 4 * P;
+    splitStage<US>(*sd, P, chunk, sp.state, sB);
+    for (uint32_t j = K - 2; j > chunk; --j) {
+      const uint32_t TB = mB + 4 * (j - chunk - 1) * (G + 1) * P;
+      splitMatvec(P, sB, wB, tabB, TB, TB + 4 * G * P);
+    }
+  }
+  // ring (with the injected state in the slot the chunk's first pair
+  // reads) and zero word
+  for (uint32_t i = tid; i <= 2 * P; i += blockDim.x) {
+    uint32_t x = 0u;
+    if (i >= injSlot && i < injSlot + P) {
+      const uint32_t j = i - injSlot;
+      if (mode == kSplitBasis) x = (j >> 5) == group ? 1u << (j & 31) : 0u;
+      else if (mode == kSplitResume) x = *ldsAt(sB + 4 * j);
+    }
+    *ldsAt(4 * i) = x;
+  }
+  if (tid < 2) *ldsAt(voteB + 4 * tid) = 0u;
+  __syncthreads();
+
+  if (active) {
     st.ac = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint4 *>(act + d.actOff), 0,
                                               16 * (P + 1) * (kSweepLeadPad + d.numPairs), 0x00020000);
     const uint32_t rows = static_cast<uint32_t>(d.colStride / P);
+    // (no records: every store is out of range and dropped)
     st.sc = __builtin_amdgcn_make_buffer_rsrc(S + d.sOff + static_cast<uint64_t>(c) * d.colStride, 0,
-                                              static_cast<uint32_t>(4 * d.colStride), 0x00020000);
+                                              stores ? static_cast<uint32_t>(4 * d.colStride) : 0u, 0x00020000);
     st.gseeds = sv;
     st.blockBytes = 16 * (P + 1);
     st.headOff = 16 * P;
@@ -467,7 +685,6 @@ __global__ __launch_bounds__(kSweepMaxThreads) void sweepKernel(
     st.rowBytes = 4 * P;
     st.rows = rows;
     // ring slot of each trip position (pair mt - u; D is even)
-    uint32_t slotB[D];
 #pragma unroll
     for (int u = 0; u < D; ++u) slotB[u] = static_cast<uint32_t>((mt - u) & 1) * P * 4;
 #pragma unroll
@@ -480,15 +697,15 @@ __global__ __launch_bounds__(kSweepMaxThreads) void sweepKernel(
       st.pos4[k] = pos * 4;
     }
     // register sets for pairs mt, mt-1, ..., mt-D+1 (block kSweepLeadPad +
-    // pair).  Each set but the last is followed by the stores of one step
-    // (to the rows of the lowest padding layers, which nothing reads), as
-    // each refill is followed by a step's stores in the loop: the compiler's
-    // count of memory operations outstanding at the loop head is then the
-    // same on entry and on the back edge, and its waits keep the full
-    // prefetch distance.
-    uint4 rs[D][RPT];
-    uint32_t head[D];
-    uint32_t loadOff = static_cast<uint32_t>(mt + static_cast<int32_t>(kSweepLeadPad)) * st.blockBytes;
+    // pair), loaded right before the loop (with other loops after them the
+    // compiler loses count of what is outstanding at the loop head and
+    // drains the prefetches on every trip).  Each set but the last is
+    // followed by the stores of one step (to the rows of the lowest padding
+    // layers, which nothing reads), as each refill is followed by a step's
+    // stores in the loop: the compiler's count of memory operations
+    // outstanding at the loop head is then the same on entry and on the
+    // back edge, and its waits keep the full prefetch distance.
+    loadOff = static_cast<uint32_t>(mt + static_cast<int32_t>(kSweepLeadPad)) * st.blockBytes;
     const uint32_t padB = (rows - 2 * kSweepLeadPad) * st.rowBytes;
 #pragma unroll
     for (int u = 0; u < D; ++u) {
@@ -504,6 +721,8 @@ __global__ __launch_bounds__(kSweepMaxThreads) void sweepKernel(
         __builtin_amdgcn_sched_barrier(0);
       }
     }
+  }
+  if (active) {
     for (;;) {
       uint32_t any = 0;
       const uint32_t myWord = voteB + 4 * (trip & 1), prevWord = voteB + 4 * ((trip + 1) & 1);
@@ -521,7 +740,7 @@ __global__ __launch_bounds__(kSweepMaxThreads) void sweepKernel(
       }
       mt -= D;
       ++trip;
-      if (lo >= 0 || mt < 0) break;
+      if (lo >= 0 || mt < stop) break;
     }
   } else {
     for (;;) {
@@ -532,11 +751,20 @@ __global__ __launch_bounds__(kSweepMaxThreads) void sweepKernel(
       for (int u = 0; u < D; ++u) ldsBarrier();
       mt -= D;
       ++trip;
-      if (lo >= 0 || mt < 0) break;
+      if (lo >= 0 || mt < stop) break;
     }
   }
   if (lo < 0) lo = 0;
-  if (tid == 0) band[d.colOff + c] = make_int2(lo, hi);
+  if (mode == kSplitTop || mode == kSplitBasis || mode == kSplitPart) {
+    // state out: the even layer of pair `stop` (written by the last step,
+    // behind its barrier)
+    uint32_t *out = sp.state + (mode == kSplitTop    ? static_cast<uint64_t>(sd->stateOff)
+                                : mode == kSplitPart ? splitPOff(*sd, P, chunk)
+                                                     : splitTOff(*sd, P, chunk) + static_cast<uint64_t>(group) * P);
+    const uint32_t slot = static_cast<uint32_t>(stop & 1) * P;
+    for (uint32_t i = tid; i < P; i += blockDim.x) out[i] = *ldsAt(4 * (slot + i));
+  }
+  if (tid == 0 && (mode == kSplitNone || mode == kSplitTop)) band[d.colOff + c] = make_int2(lo, hi);
 }
 
 // One node's frame rows (its sensitivity is their XOR), decoded from its
@@ -1397,8 +1625,9 @@ struct Pipeline::Impl {
   // inputs (one device arena, one pinned staging arena with the same layout)
   DevBuf in;
   HostBuf inHost;
-  size_t oDesc = 0, oItems = 0, oTiles = 0, oAct = 0, oSeed = 0, oColHi = 0,
-         oColLow = 0, oRankCode = 0, oSpill = 0, oProbIdx = 0, oProbVal = 0, inBytes = 0;
+  size_t oDesc = 0, oItems = 0, oTiles = 0, oAct = 0, oSeed = 0, oRankCode = 0, oSpill = 0, oProbIdx = 0,
+         oProbVal = 0, inBytes = 0;
+  std::vector<uint4> hostItems;  // the sweep items of the load, in launch order
   uint32_t V = 0, nItems = 0, nTiles = 0, B = 0, R = 0, Ccols = 0, rpt = 1;
   bool classTable = true;  // every classify workgroup's mask table fits in LDS
   uint64_t Swords = 0;
@@ -1440,6 +1669,20 @@ struct Pipeline::Impl {
   uint64_t totalTargets = 0;
   std::vector<uint32_t> vClass, vTgt;
   std::vector<VDesc> descs;
+
+  // time-chunked sweep of long columns, planned after a load's first run
+  // (splitBuf: descriptors | launch-1 items | launch-2 items | per-column
+  // split index; splitState: the split columns' state areas)
+  bool splitPlanned = false;
+  uint32_t nSplit = 0, nSplit1 = 0, nSplit3 = 0;
+  size_t splitLdsBytes = 0, oSplitItems1 = 0, oSplitItems3 = 0;
+  DevBuf splitBuf, splitState;
+  SplitIo splitIo(bool second) const {
+    const char *b = splitBuf.as<char>();
+    if (!nSplit) return SplitIo{nullptr, nullptr, nullptr, 0};
+    return SplitIo{reinterpret_cast<const uint4 *>(b + (second ? oSplitItems3 : oSplitItems1)),
+                   reinterpret_cast<const SplitDesc *>(b), splitState.as<uint32_t>(), second ? nSplit3 : nSplit1};
+  }
 
   uint32_t *flags() const { return counters.as<uint32_t>(); }
   uint32_t *bucketCount() const { return bucketCnt.as<uint32_t>(); }
@@ -1510,6 +1753,8 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   GPA_HIP(hipStreamSynchronize(m.side));
   m.sizesKnown = false;
   m.stashShift = 4;
+  m.splitPlanned = false;
+  m.nSplit = m.nSplit1 = m.nSplit3 = 0;
   const bool frozen = m.frozen;
   const uint32_t V = static_cast<uint32_t>(imgs.size());
   if (V == 0) throw InvalidArgument("no circuits to compile");
@@ -1521,7 +1766,7 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   uint64_t actOff = 0, seedOff = 0, colOff = 0, rankOff = 0, sOff = 0, spillOff = 0, probOff = 0;
   uint32_t bucketBase = 0, maxRpt = 1, maxThreads = 64;
   size_t ldsRing = 0, ldsAll = 0;  // largest ring (+ zero word), largest ring + seeds
-  std::vector<uint2> items;
+  std::vector<uint4> items;
   std::vector<uint4> tiles;
   bool classTable = true;
   std::vector<std::pair<int32_t, uint32_t>> order;  // (band estimate, item index)
@@ -1559,7 +1804,7 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
       const int32_t hiL = g.colHi[c];
       const int32_t est = hiL < 0 ? -1 : hiL - g.colLow[c] + 1;
       order.push_back({est, static_cast<uint32_t>(items.size())});
-      items.push_back(make_uint2(v, c));
+      items.push_back(make_uint4(v, c, static_cast<uint32_t>(hiL), static_cast<uint32_t>(g.colLow[c])));
     }
     for (uint32_t t = 0; t < g.numRanks; t += kClassBlock) {
       // the rows of a rank at layer l lie in layers l .. l + rowSpan
@@ -1597,7 +1842,8 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   // longest bands first (they bound the sweep's critical path)
   std::stable_sort(order.begin(), order.end(),
                    [](const auto &a, const auto &b) { return a.first > b.first; });
-  std::vector<uint2> sortedItems(items.size());
+  std::vector<uint4> &sortedItems = m.hostItems;
+  sortedItems.resize(items.size());
   for (size_t k = 0; k < order.size(); ++k) sortedItems[k] = items[order[k].second];
 
   m.V = V;
@@ -1620,12 +1866,10 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
     return o;
   };
   m.oDesc = place(sizeof(VDesc) * V);
-  m.oItems = place(sizeof(uint2) * m.nItems);
+  m.oItems = place(sizeof(uint4) * m.nItems);
   m.oTiles = place(sizeof(uint4) * m.nTiles);
   m.oAct = place(16 * actOff);
   m.oSeed = place(4 * seedOff);
-  m.oColHi = place(4 * colOff);
-  m.oColLow = place(4 * colOff);
   m.oRankCode = place(8 * rankOff);
   m.oSpill = place(16 * spillOff);
   m.oProbIdx = place(2 * rankOff);
@@ -1635,15 +1879,13 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   m.inHost.ensure(m.inBytes);
   char *hb = m.inHost.as<char>();
   std::memcpy(hb + m.oDesc, m.descs.data(), sizeof(VDesc) * V);
-  std::memcpy(hb + m.oItems, sortedItems.data(), sizeof(uint2) * m.nItems);
+  std::memcpy(hb + m.oItems, sortedItems.data(), sizeof(uint4) * m.nItems);
   std::memcpy(hb + m.oTiles, tiles.data(), sizeof(uint4) * m.nTiles);
   for (uint32_t v = 0; v < V; ++v) {
     const StepgImage &g = *imgs[v];
     const VDesc &d = m.descs[v];
     std::memcpy(hb + m.oAct + 16 * d.actOff, g.act.data(), 4 * g.act.size());
     std::memcpy(hb + m.oSeed + 4 * d.seedOff, g.seedVals.data(), 4 * g.seedVals.size());
-    std::memcpy(hb + m.oColHi + 4 * d.colOff, g.colHi.data(), 4 * g.colHi.size());
-    std::memcpy(hb + m.oColLow + 4 * d.colOff, g.colLow.data(), 4 * g.colLow.size());
     std::memcpy(hb + m.oRankCode + 8 * d.rankOff, g.rankCode.data(), 4 * g.rankCode.size());
     std::memcpy(hb + m.oSpill + 16 * d.spillOff, g.rankSpill.data(), 4 * g.rankSpill.size());
     if (d.probDict) {
@@ -1755,12 +1997,10 @@ void Pipeline::run(bool timed) {
   GPA_HIP(hipSetDevice(device_));
   char *ib = m.in.as<char>();
   const VDesc *desc = reinterpret_cast<const VDesc *>(ib + m.oDesc);
-  const uint2 *items = reinterpret_cast<const uint2 *>(ib + m.oItems);
+  const uint4 *items = reinterpret_cast<const uint4 *>(ib + m.oItems);
   const uint4 *tiles = reinterpret_cast<const uint4 *>(ib + m.oTiles);
   const uint4 *act = reinterpret_cast<const uint4 *>(ib + m.oAct);
   const uint32_t *seed = reinterpret_cast<const uint32_t *>(ib + m.oSeed);
-  const int32_t *colHi = reinterpret_cast<const int32_t *>(ib + m.oColHi);
-  const int32_t *colLow = reinterpret_cast<const int32_t *>(ib + m.oColLow);
   const RankIn rin{reinterpret_cast<const uint2 *>(ib + m.oRankCode), reinterpret_cast<const uint4 *>(ib + m.oSpill),
                    reinterpret_cast<const double *>(ib + m.oProbVal),
                    reinterpret_cast<const uint16_t *>(ib + m.oProbIdx)};
@@ -1779,10 +2019,16 @@ void Pipeline::run(bool timed) {
   // captured in a graph.
   mark(0);
   {
-#define GPA_SWEEP(RPT, G)                                                                        \
-  sweepKernel<RPT, G><<<std::max<uint32_t>(m.nItems, 1), m.sweepThreads, m.sweepLdsBytes, st>>>(  \
-      desc, items, act, seed, colHi, colLow, m.band.as<int2>(), m.S.as<uint32_t>(), m.nItems,      \
-      m.flags())
+    // launch 1: every column (split columns: their top chunk) and the
+    // middle chunks' transfer maps; launch 2: the split columns' lower chunks
+    const SplitIo sp1 = m.splitIo(false), sp2 = m.splitIo(true);
+#define GPA_SWEEP(RPT, G)                                                                             \
+  sweepKernel<RPT, G><<<std::max<uint32_t>(m.nItems + sp1.numItems, 1), m.sweepThreads, m.sweepLdsBytes, \
+                        st>>>(desc, items, act, seed, m.band.as<int2>(), m.S.as<uint32_t>(), \
+                              m.nItems, m.flags(), sp1);                                                \
+  if (sp2.numItems)                                                                                     \
+    sweepKernel<RPT, G><<<sp2.numItems, m.sweepThreads, m.splitLdsBytes, st>>>(                         \
+        desc, items, act, seed, m.band.as<int2>(), m.S.as<uint32_t>(), 0u, m.flags(), sp2)
     if (m.seedsInLds) {
       switch (m.rpt) {
       case 1: GPA_SWEEP(1, false); break;
@@ -1850,6 +2096,165 @@ void Pipeline::run(bool timed) {
   emitKernel<<<std::max<uint32_t>((B + 3) / 4, 1), 256, 0, st>>>(desc, ein, eout);
   mark(4);
   GPA_HIP(hipGetLastError());
+}
+
+// Critical-path model of the sweep, in steps (pairs) of a column swept
+// alone, measured on MI355X: a step while many columns run at once costs
+// 1.7 steps; a matvec of the entering-state chain costs 22 (a 576-node
+// layer: VALU-bound on the one compute unit of the chunk); staging the maps
+// costs 10; the second launch's set-up 3.
+constexpr int32_t kSplitMatvecPairs = 22, kSplitStagePairs = 10, kSplitLaunchPairs = 3;
+constexpr int32_t kSplitConcurrentPct = 170;
+constexpr uint32_t kSplitMaxColumns = 4;   // split columns per load
+constexpr uint32_t kSplitMaxPlain = 256;   // plain sweep items (more fill the chip anyway)
+
+static int32_t concurrentSteps(int32_t n) { return n * kSplitConcurrentPct / 100; }
+
+// Runs after a load's first run, from the column bands it found.  The
+// plain items are put in the order of their measured bands, longest first
+// (the launch order of the sweep's workgroups; the first run orders them by
+// seed range, which misses the observables' long band).  Then the
+// time-chunked sweep: the longest columns, while their band reaches the
+// bottom (layer <= 3) and K chunks are estimated to beat sweeping the
+// column whole by 10 % (the whole column: its first `rest` steps beside
+// the other columns, rest = the next longest band; split: the top chunk or
+// a middle chunk's maps beside the other columns, the second launch, then
+// the longest lower chunk with its staging and matvecs).  Writes the
+// items and descriptors to the device; the results do not depend on the
+// plan.
+static void planSweep(Pipeline::Impl &m) {
+  m.nSplit = m.nSplit1 = m.nSplit3 = 0;
+  if (!m.Ccols) return;
+  std::vector<int2> bands(m.Ccols);
+  GPA_HIP(hipMemcpy(bands.data(), m.band.p, 8ull * m.Ccols, hipMemcpyDeviceToHost));
+  auto bandLen = [&](const uint4 &it) {
+    const int2 b = bands[m.descs[it.x].colOff + (it.y & ~kItemSplit)];
+    return b.y < 0 ? -1 : b.y / 2 - b.x / 2 + 1;
+  };
+  std::stable_sort(m.hostItems.begin(), m.hostItems.end(),
+                   [&](const uint4 &a, const uint4 &b) { return bandLen(a) > bandLen(b); });
+  auto writeItems = [&]() {
+    std::memcpy(m.inHost.as<char>() + m.oItems, m.hostItems.data(), 16 * m.hostItems.size());
+    GPA_HIP(hipMemcpy(m.in.as<char>() + m.oItems, m.hostItems.data(), 16 * m.hostItems.size(),
+                      hipMemcpyHostToDevice));
+  };
+  const uint32_t knob = gSweepSplit.load();
+  const bool force = knob == 2;  // every column reaching the bottom, most chunks that fit
+  if (!knob || !m.seedsInLds || m.nItems > kSplitMaxPlain) {
+    writeItems();
+    return;
+  }
+  struct Col {
+    uint32_t v, c;
+    int32_t lo, hi, len;
+  };
+  std::vector<Col> cols;
+  for (uint32_t v = 0; v < m.V; ++v) {
+    const VDesc &d = m.descs[v];
+    for (uint32_t c = 0; c < d.numCols; ++c) {
+      const int2 b = bands[d.colOff + c];
+      if (b.y >= 0) cols.push_back({v, c, b.x, b.y, b.y / 2 - b.x / 2 + 1});
+    }
+  }
+  std::stable_sort(cols.begin(), cols.end(), [](const Col &a, const Col &b) { return a.len > b.len; });
+  std::vector<SplitDesc> descs;
+  std::vector<uint4> items1, items3;
+  uint64_t stateWords = 0;
+  size_t lds = 0;
+  for (size_t i = 0; i < cols.size() && descs.size() < kSplitMaxColumns; ++i) {
+    const Col &col = cols[i];
+    if (col.lo > 3) {
+      if (force) continue;
+      break;  // the longest column left bounds the sweep anyway
+    }
+    const VDesc &d = m.descs[col.v];
+    const uint32_t P = d.layerStride, G = (P + 31) / 32;
+    const int32_t rest = i + 1 < cols.size() ? cols[i + 1].len : 0;
+    const int32_t top = col.hi / 2, total = top + 1;
+    SplitDesc best{};
+    // must beat the whole sweep by 10 % (unless forced)
+    const int32_t whole = concurrentSteps(std::min(rest, total)) + total - std::min(rest, total);
+    int32_t bestEst = force ? INT32_MAX : whole - whole / 10;
+    for (int K = 3; K <= kSplitMaxChunks; ++K) {
+      SplitDesc s{};
+      s.v = col.v;
+      s.c = col.c;
+      s.K = static_cast<uint32_t>(K);
+      // lower chunk k pays its staging and K-2-k matvecs first: lengths
+      // shortened by that so the chunks of the second launch end together
+      // (forced plans: equal lengths)
+      auto over = [&](int k) {
+        const int32_t n = K - 2 - k;
+        return force ? 0 : n * kSplitMatvecPairs + (n ? kSplitStagePairs : 0);
+      };
+      int32_t sumOver = 0;
+      for (int k = 0; k <= K - 2; ++k) sumOver += over(k);
+      const int32_t x = std::max<int32_t>(4, (total + sumOver) / K / 4 * 4);
+      s.bound[K] = top + 1;
+      s.bound[K - 1] = s.bound[K] - x;
+      int32_t phase1 = std::max(x, rest), phase2 = 0;
+      for (int k = K - 2; k >= 1; --k) {
+        const int32_t len = std::max<int32_t>(4, (x - over(k)) / 4 * 4);
+        s.bound[k] = s.bound[k + 1] - len;
+        phase1 = std::max(phase1, len);
+        phase2 = std::max(phase2, len + over(k));
+      }
+      s.bound[0] = 0;
+      if (s.bound[1] < 4) continue;
+      phase2 = std::max(phase2, s.bound[1] + over(0));
+      const int32_t est = force ? -K : concurrentSteps(phase1) + kSplitLaunchPairs + phase2;
+      const size_t need = 4ull * (2 * P + 1 + d.numSeeds + 2) + 4ull * splitChainWords(P, K - 2);
+      if (need > kLdsBudget) continue;
+      if (est < bestEst) {
+        bestEst = est;
+        best = s;
+      }
+    }
+    if (!best.K) {
+      if (force) continue;
+      break;
+    }
+    lds = std::max<size_t>(lds, 4ull * (2 * P + 1 + d.numSeeds + 2) + 4ull * splitChainWords(P, best.K - 2));
+    const uint32_t si = static_cast<uint32_t>(descs.size());
+    best.stateOff = static_cast<uint32_t>(stateWords);
+    stateWords += splitStateWords(P, best.K);
+    if (stateWords >= 0xFFFFFFFFull) break;
+    for (uint32_t k = 1; k + 2 <= best.K; ++k) {
+      for (uint32_t g = 0; g < G; ++g) items1.push_back(make_uint4(si, kSplitBasis | k << 8 | g << 16, col.v, col.c));
+      items1.push_back(make_uint4(si, kSplitPart | k << 8, col.v, col.c));
+    }
+    for (uint32_t k = 0; k + 2 <= best.K; ++k) items3.push_back(make_uint4(si, kSplitResume | k << 8, col.v, col.c));
+    descs.push_back(best);
+  }
+  if (descs.empty()) {
+    writeItems();
+    return;
+  }
+  auto al = [](size_t x) { return (x + 255) & ~static_cast<size_t>(255); };
+  m.oSplitItems1 = al(sizeof(SplitDesc) * descs.size());
+  m.oSplitItems3 = al(m.oSplitItems1 + 16 * items1.size());
+  const size_t bytes = m.oSplitItems3 + 16 * items3.size();
+  std::vector<char> h(bytes, 0);
+  std::memcpy(h.data(), descs.data(), sizeof(SplitDesc) * descs.size());
+  std::memcpy(h.data() + m.oSplitItems1, items1.data(), 16 * items1.size());
+  std::memcpy(h.data() + m.oSplitItems3, items3.data(), 16 * items3.size());
+  m.dropGraph();  // before the buffers it may name are replaced
+  m.splitBuf.ensure(bytes, false, "split sweep");
+  m.splitState.ensure(4 * stateWords, false, "split sweep");
+  GPA_HIP(hipMemcpy(m.splitBuf.p, h.data(), bytes, hipMemcpyHostToDevice));
+  // the split columns' own items become their top chunks (in the staged
+  // inputs, which later compiles upload, and on the device now)
+  for (uint32_t si = 0; si < descs.size(); ++si)
+    for (uint4 &it : m.hostItems)
+      if (it.x == descs[si].v && it.y == descs[si].c) {
+        it.y |= kItemSplit;
+        it.w = si;
+      }
+  writeItems();
+  m.splitLdsBytes = lds;
+  m.nSplit = static_cast<uint32_t>(descs.size());
+  m.nSplit1 = static_cast<uint32_t>(items1.size());
+  m.nSplit3 = static_cast<uint32_t>(items3.size());
 }
 
 void Pipeline::download() {
@@ -1937,6 +2342,10 @@ void Pipeline::download() {
       m.stashShift = shift;
       m.dropGraph();
     }
+  }
+  if (!m.splitPlanned) {
+    m.splitPlanned = true;
+    planSweep(m);
   }
   m.lastC = C;
   m.lastT = T;
@@ -2053,6 +2462,9 @@ double Pipeline::benchDevice(uint32_t iters) {
 void setBucketFill(uint32_t fill) { gBucketFill.store(std::max<uint32_t>(fill, 1)); }
 void setStash(bool on) { gStash.store(on ? 1u : 0u); }
 bool stash() { return gStash.load() != 0; }
+void setSweepSplit(uint32_t mode) { gSweepSplit.store(mode); }
+uint32_t sweepSplit() { return gSweepSplit.load(); }
+
 void setMaskTable(bool on) { gMaskTable.store(on ? 1u : 0u); }
 bool maskTable() { return gMaskTable.load() != 0; }
 uint32_t bucketFill() { return gBucketFill.load(); }
@@ -2070,6 +2482,12 @@ uint64_t Pipeline::deviceBytes() const {
   uint64_t s = 0;
   for (const DevBuf *b : bufs) s += b->cap;
   return s;
+}
+
+void Pipeline::sweepPlan(uint32_t out[3]) const {
+  out[0] = impl_->nSplit;
+  out[1] = impl_->nSplit1;
+  out[2] = impl_->nSplit3;
 }
 
 uint64_t Pipeline::sortedItems() const {

@@ -105,6 +105,9 @@ class Pipeline {
 
   // Parity/debug access (after download or run + sync).
   void debugSensitivities(uint32_t v, std::vector<uint32_t> &cols32) const;  // ranks x cols
+  // The load's time-chunked sweep plan: {split columns, blocks the first
+  // sweep launch adds for them, blocks of the second launch}.
+  void sweepPlan(uint32_t out[3]) const;
   // Device pointers for the differentiable path (single variant, after a
   // run): the p>0 nodes of non-empty classes, grouped by class in output
   // order (compacted on each call), and each class's first member in it.
@@ -138,6 +141,12 @@ bool stash();
 // workgroups whose table would not fit).  Read at each run.
 void setMaskTable(bool on);
 bool maskTable();
+// Whether a load's long columns may be swept in time chunks after its first
+// run: 1 (default) where the estimated critical path gains, 2 every column
+// whose band reaches the bottom with the most chunks that fit (for tests), 0
+// never (every column swept whole).  Read at that run.
+void setSweepSplit(uint32_t mode);
+uint32_t sweepSplit();
 
 // Segmented XOR-merge kernels (differentiable path); pointers are device
 // pointers, stream is a hipStream_t.

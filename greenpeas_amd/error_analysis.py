@@ -190,6 +190,13 @@ class Driver:
         _native.check(self._lib.gp_driver_bench_device(self._h, iterations, ctypes.byref(us)))
         return us.value
 
+    def debug_sweep_plan(self) -> tuple[int, int, int]:
+        """(columns swept in time chunks, extra first-launch sweep blocks,
+        second-launch blocks) of this driver's current circuit."""
+        plan = (ctypes.c_uint32 * 3)()
+        _native.check(self._lib.gp_debug_sweep_plan(self._h, plan))
+        return tuple(int(x) for x in plan)
+
     def debug_sensitivities(self) -> tuple[np.ndarray, np.ndarray]:
         """(node index of each p>0 node, its sensitivity words) from the GPU."""
         n, w = ctypes.c_uint32(), ctypes.c_uint32()

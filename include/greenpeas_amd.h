@@ -144,6 +144,13 @@ int gp_debug_sensitivities(const gp_driver *driver, uint32_t *pnode_index,
  * 32 bits, 0xFFFFFFFF = none) and probs[nodes] (STEPG.hpp:40-132). */
 int gp_debug_stepg(const gp_driver *driver, uint64_t *adjacency, double *probs);
 
+/* The driver's time-chunked sweep plan (made at the first compile of a
+ * circuit, see the "sweep_split" knob): plan[0] = columns swept in chunks,
+ * plan[1] = sweep blocks the first launch adds for their middle chunks'
+ * transfer maps, plan[2] = blocks of the second launch (their lower
+ * chunks).  All zero: every column is swept whole. */
+int gp_debug_sweep_plan(const gp_driver *driver, uint32_t plan[3]);
+
 /* Host-only STEPG build (no device is touched): dims[0..3] = layers, nodes
  * per layer, nodes, fault locations.  Pass NULL arrays to query dims only,
  * then adjacency[nodes] and probs[nodes] as gp_debug_stepg. */
@@ -209,7 +216,13 @@ int gp_estimate_device_bytes(const char *circuit_text, int correlation_level,
  *   "prob_table": 1 (default) sends a circuit's probabilities as 16-bit
  *   indices into its table of distinct values when it has at most 65536 of
  *   them; 0 always sends one double per error node; read when a circuit is
- *   parsed. */
+ *   parsed.
+ *   "sweep_split": 1 (default) lets a load's first compile plan a
+ *   time-chunked sweep of columns much longer than the rest (transfer maps
+ *   of the middle chunks, then the lower chunks in parallel; identical
+ *   results); 2 splits every column whose band reaches the first layers
+ *   into as many chunks as fit (a test setting); 0 sweeps every column
+ *   whole; read at a load's first compile. */
 int gp_set_tuning(const char *name, uint32_t value);
 int gp_get_tuning(const char *name, uint32_t *value);
 

@@ -304,3 +304,76 @@ def test_repeated_compiles_with_narrowed_target_rows(oracle, name):
     drv.bench_device(3)  # graph replays with the narrowed rows
     again = drv.compile_detector_error_model()
     assert np.array_equal(again.targets, ref.targets)
+
+
+@pytest.fixture
+def sweep_split():
+    """Sets the sweep_split knob for a test; restores the default after."""
+    from greenpeas_amd import autotune
+
+    yield lambda mode: autotune.set_tuning("sweep_split", mode)
+    autotune.set_tuning("sweep_split", 1)
+
+
+def _compile_twice(text):
+    """A driver's DEM and sensitivities after its second compile (the first
+    compile of a load plans the split sweep; the second runs it)."""
+    drv = get_driver(text)
+    drv.compile_detector_error_model(text)
+    dem = drv.compile_detector_error_model(text)
+    idx, sens = drv.debug_sensitivities()
+    return dem, idx, sens, drv.last_hashes(), drv.debug_sweep_plan()
+
+
+@pytest.mark.parametrize("name", ["bb12", "surface5", "surface7", "bb6", "bb10"])
+def test_split_sweep_identical_to_whole_sweep(name, oracle, sweep_split):
+    """Columns swept in time chunks (transfer maps of the middle chunks,
+    lower chunks resumed from their entering state) give bit-identical
+    sensitivities, hashes and DEMs to sweeping every column whole (forced:
+    every column that reaches the first layers, as many chunks as fit)."""
+    text = read_circuit(name)
+    sweep_split(2)
+    split = _compile_twice(text)
+    sweep_split(0)
+    whole = _compile_twice(text)
+    assert split[4][0] >= 1 and split[4][2] >= 2  # the split path ran
+    assert whole[4] == (0, 0, 0)
+    assert np.array_equal(split[1], whole[1])
+    assert np.array_equal(split[2], whole[2])
+    assert np.array_equal(split[3], whole[3])
+    assert split[0] == whole[0]
+    ref = oracle.compile_stepg(text, 2)
+    assert np.array_equal(split[3], ref.hashes)
+
+
+def test_split_sweep_planned_for_a_long_memory(sweep_split):
+    """As planned (the default): a 60-round memory's observables column, far
+    longer than any detector column, is swept in chunks; a 12-round bb12 is
+    not (too short to gain); results equal the whole sweep's."""
+    from greenpeas_amd.codes import SurfaceCode
+
+    text = SurfaceCode(5).memory_text(60, 1e-3)
+    split = _compile_twice(text)
+    assert split[4][0] == 1 and split[4][2] >= 2
+    assert _compile_twice(read_circuit("bb12"))[4] == (0, 0, 0)
+    sweep_split(0)
+    whole = _compile_twice(text)
+    assert whole[4] == (0, 0, 0)
+    assert np.array_equal(split[2], whole[2]) and np.array_equal(split[3], whole[3])
+    assert split[0] == whole[0]
+
+
+def test_split_sweep_repeated_compiles_and_reload(sweep_split):
+    """A driver keeps its split plan over many compiles and re-plans after a
+    load of another circuit; every result equals the first."""
+    a, b = read_circuit("bb12"), read_circuit("surface5")
+    sweep_split(2)
+    drv = get_driver(a)
+    first = drv.compile_detector_error_model(a)
+    for _ in range(4):
+        assert drv.compile_detector_error_model(a) == first
+    assert drv.debug_sweep_plan()[0] >= 1
+    small = drv.compile_detector_error_model(b)
+    assert drv.compile_detector_error_model(b) == small
+    assert drv.compile_detector_error_model(a) == first
+    assert drv.compile_detector_error_model(a) == first

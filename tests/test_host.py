@@ -114,6 +114,14 @@ def test_tuning_knobs_roundtrip():
         assert autotune.get_tuning("prob_table") == 0
         autotune.set_tuning("prob_table", 1)
         assert autotune.get_tuning("prob_table") == 1
+        autotune.set_tuning("sweep_split", 0)
+        assert autotune.get_tuning("sweep_split") == 0
+        autotune.set_tuning("sweep_split", 2)
+        assert autotune.get_tuning("sweep_split") == 2
+        autotune.set_tuning("sweep_split", 1)
+        assert autotune.get_tuning("sweep_split") == 1
+        with pytest.raises(ValueError):
+            autotune.set_tuning("sweep_split", 3)
         with pytest.raises(ValueError):
             autotune.set_tuning("sweep_rpt", 3)
         with pytest.raises(ValueError):
