@@ -13,6 +13,10 @@ host, the span the reference times at Cli.hpp:293-302) and a CPU baseline
 (the oracle's forward Pauli-frame compiler, single thread, on this host).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+Other workloads (--workload, one JSON line each): batch (256 uneven
+variants batched vs one by one), grad (forward + backward), serving
+(Poisson request streams), long (40-100-round memories, whole vs
+time-chunked sweep).
 """
 
 from __future__ import annotations
@@ -105,7 +109,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--workload", default="plain", choices=("plain", "batch", "grad", "serving"),
+    ap.add_argument("--workload", default="plain", choices=("plain", "batch", "grad", "serving", "long"),
                     help="plain: the headline bb12 compile; the others print their own line")
     ap.add_argument("--variants", type=int, default=256, help="batch workload size")
     ap.add_argument("--sweep-rpt", type=int, default=0,
@@ -138,6 +142,8 @@ def main() -> int:
         return bench_grad(args)
     if args.workload == "serving":
         return bench_serving(args)
+    if args.workload == "long":
+        return bench_long(args)
 
     import numpy as np
 
@@ -316,6 +322,38 @@ def bench_grad(args) -> int:
            "errors": dd.num_errors, "forward_backward_us": us,
            "gradients_per_s": dd.num_locations / (us * 1e-6)}
     print(json.dumps(out), flush=True)
+    return 0
+
+
+def bench_long(args) -> int:
+    """Long memory experiments: device time of the compile with every
+    column swept whole and with the planned time-chunked sweep (same DEM)."""
+    from greenpeas_amd import autotune, get_driver
+    from greenpeas_amd.codes import BBCode, SurfaceCode
+
+    cases = [("surface d=7 x 100 rounds", SurfaceCode(7).memory_text(100, 1e-3)),
+             ("BB [[72,12,6]] x 60 rounds", BBCode(6).memory_text(60, 1e-3)),
+             ("BB [[144,12,12]] x 40 rounds", BBCode(12).memory_text(40, 1e-3))]
+    results = []
+    for name, text in cases:
+        row = {"circuit": name}
+        dems = []
+        for mode, key in ((0, "whole"), (1, "planned")):
+            autotune.set_tuning("sweep_split", mode)
+            d = get_driver(text)
+            d.compile_detector_error_model()
+            dems.append(d.compile_detector_error_model())  # the second compile runs the plan
+            d.bench_device(max(args.warmup, 1))
+            us = d.bench_device(max(args.steps // 4, 10))
+            row[key] = {"device_us": us, "sweep_us": d.timings()["sweep"], "plan": d.debug_sweep_plan()}
+            d.close()
+        autotune.set_tuning("sweep_split", 1)
+        row["same_dem"] = dems[0] == dems[1]
+        row["sweep_speedup"] = row["whole"]["sweep_us"] / row["planned"]["sweep_us"]
+        log(f"{name}: sweep {row['whole']['sweep_us']:.1f} -> {row['planned']['sweep_us']:.1f} us, "
+            f"device {row['whole']['device_us']:.1f} -> {row['planned']['device_us']:.1f} us")
+        results.append(row)
+    print(json.dumps({"metric": METRIC, "workload": "long", "results": results}), flush=True)
     return 0
 
 
