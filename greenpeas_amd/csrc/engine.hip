@@ -360,6 +360,9 @@ struct SweepState {
   uint32_t pos4[RPT];         // byte offsets of this thread's positions in a row or slot
 };
 
+// cache policy of a buffer store: sc0 | sc1 (system scope, written through)
+constexpr int kStoreThrough = 1 | 16;
+
 // LDS byte address of the ring word named by a 16-bit index.
 __device__ __forceinline__ uint32_t ringLo(uint32_t x) { return (x & 0xFFFFu) << 2; }
 __device__ __forceinline__ uint32_t ringHi(uint32_t x) { return (x >> 14) & 0x3FFFCu; }
@@ -404,8 +407,10 @@ __device__ __forceinline__ void sweepStep(const SweepState<RPT> &st, uint4 (&r)[
 #pragma unroll
   for (int k = 0; k < RPT; ++k) {
     *ldsAt(slotB + st.pos4[k]) = b[k];
-    __builtin_amdgcn_raw_buffer_store_b32(a[k], st.sc, st.pos4[k], oddB, 0);
-    __builtin_amdgcn_raw_buffer_store_b32(b[k], st.sc, st.pos4[k], evenB, 0);
+    // write-through (system scope): S leaves no dirty lines for the end of
+    // the kernel to write back (measured: 1 us less per compile on bb12)
+    __builtin_amdgcn_raw_buffer_store_b32(a[k], st.sc, st.pos4[k], oddB, kStoreThrough);
+    __builtin_amdgcn_raw_buffer_store_b32(b[k], st.sc, st.pos4[k], evenB, kStoreThrough);
     any |= a[k] | b[k];
   }
   loadOff -= st.blockBytes;
