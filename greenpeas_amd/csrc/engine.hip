@@ -844,15 +844,16 @@ __device__ __forceinline__ void buildMaskTable(uint32_t *tab, uint32_t l0, uint3
 
 // Calls f(target) for every set bit of one node's sensitivity row, in
 // ascending target order, visiting the columns set in the node's window
-// mask (`mrow`, maskWords words).  Columns are taken four at a time and all
-// their row words loaded before any is used, so a node waits for one memory
-// round trip per four columns.
-template <typename F>
+// mask (`mrow`, maskWords words).  Columns are taken kBatch at a time and
+// all their row words loaded before any is used, so a node waits for one
+// memory round trip per kBatch columns (16: one for nearly every node of a
+// circuit with ~15 columns per layer, at 117 VGPRs; 4 where nodes see few
+// columns).
+template <int kBatch, typename F>
 __device__ __forceinline__ void forEachTargetMasked(const uint32_t *mrow, uint32_t maskWords, const int2 *band,
                                                     const FrameRows &r, const uint32_t *__restrict__ S0,
                                                     uint64_t colStride, F &&f) {
   if (!r.ok[0] || !maskWords) return;
-  constexpr int kBatch = 4;
   uint32_t w = 0, m = mrow[0];
   for (;;) {
     uint32_t col[kBatch], x[kBatch][4];
@@ -944,7 +945,7 @@ struct RankIn {
   const uint16_t *probIdx;
 };
 
-template <bool TABLE>
+template <bool TABLE, int BATCH>
 __global__ __launch_bounds__(kClassBlock) void classifyKernel(
     const VDesc *__restrict__ desc, const uint4 *__restrict__ tiles, const int2 *__restrict__ band,
     const uint32_t *__restrict__ S, RankIn rin, ClassifyOut o) {
@@ -988,7 +989,7 @@ __global__ __launch_bounds__(kClassBlock) void classifyKernel(
       cnt++;
     };
     if constexpr (TABLE)
-      forEachTargetMasked(maskTab + (fr.span * nl + (fr.lo - l0)) * d.maskWords, d.maskWords, bandC, fr,
+      forEachTargetMasked<BATCH>(maskTab + (fr.span * nl + (fr.lo - l0)) * d.maskWords, d.maskWords, bandC, fr,
                           S + d.sOff, d.colStride, take);
     else
       forEachTargetScan(d.numCols, bandC, fr, S + d.sOff, d.colStride, take);
@@ -1679,6 +1680,9 @@ struct Pipeline::Impl {
   // (splitBuf: descriptors | launch-1 items | launch-2 items | per-column
   // split index; splitState: the split columns' state areas)
   bool splitPlanned = false;
+  // columns the class stage loads per round trip (4; 16 once a run has shown
+  // nodes see more than 12 columns on average, in loads of few workgroups)
+  uint32_t classBatch = 4;
   uint32_t nSplit = 0, nSplit1 = 0, nSplit3 = 0;
   size_t splitLdsBytes = 0, oSplitItems1 = 0, oSplitItems3 = 0;
   DevBuf splitBuf, splitState;
@@ -1759,6 +1763,7 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   m.sizesKnown = false;
   m.stashShift = 4;
   m.splitPlanned = false;
+  m.classBatch = 4;
   m.nSplit = m.nSplit1 = m.nSplit3 = 0;
   const bool frozen = m.frozen;
   const uint32_t V = static_cast<uint32_t>(imgs.size());
@@ -2058,12 +2063,15 @@ void Pipeline::run(bool timed) {
                    m.countStride, m.pslot.as<uint4>(), m.ovKey.as<uint64_t>(),
                    m.ovRank.as<uint32_t>(), m.ovBucket.as<uint32_t>(), m.flags(), m.stash.as<uint32_t>(),
                    stashMaxCols, m.stashShift};
-    if (m.classTable && gMaskTable.load())
-      classifyKernel<true><<<m.nTiles, kClassBlock, 0, st>>>(desc, tiles, m.band.as<int2>(), m.S.as<uint32_t>(),
-                                                             rin, co);
+    if (m.classTable && gMaskTable.load() && m.classBatch == 16)
+      classifyKernel<true, 16><<<m.nTiles, kClassBlock, 0, st>>>(desc, tiles, m.band.as<int2>(),
+                                                                 m.S.as<uint32_t>(), rin, co);
+    else if (m.classTable && gMaskTable.load())
+      classifyKernel<true, 4><<<m.nTiles, kClassBlock, 0, st>>>(desc, tiles, m.band.as<int2>(),
+                                                                m.S.as<uint32_t>(), rin, co);
     else
-      classifyKernel<false><<<m.nTiles, kClassBlock, 0, st>>>(desc, tiles, m.band.as<int2>(), m.S.as<uint32_t>(),
-                                                              rin, co);
+      classifyKernel<false, 4><<<m.nTiles, kClassBlock, 0, st>>>(desc, tiles, m.band.as<int2>(),
+                                                                 m.S.as<uint32_t>(), rin, co);
   }
   mark(2);
   RunOut o{m.info.as<uint4>(),
@@ -2111,6 +2119,7 @@ void Pipeline::run(bool timed) {
 constexpr int32_t kSplitMatvecPairs = 22, kSplitStagePairs = 10, kSplitLaunchPairs = 3;
 constexpr int32_t kSplitConcurrentPct = 170;
 constexpr uint32_t kSplitMaxColumns = 4;   // split columns per load
+constexpr uint32_t kClassWideMaxTiles = 2048;  // class workgroups up to which the 16-column walk is used
 constexpr uint32_t kSplitMaxPlain = 256;   // plain sweep items (more fill the chip anyway)
 
 static int32_t concurrentSteps(int32_t n) { return n * kSplitConcurrentPct / 100; }
@@ -2138,6 +2147,27 @@ static void planSweep(Pipeline::Impl &m) {
   };
   std::stable_sort(m.hostItems.begin(), m.hostItems.end(),
                    [&](const uint4 &a, const uint4 &b) { return bandLen(a) > bandLen(b); });
+  // columns a node's row window meets, on average over the error nodes: a
+  // column's band (plus the rows' span) over the circuit's layers
+  double seen = 0, nodes = 0;
+  for (uint32_t v = 0; v < m.V; ++v) {
+    const VDesc &d = m.descs[v];
+    double layers = 0;
+    for (uint32_t c = 0; c < d.numCols; ++c) {
+      const int2 b = bands[d.colOff + c];
+      if (b.y >= b.x) layers += std::min<double>(b.y - b.x + 1 + d.rowSpan, d.numLayers);
+    }
+    seen += d.numLayers ? layers / d.numLayers * d.numRanks : 0.0;
+    nodes += d.numRanks;
+  }
+  // (only while the class stage is latency-bound: a few workgroups per
+  // compute unit; large batches need the occupancy of the narrow walk,
+  // measured: 256 variants 3.78 ms with 4, 4.04 ms with 16)
+  const uint32_t batch = nodes > 0 && seen / nodes > 12.0 && m.nTiles <= kClassWideMaxTiles ? 16u : 4u;
+  if (batch != m.classBatch) {
+    m.classBatch = batch;
+    m.dropGraph();
+  }
   auto writeItems = [&]() {
     std::memcpy(m.inHost.as<char>() + m.oItems, m.hostItems.data(), 16 * m.hostItems.size());
     GPA_HIP(hipMemcpy(m.in.as<char>() + m.oItems, m.hostItems.data(), 16 * m.hostItems.size(),

@@ -290,8 +290,9 @@ def test_dem_views_outlive_later_compiles_and_the_driver():
 def test_repeated_compiles_with_narrowed_target_rows(oracle, name):
     """After a load's first compile the stored target rows shrink to the
     narrowest width that holds its largest class (surface5: 4 words;
-    bb12's 18-target classes keep 16); every compile gives the same DEM and
-    order as the STEPG restatement."""
+    bb12's 18-target classes keep 16) and the class stage may switch to 16
+    columns per round trip (bb12: ~15 columns per node); every compile
+    gives the same DEM and order as the STEPG restatement."""
     text = read_circuit(name)
     ref = oracle.compile_stepg(text, 2)
     drv = get_driver(text)
