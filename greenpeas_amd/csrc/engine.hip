@@ -112,7 +112,9 @@ __device__ __forceinline__ uint64_t mix2(uint64_t h, uint32_t t) {
 
 // Bitonic sort of 64*R (hash, rank) keys held R per lane; element index
 // e = s * 64 + lane.  Distances below 64 exchange through lane shuffles,
-// larger ones inside a lane's registers.
+// larger ones inside a lane's registers.  (Measured on bb12: DPP row shifts
+// and gfx950's permlane swaps in place of the shuffles cost 1.4 us more --
+// the extra VALU work outweighs the LDS round trips at 4 waves per SIMD.)
 template <int R>
 __device__ __forceinline__ void waveBitonicSort(uint64_t (&h)[R], uint32_t (&r)[R], uint32_t lane) {
   constexpr uint32_t N = 64 * R;
@@ -214,13 +216,21 @@ __device__ __forceinline__ void scanRuns(const uint64_t (&h)[R], const uint32_t 
     const bool hasPrev = (s > 0 || lane > 0) ? true : (cy.havePrev != 0);
     f[s] = e < cnt && (!hasPrev || h[s] != ph);
   }
+  // every slot's node records first: loads issued after this wave's
+  // stores would wait for them (memory operations retire in order)
+  uint4 infs[R];
+#pragma unroll
+  for (int s = 0; s < R; ++s) {
+    const uint32_t e = static_cast<uint32_t>(s) * 64 + lane;
+    infs[s] = e < cnt ? o.info[rk[s]] : make_uint4(0u, 0u, 0u, 0u);
+  }
 #pragma unroll
   for (int s = 0; s < R; ++s) {
     const uint32_t e = static_cast<uint32_t>(s) * 64 + lane;
     if (static_cast<uint32_t>(s) * 64 >= cnt) break;
     const bool valid = e < cnt;
     const uint32_t r = rk[s];
-    const uint4 inf = valid ? o.info[r] : make_uint4(0u, 0u, 0u, 0u);
+    const uint4 inf = infs[s];
     const uint32_t ownH2 = inf.z;
     double v = infoProb(inf);
     uint32_t hf = ownH2;
