@@ -24,7 +24,8 @@ def solo(text):
 
 
 def test_batch_equals_solo_and_oracle(oracle):
-    texts = [read_circuit(n) for n in ("surface3", "bb6", "surface5", "surface3", "bb12")]
+    texts = [read_circuit(n) for n in ("surface3", "bb6", "surface5", "surface3", "bb12",
+                                       "iceberg_surface_6_static_circl", "iceberg_surface_4_adaptive_circl")]
     texts.append(SurfaceCode(7).memory_text(9, noise=Noise(rng=np.random.default_rng(4))))
     bc = BatchCompiler()
     bc.set_circuits(texts)

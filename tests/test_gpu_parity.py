@@ -38,7 +38,9 @@ def test_compile_matches_golden_dem(name, level):
     assert np.max(np.abs(dem.probs - gold.probs) / gold.probs) <= REL
 
 
-@pytest.mark.parametrize("name", ["surface3", "surface5", "bb6", "bb12"])
+@pytest.mark.parametrize("name", ["surface3", "surface5", "bb6", "bb12", "iceberg_surface_4_static_circl",
+                                  "iceberg_surface_4_static_pheno", "iceberg_surface_6_static_circl",
+                                  "iceberg_surface_6_static_pheno"])
 def test_compile_matches_oracles(oracle, name):
     text = read_circuit(name)
     drv = get_driver(text)
@@ -378,3 +380,13 @@ def test_split_sweep_repeated_compiles_and_reload(sweep_split):
     assert drv.compile_detector_error_model(b) == small
     assert drv.compile_detector_error_model(a) == first
     assert drv.compile_detector_error_model(a) == first
+
+
+@pytest.mark.parametrize("name", ["iceberg_surface_4_adaptive_circl", "iceberg_surface_6_adaptive_pheno"])
+def test_noiseless_adaptive_iceberg_circuits(name):
+    """p = 0 circuits of the reference's concatenated code: an empty DEM with
+    the circuit's detector and observable counts."""
+    text = read_circuit(name)
+    dem = get_driver(text).compile_detector_error_model(text)
+    assert dem.num_errors == 0
+    assert dem.num_detectors == text.count("DETECTOR") and dem.num_observables == 1

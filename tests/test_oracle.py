@@ -51,7 +51,13 @@ def test_golden_check_discriminates_levels(oracle):
     assert not d.approx_equals(g)
 
 
-@pytest.mark.parametrize("name", ["surface3", "surface5", "bb6", "bb12"])
+ICEBERG_STATIC = ["iceberg_surface_4_static_circl", "iceberg_surface_4_static_pheno",
+                  "iceberg_surface_6_static_circl", "iceberg_surface_6_static_pheno"]
+ICEBERG_ADAPTIVE = ["iceberg_surface_4_adaptive_circl", "iceberg_surface_4_adaptive_pheno",
+                    "iceberg_surface_6_adaptive_circl", "iceberg_surface_6_adaptive_pheno"]
+
+
+@pytest.mark.parametrize("name", ["surface3", "surface5", "bb6", "bb12"] + ICEBERG_STATIC)
 def test_stepg_oracle_matches_frame_oracle(oracle, name):
     text = read_circuit(name)
     a = as_dem(oracle.compile_stepg(text, 2))
@@ -117,7 +123,8 @@ def _ref_lib():
     return L
 
 
-@pytest.mark.parametrize("name", ["surface3", "surface5", "bb6"])
+@pytest.mark.parametrize("name", ["surface3", "surface5", "bb6", "iceberg_surface_4_static_circl",
+                                  "iceberg_surface_6_static_circl"])
 @pytest.mark.parametrize("level", [0, 1, 2])
 def test_oracle_stepg_matches_reference_headers(oracle, name, level):
     L = _ref_lib()
@@ -241,3 +248,14 @@ def test_oracle_gradient_matches_central_differences(oracle):
         # truncation O(h^2) is far below the roundoff of L divided by h
         assert abs(fd - g[k]) <= 1e-6 * abs(g[k]) + noise / h, (k, fd, g[k])
         assert abs(fd - g[k]) <= 1e-3 * abs(g[k]) + 1e-9  # discriminating bound
+
+
+@pytest.mark.parametrize("name", ICEBERG_ADAPTIVE)
+def test_noiseless_adaptive_iceberg_circuits(oracle, name):
+    """The reference's adaptive concatenated-code circuits are generated at
+    p = 0 (ConcatenatedSurfaceCodeTest.cpp:59-68): both compilers give an
+    empty model with the circuit's detectors and observable."""
+    text = read_circuit(name)
+    for d in (as_dem(oracle.compile_stepg(text, 2)), as_dem(oracle.compile_frames(text))):
+        assert d.num_errors == 0
+        assert d.num_detectors == text.count("DETECTOR") and d.num_observables == 1
