@@ -369,7 +369,8 @@ def bench_serving(args) -> int:
     for rate in (100, 500, 1000, 2000, 5000):
         n = min(400, max(60, rate // 5))
         arrivals = poisson_arrivals(rate, n, seed=rate)
-        srv = Server(slots=2, max_batch=64)
+        # warmed with the pool's largest circuit (buffers sized, kernels loaded)
+        srv = Server(slots=2, max_batch=64, warmup=[max(pool, key=len)])
         futs = []
         t0 = time.perf_counter()
         for i, at in enumerate(arrivals):

@@ -18,6 +18,10 @@ whole budget is rejected with `MemoryError` on submission.
 Per-request results are independent of what they are batched with: each
 circuit owns its node range, column work items and sort buckets, and the
 merge order within a circuit does not depend on the batch.
+
+`warmup` circuits (e.g. the largest request expected) are compiled by every
+slot before the server takes requests, so device buffers are sized and the
+kernels loaded before the first request's latency is counted.
 """
 
 from __future__ import annotations
@@ -75,7 +79,7 @@ class Server:
 
     def __init__(self, correlation_level: CorrelationLevel = CorrelationLevel.L2, device: int = 0,
                  slots: int = 2, max_batch: int = 64, hbm_budget: int = 64 << 30,
-                 parse_threads: int = 8, compiler_factory=None):
+                 parse_threads: int = 8, compiler_factory=None, warmup=()):
         self.level = int(correlation_level)
         self.slots = slots
         self.max_batch = max_batch
@@ -93,10 +97,17 @@ class Server:
         self._closed = False
         self.completed: list[Request] = []
         self.batches: list[tuple[int, int, float]] = []  # (slot, size, device us)
+        self._warmup = [circuit_text(c) for c in warmup]
+        self._warmup_errors: list[BaseException] = []
+        self._ready = threading.Barrier(slots + 1)
         self._workers = [threading.Thread(target=self._worker, args=(i,), daemon=True)
                          for i in range(slots)]
         for w in self._workers:
             w.start()
+        self._ready.wait()  # every slot warmed up (or failed to)
+        if self._warmup_errors:
+            self.close()
+            raise RuntimeError("server warm-up failed") from self._warmup_errors[0]
 
     # ------------------------------------------------------------------
     def submit(self, circuit) -> Future:
@@ -125,7 +136,17 @@ class Server:
         return batch
 
     def _worker(self, slot: int) -> None:
-        compiler = self._factory()
+        try:
+            compiler = self._factory()
+            if self._warmup:
+                compiler.set_circuits(self._warmup, threads=self.parse_threads)
+                compiler.compile()
+        except Exception as e:  # noqa: BLE001 -- reported by the constructor
+            with self._cv:
+                self._warmup_errors.append(e)
+            self._ready.wait()
+            return
+        self._ready.wait()
         while True:
             with self._cv:
                 while not self._closed and not self._queue:

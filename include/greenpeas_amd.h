@@ -169,8 +169,9 @@ int gp_batch_destroy(gp_batch *batch);
 int gp_batch_set_circuits(gp_batch *batch, const char *const *circuit_texts,
                           uint32_t count, uint32_t threads);
 /* Compiles all circuits of the batch on the GPU.  mode 0: one batched
- * pipeline for all circuits (captured in a HIP graph and replayed when the
- * batch is unchanged); mode 1: the circuits one after another through the
+ * pipeline for all circuits (the first compile of a new set of circuits
+ * runs it directly; compiling the same set again captures it in a HIP graph
+ * and replays it); mode 1: the circuits one after another through the
  * same kernels (the one-by-one baseline). */
 int gp_batch_compile(gp_batch *batch, int mode);
 /* Runs the loaded batch once more with HIP events between the stages and

@@ -107,3 +107,17 @@ def test_circuit_summary_matches_the_parser():
         lines = [ln for ln in text.splitlines() if ln.startswith("OBSERVABLE_INCLUDE(")]
         assert obs == max(int(ln.split("(")[1].split(")")[0]) for ln in lines) + 1
     assert serving.circuit_summary("X_ERROR(0.1) 7\nMPP X2*Z9\n# 100\n")[1] == 10
+
+
+def test_warmup_runs_on_every_slot_before_requests():
+    FakeCompiler.log = []
+    srv = serving.Server(slots=3, compiler_factory=FakeCompiler, warmup=["R 0\nTICK\nM 0\n"])
+    # the constructor returns once every slot has compiled the warm-up batch
+    assert FakeCompiler.log == [["R 0\nTICK\nM 0\n"]] * 3
+    assert srv.submit("R 1\nTICK\nM 1\n").result(timeout=10) == len("R 1\nTICK\nM 1\n")
+    srv.close()
+
+
+def test_warmup_failure_is_raised_by_the_constructor():
+    with pytest.raises(RuntimeError, match="warm-up"):
+        serving.Server(slots=2, compiler_factory=FakeCompiler, warmup=["FAIL"])
