@@ -30,7 +30,7 @@ EXPORTS = (
     "gp_estimate_device_bytes", "gp_circuit_summary", "gp_set_tuning", "gp_get_tuning",
     "gp_grad_plan_create", "gp_grad_plan_destroy", "gp_grad_plan_sizes", "gp_grad_plan_dem_structure",
     "gp_grad_plan_device_arrays", "gp_grad_plan_copy_arrays", "gp_xor_merge_forward",
-    "gp_xor_merge_backward",
+    "gp_xor_merge_backward", "gp_node_rates_forward", "gp_node_rates_backward",
 )
 
 _lib = None
@@ -115,6 +115,8 @@ def load() -> ctypes.CDLL:
     sig("gp_grad_plan_copy_arrays", [P, P, P, P, P, P, P])
     sig("gp_xor_merge_forward", [P, P, P, u32, P, P])
     sig("gp_xor_merge_backward", [P, P, P, u32, P, P, P])
+    sig("gp_node_rates_forward", [P, P, P, P, P, u32, P, P])
+    sig("gp_node_rates_backward", [P, P, P, P, P, u32, P, P, P, P, u32, P, P])
     _lib = L
     return L
 

@@ -67,6 +67,43 @@ class XorMerge(torch.autograd.Function):
 xor_merge = XorMerge.apply
 
 
+class NodeRates(torch.autograd.Function):
+    """q[n] = XOR-merge of the split rates of node n's components, from the
+    location rates theta, in one kernel (gp_node_rates_forward); backward in
+    two (per-node products, then one ordered sum per location)."""
+
+    @staticmethod
+    def forward(ctx, theta: torch.Tensor, plan: "DifferentiableDem") -> torch.Tensor:
+        if theta.dtype != torch.float64 or not theta.is_cuda:
+            raise TypeError("theta must be a float64 CUDA tensor")
+        theta = theta.contiguous()
+        q = torch.empty(plan.num_nodes, dtype=torch.float64, device=theta.device)
+        lib = _native.load()
+        _native.check(lib.gp_node_rates_forward(theta.data_ptr(), plan.comp_loc32.data_ptr(),
+                                                plan.comp_kind8.data_ptr(), plan.node_start.data_ptr(),
+                                                plan.node_comp.data_ptr(), plan.num_nodes, q.data_ptr(),
+                                                _stream_ptr(theta.device)))
+        ctx.save_for_backward(theta)
+        ctx.plan = plan
+        return q
+
+    @staticmethod
+    def backward(ctx, grad_q: torch.Tensor):
+        (theta,) = ctx.saved_tensors
+        plan = ctx.plan
+        g = grad_q.contiguous().to(torch.float64)
+        grad_comp = torch.empty(max(plan.num_components, 1), dtype=torch.float64, device=theta.device)
+        grad_theta = torch.empty_like(theta)
+        lib = _native.load()
+        _native.check(lib.gp_node_rates_backward(theta.data_ptr(), plan.comp_loc32.data_ptr(),
+                                                 plan.comp_kind8.data_ptr(), plan.node_start.data_ptr(),
+                                                 plan.node_comp.data_ptr(), plan.num_nodes, g.data_ptr(),
+                                                 grad_comp.data_ptr(), plan.loc_start.data_ptr(),
+                                                 plan.loc_comp.data_ptr(), plan.num_locations,
+                                                 grad_theta.data_ptr(), _stream_ptr(theta.device)))
+        return grad_theta, None
+
+
 class DifferentiableDem:
     """Structure of a circuit's DEM, compiled once on the GPU; probabilities
     (and their gradients) for any per-location rates theta.
@@ -114,19 +151,31 @@ class DifferentiableDem:
         self.node_start = torch.from_numpy(node_start.view(np.int32)).to(dev)
         self.class_node = torch.from_numpy(class_node[:R].view(np.int32)).to(dev)
         self.class_start = torch.from_numpy(class_start.view(np.int32)).to(dev)
+        # the fused kernels' inputs: components' locations and kinds, and
+        # each location's components (ascending), a CSR over locations
+        self.comp_loc32 = torch.from_numpy(comp_loc[:max(K, 1)].view(np.int32)).to(dev)
+        self.comp_kind8 = torch.from_numpy(comp_kind[:max(K, 1)].copy()).to(dev)
+        order = np.argsort(comp_loc[:K], kind="stable").astype(np.uint32)
+        loc_start = np.zeros(L + 1, np.uint32)
+        np.cumsum(np.bincount(comp_loc[:K], minlength=L)[:L], out=loc_start[1:])
+        self.loc_start = torch.from_numpy(loc_start.view(np.int32)).to(dev)
+        self.loc_comp = torch.from_numpy(order[:max(K, 1)].view(np.int32) if K else np.zeros(1, np.int32)).to(dev)
         self.offsets = offsets
         self.targets = targets[: int(offsets[-1])]
         self.num_detectors = None
         self.num_observables = None
 
     def component_rates(self, theta: torch.Tensor) -> torch.Tensor:
+        """The components' split rates as plain tensor operations (the
+        reference formula the fused kernels are tested against)."""
         t = theta[self.comp_loc]
         s1 = 0.5 - 0.5 * torch.sqrt(torch.clamp(1 - 4 * t / 3, min=1e-300))
         s2 = 0.5 - 0.5 * torch.pow(torch.clamp(1 - 16 * t / 15, min=1e-300), 0.125)
         return torch.where(self.is_dep1, s1, torch.where(self.is_dep2, s2, t))
 
     def node_probabilities(self, theta: torch.Tensor) -> torch.Tensor:
-        return xor_merge(self.component_rates(theta), self.node_comp, self.node_start)
+        """q_n(theta) for every error node, fused on the GPU (NodeRates)."""
+        return NodeRates.apply(theta, self)
 
     def probabilities(self, theta: torch.Tensor) -> torch.Tensor:
         """DEM error probabilities p_e(theta), in the DEM's output order."""

@@ -755,6 +755,29 @@ int gp_xor_merge_forward(const double *values, const uint32_t *index, const uint
   });
 }
 
+int gp_node_rates_forward(const double *theta, const uint32_t *comp_loc, const uint8_t *comp_kind,
+                          const uint32_t *node_start, const uint32_t *node_comp, uint32_t num_nodes, double *q,
+                          void *stream) {
+  return guarded([&] {
+    if (num_nodes && (!theta || !comp_loc || !comp_kind || !node_start || !node_comp || !q))
+      throw InvalidArgument("NULL argument");
+    gpa::nodeRatesForward(theta, comp_loc, comp_kind, node_start, node_comp, num_nodes, q, stream);
+  });
+}
+
+int gp_node_rates_backward(const double *theta, const uint32_t *comp_loc, const uint8_t *comp_kind,
+                           const uint32_t *node_start, const uint32_t *node_comp, uint32_t num_nodes,
+                           const double *grad_q, double *grad_comp, const uint32_t *loc_start,
+                           const uint32_t *loc_comp, uint32_t num_locations, double *grad_theta, void *stream) {
+  return guarded([&] {
+    if ((num_nodes && (!theta || !comp_loc || !comp_kind || !node_start || !node_comp || !grad_q || !grad_comp)) ||
+        (num_locations && (!loc_start || !loc_comp || !grad_comp || !grad_theta)))
+      throw InvalidArgument("NULL argument");
+    gpa::nodeRatesBackward(theta, comp_loc, comp_kind, node_start, node_comp, num_nodes, grad_q, grad_comp,
+                           loc_start, loc_comp, num_locations, grad_theta, stream);
+  });
+}
+
 int gp_xor_merge_backward(const double *values, const uint32_t *index, const uint32_t *seg_start,
                           uint32_t num_segments, const double *grad_out, double *grad_values,
                           void *stream) {

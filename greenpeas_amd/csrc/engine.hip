@@ -1513,6 +1513,87 @@ __global__ void xorMergeBwdKernel(const double *__restrict__ values, const uint3
   }
 }
 
+// Rate of one fault component from its location's rate t, and d rate / d t
+// (Mixer.hpp:164, 184): kind 0 the rate itself; 1 one of DEPOLARIZE1's
+// three Paulis, 1/2 - sqrt(1 - 4t/3)/2; 2 one of DEPOLARIZE2's fifteen,
+// 1/2 - (1 - 16t/15)^(1/8)/2.  The root's argument is clamped at 1e-300
+// (zero derivative there), as the plain tensor formula does.  The formula
+// is the reference's (its 1/2 - x/2 cancels for small rates, ~1e-11
+// relative at t = 1e-4) and is evaluated operation by operation, without
+// fused multiply-adds, so the rates round as the compile path's do.
+__device__ __forceinline__ double componentRate(double t, uint32_t kind, double &dr) {
+  if (kind == 1) {
+    const double u = __dsub_rn(1.0, __ddiv_rn(__dmul_rn(4.0, t), 3.0));
+    const double s = sqrt(u > 1e-300 ? u : 1e-300);
+    dr = u > 1e-300 ? 1.0 / (3.0 * s) : 0.0;
+    return __dsub_rn(0.5, __dmul_rn(0.5, s));
+  }
+  if (kind == 2) {
+    const double u = __dsub_rn(1.0, __ddiv_rn(__dmul_rn(16.0, t), 15.0));
+    const double uc = u > 1e-300 ? u : 1e-300;
+    const double s = pow(uc, 0.125);
+    dr = u > 1e-300 ? s / (15.0 * uc) : 0.0;
+    return __dsub_rn(0.5, __dmul_rn(0.5, s));
+  }
+  dr = 1.0;
+  return t;
+}
+
+// q[n] = XOR-merge over node n's components c (nodeComp[nodeStart[n] ..
+// nodeStart[n+1])) of componentRate(theta[compLoc[c]], compKind[c]).
+__global__ void nodeRatesFwdKernel(const double *__restrict__ theta, const uint32_t *__restrict__ compLoc,
+                                   const uint8_t *__restrict__ compKind, const uint32_t *__restrict__ nodeStart,
+                                   const uint32_t *__restrict__ nodeComp, uint32_t numNodes, double *__restrict__ q) {
+  const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= numNodes) return;
+  double p = 0.0, dr;
+  for (uint32_t k = nodeStart[n]; k < nodeStart[n + 1]; ++k) {
+    const uint32_t c = nodeComp[k];
+    p = xmerge(p, componentRate(theta[compLoc[c]], compKind[c], dr));
+  }
+  q[n] = p;
+}
+
+// gradComp[c] = dL/dq[n] * d q[n] / d rate_c * d rate_c / d t for every
+// component c of node n (each component belongs to one node): d q / d
+// rate_c = prod over the node's other components of (1 - 2 rate).
+__global__ void nodeRatesBwdKernel(const double *__restrict__ theta, const uint32_t *__restrict__ compLoc,
+                                   const uint8_t *__restrict__ compKind, const uint32_t *__restrict__ nodeStart,
+                                   const uint32_t *__restrict__ nodeComp, uint32_t numNodes,
+                                   const double *__restrict__ gradQ, double *__restrict__ gradComp) {
+  const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= numNodes) return;
+  const uint32_t a = nodeStart[n], e = nodeStart[n + 1];
+  const double g = gradQ[n];
+  double pre = 1.0, dr;
+  for (uint32_t k = a; k < e; ++k) {
+    const uint32_t c = nodeComp[k];
+    const double r = componentRate(theta[compLoc[c]], compKind[c], dr);
+    gradComp[c] = pre * dr;
+    pre *= 1.0 - 2.0 * r;
+  }
+  double suf = 1.0;
+  for (uint32_t k = e; k > a; --k) {
+    const uint32_t c = nodeComp[k - 1];
+    const double r = componentRate(theta[compLoc[c]], compKind[c], dr);
+    gradComp[c] = g * gradComp[c] * suf;
+    suf *= 1.0 - 2.0 * r;
+  }
+}
+
+// gradTheta[l] = sum of gradComp over location l's components (locComp[
+// locStart[l] .. locStart[l+1]), ascending): one deterministic sum per
+// location, no atomics.
+__global__ void locationGradKernel(const uint32_t *__restrict__ locStart, const uint32_t *__restrict__ locComp,
+                                   const double *__restrict__ gradComp, uint32_t numLocations,
+                                   double *__restrict__ gradTheta) {
+  const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= numLocations) return;
+  double s = 0.0;
+  for (uint32_t k = locStart[l]; k < locStart[l + 1]; ++k) s += gradComp[locComp[k]];
+  gradTheta[l] = s;
+}
+
 // --------------------------------------------------------------------------
 // host-side buffers
 // --------------------------------------------------------------------------
@@ -2604,6 +2685,29 @@ void xorMergeForward(const double *values, const uint32_t *index, const uint32_t
   if (!numSegments) return;
   xorMergeFwdKernel<<<(numSegments + 255) / 256, 256, 0, static_cast<hipStream_t>(stream)>>>(
       values, index, segStart, numSegments, out);
+  GPA_HIP(hipGetLastError());
+}
+
+void nodeRatesForward(const double *theta, const uint32_t *compLoc, const uint8_t *compKind,
+                      const uint32_t *nodeStart, const uint32_t *nodeComp, uint32_t numNodes, double *q,
+                      void *stream) {
+  if (!numNodes) return;
+  nodeRatesFwdKernel<<<(numNodes + 255) / 256, 256, 0, static_cast<hipStream_t>(stream)>>>(
+      theta, compLoc, compKind, nodeStart, nodeComp, numNodes, q);
+  GPA_HIP(hipGetLastError());
+}
+
+void nodeRatesBackward(const double *theta, const uint32_t *compLoc, const uint8_t *compKind,
+                       const uint32_t *nodeStart, const uint32_t *nodeComp, uint32_t numNodes, const double *gradQ,
+                       double *gradComp, const uint32_t *locStart, const uint32_t *locComp, uint32_t numLocations,
+                       double *gradTheta, void *stream) {
+  const hipStream_t st = static_cast<hipStream_t>(stream);
+  if (numNodes)
+    nodeRatesBwdKernel<<<(numNodes + 255) / 256, 256, 0, st>>>(theta, compLoc, compKind, nodeStart, nodeComp,
+                                                               numNodes, gradQ, gradComp);
+  if (numLocations)
+    locationGradKernel<<<(numLocations + 255) / 256, 256, 0, st>>>(locStart, locComp, gradComp, numLocations,
+                                                                   gradTheta);
   GPA_HIP(hipGetLastError());
 }
 

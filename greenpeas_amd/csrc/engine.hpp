@@ -150,6 +150,17 @@ uint32_t sweepSplit();
 
 // Segmented XOR-merge kernels (differentiable path); pointers are device
 // pointers, stream is a hipStream_t.
+// Node probabilities from location rates (the split components of each
+// node XOR-merged) and their backward: per-component gradients (scratch,
+// one per component), then one sum per location.  Device pointers; stream
+// is a hipStream_t.
+void nodeRatesForward(const double *theta, const uint32_t *compLoc, const uint8_t *compKind,
+                      const uint32_t *nodeStart, const uint32_t *nodeComp, uint32_t numNodes, double *q,
+                      void *stream);
+void nodeRatesBackward(const double *theta, const uint32_t *compLoc, const uint8_t *compKind,
+                       const uint32_t *nodeStart, const uint32_t *nodeComp, uint32_t numNodes, const double *gradQ,
+                       double *gradComp, const uint32_t *locStart, const uint32_t *locComp, uint32_t numLocations,
+                       double *gradTheta, void *stream);
 void xorMergeForward(const double *values, const uint32_t *index, const uint32_t *segStart,
                      uint32_t numSegments, double *out, void *stream);
 void xorMergeBackward(const double *values, const uint32_t *index, const uint32_t *segStart,

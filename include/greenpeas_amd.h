@@ -282,6 +282,29 @@ int gp_xor_merge_backward(const double *values, const uint32_t *index,
                           const double *grad_out, double *grad_values,
                           void *stream);
 
+/* Node probabilities from fault-location rates, fused: q[n] = XOR-merge over
+ * node n's components c = node_comp[node_start[n] .. node_start[n+1]) of the
+ * component's split rate of theta[comp_loc[c]] (comp_kind 0: the rate; 1:
+ * 1/2 - sqrt(1 - 4t/3)/2, a DEPOLARIZE1 Pauli; 2: 1/2 - (1 - 16t/15)^(1/8)/2,
+ * a DEPOLARIZE2 pair -- Mixer.hpp:164, 184).  The arrays are a grad plan's
+ * (gp_grad_plan_copy_arrays).  Replaces the reference's host Mixer splits and
+ * STEPG::mergeProbabilities (STEPG.hpp:120-131) for differentiable use. */
+int gp_node_rates_forward(const double *theta, const uint32_t *comp_loc,
+                          const uint8_t *comp_kind, const uint32_t *node_start,
+                          const uint32_t *node_comp, uint32_t num_nodes,
+                          double *q, void *stream);
+/* Backward: grad_comp[c] (scratch, one per component) = grad_q[n] * dq[n] /
+ * d rate_c * d rate_c / d t, then grad_theta[l] = sum of grad_comp over
+ * location l's components loc_comp[loc_start[l] .. loc_start[l+1]) (a CSR
+ * the caller builds from comp_loc; one ordered sum per location). */
+int gp_node_rates_backward(const double *theta, const uint32_t *comp_loc,
+                           const uint8_t *comp_kind, const uint32_t *node_start,
+                           const uint32_t *node_comp, uint32_t num_nodes,
+                           const double *grad_q, double *grad_comp,
+                           const uint32_t *loc_start, const uint32_t *loc_comp,
+                           uint32_t num_locations, double *grad_theta,
+                           void *stream);
+
 #ifdef __cplusplus
 }
 #endif

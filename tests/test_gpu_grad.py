@@ -98,3 +98,29 @@ def test_bb12_gradient(ag, oracle):
             tm[k] -= h
             fd = (ag.logit_loss(dd.probabilities(tp), w) - ag.logit_loss(dd.probabilities(tm), w)).item() / (2 * h)
             assert abs(fd - g[k]) <= 1e-6 * abs(g[k]) + noise / h, (k, fd, g[k])
+
+
+@pytest.mark.parametrize("name", ["surface5", "bb12"])
+def test_fused_node_rates_match_tensor_formula(ag, name):
+    """The fused node-probability kernels (split rates + XOR-merge in one
+    forward, per-node products + per-location sums backward) against the
+    same function written as plain float64 tensor operations."""
+    dd = ag.DifferentiableDem(read_circuit(name))
+    rng = np.random.default_rng(5)
+    theta0 = np.exp(rng.uniform(np.log(1e-4), np.log(5e-2), dd.num_locations))
+    w = torch.tensor(rng.standard_normal(dd.num_nodes), device="cuda")
+    t1 = torch.tensor(theta0, device="cuda", requires_grad=True)
+    q1 = dd.node_probabilities(t1)
+    (q1 * w).sum().backward()
+    t2 = torch.tensor(theta0, device="cuda", requires_grad=True)
+    q2 = ag.xor_merge(dd.component_rates(t2), dd.node_comp, dd.node_start)
+    (q2 * w).sum().backward()
+    # the split rate 1/2 - x/2 cancels for small rates: a last-bit difference
+    # in the root becomes up to ~1e-11 relative at theta = 1e-4, so that is
+    # the bar (the kernels evaluate the same formula without contractions)
+    assert torch.allclose(q1, q2, rtol=2e-11, atol=0)
+    assert torch.allclose(t1.grad, t2.grad, rtol=1e-10, atol=1e-300)
+    # deterministic: the same gradient bits on a second run
+    t3 = torch.tensor(theta0, device="cuda", requires_grad=True)
+    (dd.node_probabilities(t3) * w).sum().backward()
+    assert torch.equal(t1.grad, t3.grad)
