@@ -332,6 +332,7 @@ def bench_long(args) -> int:
     from greenpeas_amd.codes import BBCode, SurfaceCode
 
     cases = [("surface d=7 x 100 rounds", SurfaceCode(7).memory_text(100, 1e-3)),
+             ("surface d=11 x 200 rounds", SurfaceCode(11).memory_text(200, 1e-3)),
              ("BB [[72,12,6]] x 60 rounds", BBCode(6).memory_text(60, 1e-3)),
              ("BB [[144,12,12]] x 40 rounds", BBCode(12).memory_text(40, 1e-3))]
     results = []

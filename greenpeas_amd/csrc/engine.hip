@@ -361,6 +361,7 @@ struct SweepState {
   __amdgpu_buffer_rsrc_t ac;  // the circuit's record blocks
   __amdgpu_buffer_rsrc_t sc;  // the column of S
   const uint32_t *gseeds;     // the column's seeds in global memory (GSEEDS)
+  bool seeds;                 // false: the block sweeps with no seeds (a transfer map's)
   uint32_t blockBytes;        // 16 * (layerStride + 1)
   uint32_t headOff;           // 16 * layerStride: the header within a block
   uint32_t zeroB, seedB;      // LDS byte addresses of the zero word and the first seed
@@ -396,7 +397,7 @@ __device__ __forceinline__ void sweepStep(const SweepState<RPT> &st, uint4 (&r)[
   bool seeded = false;
 #pragma unroll
   for (int k = 0; k < RPT; ++k) seeded |= r[k].w != 0xFFFFFFFFu;
-  if (__ballot(seeded) != 0ull) {
+  if (st.seeds && __ballot(seeded) != 0ull) {
     const uint32_t first = __builtin_amdgcn_readfirstlane(head);
 #pragma unroll
     for (int k = 0; k < RPT; ++k) {
@@ -440,7 +441,8 @@ __device__ __forceinline__ void sweepStep(const SweepState<RPT> &st, uint4 (&r)[
 //             its seeds, no S stores);
 //   launch 2  chunks 0..K-2, each from its entering state, formed as
 //             s = T_j s ^ p_j for j = K-2 .. k+1 starting from the top
-//             chunk's state out, storing S as the plain sweep does.
+//             chunk's state out, storing S as the plain sweep does (seeds
+//             read from global memory, leaving LDS to the chain).
 // The chunks' lengths are multiples of 4 pairs (whole trips).  State area
 // of a split column (words, P = layerStride, G = ceil(P/32)): top state out
 // [P], then per middle chunk T_k [G][P] (word g*P + i: bit b = bit of
@@ -693,6 +695,7 @@ __global__ __launch_bounds__(kSweepMaxThreads) void sweepKernel(
     st.sc = __builtin_amdgcn_make_buffer_rsrc(S + d.sOff + static_cast<uint64_t>(c) * d.colStride, 0,
                                               stores ? static_cast<uint32_t>(4 * d.colStride) : 0u, 0x00020000);
     st.gseeds = sv;
+    st.seeds = seeds;
     st.blockBytes = 16 * (P + 1);
     st.headOff = 16 * P;
     st.zeroB = 8 * P;
@@ -1943,12 +1946,12 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   // longest bands first (they bound the sweep's critical path)
   std::stable_sort(order.begin(), order.end(),
                    [](const auto &a, const auto &b) { return a.first > b.first; });
-  std::vector<uint4> &sortedItems = m.hostItems;
-  sortedItems.resize(items.size());
-  for (size_t k = 0; k < order.size(); ++k) sortedItems[k] = items[order[k].second];
+  std::vector<uint4> &launchItems = m.hostItems;
+  launchItems.resize(items.size());
+  for (size_t k = 0; k < order.size(); ++k) launchItems[k] = items[order[k].second];
 
   m.V = V;
-  m.nItems = static_cast<uint32_t>(sortedItems.size());
+  m.nItems = static_cast<uint32_t>(launchItems.size());
   m.nTiles = static_cast<uint32_t>(tiles.size());
   m.classTable = classTable;
   m.targets16 = m.narrowTargets;
@@ -1980,7 +1983,7 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   m.inHost.ensure(m.inBytes);
   char *hb = m.inHost.as<char>();
   std::memcpy(hb + m.oDesc, m.descs.data(), sizeof(VDesc) * V);
-  std::memcpy(hb + m.oItems, sortedItems.data(), sizeof(uint4) * m.nItems);
+  std::memcpy(hb + m.oItems, launchItems.data(), sizeof(uint4) * m.nItems);
   std::memcpy(hb + m.oTiles, tiles.data(), sizeof(uint4) * m.nTiles);
   for (uint32_t v = 0; v < V; ++v) {
     const StepgImage &g = *imgs[v];
@@ -2128,7 +2131,7 @@ void Pipeline::run(bool timed) {
                         st>>>(desc, items, act, seed, m.band.as<int2>(), m.S.as<uint32_t>(), \
                               m.nItems, m.flags(), sp1);                                                \
   if (sp2.numItems)                                                                                     \
-    sweepKernel<RPT, G><<<sp2.numItems, m.sweepThreads, m.splitLdsBytes, st>>>(                         \
+    sweepKernel<RPT, true><<<sp2.numItems, m.sweepThreads, m.splitLdsBytes, st>>>(                      \
         desc, items, act, seed, m.band.as<int2>(), m.S.as<uint32_t>(), 0u, m.flags(), sp2)
     if (m.seedsInLds) {
       switch (m.rpt) {
@@ -2211,7 +2214,6 @@ constexpr int32_t kSplitMatvecPairs = 22, kSplitStagePairs = 10, kSplitLaunchPai
 constexpr int32_t kSplitConcurrentPct = 170;
 constexpr uint32_t kSplitMaxColumns = 4;   // split columns per load
 constexpr uint32_t kClassWideMaxTiles = 2048;  // class workgroups up to which the 16-column walk is used
-constexpr uint32_t kSplitMaxPlain = 256;   // plain sweep items (more fill the chip anyway)
 
 static int32_t concurrentSteps(int32_t n) { return n * kSplitConcurrentPct / 100; }
 
@@ -2266,7 +2268,9 @@ static void planSweep(Pipeline::Impl &m) {
   };
   const uint32_t knob = gSweepSplit.load();
   const bool force = knob == 2;  // every column reaching the bottom, most chunks that fit
-  if (!knob || !m.seedsInLds || m.nItems > kSplitMaxPlain) {
+  // (loads with several long columns of one length, e.g. batches, are left
+  // whole by the estimate: the next longest band bounds the sweep anyway)
+  if (!knob) {
     writeItems();
     return;
   }
@@ -2329,7 +2333,9 @@ static void planSweep(Pipeline::Impl &m) {
       if (s.bound[1] < 4) continue;
       phase2 = std::max(phase2, s.bound[1] + over(0));
       const int32_t est = force ? -K : concurrentSteps(phase1) + kSplitLaunchPairs + phase2;
-      const size_t need = 4ull * (2 * P + 1 + d.numSeeds + 2) + 4ull * splitChainWords(P, K - 2);
+      // (the second launch reads seeds from global memory: its LDS holds the
+      // ring and the entering-state chain)
+      const size_t need = 4ull * (2 * P + 1 + 2) + 4ull * splitChainWords(P, K - 2);
       if (need > kLdsBudget) continue;
       if (est < bestEst) {
         bestEst = est;
@@ -2340,7 +2346,7 @@ static void planSweep(Pipeline::Impl &m) {
       if (force) continue;
       break;
     }
-    lds = std::max<size_t>(lds, 4ull * (2 * P + 1 + d.numSeeds + 2) + 4ull * splitChainWords(P, best.K - 2));
+    lds = std::max<size_t>(lds, 4ull * (2 * P + 1 + 2) + 4ull * splitChainWords(P, best.K - 2));
     const uint32_t si = static_cast<uint32_t>(descs.size());
     best.stateOff = static_cast<uint32_t>(stateWords);
     stateWords += splitStateWords(P, best.K);
