@@ -2205,11 +2205,14 @@ void Pipeline::run(bool timed) {
   GPA_HIP(hipGetLastError());
 }
 
-// Critical-path model of the sweep, in steps (pairs) of a column swept
-// alone, measured on MI355X: a step while many columns run at once costs
-// 1.7 steps; a matvec of the entering-state chain costs 22 (a 576-node
-// layer: VALU-bound on the one compute unit of the chunk); staging the maps
-// costs 10; the second launch's set-up 3.
+// Critical-path model of the sweep, in steps (pairs) of a lower chunk,
+// measured on MI355X (bb12): a step of the first launch (the top of a
+// column, or beside it) costs 1.7 steps -- the top chunk's 24 pairs took
+// 14 us, the lower chunks' ~0.3 us per pair (per-pair work, not contention:
+// the observables' column swept alone takes as long as with every other
+// column beside it); a matvec of the entering-state chain costs 22 (a
+// 576-node layer: VALU-bound on the one compute unit of the chunk); staging
+// the maps costs 10; the second launch's set-up 3.
 constexpr int32_t kSplitMatvecPairs = 22, kSplitStagePairs = 10, kSplitLaunchPairs = 3;
 constexpr int32_t kSplitConcurrentPct = 170;
 constexpr uint32_t kSplitMaxColumns = 4;   // split columns per load
@@ -2223,10 +2226,10 @@ static int32_t concurrentSteps(int32_t n) { return n * kSplitConcurrentPct / 100
 // seed range, which misses the observables' long band).  Then the
 // time-chunked sweep: the longest columns, while their band reaches the
 // bottom (layer <= 3) and K chunks are estimated to beat sweeping the
-// column whole by 10 % (the whole column: its first `rest` steps beside
-// the other columns, rest = the next longest band; split: the top chunk or
-// a middle chunk's maps beside the other columns, the second launch, then
-// the longest lower chunk with its staging and matvecs).  Writes the
+// column whole by 10 % (the whole column: its first `rest` steps at the
+// first launch's rate, rest = the next longest band; split: the top chunk
+// or a middle chunk's maps at that rate, the second launch, then the
+// longest lower chunk with its staging and matvecs).  Writes the
 // items and descriptors to the device; the results do not depend on the
 // plan.
 static void planSweep(Pipeline::Impl &m) {
