@@ -1241,8 +1241,13 @@ struct EmitOut {
   uint32_t *flags, *vBounds;
 };
 
+constexpr uint32_t kEmitStageWords = 1024;  // per wave: a bucket's targets staged in LDS
+
 __global__ __launch_bounds__(256) void emitKernel(const VDesc *__restrict__ desc, EmitIn in, EmitOut out) {
   __shared__ uint32_t red[3][4];
+  // a bucket's targets are assembled in LDS and written out as one
+  // contiguous, coalesced range (rows of scattered 2-byte stores otherwise)
+  __shared__ uint32_t stage[4][kEmitStageWords];
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t g = blockIdx.x;
   // offsets (classes, targets, members) of this workgroup's first bucket
@@ -1309,6 +1314,21 @@ __global__ __launch_bounds__(256) void emitKernel(const VDesc *__restrict__ desc
       out.member[C] = M;
     }
   }
+  // targets of this bucket: staged when they fit the wave's LDS share
+  const uint32_t Tb = in.runNnz[b];
+  const bool staged = Tb <= (out.targets16 ? 2 * kEmitStageWords : kEmitStageWords);
+  uint16_t *st16 = reinterpret_cast<uint16_t *>(stage[wave]);
+  uint32_t *st32 = stage[wave];
+  auto put = [&](uint64_t at, uint32_t t) {
+    if (staged) {
+      const uint32_t i = static_cast<uint32_t>(at - tb);
+      if (out.targets16) st16[i] = static_cast<uint16_t>(t);
+      else st32[i] = t;
+    } else if (at < out.capTargets) {
+      if (out.targets16) out.targets16[at] = static_cast<uint16_t>(t);
+      else out.targets[at] = t;
+    }
+  };
   for (uint32_t r = lane; r < nR; r += 64) {
     const uint32_t c = cb + r;
     const uint32_t rk = in.runRank[bs + r];
@@ -1335,23 +1355,26 @@ __global__ __launch_bounds__(256) void emitKernel(const VDesc *__restrict__ desc
       }
 #pragma unroll
       for (uint32_t j = 0; j < kStashTargets; ++j)
-        if (j < n && pos + j < out.capTargets) {
-          const uint32_t t = (w[j / 2] >> (16 * (j % 2))) & 0xFFFFu;
-          if (out.targets16) out.targets16[pos + j] = static_cast<uint16_t>(t);
-          else out.targets[pos + j] = t;
-        }
+        if (j < n) put(pos + j, (w[j / 2] >> (16 * (j % 2))) & 0xFFFFu);
       pos += n;
     } else {
       const FrameRows fr = decodeRows(expandRankCode(in.rankCode[rk], in.spill + d.spillOff), d.layerStride);
       forEachTargetScan(d.numCols, in.band + d.colOff, fr, in.S + d.sOff, d.colStride, [&](uint32_t tg) {
-        if (pos < out.capTargets) {
-          if (out.targets16) out.targets16[pos] = static_cast<uint16_t>(tg);
-          else out.targets[pos] = tg;
-        }
+        put(pos, tg);
         pos++;
       });
     }
     if (pos > out.capTargets) atomicOr(&out.flags[1], 1u);
+  }
+  if (staged) {
+    // (a wave's LDS accesses complete in order: no barrier)
+    for (uint32_t i = lane; i < Tb; i += 64) {
+      const uint64_t at = static_cast<uint64_t>(tb) + i;
+      if (at < out.capTargets) {
+        if (out.targets16) out.targets16[at] = st16[i];
+        else out.targets[at] = st32[i];
+      }
+    }
   }
 }
 
