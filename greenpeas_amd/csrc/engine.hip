@@ -2237,11 +2237,11 @@ void Pipeline::run(bool timed) {
 // 576-node layer: VALU-bound on the one compute unit of the chunk); staging
 // the maps costs 10; the second launch's set-up 3.
 constexpr int32_t kSplitMatvecPairs = 22, kSplitStagePairs = 10, kSplitLaunchPairs = 3;
-constexpr int32_t kSplitConcurrentPct = 170;
+constexpr int32_t kSplitFirstLaunchPct = 170;
 constexpr uint32_t kSplitMaxColumns = 4;   // split columns per load
 constexpr uint32_t kClassWideMaxTiles = 2048;  // class workgroups up to which the 16-column walk is used
 
-static int32_t concurrentSteps(int32_t n) { return n * kSplitConcurrentPct / 100; }
+static int32_t firstLaunchSteps(int32_t n) { return n * kSplitFirstLaunchPct / 100; }
 
 // Runs after a load's first run, from the column bands it found.  The
 // plain items are put in the order of their measured bands, longest first
@@ -2329,7 +2329,7 @@ static void planSweep(Pipeline::Impl &m) {
     const int32_t top = col.hi / 2, total = top + 1;
     SplitDesc best{};
     // must beat the whole sweep by 10 % (unless forced)
-    const int32_t whole = concurrentSteps(std::min(rest, total)) + total - std::min(rest, total);
+    const int32_t whole = firstLaunchSteps(std::min(rest, total)) + total - std::min(rest, total);
     int32_t bestEst = force ? INT32_MAX : whole - whole / 10;
     for (int K = 3; K <= kSplitMaxChunks; ++K) {
       SplitDesc s{};
@@ -2358,7 +2358,7 @@ static void planSweep(Pipeline::Impl &m) {
       s.bound[0] = 0;
       if (s.bound[1] < 4) continue;
       phase2 = std::max(phase2, s.bound[1] + over(0));
-      const int32_t est = force ? -K : concurrentSteps(phase1) + kSplitLaunchPairs + phase2;
+      const int32_t est = force ? -K : firstLaunchSteps(phase1) + kSplitLaunchPairs + phase2;
       // (the second launch reads seeds from global memory: its LDS holds the
       // ring and the entering-state chain)
       const size_t need = 4ull * (2 * P + 1 + 2) + 4ull * splitChainWords(P, K - 2);
