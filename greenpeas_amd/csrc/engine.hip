@@ -1254,9 +1254,21 @@ __global__ __launch_bounds__(256) void emitKernel(const VDesc *__restrict__ desc
   uint32_t base[3] = {0u, 0u, 0u};
   if (in.wgSum) {
     uint32_t part[3] = {0u, 0u, 0u};
-    for (uint32_t i = threadIdx.x; i < g; i += blockDim.x)
+    // eight totals per thread per pass, all loads in flight together (one
+    // round trip for up to 2,048 preceding workgroups)
+    for (uint32_t i0 = threadIdx.x; i0 < g; i0 += 8 * blockDim.x) {
+      uint32_t x[8][3];
 #pragma unroll
-      for (int q = 0; q < 3; ++q) part[q] += in.wgSum[3 * i + q];
+      for (uint32_t u = 0; u < 8; ++u) {
+        const uint32_t i = i0 + u * blockDim.x;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) x[u][q] = i < g ? in.wgSum[3 * i + q] : 0u;
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < 8; ++u)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) part[q] += x[u][q];
+    }
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
       const uint32_t t = waveInclusiveSum(part[q], lane);
