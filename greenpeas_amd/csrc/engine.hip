@@ -633,7 +633,7 @@ __global__ __launch_bounds__(kSweepMaxThreads) void sweepKernel(
   int32_t lo = -1;      // lowest layer of the band, once known
   uint32_t trip = 0;    // vote word parity
   // whole waves: sweepThreads is a multiple of 64
-  const bool active = __builtin_amdgcn_readfirstlane(tid >> 6) < (T >> 6);
+  const bool active = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(tid >> 6)) < (T >> 6);
   SweepState<RPT> st;
   uint32_t slotB[D];
   uint4 rs[D][RPT];
