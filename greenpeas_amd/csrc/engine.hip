@@ -991,9 +991,13 @@ __global__ __launch_bounds__(kClassBlock) void classifyKernel(
   const uint32_t W = 1u << o.stashShift;  // words per row (the LDS rows are padded to W + 1)
   uint16_t *row = reinterpret_cast<uint16_t *>(rows[wave] + lane * (W + 1));
   const bool stashed = d.numCols <= o.stashMaxCols;
+  // this node's class hash, its bucket and the slot the bucket's counter
+  // gave it (used after the wave's target rows are written: the atomic's
+  // round trip overlaps those stores)
+  uint64_t h = kFnvBasis;
+  uint32_t cnt = 0, bkt = 0, pos = 0;
   if (i < d.numRanks) {
-    uint64_t h = kFnvBasis, h2 = 0x84222325cbf29ce4ull;
-    uint32_t cnt = 0;
+    uint64_t h2 = 0x84222325cbf29ce4ull;
     const FrameRows fr = decodeRows(expandRankCode(ccode, rin.spill + d.spillOff), d.layerStride);
     auto take = [&](uint32_t tg) {
       h = (h ^ tg) * kFnvPrime;
@@ -1011,17 +1015,8 @@ __global__ __launch_bounds__(kClassBlock) void classifyKernel(
     h2 = mix2(h2, cnt);
     o.info[gi] = nodeInfo(prob, static_cast<uint32_t>(h2 ^ (h2 >> 32)), cnt);
     if (cnt) {
-      const uint32_t b = d.bucketBase + (d.bucketBits ? static_cast<uint32_t>(h >> (64 - d.bucketBits)) : 0u);
-      const uint32_t pos = atomicAdd(&o.bucketCount[static_cast<uint64_t>(b) * o.countStride], 1u);
-      if (pos < kBucketCap) {
-        o.pslot[static_cast<uint64_t>(b) * kBucketCap + pos] =
-            make_uint4(static_cast<uint32_t>(h), static_cast<uint32_t>(h >> 32), static_cast<uint32_t>(gi), 0u);
-      } else {
-        const uint32_t k = atomicAdd(&o.flags[3], 1u);
-        o.ovKey[k] = h;
-        o.ovRank[k] = static_cast<uint32_t>(gi);
-        o.ovBucket[k] = b;
-      }
+      bkt = d.bucketBase + (d.bucketBits ? static_cast<uint32_t>(h >> (64 - d.bucketBits)) : 0u);
+      pos = atomicAdd(&o.bucketCount[static_cast<uint64_t>(bkt) * o.countStride], 1u);
     }
   }
   // the wave's rows, row-major and contiguous from its first rank (a wave's
@@ -1036,6 +1031,17 @@ __global__ __launch_bounds__(kClassBlock) void classifyKernel(
       const uint32_t e = k * 64 + lane;  // element of the 64 x W block
       const uint32_t r = e >> o.stashShift, c = e & (W - 1);
       if (r < nRows) dst[e] = rows[wave][r * (W + 1) + c];
+    }
+  }
+  if (i < d.numRanks && cnt) {
+    if (pos < kBucketCap) {
+      o.pslot[static_cast<uint64_t>(bkt) * kBucketCap + pos] =
+          make_uint4(static_cast<uint32_t>(h), static_cast<uint32_t>(h >> 32), static_cast<uint32_t>(gi), 0u);
+    } else {
+      const uint32_t k = atomicAdd(&o.flags[3], 1u);
+      o.ovKey[k] = h;
+      o.ovRank[k] = static_cast<uint32_t>(gi);
+      o.ovBucket[k] = bkt;
     }
   }
 }
