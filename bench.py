@@ -128,7 +128,7 @@ def main() -> int:
         print(json.dumps({"copy_bytes": args.calibrate_copy, "copies": 5}), flush=True)
         return 0
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); the DEM "
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the DEM "
               "compile path shards across GPUs by circuit with no communication", file=sys.stderr)
         return 2
 
