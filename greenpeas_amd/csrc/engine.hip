@@ -149,9 +149,10 @@ __device__ __forceinline__ void waveBitonicSort(uint64_t (&h)[R], uint32_t (&r)[
           const uint32_t ro = __shfl_xor(r[s], static_cast<int>(j));
           const bool lower = (lane & j) == 0;
           const bool asc = (e & k) == 0;
-          const bool otherLess = keyLess(ho, ro, h[s], r[s]);
           const bool keepMin = lower == asc;
-          const bool take = keepMin ? otherLess : keyLess(h[s], r[s], ho, ro);
+          // keys are distinct (ranks are) except the identical padding
+          // entries, whose exchange changes nothing: one comparison decides
+          const bool take = keyLess(ho, ro, h[s], r[s]) == keepMin;
           if (take) {
             h[s] = ho;
             r[s] = ro;
