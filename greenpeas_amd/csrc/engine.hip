@@ -130,7 +130,8 @@ __device__ __forceinline__ void waveBitonicSort(uint64_t (&h)[R], uint32_t (&r)[
           if (sp > s) {
             const uint32_t e = static_cast<uint32_t>(s) * 64 + lane;
             const bool asc = (e & k) == 0;
-            const bool sw = asc ? keyLess(h[sp], r[sp], h[s], r[s]) : keyLess(h[s], r[s], h[sp], r[sp]);
+            // one comparison (distinct keys, or identical padding)
+            const bool sw = keyLess(h[sp], r[sp], h[s], r[s]) == asc;
             if (sw) {
               uint64_t th = h[s];
               h[s] = h[sp];
