@@ -859,34 +859,46 @@ __device__ __forceinline__ void buildMaskTable(uint32_t *tab, uint32_t l0, uint3
 
 // Calls f(target) for every set bit of one node's sensitivity row, in
 // ascending target order, visiting the columns set in the node's window
-// mask (`mrow`, maskWords words).  Columns are taken kBatch at a time and
-// all their row words loaded before any is used, so a node waits for one
-// memory round trip per kBatch columns (16: one for nearly every node of a
-// circuit with ~15 columns per layer, at 117 VGPRs; 4 where nodes see few
-// columns).
+// mask (`mrow`, maskWords words, read two at a time: circuits with at most
+// 64 columns never reload it).  Columns are taken kBatch at a time: their
+// bands are read together, then the row words that lie in each column's
+// band are loaded before any is used, so a node waits for one memory round
+// trip per kBatch columns (16: one for nearly every node of a circuit with
+// ~15 columns per layer; 4 where nodes see few columns).
 template <int kBatch, typename F>
 __device__ __forceinline__ void forEachTargetMasked(const uint32_t *mrow, uint32_t maskWords, const int2 *band,
                                                     const FrameRows &r, const uint32_t *__restrict__ S0,
                                                     uint64_t colStride, F &&f) {
   if (!r.ok[0] || !maskWords) return;
-  uint32_t w = 0, m = mrow[0];
+  uint32_t w = 0;  // the window's low word
+  uint64_t m = mrow[0] | (maskWords > 1 ? static_cast<uint64_t>(mrow[1]) << 32 : 0ull);
   for (;;) {
     uint32_t col[kBatch], x[kBatch][4];
 #pragma unroll
     for (int j = 0; j < kBatch; ++j) {
-      while (m == 0u && w + 1 < maskWords) m = mrow[++w];
-      col[j] = m ? w * 32 + static_cast<uint32_t>(__builtin_ctz(m)) : NONE;
+      while (m == 0ull && w + 2 < maskWords) {
+        w += 2;
+        m = mrow[w] | (w + 1 < maskWords ? static_cast<uint64_t>(mrow[w + 1]) << 32 : 0ull);
+      }
+      col[j] = m ? w * 32 + static_cast<uint32_t>(__builtin_ctzll(m)) : NONE;
       m &= m - 1;
     }
+    int2 bd[kBatch];
+#pragma unroll
+    for (int j = 0; j < kBatch; ++j) bd[j] = col[j] != NONE ? band[col[j]] : make_int2(1, 0);
 #pragma unroll
     for (int j = 0; j < kBatch; ++j)
 #pragma unroll
-      for (int q = 0; q < 4; ++q)
-        x[j][q] = (col[j] != NONE && r.ok[q]) ? S0[static_cast<uint64_t>(col[j]) * colStride + r.idx[q]] : 0u;
+      for (int q = 0; q < 4; ++q) {
+        const int32_t l = static_cast<int32_t>(r.lay[q]);
+        x[j][q] = (r.ok[q] && bd[j].x <= l && l <= bd[j].y)
+                      ? S0[static_cast<uint64_t>(col[j]) * colStride + r.idx[q]]
+                      : 0u;
+      }
 #pragma unroll
     for (int j = 0; j < kBatch; ++j) {
       if (col[j] == NONE) continue;
-      uint32_t v = bandXor(r, band[col[j]], x[j]);
+      uint32_t v = x[j][0] ^ x[j][1] ^ x[j][2] ^ x[j][3];
       while (v) {
         f(col[j] * 32 + static_cast<uint32_t>(__builtin_ctz(v)));
         v &= v - 1;
@@ -1004,7 +1016,9 @@ __global__ __launch_bounds__(kClassBlock) void classifyKernel(
     auto take = [&](uint32_t tg) {
       h = (h ^ tg) * kFnvPrime;
       h2 = mix2(h2, tg);
-      if (cnt < 2 * W) row[cnt] = static_cast<uint16_t>(tg);
+      // targets past the row's end overwrite its last entry: the emit stage
+      // reads rows of classes with at most 2 W targets only
+      row[cnt < 2 * W ? cnt : 2 * W - 1] = static_cast<uint16_t>(tg);
       cnt++;
     };
     if constexpr (TABLE)

@@ -223,6 +223,25 @@ def test_emit_without_target_rows(oracle):
         autotune.set_tuning("stash", 1)
 
 
+@pytest.mark.parametrize("d,rounds", [(5, 100), (7, 100), (5, 260)])
+def test_class_stage_masks_wider_than_64_columns(oracle, d, rounds):
+    """Circuits with more than 64 detector columns (2,400 to 6,240
+    detectors: 3, 5 and 7 mask words per node window) walk their column
+    masks two words at a time, reloading the window: identical hashes,
+    order, targets and probabilities to the STEPG restatement."""
+    from greenpeas_amd.codes import SurfaceCode
+
+    text = SurfaceCode(d).memory_text(rounds, 1e-3)
+    drv = get_driver(text)
+    dem = drv.compile_detector_error_model()
+    assert (dem.num_detectors + dem.num_observables + 31) // 32 > 64
+    ref = oracle.compile_stepg(text, 2)
+    assert np.array_equal(drv.last_hashes(), ref.hashes)
+    assert np.array_equal(dem.offsets, ref.offsets)
+    assert np.array_equal(dem.targets, ref.targets)
+    assert np.max(np.abs(dem.probs - ref.probs) / ref.probs) <= REL
+
+
 @pytest.mark.parametrize("stash", [1, 0])
 def test_class_stage_without_mask_tables(oracle, stash):
     """With the class stage's LDS column masks switched off, every node tests
