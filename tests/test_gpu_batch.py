@@ -27,6 +27,8 @@ def test_batch_equals_solo_and_oracle(oracle):
     texts = [read_circuit(n) for n in ("surface3", "bb6", "surface5", "surface3", "bb12",
                                        "iceberg_surface_6_static_circl", "iceberg_surface_4_adaptive_circl")]
     texts.append(SurfaceCode(7).memory_text(9, noise=Noise(rng=np.random.default_rng(4))))
+    # 76 detector columns: three mask words per window beside circuits with one and two
+    texts.append(SurfaceCode(5).memory_text(100, 1e-3))
     bc = BatchCompiler()
     bc.set_circuits(texts)
     dems = bc.compile()
