@@ -103,11 +103,14 @@ __device__ __forceinline__ bool keyLess(uint64_t ha, uint32_t ra, uint64_t hb, u
   return ha < hb || (ha == hb && ra < rb);
 }
 
-// Second, independent 64-bit hash of a class: used only to detect FNV
-// collisions between different classes inside one equal-hash run.
-__device__ __forceinline__ uint64_t mix2(uint64_t h, uint32_t t) {
-  h ^= (static_cast<uint64_t>(t) + 0x9E3779B97F4A7C15ull) + (h << 6) + (h >> 2);
-  return h * 0xD6E8FEB86659FD93ull;
+// Second, independent 32-bit hash of a class: used only to detect FNV
+// collisions between different classes inside one equal-hash run (the class
+// size is compared too).  Each step is a bijection of h for a fixed target
+// (xor, multiply by an odd constant, xorshift), so two lists of one length
+// that differ in a single target always get different hashes.
+__device__ __forceinline__ uint32_t mix2(uint32_t h, uint32_t t) {
+  h = (h ^ t) * 0x9E3779B1u;
+  return h ^ (h >> 16);
 }
 
 // Bitonic sort of 64*R (hash, rank) keys held R per lane; element index
@@ -1011,7 +1014,7 @@ __global__ __launch_bounds__(kClassBlock) void classifyKernel(
   uint64_t h = kFnvBasis;
   uint32_t cnt = 0, bkt = 0, pos = 0;
   if (i < d.numRanks) {
-    uint64_t h2 = 0x84222325cbf29ce4ull;
+    uint32_t h2 = 0x2545F491u;
     const FrameRows fr = decodeRows(expandRankCode(ccode, rin.spill + d.spillOff), d.layerStride);
     auto take = [&](uint32_t tg) {
       h = (h ^ tg) * kFnvPrime;
@@ -1029,7 +1032,7 @@ __global__ __launch_bounds__(kClassBlock) void classifyKernel(
     if (cnt && cnt <= 32)
       for (uint32_t k = cnt; k < 32; ++k) h = (h ^ 0xFFFFFFFFull) * kFnvPrime;
     h2 = mix2(h2, cnt);
-    o.info[gi] = nodeInfo(prob, static_cast<uint32_t>(h2 ^ (h2 >> 32)), cnt);
+    o.info[gi] = nodeInfo(prob, h2, cnt);
     if (cnt) {
       bkt = d.bucketBase + (d.bucketBits ? static_cast<uint32_t>(h >> (64 - d.bucketBits)) : 0u);
       pos = atomicAdd(&o.bucketCount[static_cast<uint64_t>(bkt) * o.countStride], 1u);
