@@ -36,13 +36,68 @@ def _stream_ptr(device: torch.device) -> int:
     return torch.cuda.current_stream(device).cuda_stream
 
 
+def _check_index_tensors(values: torch.Tensor, index: torch.Tensor, seg_start: torch.Tensor,
+                         require_cuda: bool = True) -> None:
+    """Host-side checks that need no device round trip: the kernels read
+    values as float64 and index and seg_start as contiguous uint32 arrays,
+    all on one device."""
+    if not isinstance(values, torch.Tensor) or values.dtype != torch.float64:
+        raise TypeError("values must be a float64 tensor")
+    if require_cuda and not values.is_cuda:
+        raise TypeError("values must be a CUDA tensor")
+    if values.dim() != 1:
+        raise ValueError("values must be one-dimensional")
+    for name, t in (("index", index), ("seg_start", seg_start)):
+        if not isinstance(t, torch.Tensor) or t.dtype != torch.int32:
+            raise TypeError(f"{name} must be an int32 tensor (read as uint32), got "
+                            f"{getattr(t, 'dtype', type(t).__name__)}")
+        if t.device != values.device:
+            raise ValueError(f"{name} is on {t.device}, values on {values.device}")
+        if t.dim() != 1 or not t.is_contiguous():
+            raise ValueError(f"{name} must be a contiguous one-dimensional tensor")
+    if seg_start.numel() < 1:
+        raise ValueError("seg_start needs at least one entry")
+
+
+def _check_segments(values: torch.Tensor, index: torch.Tensor, seg_start: torch.Tensor) -> bool:
+    """Checks the segment structure on the tensors' device (one round trip):
+    seg_start[0] == 0, non-decreasing, seg_start[-1] <= index.numel(), every
+    used index (positions below seg_start[-1]) within values.  Returns
+    whether a used index repeats."""
+    s = seg_start.to(torch.int64)
+    used = index.to(torch.int64)
+    pos = torch.arange(index.numel(), device=index.device)
+    in_use = pos < s[-1]
+    bad_idx = ((used < 0) | (used >= values.numel())) & in_use
+    # positions past the segments get distinct keys above every index
+    key = torch.sort(torch.where(in_use, used, values.numel() + pos)).values
+    repeated = (key[1:] == key[:-1]).any() if key.numel() > 1 else torch.zeros((), dtype=torch.bool,
+                                                                               device=key.device)
+    decreasing = (s[1:] < s[:-1]).any() if s.numel() > 1 else torch.zeros((), dtype=torch.bool,
+                                                                          device=s.device)
+    flags = torch.stack([s[0] != 0, decreasing, s[-1] > index.numel(), bad_idx.any(),
+                         repeated]).cpu().tolist()
+    if flags[0]:
+        raise ValueError("seg_start[0] must be 0")
+    if flags[1]:
+        raise ValueError("seg_start must be non-decreasing")
+    if flags[2]:
+        raise ValueError(f"seg_start[-1] exceeds index.numel() = {index.numel()}")
+    if flags[3]:
+        raise ValueError(f"index holds entries outside values (numel {values.numel()})")
+    return bool(flags[4])
+
+
 class XorMerge(torch.autograd.Function):
-    """out[s] = XOR-merge of values[index[k]], k in [seg_start[s], seg_start[s+1])."""
+    """out[s] = XOR-merge of values[index[k]], k in [seg_start[s], seg_start[s+1]).
+
+    The kernels need each value in at most one position of ``index``;
+    ``xor_merge`` checks that (and the segment structure) and routes repeated
+    indices through a gathered copy."""
 
     @staticmethod
     def forward(ctx, values: torch.Tensor, index: torch.Tensor, seg_start: torch.Tensor) -> torch.Tensor:
-        if values.dtype != torch.float64 or not values.is_cuda:
-            raise TypeError("XorMerge needs a float64 CUDA tensor")
+        _check_index_tensors(values, index, seg_start)
         values = values.contiguous()
         nseg = seg_start.numel() - 1
         out = torch.empty(nseg, dtype=torch.float64, device=values.device)
@@ -64,7 +119,23 @@ class XorMerge(torch.autograd.Function):
         return grad, None, None
 
 
-xor_merge = XorMerge.apply
+def xor_merge(values: torch.Tensor, index: torch.Tensor, seg_start: torch.Tensor,
+              validate: bool = True) -> torch.Tensor:
+    """Segmented XOR-merge with a hand-written backward (see XorMerge).
+
+    index and seg_start are int32 tensors (their bits are read as uint32) on
+    values' device.  With ``validate`` (default) the segment structure is
+    checked on the device first (one synchronization) and repeated indices
+    are handled by merging a gathered copy of the values, whose gradient is
+    then summed back per value; ``validate=False`` is for index arrays known
+    to be valid partial permutations (a DEM plan's own)."""
+    _check_index_tensors(values, index, seg_start)
+    if validate and _check_segments(values, index, seg_start):
+        n = index.numel()
+        gathered = values[index.to(torch.int64)]
+        ident = torch.arange(n, dtype=torch.int32, device=values.device)
+        return XorMerge.apply(gathered, ident, seg_start)
+    return XorMerge.apply(values, index, seg_start)
 
 
 class NodeRates(torch.autograd.Function):
@@ -182,7 +253,7 @@ class DifferentiableDem:
         if theta.shape != (self.num_locations,):
             raise ValueError(f"theta must have shape ({self.num_locations},)")
         q = self.node_probabilities(theta.to(self.device, torch.float64))
-        return xor_merge(q, self.class_node, self.class_start)
+        return xor_merge(q, self.class_node, self.class_start, validate=False)
 
     def dem(self, theta: torch.Tensor, num_detectors: int, num_observables: int) -> DetectorErrorModel:
         p = self.probabilities(theta).detach().cpu().numpy()

@@ -1006,7 +1006,9 @@ __global__ __launch_bounds__(kClassBlock) void classifyKernel(
   else
     __syncthreads();
   const uint32_t W = 1u << o.stashShift;  // words per row (the LDS rows are padded to W + 1)
-  uint16_t *row = reinterpret_cast<uint16_t *>(rows[wave] + lane * (W + 1));
+  // 16-bit entries of the 32-bit rows: a type that may alias them
+  typedef uint16_t __attribute__((may_alias)) u16_alias;
+  u16_alias *row = reinterpret_cast<u16_alias *>(rows[wave] + lane * (W + 1));
   const bool stashed = d.numCols <= o.stashMaxCols;
   // this node's class hash, its bucket and the slot the bucket's counter
   // gave it (used after the wave's target rows are written: the atomic's
@@ -1038,8 +1040,13 @@ __global__ __launch_bounds__(kClassBlock) void classifyKernel(
       pos = atomicAdd(&o.bucketCount[static_cast<uint64_t>(bkt) * o.countStride], 1u);
     }
   }
-  // the wave's rows, row-major and contiguous from its first rank (a wave's
-  // LDS accesses are in order, so no barrier is needed)
+  // the wave's rows, row-major and contiguous from its first rank: every
+  // lane's row writes are made visible to the wave (LDS accesses of a wave
+  // complete in order; the fences keep the compiler from moving the reads
+  // above the writes)
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
   const uint32_t waveFirst = t.y + wave * 64;
   if (stashed && waveFirst < d.numRanks) {
     const uint32_t nRows = d.numRanks - waveFirst < 64 ? d.numRanks - waveFirst : 64;
@@ -1354,7 +1361,8 @@ __global__ __launch_bounds__(256) void emitKernel(const VDesc *__restrict__ desc
   // targets of this bucket: staged when they fit the wave's LDS share
   const uint32_t Tb = in.runNnz[b];
   const bool staged = Tb <= (out.targets16 ? 2 * kEmitStageWords : kEmitStageWords);
-  uint16_t *st16 = reinterpret_cast<uint16_t *>(stage[wave]);
+  typedef uint16_t __attribute__((may_alias)) u16_alias;
+  u16_alias *st16 = reinterpret_cast<u16_alias *>(stage[wave]);
   uint32_t *st32 = stage[wave];
   auto put = [&](uint64_t at, uint32_t t) {
     if (staged) {
@@ -1404,7 +1412,10 @@ __global__ __launch_bounds__(256) void emitKernel(const VDesc *__restrict__ desc
     if (pos > out.capTargets) atomicOr(&out.flags[1], 1u);
   }
   if (staged) {
-    // (a wave's LDS accesses complete in order: no barrier)
+    // every lane's staged targets visible to the wave before the read-out
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
     for (uint32_t i = lane; i < Tb; i += 64) {
       const uint64_t at = static_cast<uint64_t>(tb) + i;
       if (at < out.capTargets) {

@@ -275,8 +275,12 @@ int gp_xor_merge_forward(const double *values, const uint32_t *index,
                          const uint32_t *seg_start, uint32_t num_segments,
                          double *out, void *stream);
 /* Backward: grad_values[index[k]] = grad_out[s] * prod_{j != k in s}
- * (1 - 2 values[index[j]]).  Each values entry belongs to at most one
- * segment; entries in no segment are left untouched. */
+ * (1 - 2 values[index[j]]).  index must not name a value twice (within or
+ * across segments: a partial permutation, as a DEM plan's class members
+ * are); entries in no segment are left untouched.  Repeated values are
+ * merged from a gathered copy with an identity index instead, their
+ * gradients summed back by the caller (greenpeas_amd.autograd.xor_merge does
+ * this). */
 int gp_xor_merge_backward(const double *values, const uint32_t *index,
                           const uint32_t *seg_start, uint32_t num_segments,
                           const double *grad_out, double *grad_values,

@@ -61,7 +61,8 @@ def test_large_surface_codes_are_valid_css(d):
 
 
 def test_memory_with_oracle(oracle):
-    """A generated d15 circuit compiles identically with both oracles."""
+    """A generated d5 memory with seeded per-instruction rates and two
+    reset-conditioned branches compiles identically with both oracles."""
     import numpy as np
 
     text = codes.SurfaceCode(5).memory_text(7, noise=codes.Noise(rng=np.random.default_rng(3)),

@@ -124,3 +124,23 @@ def test_fused_node_rates_match_tensor_formula(ag, name):
     t3 = torch.tensor(theta0, device="cuda", requires_grad=True)
     (dd.node_probabilities(t3) * w).sum().backward()
     assert torch.equal(t1.grad, t3.grad)
+
+
+def test_xor_merge_with_repeated_indices(ag):
+    """A value used at several positions (within and across segments) gets
+    the sum of its positions' gradients, as torch's own fp64 graph of the
+    same op does."""
+    rng = np.random.default_rng(3)
+    idx = np.array([4, 1, 4, 2, 0, 1, 3, 4, 2], np.int32)
+    seg = np.array([0, 3, 3, 6, 9], np.int32)
+    vals = rng.uniform(0.01, 0.45, 5)
+    v = torch.tensor(vals, dtype=torch.float64, device="cuda", requires_grad=True)
+    out = ag.xor_merge(v, torch.tensor(idx, device="cuda"), torch.tensor(seg, device="cuda"))
+    g = torch.tensor(rng.standard_normal(4), dtype=torch.float64, device="cuda")
+    (out * g).sum().backward()
+    vc = torch.tensor(vals, dtype=torch.float64, requires_grad=True)
+    ref = torch.stack([(1 - torch.prod(1 - 2 * vc[torch.tensor(idx[seg[s]:seg[s + 1]], dtype=torch.long)])) / 2
+                       for s in range(4)])
+    (ref * g.cpu()).sum().backward()
+    assert torch.allclose(out.detach().cpu(), ref.detach(), rtol=1e-13, atol=1e-300)
+    assert torch.allclose(v.grad.cpu(), vc.grad, rtol=1e-12, atol=1e-300)
