@@ -16,6 +16,8 @@ LIB_PATH = os.path.join(_HERE, "libgreenpeas_amd.so")
 GP_OK = 0
 GP_INVALID_ARGUMENT = 1
 GP_RUNTIME_ERROR = 2
+# components gp_node_rates_backward_noalloc can hold (GP_NOALLOC_MAX_COMPONENTS)
+NOALLOC_MAX_COMPONENTS = 1 << 20
 
 # every entry point declared in include/greenpeas_amd.h
 EXPORTS = (
@@ -31,6 +33,7 @@ EXPORTS = (
     "gp_grad_plan_create", "gp_grad_plan_destroy", "gp_grad_plan_sizes", "gp_grad_plan_dem_structure",
     "gp_grad_plan_device_arrays", "gp_grad_plan_copy_arrays", "gp_xor_merge_forward",
     "gp_xor_merge_backward", "gp_node_rates_forward", "gp_node_rates_backward",
+    "gp_node_rates_backward_noalloc",
 )
 
 _lib = None
@@ -117,6 +120,7 @@ def load() -> ctypes.CDLL:
     sig("gp_xor_merge_backward", [P, P, P, u32, P, P, P])
     sig("gp_node_rates_forward", [P, P, P, P, P, u32, P, P])
     sig("gp_node_rates_backward", [P, P, P, P, P, u32, P, P, P, P, u32, P, P])
+    sig("gp_node_rates_backward_noalloc", [P, P, P, P, P, u32, u32, P, P, P, u32, P, P])
     _lib = L
     return L
 

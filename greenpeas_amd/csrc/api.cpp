@@ -778,6 +778,22 @@ int gp_node_rates_backward(const double *theta, const uint32_t *comp_loc, const 
   });
 }
 
+static_assert(gpa::kNoAllocMaxComponents == GP_NOALLOC_MAX_COMPONENTS, "header and engine disagree");
+
+int gp_node_rates_backward_noalloc(const double *theta, const uint32_t *comp_loc, const uint8_t *comp_kind,
+                                   const uint32_t *node_start, const uint32_t *node_comp, uint32_t num_nodes,
+                                   uint32_t num_components, const double *grad_q, const uint32_t *loc_start,
+                                   const uint32_t *loc_comp, uint32_t num_locations, double *grad_theta,
+                                   void *stream) {
+  return guarded([&] {
+    if ((num_nodes && (!theta || !comp_loc || !comp_kind || !node_start || !node_comp || !grad_q)) ||
+        (num_locations && (!loc_start || !loc_comp || !grad_theta)))
+      throw InvalidArgument("NULL argument");
+    gpa::nodeRatesBackwardNoAlloc(theta, comp_loc, comp_kind, node_start, node_comp, num_nodes, num_components,
+                                  grad_q, loc_start, loc_comp, num_locations, grad_theta, stream);
+  });
+}
+
 int gp_xor_merge_backward(const double *values, const uint32_t *index, const uint32_t *seg_start,
                           uint32_t num_segments, const double *grad_out, double *grad_values,
                           void *stream) {

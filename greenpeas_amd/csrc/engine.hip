@@ -2799,4 +2799,62 @@ void xorMergeBackward(const double *values, const uint32_t *index, const uint32_
   GPA_HIP(hipGetLastError());
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free backward of the node rates: the per-component gradients
+// live in a fixed device array instead of a buffer the caller passes in.
+// Each GPU holds its own copy for the life of the process; its address on a
+// device is looked up once, with that device current, and used from then on
+// (the array's name in host code is only a stand-in for the symbol).
+// ---------------------------------------------------------------------------
+
+static __device__ __align__(256) double scratch[kNoAllocMaxComponents];
+
+namespace {
+
+constexpr int kScratchDevices = 64;
+std::mutex gScratchMu;
+double *gScratchPtr[kScratchDevices] = {};
+
+double *scratchOn(int device) {
+  if (device < 0 || device >= kScratchDevices)
+    throw RuntimeError("device ordinal " + std::to_string(device) + " out of range");
+  std::lock_guard<std::mutex> lk(gScratchMu);
+  if (!gScratchPtr[device]) {
+    int prev = 0;
+    GPA_HIP(hipGetDevice(&prev));
+    GPA_HIP(hipSetDevice(device));
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    const hipError_t r = hipSetDevice(prev);
+    if (e != hipSuccess)
+      throw RuntimeError(std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+    if (r != hipSuccess) throw RuntimeError(std::string("hipSetDevice failed: ") + hipGetErrorString(r));
+    if (!p) throw RuntimeError("hipGetSymbolAddress(scratch) returned NULL");
+    gScratchPtr[device] = static_cast<double *>(p);
+  }
+  return gScratchPtr[device];
+}
+
+}  // namespace
+
+void nodeRatesBackwardNoAlloc(const double *theta, const uint32_t *compLoc, const uint8_t *compKind,
+                              const uint32_t *nodeStart, const uint32_t *nodeComp, uint32_t numNodes,
+                              uint32_t numComponents, const double *gradQ, const uint32_t *locStart,
+                              const uint32_t *locComp, uint32_t numLocations, double *gradTheta, void *stream) {
+  if (numComponents > kNoAllocMaxComponents)
+    throw InvalidArgument("num_components = " + std::to_string(numComponents) +
+                          " exceeds the allocation-free maximum of " + std::to_string(kNoAllocMaxComponents) +
+                          "; use gp_node_rates_backward with a caller buffer");
+  if (!numNodes && !numLocations) return;
+  const hipStream_t st = static_cast<hipStream_t>(stream);
+  int device = 0;
+  if (st) GPA_HIP(hipStreamGetDevice(st, &device));
+  else GPA_HIP(hipGetDevice(&device));
+  // the same kernels as nodeRatesBackward, with the fixed array as their
+  // per-component buffer (every entry the location sums read is written by
+  // the node kernel of the same call, so nothing needs clearing)
+  nodeRatesBackward(theta, compLoc, compKind, nodeStart, nodeComp, numNodes, gradQ, scratchOn(device), locStart,
+                    locComp, numLocations, gradTheta, stream);
+}
+
 }  // namespace gpa

@@ -163,6 +163,16 @@ void nodeRatesBackward(const double *theta, const uint32_t *compLoc, const uint8
                        double *gradTheta, void *stream);
 void xorMergeForward(const double *values, const uint32_t *index, const uint32_t *segStart,
                      uint32_t numSegments, double *out, void *stream);
+// Components the allocation-free backward can hold: its per-component
+// gradients live in a fixed device array of this many doubles (8 MiB) per GPU.
+constexpr uint32_t kNoAllocMaxComponents = 1u << 20;
+// nodeRatesBackward without a gradComp buffer: the fixed array takes its
+// place (numComponents above kNoAllocMaxComponents throws InvalidArgument;
+// calls overlapping on one GPU share the array and race).  Allocates nothing.
+void nodeRatesBackwardNoAlloc(const double *theta, const uint32_t *compLoc, const uint8_t *compKind,
+                              const uint32_t *nodeStart, const uint32_t *nodeComp, uint32_t numNodes,
+                              uint32_t numComponents, const double *gradQ, const uint32_t *locStart,
+                              const uint32_t *locComp, uint32_t numLocations, double *gradTheta, void *stream);
 void xorMergeBackward(const double *values, const uint32_t *index, const uint32_t *segStart,
                       uint32_t numSegments, const double *gradOut, double *gradValues,
                       void *stream);

@@ -309,6 +309,27 @@ int gp_node_rates_backward(const double *theta, const uint32_t *comp_loc,
                            uint32_t num_locations, double *grad_theta,
                            void *stream);
 
+/* Allocation-free variant of gp_node_rates_backward.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving or fitting loops).  The
+ * per-component gradients live in a fixed device array of
+ * GP_NOALLOC_MAX_COMPONENTS doubles (8 MiB) that each GPU the library runs
+ * on holds until the process ends, so there is no grad_comp argument;
+ * num_components (greater than every index in node_comp and loc_comp) above
+ * that maximum is rejected with GP_INVALID_ARGUMENT.  Calls no hipMalloc or
+ * hipFree.  Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU (other streams or threads) race; calls queued on one
+ * stream are ordered.  Every component loc_comp names must be one node_comp
+ * names (as in a grad plan), since each call writes the entries it reads.
+ * Results are identical to gp_node_rates_backward's. */
+#define GP_NOALLOC_MAX_COMPONENTS 1048576u
+int gp_node_rates_backward_noalloc(const double *theta, const uint32_t *comp_loc,
+                                   const uint8_t *comp_kind, const uint32_t *node_start,
+                                   const uint32_t *node_comp, uint32_t num_nodes,
+                                   uint32_t num_components, const double *grad_q,
+                                   const uint32_t *loc_start, const uint32_t *loc_comp,
+                                   uint32_t num_locations, double *grad_theta, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
