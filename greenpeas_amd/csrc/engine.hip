@@ -113,13 +113,16 @@ __device__ __forceinline__ uint32_t mix2(uint32_t h, uint32_t t) {
   return h ^ (h >> 16);
 }
 
-// Bitonic sort of 64*R (hash, rank) keys held R per lane; element index
-// e = s * 64 + lane.  Distances below 64 exchange through lane shuffles,
-// larger ones inside a lane's registers.  (Measured on bb12: DPP row shifts
-// and gfx950's permlane swaps in place of the shuffles cost 1.4 us more --
-// the extra VALU work outweighs the LDS round trips at 4 waves per SIMD.)
+// Bitonic sort of 64*R (hash, rank) keys held R per lane, each with a
+// payload word that moves with it; element index e = s * 64 + lane.
+// Distances below 64 exchange through lane shuffles, larger ones inside a
+// lane's registers.  (Measured on bb12: DPP row shifts and gfx950's
+// permlane swaps in place of the shuffles cost 1.4 us more -- the extra
+// VALU work outweighs the LDS round trips at 4 waves per SIMD.)  Used where
+// the packed 64-bit key below cannot order a bucket.
 template <int R>
-__device__ __forceinline__ void waveBitonicSort(uint64_t (&h)[R], uint32_t (&r)[R], uint32_t lane) {
+__device__ __forceinline__ void waveBitonicSort(uint64_t (&h)[R], uint32_t (&r)[R], uint32_t (&x)[R],
+                                                uint32_t lane) {
   constexpr uint32_t N = 64 * R;
 #pragma unroll
   for (uint32_t k = 2; k <= N; k <<= 1) {
@@ -142,6 +145,9 @@ __device__ __forceinline__ void waveBitonicSort(uint64_t (&h)[R], uint32_t (&r)[
               uint32_t tr = r[s];
               r[s] = r[sp];
               r[sp] = tr;
+              uint32_t tx = x[s];
+              x[s] = x[sp];
+              x[sp] = tx;
             }
           }
         }
@@ -151,6 +157,7 @@ __device__ __forceinline__ void waveBitonicSort(uint64_t (&h)[R], uint32_t (&r)[
           const uint32_t e = static_cast<uint32_t>(s) * 64 + lane;
           const uint64_t ho = __shfl_xor(h[s], static_cast<int>(j));
           const uint32_t ro = __shfl_xor(r[s], static_cast<int>(j));
+          const uint32_t xo = __shfl_xor(x[s], static_cast<int>(j));
           const bool lower = (lane & j) == 0;
           const bool asc = (e & k) == 0;
           const bool keepMin = lower == asc;
@@ -160,7 +167,47 @@ __device__ __forceinline__ void waveBitonicSort(uint64_t (&h)[R], uint32_t (&r)[
           if (take) {
             h[s] = ho;
             r[s] = ro;
+            x[s] = xo;
           }
+        }
+      }
+    }
+  }
+}
+
+// Bitonic sort of 64*R packed 64-bit keys, R per lane: one 64-bit compare
+// and one 64-bit exchange per step (the bucket stage's common case, where
+// the key packs the hash bits below the bucket digit, the circuit-local
+// rank and the slot index).  Keys are distinct except identical padding.
+template <int R>
+__device__ __forceinline__ void waveBitonicSort64(uint64_t (&key)[R], uint32_t lane) {
+  constexpr uint32_t N = 64 * R;
+#pragma unroll
+  for (uint32_t k = 2; k <= N; k <<= 1) {
+#pragma unroll
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      if (j >= 64) {
+        const uint32_t js = j >> 6;
+#pragma unroll
+        for (int s = 0; s < R; ++s) {
+          const int sp = s ^ static_cast<int>(js);
+          if (sp > s) {
+            const uint32_t e = static_cast<uint32_t>(s) * 64 + lane;
+            const bool asc = (e & k) == 0;
+            if ((key[sp] < key[s]) == asc) {
+              const uint64_t t = key[s];
+              key[s] = key[sp];
+              key[sp] = t;
+            }
+          }
+        }
+      } else {
+#pragma unroll
+        for (int s = 0; s < R; ++s) {
+          const uint32_t e = static_cast<uint32_t>(s) * 64 + lane;
+          const uint64_t ko = __shfl_xor(key[s], static_cast<int>(j));
+          const bool keepMin = ((lane & j) == 0) == ((e & k) == 0);
+          if ((ko < key[s]) == keepMin) key[s] = ko;
         }
       }
     }
@@ -179,8 +226,18 @@ struct RunCarry {
   uint32_t havePrev;
 };
 
-// Per-node record written by classify and read by the merge, one 16-byte
-// load per node: {probability (2 words), 32-bit second hash, class size}.
+// Bucket slot written by classify, one per (node, bucket slot), 16 bytes:
+// {hash low, hash high, rank, bucketBits << 24 | circuit-local rank}; a local
+// rank of kLocalRankNone marks circuits too large for the packed sort key.
+// Beside it, a node record per rank, written in rank order: {probability
+// low, high, second hash, class size}.  The bucket stage keeps both in LDS as
+// one 32-byte SlotRec {slot, node record}.
+struct SlotRec {
+  uint4 a, b;
+};
+constexpr uint32_t kLocalRankBits = 24;
+constexpr uint32_t kLocalRankNone = (1u << kLocalRankBits) - 1;
+
 __device__ __forceinline__ uint4 nodeInfo(double p, uint32_t h2, uint32_t nnz) {
   const uint64_t b = __double_as_longlong(p);
   return make_uint4(static_cast<uint32_t>(b), static_cast<uint32_t>(b >> 32), h2, nnz);
@@ -190,7 +247,6 @@ __device__ __forceinline__ double infoProb(const uint4 &x) {
 }
 
 struct RunOut {
-  const uint4 *info;       // rank -> nodeInfo
   uint32_t *sortedRank;    // bucket-sorted ranks (in place over the bucket slots)
   uint32_t *runRank;       // per run: representative (smallest) rank
   double *runP;            // per run: merged probability
@@ -203,10 +259,11 @@ struct RunOut {
 // Segmented XOR-merge of one chunk of sorted keys (cnt valid elements,
 // positions chunkBase.. within bucket starting at bs) continuing the runs in
 // `cy`.  Runs are maximal sequences of equal hash; each run's probability is
-// the XOR-merge of its members, its representative the first member.  The
-// values the merge needs come from each member's node record (rank-indexed).
+// the XOR-merge of its members, its representative the first member.  Each
+// element comes with its probability, second hash and class size.
 template <int R>
-__device__ __forceinline__ void scanRuns(const uint64_t (&h)[R], const uint32_t (&rk)[R], uint32_t cnt,
+__device__ __forceinline__ void scanRuns(const uint64_t (&h)[R], const uint32_t (&rk)[R], const double (&pv)[R],
+                                         const uint32_t (&h2v)[R], const uint32_t (&nzv)[R], uint32_t cnt,
                                          uint32_t chunkBase, uint32_t bs, RunCarry &cy, uint32_t lane,
                                          const RunOut &o) {
   bool f[R];
@@ -221,34 +278,25 @@ __device__ __forceinline__ void scanRuns(const uint64_t (&h)[R], const uint32_t 
     const bool hasPrev = (s > 0 || lane > 0) ? true : (cy.havePrev != 0);
     f[s] = e < cnt && (!hasPrev || h[s] != ph);
   }
-  // every slot's node records first: loads issued after this wave's
-  // stores would wait for them (memory operations retire in order)
-  uint4 infs[R];
-#pragma unroll
-  for (int s = 0; s < R; ++s) {
-    const uint32_t e = static_cast<uint32_t>(s) * 64 + lane;
-    infs[s] = e < cnt ? o.info[rk[s]] : make_uint4(0u, 0u, 0u, 0u);
-  }
 #pragma unroll
   for (int s = 0; s < R; ++s) {
     const uint32_t e = static_cast<uint32_t>(s) * 64 + lane;
     if (static_cast<uint32_t>(s) * 64 >= cnt) break;
     const bool valid = e < cnt;
     const uint32_t r = rk[s];
-    const uint4 inf = infs[s];
-    const uint32_t ownH2 = inf.z;
-    double v = infoProb(inf);
+    const uint32_t ownH2 = h2v[s];
+    double v = pv[s];
     uint32_t hf = ownH2;
     int fl = f[s] ? 1 : 0;
-    const uint32_t nz = f[s] ? inf.w : 0u;
+    const uint32_t nz = f[s] ? nzv[s] : 0u;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
-      const double pv = __shfl_up(v, d);
+      const double pvs = __shfl_up(v, d);
       const uint32_t ph2 = __shfl_up(hf, d);
       const int pf = __shfl_up(fl, d);
       if (lane >= static_cast<uint32_t>(d)) {
         if (!fl) {
-          v = xmerge(pv, v);
+          v = xmerge(pvs, v);
           hf = ph2;
         }
         fl |= pf;
@@ -958,9 +1006,9 @@ struct ClassifyOut {
   uint4 *info;            // per node: nodeInfo(probability, second hash, class size)
   uint32_t *bucketCount;  // bucket b's counter at b * countStride
   uint32_t countStride;
-  uint4 *pslot;           // bucket slots: {hash low, hash high, rank, 0}
-  uint64_t *ovKey;        // overflow list (count in flags[3])
-  uint32_t *ovRank, *ovBucket;
+  uint4 *pslot;           // bucket slots
+  uint4 *ovSlot;          // overflow list (count in flags[3]) and each entry's bucket
+  uint32_t *ovBucket;
   uint32_t *flags;
   uint32_t *stash;
   uint32_t stashMaxCols;  // circuits with more columns get no rows (0: none do)
@@ -1060,34 +1108,144 @@ __global__ __launch_bounds__(kClassBlock) void classifyKernel(
     }
   }
   if (i < d.numRanks && cnt) {
+    const uint32_t local = d.numRanks < kLocalRankNone ? i : kLocalRankNone;
+    const uint4 slot = make_uint4(static_cast<uint32_t>(h), static_cast<uint32_t>(h >> 32),
+                                  static_cast<uint32_t>(gi), d.bucketBits << kLocalRankBits | local);
     if (pos < kBucketCap) {
-      o.pslot[static_cast<uint64_t>(bkt) * kBucketCap + pos] =
-          make_uint4(static_cast<uint32_t>(h), static_cast<uint32_t>(h >> 32), static_cast<uint32_t>(gi), 0u);
+      o.pslot[static_cast<uint64_t>(bkt) * kBucketCap + pos] = slot;
     } else {
       const uint32_t k = atomicAdd(&o.flags[3], 1u);
-      o.ovKey[k] = h;
-      o.ovRank[k] = static_cast<uint32_t>(gi);
+      o.ovSlot[k] = slot;
       o.ovBucket[k] = bkt;
     }
   }
 }
 
+// A wave's LDS writes made visible to its other lanes (a wave's LDS
+// accesses complete in order; the fences keep the compiler from moving the
+// reads above the writes).
+__device__ __forceinline__ void waveLdsSync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+}
+
+// Element e - 1's value of a register array held 64 per slot (lane 0 of
+// slot s takes lane 63 of slot s - 1; e = 0 gets `first`).
+template <int R, typename T>
+__device__ __forceinline__ T prevElement(const T (&v)[R], int s, uint32_t lane, T first) {
+  T p = __shfl_up(v[s], 1);
+  const T prevSlotLast = __shfl(v[s > 0 ? s - 1 : 0], 63);
+  if (lane == 0) p = s > 0 ? prevSlotLast : first;
+  return p;
+}
+
+// One bucket of at most 64 R items, by one wave.  The sort runs on one
+// 64-bit key per item: the 32 hash bits right below the bucket digit, the
+// circuit-local rank and the slot position.  Where every pair of items with
+// equal key bits has equal hashes (checked after the sort; two distinct
+// hashes sharing those 32 bits within a bucket are rare), that order is the
+// (hash, rank) order; otherwise, and for circuits whose ranks do not fit the
+// key, the bucket is sorted again on the full (hash, rank).  The items' node
+// records are requested as soon as their slots arrive, so that gather's
+// latency passes during the sort; slots and records then meet in the wave's
+// LDS by slot position, where the merge reads them in sorted order.
 template <int R>
-__device__ void bucketWave(uint32_t b, uint32_t bs, uint32_t cnt, uint32_t lane,
-                           const uint4 *__restrict__ pslot, const RunOut &o,
+__device__ void bucketWave(uint32_t b, uint32_t bs, uint32_t cnt, uint32_t lane, const uint4 *__restrict__ pslot,
+                           const uint4 *__restrict__ info, SlotRec *pay, const RunOut &o,
                            uint32_t *__restrict__ runCount, uint32_t *__restrict__ runNnz) {
-  uint64_t h[R];
-  uint32_t r[R];
+  uint64_t key[R], h[R];
+  uint32_t r[R], li[R];
+  uint32_t meta = 0;
 #pragma unroll
   for (int s = 0; s < R; ++s) {
     const uint32_t e = static_cast<uint32_t>(s) * 64 + lane;
-    const uint4 x = e < cnt ? pslot[bs + e] : make_uint4(~0u, ~0u, NONE, 0u);
-    h[s] = (static_cast<uint64_t>(x.y) << 32) | x.x;
-    r[s] = x.z;
+    key[s] = ~0ull;
+    h[s] = ~0ull;
+    r[s] = NONE;
+    li[s] = e;
+    if (e < cnt) {
+      const uint4 x = pslot[bs + e];
+      pay[e].a = x;
+      h[s] = (static_cast<uint64_t>(x.y) << 32) | x.x;
+      r[s] = x.z;
+      const uint32_t bits = x.w >> kLocalRankBits;
+      const uint64_t sub = (h[s] << bits) >> 32;
+      key[s] = (sub << 32) | (static_cast<uint64_t>(x.w & kLocalRankNone) << 8) | e;
+      if (s == 0) meta = x.w;
+    }
   }
-  waveBitonicSort<R>(h, r, lane);
+  // node records in flight during the sort
+  uint4 rec[R];
+#pragma unroll
+  for (int s = 0; s < R; ++s) {
+    const uint32_t e = static_cast<uint32_t>(s) * 64 + lane;
+    rec[s] = e < cnt ? info[r[s]] : make_uint4(0u, 0u, 0u, 0u);
+  }
+  // every item of a bucket comes from one circuit (lane 0's item exists)
+  const bool packed = (__builtin_amdgcn_readfirstlane(meta) & kLocalRankNone) != kLocalRankNone;
+  if (packed) waveBitonicSort64<R>(key, lane);
+#pragma unroll
+  for (int s = 0; s < R; ++s) {
+    const uint32_t e = static_cast<uint32_t>(s) * 64 + lane;
+    if (e < cnt) pay[e].b = rec[s];
+  }
+  waveLdsSync();
+  bool ordered = false;
+  double pv[R];
+  uint32_t h2v[R], nzv[R];
+  if (packed) {
+    bool clash = false;
+    uint32_t hi[R];
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+      const uint32_t e = static_cast<uint32_t>(s) * 64 + lane;
+      li[s] = static_cast<uint32_t>(key[s]) & 0xFFu;
+      hi[s] = static_cast<uint32_t>(key[s] >> 32);
+      h[s] = ~0ull;
+      r[s] = NONE;
+      if (e < cnt) {
+        const SlotRec x = pay[li[s]];
+        h[s] = (static_cast<uint64_t>(x.a.y) << 32) | x.a.x;
+        r[s] = x.a.z;
+        pv[s] = infoProb(x.b);
+        h2v[s] = x.b.z;
+        nzv[s] = x.b.w;
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+      const uint32_t e = static_cast<uint32_t>(s) * 64 + lane;
+      const uint32_t phi = prevElement<R>(hi, s, lane, 0u);
+      const uint64_t ph = prevElement<R>(h, s, lane, static_cast<uint64_t>(0));
+      if (e < cnt && e > 0 && phi == hi[s] && ph != h[s]) clash = true;
+    }
+    ordered = __ballot(clash) == 0ull;
+  }
+  if (!ordered) {
+    waveBitonicSort<R>(h, r, li, lane);
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+      const uint32_t e = static_cast<uint32_t>(s) * 64 + lane;
+      if (e < cnt) {
+        const uint4 x = pay[li[s]].b;
+        pv[s] = infoProb(x);
+        h2v[s] = x.z;
+        nzv[s] = x.w;
+      }
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < R; ++s) {
+    const uint32_t e = static_cast<uint32_t>(s) * 64 + lane;
+    if (e >= cnt) {
+      pv[s] = 0.0;
+      h2v[s] = 0u;
+      nzv[s] = 0u;
+    }
+  }
   RunCarry cy{0.0, 0ull, 0u, 0u, 0u, 0u};
-  scanRuns<R>(h, r, cnt, 0u, bs, cy, lane, o);
+  scanRuns<R>(h, r, pv, h2v, nzv, cnt, 0u, bs, cy, lane, o);
   if (lane == 0) {
     runCount[b] = cy.runs;
     runNnz[b] = cy.nnzSum;
@@ -1098,8 +1256,9 @@ struct BucketIo {
   uint32_t *bucketCount;  // read and reset to zero for the next run
   uint32_t countStride;
   const uint4 *pslot;     // slots written by classify
-  const uint64_t *ovKey;  // overflow list (count in flags[3])
-  const uint32_t *ovRank, *ovBucket;
+  const uint4 *info;      // node records written by classify
+  const uint4 *ovSlot;    // overflow list (count in flags[3]) and each entry's bucket
+  const uint32_t *ovBucket;
   uint32_t *runCount, *runNnz, *itemCount;  // per bucket
   uint32_t *slotBase;     // per bucket: where its sorted items and runs start
   uint32_t *wgSum;        // per workgroup: its buckets' runs, targets, items
@@ -1110,8 +1269,9 @@ struct BucketIo {
 
 // An oversized bucket (more than kBucketCap items), by the whole workgroup:
 // its items (slots, then its entries of the overflow list) are copied to a
-// workspace, bitonic-sorted there, and merged chunk by chunk by wave 0; the
-// sorted items and runs go to an arena range taken with one atomic.
+// workspace, bitonic-sorted there on (hash, rank), and merged chunk by chunk
+// by wave 0 with the items' node records; the sorted items and runs go to an
+// arena range taken with one atomic.
 __device__ void bigBucket(uint32_t b, uint32_t cnt, const BucketIo &io, const RunOut &o) {
   __shared__ uint32_t shOff, shFill;
   const uint32_t lane = threadIdx.x & 63;
@@ -1125,7 +1285,7 @@ __device__ void bigBucket(uint32_t b, uint32_t cnt, const BucketIo &io, const Ru
   while (P < cnt) P <<= 1;
   uint64_t *K = io.scratchKey + 2ull * off;  // P < 2 * cnt
   uint32_t *Rk = io.scratchRank + 2ull * off;
-  const uint64_t slot0 = static_cast<uint64_t>(b) * kBucketCap;
+  const uint32_t slot0 = b * kBucketCap;
   for (uint32_t i = threadIdx.x; i < kBucketCap; i += blockDim.x) {
     const uint4 x = io.pslot[slot0 + i];
     K[i] = (static_cast<uint64_t>(x.y) << 32) | x.x;
@@ -1135,8 +1295,9 @@ __device__ void bigBucket(uint32_t b, uint32_t cnt, const BucketIo &io, const Ru
   for (uint32_t k = threadIdx.x; k < nov; k += blockDim.x)
     if (io.ovBucket[k] == b) {
       const uint32_t at = atomicAdd(&shFill, 1u);
-      K[at] = io.ovKey[k];
-      Rk[at] = io.ovRank[k];
+      const uint4 x = io.ovSlot[k];
+      K[at] = (static_cast<uint64_t>(x.y) << 32) | x.x;
+      Rk[at] = x.z;
     }
   for (uint32_t i = cnt + threadIdx.x; i < P; i += blockDim.x) {
     K[i] = ~0ull;
@@ -1167,12 +1328,19 @@ __device__ void bigBucket(uint32_t b, uint32_t cnt, const BucketIo &io, const Ru
     RunCarry cy{0.0, 0ull, 0u, 0u, 0u, 0u};
     for (uint32_t c0 = 0; c0 < cnt; c0 += 64) {
       uint64_t h[1];
-      uint32_t r[1];
+      uint32_t r[1], h2v[1] = {0u}, nzv[1] = {0u};
+      double pv[1] = {0.0};
       const uint32_t e = c0 + lane;
       h[0] = e < cnt ? K[e] : ~0ull;
       r[0] = e < cnt ? Rk[e] : NONE;
+      if (e < cnt) {
+        const uint4 x = io.info[r[0]];
+        pv[0] = infoProb(x);
+        h2v[0] = x.z;
+        nzv[0] = x.w;
+      }
       const uint32_t n = cnt - c0 < 64 ? cnt - c0 : 64;
-      scanRuns<1>(h, r, n, c0, base, cy, lane, o);
+      scanRuns<1>(h, r, pv, h2v, nzv, n, c0, base, cy, lane, o);
     }
     if (lane == 0) {
       io.runCount[b] = cy.runs;
@@ -1188,8 +1356,9 @@ __device__ void bigBucket(uint32_t b, uint32_t cnt, const BucketIo &io, const Ru
 // workgroup (bigBucket).  Each bucket's counter is reset for the next run.
 __global__ __launch_bounds__(256) void bucketKernel(uint32_t numBuckets, BucketIo io, RunOut o) {
   __shared__ uint32_t bigB[4], bigCnt[4], nBig;
-  const uint32_t lane = threadIdx.x & 63;
-  const uint32_t b = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+  __shared__ SlotRec pay[4][kBucketCap];  // each wave's bucket records, by slot position
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t b = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wave);
   if (threadIdx.x == 0) nBig = 0;
   __syncthreads();
   if (b < numBuckets) {
@@ -1208,11 +1377,11 @@ __global__ __launch_bounds__(256) void bucketKernel(uint32_t numBuckets, BucketI
           io.runNnz[b] = 0;
         }
       } else if (cnt <= 64) {
-        bucketWave<1>(b, bs, cnt, lane, io.pslot, o, io.runCount, io.runNnz);
+        bucketWave<1>(b, bs, cnt, lane, io.pslot, io.info, pay[wave], o, io.runCount, io.runNnz);
       } else if (cnt <= 128) {
-        bucketWave<2>(b, bs, cnt, lane, io.pslot, o, io.runCount, io.runNnz);
+        bucketWave<2>(b, bs, cnt, lane, io.pslot, io.info, pay[wave], o, io.runCount, io.runNnz);
       } else {
-        bucketWave<4>(b, bs, cnt, lane, io.pslot, o, io.runCount, io.runNnz);
+        bucketWave<4>(b, bs, cnt, lane, io.pslot, io.info, pay[wave], o, io.runCount, io.runNnz);
       }
     } else if (lane == 0) {
       const uint32_t k = atomicAdd(&nBig, 1u);
@@ -1672,27 +1841,94 @@ __global__ void locationGradKernel(const uint32_t *__restrict__ locStart, const 
 // host-side buffers
 // --------------------------------------------------------------------------
 
+// Device allocations are made in whole 2 MiB units, so every buffer starts
+// on its own 2 MiB boundary.  (Measured: left to the runtime, buffers below a
+// few MiB are packed into shared small-allocation blocks, and the class
+// stage's 246k bucket-counter atomics on bb12 ran 7 us slower when the
+// counters landed in such a block.)
+constexpr size_t kAllocGrain = size_t{2} << 20;
+size_t roundGrain(size_t x) { return (x + kAllocGrain - 1) & ~(kAllocGrain - 1); }
+
+// A device buffer: its own allocation (`owned`), or a range of a pipeline's
+// work arena (see WorkArena).
 struct DevBuf {
   void *p = nullptr;
   size_t cap = 0;
-  ~DevBuf() {
-    if (p) (void)hipFree(p);
+  bool owned = false;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p && owned) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    owned = false;
   }
+  // Its own allocation of at least `bytes` (kept when large enough).
   void ensure(size_t bytes, bool frozen, const char *what) {
     if (bytes <= cap) return;
     if (frozen)
       throw RuntimeError(std::string("circuit exceeds the driver's capacity (") + what +
                          "); create the driver from a larger circuit");
-    if (p) GPA_HIP(hipFree(p));
-    p = nullptr;
-    cap = 0;
-    const size_t sz = std::max<size_t>(bytes, 256);
+    release();
+    const size_t sz = roundGrain(std::max<size_t>(bytes, 256));
     GPA_HIP(hipMalloc(&p, sz));
     cap = sz;
+    owned = true;
   }
   template <typename T>
   T *as() const {
     return static_cast<T *>(p);
+  }
+};
+
+// The work buffers of one load laid out in a single device allocation: each
+// buffer of at least 1 MiB starts on a 2 MiB boundary, smaller ones are
+// packed at 4 KiB alignment.  A load requests every buffer's size; the
+// layout is kept while every request fits its buffer's capacity, and is
+// otherwise redone with each capacity raised to its request (a frozen
+// pipeline throws instead).
+struct WorkArena {
+  struct Req {
+    DevBuf *buf;
+    size_t bytes;
+    const char *what;
+  };
+  DevBuf mem;
+  std::vector<Req> reqs;
+  void request(DevBuf &b, size_t bytes, const char *what) { reqs.push_back(Req{&b, std::max<size_t>(bytes, 256), what}); }
+  void commit(bool frozen) {
+    // this load's requests (none stay behind if a capacity check throws)
+    std::vector<Req> rq;
+    rq.swap(reqs);
+    bool fits = mem.p != nullptr;
+    for (const Req &r : rq)
+      if (r.bytes > r.buf->cap) {
+        if (frozen)
+          throw RuntimeError(std::string("circuit exceeds the driver's capacity (") + r.what +
+                             "); create the driver from a larger circuit");
+        fits = false;
+      }
+    if (!fits) {
+      size_t off = 0;
+      std::vector<size_t> offs(rq.size()), caps(rq.size());
+      for (size_t k = 0; k < rq.size(); ++k) {
+        const size_t c = std::max(rq[k].bytes, rq[k].buf->cap);
+        const size_t align = c >= (size_t{1} << 20) ? kAllocGrain : 4096;
+        off = (off + align - 1) & ~(align - 1);
+        offs[k] = off;
+        caps[k] = c;
+        off += c;
+      }
+      mem.release();
+      mem.ensure(off, false, "work buffers");
+      for (size_t k = 0; k < rq.size(); ++k) {
+        rq[k].buf->p = mem.as<char>() + offs[k];
+        rq[k].buf->cap = caps[k];
+        rq[k].buf->owned = false;
+      }
+    }
   }
 };
 
@@ -1812,15 +2048,16 @@ struct Pipeline::Impl {
   uint32_t countStride = 1;
 
   // per node: stash, info (probability, second hash, class size), overflow
-  // list (ovKey/ovRank/ovBucket);
+  // list (ovSlot/ovBucket);
   // per bucket slot (numBuckets * kBucketCap, plus a numRanks arena for
   // oversized buckets): pslot (classify's records), prank (the sorted
   // ranks) and the run arrays;
   // per bucket: counters, slotBase, item/run/target counts and their scans
-  DevBuf band, S, stash, info, counters, bucketCnt, ovKey, ovRank, ovBucket, pslot, prank,
+  DevBuf band, S, stash, info, counters, bucketCnt, ovSlot, ovBucket, pslot, prank,
       runRank, runP, runHash, runTgtPre, runMember, runCount, runNnz, itemCount, slotBase, wgSum, classStart,
       tgtStart, memberStart, outP, outOff, outHash, outMember, outTargets, scratchKey, scratchRank,
       scanTmp, vBounds, members;
+  WorkArena work;  // holds the buffers above except in, outTargets, members and the split buffers
   uint64_t capTargets = 0;
 
   // host results: the block of the last download, blocks released by
@@ -2072,55 +2309,55 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
     }
   }
 
-  // work buffers
+  // work buffers: one arena (see WorkArena), the targets' own allocation
   const uint64_t R = std::max<uint64_t>(m.R, 1), B = m.B;
-  m.band.ensure(8 * std::max<uint32_t>(m.Ccols, 1), frozen, "columns");
-  m.S.ensure(4 * std::max<uint64_t>(m.Swords, 1), frozen, "sensitivity matrix");
-  m.info.ensure(16 * R, frozen, "nodes");
-  m.stash.ensure(4ull * kStashWords * R, frozen, "nodes");
-  m.ovKey.ensure(8 * R, frozen, "nodes");
-  m.ovRank.ensure(4 * R, frozen, "nodes");
-  m.ovBucket.ensure(4 * R, frozen, "nodes");
-  m.counters.ensure(4 * kFlagWords, frozen, "buckets");
+  WorkArena &wa = m.work;
+  wa.request(m.band, 8 * std::max<uint32_t>(m.Ccols, 1), "columns");
+  wa.request(m.S, 4 * std::max<uint64_t>(m.Swords, 1), "sensitivity matrix");
+  wa.request(m.info, 16 * R, "nodes");
+  wa.request(m.stash, 4ull * kStashWords * R, "nodes");
+  wa.request(m.ovSlot, 16 * R, "nodes");
+  wa.request(m.ovBucket, 4 * R, "nodes");
+  wa.request(m.counters, 4 * kFlagWords, "buckets");
   m.countStride = 4;
   while (m.countStride > 1 && static_cast<uint64_t>(B) * m.countStride > (1u << 17)) m.countStride >>= 1;
   // (B + 1) * countStride <= 2^17 + 32 words whatever B is, so a reserved
   // pipeline fits every later load)
-  m.bucketCnt.ensure(4 * std::max<uint64_t>((static_cast<uint64_t>(B) + 1) * m.countStride, (1u << 17) + 32),
-                     frozen, "buckets");
+  wa.request(m.bucketCnt, 4 * std::max<uint64_t>((static_cast<uint64_t>(B) + 1) * m.countStride, (1u << 17) + 32),
+             "buckets");
   // bucket slots, then an arena of numRanks entries for oversized buckets
   const uint64_t slots = B * static_cast<uint64_t>(kBucketCap) + R;
   if (slots >= 0xFFFFFFFFull) throw RuntimeError("too many sort buckets in one load");
-  m.pslot.ensure(16 * slots, frozen, "nodes");
-  m.prank.ensure(4 * slots, frozen, "nodes");
-  m.runRank.ensure(4 * slots, frozen, "nodes");
-  m.runP.ensure(8 * slots, frozen, "nodes");
-  m.runHash.ensure(8 * slots, frozen, "nodes");
-  m.runTgtPre.ensure(4 * slots, frozen, "nodes");
-  m.runMember.ensure(4 * slots, frozen, "nodes");
-  m.runCount.ensure(4 * (B + 1), frozen, "buckets");
-  m.runNnz.ensure(4 * (B + 1), frozen, "buckets");
-  m.itemCount.ensure(4 * (B + 1), frozen, "buckets");
-  m.slotBase.ensure(4 * (B + 1), frozen, "buckets");
-  m.wgSum.ensure(4 * 3 * ((B + 3) / 4 + 1), frozen, "buckets");
-  m.classStart.ensure(4 * (B + 1), frozen, "buckets");
-  m.tgtStart.ensure(4 * (B + 1), frozen, "buckets");
-  m.memberStart.ensure(4 * (B + 1), frozen, "buckets");
-  m.outP.ensure(8 * R, frozen, "nodes");
-  m.outOff.ensure(4 * (R + 1), frozen, "nodes");
-  m.outHash.ensure(8 * R, frozen, "nodes");
-  m.outMember.ensure(4 * (R + 1), frozen, "nodes");
+  wa.request(m.pslot, 16 * slots, "nodes");
+  wa.request(m.prank, 4 * slots, "nodes");
+  wa.request(m.runRank, 4 * slots, "nodes");
+  wa.request(m.runP, 8 * slots, "nodes");
+  wa.request(m.runHash, 8 * slots, "nodes");
+  wa.request(m.runTgtPre, 4 * slots, "nodes");
+  wa.request(m.runMember, 4 * slots, "nodes");
+  wa.request(m.runCount, 4 * (B + 1), "buckets");
+  wa.request(m.runNnz, 4 * (B + 1), "buckets");
+  wa.request(m.itemCount, 4 * (B + 1), "buckets");
+  wa.request(m.slotBase, 4 * (B + 1), "buckets");
+  wa.request(m.wgSum, 4 * 3 * ((B + 3) / 4 + 1), "buckets");
+  wa.request(m.classStart, 4 * (B + 1), "buckets");
+  wa.request(m.tgtStart, 4 * (B + 1), "buckets");
+  wa.request(m.memberStart, 4 * (B + 1), "buckets");
+  wa.request(m.outP, 8 * R, "nodes");
+  wa.request(m.outOff, 4 * (R + 1), "nodes");
+  wa.request(m.outHash, 8 * R, "nodes");
+  wa.request(m.outMember, 4 * (R + 1), "nodes");
+  wa.request(m.scratchKey, 16 * R, "nodes");
+  wa.request(m.scratchRank, 8 * R, "nodes");
+  wa.request(m.scanTmp, 4 * kScanArrays * ((B + kScanTile - 1) / kScanTile + 2), "buckets");
+  wa.request(m.vBounds, 4 * (2 * V + 8), "circuits");
+  wa.commit(frozen);
   if (m.capTargets == 0 || (!frozen && m.capTargets < 16 * R)) {
     m.outTargets.ensure(4 * 16 * R, frozen, "targets");
     m.capTargets = m.outTargets.cap / 4;
   }
-  m.scratchKey.ensure(16 * R, frozen, "nodes");
-  m.scratchRank.ensure(8 * R, frozen, "nodes");
-  m.scanTmp.ensure(4 * kScanArrays * ((B + kScanTile - 1) / kScanTile + 2), frozen, "buckets");
-  m.vBounds.ensure(4 * (2 * V + 8), frozen, "circuits");
   m.hVB.ensure(4 * (2 * V + 8));
   m.hFlags.ensure(4 * kFlagWords);
-
   if (m.sweepLdsBytes > 64 * 1024) {
     const void *fns[] = {reinterpret_cast<const void *>(sweepKernel<1, false>),
                          reinterpret_cast<const void *>(sweepKernel<2, false>),
@@ -2224,10 +2461,10 @@ void Pipeline::run(bool timed) {
   mark(1);
   m.joinSide();  // classify reads the error nodes' inputs
   if (m.nTiles) {
-    ClassifyOut co{m.info.as<uint4>(), m.bucketCount(),
-                   m.countStride, m.pslot.as<uint4>(), m.ovKey.as<uint64_t>(),
-                   m.ovRank.as<uint32_t>(), m.ovBucket.as<uint32_t>(), m.flags(), m.stash.as<uint32_t>(),
-                   stashMaxCols, m.stashShift};
+    ClassifyOut co{m.info.as<uint4>(),     m.bucketCount(),          m.countStride,
+                   m.pslot.as<uint4>(),    m.ovSlot.as<uint4>(),     m.ovBucket.as<uint32_t>(),
+                   m.flags(),              m.stash.as<uint32_t>(),   stashMaxCols,
+                   m.stashShift};
     if (m.classTable && gMaskTable.load() && m.classBatch == 16)
       classifyKernel<true, 16><<<m.nTiles, kClassBlock, 0, st>>>(desc, tiles, m.band.as<int2>(),
                                                                  m.S.as<uint32_t>(), rin, co);
@@ -2239,18 +2476,20 @@ void Pipeline::run(bool timed) {
                                                                  m.S.as<uint32_t>(), rin, co);
   }
   mark(2);
-  RunOut o{m.info.as<uint4>(),
-           m.prank.as<uint32_t>(),
+  RunOut o{m.prank.as<uint32_t>(),
            m.runRank.as<uint32_t>(),
            m.runP.as<double>(),
            m.runHash.as<uint64_t>(),
            m.runTgtPre.as<uint32_t>(),
            m.runMember.as<uint32_t>(),
            m.flags()};
-  BucketIo bio{m.bucketCount(), m.countStride, m.pslot.as<uint4>(), m.ovKey.as<uint64_t>(),
-               m.ovRank.as<uint32_t>(), m.ovBucket.as<uint32_t>(), m.runCount.as<uint32_t>(),
-               m.runNnz.as<uint32_t>(), m.itemCount.as<uint32_t>(), m.slotBase.as<uint32_t>(),
-               m.wgSum.as<uint32_t>(), m.scratchKey.as<uint64_t>(), m.scratchRank.as<uint32_t>(),
+  BucketIo bio{m.bucketCount(),           m.countStride,
+               m.pslot.as<uint4>(),         m.info.as<uint4>(),
+               m.ovSlot.as<uint4>(),        m.ovBucket.as<uint32_t>(),
+               m.runCount.as<uint32_t>(),
+               m.runNnz.as<uint32_t>(),     m.itemCount.as<uint32_t>(),
+               m.slotBase.as<uint32_t>(),   m.wgSum.as<uint32_t>(),
+               m.scratchKey.as<uint64_t>(), m.scratchRank.as<uint32_t>(),
                B * kBucketCap};
   if (B) bucketKernel<<<(B + 3) / 4, 256, 0, st>>>(B, bio, o);
   mark(3);
@@ -2677,17 +2916,10 @@ uint32_t bucketFill() { return gBucketFill.load(); }
 
 uint64_t Pipeline::deviceBytes() const {
   const Impl &m = *impl_;
-  const DevBuf *bufs[] = {&m.in,        &m.band,     &m.S,          &m.stash,     &m.info,
-                          &m.counters,   &m.bucketCnt, &m.ovKey,
-                          &m.ovRank,    &m.ovBucket,   &m.pslot,      &m.prank,     &m.runRank,
-                          &m.runP,      &m.runHash,    &m.runTgtPre,  &m.runMember, &m.runCount,
-                          &m.runNnz,    &m.itemCount,  &m.slotBase, &m.wgSum,   &m.classStart, &m.tgtStart,
-                          &m.memberStart, &m.outP,     &m.outOff,     &m.outHash,   &m.outMember,
-                          &m.outTargets, &m.scratchKey, &m.scratchRank, &m.scanTmp, &m.vBounds,
-                          &m.members};
-  uint64_t s = 0;
-  for (const DevBuf *b : bufs) s += b->cap;
-  return s;
+  // the work arena and the buffers with their own allocations
+  uint64_t b = m.work.mem.cap;
+  for (const DevBuf *x : {&m.in, &m.outTargets, &m.members, &m.splitBuf, &m.splitState}) b += x->cap;
+  return b;
 }
 
 void Pipeline::sweepPlan(uint32_t out[3]) const {
