@@ -726,6 +726,7 @@ int gp_set_tuning(const char *name, uint32_t value) {
     else if (std::string(name) == "stash") gpa::setStash(value != 0);
     else if (std::string(name) == "mask_table") gpa::setMaskTable(value != 0);
     else if (std::string(name) == "prob_table") gpa::setProbTable(value != 0);
+    else if (std::string(name) == "packed_sort") gpa::setPackedSort(value != 0);
     else if (std::string(name) == "sweep_split") {
       if (value > 2) throw InvalidArgument("sweep_split must be 0, 1 or 2");
       gpa::setSweepSplit(value);
@@ -742,6 +743,7 @@ int gp_get_tuning(const char *name, uint32_t *value) {
     else if (std::string(name) == "stash") *value = gpa::stash() ? 1u : 0u;
     else if (std::string(name) == "mask_table") *value = gpa::maskTable() ? 1u : 0u;
     else if (std::string(name) == "prob_table") *value = gpa::probTable() ? 1u : 0u;
+    else if (std::string(name) == "packed_sort") *value = gpa::packedSort() ? 1u : 0u;
     else if (std::string(name) == "sweep_split") *value = gpa::sweepSplit();
     else throw InvalidArgument(std::string("unknown tuning parameter: ") + name);
   });

@@ -141,6 +141,12 @@ bool stash();
 // workgroups whose table would not fit).  Read at each run.
 void setMaskTable(bool on);
 bool maskTable();
+// Whether the bucket stage may sort on the packed 64-bit key (hash bits
+// below the bucket digit, circuit-local rank, slot) -- on by default; off,
+// every bucket takes the exact (hash, rank) sort that the packed path falls
+// back to.  Same results either way.  Read at each run.
+void setPackedSort(bool on);
+bool packedSort();
 // Whether a load's long columns may be swept in time chunks after its first
 // run: 1 (default) where the estimated critical path gains, 2 every column
 // whose band reaches the bottom with the most chunks that fit (for tests), 0

@@ -128,7 +128,9 @@ int gp_driver_timings(const gp_driver *driver, double timings_us[8]);
 /* Compiles the already-uploaded circuit `iterations` times back to back on
  * the device (no host transfer, no DEM download) and returns the mean
  * device time per compile in microseconds; the driver keeps the result of the
- * last iteration on the device.  Used to time the device-resident path. */
+ * last iteration on the device.  Used to time the device-resident path.  The
+ * compiles go out as HIP graph replays or as direct launches, whichever the
+ * first call after a load measured faster over a few of each. */
 int gp_driver_bench_device(gp_driver *driver, uint32_t iterations,
                            double *mean_us);
 
@@ -218,6 +220,12 @@ int gp_estimate_device_bytes(const char *circuit_text, int correlation_level,
  *   indices into its table of distinct values when it has at most 65536 of
  *   them; 0 always sends one double per error node; read when a circuit is
  *   parsed.
+ *   "packed_sort": 1 (default) lets the bucket stage sort on one packed
+ *   64-bit key per node (the 32 hash bits below the bucket digit, the
+ *   node's rank within its circuit, its slot), falling back to the exact
+ *   (hash, rank) sort where two distinct hashes share those bits or a
+ *   circuit has 2^24 - 1 or more error nodes; 0 always sorts exactly (same
+ *   results); read at each compile.
  *   "sweep_split": 1 (default) lets a load's first compile plan a
  *   time-chunked sweep of columns much longer than the rest (transfer maps
  *   of the middle chunks, then the lower chunks in parallel; identical

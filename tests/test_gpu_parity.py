@@ -223,6 +223,37 @@ def test_emit_without_target_rows(oracle):
         autotune.set_tuning("stash", 1)
 
 
+def test_exact_bucket_sort_equals_packed_key_sort(oracle):
+    """The bucket stage's exact (hash, rank) sort -- where the packed 64-bit
+    key cannot order a bucket: two distinct hashes sharing the 32 bits below
+    the bucket digit, or a circuit with 2^24 - 1 or more error nodes -- gives
+    the same DEMs bit for bit as the packed key, alone and in a batch (one
+    circuit has 1,200-member classes, whose buckets take the oversized
+    path)."""
+    from greenpeas_amd import autotune
+    from greenpeas_amd.batch import BatchCompiler
+
+    names = ("surface3", "surface5", "bb12", "iceberg_surface_6_static_circl")
+    texts = [read_circuit(n) for n in names]
+    texts.append("R 0 1\n" + "TICK\nDEPOLARIZE1(0.001) 0 1\n" * 600
+                 + "TICK\nM 0 1\nDETECTOR rec[-2]\nDETECTOR rec[-1]\n")
+    packed = [get_driver(t).compile_detector_error_model() for t in texts]
+    bc = BatchCompiler()
+    bc.set_circuits(texts)
+    packed_batch = bc.compile()
+    try:
+        autotune.set_tuning("packed_sort", 0)
+        for t, dem0 in zip(texts, packed):
+            assert get_driver(t).compile_detector_error_model() == dem0
+        bc2 = BatchCompiler()
+        bc2.set_circuits(texts)
+        assert bc2.compile() == packed_batch
+    finally:
+        autotune.set_tuning("packed_sort", 1)
+    ref = oracle.compile_stepg(texts[2], 2)
+    assert np.array_equal(packed[2].targets, ref.targets)
+
+
 @pytest.mark.parametrize("d,rounds", [(5, 100), (7, 100), (5, 260)])
 def test_class_stage_masks_wider_than_64_columns(oracle, d, rounds):
     """Circuits with more than 64 detector columns (2,400 to 6,240

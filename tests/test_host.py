@@ -114,6 +114,10 @@ def test_tuning_knobs_roundtrip():
         assert autotune.get_tuning("prob_table") == 0
         autotune.set_tuning("prob_table", 1)
         assert autotune.get_tuning("prob_table") == 1
+        autotune.set_tuning("packed_sort", 0)
+        assert autotune.get_tuning("packed_sort") == 0
+        autotune.set_tuning("packed_sort", 1)
+        assert autotune.get_tuning("packed_sort") == 1
         autotune.set_tuning("sweep_split", 0)
         assert autotune.get_tuning("sweep_split") == 0
         autotune.set_tuning("sweep_split", 2)
