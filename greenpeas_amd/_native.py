@@ -26,7 +26,7 @@ EXPORTS = (
     "gp_dem_shape", "gp_dem_copy", "gp_dem_acquire", "gp_dem_release", "gp_dem_text", "gp_driver_params",
     "gp_driver_timings",
     "gp_driver_bench_device", "gp_debug_pnodes", "gp_debug_sensitivities", "gp_debug_stepg", "gp_debug_sweep_plan",
-    "gp_build_stepg_host",
+    "gp_build_stepg_host", "gp_debug_image_digest",
     "gp_batch_create", "gp_batch_destroy", "gp_batch_set_circuits", "gp_batch_compile",
     "gp_batch_stage_us", "gp_batch_device_us", "gp_batch_dem_shape", "gp_batch_dem_copy", "gp_batch_device_bytes",
     "gp_estimate_device_bytes", "gp_circuit_summary", "gp_set_tuning", "gp_get_tuning",
@@ -97,6 +97,7 @@ def load() -> ctypes.CDLL:
     sig("gp_debug_stepg", [P, P, P])
     sig("gp_debug_sweep_plan", [P, P])
     sig("gp_build_stepg_host", [cs, i32, P, P, P])
+    sig("gp_debug_image_digest", [cs, i32, i32, P])
     sig("gp_batch_create", [i32, i32, PP])
     sig("gp_batch_destroy", [P])
     sig("gp_batch_set_circuits", [P, ctypes.POINTER(cs), u32, u32])

@@ -12,7 +12,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libgreenpeas_amd.so")
-SOURCES = ["circuit.cpp", "stepg.cpp", "engine.hip", "api.cpp"]
+SOURCES = ["circuit.cpp", "workers.cpp", "stepg.cpp", "engine.hip", "api.cpp"]
 ARCH = os.environ.get("GREENPEAS_AMD_ARCH", "gfx950")
 
 

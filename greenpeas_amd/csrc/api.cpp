@@ -78,11 +78,13 @@ struct Parsed {
   gpa::StepgImage image;
 };
 
+// `parallel`: large circuits are built on the worker pool (callers that
+// build many circuits on their own threads pass false).
 void parseInto(const char *text, int level, Parsed &p, bool keepComponents = false,
-               const double *theta = nullptr) {
+               const double *theta = nullptr, bool parallel = true) {
   p.circuit = gpa::Circuit::parse(text);
-  gpa::buildStepg(p.circuit, level, p.stepg, theta, keepComponents);
-  gpa::buildImage(p.stepg, p.image);
+  gpa::buildStepg(p.circuit, level, p.stepg, theta, keepComponents, parallel);
+  gpa::buildImage(p.stepg, p.image, parallel);
 }
 
 }  // namespace
@@ -356,13 +358,47 @@ int gp_build_stepg_host(const char *circuit_text, int correlation_level, uint32_
     if (!dims) throw InvalidArgument("dims is NULL");
     gpa::Circuit c = gpa::Circuit::parse(circuit_text);
     gpa::Stepg g;
-    gpa::buildStepg(c, correlation_level, g);
+    gpa::buildStepg(c, correlation_level, g, nullptr, false, true);
     dims[0] = g.numLayers;
     dims[1] = g.npl;
     dims[2] = g.numNodes;
     dims[3] = g.numLocations;
     if (adjacency) std::memcpy(adjacency, g.adjacency.data(), 8ull * g.numNodes);
     if (probs) std::memcpy(probs, g.probs.data(), 8ull * g.numNodes);
+  });
+}
+
+int gp_debug_image_digest(const char *circuit_text, int correlation_level, int parallel, uint64_t digest[2]) {
+  return guarded([&] {
+    requireText(circuit_text);
+    checkLevel(correlation_level);
+    if (!digest) throw InvalidArgument("digest is NULL");
+    gpa::Circuit c = gpa::Circuit::parse(circuit_text);
+    gpa::Stepg g;
+    gpa::StepgImage img;
+    gpa::buildStepg(c, correlation_level, g, nullptr, false, parallel != 0);
+    gpa::buildImage(g, img, parallel != 0);
+    // FNV-1a-64 over the STEPG's and the image's arrays and sizes
+    uint64_t h = 0xcbf29ce484222325ull, n = 0;
+    auto mix = [&](const void *p, size_t bytes) {
+      const unsigned char *b = static_cast<const unsigned char *>(p);
+      for (size_t i = 0; i < bytes; ++i) h = (h ^ b[i]) * 0x100000001b3ull;
+      n += bytes;
+    };
+    auto vec = [&](const auto &v) {
+      const uint64_t sz = v.size();
+      mix(&sz, 8);
+      if (sz) mix(v.data(), sz * sizeof(v[0]));
+    };
+    vec(g.adjacency), vec(g.probs), vec(g.keys), vec(g.measBits);
+    const uint32_t dims[] = {img.numLayers, img.npl, img.numCols, img.numRanks, img.numSeeds, img.numActive,
+                             img.maxActive, img.sweepThreads, img.sweepRpt, img.layerStride, img.rowSpan,
+                             img.numPairs};
+    mix(dims, sizeof dims);
+    vec(img.layerAct), vec(img.act), vec(img.seedVals), vec(img.colHi), vec(img.colLow), vec(img.rankLayer);
+    vec(img.rankProb), vec(img.rankNode), vec(img.rankCode), vec(img.rankSpill), vec(img.probTab), vec(img.probIdx);
+    digest[0] = h;
+    digest[1] = n;
   });
 }
 
@@ -400,7 +436,7 @@ int gp_batch_set_circuits(gp_batch *batch, const char *const *texts, uint32_t co
     auto work = [&] {
       for (uint32_t i; (i = next.fetch_add(1)) < count;) {
         try {
-          parseInto(texts[i], batch->level, *cs[i]);
+          parseInto(texts[i], batch->level, *cs[i], false, nullptr, nt == 1);
         } catch (const std::invalid_argument &e) {
           errs[i] = e.what();
           kinds[i] = 1;

@@ -1,6 +1,10 @@
 // circuit.cpp -- stim circuit text parser (see circuit.hpp).
 #include "circuit.hpp"
 
+#include <vector>
+
+#include "workers.hpp"
+
 #include <algorithm>
 #include <cctype>
 #include <charconv>
@@ -141,29 +145,28 @@ struct Parser {
 
     const bool recTargets = kinds[0] == OpKind::Detector || kinds[0] == OpKind::Observable;
     const uint32_t t0 = static_cast<uint32_t>(c.targets.size());
-    size_t p = 0;
-    while (p < rest.size()) {
-      while (p < rest.size() && std::isspace(static_cast<unsigned char>(rest[p]))) p++;
-      if (p >= rest.size()) break;
-      size_t e = p;
-      while (e < rest.size() && !std::isspace(static_cast<unsigned char>(rest[e]))) e++;
-      std::string_view tok = rest.substr(p, e - p);
+    const char *p = rest.data(), *const pe = rest.data() + rest.size();
+    auto space = [](char ch) { return ch == ' ' || ch == '\t' || ch == '\r' || ch == '\v' || ch == '\f'; };
+    while (p < pe) {
+      while (p < pe && space(*p)) p++;
+      if (p >= pe) break;
+      const char *e = p;
+      while (e < pe && !space(*e)) e++;
+      // decimal digits [d, de) of the token: all of it, or between "rec[-" and "]"
+      const char *d = p, *de = e;
       if (recTargets) {
-        if (tok.size() < 7 || !(tok[0] == 'r' || tok[0] == 'R') || tok.substr(1, 4) != "ec[-" ||
-            tok.back() != ']')
-          fail("bad record target '" + std::string(tok) + "'");
-        uint32_t v = 0;
-        auto [ptr, ec] = std::from_chars(tok.data() + 5, tok.data() + tok.size() - 1, v);
-        if (ec != std::errc() || ptr != tok.data() + tok.size() - 1 || v == 0 || v > 0x7FFFFFFFu)
-          fail("bad record target '" + std::string(tok) + "'");
-        c.targets.push_back(-static_cast<int32_t>(v));
-      } else {
-        uint32_t v = 0;
-        auto [ptr, ec] = std::from_chars(tok.data(), tok.data() + tok.size(), v);
-        if (ec != std::errc() || ptr != tok.data() + tok.size() || v > 0x7FFFFFFFu)
-          fail("bad target '" + std::string(tok) + "'");
-        c.targets.push_back(static_cast<int32_t>(v));
+        if (e - p < 7 || !(p[0] == 'r' || p[0] == 'R') || std::string_view(p + 1, 4) != "ec[-" || e[-1] != ']')
+          fail("bad record target '" + std::string(p, e) + "'");
+        d = p + 5;
+        de = e - 1;
       }
+      uint64_t v = 0;
+      const bool ok = de > d && de - d <= 10 && std::all_of(d, de, [](char ch) { return ch >= '0' && ch <= '9'; });
+      if (ok)
+        for (const char *q = d; q < de; ++q) v = v * 10 + static_cast<uint64_t>(*q - '0');
+      if (!ok || v > 0x7FFFFFFFu || (recTargets && v == 0))
+        fail(std::string(recTargets ? "bad record target '" : "bad target '") + std::string(p, e) + "'");
+      c.targets.push_back(recTargets ? -static_cast<int32_t>(v) : static_cast<int32_t>(v));
       p = e;
     }
     const uint32_t nt = static_cast<uint32_t>(c.targets.size()) - t0;
@@ -208,10 +211,64 @@ struct Parser {
 
 }  // namespace
 
+// Texts without REPEAT blocks above this size are parsed in line-aligned
+// chunks on the worker pool and joined in order (same ops, targets and error
+// messages as one pass).
+constexpr size_t kParallelParseMinBytes = 65536;
+
 Circuit Circuit::parse(std::string_view text) {
   Circuit c;
-  Parser p{text};
-  p.block(c, 0);
+  const bool chunked = text.size() >= kParallelParseMinBytes && text.find('{') == std::string_view::npos &&
+                       parallelWidth() > 1;
+  if (!chunked) {
+    Parser p{text};
+    c.targets.reserve(text.size() / 3);
+    p.block(c, 0);
+  } else {
+    const uint32_t parts = std::min<uint32_t>(4 * parallelWidth(), static_cast<uint32_t>(text.size() / 16384));
+    std::vector<size_t> cut(parts + 1, text.size());
+    cut[0] = 0;
+    for (uint32_t k = 1; k < parts; ++k) {
+      size_t at = std::max(cut[k - 1], text.size() * k / parts);
+      const size_t nl = text.find('\n', at);
+      cut[k] = nl == std::string_view::npos ? text.size() : nl + 1;
+    }
+    // each chunk's first line number
+    std::vector<uint32_t> lines(parts + 1, 0);
+    parallelFor(parts, [&](uint32_t k) {
+      lines[k + 1] = static_cast<uint32_t>(std::count(text.begin() + cut[k], text.begin() + cut[k + 1], '\n'));
+    });
+    for (uint32_t k = 0; k < parts; ++k) lines[k + 1] += lines[k];
+    std::vector<Circuit> pieces(parts);
+    parallelFor(parts, [&](uint32_t k) {
+      Parser p{text.substr(cut[k], cut[k + 1] - cut[k])};
+      p.line = lines[k];
+      pieces[k].targets.reserve((cut[k + 1] - cut[k]) / 3);
+      p.block(pieces[k], 0);
+    });
+    size_t nOps = 0, nTargets = 0;
+    for (const Circuit &pc : pieces) {
+      nOps += pc.ops.size();
+      nTargets += pc.targets.size();
+    }
+    c.ops.reserve(nOps);
+    c.targets.resize(nTargets);
+    std::vector<size_t> tOff(parts + 1, 0), oOff(parts + 1, 0);
+    for (uint32_t k = 0; k < parts; ++k) {
+      tOff[k + 1] = tOff[k] + pieces[k].targets.size();
+      oOff[k + 1] = oOff[k] + pieces[k].ops.size();
+    }
+    if (nTargets >= (1ull << 32)) throw InvalidArgument("circuit too large: more than 2^32 targets");
+    c.ops.resize(nOps);
+    parallelFor(parts, [&](uint32_t k) {
+      std::copy(pieces[k].targets.begin(), pieces[k].targets.end(), c.targets.begin() + tOff[k]);
+      for (size_t i = 0; i < pieces[k].ops.size(); ++i) {
+        Op o = pieces[k].ops[i];
+        o.t0 += static_cast<uint32_t>(tOff[k]);
+        c.ops[oOff[k] + i] = o;
+      }
+    });
+  }
   // Stats, as stim's compute_stats feeds CircuitParameters
   // (Circuit.hpp:68-79).
   int64_t maxQubit = -1;

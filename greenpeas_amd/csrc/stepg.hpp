@@ -102,9 +102,11 @@ struct Stepg {
 
 // Builds the STEPG of `c` at `level`.  `theta` (if non-null) overrides the
 // rate of every fault location.  Throws RuntimeError on flow-rule violations
-// and invalid rates (the reference's std::runtime_error cases).
+// and invalid rates (the reference's std::runtime_error cases).  `parallel`
+// lets a large circuit's layer ranges be built on the worker pool (same
+// result).
 void buildStepg(const Circuit &c, int level, Stepg &g, const double *theta = nullptr,
-                bool recordComponents = false);
+                bool recordComponents = false, bool parallel = false);
 
 // Device-ready arrays of one circuit (see the file comment).
 struct StepgImage {
@@ -130,7 +132,8 @@ struct StepgImage {
   std::vector<uint16_t> probIdx;   // numRanks indices into probTab
 };
 
-void buildImage(const Stepg &g, StepgImage &img);
+// `parallel` lets a large image be built on the worker pool (same image).
+void buildImage(const Stepg &g, StepgImage &img, bool parallel = false);
 
 // Tuning knob: the fewest sweep records per thread an image is built with
 // (1, 2, 4 or 8; more when a layer does not fit 1024 threads).  Fewer waves

@@ -159,6 +159,13 @@ int gp_debug_sweep_plan(const gp_driver *driver, uint32_t plan[3]);
 int gp_build_stepg_host(const char *circuit_text, int correlation_level,
                         uint32_t dims[4], uint64_t *adjacency, double *probs);
 
+/* Host STEPG and device image of `circuit_text` built on the calling thread
+ * alone (parallel = 0) or on the library's worker pool (parallel = 1),
+ * reduced to a digest: digest[0] = FNV-1a-64 over all of their arrays,
+ * digest[1] = bytes hashed.  Both builds must agree (a parity check of the
+ * pooled build; no device is touched). */
+int gp_debug_image_digest(const char *circuit_text, int correlation_level, int parallel, uint64_t digest[2]);
+
 /* ------------------------------------------------------------------------
  * Batched compile: many circuits in one set of launches (no counterpart in
  * the reference, which compiles one circuit per Driver::compile call).

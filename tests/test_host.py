@@ -133,3 +133,66 @@ def test_tuning_knobs_roundtrip():
     finally:
         autotune.set_tuning("bucket_fill", old_fill)
         autotune.set_tuning("sweep_rpt", old_rpt)
+
+
+def _digest(text, level, parallel):
+    import numpy as np
+
+    from greenpeas_amd import _native
+
+    d = np.zeros(2, np.uint64)
+    _native.check(_native.load().gp_debug_image_digest(text.encode(), level, parallel, d.ctypes.data))
+    return tuple(int(x) for x in d)
+
+
+@pytest.mark.parametrize("name", ["surface3", "bb6", "bb12", "iceberg_surface_6_static_circl"])
+@pytest.mark.parametrize("level", [0, 1, 2])
+def test_pooled_host_build_equals_single_thread(name, level):
+    """The parse, STEPG and device image built on the worker pool (layer
+    ranges, pair blocks and rank ranges in parallel, joined in order) are
+    byte for byte the single-thread build."""
+    text = read_circuit(name)
+    assert _digest(text, level, 1) == _digest(text, level, 0)
+
+
+def test_pooled_host_build_generated_circuits():
+    """Larger generated circuits: noisy surface d=11 x 40 rounds with a
+    rate per instruction, a reset-branch variant, BB [[144,12,12]] x 18."""
+    import numpy as np
+
+    from greenpeas_amd.batch import variant_circuits
+    from greenpeas_amd.codes import BBCode, Noise, SurfaceCode
+
+    texts = [SurfaceCode(11).memory_text(40, noise=Noise(rng=np.random.default_rng(8))),
+             variant_circuits(1, seed=3, distances=(9,))[0], BBCode(12).memory_text(18, 1e-3)]
+    # noise only at the start: the active nodes spread from two error nodes
+    # through every layer range of the pooled pass (20 qubits x 600 CX layers)
+    rng = np.random.default_rng(5)
+    lines = ["R " + " ".join(map(str, range(20))), "TICK", "X_ERROR(0.01) 0", "DEPOLARIZE1(0.001) 3"]
+    for layer in range(600):
+        q = rng.permutation(20)
+        lines += ["TICK", "CX " + " ".join(map(str, q[:8]))]
+        if layer % 50 == 0:
+            lines.append("H " + " ".join(map(str, q[8:11])))
+    lines += ["TICK", "M " + " ".join(map(str, range(20)))]
+    lines += [f"DETECTOR rec[-{20 - q}]" for q in range(20)] + ["OBSERVABLE_INCLUDE(0) rec[-1] rec[-2]"]
+    texts.append("\n".join(lines) + "\n")
+    for t in texts:
+        assert _digest(t, 2, 1) == _digest(t, 2, 0)
+
+
+def test_pooled_build_errors_match():
+    """A flow-rule violation deep in a long circuit raises the same error
+    from the pooled build as from one pass, and the next build (reusing the
+    per-thread scratch) is still exact."""
+    text = ("R 0 1 2\n" + "TICK\nDEPOLARIZE1(0.001) 0 1 2\nCX 0 1\n" * 3000
+            + "TICK\nDEPOLARIZE2(0.01) 0 1\nDEPOLARIZE2(0.01) 1 2\n"
+            + "TICK\nDEPOLARIZE1(0.001) 0 1 2\n" * 3000 + "TICK\nM 0 1 2\nDETECTOR rec[-1]\n")
+    msgs = []
+    for par in (0, 1):
+        with pytest.raises(RuntimeError) as e:
+            _digest(text, 2, par)
+        msgs.append(str(e.value))
+    assert msgs[0] == msgs[1]
+    good = read_circuit("bb6")
+    assert _digest(good, 2, 1) == _digest(good, 2, 0)
