@@ -22,7 +22,8 @@ NOALLOC_MAX_COMPONENTS = 1 << 20
 # every entry point declared in include/greenpeas_amd.h
 EXPORTS = (
     "gp_last_error", "gp_version",
-    "gp_get_driver", "gp_driver_destroy", "gp_compile_detector_error_model", "gp_driver_parse",
+    "gp_get_driver", "gp_get_driver_on_stream", "gp_driver_set_stream", "gp_driver_load_stepg",
+    "gp_driver_destroy", "gp_compile_detector_error_model", "gp_driver_parse",
     "gp_dem_shape", "gp_dem_copy", "gp_dem_acquire", "gp_dem_release", "gp_dem_text", "gp_driver_params",
     "gp_driver_timings",
     "gp_driver_bench_device", "gp_debug_pnodes", "gp_debug_sensitivities", "gp_debug_stepg", "gp_debug_sweep_plan",
@@ -56,6 +57,23 @@ class DemView(ctypes.Structure):
     ]
 
 
+class Stepg(ctypes.Structure):
+    """gp_stepg: a caller's STEPG in the reference's layout."""
+
+    _fields_ = [
+        ("correlation_level", ctypes.c_int),
+        ("num_qubits", ctypes.c_uint32),
+        ("num_layers", ctypes.c_uint32),
+        ("num_measurements", ctypes.c_uint32),
+        ("num_detectors", ctypes.c_uint32),
+        ("num_observables", ctypes.c_uint32),
+        ("adjacency", ctypes.c_void_p),
+        ("probs", ctypes.c_void_p),
+        ("keys", ctypes.c_void_p),
+        ("sensitivity", ctypes.c_void_p),
+    ]
+
+
 def load() -> ctypes.CDLL:
     """Loads the native library (once) and declares its signatures."""
     global _lib
@@ -81,6 +99,9 @@ def load() -> ctypes.CDLL:
     sig("gp_last_error", [], cs)
     sig("gp_version", [], cs)
     sig("gp_get_driver", [cs, i32, i32, PP])
+    sig("gp_get_driver_on_stream", [cs, i32, i32, P, PP])
+    sig("gp_driver_set_stream", [P, P])
+    sig("gp_driver_load_stepg", [P, ctypes.POINTER(Stepg)])
     sig("gp_driver_destroy", [P])
     sig("gp_compile_detector_error_model", [P, cs])
     sig("gp_driver_parse", [P, cs])

@@ -46,7 +46,45 @@ def build(verbose: bool = False) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_pymodule(verbose)
+    build_facade_check(verbose)
     return LIB
+
+
+def build_pymodule(verbose: bool = False) -> str:
+    """The pybind11 module greenpeas_amd._gppy (host code over the C++
+    facade, linked to libgreenpeas_amd.so next to it)."""
+    import sysconfig
+
+    import pybind11
+
+    out = os.path.join(HERE, "_gppy" + sysconfig.get_config_var("EXT_SUFFIX"))
+    tmp = out + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-fvisibility=hidden",
+           "-I", pybind11.get_include(), "-I", sysconfig.get_paths()["include"],
+           os.path.join(CSRC, "gppy.cpp"), "-o", tmp, "-L", HERE, "-l:libgreenpeas_amd.so",
+           "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, out)
+    return out
+
+
+def build_facade_check(verbose: bool = False) -> str:
+    """tests/cpp/facade_check: a C++ program over include/greenpeas_amd.hpp
+    that the tests run (it compiles a circuit file and prints the DEM)."""
+    src = os.path.join(HERE, "..", "tests", "cpp", "facade_check.cpp")
+    out = os.path.join(HERE, "..", "tests", "cpp", "facade_check")
+    if not os.path.exists(src):
+        return ""
+    cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(HERE, "..", "include"), src, "-o", out + ".tmp",
+           "-L", HERE, "-l:libgreenpeas_amd.so", "-Wl,-rpath,$ORIGIN/../../greenpeas_amd"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(out + ".tmp", out)
+    return out
 
 
 if __name__ == "__main__":

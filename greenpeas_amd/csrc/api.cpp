@@ -136,7 +136,8 @@ const char *gp_last_error(void) { return g_lastError.c_str(); }
 
 const char *gp_version(void) { return "greenpeas_amd 0.1.0 (gfx950)"; }
 
-int gp_get_driver(const char *circuit_text, int correlation_level, int device, gp_driver **out) {
+int gp_get_driver_on_stream(const char *circuit_text, int correlation_level, int device, void *stream,
+                            gp_driver **out) {
   return guarded([&] {
     if (!out) throw InvalidArgument("out is NULL");
     requireText(circuit_text);
@@ -146,6 +147,7 @@ int gp_get_driver(const char *circuit_text, int correlation_level, int device, g
     d->device = device;
     parseInto(circuit_text, correlation_level, d->parsed);
     d->pipe = std::make_unique<gpa::Pipeline>(device, false);
+    if (stream) d->pipe->setStream(stream);
     // the DEM is the reference's (no hashes; they are read on request),
     // with 16-bit targets where they fit
     d->pipe->setDownloadHashes(false);
@@ -154,6 +156,88 @@ int gp_get_driver(const char *circuit_text, int correlation_level, int device, g
     d->haveCircuit = true;
     d->loaded = true;
     *out = d.release();
+  });
+}
+
+int gp_get_driver(const char *circuit_text, int correlation_level, int device, gp_driver **out) {
+  return gp_get_driver_on_stream(circuit_text, correlation_level, device, nullptr, out);
+}
+
+int gp_driver_set_stream(gp_driver *driver, void *stream) {
+  return guarded([&] {
+    if (!driver) throw InvalidArgument("driver is NULL");
+    driver->pipe->setStream(stream);
+  });
+}
+
+int gp_driver_load_stepg(gp_driver *driver, const gp_stepg *st) {
+  return guarded([&] {
+    if (!driver || !st) throw InvalidArgument("NULL argument");
+    checkLevel(st->correlation_level);
+    if (st->correlation_level != driver->level)
+      throw InvalidArgument("the STEPG's correlation level differs from the driver's");
+    gpa::Stepg &g = driver->parsed.stepg;
+    g.level = st->correlation_level;
+    g.k = gpa::nodesPerQubit(g.level);
+    g.numLayers = st->num_layers;
+    g.numQubits = st->num_qubits;
+    const uint64_t npl = static_cast<uint64_t>(g.numQubits) * g.k, nn = npl * g.numLayers;
+    if (nn >= (1ull << 31) || npl >= (1u << 30)) throw RuntimeError("STEPG too large: more than 2^31 nodes");
+    g.npl = static_cast<uint32_t>(npl);
+    g.numNodes = static_cast<uint32_t>(nn);
+    g.numMeasurements = st->num_measurements;
+    g.numDetectors = st->num_detectors;
+    g.numObservables = st->num_observables;
+    g.numLocations = 0;  // not part of a STEPG
+    g.numCols = static_cast<uint32_t>((static_cast<uint64_t>(g.numDetectors) + g.numObservables + 31) / 32);
+    const uint32_t N = g.numNodes, M = g.numMeasurements;
+    const uint32_t W = (g.numDetectors + g.numObservables + 63) / 64;
+    if (N && (!st->adjacency || !st->probs)) throw InvalidArgument("STEPG arrays are NULL");
+    if (M && (!st->keys || (W && !st->sensitivity))) throw InvalidArgument("STEPG sensitivity map is NULL");
+    // flows run forward by one or two layers (Mixer.hpp rules), keys name
+    // nodes, probabilities lie in [0, 1]
+    for (uint32_t n = 0; n < N; ++n) {
+      const uint64_t a = st->adjacency[n];
+      const uint32_t l = n / g.npl;
+      for (uint32_t c : {static_cast<uint32_t>(a), static_cast<uint32_t>(a >> 32)}) {
+        if (c == gpa::kNone) continue;
+        const uint32_t cl = c < N ? c / g.npl : 0u;
+        if (c >= N || cl <= l || cl > l + 2)
+          throw InvalidArgument("STEPG node " + std::to_string(n) + " flows to " + std::to_string(c) +
+                                ", not one or two layers later");
+      }
+      if (!(st->probs[n] >= 0.0 && st->probs[n] <= 1.0))
+        throw InvalidArgument("STEPG node " + std::to_string(n) + " has probability outside [0, 1]");
+    }
+    g.adjacency.assign(st->adjacency, st->adjacency + N);
+    g.probs.assign(st->probs, st->probs + N);
+    g.keys.assign(st->keys, st->keys + M);
+    for (uint32_t m = 0; m < M; ++m)
+      if (g.keys[m] >= N) throw InvalidArgument("STEPG key of measurement " + std::to_string(m) + " names no node");
+    // 64-bit words, column-major -> 32-bit columns per measurement
+    g.measBits.assign(static_cast<size_t>(M) * g.numCols, 0u);
+    for (uint32_t c = 0; c < g.numCols; ++c)
+      for (uint32_t m = 0; m < M; ++m) {
+        const uint64_t w = st->sensitivity[static_cast<size_t>(c / 2) * M + m];
+        g.measBits[static_cast<size_t>(m) * g.numCols + c] = static_cast<uint32_t>(c % 2 ? w >> 32 : w);
+      }
+    // bits past the last observable are not targets
+    const uint32_t T = g.numDetectors + g.numObservables;
+    if (T % 32)
+      for (uint32_t m = 0; m < M; ++m) g.measBits[static_cast<size_t>(m) * g.numCols + g.numCols - 1] &= (1u << (T % 32)) - 1;
+    g.compNode.clear();
+    g.compLoc.clear();
+    g.compKind.clear();
+    gpa::buildImage(g, driver->parsed.image, true);
+    gpa::Circuit &c = driver->parsed.circuit;
+    c = gpa::Circuit{};
+    c.numQubits = g.numQubits;
+    c.numTicks = g.numLayers ? g.numLayers - 1 : 0;
+    c.numMeasurements = M;
+    c.numDetectors = g.numDetectors;
+    c.numObservables = g.numObservables;
+    driver->haveCircuit = true;
+    driver->loaded = false;
   });
 }
 

@@ -2059,7 +2059,9 @@ size_t align256(size_t x) { return (x + 255) & ~static_cast<size_t>(255); }
 
 struct Pipeline::Impl {
   int device;
-  hipStream_t stream = nullptr;
+  // the stream the pipeline's work is queued on: its own, or a caller's
+  // (setStream)
+  hipStream_t stream = nullptr, ownStream = nullptr;
   hipEvent_t ev[8] = {};  // [0, 5): stage marks of a timed run; 6, 7: benchDevice
   // upload: the sweep's inputs on `stream`, the error nodes' inputs on
   // `side` (overlapping the sweep; classify waits for evSide)
@@ -2178,7 +2180,8 @@ struct Pipeline::Impl {
 Pipeline::Pipeline(int device, bool growable) : device_(device), growable_(growable), impl_(new Impl) {
   impl_->device = device;
   GPA_HIP(hipSetDevice(device));
-  GPA_HIP(hipStreamCreateWithFlags(&impl_->stream, hipStreamNonBlocking));
+  GPA_HIP(hipStreamCreateWithFlags(&impl_->ownStream, hipStreamNonBlocking));
+  impl_->stream = impl_->ownStream;
   GPA_HIP(hipStreamCreateWithFlags(&impl_->side, hipStreamNonBlocking));
   for (auto &e : impl_->ev) GPA_HIP(hipEventCreate(&e));
   GPA_HIP(hipEventCreate(&impl_->evUp0));
@@ -2198,11 +2201,21 @@ Pipeline::~Pipeline() {
   if (impl_->side) (void)hipStreamDestroy(impl_->side);
   for (auto &e : impl_->evPool) (void)hipEventDestroy(e);
   impl_->dropGraph();
-  if (impl_->stream) (void)hipStreamDestroy(impl_->stream);
+  if (impl_->ownStream) (void)hipStreamDestroy(impl_->ownStream);
   delete impl_;
 }
 
 void *Pipeline::stream() const { return impl_->stream; }
+
+void Pipeline::setStream(void *stream) {
+  Impl &m = *impl_;
+  GPA_HIP(hipSetDevice(device_));
+  m.joinSide();
+  // work queued so far completes first; later work goes to the new stream
+  GPA_HIP(hipStreamSynchronize(m.stream));
+  GPA_HIP(hipStreamSynchronize(m.side));
+  m.stream = stream ? static_cast<hipStream_t>(stream) : m.ownStream;
+}
 
 static uint32_t bucketBitsFor(uint32_t numRanks) {
   uint32_t b = 0;

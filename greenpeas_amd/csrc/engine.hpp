@@ -102,6 +102,11 @@ class Pipeline {
   uint64_t deviceBytes() const;
   int device() const { return device_; }
   void *stream() const;
+  // Queues later work on `stream` (a hipStream_t of the pipeline's device;
+  // nullptr: the pipeline's own stream again), after the work queued so far
+  // has completed.  Uploads still overlap the sweep on a second stream of
+  // the pipeline's own.
+  void setStream(void *stream);
 
   // Parity/debug access (after download or run + sync).
   void debugSensitivities(uint32_t v, std::vector<uint32_t> &cols32) const;  // ranks x cols

@@ -54,6 +54,48 @@ const char *gp_version(void);
 int gp_get_driver(const char *circuit_text, int correlation_level, int device,
                   gp_driver **out);
 
+/* get_driver with the driver's work queued on a caller's stream (a
+ * hipStream_t of `device`; NULL: a stream of the driver's own), as
+ * gp_get_driver otherwise.  No counterpart in the reference, which works on
+ * the legacy default stream (CUDA.hpp:161-163). */
+int gp_get_driver_on_stream(const char *circuit_text, int correlation_level, int device,
+                            void *stream, gp_driver **out);
+
+/* Queues the driver's later work on `stream` (NULL: its own stream again),
+ * after the work queued so far has completed.  A driver is used by one
+ * thread at a time. */
+int gp_driver_set_stream(gp_driver *driver, void *stream);
+
+/* A space-time error-propagation graph built by the caller, in the
+ * reference's layout (Circuit::stepg and Circuit::sensitivityMap,
+ * Circuit.hpp:200-231; STEPG.hpp:40-132): node = layer * num_qubits * k +
+ * qubit * k + slot with k = 3, 5, 8 nodes per qubit at L0, L1, L2
+ * (Mixer.hpp:59-84). */
+typedef struct gp_stepg {
+  int correlation_level;
+  uint32_t num_qubits, num_layers, num_measurements, num_detectors, num_observables;
+  /* per node: child0 | child1 << 32, 0xFFFFFFFF = none; children lie one or
+   * two layers later */
+  const uint64_t *adjacency;
+  const double *probs;           /* per node, in [0, 1] */
+  const uint32_t *keys;          /* per measurement: its node */
+  /* per measurement, ceil((num_detectors + num_observables) / 64) 64-bit
+   * words, column-major (word w of measurement m at
+   * sensitivity[w * num_measurements + m]): bit t = detector t, or
+   * observable t - num_detectors */
+  const uint64_t *sensitivity;
+} gp_stepg;
+
+/* Loads a caller's STEPG into the driver in place of parsing circuit text
+ * (the reference's Driver state before compileDetectorErrorModel(),
+ * Driver.hpp:208-211: the span it times, Cli.hpp:293-302).  The next
+ * gp_compile_detector_error_model(driver, NULL) compiles it; capacity rules
+ * are a parsed circuit's.  GP_INVALID_ARGUMENT for a level other than the
+ * driver's, flows that do not run one or two layers forward, keys that name
+ * no node or probabilities outside [0, 1]; GP_RUNTIME_ERROR where the graph
+ * breaks the structure the sweep relies on (as for a parsed circuit). */
+int gp_driver_load_stepg(gp_driver *driver, const gp_stepg *stepg);
+
 /* Releases the driver and all of its host and device memory. */
 int gp_driver_destroy(gp_driver *driver);
 
