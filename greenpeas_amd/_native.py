@@ -30,7 +30,8 @@ EXPORTS = (
     "gp_build_stepg_host", "gp_debug_image_digest",
     "gp_batch_create", "gp_batch_destroy", "gp_batch_set_circuits", "gp_batch_compile",
     "gp_batch_stage_us", "gp_batch_device_us", "gp_batch_dem_shape", "gp_batch_dem_copy", "gp_batch_device_bytes",
-    "gp_estimate_device_bytes", "gp_circuit_summary", "gp_set_tuning", "gp_get_tuning",
+    "gp_estimate_device_bytes", "gp_prepare_circuit", "gp_prepared_destroy", "gp_prepared_device_bytes",
+    "gp_batch_set_prepared", "gp_batch_bytes_after", "gp_device_memory", "gp_circuit_summary", "gp_set_tuning", "gp_get_tuning",
     "gp_grad_plan_create", "gp_grad_plan_destroy", "gp_grad_plan_sizes", "gp_grad_plan_dem_structure",
     "gp_grad_plan_device_arrays", "gp_grad_plan_copy_arrays", "gp_xor_merge_forward",
     "gp_xor_merge_backward", "gp_node_rates_forward", "gp_node_rates_backward",
@@ -129,6 +130,12 @@ def load() -> ctypes.CDLL:
     sig("gp_batch_dem_copy", [P, u32, P, P, P, P])
     sig("gp_batch_device_bytes", [P, pu64])
     sig("gp_estimate_device_bytes", [cs, i32, pu64])
+    sig("gp_prepare_circuit", [cs, i32, PP])
+    sig("gp_prepared_destroy", [P])
+    sig("gp_prepared_device_bytes", [P, pu64])
+    sig("gp_batch_set_prepared", [P, P, u32])
+    sig("gp_batch_bytes_after", [P, P, u32, pu64])
+    sig("gp_device_memory", [i32, pu64, pu64])
     sig("gp_circuit_summary", [cs, P])
     sig("gp_set_tuning", [cs, u32])
     sig("gp_get_tuning", [cs, pu32])

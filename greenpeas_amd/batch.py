@@ -21,6 +21,55 @@ from .dem import DetectorErrorModel
 from .error_analysis import CorrelationLevel, _read_dem, circuit_text
 
 
+class PreparedCircuit:
+    """A circuit parsed and built up to its device image on the host (the
+    host half of a compile: gp_prepare_circuit), ready to be loaded into a
+    batch without parsing again."""
+
+    def __init__(self, circuit, correlation_level: CorrelationLevel = CorrelationLevel.L2):
+        self._lib = _native.load()
+        self._h = ctypes.c_void_p()
+        self.level = int(correlation_level)
+        _native.check(self._lib.gp_prepare_circuit(circuit_text(circuit).encode(), self.level,
+                                                   ctypes.byref(self._h)))
+
+    def device_bytes(self) -> int:
+        """Device bytes a new batch context allocates to compile this circuit
+        alone (gp_prepared_device_bytes)."""
+        b = ctypes.c_uint64()
+        _native.check(self._lib.gp_prepared_device_bytes(self._h, ctypes.byref(b)))
+        return int(b.value)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            self._lib.gp_prepared_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001
+            pass
+
+
+def prepare(circuit, correlation_level: CorrelationLevel = CorrelationLevel.L2) -> PreparedCircuit:
+    return PreparedCircuit(circuit, correlation_level)
+
+
+def _handles(prepared) -> ctypes.Array:
+    hs = [p._h.value for p in prepared]
+    if not all(hs):
+        raise ValueError("a prepared circuit was closed")
+    return (ctypes.c_void_p * len(hs))(*hs)
+
+
+def device_memory(device: int = 0) -> tuple[int, int]:
+    """(free, total) bytes of device memory (hipMemGetInfo)."""
+    f, t = ctypes.c_uint64(), ctypes.c_uint64()
+    _native.check(_native.load().gp_device_memory(device, ctypes.byref(f), ctypes.byref(t)))
+    return int(f.value), int(t.value)
+
+
 class BatchCompiler:
     """Compiles a list of circuits on one GPU; device buffers grow on demand
     and are reused across batches."""
@@ -48,6 +97,22 @@ class BatchCompiler:
         arr = (ctypes.c_char_p * len(texts))(*texts)
         _native.check(self._lib.gp_batch_set_circuits(self._h, arr, len(texts), threads))
         self.count = len(texts)
+
+    def set_prepared(self, prepared) -> None:
+        """Makes prepared circuits (PreparedCircuit, at this batch's level)
+        the batch's circuits; nothing is parsed."""
+        prepared = list(prepared)
+        _native.check(self._lib.gp_batch_set_prepared(self._h, _handles(prepared), len(prepared)))
+        self.count = len(prepared)
+
+    def bytes_after(self, prepared) -> int:
+        """Device bytes this compiler holds after compiling `prepared` (its
+        buffers where they fit, grown where not; gp_batch_bytes_after)."""
+        prepared = list(prepared)
+        b = ctypes.c_uint64()
+        _native.check(self._lib.gp_batch_bytes_after(self._h, _handles(prepared), len(prepared),
+                                                     ctypes.byref(b)))
+        return int(b.value)
 
     def compile(self, one_by_one: bool = False) -> list[DetectorErrorModel]:
         """Compiles every circuit of the batch; returns their DEMs in order.

@@ -102,7 +102,7 @@ struct gp_driver {
 struct gp_batch {
   int level = 2;
   int device = 0;
-  std::vector<std::unique_ptr<Parsed>> circuits;
+  std::vector<std::shared_ptr<const gpa::StepgImage>> images;  // the circuits' device images
   std::unique_ptr<gpa::Pipeline> pipe;
   std::unique_ptr<gpa::Pipeline> single;  // one-by-one baseline
   std::vector<gpa::VariantResult> results;
@@ -111,6 +111,13 @@ struct gp_batch {
   std::vector<std::vector<uint64_t>> ownHash;
   double deviceUs = 0;
   bool loaded = false;
+};
+
+// A circuit parsed and built up to its device image (the host half of a
+// compile), shared by the batches it is loaded into.
+struct gp_prepared {
+  int level = 2;
+  std::shared_ptr<const gpa::StepgImage> image;
 };
 
 struct gp_grad_plan {
@@ -510,8 +517,7 @@ int gp_batch_set_circuits(gp_batch *batch, const char *const *texts, uint32_t co
   return guarded([&] {
     if (!batch || (!texts && count)) throw InvalidArgument("NULL argument");
     for (uint32_t i = 0; i < count; ++i) requireText(texts[i]);
-    std::vector<std::unique_ptr<Parsed>> cs(count);
-    for (auto &c : cs) c = std::make_unique<Parsed>();
+    std::vector<std::shared_ptr<const gpa::StepgImage>> imgs(count);
     uint32_t nt = threads ? threads : std::max(1u, std::thread::hardware_concurrency());
     nt = std::min<uint32_t>(nt, std::max<uint32_t>(count, 1));
     std::vector<std::string> errs(count);
@@ -520,7 +526,9 @@ int gp_batch_set_circuits(gp_batch *batch, const char *const *texts, uint32_t co
     auto work = [&] {
       for (uint32_t i; (i = next.fetch_add(1)) < count;) {
         try {
-          parseInto(texts[i], batch->level, *cs[i], false, nullptr, nt == 1);
+          Parsed p;
+          parseInto(texts[i], batch->level, p, false, nullptr, nt == 1);
+          imgs[i] = std::make_shared<const gpa::StepgImage>(std::move(p.image));
         } catch (const std::invalid_argument &e) {
           errs[i] = e.what();
           kinds[i] = 1;
@@ -538,9 +546,87 @@ int gp_batch_set_circuits(gp_batch *batch, const char *const *texts, uint32_t co
       if (kinds[i] == 1) throw InvalidArgument("circuit " + std::to_string(i) + ": " + errs[i]);
       if (kinds[i] == 2) throw RuntimeError("circuit " + std::to_string(i) + ": " + errs[i]);
     }
-    batch->circuits = std::move(cs);
+    batch->images = std::move(imgs);
     batch->results.clear();
     batch->loaded = false;
+  });
+}
+
+int gp_prepare_circuit(const char *circuit_text, int correlation_level, gp_prepared **out) {
+  return guarded([&] {
+    requireText(circuit_text);
+    checkLevel(correlation_level);
+    if (!out) throw InvalidArgument("out is NULL");
+    Parsed p;
+    parseInto(circuit_text, correlation_level, p);
+    auto h = std::make_unique<gp_prepared>();
+    h->level = correlation_level;
+    h->image = std::make_shared<const gpa::StepgImage>(std::move(p.image));
+    *out = h.release();
+  });
+}
+
+int gp_prepared_destroy(gp_prepared *prepared) {
+  return guarded([&] { delete prepared; });
+}
+
+int gp_prepared_device_bytes(const gp_prepared *prepared, uint64_t *bytes) {
+  return guarded([&] {
+    if (!prepared || !bytes) throw InvalidArgument("NULL argument");
+    *bytes = gpa::Pipeline::deviceBytesFor({prepared->image.get()}, true);
+  });
+}
+
+// The images of `count` prepared circuits of the batch's level.
+static std::vector<const gpa::StepgImage *> preparedImages(const gp_batch *batch, const gp_prepared *const *prepared,
+                                                           uint32_t count) {
+  if (!prepared && count) throw InvalidArgument("prepared is NULL");
+  std::vector<const gpa::StepgImage *> imgs(count);
+  for (uint32_t i = 0; i < count; ++i) {
+    if (!prepared[i]) throw InvalidArgument("prepared circuit " + std::to_string(i) + " is NULL");
+    if (prepared[i]->level != batch->level)
+      throw InvalidArgument("prepared circuit " + std::to_string(i) + " was built at correlation level " +
+                            std::to_string(prepared[i]->level) + ", the batch compiles at level " +
+                            std::to_string(batch->level));
+    imgs[i] = prepared[i]->image.get();
+  }
+  return imgs;
+}
+
+int gp_batch_set_prepared(gp_batch *batch, const gp_prepared *const *prepared, uint32_t count) {
+  return guarded([&] {
+    if (!batch) throw InvalidArgument("batch is NULL");
+    preparedImages(batch, prepared, count);
+    std::vector<std::shared_ptr<const gpa::StepgImage>> imgs(count);
+    for (uint32_t i = 0; i < count; ++i) imgs[i] = prepared[i]->image;
+    batch->images = std::move(imgs);
+    batch->results.clear();
+    batch->loaded = false;
+  });
+}
+
+int gp_batch_bytes_after(const gp_batch *batch, const gp_prepared *const *prepared, uint32_t count,
+                         uint64_t *bytes) {
+  return guarded([&] {
+    if (!batch || !bytes) throw InvalidArgument("NULL argument");
+    if (!count) throw InvalidArgument("no circuits");
+    *bytes = batch->pipe->deviceBytesAfter(preparedImages(batch, prepared, count)) +
+             (batch->single ? batch->single->deviceBytes() : 0);
+  });
+}
+
+int gp_device_memory(int device, uint64_t *free_bytes, uint64_t *total_bytes) {
+  return guarded([&] {
+    if (!free_bytes || !total_bytes) throw InvalidArgument("NULL argument");
+    int cur = 0;
+    if (hipGetDevice(&cur) != hipSuccess) throw RuntimeError("no HIP device");
+    if (hipSetDevice(device) != hipSuccess) throw InvalidArgument("invalid device " + std::to_string(device));
+    size_t f = 0, t = 0;
+    const hipError_t e = hipMemGetInfo(&f, &t);
+    (void)hipSetDevice(cur);
+    if (e != hipSuccess) throw RuntimeError(std::string("hipMemGetInfo: ") + hipGetErrorString(e));
+    *free_bytes = f;
+    *total_bytes = t;
   });
 }
 
@@ -583,7 +669,7 @@ static void keepResult(gp_batch *b, uint32_t i, const gpa::VariantResult &r) {
 int gp_batch_compile(gp_batch *batch, int mode) {
   return guarded([&] {
     if (!batch) throw InvalidArgument("batch is NULL");
-    const uint32_t n = static_cast<uint32_t>(batch->circuits.size());
+    const uint32_t n = static_cast<uint32_t>(batch->images.size());
     if (!n) throw RuntimeError("batch has no circuits");
     batch->results.assign(n, gpa::VariantResult{});
     batch->ownP.assign(n, {});
@@ -595,7 +681,7 @@ int gp_batch_compile(gp_batch *batch, int mode) {
         // a new batch (the serving case): one direct run, timed by its
         // stage events -- capturing a graph would cost more than it saves
         std::vector<const gpa::StepgImage *> imgs;
-        for (auto &c : batch->circuits) imgs.push_back(&c->image);
+        for (auto &c : batch->images) imgs.push_back(c.get());
         batch->pipe->load(imgs);
         batch->loaded = true;
         batch->pipe->run(true);
@@ -617,7 +703,7 @@ int gp_batch_compile(gp_batch *batch, int mode) {
       if (!batch->single) batch->single = std::make_unique<gpa::Pipeline>(batch->device, true);
       double total = 0;
       for (uint32_t i = 0; i < n; ++i) {
-        batch->single->load({&batch->circuits[i]->image});
+        batch->single->load({batch->images[i].get()});
         // one untimed compile first, as the batched timing follows earlier
         // runs of its load (a load's first run sizes the target rows)
         batch->single->run(false);
@@ -707,7 +793,7 @@ int gp_estimate_device_bytes(const char *circuit_text, int correlation_level, ui
     if (!bytes) throw InvalidArgument("bytes is NULL");
     Parsed p;
     parseInto(circuit_text, correlation_level, p);
-    *bytes = gpa::imageDeviceBytes(p.image);
+    *bytes = gpa::Pipeline::deviceBytesFor({&p.image}, true);
   });
 }
 

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <memory>
@@ -502,6 +503,7 @@ __device__ __forceinline__ void sweepStep(const SweepState<RPT> &st, uint4 (&r)[
 // [P], then per middle chunk T_k [G][P] (word g*P + i: bit b = bit of
 // output node i for input node 32g + b) and p_k [P].
 constexpr int kSplitMaxChunks = 6;
+constexpr uint32_t kSplitMaxColumns = 4;  // split columns per load
 struct SplitDesc {
   uint32_t v, c, K, stateOff;
   int32_t bound[kSplitMaxChunks + 1];  // first pair of chunk k; bound[K] = top pair + 1
@@ -1958,7 +1960,18 @@ struct WorkArena {
   DevBuf mem;
   std::vector<Req> reqs;
   void request(DevBuf &b, size_t bytes, const char *what) { reqs.push_back(Req{&b, std::max<size_t>(bytes, 256), what}); }
-  void commit(bool frozen) {
+  // Arena bytes for buffers of these capacities (the layout commit makes).
+  static size_t layoutBytes(const std::vector<size_t> &caps) {
+    size_t off = 0;
+    for (size_t c : caps) {
+      const size_t align = c >= (size_t{1} << 20) ? kAllocGrain : 4096;
+      off = (off + align - 1) & ~(align - 1);
+      off += c;
+    }
+    return roundGrain(std::max<size_t>(off, 256));
+  }
+  // headroom: a buffer that must grow takes this much more than asked
+  void commit(bool frozen, double headroom = 1.0) {
     // this load's requests (none stay behind if a capacity check throws)
     std::vector<Req> rq;
     rq.swap(reqs);
@@ -1974,7 +1987,8 @@ struct WorkArena {
       size_t off = 0;
       std::vector<size_t> offs(rq.size()), caps(rq.size());
       for (size_t k = 0; k < rq.size(); ++k) {
-        const size_t c = std::max(rq[k].bytes, rq[k].buf->cap);
+        const size_t grown = rq[k].bytes > rq[k].buf->cap ? static_cast<size_t>(rq[k].bytes * headroom) : 0;
+        const size_t c = std::max({rq[k].bytes, rq[k].buf->cap, grown});
         const size_t align = c >= (size_t{1} << 20) ? kAllocGrain : 4096;
         off = (off + align - 1) & ~(align - 1);
         offs[k] = off;
@@ -2124,6 +2138,13 @@ struct Pipeline::Impl {
       scanTmp, vBounds, members;
   WorkArena work;  // holds the buffers above except in, outTargets, members and the split buffers
   uint64_t capTargets = 0;
+  // the arena's buffers, in the order workBufferSizes() sizes them
+  std::array<DevBuf *, 31> workBufs() {
+    return {&band,      &S,          &info,        &stash,      &ovSlot,      &ovBucket,  &counters, &bucketCnt,
+            &pslot,     &prank,      &runRank,     &runP,       &runHash,     &runTgtPre, &runMember, &runCount,
+            &runNnz,    &itemCount,  &slotBase,    &wgSum,      &classStart,  &tgtStart,  &memberStart, &outP,
+            &outOff,    &outHash,    &outMember,   &scratchKey, &scratchRank, &scanTmp,   &vBounds};
+  }
 
   // host results: the block of the last download, blocks released by
   // their holders for reuse
@@ -2231,6 +2252,61 @@ void Pipeline::reserve(const std::vector<const StepgImage *> &imgs) {
 }
 
 static constexpr size_t kLdsBudget = 150 * 1024;  // of gfx950's 160 KB per workgroup
+
+// Growable pipelines (batches, serving slots) that must grow a buffer take
+// this much more than asked, so a stream of batches of varying size settles
+// after a few loads instead of reallocating on each larger one.
+constexpr double kGrowHeadroom = 1.25;
+
+// bucket counters sit this many words apart (16 bytes while there are few
+// buckets: see Impl::countStride)
+static uint32_t bucketCountStride(uint64_t B) {
+  uint32_t cs = 4;
+  while (cs > 1 && B * cs > (1u << 17)) cs >>= 1;
+  return cs;
+}
+
+// Every work buffer of a load, in the arena's order, by its totals: R error
+// nodes, B sort buckets, V circuits, Ccols detector columns, Swords words of
+// S.  req(index, bytes, what) is called once per buffer.
+constexpr uint32_t kWorkBuffers = 31;
+template <typename F>
+static void workBufferSizes(uint64_t R, uint64_t B, uint64_t V, uint64_t Ccols, uint64_t Swords,
+                            uint32_t countStride, F &&req) {
+  R = std::max<uint64_t>(R, 1);
+  // bucket slots, then an arena of R entries for oversized buckets
+  const uint64_t slots = B * static_cast<uint64_t>(kBucketCap) + R;
+  if (slots >= 0xFFFFFFFFull) throw RuntimeError("too many sort buckets in one load");
+  uint32_t k = 0;
+  req(k++, 8 * std::max<uint64_t>(Ccols, 1), "columns");                // band
+  req(k++, 4 * std::max<uint64_t>(Swords, 1), "sensitivity matrix");    // S
+  req(k++, 16 * R, "nodes");                                             // info
+  req(k++, 4ull * kStashWords * R, "nodes");                             // stash
+  req(k++, 16 * R, "nodes");                                             // ovSlot
+  req(k++, 4 * R, "nodes");                                              // ovBucket
+  req(k++, 4 * kFlagWords, "buckets");                                   // counters
+  // (B + 1) * countStride <= 2^17 + 32 words whatever B is, so a reserved
+  // pipeline fits every later load
+  req(k++, 4 * std::max<uint64_t>((B + 1) * countStride, (1u << 17) + 32), "buckets");  // bucketCnt
+  req(k++, 16 * slots, "nodes");                                         // pslot
+  req(k++, 4 * slots, "nodes");                                          // prank
+  req(k++, 4 * slots, "nodes");                                          // runRank
+  req(k++, 8 * slots, "nodes");                                          // runP
+  req(k++, 8 * slots, "nodes");                                          // runHash
+  req(k++, 4 * slots, "nodes");                                          // runTgtPre
+  req(k++, 4 * slots, "nodes");                                          // runMember
+  for (int q = 0; q < 4; ++q) req(k++, 4 * (B + 1), "buckets");          // runCount, runNnz, itemCount, slotBase
+  req(k++, 4 * 3 * ((B + 3) / 4 + 1), "buckets");                        // wgSum
+  for (int q = 0; q < 3; ++q) req(k++, 4 * (B + 1), "buckets");          // classStart, tgtStart, memberStart
+  req(k++, 8 * R, "nodes");                                              // outP
+  req(k++, 4 * (R + 1), "nodes");                                        // outOff
+  req(k++, 8 * R, "nodes");                                              // outHash
+  req(k++, 4 * (R + 1), "nodes");                                        // outMember
+  req(k++, 16 * R, "nodes");                                             // scratchKey
+  req(k++, 8 * R, "nodes");                                              // scratchRank
+  req(k++, 4 * kScanArrays * ((B + kScanTile - 1) / kScanTile + 2), "buckets");  // scanTmp
+  req(k++, 4 * (2 * V + 8), "circuits");                                 // vBounds
+}
 
 void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   Impl &m = *impl_;
@@ -2389,46 +2465,12 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   // work buffers: one arena (see WorkArena), the targets' own allocation
   const uint64_t R = std::max<uint64_t>(m.R, 1), B = m.B;
   WorkArena &wa = m.work;
-  wa.request(m.band, 8 * std::max<uint32_t>(m.Ccols, 1), "columns");
-  wa.request(m.S, 4 * std::max<uint64_t>(m.Swords, 1), "sensitivity matrix");
-  wa.request(m.info, 16 * R, "nodes");
-  wa.request(m.stash, 4ull * kStashWords * R, "nodes");
-  wa.request(m.ovSlot, 16 * R, "nodes");
-  wa.request(m.ovBucket, 4 * R, "nodes");
-  wa.request(m.counters, 4 * kFlagWords, "buckets");
-  m.countStride = 4;
-  while (m.countStride > 1 && static_cast<uint64_t>(B) * m.countStride > (1u << 17)) m.countStride >>= 1;
-  // (B + 1) * countStride <= 2^17 + 32 words whatever B is, so a reserved
-  // pipeline fits every later load)
-  wa.request(m.bucketCnt, 4 * std::max<uint64_t>((static_cast<uint64_t>(B) + 1) * m.countStride, (1u << 17) + 32),
-             "buckets");
-  // bucket slots, then an arena of numRanks entries for oversized buckets
-  const uint64_t slots = B * static_cast<uint64_t>(kBucketCap) + R;
-  if (slots >= 0xFFFFFFFFull) throw RuntimeError("too many sort buckets in one load");
-  wa.request(m.pslot, 16 * slots, "nodes");
-  wa.request(m.prank, 4 * slots, "nodes");
-  wa.request(m.runRank, 4 * slots, "nodes");
-  wa.request(m.runP, 8 * slots, "nodes");
-  wa.request(m.runHash, 8 * slots, "nodes");
-  wa.request(m.runTgtPre, 4 * slots, "nodes");
-  wa.request(m.runMember, 4 * slots, "nodes");
-  wa.request(m.runCount, 4 * (B + 1), "buckets");
-  wa.request(m.runNnz, 4 * (B + 1), "buckets");
-  wa.request(m.itemCount, 4 * (B + 1), "buckets");
-  wa.request(m.slotBase, 4 * (B + 1), "buckets");
-  wa.request(m.wgSum, 4 * 3 * ((B + 3) / 4 + 1), "buckets");
-  wa.request(m.classStart, 4 * (B + 1), "buckets");
-  wa.request(m.tgtStart, 4 * (B + 1), "buckets");
-  wa.request(m.memberStart, 4 * (B + 1), "buckets");
-  wa.request(m.outP, 8 * R, "nodes");
-  wa.request(m.outOff, 4 * (R + 1), "nodes");
-  wa.request(m.outHash, 8 * R, "nodes");
-  wa.request(m.outMember, 4 * (R + 1), "nodes");
-  wa.request(m.scratchKey, 16 * R, "nodes");
-  wa.request(m.scratchRank, 8 * R, "nodes");
-  wa.request(m.scanTmp, 4 * kScanArrays * ((B + kScanTile - 1) / kScanTile + 2), "buckets");
-  wa.request(m.vBounds, 4 * (2 * V + 8), "circuits");
-  wa.commit(frozen);
+  m.countStride = bucketCountStride(B);
+  const auto bufs = m.workBufs();
+  static_assert(std::tuple_size<decltype(bufs)>::value == kWorkBuffers, "one DevBuf per work buffer");
+  workBufferSizes(R, B, V, m.Ccols, m.Swords, m.countStride,
+                  [&](uint32_t k, uint64_t bytes, const char *what) { wa.request(*bufs[k], bytes, what); });
+  wa.commit(frozen, growable_ ? kGrowHeadroom : 1.0);
   if (m.capTargets == 0 || (!frozen && m.capTargets < 16 * R)) {
     m.outTargets.ensure(4 * 16 * R, frozen, "targets");
     m.capTargets = m.outTargets.cap / 4;
@@ -2452,6 +2494,100 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   // bucket counters start at zero; every run's bucket stage leaves them zeroed
   GPA_HIP(hipMemsetAsync(m.bucketCnt.p, 0, m.bucketCnt.cap, m.stream));
   upload();
+}
+
+namespace {
+
+// The totals a load forms from its circuits (see load()).
+struct LoadTotals {
+  uint64_t R = 0, B = 0, V = 0, Ccols = 0, Swords = 0, inBytes = 0;
+  uint32_t maxP = 64;  // widest layer stride
+};
+
+LoadTotals loadTotals(const std::vector<const StepgImage *> &imgs) {
+  LoadTotals t;
+  uint64_t actOff = 0, seedOff = 0, spill = 0, probOff = 0, nTiles = 0;
+  for (const StepgImage *g : imgs) {
+    t.B += uint64_t{1} << bucketBitsFor(g->numRanks);
+    nTiles += (g->numRanks + kClassBlock - 1) / kClassBlock;
+    actOff += static_cast<uint64_t>(kSweepLeadPad + g->numPairs) * (g->layerStride + 1);
+    seedOff += static_cast<uint64_t>(g->numCols) * g->numSeeds;
+    t.Ccols += g->numCols;
+    t.R += g->numRanks;
+    spill += g->rankSpill.size() / 4;
+    probOff += g->probTab.empty() ? g->numRanks : g->probTab.size();
+    t.Swords += 2ull * (g->numPairs + kSweepLeadPad) * g->layerStride * g->numCols;
+    t.maxP = std::max(t.maxP, g->layerStride);
+  }
+  t.V = imgs.size();
+  // the input arena: each array at a 256-byte boundary
+  t.inBytes = align256(sizeof(VDesc) * t.V) + align256(16 * t.Ccols) + align256(16 * nTiles) + align256(16 * actOff) +
+              align256(4 * seedOff) + align256(8 * t.R) + align256(16 * spill) + align256(2 * t.R) +
+              align256(8 * probOff);
+  return t;
+}
+
+// Bytes of a buffer after DevBuf::ensure(bytes) on one of capacity `cap`.
+uint64_t afterEnsure(size_t cap, uint64_t bytes) { return bytes <= cap ? cap : roundGrain(std::max<uint64_t>(bytes, 256)); }
+
+// Capacities a pipeline holds (all zero: a new pipeline).
+struct HeldCaps {
+  std::array<size_t, kWorkBuffers> work{};
+  bool arena = false;  // the arena is allocated
+  size_t arenaCap = 0, in = 0, targets = 0, members = 0, splitBuf = 0, splitState = 0;
+  uint64_t capTargets = 0;
+};
+
+// Device bytes a growable pipeline holding `h` holds after loading a load
+// with totals `t` and compiling it: the work arena as WorkArena::commit
+// lays it out, the input arena and targets as load() sizes them, members as
+// held, and the time-chunked sweep's buffers at their largest (at most
+// kSplitMaxColumns columns of kSplitMaxChunks chunks of the widest circuit).
+// Every buffer is freed before its replacement is allocated, so this also
+// bounds the peak during the load and its compiles -- unless the DEM has
+// more than 16 targets per error node, when the target buffer grows.
+uint64_t footprint(const LoadTotals &t, const HeldCaps &h, double headroom) {
+  const uint64_t R = std::max<uint64_t>(t.R, 1);
+  std::vector<size_t> caps;
+  bool fits = h.arena;
+  workBufferSizes(R, t.B, t.V, t.Ccols, t.Swords, bucketCountStride(t.B), [&](uint32_t k, uint64_t bytes, const char *) {
+    const size_t b = std::max<size_t>(bytes, 256), cap = h.work[k];
+    const size_t grown = b > cap ? static_cast<size_t>(b * headroom) : 0;
+    if (b > cap) fits = false;
+    caps.push_back(std::max({b, cap, grown}));
+  });
+  uint64_t total = fits ? h.arenaCap : WorkArena::layoutBytes(caps);
+  total += afterEnsure(h.in, t.inBytes);
+  total += h.capTargets == 0 || h.capTargets < 16 * R ? afterEnsure(h.targets, 4 * 16 * R) : h.targets;
+  total += h.members;
+  const uint64_t G = (t.maxP + 31) / 32;
+  const uint64_t splitItems = 4096 + 16ull * kSplitMaxColumns * (kSplitMaxChunks * (G + 1) + kSplitMaxChunks);
+  total += afterEnsure(h.splitBuf, splitItems);
+  total += afterEnsure(h.splitState, 4ull * kSplitMaxColumns * splitStateWords(t.maxP, kSplitMaxChunks));
+  return total;
+}
+
+}  // namespace
+
+uint64_t Pipeline::deviceBytesFor(const std::vector<const StepgImage *> &imgs, bool growable) {
+  return footprint(loadTotals(imgs), HeldCaps{}, growable ? kGrowHeadroom : 1.0);
+}
+
+uint64_t Pipeline::deviceBytesAfter(const std::vector<const StepgImage *> &imgs) const {
+  Impl &m = *impl_;
+  if (!growable_) return deviceBytes();  // a frozen pipeline never grows (an oversized load throws)
+  HeldCaps h;
+  const auto bufs = m.workBufs();
+  for (uint32_t k = 0; k < kWorkBuffers; ++k) h.work[k] = bufs[k]->cap;
+  h.arena = m.work.mem.p != nullptr;
+  h.arenaCap = m.work.mem.cap;
+  h.in = m.in.cap;
+  h.targets = m.outTargets.cap;
+  h.capTargets = m.capTargets;
+  h.members = m.members.cap;
+  h.splitBuf = m.splitBuf.cap;
+  h.splitState = m.splitState.cap;
+  return footprint(loadTotals(imgs), h, kGrowHeadroom);
 }
 
 void Pipeline::upload() {
@@ -2602,7 +2738,6 @@ void Pipeline::run(bool timed) {
 // the maps costs 10; the second launch's set-up 3.
 constexpr int32_t kSplitMatvecPairs = 22, kSplitStagePairs = 10, kSplitLaunchPairs = 3;
 constexpr int32_t kSplitFirstLaunchPct = 170;
-constexpr uint32_t kSplitMaxColumns = 4;   // split columns per load
 constexpr uint32_t kClassWideMaxTiles = 2048;  // class workgroups up to which the 16-column walk is used
 
 static int32_t firstLaunchSteps(int32_t n) { return n * kSplitFirstLaunchPct / 100; }

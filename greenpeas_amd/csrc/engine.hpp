@@ -100,6 +100,15 @@ class Pipeline {
   // circuits; synchronizes).  For measurement.
   uint64_t sortedItems() const;
   uint64_t deviceBytes() const;
+  // Device bytes a fresh pipeline allocates to load and compile `imgs`
+  // (growable: with the headroom a growing pipeline takes), including a
+  // time-chunked sweep's buffers at their largest: an upper bound of what
+  // deviceBytes() reports after such a load and its compiles.
+  static uint64_t deviceBytesFor(const std::vector<const StepgImage *> &imgs, bool growable);
+  // Device bytes this pipeline holds after loading and compiling `imgs`:
+  // its buffers where the load fits them, grown (with headroom) where not.
+  // An upper bound of its footprint during that load and its compiles too.
+  uint64_t deviceBytesAfter(const std::vector<const StepgImage *> &imgs) const;
   int device() const { return device_; }
   void *stream() const;
   // Queues later work on `stream` (a hipStream_t of the pipeline's device;

@@ -929,22 +929,4 @@ void buildImage(const Stepg &g, StepgImage &img, bool parallel) {
   }
 }
 
-uint64_t imageDeviceBytes(const StepgImage &img) {
-  const uint64_t R = img.numRanks;
-  // sort buckets at the default first-digit fill (64 nodes per bucket),
-  // 256 slots each, plus an arena of R slots for oversized buckets
-  uint64_t B = 1;
-  while (B * 64 < R) B <<= 1;
-  const uint64_t slots = B * 256 + R;
-  uint64_t b = 0;
-  b += static_cast<uint64_t>(kSweepLeadPad + img.numPairs) * (img.layerStride + 1) * 16ull + img.seedVals.size() * 4ull;
-  b += img.numCols * 16ull + R * 16ull + img.rankSpill.size() * 4ull + img.probTab.size() * 8ull;  // column bands, probabilities, frame rows
-  b += 2ull * (img.numPairs + kSweepLeadPad) * img.layerStride * img.numCols * 4ull;  // frame sensitivities
-  b += R * (16 + 64 + 16 + 24);                 // node records, target rows, overflow list, sort workspace
-  b += slots * (8 + 4 + 4 + 8 + 8 + 4 + 4);     // bucket slots: keys, ranks, run arrays
-  b += (B + 1) * 4 * 9;                         // per-bucket counters, counts and offsets
-  b += R * (8 + 4 + 8 + 4) + R * 16 * 4;        // outputs (targets sized 16 per class at first)
-  return b;
-}
-
 }  // namespace gpa

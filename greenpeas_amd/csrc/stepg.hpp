@@ -146,7 +146,4 @@ uint32_t sweepMinRpt();
 void setProbTable(bool on);
 bool probTable();
 
-// Device bytes the pipeline needs for an image (used by admission control).
-uint64_t imageDeviceBytes(const StepgImage &img);
-
 }  // namespace gpa

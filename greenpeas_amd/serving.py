@@ -1,19 +1,28 @@
 """Serving loop: continuous batching of DEM compile requests on one GPU.
 
 No counterpart in the reference (which compiles synchronously, one circuit
-per call).  Requests arrive at any time; `slots` worker threads each own a
-batched compiler with its own HIP stream and device buffers, so batches on
-different slots overlap on the device.  Whenever a slot is idle it admits
-the queued requests, oldest first, while
+per call).  A request passes two stages:
 
-* the batch holds fewer than `max_batch` requests, and
-* the estimated device footprint of everything admitted and still running,
-  across all slots, stays within `hbm_budget` bytes,
+1. prepare -- on a pool of `prepare_threads` host threads the circuit text
+   is parsed and built into its device image (gp_prepare_circuit), and the
+   device bytes a new batch needs for it alone are known exactly
+   (gp_prepared_device_bytes).  A request needing more than the whole
+   budget fails here with MemoryError.
+2. compile -- `slots` worker threads each own a batched compiler (its own
+   HIP stream and device buffers).  An idle slot admits prepared requests,
+   oldest first, while the batch holds fewer than `max_batch` requests and
+   the bytes every other slot holds (or has reserved for its running batch)
+   plus the bytes this slot will hold after compiling the batch
+   (gp_batch_bytes_after: its buffers where they fit, grown where not; an
+   upper bound of its peak too) stay within the budget.  Then it compiles
+   them in one batched pipeline and completes their futures.
 
-then compiles them in one batched pipeline and completes their futures.
-Requests that arrive while every slot is busy wait for the next free slot
-(continuous batching, not a fixed batch queue).  A request larger than the
-whole budget is rejected with `MemoryError` on submission.
+So host builds of later requests overlap the device work of earlier batches,
+and batches on different slots overlap on the device.  The budget is
+`hbm_budget` bytes or, by default, the device's free memory once the slots
+have warmed up plus what they hold, less a margin of 5 % of the device (at
+least 1 GiB).  A slot whose own buffers keep the oldest request out while
+no other slot holds anything frees them (a new compiler) and admits again.
 
 Per-request results are independent of what they are batched with: each
 circuit owns its node range, column work items and sort buckets, and the
@@ -26,10 +35,11 @@ kernels loaded before the first request's latency is counted.
 
 from __future__ import annotations
 
-import collections
+import heapq
+import itertools
 import threading
 import time
-from concurrent.futures import Future
+from concurrent.futures import Future, ThreadPoolExecutor
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -46,136 +56,189 @@ def circuit_summary(text: str) -> tuple[int, int, int, int]:
     return tuple(int(x) for x in out)
 
 
-def estimate_bytes(text: str, level: int = 2) -> int:
-    """Cheap upper estimate of the device bytes one circuit needs in the
-    batched pipeline (from circuit_summary; builds nothing)."""
-    layers, qubits, dets, obs = circuit_summary(text)
-    k = (3, 5, 8)[level]
-    nodes = layers * qubits * k
-    ranks = nodes * 3 // 5 + 64
-    cols = (dets + obs + 31) // 32
-    # sort buckets (64 nodes each at the default fill, 256 slots per bucket,
-    # plus an arena of one slot per node), 40 bytes per slot
-    buckets = 1
-    while buckets * 64 < ranks:
-        buckets <<= 1
-    slots = buckets * 256 + ranks
-    return int(ranks * (cols * 4 + 232) + slots * 40 + cols * 4096 + nodes * 24
-               + layers * cols * 4 + 65536)
-
-
 @dataclass
 class Request:
     text: str
-    nbytes: int
+    seq: int
     t_submit: float
     future: Future = field(default_factory=Future)
+    prepared: object = None
+    nbytes: int = 0  # device bytes a new batch needs for this request alone
+    t_ready: float = 0.0
     t_start: float = 0.0
     t_done: float = 0.0
 
+    def __lt__(self, other: "Request") -> bool:
+        return self.seq < other.seq
+
 
 class Server:
-    """Continuous-batching DEM compile server (see the module docstring)."""
+    """Continuous-batching DEM compile server (see the module docstring).
+
+    compiler_factory() makes a slot's compiler (default: a BatchCompiler on
+    `device`): set_prepared(list), compile() -> results in order,
+    device_us(), device_bytes(), bytes_after(list), close().
+    preparer(text) makes a prepared request (default: batch.prepare at the
+    server's level): device_bytes(), close()."""
 
     def __init__(self, correlation_level: CorrelationLevel = CorrelationLevel.L2, device: int = 0,
-                 slots: int = 2, max_batch: int = 64, hbm_budget: int = 64 << 30,
-                 parse_threads: int = 8, compiler_factory=None, warmup=()):
+                 slots: int = 2, max_batch: int = 64, hbm_budget: int | None = None,
+                 prepare_threads: int = 8, compiler_factory=None, preparer=None, warmup=()):
         self.level = int(correlation_level)
         self.slots = slots
         self.max_batch = max_batch
-        self.hbm_budget = hbm_budget
-        self.parse_threads = parse_threads
+        on_device = compiler_factory is None
         if compiler_factory is None:
             from .batch import BatchCompiler
 
             def compiler_factory():
                 return BatchCompiler(CorrelationLevel(self.level), device)
+        if preparer is None:
+            from .batch import prepare
+
+            def preparer(text):
+                return prepare(text, CorrelationLevel(self.level))
         self._factory = compiler_factory
-        self._queue: collections.deque[Request] = collections.deque()
+        self._preparer = preparer
         self._cv = threading.Condition()
-        self._in_use = 0  # estimated bytes admitted and not yet finished
+        self._ready: list[Request] = []  # prepared, not yet admitted (a heap by submission order)
+        self._held = [0] * slots  # device bytes each slot holds or has reserved
+        self._preparing = 0
+        self._seq = itertools.count()
         self._closed = False
+        self.hbm_budget = hbm_budget if hbm_budget is not None else 1 << 62
         self.completed: list[Request] = []
         self.batches: list[tuple[int, int, float]] = []  # (slot, size, device us)
         self._warmup = [circuit_text(c) for c in warmup]
         self._warmup_errors: list[BaseException] = []
-        self._ready = threading.Barrier(slots + 1)
+        self._started = threading.Barrier(slots + 1)
+        self._go = threading.Event()
+        self._pool = ThreadPoolExecutor(max(1, prepare_threads), thread_name_prefix="gp-prepare")
         self._workers = [threading.Thread(target=self._worker, args=(i,), daemon=True)
                          for i in range(slots)]
         for w in self._workers:
             w.start()
-        self._ready.wait()  # every slot warmed up (or failed to)
+        self._started.wait()  # every slot warmed up (or failed to)
         if self._warmup_errors:
             self.close()
             raise RuntimeError("server warm-up failed") from self._warmup_errors[0]
+        if hbm_budget is None and on_device:
+            from .batch import device_memory
+
+            free, total = device_memory(device)
+            self.hbm_budget = free + sum(self._held) - max(1 << 30, total // 20)
+        self._go.set()
 
     # ------------------------------------------------------------------
     def submit(self, circuit) -> Future:
+        """Queues a circuit; its future gets the DEM, or the error that its
+        build or compile raised (MemoryError: larger than the budget)."""
         text = circuit_text(circuit)
-        nbytes = estimate_bytes(text, self.level)
-        if nbytes > self.hbm_budget:
-            raise MemoryError(f"request needs ~{nbytes} device bytes, budget is {self.hbm_budget}")
-        req = Request(text, nbytes, time.perf_counter())
         with self._cv:
             if self._closed:
                 raise RuntimeError("server is closed")
-            self._queue.append(req)
-            self._cv.notify_all()
+            req = Request(text, next(self._seq), time.perf_counter())
+            self._preparing += 1
+        self._pool.submit(self._prepare, req)
         return req.future
 
-    def _admit(self) -> list[Request]:
-        """Oldest-first admission under the batch-size and HBM limits."""
+    def _prepare(self, req: Request) -> None:
+        err = None
+        try:
+            req.prepared = self._preparer(req.text)
+            req.nbytes = int(req.prepared.device_bytes())
+            if req.nbytes > self.hbm_budget:
+                err = MemoryError(f"request needs {req.nbytes} device bytes, the budget is {self.hbm_budget}")
+        except BaseException as e:  # noqa: BLE001 -- delivered to the future
+            err = e
+        req.t_ready = time.perf_counter()
+        with self._cv:
+            self._preparing -= 1
+            if err is None:
+                heapq.heappush(self._ready, req)
+            self._cv.notify_all()
+        if err is not None:
+            if req.prepared is not None:
+                req.prepared.close()
+            req.future.set_exception(err)
+
+    def _admit(self, slot: int, compiler) -> list[Request] | None:
+        """Oldest-first admission under the batch-size and memory limits
+        (under the lock).  None: this slot's own buffers keep the oldest
+        request out while no other slot holds anything."""
+        others = sum(h for t, h in enumerate(self._held) if t != slot)
         batch: list[Request] = []
-        while self._queue and len(batch) < self.max_batch:
-            r = self._queue[0]
-            if self._in_use + r.nbytes > self.hbm_budget and (batch or self._in_use):
+        after = self._held[slot]
+        while self._ready and len(batch) < self.max_batch:
+            r = self._ready[0]
+            need = compiler.bytes_after([x.prepared for x in batch] + [r.prepared])
+            if others + need > self.hbm_budget:
+                if not batch and others == 0 and self._held[slot] > 0:
+                    return None
                 break
-            self._queue.popleft()
-            self._in_use += r.nbytes
+            heapq.heappop(self._ready)
             batch.append(r)
+            after = need
+        if batch:
+            self._held[slot] = max(self._held[slot], after)
         return batch
 
     def _worker(self, slot: int) -> None:
+        compiler = None
         try:
             compiler = self._factory()
             if self._warmup:
-                compiler.set_circuits(self._warmup, threads=self.parse_threads)
+                prepared = [self._preparer(t) for t in self._warmup]
+                compiler.set_prepared(prepared)
                 compiler.compile()
+                for p in prepared:
+                    p.close()
+            self._held[slot] = int(compiler.device_bytes())
         except Exception as e:  # noqa: BLE001 -- reported by the constructor
             with self._cv:
                 self._warmup_errors.append(e)
-            self._ready.wait()
+            self._started.wait()
             return
-        self._ready.wait()
+        self._started.wait()
+        self._go.wait()
         while True:
             with self._cv:
-                while not self._closed and not self._queue:
+                while True:
+                    if self._closed and not self._ready and not self._preparing:
+                        compiler.close()
+                        return
+                    batch = self._admit(slot, compiler)
+                    if batch is None or batch:
+                        break
                     self._cv.wait()
-                if self._closed and not self._queue:
-                    return
-                batch = self._admit()
-                if not batch:
-                    self._cv.wait(0.001)
-                    continue
+                if batch is None:
+                    self._held[slot] = 0
+            if batch is None:
+                compiler.close()  # frees this slot's buffers
+                compiler = self._factory()
+                continue
             t0 = time.perf_counter()
             for r in batch:
                 r.t_start = t0
             try:
-                compiler.set_circuits([r.text for r in batch], threads=self.parse_threads)
+                compiler.set_prepared([r.prepared for r in batch])
                 dems = compiler.compile()
                 dev_us = compiler.device_us()
                 err = None
             except Exception as e:  # noqa: BLE001 -- delivered to every future
                 dems, dev_us, err = None, 0.0, e
+            held = int(compiler.device_bytes())
             t1 = time.perf_counter()
             with self._cv:
-                self._in_use -= sum(r.nbytes for r in batch)
+                self._held[slot] = held
                 self.batches.append((slot, len(batch), dev_us))
-                for i, r in enumerate(batch):
+                for r in batch:
                     r.t_done = t1
                     self.completed.append(r)
                 self._cv.notify_all()
             for i, r in enumerate(batch):
+                r.prepared.close()
+                r.prepared = None
                 if err is None:
                     r.future.set_result(dems[i])
                 else:
@@ -183,12 +246,15 @@ class Server:
 
     # ------------------------------------------------------------------
     def close(self, wait: bool = True) -> None:
+        """Stops taking requests; the queued ones are still compiled."""
         with self._cv:
             self._closed = True
             self._cv.notify_all()
+        self._go.set()
         if wait:
             for w in self._workers:
                 w.join()
+        self._pool.shutdown(wait=wait)
 
     def __enter__(self):
         return self
@@ -196,19 +262,30 @@ class Server:
     def __exit__(self, *exc):
         self.close()
 
+    def held_bytes(self) -> list[int]:
+        """Device bytes each slot holds (or has reserved for its batch)."""
+        with self._cv:
+            return list(self._held)
+
     def stats(self) -> dict:
-        lat = np.array([r.t_done - r.t_submit for r in self.completed]) * 1e3
+        done = list(self.completed)
+        lat = np.array([r.t_done - r.t_submit for r in done]) * 1e3
         if not len(lat):
             return {"requests": 0}
-        span = max(r.t_done for r in self.completed) - min(r.t_submit for r in self.completed)
+        prep = np.array([r.t_ready - r.t_submit for r in done]) * 1e3
+        queue = np.array([r.t_start - r.t_ready for r in done]) * 1e3
+        span = max(r.t_done for r in done) - min(r.t_submit for r in done)
         return {
             "requests": int(len(lat)),
             "p50_ms": float(np.percentile(lat, 50)),
             "p99_ms": float(np.percentile(lat, 99)),
             "mean_ms": float(lat.mean()),
+            "prepare_p50_ms": float(np.percentile(prep, 50)),
+            "queue_p50_ms": float(np.percentile(queue, 50)),
             "throughput_rps": float(len(lat) / span) if span > 0 else float("inf"),
             "batches": len(self.batches),
             "mean_batch": float(np.mean([b[1] for b in self.batches])),
+            "budget_bytes": int(self.hbm_budget),
         }
 
 

@@ -246,9 +246,40 @@ int gp_batch_device_bytes(const gp_batch *batch, uint64_t *bytes);
  * queued requests): summary = {layers (TICKs + 1, REPEAT bodies counted per
  * repetition), qubits (highest qubit target + 1), detectors, observables}. */
 int gp_circuit_summary(const char *circuit_text, uint64_t summary[4]);
-/* Estimated device bytes to compile circuit text at a level (admission). */
+/* Device bytes a new batch context allocates to compile circuit text at a
+ * level alone (its buffers with their growth headroom; a driver created from
+ * the circuit holds at most this).  An upper bound of the peak during the
+ * compile unless the DEM has more than 16 targets per error node. */
 int gp_estimate_device_bytes(const char *circuit_text, int correlation_level,
                              uint64_t *bytes);
+
+/* ------------------------------------------------------------------------
+ * Prepared circuits and memory admission (no counterpart in the reference;
+ * for servers that build circuits on their own threads and admit batches
+ * by device memory).
+ * --------------------------------------------------------------------- */
+
+typedef struct gp_prepared gp_prepared;
+/* Parses circuit text and builds its STEPG and device image (the host half
+ * of a compile) on the calling thread (and the library's worker pool when
+ * it is free).  Any number of threads may prepare circuits at once. */
+int gp_prepare_circuit(const char *circuit_text, int correlation_level, gp_prepared **out);
+int gp_prepared_destroy(gp_prepared *prepared);
+/* gp_estimate_device_bytes of the prepared circuit (no parse). */
+int gp_prepared_device_bytes(const gp_prepared *prepared, uint64_t *bytes);
+/* Makes `count` prepared circuits (of the batch's correlation level) the
+ * batch's circuits, as gp_batch_set_circuits would; the batch shares their
+ * images, so the caller may destroy its handles right after. */
+int gp_batch_set_prepared(gp_batch *batch, const gp_prepared *const *prepared, uint32_t count);
+/* Device bytes the batch holds after compiling these prepared circuits in
+ * mode 0: its buffers where they fit, grown with headroom where not.  Every
+ * buffer is freed before it is replaced, so this bounds the batch's peak
+ * during that compile too (unless the DEM has more than 16 targets per
+ * error node).  Nothing is loaded. */
+int gp_batch_bytes_after(const gp_batch *batch, const gp_prepared *const *prepared, uint32_t count,
+                         uint64_t *bytes);
+/* Free and total device memory of `device` (hipMemGetInfo). */
+int gp_device_memory(int device, uint64_t *free_bytes, uint64_t *total_bytes);
 
 /* Autotuning knobs (process-wide):
  *   "bucket_fill": mean number of (hash, node) items per sort bucket the

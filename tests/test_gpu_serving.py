@@ -74,3 +74,68 @@ def test_config5_poisson_stream_over_the_full_pool():
     st = srv.stats()
     assert st["requests"] == len(arrivals)
     assert st["batches"] < len(arrivals)
+
+
+def test_footprint_bounds_what_the_batch_holds():
+    """Admission's footprint over the serving pool (surface d 5..21, bb6 and
+    bb12): a new batch context's figure for one circuit equals the prepared
+    circuit's own, and over a seeded sequence of batches of 1..24 circuits
+    the bytes predicted before each compile bound what the context holds
+    after it (gp_batch_device_bytes), and equal it when nothing grew."""
+    from greenpeas_amd.batch import BatchCompiler, prepare
+
+    pool = request_pool(48, seed=77)
+    prepared = [prepare(t) for t in pool]
+    bc = BatchCompiler()
+    assert bc.bytes_after(prepared[:1]) == prepared[0].device_bytes()
+    rng = np.random.default_rng(11)
+    grew = 0
+    for step in range(12):
+        pick = rng.choice(len(pool), int(rng.integers(1, 25)), replace=False)
+        batch = [prepared[i] for i in pick]
+        before = bc.device_bytes()
+        est = bc.bytes_after(batch)
+        assert est >= before
+        bc.set_prepared(batch)
+        bc.run()
+        held = bc.device_bytes()
+        print(f"batch {step}: {len(batch)} circuits, held {before >> 20} -> {held >> 20} MiB, "
+              f"predicted {est >> 20} MiB", flush=True)
+        assert held <= est
+        if est == before:
+            assert held == before
+        grew += held > before
+    assert grew >= 1
+    bc.close()
+
+
+def test_prepared_circuits_compile_as_their_text():
+    from greenpeas_amd.batch import BatchCompiler, prepare
+
+    pool = request_pool(6, seed=3, surface=(5, 7), bb=(6,), bb_rounds=(6, 7))
+    a = BatchCompiler()
+    a.set_circuits(pool)
+    want = a.compile()
+    b = BatchCompiler()
+    prepared = [prepare(t) for t in pool]
+    b.set_prepared(prepared)
+    for p in prepared:
+        p.close()  # the batch keeps its own reference to each image
+    assert b.compile() == want
+    with pytest.raises(ValueError, match="correlation level"):
+        b.set_prepared([prepare(pool[0], 1)])
+    a.close()
+    b.close()
+
+
+def test_default_budget_comes_from_device_memory():
+    from greenpeas_amd.batch import device_memory
+
+    free, total = device_memory(0)
+    assert 0 < free <= total
+    with Server(slots=2, warmup=["R 0\nTICK\nM 0\nDETECTOR rec[-1]\n"]) as srv:
+        held = sum(srv.held_bytes())
+        assert held > 0
+        # free memory at start-up, less the margin: what the slots hold
+        # counts inside the budget, not against it
+        assert abs(srv.hbm_budget - (free - max(1 << 30, total // 20))) < (256 << 20)

@@ -79,6 +79,27 @@ def test_estimate_device_bytes_grows_with_circuit():
     a = estimate_device_bytes(read_circuit("surface3"))
     b = estimate_device_bytes(read_circuit("bb12"))
     assert 0 < a < b
+    # whole 2 MiB allocation units
+    assert a % (2 << 20) == 0 and b % (2 << 20) == 0
+
+
+def test_prepared_circuit_estimate_matches_the_text_estimate():
+    """A prepared circuit (parsed and built on the host, no device) reports
+    the same footprint as the text estimate, for every level; a level the
+    library does not know is rejected."""
+    from greenpeas_amd.batch import PreparedCircuit
+
+    for name in ("surface3", "bb6"):
+        text = read_circuit(name)
+        for level in (0, 1, 2):
+            p = PreparedCircuit(text, level)
+            assert p.device_bytes() == estimate_device_bytes(text, level)
+            p.close()
+            p.close()  # a second close is a no-op
+    with pytest.raises(ValueError):
+        PreparedCircuit(read_circuit("surface3"), 3)
+    with pytest.raises(ValueError):
+        PreparedCircuit("FOO 0\n", 2)
 
 
 def test_product_overlapping_channels(oracle):

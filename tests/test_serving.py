@@ -1,6 +1,7 @@
-"""Serving loop host logic on the CPU: admission by estimated HBM footprint,
-continuous batching across slots, rejection of oversized requests and error
-delivery, with a fake compiler standing in for the GPU."""
+"""Serving loop host logic on the CPU: admission by device footprint,
+continuous batching across slots, rejection of oversized requests, error
+delivery and warm-up, with a fake preparer and compiler standing in for the
+library and the GPU."""
 
 from __future__ import annotations
 
@@ -11,24 +12,55 @@ import pytest
 
 from conftest import read_circuit
 from greenpeas_amd import serving
-from greenpeas_amd.dem import DetectorErrorModel
-from greenpeas_amd.error_analysis import estimate_device_bytes
+
+
+class FakePrepared:
+    """A 'prepared' request: its text; it needs len(text) bytes alone."""
+
+    closed = 0
+
+    def __init__(self, text):
+        if "BUILD_FAIL" in text:
+            raise ValueError("planted build failure")
+        self.text = text
+
+    def device_bytes(self):
+        return len(self.text)
+
+    def close(self):
+        FakePrepared.closed += 1
 
 
 class FakeCompiler:
-    """Records batches; 'compiles' by returning the text length."""
+    """Records batches; 'compiles' by returning the text lengths.  A batch
+    needs the sum of its requests' bytes; one that does not fit what the
+    compiler holds adds that much to it (old buffers are kept)."""
 
     log: list = []
     lock = threading.Lock()
     delay = 0.02
+    made = 0
 
-    def set_circuits(self, texts, threads=0):
-        self.texts = list(texts)
+    def __init__(self):
+        self.held = 0
+        self.closed = False
+        with FakeCompiler.lock:
+            FakeCompiler.made += 1
+
+    def bytes_after(self, prepared):
+        need = sum(p.device_bytes() for p in prepared)
+        return self.held + need if need > self.held else self.held
+
+    def set_prepared(self, prepared):
+        self.texts = [p.text for p in prepared]
+        self.need = sum(len(t) for t in self.texts)
 
     def compile(self):
         with FakeCompiler.lock:
             FakeCompiler.log.append(list(self.texts))
         time.sleep(FakeCompiler.delay)
+        if self.need > self.held:
+            self.held += self.need
         if any("FAIL" in t for t in self.texts):
             raise RuntimeError("planted failure")
         return [len(t) for t in self.texts]
@@ -36,52 +68,99 @@ class FakeCompiler:
     def device_us(self):
         return 1.0
 
+    def device_bytes(self):
+        return self.held
 
-def test_estimate_is_an_upper_bound_of_the_real_footprint():
-    for name in ("surface3", "surface5", "bb6", "bb12"):
-        text = read_circuit(name)
-        assert serving.estimate_bytes(text) >= 0.8 * estimate_device_bytes(text)
+    def close(self):
+        self.closed = True
+
+
+def server(**kw):
+    return serving.Server(compiler_factory=FakeCompiler, preparer=FakePrepared, **kw)
 
 
 def test_continuous_batching_and_admission():
     FakeCompiler.log = []
     texts = [f"R 0\nTICK\nM {i}\n" for i in range(40)]
-    one = serving.estimate_bytes(texts[0])
-    srv = serving.Server(slots=2, max_batch=8, hbm_budget=one * 10, compiler_factory=FakeCompiler)
+    one = len(texts[0])
+    srv = server(slots=2, max_batch=8, hbm_budget=one * 10)
+    held_seen = []
     futs = [srv.submit(t) for t in texts]
+    while not all(f.done() for f in futs):
+        held_seen.append(sum(srv.held_bytes()))
+        time.sleep(0.002)
     assert [f.result(timeout=10) for f in futs] == [len(t) for t in texts]
     srv.close()
     sizes = [len(b) for b in FakeCompiler.log]
     assert sum(sizes) == 40
     assert max(sizes) <= 8
-    # the two slots never hold more than the budget together
-    assert all(s <= 10 for s in sizes)
+    # what the slots hold and reserve together never exceeds the budget
+    assert max(held_seen) <= one * 10
     st = srv.stats()
     assert st["requests"] == 40 and st["p99_ms"] >= st["p50_ms"] > 0
+    assert st["budget_bytes"] == one * 10
 
 
-def test_oversized_request_rejected():
-    srv = serving.Server(slots=1, hbm_budget=1000, compiler_factory=FakeCompiler)
+def test_oversized_request_fails_its_future():
+    srv = server(slots=1, hbm_budget=1000)
+    fut = srv.submit(read_circuit("bb12"))
     with pytest.raises(MemoryError):
-        srv.submit(read_circuit("bb12"))
+        fut.result(timeout=10)
+    assert srv.submit("R 0\nTICK\nM 0\n").result(timeout=10) == len("R 0\nTICK\nM 0\n")
+    srv.close()
+
+
+def test_build_errors_reach_only_their_own_future():
+    srv = server(slots=1, max_batch=4)
+    bad = srv.submit("R 0\n# BUILD_FAIL\n")
+    good = srv.submit("R 1\nTICK\nM 1\n")
+    with pytest.raises(ValueError, match="planted build"):
+        bad.result(timeout=5)
+    assert good.result(timeout=5) == len("R 1\nTICK\nM 1\n")
     srv.close()
 
 
 def test_errors_reach_every_future_of_the_batch():
     FakeCompiler.log = []
-    srv = serving.Server(slots=1, max_batch=4, compiler_factory=FakeCompiler)
+    srv = server(slots=1, max_batch=4)
     FakeCompiler.delay = 0.2
-    first = srv.submit("R 0\nTICK\nM 0\n")  # occupies the slot
-    time.sleep(0.05)
-    f1 = srv.submit("R 0\nTICK\nM 0\n# FAIL\n")
-    f2 = srv.submit("R 1\nTICK\nM 1\n")
-    assert first.result(timeout=5) > 0
-    with pytest.raises(RuntimeError, match="planted"):
-        f1.result(timeout=5)
-    with pytest.raises(RuntimeError, match="planted"):
-        f2.result(timeout=5)
-    FakeCompiler.delay = 0.02
+    try:
+        first = srv.submit("R 0\nTICK\nM 0\n")  # occupies the slot
+        time.sleep(0.05)
+        f1 = srv.submit("R 0\nTICK\nM 0\n# FAIL\n")
+        f2 = srv.submit("R 1\nTICK\nM 1\n")
+        assert first.result(timeout=5) > 0
+        with pytest.raises(RuntimeError, match="planted"):
+            f1.result(timeout=5)
+        with pytest.raises(RuntimeError, match="planted"):
+            f2.result(timeout=5)
+    finally:
+        FakeCompiler.delay = 0.02
+        srv.close()
+
+
+def test_slot_frees_its_buffers_when_they_alone_block_a_request():
+    """A slot that holds buffers from an earlier batch and cannot grow them
+    within the budget for the oldest request, while no other slot holds
+    anything, takes a new compiler (its buffers freed) and admits it."""
+    FakeCompiler.made = 0
+    srv = server(slots=1, max_batch=8, hbm_budget=100)
+    first, second = "A" * 60, "B" * 70  # 60 + 70 > 100: no room beside the first's buffers
+    assert srv.submit(first).result(timeout=5) == 60
+    assert srv.held_bytes() == [60]
+    assert srv.submit(second).result(timeout=5) == 70
+    assert srv.held_bytes() == [70]
     srv.close()
+    assert FakeCompiler.made == 2
+
+
+def test_queued_requests_are_compiled_after_close():
+    srv = server(slots=2, max_batch=4)
+    futs = [srv.submit("R %d\nTICK\nM 0\n" % i) for i in range(12)]
+    srv.close()
+    assert all(f.done() and not f.exception() for f in futs)
+    with pytest.raises(RuntimeError, match="closed"):
+        srv.submit("R 0\n")
 
 
 def test_request_pool_shapes():
@@ -111,7 +190,7 @@ def test_circuit_summary_matches_the_parser():
 
 def test_warmup_runs_on_every_slot_before_requests():
     FakeCompiler.log = []
-    srv = serving.Server(slots=3, compiler_factory=FakeCompiler, warmup=["R 0\nTICK\nM 0\n"])
+    srv = server(slots=3, warmup=["R 0\nTICK\nM 0\n"])
     # the constructor returns once every slot has compiled the warm-up batch
     assert FakeCompiler.log == [["R 0\nTICK\nM 0\n"]] * 3
     assert srv.submit("R 1\nTICK\nM 1\n").result(timeout=10) == len("R 1\nTICK\nM 1\n")
@@ -120,4 +199,4 @@ def test_warmup_runs_on_every_slot_before_requests():
 
 def test_warmup_failure_is_raised_by_the_constructor():
     with pytest.raises(RuntimeError, match="warm-up"):
-        serving.Server(slots=2, compiler_factory=FakeCompiler, warmup=["FAIL"])
+        server(slots=2, warmup=["FAIL"])
