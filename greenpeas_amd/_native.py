@@ -29,7 +29,7 @@ EXPORTS = (
     "gp_driver_bench_device", "gp_debug_pnodes", "gp_debug_sensitivities", "gp_debug_stepg", "gp_debug_sweep_plan",
     "gp_build_stepg_host", "gp_debug_image_digest",
     "gp_batch_create", "gp_batch_destroy", "gp_batch_set_circuits", "gp_batch_compile",
-    "gp_batch_stage_us", "gp_batch_device_us", "gp_batch_dem_shape", "gp_batch_dem_copy", "gp_batch_device_bytes",
+    "gp_batch_stage_us", "gp_batch_device_us", "gp_batch_dem_shape", "gp_batch_dem_copy", "gp_batch_device_bytes", "gp_batch_layout_stats",
     "gp_estimate_device_bytes", "gp_prepare_circuit", "gp_prepared_destroy", "gp_prepared_device_bytes",
     "gp_batch_set_prepared", "gp_batch_bytes_after", "gp_device_memory", "gp_circuit_summary", "gp_set_tuning", "gp_get_tuning",
     "gp_grad_plan_create", "gp_grad_plan_destroy", "gp_grad_plan_sizes", "gp_grad_plan_dem_structure",
@@ -129,6 +129,7 @@ def load() -> ctypes.CDLL:
     sig("gp_batch_dem_shape", [P, u32, pu32, pu64, pu32, pu32])
     sig("gp_batch_dem_copy", [P, u32, P, P, P, P])
     sig("gp_batch_device_bytes", [P, pu64])
+    sig("gp_batch_layout_stats", [P, P])
     sig("gp_estimate_device_bytes", [cs, i32, pu64])
     sig("gp_prepare_circuit", [cs, i32, PP])
     sig("gp_prepared_destroy", [P])

@@ -152,6 +152,14 @@ class BatchCompiler:
         _native.check(self._lib.gp_batch_device_us(self._h, ctypes.byref(us)))
         return us.value
 
+    def layout_stats(self) -> dict:
+        """{"s_words": words of the sensitivity matrix, "band_relayouts":
+        compiles that found a column band below its stored layers and ran
+        again with more} (gp_batch_layout_stats)."""
+        st = (ctypes.c_uint64 * 2)()
+        _native.check(self._lib.gp_batch_layout_stats(self._h, st))
+        return {"s_words": int(st[0]), "band_relayouts": int(st[1])}
+
     def device_bytes(self) -> int:
         b = ctypes.c_uint64()
         _native.check(self._lib.gp_batch_device_bytes(self._h, ctypes.byref(b)))

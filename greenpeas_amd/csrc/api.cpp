@@ -774,6 +774,14 @@ int gp_batch_device_bytes(const gp_batch *batch, uint64_t *bytes) {
   });
 }
 
+int gp_batch_layout_stats(const gp_batch *batch, uint64_t stats[2]) {
+  return guarded([&] {
+    if (!batch || !stats) throw InvalidArgument("NULL argument");
+    stats[0] = batch->pipe->sWords();
+    stats[1] = batch->pipe->bandRelayouts();
+  });
+}
+
 int gp_circuit_summary(const char *circuit_text, uint64_t summary[4]) {
   return guarded([&] {
     requireText(circuit_text);
@@ -933,6 +941,7 @@ int gp_set_tuning(const char *name, uint32_t value) {
     else if (std::string(name) == "mask_table") gpa::setMaskTable(value != 0);
     else if (std::string(name) == "prob_table") gpa::setProbTable(value != 0);
     else if (std::string(name) == "packed_sort") gpa::setPackedSort(value != 0);
+    else if (std::string(name) == "s_bands") gpa::setSBands(value);
     else if (std::string(name) == "sweep_split") {
       if (value > 2) throw InvalidArgument("sweep_split must be 0, 1 or 2");
       gpa::setSweepSplit(value);
@@ -950,6 +959,7 @@ int gp_get_tuning(const char *name, uint32_t *value) {
     else if (std::string(name) == "mask_table") *value = gpa::maskTable() ? 1u : 0u;
     else if (std::string(name) == "prob_table") *value = gpa::probTable() ? 1u : 0u;
     else if (std::string(name) == "packed_sort") *value = gpa::packedSort() ? 1u : 0u;
+    else if (std::string(name) == "s_bands") *value = gpa::sBands();
     else if (std::string(name) == "sweep_split") *value = gpa::sweepSplit();
     else throw InvalidArgument(std::string("unknown tuning parameter: ") + name);
   });

@@ -42,6 +42,7 @@ std::atomic<uint32_t> gStash{1};          // target rows on (1) or off (0)
 std::atomic<uint32_t> gMaskTable{1};      // classify's LDS mask tables allowed (1) or not (0)
 std::atomic<uint32_t> gSweepSplit{1};     // long columns swept in time chunks (1; 2: always) or whole (0)
 std::atomic<uint32_t> gPackedSort{1};     // bucket sort on the packed 64-bit key (1) or always exact (0)
+std::atomic<uint32_t> gSBands{1};         // growable loads' S by column band (1), whole (0), tight (2)
 
 // Per-circuit descriptor (device and host).
 struct VDesc {
@@ -50,8 +51,7 @@ struct VDesc {
   uint32_t bucketBase, numBuckets, maskWords, layerStride;
   uint32_t sweepThreads, sweepRpt, rowSpan, probDict;  // rowSpan: see StepgImage
   uint32_t numPairs, pad0, pad1, pad2;
-  uint64_t actOff, seedOff, colOff, rankOff, sOff;
-  uint64_t colStride;  // words per S column: 2 (numPairs + kSweepLeadPad) rows of layerStride
+  uint64_t actOff, seedOff, colOff, rankOff;
   uint64_t spillOff;   // first spill entry of the circuit's ranks
   uint64_t probOff;    // first probability (table entry if probDict, else the ranks' own)
 };
@@ -76,8 +76,18 @@ __device__ __forceinline__ double rankProbability(const VDesc &d, const double *
 
 // Counter block zeroed at the start of every run: [flags | bucket counts].
 // Per-run flag words, zeroed by the sweep kernel: [0] hash collision,
-// [1] target overflow, [3] overflow-list length, [5] oversized-bucket arena fill
+// [1] target overflow, [3] overflow-list length, [5] oversized-bucket arena fill;
+// and [kFlagBandOverflow], zeroed by the host: a column's band reached below
+// the rows its region of S stores
 constexpr uint32_t kFlagWords = 16;
+constexpr uint32_t kFlagBandOverflow = kFlagWords - 1;
+
+// S holds, per detector column, the rows of layers base .. top (the rows
+// its band can reach) at a region of its own, then two rows that the
+// sweep's stores for layers below base go to (nothing reads them):
+// colRegion = {region offset (words) low, high, base (even), rows stored
+// before those two}; colAdj = region offset - base * layerStride, so the
+// word of layer l >= base, position j is S[colAdj + l * layerStride + j].
 
 // --------------------------------------------------------------------------
 // wave64 helpers
@@ -421,7 +431,8 @@ struct SweepState {
   uint32_t headOff;           // 16 * layerStride: the header within a block
   uint32_t zeroB, seedB;      // LDS byte addresses of the zero word and the first seed
   uint32_t rowBytes;          // 4 * layerStride
-  uint32_t rows;              // rows per column (layers, padding layers included)
+  int32_t base;               // first layer the column's region stores
+  uint32_t trash;             // its row for the even layers below base (the odd ones: the next)
   uint32_t off[RPT];          // byte offsets of this thread's records in a block
   uint32_t pos4[RPT];         // byte offsets of this thread's positions in a row or slot
 };
@@ -466,10 +477,11 @@ __device__ __forceinline__ void sweepStep(const SweepState<RPT> &st, uint4 (&r)[
       }
     }
   }
-  // rows of layers 2m (even) and 2m + 1 (odd); padding layers below 0 wrap
-  // to the rows after the circuit's
-  const int32_t evenRow = 2 * m < 0 ? 2 * m + static_cast<int32_t>(st.rows) : 2 * m;
-  const uint32_t evenB = static_cast<uint32_t>(evenRow) * st.rowBytes, oddB = evenB + st.rowBytes;
+  // rows of layers 2m (even) and 2m + 1 (odd) in the column's region (base
+  // is even); layers below base (padding layers below 0 among them) go to
+  // the two rows after the stored ones
+  const uint32_t evenRow = 2 * m < st.base ? st.trash : static_cast<uint32_t>(2 * m - st.base);
+  const uint32_t evenB = evenRow * st.rowBytes, oddB = evenB + st.rowBytes;
 #pragma unroll
   for (int k = 0; k < RPT; ++k) {
     *ldsAt(slotB + st.pos4[k]) = b[k];
@@ -625,13 +637,13 @@ __device__ void splitStage(const SplitDesc &sd, uint32_t P, uint32_t k, const ui
 template <int RPT, bool GSEEDS>
 __global__ __launch_bounds__(kSweepMaxThreads) void sweepKernel(
     const VDesc *__restrict__ desc, const uint4 *__restrict__ items, const uint4 *__restrict__ act,
-    const uint32_t *__restrict__ seedVals, int2 *__restrict__ band, uint32_t *__restrict__ S,
-    uint32_t numItems, uint32_t *__restrict__ flags, SplitIo sp) {
+    const uint32_t *__restrict__ seedVals, const uint4 *__restrict__ colRegion, int2 *__restrict__ band,
+    uint32_t *__restrict__ S, uint32_t numItems, uint32_t *__restrict__ flags, SplitIo sp) {
   // dynamic LDS only, addressed from 0:
   // ring[2P] | zero word | seeds (unless GSEEDS) | vote[2] | (resumed
   // chunks: the entering-state chain's words)
   const uint32_t tid = threadIdx.x;
-  if (blockIdx.x == 0 && tid < kFlagWords) flags[tid] = 0u;
+  if (blockIdx.x == 0 && tid < kFlagBandOverflow) flags[tid] = 0u;
   if (blockIdx.x >= numItems + sp.numItems) return;
   // the item, then the circuit's descriptor (and the split's): two round
   // trips before the seed, state and record loads
@@ -746,10 +758,12 @@ __global__ __launch_bounds__(kSweepMaxThreads) void sweepKernel(
   if (active) {
     st.ac = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint4 *>(act + d.actOff), 0,
                                               16 * (P + 1) * (kSweepLeadPad + d.numPairs), 0x00020000);
-    const uint32_t rows = static_cast<uint32_t>(d.colStride / P);
+    const uint4 reg = colRegion[d.colOff + c];
+    const uint32_t rows = __builtin_amdgcn_readfirstlane(reg.w) + 2;
     // (no records: every store is out of range and dropped)
-    st.sc = __builtin_amdgcn_make_buffer_rsrc(S + d.sOff + static_cast<uint64_t>(c) * d.colStride, 0,
-                                              stores ? static_cast<uint32_t>(4 * d.colStride) : 0u, 0x00020000);
+    st.sc = __builtin_amdgcn_make_buffer_rsrc(
+        S + ((static_cast<uint64_t>(__builtin_amdgcn_readfirstlane(reg.y)) << 32) | __builtin_amdgcn_readfirstlane(reg.x)),
+        0, stores ? 4 * rows * P : 0u, 0x00020000);
     st.gseeds = sv;
     st.seeds = seeds;
     st.blockBytes = 16 * (P + 1);
@@ -757,7 +771,8 @@ __global__ __launch_bounds__(kSweepMaxThreads) void sweepKernel(
     st.zeroB = 8 * P;
     st.seedB = 8 * P + 4;
     st.rowBytes = 4 * P;
-    st.rows = rows;
+    st.base = static_cast<int32_t>(__builtin_amdgcn_readfirstlane(reg.z));
+    st.trash = rows - 2;
     // ring slot of each trip position (pair mt - u; D is even)
 #pragma unroll
     for (int u = 0; u < D; ++u) slotB[u] = static_cast<uint32_t>((mt - u) & 1) * P * 4;
@@ -780,7 +795,7 @@ __global__ __launch_bounds__(kSweepMaxThreads) void sweepKernel(
     // outstanding at the loop head is then the same on entry and on the
     // back edge, and its waits keep the full prefetch distance.
     loadOff = static_cast<uint32_t>(mt + static_cast<int32_t>(kSweepLeadPad)) * st.blockBytes;
-    const uint32_t padB = (rows - 2 * kSweepLeadPad) * st.rowBytes;
+    const uint32_t padB = st.trash * st.rowBytes;
 #pragma unroll
     for (int u = 0; u < D; ++u) {
       loadBlock<RPT>(rs[u], head[u], st.ac, loadOff, st.off, st.headOff);
@@ -838,7 +853,13 @@ __global__ __launch_bounds__(kSweepMaxThreads) void sweepKernel(
     const uint32_t slot = static_cast<uint32_t>(stop & 1) * P;
     for (uint32_t i = tid; i < P; i += blockDim.x) out[i] = *ldsAt(4 * (slot + i));
   }
-  if (tid == 0 && (mode == kSplitNone || mode == kSplitTop)) band[d.colOff + c] = make_int2(lo, hi);
+  if (tid == 0 && (mode == kSplitNone || mode == kSplitTop)) {
+    // a band below the stored rows is cut at them (readers stay inside the
+    // region) and reported: the host stores more rows and runs again
+    const int32_t base = static_cast<int32_t>(colRegion[d.colOff + c].z);
+    if (lo < base) atomicOr(&flags[kFlagBandOverflow], 1u);
+    band[d.colOff + c] = make_int2(lo < base ? base : lo, hi);
+  }
 }
 
 // One node's frame rows (its sensitivity is their XOR), decoded from its
@@ -921,8 +942,8 @@ __device__ __forceinline__ void buildMaskTable(uint32_t *tab, uint32_t l0, uint3
 // ~15 columns per layer; 4 where nodes see few columns).
 template <int kBatch, typename F>
 __device__ __forceinline__ void forEachTargetMasked(const uint32_t *mrow, uint32_t maskWords, const int2 *band,
-                                                    const FrameRows &r, const uint32_t *__restrict__ S0,
-                                                    uint64_t colStride, F &&f) {
+                                                    const int64_t *adj, const FrameRows &r,
+                                                    const uint32_t *__restrict__ S, F &&f) {
   if (!r.ok[0] || !maskWords) return;
   uint32_t w = 0;  // the window's low word
   uint64_t m = mrow[0] | (maskWords > 1 ? static_cast<uint64_t>(mrow[1]) << 32 : 0ull);
@@ -938,16 +959,18 @@ __device__ __forceinline__ void forEachTargetMasked(const uint32_t *mrow, uint32
       m &= m - 1;
     }
     int2 bd[kBatch];
+    int64_t aj[kBatch];
 #pragma unroll
-    for (int j = 0; j < kBatch; ++j) bd[j] = col[j] != NONE ? band[col[j]] : make_int2(1, 0);
+    for (int j = 0; j < kBatch; ++j) {
+      bd[j] = col[j] != NONE ? band[col[j]] : make_int2(1, 0);
+      aj[j] = col[j] != NONE ? adj[col[j]] : 0;
+    }
 #pragma unroll
     for (int j = 0; j < kBatch; ++j)
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int32_t l = static_cast<int32_t>(r.lay[q]);
-        x[j][q] = (r.ok[q] && bd[j].x <= l && l <= bd[j].y)
-                      ? S0[static_cast<uint64_t>(col[j]) * colStride + r.idx[q]]
-                      : 0u;
+        x[j][q] = (r.ok[q] && bd[j].x <= l && l <= bd[j].y) ? S[aj[j] + r.idx[q]] : 0u;
       }
 #pragma unroll
     for (int j = 0; j < kBatch; ++j) {
@@ -967,15 +990,20 @@ __device__ __forceinline__ void forEachTargetMasked(const uint32_t *mrow, uint32
 // the few nodes without a stashed target row, and by classify when a
 // workgroup's mask table would not fit in LDS.
 template <typename F>
-__device__ __forceinline__ void forEachTargetScan(uint32_t numCols, const int2 *band, const FrameRows &r,
-                                                  const uint32_t *__restrict__ S0, uint64_t colStride, F &&f) {
+__device__ __forceinline__ void forEachTargetScan(uint32_t numCols, const int2 *band, const int64_t *adj,
+                                                  const FrameRows &r, const uint32_t *__restrict__ S, F &&f) {
   if (!r.ok[0]) return;
   for (uint32_t c = 0; c < numCols; ++c) {
     const int2 bd = band[c];
     if (bd.y < static_cast<int32_t>(r.lo) || bd.x > static_cast<int32_t>(r.lo + r.span)) continue;
+    const int64_t a = adj[c];
     uint32_t x[4];
+    // rows outside the band are not read (they may lie outside the region)
 #pragma unroll
-    for (int q = 0; q < 4; ++q) x[q] = r.ok[q] ? S0[static_cast<uint64_t>(c) * colStride + r.idx[q]] : 0u;
+    for (int q = 0; q < 4; ++q) {
+      const int32_t l = static_cast<int32_t>(r.lay[q]);
+      x[q] = r.ok[q] && bd.x <= l && l <= bd.y ? S[a + r.idx[q]] : 0u;
+    }
     uint32_t v = bandXor(r, bd, x);
     while (v) {
       f(c * 32 + static_cast<uint32_t>(__builtin_ctz(v)));
@@ -1030,12 +1058,13 @@ struct RankIn {
 template <bool TABLE, int BATCH>
 __global__ __launch_bounds__(kClassBlock) void classifyKernel(
     const VDesc *__restrict__ desc, const uint4 *__restrict__ tiles, const int2 *__restrict__ band,
-    const uint32_t *__restrict__ S, RankIn rin, ClassifyOut o) {
+    const int64_t *__restrict__ colAdj, const uint32_t *__restrict__ S, RankIn rin, ClassifyOut o) {
   // per wave: 64 rows of kStashWords words, padded to kStashWords + 1 so
   // that a lane's writes and the row-major read-out avoid bank conflicts
   __shared__ uint32_t rows[kClassBlock / 64][64 * (kStashWords + 1)];
   __shared__ uint32_t maskTab[TABLE ? kMaskTableWords : 1];
   __shared__ int2 bandTab[kBandTableCols];
+  __shared__ int64_t adjTab[kBandTableCols];
   // tile: {circuit, first rank, lowest layer of its ranks' rows, layers
   // spanned by them}; the host picks TABLE only when every workgroup's
   // table fits kMaskTableWords
@@ -1049,9 +1078,14 @@ __global__ __launch_bounds__(kClassBlock) void classifyKernel(
   const double prob = rankProbability(d, rin.probVal, rin.probIdx, gi);
   const uint32_t l0 = t.z, nl = t.w;
   const int2 *bandC = band + d.colOff;
+  const int64_t *adjC = colAdj + d.colOff;
   if (d.numCols <= kBandTableCols) {
-    for (uint32_t c = threadIdx.x; c < d.numCols; c += blockDim.x) bandTab[c] = bandC[c];
+    for (uint32_t c = threadIdx.x; c < d.numCols; c += blockDim.x) {
+      bandTab[c] = bandC[c];
+      adjTab[c] = adjC[c];
+    }
     bandC = bandTab;  // filled by the barrier below
+    adjC = adjTab;
   }
   if constexpr (TABLE)
     buildMaskTable(maskTab, l0, nl, d.rowSpan, d.numCols, d.maskWords, band + d.colOff);
@@ -1079,10 +1113,10 @@ __global__ __launch_bounds__(kClassBlock) void classifyKernel(
       cnt++;
     };
     if constexpr (TABLE)
-      forEachTargetMasked<BATCH>(maskTab + (fr.span * nl + (fr.lo - l0)) * d.maskWords, d.maskWords, bandC, fr,
-                          S + d.sOff, d.colStride, take);
+      forEachTargetMasked<BATCH>(maskTab + (fr.span * nl + (fr.lo - l0)) * d.maskWords, d.maskWords, bandC, adjC,
+                                 fr, S, take);
     else
-      forEachTargetScan(d.numCols, bandC, fr, S + d.sOff, d.colStride, take);
+      forEachTargetScan(d.numCols, bandC, adjC, fr, S, take);
     if (cnt && cnt <= 32)
       for (uint32_t k = cnt; k < 32; ++k) h = (h ^ 0xFFFFFFFFull) * kFnvPrime;
     h2 = mix2(h2, cnt);
@@ -1441,6 +1475,7 @@ struct EmitIn {
   uint32_t stashMaxCols;
   uint32_t stashShift;  // row words = 1 << stashShift
   const int2 *band;
+  const int64_t *colAdj;
 };
 struct EmitOut {
   double *p;
@@ -1635,7 +1670,7 @@ __global__ __launch_bounds__(256) void emitKernel(const VDesc *__restrict__ desc
       pos += n;
     } else {
       const FrameRows fr = decodeRows(expandRankCode(in.rankCode[rk], in.spill + d.spillOff), d.layerStride);
-      forEachTargetScan(d.numCols, in.band + d.colOff, fr, in.S + d.sOff, d.colStride, [&](uint32_t tg) {
+      forEachTargetScan(d.numCols, in.band + d.colOff, in.colAdj + d.colOff, fr, in.S, [&](uint32_t tg) {
         put(pos, tg);
         pos++;
       });
@@ -1971,6 +2006,7 @@ struct WorkArena {
     return roundGrain(std::max<size_t>(off, 256));
   }
   // headroom: a buffer that must grow takes this much more than asked
+  // (not on the arena's first allocation)
   void commit(bool frozen, double headroom = 1.0) {
     // this load's requests (none stay behind if a capacity check throws)
     std::vector<Req> rq;
@@ -1987,7 +2023,8 @@ struct WorkArena {
       size_t off = 0;
       std::vector<size_t> offs(rq.size()), caps(rq.size());
       for (size_t k = 0; k < rq.size(); ++k) {
-        const size_t grown = rq[k].bytes > rq[k].buf->cap ? static_cast<size_t>(rq[k].bytes * headroom) : 0;
+        const size_t grown =
+            mem.p && rq[k].bytes > rq[k].buf->cap ? static_cast<size_t>(rq[k].bytes * headroom) : 0;
         const size_t c = std::max({rq[k].bytes, rq[k].buf->cap, grown});
         const size_t align = c >= (size_t{1} << 20) ? kAllocGrain : 4096;
         off = (off + align - 1) & ~(align - 1);
@@ -2132,15 +2169,26 @@ struct Pipeline::Impl {
   // oversized buckets): pslot (classify's records), prank (the sorted
   // ranks) and the run arrays;
   // per bucket: counters, slotBase, item/run/target counts and their scans
-  DevBuf band, S, stash, info, counters, bucketCnt, ovSlot, ovBucket, pslot, prank,
+  DevBuf band, stash, info, counters, bucketCnt, ovSlot, ovBucket, pslot, prank,
       runRank, runP, runHash, runTgtPre, runMember, runCount, runNnz, itemCount, slotBase, wgSum, classStart,
       tgtStart, memberStart, outP, outOff, outHash, outMember, outTargets, scratchKey, scratchRank,
       scanTmp, vBounds, members;
   WorkArena work;  // holds the buffers above except in, outTargets, members and the split buffers
   uint64_t capTargets = 0;
+  // S (its own allocation: it grows when a band needs more rows) and its
+  // per-column layout (see colRegion)
+  DevBuf S;
+  struct ColLayout {
+    uint32_t P;     // layer stride
+    uint32_t top1;  // 1 + the top layer the sweep stores (0: no seeds)
+    uint32_t base;  // first layer stored
+  };
+  std::vector<ColLayout> cols;  // every column of the load
+  size_t oColRegion = 0, oColAdj = 0;
+  uint32_t bandRelayouts = 0;  // loads' columns given more rows after a run
   // the arena's buffers, in the order workBufferSizes() sizes them
-  std::array<DevBuf *, 31> workBufs() {
-    return {&band,      &S,          &info,        &stash,      &ovSlot,      &ovBucket,  &counters, &bucketCnt,
+  std::array<DevBuf *, 30> workBufs() {
+    return {&band,                    &info,        &stash,      &ovSlot,      &ovBucket,  &counters, &bucketCnt,
             &pslot,     &prank,      &runRank,     &runP,       &runHash,     &runTgtPre, &runMember, &runCount,
             &runNnz,    &itemCount,  &slotBase,    &wgSum,      &classStart,  &tgtStart,  &memberStart, &outP,
             &outOff,    &outHash,    &outMember,   &scratchKey, &scratchRank, &scanTmp,   &vBounds};
@@ -2253,9 +2301,9 @@ void Pipeline::reserve(const std::vector<const StepgImage *> &imgs) {
 
 static constexpr size_t kLdsBudget = 150 * 1024;  // of gfx950's 160 KB per workgroup
 
-// Growable pipelines (batches, serving slots) that must grow a buffer take
-// this much more than asked, so a stream of batches of varying size settles
-// after a few loads instead of reallocating on each larger one.
+// Growable pipelines (batches, serving slots) that must grow a buffer they
+// hold take this much more than asked, so a stream of batches of varying
+// size settles after a few loads instead of reallocating on each larger one.
 constexpr double kGrowHeadroom = 1.25;
 
 // bucket counters sit this many words apart (16 bytes while there are few
@@ -2267,19 +2315,17 @@ static uint32_t bucketCountStride(uint64_t B) {
 }
 
 // Every work buffer of a load, in the arena's order, by its totals: R error
-// nodes, B sort buckets, V circuits, Ccols detector columns, Swords words of
-// S.  req(index, bytes, what) is called once per buffer.
-constexpr uint32_t kWorkBuffers = 31;
+// nodes, B sort buckets, V circuits, Ccols detector columns.  req(index,
+// bytes, what) is called once per buffer.
+constexpr uint32_t kWorkBuffers = 30;
 template <typename F>
-static void workBufferSizes(uint64_t R, uint64_t B, uint64_t V, uint64_t Ccols, uint64_t Swords,
-                            uint32_t countStride, F &&req) {
+static void workBufferSizes(uint64_t R, uint64_t B, uint64_t V, uint64_t Ccols, uint32_t countStride, F &&req) {
   R = std::max<uint64_t>(R, 1);
   // bucket slots, then an arena of R entries for oversized buckets
   const uint64_t slots = B * static_cast<uint64_t>(kBucketCap) + R;
   if (slots >= 0xFFFFFFFFull) throw RuntimeError("too many sort buckets in one load");
   uint32_t k = 0;
   req(k++, 8 * std::max<uint64_t>(Ccols, 1), "columns");                // band
-  req(k++, 4 * std::max<uint64_t>(Swords, 1), "sensitivity matrix");    // S
   req(k++, 16 * R, "nodes");                                             // info
   req(k++, 4ull * kStashWords * R, "nodes");                             // stash
   req(k++, 16 * R, "nodes");                                             // ovSlot
@@ -2308,6 +2354,66 @@ static void workBufferSizes(uint64_t R, uint64_t B, uint64_t V, uint64_t Ccols, 
   req(k++, 4 * (2 * V + 8), "circuits");                                 // vBounds
 }
 
+// The layers of S a column stores: base .. top, top = the odd layer of the
+// sweep's first pair.  Banded (growable pipelines): base = the column's
+// lowest seed layer less twice the circuit's seed gap and 4 layers (made
+// even), enough for the bands of memory experiments, whose detectors
+// compare one round's measurements with the round before; columns holding
+// observables, whose bands reach back to the start, and unbanded (frozen)
+// pipelines store every layer (base 0).  A band that reaches below base
+// anyway is reported by the sweep, and the load stores more rows.
+static Pipeline::Impl::ColLayout columnLayout(const StepgImage &g, uint32_t c, bool banded) {
+  const int32_t hi = g.colHi[c];
+  if (hi < 0) return {g.layerStride, 0, 0};
+  const uint32_t top1 = 2 * static_cast<uint32_t>(hi / 2) + 2;
+  uint32_t base = 0;
+  const uint32_t mode = gSBands.load();
+  if (banded && mode && (c + 1) * 32 <= g.numDetectors) {
+    const int64_t b = static_cast<int64_t>(g.colLow[c]) - (mode == 2 ? 0 : 2 * static_cast<int64_t>(g.seedGap) + 4);
+    base = b > 0 ? static_cast<uint32_t>(b) & ~1u : 0u;
+  }
+  return {g.layerStride, top1, base};
+}
+
+// Words of S a column's region takes: its rows and the two rows below-base
+// stores go to.
+static uint64_t regionWords(const Pipeline::Impl::ColLayout &c) {
+  return c.top1 ? static_cast<uint64_t>(c.top1 - c.base + 2) * c.P : 0u;
+}
+
+// Bytes DevBuf::ensure allocates for `bytes` (with `head` more when it grows).
+static size_t grownBytes(size_t bytes, double head) {
+  return roundGrain(std::max<size_t>(static_cast<size_t>(static_cast<double>(bytes) * head), 256));
+}
+
+// Lays S out from m.cols: the regions (colRegion, colAdj) in the staged
+// inputs, and on the device when `device`; S's capacity (grown with `head`;
+// a frozen pipeline throws instead).
+static void layoutS(Pipeline::Impl &m, bool frozen, double head, bool device) {
+  uint4 *reg = reinterpret_cast<uint4 *>(m.inHost.as<char>() + m.oColRegion);
+  int64_t *adj = reinterpret_cast<int64_t *>(m.inHost.as<char>() + m.oColAdj);
+  uint64_t off = 0;
+  for (size_t k = 0; k < m.cols.size(); ++k) {
+    const Pipeline::Impl::ColLayout &c = m.cols[k];
+    reg[k] = make_uint4(static_cast<uint32_t>(off), static_cast<uint32_t>(off >> 32), c.base,
+                        c.top1 ? c.top1 - c.base : 0u);
+    adj[k] = static_cast<int64_t>(off) - static_cast<int64_t>(c.base) * c.P;
+    off += regionWords(c);
+  }
+  m.Swords = off;
+  const size_t bytes = 4 * std::max<uint64_t>(off, 1);
+  if (bytes > m.S.cap) {
+    if (frozen) m.S.ensure(bytes, true, "sensitivity matrix");  // throws
+    const double grow = m.S.cap ? head : 1.0;  // (no headroom on the first allocation)
+    m.S.release();
+    m.S.ensure(static_cast<size_t>(static_cast<double>(bytes) * grow), false, "sensitivity matrix");
+  }
+  if (device && !m.cols.empty()) {
+    GPA_HIP(hipMemcpy(m.in.as<char>() + m.oColRegion, reg, 16 * m.cols.size(), hipMemcpyHostToDevice));
+    GPA_HIP(hipMemcpy(m.in.as<char>() + m.oColAdj, adj, 8 * m.cols.size(), hipMemcpyHostToDevice));
+  }
+}
+
 void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   Impl &m = *impl_;
   GPA_HIP(hipSetDevice(device_));
@@ -2328,13 +2434,14 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   // layout
   m.descs.assign(V, VDesc{});
   variants_.assign(V, VariantInfo{});
-  uint64_t actOff = 0, seedOff = 0, colOff = 0, rankOff = 0, sOff = 0, spillOff = 0, probOff = 0;
+  uint64_t actOff = 0, seedOff = 0, colOff = 0, rankOff = 0, spillOff = 0, probOff = 0;
   uint32_t bucketBase = 0, maxRpt = 1, maxThreads = 64;
   size_t ldsRing = 0, ldsAll = 0;  // largest ring (+ zero word), largest ring + seeds
   std::vector<uint4> items;
   std::vector<uint4> tiles;
   bool classTable = true;
   std::vector<std::pair<int32_t, uint32_t>> order;  // (band estimate, item index)
+  m.cols.clear();
   for (uint32_t v = 0; v < V; ++v) {
     const StepgImage &g = *imgs[v];
     VDesc &d = m.descs[v];
@@ -2357,9 +2464,8 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
     d.seedOff = seedOff;
     d.colOff = colOff;
     d.rankOff = rankOff;
-    d.sOff = sOff;
     d.numPairs = g.numPairs;
-    d.colStride = 2ull * (g.numPairs + kSweepLeadPad) * g.layerStride;
+    for (uint32_t c = 0; c < g.numCols; ++c) m.cols.push_back(columnLayout(g, c, growable_));
     d.spillOff = spillOff;
     d.probOff = probOff;
     d.probDict = g.probTab.empty() ? 0u : 1u;
@@ -2384,7 +2490,6 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
     rankOff += g.numRanks;
     spillOff += g.rankSpill.size() / 4;
     probOff += d.probDict ? g.probTab.size() : g.numRanks;
-    sOff += d.colStride * g.numCols;
     bucketBase += d.numBuckets;
     maxRpt = std::max(maxRpt, g.sweepRpt);
     maxThreads = std::max(maxThreads, g.sweepThreads);
@@ -2421,7 +2526,6 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   m.B = bucketBase;
   m.R = static_cast<uint32_t>(rankOff);
   m.Ccols = static_cast<uint32_t>(colOff);
-  m.Swords = sOff;
 
   // input arena
   size_t off = 0;
@@ -2433,6 +2537,8 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   m.oDesc = place(sizeof(VDesc) * V);
   m.oItems = place(sizeof(uint4) * m.nItems);
   m.oTiles = place(sizeof(uint4) * m.nTiles);
+  m.oColRegion = place(16 * m.Ccols);
+  m.oColAdj = place(8 * m.Ccols);
   m.oAct = place(16 * actOff);
   m.oSeed = place(4 * seedOff);
   m.oRankCode = place(8 * rankOff);
@@ -2446,6 +2552,7 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   std::memcpy(hb + m.oDesc, m.descs.data(), sizeof(VDesc) * V);
   std::memcpy(hb + m.oItems, launchItems.data(), sizeof(uint4) * m.nItems);
   std::memcpy(hb + m.oTiles, tiles.data(), sizeof(uint4) * m.nTiles);
+  layoutS(m, frozen, growable_ ? kGrowHeadroom : 1.0, false);
   for (uint32_t v = 0; v < V; ++v) {
     const StepgImage &g = *imgs[v];
     const VDesc &d = m.descs[v];
@@ -2468,7 +2575,7 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   m.countStride = bucketCountStride(B);
   const auto bufs = m.workBufs();
   static_assert(std::tuple_size<decltype(bufs)>::value == kWorkBuffers, "one DevBuf per work buffer");
-  workBufferSizes(R, B, V, m.Ccols, m.Swords, m.countStride,
+  workBufferSizes(R, B, V, m.Ccols, m.countStride,
                   [&](uint32_t k, uint64_t bytes, const char *what) { wa.request(*bufs[k], bytes, what); });
   wa.commit(frozen, growable_ ? kGrowHeadroom : 1.0);
   if (m.capTargets == 0 || (!frozen && m.capTargets < 16 * R)) {
@@ -2491,8 +2598,10 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
                                   static_cast<int>(m.sweepLdsBytes)));
   }
 
-  // bucket counters start at zero; every run's bucket stage leaves them zeroed
+  // bucket counters start at zero; every run's bucket stage leaves them
+  // zeroed; so does the band-overflow flag
   GPA_HIP(hipMemsetAsync(m.bucketCnt.p, 0, m.bucketCnt.cap, m.stream));
+  GPA_HIP(hipMemsetAsync(m.counters.as<uint32_t>() + kFlagBandOverflow, 0, 4, m.stream));
   upload();
 }
 
@@ -2504,7 +2613,7 @@ struct LoadTotals {
   uint32_t maxP = 64;  // widest layer stride
 };
 
-LoadTotals loadTotals(const std::vector<const StepgImage *> &imgs) {
+LoadTotals loadTotals(const std::vector<const StepgImage *> &imgs, bool banded) {
   LoadTotals t;
   uint64_t actOff = 0, seedOff = 0, spill = 0, probOff = 0, nTiles = 0;
   for (const StepgImage *g : imgs) {
@@ -2516,12 +2625,13 @@ LoadTotals loadTotals(const std::vector<const StepgImage *> &imgs) {
     t.R += g->numRanks;
     spill += g->rankSpill.size() / 4;
     probOff += g->probTab.empty() ? g->numRanks : g->probTab.size();
-    t.Swords += 2ull * (g->numPairs + kSweepLeadPad) * g->layerStride * g->numCols;
+    for (uint32_t c = 0; c < g->numCols; ++c) t.Swords += regionWords(columnLayout(*g, c, banded));
     t.maxP = std::max(t.maxP, g->layerStride);
   }
   t.V = imgs.size();
   // the input arena: each array at a 256-byte boundary
-  t.inBytes = align256(sizeof(VDesc) * t.V) + align256(16 * t.Ccols) + align256(16 * nTiles) + align256(16 * actOff) +
+  t.inBytes = align256(sizeof(VDesc) * t.V) + align256(16 * t.Ccols) + align256(16 * nTiles) +
+              align256(16 * t.Ccols) + align256(8 * t.Ccols) + align256(16 * actOff) +
               align256(4 * seedOff) + align256(8 * t.R) + align256(16 * spill) + align256(2 * t.R) +
               align256(8 * probOff);
   return t;
@@ -2534,7 +2644,7 @@ uint64_t afterEnsure(size_t cap, uint64_t bytes) { return bytes <= cap ? cap : r
 struct HeldCaps {
   std::array<size_t, kWorkBuffers> work{};
   bool arena = false;  // the arena is allocated
-  size_t arenaCap = 0, in = 0, targets = 0, members = 0, splitBuf = 0, splitState = 0;
+  size_t arenaCap = 0, in = 0, S = 0, targets = 0, members = 0, splitBuf = 0, splitState = 0;
   uint64_t capTargets = 0;
 };
 
@@ -2550,14 +2660,16 @@ uint64_t footprint(const LoadTotals &t, const HeldCaps &h, double headroom) {
   const uint64_t R = std::max<uint64_t>(t.R, 1);
   std::vector<size_t> caps;
   bool fits = h.arena;
-  workBufferSizes(R, t.B, t.V, t.Ccols, t.Swords, bucketCountStride(t.B), [&](uint32_t k, uint64_t bytes, const char *) {
+  workBufferSizes(R, t.B, t.V, t.Ccols, bucketCountStride(t.B), [&](uint32_t k, uint64_t bytes, const char *) {
     const size_t b = std::max<size_t>(bytes, 256), cap = h.work[k];
-    const size_t grown = b > cap ? static_cast<size_t>(b * headroom) : 0;
+    const size_t grown = h.arena && b > cap ? static_cast<size_t>(b * headroom) : 0;
     if (b > cap) fits = false;
     caps.push_back(std::max({b, cap, grown}));
   });
   uint64_t total = fits ? h.arenaCap : WorkArena::layoutBytes(caps);
   total += afterEnsure(h.in, t.inBytes);
+  const uint64_t sBytes = 4 * std::max<uint64_t>(t.Swords, 1);
+  total += sBytes <= h.S ? h.S : grownBytes(sBytes, h.S ? headroom : 1.0);
   total += h.capTargets == 0 || h.capTargets < 16 * R ? afterEnsure(h.targets, 4 * 16 * R) : h.targets;
   total += h.members;
   const uint64_t G = (t.maxP + 31) / 32;
@@ -2570,7 +2682,7 @@ uint64_t footprint(const LoadTotals &t, const HeldCaps &h, double headroom) {
 }  // namespace
 
 uint64_t Pipeline::deviceBytesFor(const std::vector<const StepgImage *> &imgs, bool growable) {
-  return footprint(loadTotals(imgs), HeldCaps{}, growable ? kGrowHeadroom : 1.0);
+  return footprint(loadTotals(imgs, growable), HeldCaps{}, growable ? kGrowHeadroom : 1.0);
 }
 
 uint64_t Pipeline::deviceBytesAfter(const std::vector<const StepgImage *> &imgs) const {
@@ -2581,13 +2693,14 @@ uint64_t Pipeline::deviceBytesAfter(const std::vector<const StepgImage *> &imgs)
   for (uint32_t k = 0; k < kWorkBuffers; ++k) h.work[k] = bufs[k]->cap;
   h.arena = m.work.mem.p != nullptr;
   h.arenaCap = m.work.mem.cap;
+  h.S = m.S.cap;
   h.in = m.in.cap;
   h.targets = m.outTargets.cap;
   h.capTargets = m.capTargets;
   h.members = m.members.cap;
   h.splitBuf = m.splitBuf.cap;
   h.splitState = m.splitState.cap;
-  return footprint(loadTotals(imgs), h, kGrowHeadroom);
+  return footprint(loadTotals(imgs, true), h, kGrowHeadroom);
 }
 
 void Pipeline::upload() {
@@ -2626,6 +2739,8 @@ void Pipeline::run(bool timed) {
   const uint4 *tiles = reinterpret_cast<const uint4 *>(ib + m.oTiles);
   const uint4 *act = reinterpret_cast<const uint4 *>(ib + m.oAct);
   const uint32_t *seed = reinterpret_cast<const uint32_t *>(ib + m.oSeed);
+  const uint4 *colRegion = reinterpret_cast<const uint4 *>(ib + m.oColRegion);
+  const int64_t *colAdj = reinterpret_cast<const int64_t *>(ib + m.oColAdj);
   const RankIn rin{reinterpret_cast<const uint2 *>(ib + m.oRankCode), reinterpret_cast<const uint4 *>(ib + m.oSpill),
                    reinterpret_cast<const double *>(ib + m.oProbVal),
                    reinterpret_cast<const uint16_t *>(ib + m.oProbIdx)};
@@ -2649,11 +2764,11 @@ void Pipeline::run(bool timed) {
     const SplitIo sp1 = m.splitIo(false), sp2 = m.splitIo(true);
 #define GPA_SWEEP(RPT, G)                                                                             \
   sweepKernel<RPT, G><<<std::max<uint32_t>(m.nItems + sp1.numItems, 1), m.sweepThreads, m.sweepLdsBytes, \
-                        st>>>(desc, items, act, seed, m.band.as<int2>(), m.S.as<uint32_t>(), \
+                        st>>>(desc, items, act, seed, colRegion, m.band.as<int2>(), m.S.as<uint32_t>(), \
                               m.nItems, m.flags(), sp1);                                                \
   if (sp2.numItems)                                                                                     \
     sweepKernel<RPT, true><<<sp2.numItems, m.sweepThreads, m.splitLdsBytes, st>>>(                      \
-        desc, items, act, seed, m.band.as<int2>(), m.S.as<uint32_t>(), 0u, m.flags(), sp2)
+        desc, items, act, seed, colRegion, m.band.as<int2>(), m.S.as<uint32_t>(), 0u, m.flags(), sp2)
     if (m.seedsInLds) {
       switch (m.rpt) {
       case 1: GPA_SWEEP(1, false); break;
@@ -2679,13 +2794,13 @@ void Pipeline::run(bool timed) {
                    m.flags(),              m.stash.as<uint32_t>(),   stashMaxCols,
                    m.stashShift,           gPackedSort.load()};
     if (m.classTable && gMaskTable.load() && m.classBatch == 16)
-      classifyKernel<true, 16><<<m.nTiles, kClassBlock, 0, st>>>(desc, tiles, m.band.as<int2>(),
+      classifyKernel<true, 16><<<m.nTiles, kClassBlock, 0, st>>>(desc, tiles, m.band.as<int2>(), colAdj,
                                                                  m.S.as<uint32_t>(), rin, co);
     else if (m.classTable && gMaskTable.load())
-      classifyKernel<true, 4><<<m.nTiles, kClassBlock, 0, st>>>(desc, tiles, m.band.as<int2>(),
+      classifyKernel<true, 4><<<m.nTiles, kClassBlock, 0, st>>>(desc, tiles, m.band.as<int2>(), colAdj,
                                                                 m.S.as<uint32_t>(), rin, co);
     else
-      classifyKernel<false, 4><<<m.nTiles, kClassBlock, 0, st>>>(desc, tiles, m.band.as<int2>(),
+      classifyKernel<false, 4><<<m.nTiles, kClassBlock, 0, st>>>(desc, tiles, m.band.as<int2>(), colAdj,
                                                                  m.S.as<uint32_t>(), rin, co);
   }
   mark(2);
@@ -2719,7 +2834,7 @@ void Pipeline::run(bool timed) {
              m.tgtStart.as<uint32_t>(), m.memberStart.as<uint32_t>(), m.runRank.as<uint32_t>(),
              m.runTgtPre.as<uint32_t>(), m.runMember.as<uint32_t>(), m.runP.as<double>(),
              m.runHash.as<uint64_t>(), m.runNnz.as<uint32_t>(), m.S.as<uint32_t>(),
-             m.stash.as<uint32_t>(), rin.code, rin.spill, stashMaxCols, m.stashShift, m.band.as<int2>()};
+             m.stash.as<uint32_t>(), rin.code, rin.spill, stashMaxCols, m.stashShift, m.band.as<int2>(), colAdj};
   EmitOut eout{m.outP.as<double>(), m.outOff.as<uint32_t>(), m.outHash.as<uint64_t>(), m.outMember.as<uint32_t>(),
                m.outTargets.as<uint32_t>(), m.targets16 ? m.outTargets.as<uint16_t>() : nullptr, m.capTargets,
                m.flags(), m.vBounds.as<uint32_t>()};
@@ -2896,6 +3011,17 @@ static void planSweep(Pipeline::Impl &m) {
   std::memcpy(h.data() + m.oSplitItems1, items1.data(), 16 * items1.size());
   std::memcpy(h.data() + m.oSplitItems3, items3.data(), 16 * items3.size());
   m.dropGraph();  // before the buffers it may name are replaced
+  // a split column's band is reported from layer 0 (its top chunk does not
+  // reach the bottom): it stores every layer
+  bool relayout = false;
+  for (const SplitDesc &sd : descs) {
+    Pipeline::Impl::ColLayout &cl = m.cols[m.descs[sd.v].colOff + sd.c];
+    if (cl.base) {
+      cl.base = 0;
+      relayout = true;
+    }
+  }
+  if (relayout) layoutS(m, m.frozen, kGrowHeadroom, true);
   m.splitBuf.ensure(bytes, false, "split sweep");
   m.splitState.ensure(4 * stateWords, false, "split sweep");
   GPA_HIP(hipMemcpy(m.splitBuf.p, h.data(), bytes, hipMemcpyHostToDevice));
@@ -2963,6 +3089,23 @@ void Pipeline::download() {
   }
   const uint32_t *vb = m.hVB.as<uint32_t>();
   const uint32_t *fl = m.hFlags.as<uint32_t>();
+  if (fl[kFlagBandOverflow]) {
+    // a column's band reached below its stored rows (cut there, so this
+    // run's classes are wrong): the columns cut at their base store every
+    // layer from now on, and the load runs again
+    std::vector<int2> bands(m.Ccols);
+    GPA_HIP(hipMemcpy(bands.data(), m.band.p, 8ull * m.Ccols, hipMemcpyDeviceToHost));
+    for (uint32_t k = 0; k < m.Ccols; ++k)
+      if (m.cols[k].base && bands[k].y >= 0 && bands[k].x <= static_cast<int32_t>(m.cols[k].base)) m.cols[k].base = 0;
+    ++m.bandRelayouts;
+    m.dropGraph();
+    layoutS(m, m.frozen, kGrowHeadroom, true);
+    GPA_HIP(hipMemsetAsync(m.counters.as<uint32_t>() + kFlagBandOverflow, 0, 4, m.stream));
+    m.sizesKnown = false;
+    run(false);
+    download();
+    return;
+  }
   if (fl[0]) throw RuntimeError("64-bit class hash collision between different error classes");
   m.totalClasses = vb[2 * V];
   m.totalTargets = vb[2 * V + 1];
@@ -3142,6 +3285,13 @@ uint32_t sweepSplit() { return gSweepSplit.load(); }
 
 void setMaskTable(bool on) { gMaskTable.store(on ? 1u : 0u); }
 void setPackedSort(bool on) { gPackedSort.store(on ? 1u : 0u); }
+void setSBands(uint32_t mode) {
+  if (mode > 2) throw InvalidArgument("s_bands must be 0, 1 or 2");
+  gSBands.store(mode);
+}
+uint32_t sBands() { return gSBands.load(); }
+uint32_t Pipeline::bandRelayouts() const { return impl_->bandRelayouts; }
+uint64_t Pipeline::sWords() const { return impl_->Swords; }
 bool packedSort() { return gPackedSort.load() != 0; }
 bool maskTable() { return gMaskTable.load() != 0; }
 uint32_t bucketFill() { return gBucketFill.load(); }
@@ -3150,7 +3300,7 @@ uint64_t Pipeline::deviceBytes() const {
   const Impl &m = *impl_;
   // the work arena and the buffers with their own allocations
   uint64_t b = m.work.mem.cap;
-  for (const DevBuf *x : {&m.in, &m.outTargets, &m.members, &m.splitBuf, &m.splitState}) b += x->cap;
+  for (const DevBuf *x : {&m.in, &m.S, &m.outTargets, &m.members, &m.splitBuf, &m.splitState}) b += x->cap;
   return b;
 }
 
@@ -3176,10 +3326,10 @@ void Pipeline::debugSensitivities(uint32_t v, std::vector<uint32_t> &cols32) con
   GPA_HIP(hipSetDevice(device_));
   GPA_HIP(hipStreamSynchronize(m.stream));
   const VDesc &d = m.descs.at(v);
-  std::vector<uint32_t> S(static_cast<size_t>(d.colStride) * d.numCols);
+  std::vector<uint32_t> S(m.Swords);
   std::vector<int2> band(d.numCols);
-  if (!S.empty())
-    GPA_HIP(hipMemcpy(S.data(), m.S.as<uint32_t>() + d.sOff, 4 * S.size(), hipMemcpyDeviceToHost));
+  if (!S.empty()) GPA_HIP(hipMemcpy(S.data(), m.S.p, 4 * S.size(), hipMemcpyDeviceToHost));
+  const int64_t *adj = reinterpret_cast<const int64_t *>(m.inHost.as<char>() + m.oColAdj) + d.colOff;
   if (d.numCols)
     GPA_HIP(hipMemcpy(band.data(), m.band.as<int2>() + d.colOff, 8 * d.numCols, hipMemcpyDeviceToHost));
   // each rank's word = XOR of its in-band frame rows
@@ -3203,7 +3353,7 @@ void Pipeline::debugSensitivities(uint32_t v, std::vector<uint32_t> &cols32) con
         const int32_t l = static_cast<int32_t>(cd >> kRowPosBits);
         if (l < band[c].x || l > band[c].y) continue;
         const uint64_t row = static_cast<uint64_t>(l) * d.layerStride + (cd & ((1u << kRowPosBits) - 1));
-        w ^= S[static_cast<size_t>(c) * d.colStride + row];
+        w ^= S[static_cast<size_t>(adj[c] + static_cast<int64_t>(row))];
       }
       cols32[static_cast<size_t>(i) * d.numCols + c] = w;
     }

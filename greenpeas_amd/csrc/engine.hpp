@@ -109,6 +109,10 @@ class Pipeline {
   // its buffers where the load fits them, grown (with headroom) where not.
   // An upper bound of its footprint during that load and its compiles too.
   uint64_t deviceBytesAfter(const std::vector<const StepgImage *> &imgs) const;
+  // Times a run of this pipeline found a column band below its stored rows
+  // (and the load then stored more rows), and S's words now.
+  uint32_t bandRelayouts() const;
+  uint64_t sWords() const;
   int device() const { return device_; }
   void *stream() const;
   // Queues later work on `stream` (a hipStream_t of the pipeline's device;
@@ -161,6 +165,13 @@ bool maskTable();
 // back to.  Same results either way.  Read at each run.
 void setPackedSort(bool on);
 bool packedSort();
+// How growable pipelines (batches, serving) lay out S: 1 (default) each
+// detector column stores the layers its band can reach (from its lowest
+// seed layer less twice the circuit's seed gap), 0 every layer, 2 only
+// from its lowest seed layer (bands reach below that: exercises the re-run
+// with more rows; for tests).  Same results in every mode.  Read at load.
+void setSBands(uint32_t mode);
+uint32_t sBands();
 // Whether a load's long columns may be swept in time chunks after its first
 // run: 1 (default) where the estimated critical path gains, 2 every column
 // whose band reaches the bottom with the most chunks that fit (for tests), 0

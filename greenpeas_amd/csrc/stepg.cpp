@@ -578,6 +578,17 @@ void buildImage(const Stepg &g, StepgImage &img, bool parallel) {
     for (uint32_t c = 0; c < C; ++c) row[c] ^= bits[c];
   }
   const uint32_t S = static_cast<uint32_t>(seedNode.size());
+  {
+    std::vector<uint8_t> seeded(L, 0);
+    for (uint32_t n : seedNode) seeded[n / npl] = 1;
+    uint32_t last = 0, gap = 0;
+    for (uint32_t l = 0; l < L; ++l)
+      if (seeded[l]) {
+        gap = std::max(gap, l - last);
+        last = l;
+      }
+    img.seedGap = gap;
+  }
   img.colHi.assign(C, -1);
   img.colLow.assign(C, -1);
   for (uint32_t s = 0; s < S; ++s) {

@@ -118,6 +118,9 @@ struct StepgImage {
   // largest (frame row layer - rank layer) over all ranks' frame rows
   uint32_t rowSpan = 0;
   uint32_t numPairs = 0;           // (numLayers + 1) / 2 sweep steps
+  // largest distance in layers between consecutive layers holding seeds
+  // (from layer 0; a memory experiment's round length)
+  uint32_t seedGap = 0;
   std::vector<uint32_t> layerAct;  // numLayers + 1 offsets of each layer's first active node
   std::vector<uint32_t> act;       // 4 * (kSweepLeadPad + numPairs) * (layerStride + 1): padding
                                    // blocks, then one block per pair (see the file comment)

@@ -241,15 +241,21 @@ int gp_batch_dem_copy(const gp_batch *batch, uint32_t index, double *probs,
                       uint32_t *offsets, uint32_t *targets, uint64_t *hashes);
 /* Bytes of device memory the batch holds now. */
 int gp_batch_device_bytes(const gp_batch *batch, uint64_t *bytes);
+/* Layout of the batch's sensitivity matrix: stats[0] = its 32-bit words
+ * (each detector column stores the layers its time band can reach; see
+ * "s_bands"), stats[1] = times a compile found a band reaching below a
+ * column's stored layers and compiled again with more. */
+int gp_batch_layout_stats(const gp_batch *batch, uint64_t stats[2]);
 /* Sizes read off circuit text in one pass, without parsing or building
  * anything (no counterpart in the reference; for memory admission of
  * queued requests): summary = {layers (TICKs + 1, REPEAT bodies counted per
  * repetition), qubits (highest qubit target + 1), detectors, observables}. */
 int gp_circuit_summary(const char *circuit_text, uint64_t summary[4]);
 /* Device bytes a new batch context allocates to compile circuit text at a
- * level alone (its buffers with their growth headroom; a driver created from
- * the circuit holds at most this).  An upper bound of the peak during the
- * compile unless the DEM has more than 16 targets per error node. */
+ * level alone.  An upper bound of the peak during the compile unless the
+ * DEM has more than 16 targets per error node or a detector column's time
+ * band reaches further back than the rows the batch stores for it (see
+ * "s_bands"). */
 int gp_estimate_device_bytes(const char *circuit_text, int correlation_level,
                              uint64_t *bytes);
 
@@ -306,6 +312,14 @@ int gp_device_memory(int device, uint64_t *free_bytes, uint64_t *total_bytes);
  *   (hash, rank) sort where two distinct hashes share those bits or a
  *   circuit has 2^24 - 1 or more error nodes; 0 always sorts exactly (same
  *   results); read at each compile.
+ *   "s_bands": how batches lay out the sensitivity matrix: 1 (default)
+ *   each detector column stores only the layers its time band can reach,
+ *   from its lowest seed layer less twice the circuit's largest gap between
+ *   seeded layers (columns holding observables store every layer); 0 every
+ *   column stores every layer; 2 from its lowest seed layer only, so that
+ *   bands reach below and compiles run again with more rows (a test
+ *   setting); same results; read at each load.  Drivers always store every
+ *   layer (their capacity is fixed at creation).
  *   "sweep_split": 1 (default) lets a load's first compile plan a
  *   time-chunked sweep of columns much longer than the rest (transfer maps
  *   of the middle chunks, then the lower chunks in parallel; identical

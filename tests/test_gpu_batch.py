@@ -132,6 +132,12 @@ def test_config3_256_variants_batched_equals_one_by_one(oracle):
     bc = BatchCompiler()
     bc.set_circuits(texts)
     batched = bc.compile()
+    # S stores each detector column's band only, and no band reached below
+    # the rows its column stores
+    held = bc.device_bytes()
+    stats = bc.layout_stats()
+    print(f"256 variants: {held / 2**30:.2f} GiB of device memory, {stats}", flush=True)
+    assert stats["band_relayouts"] == 0
     one = bc.compile(one_by_one=True)
     for i, (a, b) in enumerate(zip(batched, one)):
         assert a == b, f"variant {i} differs batched vs one by one"
@@ -144,3 +150,36 @@ def test_config3_256_variants_batched_equals_one_by_one(oracle):
         ref = oracle.compile_stepg(texts[i], 2)
         assert np.array_equal(batched[i].hashes, ref.hashes), f"variant {i}: order differs from the oracle"
         assert batched[i].approx_equals(as_dem(ref), 1e-12), f"variant {i} differs from the oracle"
+
+
+def test_sensitivity_layouts_give_identical_dems():
+    """Whole columns (s_bands 0), column bands (1, the default) and bands
+    cut at each column's lowest seed layer (2: bands reach below the stored
+    rows, the compile runs again with more) give the same DEMs, equal to
+    each circuit compiled alone by a driver; banded S is smaller."""
+    from greenpeas_amd import autotune
+
+    texts = variant_circuits(6, seed=21, distances=(5, 7)) + [
+        read_circuit(n) for n in ("bb12", "surface5", "iceberg_surface_6_adaptive_circl")]
+    want = [solo(t) for t in texts]
+    words = {}
+    try:
+        for mode in (0, 2, 1):
+            autotune.set_tuning("s_bands", mode)
+            bc = BatchCompiler()
+            bc.set_circuits(texts)
+            got = bc.compile()
+            stats = bc.layout_stats()
+            print(f"s_bands {mode}: {stats}", flush=True)
+            for i, (a, b) in enumerate(zip(got, want)):
+                assert a == b, f"s_bands {mode}: circuit {i} differs from its solo compile"
+            assert bc.compile() == got  # the relaid-out load again (graph replay)
+            words[mode] = stats["s_words"]
+            if mode == 2:
+                assert stats["band_relayouts"] >= 1
+            else:
+                assert stats["band_relayouts"] == 0
+            bc.close()
+    finally:
+        autotune.set_tuning("s_bands", 1)
+    assert words[1] < words[0]

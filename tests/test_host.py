@@ -112,8 +112,8 @@ def test_product_overlapping_channels(oracle):
 
 
 def test_tuning_knobs_roundtrip():
-    """Both knobs set and read back through the C-ABI (no device work); bad
-    names and sweep_rpt values are rejected with ValueError."""
+    """Every knob set and read back through the C-ABI (no device work); bad
+    names and out-of-range values are rejected with ValueError."""
     from greenpeas_amd import autotune
 
     old_fill, old_rpt = autotune.get_tuning("bucket_fill"), autotune.get_tuning("sweep_rpt")
@@ -147,6 +147,11 @@ def test_tuning_knobs_roundtrip():
         assert autotune.get_tuning("sweep_split") == 1
         with pytest.raises(ValueError):
             autotune.set_tuning("sweep_split", 3)
+        for m in (0, 2, 1):
+            autotune.set_tuning("s_bands", m)
+            assert autotune.get_tuning("s_bands") == m
+        with pytest.raises(ValueError):
+            autotune.set_tuning("s_bands", 3)
         with pytest.raises(ValueError):
             autotune.set_tuning("sweep_rpt", 3)
         with pytest.raises(ValueError):
