@@ -32,6 +32,7 @@ EXPORTS = (
     "gp_batch_stage_us", "gp_batch_device_us", "gp_batch_dem_shape", "gp_batch_dem_copy", "gp_batch_device_bytes", "gp_batch_layout_stats",
     "gp_estimate_device_bytes", "gp_prepare_circuit", "gp_prepared_destroy", "gp_prepared_device_bytes",
     "gp_batch_set_prepared", "gp_batch_bytes_after", "gp_device_memory", "gp_circuit_summary", "gp_set_tuning", "gp_get_tuning",
+    "gp_set_tuning_class", "gp_get_tuning_class", "gp_clear_tuning_classes",
     "gp_grad_plan_create", "gp_grad_plan_destroy", "gp_grad_plan_sizes", "gp_grad_plan_dem_structure",
     "gp_grad_plan_device_arrays", "gp_grad_plan_copy_arrays", "gp_xor_merge_forward",
     "gp_xor_merge_backward", "gp_node_rates_forward", "gp_node_rates_backward",
@@ -140,6 +141,9 @@ def load() -> ctypes.CDLL:
     sig("gp_circuit_summary", [cs, P])
     sig("gp_set_tuning", [cs, u32])
     sig("gp_get_tuning", [cs, pu32])
+    sig("gp_set_tuning_class", [i32, u32, cs, u32])
+    sig("gp_get_tuning_class", [i32, u32, cs, pu32])
+    sig("gp_clear_tuning_classes", [])
     sig("gp_grad_plan_create", [cs, i32, i32, PP])
     sig("gp_grad_plan_destroy", [P])
     sig("gp_grad_plan_sizes", [P, P])

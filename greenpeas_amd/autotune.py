@@ -1,4 +1,4 @@
-"""Autotuner for the pipeline's two launch-shape knobs.
+"""Autotuner for the pipeline's launch-shape knobs.
 
 * ``bucket_fill`` -- the sort's first radix digit: the pipeline partitions
   each circuit's (hash, node) items into 2^b buckets by the top b hash bits
@@ -6,27 +6,37 @@
   the smallest width with ``bucket_fill * 2^b >= nodes``.  Small buckets mean
   a short register sort but more buckets to scan and emit; large ones the
   opposite.
+* ``bucket_cap`` -- slots per bucket (64, 128, 256): with the digit width it
+  is the shape of the hash table; nodes past a bucket's slots go to an
+  overflow list sorted by the bucket's whole workgroup.  Fewer slots mean
+  less memory and a denser table, more overflow.
 * ``sweep_rpt`` -- the fewest STEPG records per thread in the sensitivity
-  sweep: more records per thread means fewer waves meeting at each layer's
-  barrier and more work per wave between barriers.
+  sweep (the wave tile): more records per thread means fewer waves meeting
+  at each layer's barrier and more work per wave between barriers.
 
-``tune_driver`` / ``tune_batch`` time candidate values on the GPU for a
-workload (one knob after the other) and keep the fastest; results are cached
-per (level, size class) in a JSON file so later drivers and batches pick
-them up with ``apply_cached``.
+``tune_driver`` times candidate values on the GPU for one circuit (one knob
+after the other, each kept for its size class while the next is timed) and
+keeps the fastest for circuits of its size class (level, floor(log2 STEPG
+nodes)); results are cached in a JSON file.  The library applies a size
+class's tuned values itself when it parses or loads a circuit of that class
+(gp_set_tuning_class); ``install_cached`` hands it the cache, once per
+process, before the first driver or batch is made.  ``tune_batch`` tunes the
+process-wide values for a batch.  Every knob leaves the DEMs bit-identical.
 """
 
 from __future__ import annotations
 
 import ctypes
 import json
-import math
 import os
+import threading
 
 from . import _native
 
 CANDIDATES = (24, 32, 48, 64, 96, 128)  # bucket_fill
+CAP_CANDIDATES = (64, 128, 256)         # bucket_cap
 RPT_CANDIDATES = (1, 2, 4, 8)            # sweep_rpt
+KNOBS = ("sweep_rpt", "bucket_fill", "bucket_cap")
 CACHE = os.environ.get("GREENPEAS_AMD_TUNE_CACHE",
                        os.path.join(os.path.expanduser("~"), ".cache", "greenpeas_amd_tune.json"))
 
@@ -41,6 +51,23 @@ def get_tuning(name: str) -> int:
     return int(v.value)
 
 
+def set_tuning_class(level: int, size_class: int, name: str, value: int) -> None:
+    """Knob `name` = `value` for circuits of this level and size class (0
+    removes the entry)."""
+    _native.check(_native.load().gp_set_tuning_class(int(level), int(size_class), name.encode(), int(value)))
+
+
+def get_tuning_class(level: int, size_class: int, name: str) -> int:
+    v = ctypes.c_uint32()
+    _native.check(_native.load().gp_get_tuning_class(int(level), int(size_class), name.encode(),
+                                                     ctypes.byref(v)))
+    return int(v.value)
+
+
+def clear_tuning_classes() -> None:
+    _native.check(_native.load().gp_clear_tuning_classes())
+
+
 def set_bucket_fill(fill: int) -> None:
     set_tuning("bucket_fill", fill)
 
@@ -50,7 +77,8 @@ def get_bucket_fill() -> int:
 
 
 def size_class(num_nodes: int) -> int:
-    return int(math.log2(max(num_nodes, 1)))
+    """floor(log2 nodes), the library's size class of a circuit."""
+    return max(int(num_nodes), 1).bit_length() - 1
 
 
 def _load_cache() -> dict:
@@ -69,9 +97,49 @@ def _store_cache(cache: dict) -> None:
     os.replace(tmp, CACHE)
 
 
+_installed = False
+_install_lock = threading.Lock()
+
+
+def install_cached(force: bool = False) -> int:
+    """Hands every cached size class's knobs to the library (once per
+    process unless `force`); returns the number of classes installed."""
+    global _installed
+    with _install_lock:
+        if _installed and not force:
+            return 0
+        _installed = True
+        n = 0
+        for key, best in _load_cache().items():
+            try:
+                level, cls = key[1:].split(":")
+                level, cls = int(level), int(cls)
+            except ValueError:
+                continue
+            if isinstance(best, int):  # older cache entries hold the fill only
+                best = {"bucket_fill": best}
+            for name in KNOBS:
+                if name in best:
+                    set_tuning_class(level, cls, name, int(best[name]))
+            n += 1
+        return n
+
+
+def _tune_class_knob(level: int, cls: int, name: str, candidates, measure) -> dict:
+    """Times `measure()` with each candidate value of knob `name` for the
+    size class; keeps the fastest there and returns {"best", "us"}."""
+    results = {}
+    for v in candidates:
+        set_tuning_class(level, cls, name, v)
+        results[v] = measure()
+    best = min(results, key=results.get)
+    set_tuning_class(level, cls, name, best)
+    return {"best": best, "us": results}
+
+
 def _tune_knob(name: str, candidates, measure) -> dict:
-    """Times `measure()` with each candidate value of knob `name`; leaves the
-    fastest set and returns {"best": value, "us": {value: us}}."""
+    """Times `measure()` with each candidate value of the process-wide knob
+    `name`; leaves the fastest set and returns {"best", "us"}."""
     results = {}
     old = get_tuning(name)
     try:
@@ -86,33 +154,40 @@ def _tune_knob(name: str, candidates, measure) -> dict:
 
 
 def tune_driver(circuit, level: int = 2, iterations: int = 20, candidates=CANDIDATES,
-                rpt_candidates=RPT_CANDIDATES) -> dict:
-    """Times the candidate sweep_rpt, then bucket_fill values on one
-    circuit's device pipeline; keeps the fastest in the process and the cache."""
+                rpt_candidates=RPT_CANDIDATES, cap_candidates=CAP_CANDIDATES, cache: bool = True) -> dict:
+    """Times the candidate sweep_rpt, bucket_fill and bucket_cap values on
+    one circuit's device pipeline, one after the other, for its size class;
+    keeps the fastest for that class in the library (and the cache)."""
     from .error_analysis import Driver
 
-    nodes = 0
-
     def measure():
-        nonlocal nodes
         drv = Driver(circuit, level)
         try:
             drv.compile_detector_error_model()
-            nodes = drv.params["num_nodes"]
             return drv.bench_device(iterations)
         finally:
             drv.close()
 
-    rpt = _tune_knob("sweep_rpt", rpt_candidates, measure)
-    fill = _tune_knob("bucket_fill", candidates, measure)
-    cache = _load_cache()
-    cache[f"L{level}:{size_class(nodes)}"] = {"bucket_fill": fill["best"], "sweep_rpt": rpt["best"]}
-    _store_cache(cache)
-    return {"best": fill["best"], "us": fill["us"], "sweep_rpt": rpt}
+    drv = Driver(circuit, level)
+    nodes = drv.params["num_nodes"]
+    drv.close()
+    cls = size_class(nodes)
+    out = {"size_class": cls}
+    for name, cands in (("sweep_rpt", rpt_candidates), ("bucket_fill", candidates),
+                        ("bucket_cap", cap_candidates)):
+        out[name] = _tune_class_knob(level, cls, name, cands, measure)
+    if cache:
+        c = _load_cache()
+        c[f"L{level}:{cls}"] = {name: out[name]["best"] for name in KNOBS}
+        _store_cache(c)
+    out["best"] = out["bucket_fill"]["best"]
+    return out
 
 
-def tune_batch(circuits, level: int = 2, candidates=CANDIDATES, rpt_candidates=RPT_CANDIDATES) -> dict:
-    """Same, for a batch of circuits compiled in one pipeline."""
+def tune_batch(circuits, level: int = 2, candidates=CANDIDATES, rpt_candidates=RPT_CANDIDATES,
+               cap_candidates=CAP_CANDIDATES) -> dict:
+    """The process-wide knobs for a batch compiled in one pipeline (size
+    class entries are set aside while timing and handed back after)."""
     from .batch import BatchCompiler
 
     def measure():
@@ -123,20 +198,20 @@ def tune_batch(circuits, level: int = 2, candidates=CANDIDATES, rpt_candidates=R
         finally:
             bc.close()
 
-    rpt = _tune_knob("sweep_rpt", rpt_candidates, measure)
-    fill = _tune_knob("bucket_fill", candidates, measure)
-    return {"best": fill["best"], "us": fill["us"], "sweep_rpt": rpt}
+    clear_tuning_classes()
+    try:
+        out = {name: _tune_knob(name, cands, measure)
+               for name, cands in (("sweep_rpt", rpt_candidates), ("bucket_fill", candidates),
+                                   ("bucket_cap", cap_candidates))}
+    finally:
+        install_cached(force=True)
+    out["best"] = out["bucket_fill"]["best"]
+    return out
 
 
 def apply_cached(num_nodes: int, level: int = 2) -> int | None:
-    """Sets the cached best knobs for circuits of this size, if any; returns
-    the bucket fill (None when nothing is cached)."""
-    best = _load_cache().get(f"L{level}:{size_class(num_nodes)}")
-    if best is None:
-        return None
-    if isinstance(best, int):  # older cache entries hold the fill only
-        best = {"bucket_fill": best}
-    set_tuning("bucket_fill", int(best["bucket_fill"]))
-    if "sweep_rpt" in best:
-        set_tuning("sweep_rpt", int(best["sweep_rpt"]))
-    return int(best["bucket_fill"])
+    """Installs the cached classes (see install_cached) and returns the
+    tuned bucket fill of circuits of this size (None when none is cached)."""
+    install_cached()
+    v = get_tuning_class(level, size_class(num_nodes), "bucket_fill")
+    return v or None

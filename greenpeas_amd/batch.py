@@ -16,7 +16,7 @@ import ctypes
 
 import numpy as np
 
-from . import _native
+from . import _native, autotune
 from .dem import DetectorErrorModel
 from .error_analysis import CorrelationLevel, _read_dem, circuit_text
 
@@ -76,6 +76,7 @@ class BatchCompiler:
 
     def __init__(self, correlation_level: CorrelationLevel = CorrelationLevel.L2, device: int = 0):
         self._lib = _native.load()
+        autotune.install_cached()  # tuned knobs per size class, once per process
         self._h = ctypes.c_void_p()
         _native.check(self._lib.gp_batch_create(int(correlation_level), device, ctypes.byref(self._h)))
         self.count = 0
@@ -155,10 +156,12 @@ class BatchCompiler:
     def layout_stats(self) -> dict:
         """{"s_words": words of the sensitivity matrix, "band_relayouts":
         compiles that found a column band below its stored layers and ran
-        again with more} (gp_batch_layout_stats)."""
-        st = (ctypes.c_uint64 * 2)()
+        again with more, "buckets": sort buckets, "bucket_cap": slots per
+        bucket} (gp_batch_layout_stats)."""
+        st = (ctypes.c_uint64 * 4)()
         _native.check(self._lib.gp_batch_layout_stats(self._h, st))
-        return {"s_words": int(st[0]), "band_relayouts": int(st[1])}
+        return {"s_words": int(st[0]), "band_relayouts": int(st[1]), "buckets": int(st[2]),
+                "bucket_cap": int(st[3])}
 
     def device_bytes(self) -> int:
         b = ctypes.c_uint64()

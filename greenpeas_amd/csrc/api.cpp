@@ -774,12 +774,35 @@ int gp_batch_device_bytes(const gp_batch *batch, uint64_t *bytes) {
   });
 }
 
-int gp_batch_layout_stats(const gp_batch *batch, uint64_t stats[2]) {
+int gp_batch_layout_stats(const gp_batch *batch, uint64_t stats[4]) {
   return guarded([&] {
     if (!batch || !stats) throw InvalidArgument("NULL argument");
     stats[0] = batch->pipe->sWords();
     stats[1] = batch->pipe->bandRelayouts();
+    stats[2] = batch->pipe->numBuckets();
+    stats[3] = batch->pipe->bucketCap();
   });
+}
+
+int gp_set_tuning_class(int correlation_level, uint32_t size_class, const char *name, uint32_t value) {
+  return guarded([&] {
+    checkLevel(correlation_level);
+    if (!name) throw InvalidArgument("name is NULL");
+    gpa::setClassTuning(correlation_level, size_class, name, value);
+  });
+}
+
+int gp_get_tuning_class(int correlation_level, uint32_t size_class, const char *name, uint32_t *value) {
+  return guarded([&] {
+    checkLevel(correlation_level);
+    if (!name || !value) throw InvalidArgument("NULL argument");
+    if (size_class > 63) throw InvalidArgument("size class must be at most 63");
+    *value = gpa::classTuning(name, correlation_level, uint64_t{1} << size_class, 0u);
+  });
+}
+
+int gp_clear_tuning_classes(void) {
+  return guarded([&] { gpa::clearClassTuning(); });
 }
 
 int gp_circuit_summary(const char *circuit_text, uint64_t summary[4]) {
@@ -942,6 +965,7 @@ int gp_set_tuning(const char *name, uint32_t value) {
     else if (std::string(name) == "prob_table") gpa::setProbTable(value != 0);
     else if (std::string(name) == "packed_sort") gpa::setPackedSort(value != 0);
     else if (std::string(name) == "s_bands") gpa::setSBands(value);
+    else if (std::string(name) == "bucket_cap") gpa::setBucketCap(value);
     else if (std::string(name) == "sweep_split") {
       if (value > 2) throw InvalidArgument("sweep_split must be 0, 1 or 2");
       gpa::setSweepSplit(value);
@@ -960,6 +984,7 @@ int gp_get_tuning(const char *name, uint32_t *value) {
     else if (std::string(name) == "prob_table") *value = gpa::probTable() ? 1u : 0u;
     else if (std::string(name) == "packed_sort") *value = gpa::packedSort() ? 1u : 0u;
     else if (std::string(name) == "s_bands") *value = gpa::sBands();
+    else if (std::string(name) == "bucket_cap") *value = gpa::bucketCap();
     else if (std::string(name) == "sweep_split") *value = gpa::sweepSplit();
     else throw InvalidArgument(std::string("unknown tuning parameter: ") + name);
   });

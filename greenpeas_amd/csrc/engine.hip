@@ -38,6 +38,7 @@ constexpr int kScanBlock = 1024;
 constexpr int kScanTile = 4 * kScanBlock;
 constexpr int kBucketItemsPerWave = 256;  // wave-register sort capacity (4 per lane)
 std::atomic<uint32_t> gBucketFill{64};    // mean items per bucket the digit width aims at
+std::atomic<uint32_t> gBucketCap{256};    // slots per sort bucket (64, 128 or 256)
 std::atomic<uint32_t> gStash{1};          // target rows on (1) or off (0)
 std::atomic<uint32_t> gMaskTable{1};      // classify's LDS mask tables allowed (1) or not (0)
 std::atomic<uint32_t> gSweepSplit{1};     // long columns swept in time chunks (1; 2: always) or whole (0)
@@ -50,8 +51,11 @@ struct VDesc {
   uint32_t numObservables, numRanks, numSeeds, bucketBits;
   uint32_t bucketBase, numBuckets, maskWords, layerStride;
   uint32_t sweepThreads, sweepRpt, rowSpan, probDict;  // rowSpan: see StepgImage
-  uint32_t numPairs, pad0, pad1, pad2;
+  uint32_t numPairs;
+  uint32_t colRows;  // rows of every column's region of S (0: regions differ, see colAdj)
+  uint32_t pad1, pad2;
   uint64_t actOff, seedOff, colOff, rankOff;
+  uint64_t sOff;       // the circuit's first word of S (its columns' regions follow)
   uint64_t spillOff;   // first spill entry of the circuit's ranks
   uint64_t probOff;    // first probability (table entry if probDict, else the ranks' own)
 };
@@ -86,8 +90,10 @@ constexpr uint32_t kFlagBandOverflow = kFlagWords - 1;
 // its band can reach) at a region of its own, then two rows that the
 // sweep's stores for layers below base go to (nothing reads them):
 // colRegion = {region offset (words) low, high, base (even), rows stored
-// before those two}; colAdj = region offset - base * layerStride, so the
-// word of layer l >= base, position j is S[colAdj + l * layerStride + j].
+// before those two}; colAdj = (region offset - the circuit's sOff) /
+// layerStride - base (rows), so the word of layer l >= base, position j is
+// S[sOff + (colAdj + l) * layerStride + j].  A circuit's S stays below 2^32
+// words, so those offsets are 32-bit.
 
 // --------------------------------------------------------------------------
 // wave64 helpers
@@ -236,6 +242,54 @@ struct RunCarry {
   uint32_t runs;
   uint32_t nnzSum;
   uint32_t havePrev;
+  uint32_t openStart;  // bucket position of the first member of the run still open
+};
+
+// The XOR-merge of a run's members a(0..L-1) (in sorted order) in the
+// association of a segmented Hillis-Steele scan over the whole run:
+// V(j, 0) = a(j), V(j, k) = xmerge(V(j - 2^(k-1), k - 1), V(j, k - 1)) where
+// j >= 2^(k-1) (else V(j, k - 1)); the run's value is V(L - 1, K), 2^K >= L.
+// A wave's 64-lane scan gives exactly this to every run that lies within
+// one row of 64 sorted positions; a run across rows of at most 64 members
+// gets it from a second scan of its members alone (scanRuns), a longer one
+// from the evaluation below, so a class's probability does not depend on
+// where its run falls in its bucket (nor on the bucket layout).  Counted
+// from the run's end (q = L - 1 - j), the blocks are the aligned blocks of
+// a binary tree, each the merge of its upper half (earlier members) into
+// its lower half: built bottom-up like a binary counter, with one pending
+// block per level in `slots`.
+template <typename A>
+__device__ __forceinline__ double hsRunMerge(const A &a, uint32_t first, uint32_t L) {
+  double *slots = a.slots;
+  for (uint32_t q = 0; q < L; ++q) {
+    double v = a.get(first + L - 1 - q);
+    uint32_t k = 0;
+    for (; (q >> k) & 1u; ++k) v = xmerge(v, slots[k]);
+    slots[k] = v;
+  }
+  double acc = 0.0;
+  bool have = false;
+  for (uint32_t k = 0; k < 32 && (L >> k) != 0; ++k)
+    if ((L >> k) & 1u) {
+      acc = have ? xmerge(acc, slots[k]) : slots[k];
+      have = true;
+    }
+  return acc;
+}
+
+// Members' values by bucket position: a wave's sorted values in LDS, or
+// read from the node records of the bucket's sorted ranks; `slots`: 32
+// doubles of LDS for hsRunMerge.
+struct LdsRunValues {
+  const double *p;
+  double *slots;
+  __device__ double get(uint32_t i) const { return p[i]; }
+};
+struct NodeRunValues {
+  const uint32_t *rank;
+  const uint4 *info;
+  double *slots;
+  __device__ double get(uint32_t i) const;
 };
 
 // Bucket slot written by classify, one per (node, bucket slot), 16 bytes:
@@ -258,6 +312,8 @@ __device__ __forceinline__ double infoProb(const uint4 &x) {
   return __longlong_as_double(static_cast<long long>((static_cast<uint64_t>(x.y) << 32) | x.x));
 }
 
+__device__ double NodeRunValues::get(uint32_t i) const { return infoProb(info[rank[i]]); }
+
 struct RunOut {
   uint32_t *sortedRank;    // bucket-sorted ranks (in place over the bucket slots)
   uint32_t *runRank;       // per run: representative (smallest) rank
@@ -273,11 +329,11 @@ struct RunOut {
 // `cy`.  Runs are maximal sequences of equal hash; each run's probability is
 // the XOR-merge of its members, its representative the first member.  Each
 // element comes with its probability, second hash and class size.
-template <int R>
+template <int R, typename MV>
 __device__ __forceinline__ void scanRuns(const uint64_t (&h)[R], const uint32_t (&rk)[R], const double (&pv)[R],
                                          const uint32_t (&h2v)[R], const uint32_t (&nzv)[R], uint32_t cnt,
                                          uint32_t chunkBase, uint32_t bs, RunCarry &cy, uint32_t lane,
-                                         const RunOut &o) {
+                                         const RunOut &o, const MV &members) {
   bool f[R];
 #pragma unroll
   for (int s = 0; s < R; ++s) {
@@ -314,11 +370,39 @@ __device__ __forceinline__ void scanRuns(const uint64_t (&h)[R], const uint32_t 
         fl |= pf;
       }
     }
-    if (!fl) {  // the run began before this slot
+    const bool across = !fl;  // the run began before this row
+    if (across) {
       v = xmerge(cy.p, v);
       hf = cy.h2;
     }
     const uint64_t bal = __ballot(f[s]);
+    // A run begun in an earlier row: its merge in the association of one
+    // scan over the run (its members right-aligned in a row of their own;
+    // longer runs by hsRunMerge), for the lane where it ends in this row
+    // (or, continuing, the row's last; a later row then writes it again).
+    double canon = 0.0;
+    if (!(bal & 1ull)) {
+      const uint32_t rowValid = cnt - static_cast<uint32_t>(s) * 64 < 64 ? cnt - static_cast<uint32_t>(s) * 64 : 64u;
+      const uint32_t endLane = bal ? static_cast<uint32_t>(__builtin_ctzll(bal)) - 1 : rowValid - 1;
+      const uint32_t L = chunkBase + static_cast<uint32_t>(s) * 64 + endLane - cy.openStart + 1;
+      if (L <= 64) {
+        const uint32_t first = 64 - L;  // lane of the run's first member
+        double w = lane >= first ? members.get(cy.openStart + lane - first) : 0.0;
+        int wf = lane == first ? 1 : 0;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+          const double pw = __shfl_up(w, d);
+          const int pf = __shfl_up(wf, d);
+          if (lane >= static_cast<uint32_t>(d)) {
+            if (!wf) w = xmerge(pw, w);
+            wf |= pf;
+          }
+        }
+        canon = __shfl(w, 63);
+      } else {
+        canon = hsRunMerge(members, cy.openStart, L);
+      }
+    }
     const uint32_t before = __popcll(bal & lanemaskLe(lane));
     const uint32_t runIdx = cy.runs + before - 1;
     const uint32_t incl = waveInclusiveSum(nz, lane);
@@ -335,10 +419,13 @@ __device__ __forceinline__ void scanRuns(const uint64_t (&h)[R], const uint32_t 
         o.runMember[bs + runIdx] = chunkBase + e;
       }
       if (hf != ownH2) atomicOr(&o.flags[0], 1u);
-      if (e + 1 == cnt || nf) o.runP[bs + runIdx] = v;
+      if (e + 1 == cnt || nf) {
+        o.runP[bs + runIdx] = across ? canon : v;
+      }
     }
     cy.p = __shfl(v, 63);
     cy.h2 = __shfl(hf, 63);
+    if (bal) cy.openStart = chunkBase + static_cast<uint32_t>(s) * 64 + 63 - static_cast<uint32_t>(__clzll(bal));
     cy.runs += __popcll(bal);
     cy.nnzSum += __shfl(incl, 63);
   }
@@ -940,10 +1027,10 @@ __device__ __forceinline__ void buildMaskTable(uint32_t *tab, uint32_t l0, uint3
 // band are loaded before any is used, so a node waits for one memory round
 // trip per kBatch columns (16: one for nearly every node of a circuit with
 // ~15 columns per layer; 4 where nodes see few columns).
-template <int kBatch, typename F>
+template <int kBatch, bool UNIFORM, typename F>
 __device__ __forceinline__ void forEachTargetMasked(const uint32_t *mrow, uint32_t maskWords, const int2 *band,
-                                                    const int64_t *adj, const FrameRows &r,
-                                                    const uint32_t *__restrict__ S, F &&f) {
+                                                    const int32_t *adj, uint32_t P, uint32_t colWords,
+                                                    const FrameRows &r, const uint32_t *__restrict__ S0, F &&f) {
   if (!r.ok[0] || !maskWords) return;
   uint32_t w = 0;  // the window's low word
   uint64_t m = mrow[0] | (maskWords > 1 ? static_cast<uint64_t>(mrow[1]) << 32 : 0ull);
@@ -959,18 +1046,21 @@ __device__ __forceinline__ void forEachTargetMasked(const uint32_t *mrow, uint32
       m &= m - 1;
     }
     int2 bd[kBatch];
-    int64_t aj[kBatch];
+    uint32_t aj[kBatch];
 #pragma unroll
     for (int j = 0; j < kBatch; ++j) {
       bd[j] = col[j] != NONE ? band[col[j]] : make_int2(1, 0);
-      aj[j] = col[j] != NONE ? adj[col[j]] : 0;
+      if constexpr (UNIFORM)
+        aj[j] = col[j] != NONE ? col[j] * colWords : 0u;  // regions of one size, every layer stored
+      else
+        aj[j] = col[j] != NONE ? static_cast<uint32_t>(adj[col[j]]) * P : 0u;
     }
 #pragma unroll
     for (int j = 0; j < kBatch; ++j)
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int32_t l = static_cast<int32_t>(r.lay[q]);
-        x[j][q] = (r.ok[q] && bd[j].x <= l && l <= bd[j].y) ? S[aj[j] + r.idx[q]] : 0u;
+        x[j][q] = (r.ok[q] && bd[j].x <= l && l <= bd[j].y) ? S0[aj[j] + r.idx[q]] : 0u;
       }
 #pragma unroll
     for (int j = 0; j < kBatch; ++j) {
@@ -990,19 +1080,19 @@ __device__ __forceinline__ void forEachTargetMasked(const uint32_t *mrow, uint32
 // the few nodes without a stashed target row, and by classify when a
 // workgroup's mask table would not fit in LDS.
 template <typename F>
-__device__ __forceinline__ void forEachTargetScan(uint32_t numCols, const int2 *band, const int64_t *adj,
-                                                  const FrameRows &r, const uint32_t *__restrict__ S, F &&f) {
+__device__ __forceinline__ void forEachTargetScan(uint32_t numCols, const int2 *band, const int32_t *adj, uint32_t P,
+                                                  const FrameRows &r, const uint32_t *__restrict__ S0, F &&f) {
   if (!r.ok[0]) return;
   for (uint32_t c = 0; c < numCols; ++c) {
     const int2 bd = band[c];
     if (bd.y < static_cast<int32_t>(r.lo) || bd.x > static_cast<int32_t>(r.lo + r.span)) continue;
-    const int64_t a = adj[c];
+    const uint32_t a = static_cast<uint32_t>(adj[c]) * P;
     uint32_t x[4];
     // rows outside the band are not read (they may lie outside the region)
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int32_t l = static_cast<int32_t>(r.lay[q]);
-      x[q] = r.ok[q] && bd.x <= l && l <= bd.y ? S[a + r.idx[q]] : 0u;
+      x[q] = r.ok[q] && bd.x <= l && l <= bd.y ? S0[a + r.idx[q]] : 0u;
     }
     uint32_t v = bandXor(r, bd, x);
     while (v) {
@@ -1045,6 +1135,7 @@ struct ClassifyOut {
   uint32_t stashMaxCols;  // circuits with more columns get no rows (0: none do)
   uint32_t stashShift;    // a row holds 1 << stashShift words (2..4)
   uint32_t packedKey;     // 0: slots carry kLocalRankNone (the bucket stage sorts exactly)
+  uint32_t bucketCap;     // slots per bucket
 };
 
 // Ranks' inputs: compact frame rows, spill entries, probabilities.
@@ -1058,13 +1149,13 @@ struct RankIn {
 template <bool TABLE, int BATCH>
 __global__ __launch_bounds__(kClassBlock) void classifyKernel(
     const VDesc *__restrict__ desc, const uint4 *__restrict__ tiles, const int2 *__restrict__ band,
-    const int64_t *__restrict__ colAdj, const uint32_t *__restrict__ S, RankIn rin, ClassifyOut o) {
+    const int32_t *__restrict__ colAdj, const uint32_t *__restrict__ S, RankIn rin, ClassifyOut o) {
   // per wave: 64 rows of kStashWords words, padded to kStashWords + 1 so
   // that a lane's writes and the row-major read-out avoid bank conflicts
   __shared__ uint32_t rows[kClassBlock / 64][64 * (kStashWords + 1)];
   __shared__ uint32_t maskTab[TABLE ? kMaskTableWords : 1];
   __shared__ int2 bandTab[kBandTableCols];
-  __shared__ int64_t adjTab[kBandTableCols];
+  __shared__ int32_t adjTab[kBandTableCols];
   // tile: {circuit, first rank, lowest layer of its ranks' rows, layers
   // spanned by them}; the host picks TABLE only when every workgroup's
   // table fits kMaskTableWords
@@ -1078,11 +1169,11 @@ __global__ __launch_bounds__(kClassBlock) void classifyKernel(
   const double prob = rankProbability(d, rin.probVal, rin.probIdx, gi);
   const uint32_t l0 = t.z, nl = t.w;
   const int2 *bandC = band + d.colOff;
-  const int64_t *adjC = colAdj + d.colOff;
+  const int32_t *adjC = colAdj + d.colOff;
   if (d.numCols <= kBandTableCols) {
     for (uint32_t c = threadIdx.x; c < d.numCols; c += blockDim.x) {
       bandTab[c] = bandC[c];
-      adjTab[c] = adjC[c];
+      adjTab[c] = adjC[c];  // (read by the scan path in every layout)
     }
     bandC = bandTab;  // filled by the barrier below
     adjC = adjTab;
@@ -1113,10 +1204,16 @@ __global__ __launch_bounds__(kClassBlock) void classifyKernel(
       cnt++;
     };
     if constexpr (TABLE)
-      forEachTargetMasked<BATCH>(maskTab + (fr.span * nl + (fr.lo - l0)) * d.maskWords, d.maskWords, bandC, adjC,
-                                 fr, S, take);
+    {
+      const uint32_t *mrow = maskTab + (fr.span * nl + (fr.lo - l0)) * d.maskWords;
+      if (d.colRows)
+        forEachTargetMasked<BATCH, true>(mrow, d.maskWords, bandC, adjC, d.layerStride, d.colRows * d.layerStride,
+                                         fr, S + d.sOff, take);
+      else
+        forEachTargetMasked<BATCH, false>(mrow, d.maskWords, bandC, adjC, d.layerStride, 0u, fr, S + d.sOff, take);
+    }
     else
-      forEachTargetScan(d.numCols, bandC, adjC, fr, S, take);
+      forEachTargetScan(d.numCols, bandC, adjC, d.layerStride, fr, S + d.sOff, take);
     if (cnt && cnt <= 32)
       for (uint32_t k = cnt; k < 32; ++k) h = (h ^ 0xFFFFFFFFull) * kFnvPrime;
     h2 = mix2(h2, cnt);
@@ -1149,8 +1246,8 @@ __global__ __launch_bounds__(kClassBlock) void classifyKernel(
     const uint32_t local = o.packedKey && d.numRanks < kLocalRankNone ? i : kLocalRankNone;
     const uint4 slot = make_uint4(static_cast<uint32_t>(h), static_cast<uint32_t>(h >> 32),
                                   static_cast<uint32_t>(gi), d.bucketBits << kLocalRankBits | local);
-    if (pos < kBucketCap) {
-      o.pslot[static_cast<uint64_t>(bkt) * kBucketCap + pos] = slot;
+    if (pos < o.bucketCap) {
+      o.pslot[static_cast<uint64_t>(bkt) * o.bucketCap + pos] = slot;
     } else {
       const uint32_t k = atomicAdd(&o.flags[3], 1u);
       o.ovSlot[k] = slot;
@@ -1283,8 +1380,21 @@ __device__ void bucketWave(uint32_t b, uint32_t bs, uint32_t cnt, uint32_t lane,
       nzv[s] = 0u;
     }
   }
-  RunCarry cy{0.0, 0ull, 0u, 0u, 0u, 0u};
-  scanRuns<R>(h, r, pv, h2v, nzv, cnt, 0u, bs, cy, lane, o);
+  // the members' values by sorted position over the wave's LDS records
+  // (read for runs across rows of 64; every record read is done), then
+  // hsRunMerge's slots
+  double *vals = reinterpret_cast<double *>(pay);
+  if constexpr (R > 1) {
+    waveLdsSync();
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+      const uint32_t e = static_cast<uint32_t>(s) * 64 + lane;
+      if (e < cnt) vals[e] = pv[s];
+    }
+    waveLdsSync();
+  }
+  RunCarry cy{0.0, 0ull, 0u, 0u, 0u, 0u, 0u};
+  scanRuns<R>(h, r, pv, h2v, nzv, cnt, 0u, bs, cy, lane, o, LdsRunValues{vals, vals + 64 * R});
   if (lane == 0) {
     runCount[b] = cy.runs;
     runNnz[b] = cy.nnzSum;
@@ -1305,7 +1415,8 @@ struct BucketIo {
   uint32_t *wgSum;        // per workgroup: its buckets' runs, targets, items
   uint64_t *scratchKey;   // sort workspace for oversized buckets (2 * arenaSize)
   uint32_t *scratchRank;
-  uint32_t arenaBase;     // numBuckets * kBucketCap: oversized buckets' items and runs follow
+  uint32_t arenaBase;     // numBuckets * cap: oversized buckets' items and runs follow
+  uint32_t cap;           // slots per bucket (at most kBucketCap)
 };
 
 // An oversized bucket (more than kBucketCap items), by the whole workgroup:
@@ -1315,10 +1426,11 @@ struct BucketIo {
 // arena range taken with one atomic.
 __device__ void bigBucket(uint32_t b, uint32_t cnt, const BucketIo &io, const RunOut &o, uint32_t *tot) {
   __shared__ uint32_t shOff, shFill;
+  __shared__ double hsSlots[32];
   const uint32_t lane = threadIdx.x & 63;
   if (threadIdx.x == 0) {
     shOff = atomicAdd(&o.flags[5], cnt);  // total items never exceed the arena
-    shFill = kBucketCap;
+    shFill = io.cap;
   }
   __syncthreads();
   const uint32_t off = shOff;
@@ -1326,8 +1438,8 @@ __device__ void bigBucket(uint32_t b, uint32_t cnt, const BucketIo &io, const Ru
   while (P < cnt) P <<= 1;
   uint64_t *K = io.scratchKey + 2ull * off;  // P < 2 * cnt
   uint32_t *Rk = io.scratchRank + 2ull * off;
-  const uint32_t slot0 = b * kBucketCap;
-  for (uint32_t i = threadIdx.x; i < kBucketCap; i += blockDim.x) {
+  const uint32_t slot0 = b * io.cap;
+  for (uint32_t i = threadIdx.x; i < io.cap; i += blockDim.x) {
     const uint4 x = io.pslot[slot0 + i];
     K[i] = (static_cast<uint64_t>(x.y) << 32) | x.x;
     Rk[i] = x.z;
@@ -1366,7 +1478,9 @@ __device__ void bigBucket(uint32_t b, uint32_t cnt, const BucketIo &io, const Ru
     }
   const uint32_t base = io.arenaBase + off;
   if (threadIdx.x < 64) {
-    RunCarry cy{0.0, 0ull, 0u, 0u, 0u, 0u};
+    RunCarry cy{0.0, 0ull, 0u, 0u, 0u, 0u, 0u};
+    // members' values for runs across chunks: from the sorted ranks' records
+    const NodeRunValues members{Rk, io.info, hsSlots};
     for (uint32_t c0 = 0; c0 < cnt; c0 += 64) {
       uint64_t h[1];
       uint32_t r[1], h2v[1] = {0u}, nzv[1] = {0u};
@@ -1381,7 +1495,7 @@ __device__ void bigBucket(uint32_t b, uint32_t cnt, const BucketIo &io, const Ru
         nzv[0] = x.w;
       }
       const uint32_t n = cnt - c0 < 64 ? cnt - c0 : 64;
-      scanRuns<1>(h, r, pv, h2v, nzv, n, c0, base, cy, lane, o);
+      scanRuns<1>(h, r, pv, h2v, nzv, n, c0, base, cy, lane, o, members);
     }
     if (lane == 0) {
       io.runCount[b] = cy.runs;
@@ -1407,7 +1521,7 @@ __global__ __launch_bounds__(256) void bucketKernel(uint32_t numBuckets, BucketI
   if (lane < 3) tot[wave][lane] = 0u;
   __syncthreads();
   if (b < numBuckets) {
-    const uint32_t bs = b * kBucketCap;
+    const uint32_t bs = b * io.cap;
     // the bucket's first 64 slots travel with its counter (slots past the
     // count hold stale records and are never used)
     const uint4 first = io.pslot[bs + lane];
@@ -1418,7 +1532,7 @@ __global__ __launch_bounds__(256) void bucketKernel(uint32_t numBuckets, BucketI
       io.itemCount[b] = cnt;
       tot[wave][2] = cnt;
     }
-    if (cnt <= kBucketCap) {
+    if (cnt <= io.cap) {
       if (lane == 0) io.slotBase[b] = bs;
       if (cnt == 0) {
         if (lane == 0) {
@@ -1475,7 +1589,8 @@ struct EmitIn {
   uint32_t stashMaxCols;
   uint32_t stashShift;  // row words = 1 << stashShift
   const int2 *band;
-  const int64_t *colAdj;
+  const int32_t *colAdj;
+  uint32_t bucketCap;  // slots per bucket
 };
 struct EmitOut {
   double *p;
@@ -1511,7 +1626,7 @@ __global__ __launch_bounds__(256) void emitKernel(const VDesc *__restrict__ desc
     qItems = in.itemCount[g * 4 + lane];
     qBase = in.slotBase[g * 4 + lane];
   }
-  const uint32_t bs0 = b * kBucketCap;
+  const uint32_t bs0 = b * in.bucketCap;
   uint32_t sRank = 0, sPre = 0, sNext = 0, sMember = 0;
   double sP = 0.0;
   uint64_t sHash = 0;
@@ -1670,7 +1785,8 @@ __global__ __launch_bounds__(256) void emitKernel(const VDesc *__restrict__ desc
       pos += n;
     } else {
       const FrameRows fr = decodeRows(expandRankCode(in.rankCode[rk], in.spill + d.spillOff), d.layerStride);
-      forEachTargetScan(d.numCols, in.band + d.colOff, in.colAdj + d.colOff, fr, in.S, [&](uint32_t tg) {
+      forEachTargetScan(d.numCols, in.band + d.colOff, in.colAdj + d.colOff, d.layerStride, fr, in.S + d.sOff,
+                        [&](uint32_t tg) {
         put(pos, tg);
         pos++;
       });
@@ -2186,6 +2302,8 @@ struct Pipeline::Impl {
   std::vector<ColLayout> cols;  // every column of the load
   size_t oColRegion = 0, oColAdj = 0;
   uint32_t bandRelayouts = 0;  // loads' columns given more rows after a run
+  uint32_t bucketCap = kBucketCap;  // slots per sort bucket of the load
+  bool uniformCols = false;  // every column of a circuit stores all its layers in regions of one size
   // the arena's buffers, in the order workBufferSizes() sizes them
   std::array<DevBuf *, 30> workBufs() {
     return {&band,                    &info,        &stash,      &ovSlot,      &ovBucket,  &counters, &bucketCnt,
@@ -2286,11 +2404,24 @@ void Pipeline::setStream(void *stream) {
   m.stream = stream ? static_cast<hipStream_t>(stream) : m.ownStream;
 }
 
-static uint32_t bucketBitsFor(uint32_t numRanks) {
+// First radix digit width of a circuit: the smallest b with fill * 2^b >=
+// its error nodes (fill: its size class's tuned value, else bucket_fill).
+static uint32_t bucketBitsFor(const StepgImage &g) {
   uint32_t b = 0;
-  const uint64_t fill = gBucketFill.load(std::memory_order_relaxed);
-  while (b < 24 && (fill << b) < numRanks) b++;
+  const uint64_t nodes = static_cast<uint64_t>(g.numLayers) * g.npl;
+  const uint64_t fill = classTuning("bucket_fill", g.level, nodes, gBucketFill.load(std::memory_order_relaxed));
+  while (b < 24 && (fill << b) < g.numRanks) b++;
   return b;
+}
+
+// Slots per sort bucket of a load: the largest tuned value among its
+// circuits' size classes (bucket_cap where a class has none).
+static uint32_t bucketCapFor(const std::vector<const StepgImage *> &imgs) {
+  uint32_t cap = 0;
+  for (const StepgImage *g : imgs)
+    cap = std::max(cap, classTuning("bucket_cap", g->level, static_cast<uint64_t>(g->numLayers) * g->npl,
+                                    gBucketCap.load(std::memory_order_relaxed)));
+  return cap ? cap : kBucketCap;
 }
 
 void Pipeline::reserve(const std::vector<const StepgImage *> &imgs) {
@@ -2315,14 +2446,15 @@ static uint32_t bucketCountStride(uint64_t B) {
 }
 
 // Every work buffer of a load, in the arena's order, by its totals: R error
-// nodes, B sort buckets, V circuits, Ccols detector columns.  req(index,
-// bytes, what) is called once per buffer.
+// nodes, B sort buckets of `cap` slots, V circuits, Ccols detector columns.
+// req(index, bytes, what) is called once per buffer.
 constexpr uint32_t kWorkBuffers = 30;
 template <typename F>
-static void workBufferSizes(uint64_t R, uint64_t B, uint64_t V, uint64_t Ccols, uint32_t countStride, F &&req) {
+static void workBufferSizes(uint64_t R, uint64_t B, uint64_t V, uint64_t Ccols, uint32_t countStride, uint32_t cap,
+                            F &&req) {
   R = std::max<uint64_t>(R, 1);
   // bucket slots, then an arena of R entries for oversized buckets
-  const uint64_t slots = B * static_cast<uint64_t>(kBucketCap) + R;
+  const uint64_t slots = B * static_cast<uint64_t>(cap) + R;
   if (slots >= 0xFFFFFFFFull) throw RuntimeError("too many sort buckets in one load");
   uint32_t k = 0;
   req(k++, 8 * std::max<uint64_t>(Ccols, 1), "columns");                // band
@@ -2381,6 +2513,14 @@ static uint64_t regionWords(const Pipeline::Impl::ColLayout &c) {
   return c.top1 ? static_cast<uint64_t>(c.top1 - c.base + 2) * c.P : 0u;
 }
 
+// Rows of a circuit's regions when they are all of one size (frozen
+// pipelines): its highest column's stored layers and the two more.
+static uint32_t uniformColRows(const StepgImage &g) {
+  uint32_t top1 = 0;
+  for (uint32_t c = 0; c < g.numCols; ++c) top1 = std::max(top1, columnLayout(g, c, false).top1);
+  return top1 + 2;
+}
+
 // Bytes DevBuf::ensure allocates for `bytes` (with `head` more when it grows).
 static size_t grownBytes(size_t bytes, double head) {
   return roundGrain(std::max<size_t>(static_cast<size_t>(static_cast<double>(bytes) * head), 256));
@@ -2391,14 +2531,28 @@ static size_t grownBytes(size_t bytes, double head) {
 // a frozen pipeline throws instead).
 static void layoutS(Pipeline::Impl &m, bool frozen, double head, bool device) {
   uint4 *reg = reinterpret_cast<uint4 *>(m.inHost.as<char>() + m.oColRegion);
-  int64_t *adj = reinterpret_cast<int64_t *>(m.inHost.as<char>() + m.oColAdj);
+  int32_t *adj = reinterpret_cast<int32_t *>(m.inHost.as<char>() + m.oColAdj);
+  VDesc *hd = reinterpret_cast<VDesc *>(m.inHost.as<char>() + m.oDesc);
   uint64_t off = 0;
-  for (size_t k = 0; k < m.cols.size(); ++k) {
-    const Pipeline::Impl::ColLayout &c = m.cols[k];
-    reg[k] = make_uint4(static_cast<uint32_t>(off), static_cast<uint32_t>(off >> 32), c.base,
-                        c.top1 ? c.top1 - c.base : 0u);
-    adj[k] = static_cast<int64_t>(off) - static_cast<int64_t>(c.base) * c.P;
-    off += regionWords(c);
+  for (size_t v = 0; v < m.descs.size(); ++v) {
+    VDesc &d = m.descs[v];
+    d.sOff = hd[v].sOff = off;
+    // frozen pipelines: one region size per circuit (classify then needs no
+    // per-column offsets)
+    uint32_t rows = 0;
+    if (m.uniformCols)
+      for (uint32_t c = 0; c < d.numCols; ++c) rows = std::max(rows, m.cols[d.colOff + c].top1 + 2);
+    d.colRows = hd[v].colRows = rows;
+    for (uint32_t c = 0; c < d.numCols; ++c) {
+      const uint64_t k = d.colOff + c;
+      const Pipeline::Impl::ColLayout &cl = m.cols[k];
+      reg[k] = make_uint4(static_cast<uint32_t>(off), static_cast<uint32_t>(off >> 32), cl.base,
+                          rows ? rows - 2 : cl.top1 ? cl.top1 - cl.base : 0u);
+      adj[k] = static_cast<int32_t>((off - d.sOff) / cl.P) - static_cast<int32_t>(cl.base);
+      off += rows ? static_cast<uint64_t>(rows) * cl.P : regionWords(cl);
+    }
+    if (off - d.sOff >= (uint64_t{1} << 31))
+      throw RuntimeError("circuit too large: its sensitivity matrix exceeds 2^31 words");
   }
   m.Swords = off;
   const size_t bytes = 4 * std::max<uint64_t>(off, 1);
@@ -2410,7 +2564,8 @@ static void layoutS(Pipeline::Impl &m, bool frozen, double head, bool device) {
   }
   if (device && !m.cols.empty()) {
     GPA_HIP(hipMemcpy(m.in.as<char>() + m.oColRegion, reg, 16 * m.cols.size(), hipMemcpyHostToDevice));
-    GPA_HIP(hipMemcpy(m.in.as<char>() + m.oColAdj, adj, 8 * m.cols.size(), hipMemcpyHostToDevice));
+    GPA_HIP(hipMemcpy(m.in.as<char>() + m.oColAdj, adj, 4 * m.cols.size(), hipMemcpyHostToDevice));
+    GPA_HIP(hipMemcpy(m.in.as<char>() + m.oDesc, hd, sizeof(VDesc) * m.descs.size(), hipMemcpyHostToDevice));
   }
 }
 
@@ -2442,6 +2597,7 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   bool classTable = true;
   std::vector<std::pair<int32_t, uint32_t>> order;  // (band estimate, item index)
   m.cols.clear();
+  m.uniformCols = !growable_;
   for (uint32_t v = 0; v < V; ++v) {
     const StepgImage &g = *imgs[v];
     VDesc &d = m.descs[v];
@@ -2452,7 +2608,7 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
     d.numObservables = g.numObservables;
     d.numRanks = g.numRanks;
     d.numSeeds = g.numSeeds;
-    d.bucketBits = bucketBitsFor(g.numRanks);
+    d.bucketBits = bucketBitsFor(g);
     d.numBuckets = 1u << d.bucketBits;
     d.bucketBase = bucketBase;
     d.maskWords = (g.numCols + 31) / 32;
@@ -2538,7 +2694,7 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   m.oItems = place(sizeof(uint4) * m.nItems);
   m.oTiles = place(sizeof(uint4) * m.nTiles);
   m.oColRegion = place(16 * m.Ccols);
-  m.oColAdj = place(8 * m.Ccols);
+  m.oColAdj = place(4 * m.Ccols);
   m.oAct = place(16 * actOff);
   m.oSeed = place(4 * seedOff);
   m.oRankCode = place(8 * rankOff);
@@ -2575,7 +2731,9 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   m.countStride = bucketCountStride(B);
   const auto bufs = m.workBufs();
   static_assert(std::tuple_size<decltype(bufs)>::value == kWorkBuffers, "one DevBuf per work buffer");
-  workBufferSizes(R, B, V, m.Ccols, m.countStride,
+  // (a frozen pipeline keeps the slots per bucket it was sized with)
+  if (!frozen) m.bucketCap = bucketCapFor(imgs);
+  workBufferSizes(R, B, V, m.Ccols, m.countStride, m.bucketCap,
                   [&](uint32_t k, uint64_t bytes, const char *what) { wa.request(*bufs[k], bytes, what); });
   wa.commit(frozen, growable_ ? kGrowHeadroom : 1.0);
   if (m.capTargets == 0 || (!frozen && m.capTargets < 16 * R)) {
@@ -2610,6 +2768,7 @@ namespace {
 // The totals a load forms from its circuits (see load()).
 struct LoadTotals {
   uint64_t R = 0, B = 0, V = 0, Ccols = 0, Swords = 0, inBytes = 0;
+  uint32_t cap = kBucketCap;  // slots per bucket
   uint32_t maxP = 64;  // widest layer stride
 };
 
@@ -2617,7 +2776,7 @@ LoadTotals loadTotals(const std::vector<const StepgImage *> &imgs, bool banded) 
   LoadTotals t;
   uint64_t actOff = 0, seedOff = 0, spill = 0, probOff = 0, nTiles = 0;
   for (const StepgImage *g : imgs) {
-    t.B += uint64_t{1} << bucketBitsFor(g->numRanks);
+    t.B += uint64_t{1} << bucketBitsFor(*g);
     nTiles += (g->numRanks + kClassBlock - 1) / kClassBlock;
     actOff += static_cast<uint64_t>(kSweepLeadPad + g->numPairs) * (g->layerStride + 1);
     seedOff += static_cast<uint64_t>(g->numCols) * g->numSeeds;
@@ -2625,13 +2784,17 @@ LoadTotals loadTotals(const std::vector<const StepgImage *> &imgs, bool banded) 
     t.R += g->numRanks;
     spill += g->rankSpill.size() / 4;
     probOff += g->probTab.empty() ? g->numRanks : g->probTab.size();
-    for (uint32_t c = 0; c < g->numCols; ++c) t.Swords += regionWords(columnLayout(*g, c, banded));
+    if (banded)
+      for (uint32_t c = 0; c < g->numCols; ++c) t.Swords += regionWords(columnLayout(*g, c, true));
+    else
+      t.Swords += static_cast<uint64_t>(g->numCols) * uniformColRows(*g) * g->layerStride;
     t.maxP = std::max(t.maxP, g->layerStride);
   }
   t.V = imgs.size();
+  t.cap = bucketCapFor(imgs);
   // the input arena: each array at a 256-byte boundary
   t.inBytes = align256(sizeof(VDesc) * t.V) + align256(16 * t.Ccols) + align256(16 * nTiles) +
-              align256(16 * t.Ccols) + align256(8 * t.Ccols) + align256(16 * actOff) +
+              align256(16 * t.Ccols) + align256(4 * t.Ccols) + align256(16 * actOff) +
               align256(4 * seedOff) + align256(8 * t.R) + align256(16 * spill) + align256(2 * t.R) +
               align256(8 * probOff);
   return t;
@@ -2660,7 +2823,7 @@ uint64_t footprint(const LoadTotals &t, const HeldCaps &h, double headroom) {
   const uint64_t R = std::max<uint64_t>(t.R, 1);
   std::vector<size_t> caps;
   bool fits = h.arena;
-  workBufferSizes(R, t.B, t.V, t.Ccols, bucketCountStride(t.B), [&](uint32_t k, uint64_t bytes, const char *) {
+  workBufferSizes(R, t.B, t.V, t.Ccols, bucketCountStride(t.B), t.cap, [&](uint32_t k, uint64_t bytes, const char *) {
     const size_t b = std::max<size_t>(bytes, 256), cap = h.work[k];
     const size_t grown = h.arena && b > cap ? static_cast<size_t>(b * headroom) : 0;
     if (b > cap) fits = false;
@@ -2740,7 +2903,7 @@ void Pipeline::run(bool timed) {
   const uint4 *act = reinterpret_cast<const uint4 *>(ib + m.oAct);
   const uint32_t *seed = reinterpret_cast<const uint32_t *>(ib + m.oSeed);
   const uint4 *colRegion = reinterpret_cast<const uint4 *>(ib + m.oColRegion);
-  const int64_t *colAdj = reinterpret_cast<const int64_t *>(ib + m.oColAdj);
+  const int32_t *colAdj = reinterpret_cast<const int32_t *>(ib + m.oColAdj);
   const RankIn rin{reinterpret_cast<const uint2 *>(ib + m.oRankCode), reinterpret_cast<const uint4 *>(ib + m.oSpill),
                    reinterpret_cast<const double *>(ib + m.oProbVal),
                    reinterpret_cast<const uint16_t *>(ib + m.oProbIdx)};
@@ -2792,7 +2955,7 @@ void Pipeline::run(bool timed) {
     ClassifyOut co{m.info.as<uint4>(),     m.bucketCount(),          m.countStride,
                    m.pslot.as<uint4>(),    m.ovSlot.as<uint4>(),     m.ovBucket.as<uint32_t>(),
                    m.flags(),              m.stash.as<uint32_t>(),   stashMaxCols,
-                   m.stashShift,           gPackedSort.load()};
+                   m.stashShift,           gPackedSort.load(),       m.bucketCap};
     if (m.classTable && gMaskTable.load() && m.classBatch == 16)
       classifyKernel<true, 16><<<m.nTiles, kClassBlock, 0, st>>>(desc, tiles, m.band.as<int2>(), colAdj,
                                                                  m.S.as<uint32_t>(), rin, co);
@@ -2818,7 +2981,7 @@ void Pipeline::run(bool timed) {
                m.runNnz.as<uint32_t>(),     m.itemCount.as<uint32_t>(),
                m.slotBase.as<uint32_t>(),   m.wgSum.as<uint32_t>(),
                m.scratchKey.as<uint64_t>(), m.scratchRank.as<uint32_t>(),
-               B * kBucketCap};
+               B * m.bucketCap, m.bucketCap};
   if (B) bucketKernel<<<(B + 3) / 4, 256, 0, st>>>(B, bio, o);
   mark(3);
   // bucket offsets: inside emit from the workgroup totals, or by the scans
@@ -2834,7 +2997,8 @@ void Pipeline::run(bool timed) {
              m.tgtStart.as<uint32_t>(), m.memberStart.as<uint32_t>(), m.runRank.as<uint32_t>(),
              m.runTgtPre.as<uint32_t>(), m.runMember.as<uint32_t>(), m.runP.as<double>(),
              m.runHash.as<uint64_t>(), m.runNnz.as<uint32_t>(), m.S.as<uint32_t>(),
-             m.stash.as<uint32_t>(), rin.code, rin.spill, stashMaxCols, m.stashShift, m.band.as<int2>(), colAdj};
+             m.stash.as<uint32_t>(), rin.code, rin.spill, stashMaxCols, m.stashShift, m.band.as<int2>(), colAdj,
+             m.bucketCap};
   EmitOut eout{m.outP.as<double>(), m.outOff.as<uint32_t>(), m.outHash.as<uint64_t>(), m.outMember.as<uint32_t>(),
                m.outTargets.as<uint32_t>(), m.targets16 ? m.outTargets.as<uint16_t>() : nullptr, m.capTargets,
                m.flags(), m.vBounds.as<uint32_t>()};
@@ -3291,6 +3455,13 @@ void setSBands(uint32_t mode) {
 }
 uint32_t sBands() { return gSBands.load(); }
 uint32_t Pipeline::bandRelayouts() const { return impl_->bandRelayouts; }
+uint32_t Pipeline::bucketCap() const { return impl_->bucketCap; }
+uint32_t Pipeline::numBuckets() const { return impl_->B; }
+void setBucketCap(uint32_t cap) {
+  if (cap != 64 && cap != 128 && cap != 256) throw InvalidArgument("bucket_cap must be 64, 128 or 256");
+  gBucketCap.store(cap);
+}
+uint32_t bucketCap() { return gBucketCap.load(); }
 uint64_t Pipeline::sWords() const { return impl_->Swords; }
 bool packedSort() { return gPackedSort.load() != 0; }
 bool maskTable() { return gMaskTable.load() != 0; }
@@ -3329,7 +3500,7 @@ void Pipeline::debugSensitivities(uint32_t v, std::vector<uint32_t> &cols32) con
   std::vector<uint32_t> S(m.Swords);
   std::vector<int2> band(d.numCols);
   if (!S.empty()) GPA_HIP(hipMemcpy(S.data(), m.S.p, 4 * S.size(), hipMemcpyDeviceToHost));
-  const int64_t *adj = reinterpret_cast<const int64_t *>(m.inHost.as<char>() + m.oColAdj) + d.colOff;
+  const int32_t *adj = reinterpret_cast<const int32_t *>(m.inHost.as<char>() + m.oColAdj) + d.colOff;
   if (d.numCols)
     GPA_HIP(hipMemcpy(band.data(), m.band.as<int2>() + d.colOff, 8 * d.numCols, hipMemcpyDeviceToHost));
   // each rank's word = XOR of its in-band frame rows
@@ -3353,7 +3524,7 @@ void Pipeline::debugSensitivities(uint32_t v, std::vector<uint32_t> &cols32) con
         const int32_t l = static_cast<int32_t>(cd >> kRowPosBits);
         if (l < band[c].x || l > band[c].y) continue;
         const uint64_t row = static_cast<uint64_t>(l) * d.layerStride + (cd & ((1u << kRowPosBits) - 1));
-        w ^= S[static_cast<size_t>(adj[c] + static_cast<int64_t>(row))];
+        w ^= S[d.sOff + static_cast<uint32_t>(static_cast<uint32_t>(adj[c]) * d.layerStride + row)];
       }
       cols32[static_cast<size_t>(i) * d.numCols + c] = w;
     }

@@ -113,6 +113,9 @@ class Pipeline {
   // (and the load then stored more rows), and S's words now.
   uint32_t bandRelayouts() const;
   uint64_t sWords() const;
+  // Sort buckets of the load and slots per bucket.
+  uint32_t numBuckets() const;
+  uint32_t bucketCap() const;
   int device() const { return device_; }
   void *stream() const;
   // Queues later work on `stream` (a hipStream_t of the pipeline's device;
@@ -172,6 +175,13 @@ bool packedSort();
 // with more rows; for tests).  Same results in every mode.  Read at load.
 void setSBands(uint32_t mode);
 uint32_t sBands();
+// Slots per sort bucket (64, 128 or 256; default 256): classify writes a
+// bucket's first `cap` nodes to its slots (the rest to an overflow list
+// sorted by the workgroup); slot storage is numBuckets * cap + nodes.
+// Same results for every value.  Read at load (a size class's tuned value
+// first).
+void setBucketCap(uint32_t cap);
+uint32_t bucketCap();
 // Whether a load's long columns may be swept in time chunks after its first
 // run: 1 (default) where the estimated critical path gains, 2 every column
 // whose band reaches the bottom with the most chunks that fit (for tests), 0

@@ -4,7 +4,10 @@
 #include "workers.hpp"
 
 #include <algorithm>
+#include <array>
 #include <atomic>
+#include <map>
+#include <mutex>
 #include <cstring>
 #include <unordered_map>
 #include <cmath>
@@ -423,6 +426,55 @@ void setProbTable(bool on) { gProbTable.store(on ? 1u : 0u); }
 bool probTable() { return gProbTable.load() != 0; }
 
 namespace {
+// knob index: bucket_fill, bucket_cap, sweep_rpt
+int knobIndex(const char *name) {
+  const std::string n = name ? name : "";
+  if (n == "bucket_fill") return 0;
+  if (n == "bucket_cap") return 1;
+  if (n == "sweep_rpt") return 2;
+  throw InvalidArgument("no per-size-class tuning for knob " + n);
+}
+std::mutex gClassMu;
+std::map<uint32_t, std::array<uint32_t, 3>> gClassKnobs;  // level << 8 | size class -> values (0 = unset)
+}  // namespace
+
+uint32_t sizeClassOf(uint64_t numNodes) {
+  uint32_t c = 0;
+  while (c < 63 && (numNodes >> (c + 1)) != 0) ++c;
+  return c;
+}
+
+void setClassTuning(int level, uint32_t sizeClass, const char *name, uint32_t value) {
+  const int k = knobIndex(name);
+  nodesPerQubit(level);  // validates the level
+  if (sizeClass > 63) throw InvalidArgument("size class must be at most 63");
+  if (value) {
+    if (k == 0 && (value < 1 || value > 4096)) throw InvalidArgument("bucket_fill must be in 1..4096");
+    if (k == 1 && value != 64 && value != 128 && value != 256)
+      throw InvalidArgument("bucket_cap must be 64, 128 or 256");
+    if (k == 2 && value != 1 && value != 2 && value != 4 && value != 8)
+      throw InvalidArgument("sweep_rpt must be 1, 2, 4 or 8");
+  }
+  std::lock_guard<std::mutex> lk(gClassMu);
+  auto &e = gClassKnobs[static_cast<uint32_t>(level) << 8 | sizeClass];
+  e[k] = value;
+  if (!e[0] && !e[1] && !e[2]) gClassKnobs.erase(static_cast<uint32_t>(level) << 8 | sizeClass);
+}
+
+void clearClassTuning() {
+  std::lock_guard<std::mutex> lk(gClassMu);
+  gClassKnobs.clear();
+}
+
+uint32_t classTuning(const char *name, int level, uint64_t numNodes, uint32_t fallback) {
+  const int k = knobIndex(name);
+  std::lock_guard<std::mutex> lk(gClassMu);
+  if (gClassKnobs.empty()) return fallback;
+  auto it = gClassKnobs.find(static_cast<uint32_t>(level) << 8 | sizeClassOf(numNodes));
+  return it != gClassKnobs.end() && it->second[k] ? it->second[k] : fallback;
+}
+
+namespace {
 
 // Node-indexed scratch of buildImage, kept per thread between builds (a
 // bb12 image needs ~4 MB of it; fresh pages each time cost more than the
@@ -451,6 +503,7 @@ void buildImage(const Stepg &g, StepgImage &img, bool parallel) {
   img.numDetectors = g.numDetectors;
   img.numObservables = g.numObservables;
   img.numLocations = g.numLocations;
+  img.level = g.level;
   img.rowSpan = 0;
   ImageScratch &ws = tScratch;
   // work split over the worker pool (a small circuit takes one part)
@@ -642,7 +695,8 @@ void buildImage(const Stepg &g, StepgImage &img, bool parallel) {
   // sweep shape: the fewest records per thread (a power of two) that fit
   // the layer in one workgroup, then the fewest 64-lane waves
   uint32_t rpt = 1;
-  while (rpt < sweepMinRpt()) rpt <<= 1;
+  const uint32_t minRpt = classTuning("sweep_rpt", g.level, N, sweepMinRpt());
+  while (rpt < minRpt) rpt <<= 1;
   while (rpt < 8 && static_cast<uint64_t>(rpt) * kSweepMaxThreads < maxAct) rpt <<= 1;
   if (static_cast<uint64_t>(rpt) * kSweepMaxThreads < maxAct)
     throw RuntimeError("circuit has too many active STEPG nodes per layer for the LDS sweep (" +

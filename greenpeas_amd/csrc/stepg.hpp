@@ -110,6 +110,7 @@ void buildStepg(const Circuit &c, int level, Stepg &g, const double *theta = nul
 
 // Device-ready arrays of one circuit (see the file comment).
 struct StepgImage {
+  int level = 2;
   uint32_t numLayers = 0, npl = 0, numCols = 0, numDetectors = 0, numObservables = 0;
   uint32_t numRanks = 0, numSeeds = 0, numActive = 0, numLocations = 0, maxActive = 0;
   // sweep shape: record j*sweepThreads + t of a layer belongs to thread t
@@ -148,5 +149,15 @@ uint32_t sweepMinRpt();
 // parsed.
 void setProbTable(bool on);
 bool probTable();
+
+// Tuned knobs per size class: circuits at `level` whose STEPG has n nodes,
+// size class floor(log2 n), take `value` for knob `name` ("bucket_fill",
+// "bucket_cap" or "sweep_rpt"; value 0 removes the entry) instead of the
+// process-wide setting.  Read where the process-wide knob is read.
+void setClassTuning(int level, uint32_t sizeClass, const char *name, uint32_t value);
+void clearClassTuning();
+// The knob's value for such a circuit: its class entry, else `fallback`.
+uint32_t classTuning(const char *name, int level, uint64_t numNodes, uint32_t fallback);
+uint32_t sizeClassOf(uint64_t numNodes);
 
 }  // namespace gpa

@@ -26,7 +26,7 @@ import os
 
 import numpy as np
 
-from . import _native
+from . import _native, autotune
 from .dem import DetectorErrorModel
 
 
@@ -111,6 +111,7 @@ class Driver:
     def __init__(self, circuit, correlation_level: CorrelationLevel = CorrelationLevel.L2,
                  device: int = 0):
         self._lib = _native.load()
+        autotune.install_cached()  # tuned knobs per size class, once per process
         self._h = ctypes.c_void_p()
         self.correlation_level = CorrelationLevel(correlation_level)
         self.device = device

@@ -241,11 +241,12 @@ int gp_batch_dem_copy(const gp_batch *batch, uint32_t index, double *probs,
                       uint32_t *offsets, uint32_t *targets, uint64_t *hashes);
 /* Bytes of device memory the batch holds now. */
 int gp_batch_device_bytes(const gp_batch *batch, uint64_t *bytes);
-/* Layout of the batch's sensitivity matrix: stats[0] = its 32-bit words
- * (each detector column stores the layers its time band can reach; see
- * "s_bands"), stats[1] = times a compile found a band reaching below a
- * column's stored layers and compiled again with more. */
-int gp_batch_layout_stats(const gp_batch *batch, uint64_t stats[2]);
+/* Device layout of the batch's load: stats[0] = 32-bit words of its
+ * sensitivity matrix (each detector column stores the layers its time band
+ * can reach; see "s_bands"), stats[1] = times a compile found a band
+ * reaching below a column's stored layers and compiled again with more,
+ * stats[2] = sort buckets, stats[3] = slots per bucket. */
+int gp_batch_layout_stats(const gp_batch *batch, uint64_t stats[4]);
 /* Sizes read off circuit text in one pass, without parsing or building
  * anything (no counterpart in the reference; for memory admission of
  * queued requests): summary = {layers (TICKs + 1, REPEAT bodies counted per
@@ -320,6 +321,12 @@ int gp_device_memory(int device, uint64_t *free_bytes, uint64_t *total_bytes);
  *   bands reach below and compiles run again with more rows (a test
  *   setting); same results; read at each load.  Drivers always store every
  *   layer (their capacity is fixed at creation).
+ *   "bucket_cap": slots per sort bucket, 64, 128 or 256 (default 256):
+ *   classify writes a bucket's first bucket_cap nodes to its slots and the
+ *   rest to an overflow list that the bucket's workgroup sorts; slot
+ *   storage is buckets x bucket_cap + nodes; with bucket_fill it sets the
+ *   shape of the hash table; same results; read at each load (a driver
+ *   keeps the value it was created with).
  *   "sweep_split": 1 (default) lets a load's first compile plan a
  *   time-chunked sweep of columns much longer than the rest (transfer maps
  *   of the middle chunks, then the lower chunks in parallel; identical
@@ -328,6 +335,16 @@ int gp_device_memory(int device, uint64_t *free_bytes, uint64_t *total_bytes);
  *   whole; read at a load's first compile. */
 int gp_set_tuning(const char *name, uint32_t value);
 int gp_get_tuning(const char *name, uint32_t *value);
+/* Tuned knobs per size class (what an autotuner installs): circuits at
+ * `correlation_level` whose STEPG has n nodes with floor(log2 n) ==
+ * size_class take `value` for "bucket_fill", "bucket_cap" or "sweep_rpt"
+ * instead of the process-wide setting, when they are parsed (sweep_rpt) or
+ * loaded (the others; a batch takes the largest bucket_cap among its
+ * circuits).  value 0 removes the entry; gp_get_tuning_class reads 0 for
+ * none. */
+int gp_set_tuning_class(int correlation_level, uint32_t size_class, const char *name, uint32_t value);
+int gp_get_tuning_class(int correlation_level, uint32_t size_class, const char *name, uint32_t *value);
+int gp_clear_tuning_classes(void);
 
 /* ------------------------------------------------------------------------
  * Differentiable probabilities (no counterpart in the reference).
