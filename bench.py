@@ -16,15 +16,17 @@ The same JSON line also carries:
     span the reference times, Cli.hpp:293-302), (iii) end to end from circuit
     text (parse + STEPG build + (ii));
   * the batched workload (256 seeded uneven variants, batched vs one by one),
-    the differentiable workload (forward + backward on bb12) and the serving
-    workload (Poisson streams at 100 and 1000 requests/s);
+    the differentiable workload (forward + backward on bb12), the serving
+    workload (Poisson streams at 100 and 1000 requests/s) and the adaptive
+    Iceberg x surface workload (64 seeded branches of d = 6, the reference's
+    recompile-per-branch loop against one batch);
   * a CPU baseline: the oracle's forward Pauli-frame compiler (a port with
     stim's semantics, not stim) on one core and on every core of this host.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
 Other workloads (--workload, one JSON line each): batch, grad, serving (the
 full rate sweep 100-5000 requests/s), long (40-200-round memories, whole vs
-time-chunked sweep).
+time-chunked sweep), adaptive (Iceberg x surface d = 4 and 6).
 """
 
 from __future__ import annotations
@@ -401,6 +403,52 @@ def measure_serving(rates, max_requests: int = 400) -> list:
     return results
 
 
+def measure_adaptive(d: int = 6, count: int = 64, p: float = 1e-3) -> dict:
+    """The adaptive Iceberg x surface-code workload (Cli.hpp:333-432): `count`
+    seeded branches of the d-distance memory (rounds = d), compiled (i) as
+    the reference does, one driver sized from the static circuit recompiling
+    each branch (host parse + STEPG + device + DEM download per branch), and
+    (ii) all branches in one batch (host builds on the prepare threads, one
+    set of launches)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from greenpeas_amd import get_driver
+    from greenpeas_amd.batch import BatchCompiler, prepare
+    from greenpeas_amd.codes import count_locations
+    from greenpeas_amd.iceberg import IcebergSurfaceCode, adaptive_branches
+
+    static, _ = IcebergSurfaceCode(d).memory(d, p)
+    branches = adaptive_branches(d, count, p=p, seed=2026)
+    locs = sum(count_locations(t) for t in branches)
+    drv = get_driver(static)
+    drv.compile_detector_error_model()
+    for t in branches[:4]:  # warm
+        drv.compile_detector_error_model(t)
+    t0 = time.perf_counter()
+    for t in branches:
+        drv.compile_detector_error_model(t)
+    loop_s = time.perf_counter() - t0
+    drv.close()
+    bc = BatchCompiler()
+    bc.set_circuits(branches)
+    bc.run()  # warm (buffers sized)
+    with ThreadPoolExecutor(8) as pool:
+        t0 = time.perf_counter()
+        prepared = list(pool.map(prepare, branches))
+        bc.set_prepared(prepared)
+        dev_us = bc.run()
+        dems = [bc.dem(i) for i in range(len(branches))]
+        batch_s = time.perf_counter() - t0
+    bc.close()
+    out = {"code": f"Iceberg x surface d={d}, {d} rounds, adaptive, p={p}", "branches": count,
+           "error_locations": locs, "distinct_circuits": len(set(branches)),
+           "loop_us_per_branch": loop_s / count * 1e6, "batch_us_per_branch": batch_s / count * 1e6,
+           "batch_device_us": dev_us, "speedup": loop_s / batch_s, "errors_first": dems[0].num_errors}
+    log(f"adaptive d={d}: {count} branches, loop {out['loop_us_per_branch']:.0f} us/branch, "
+        f"batch {out['batch_us_per_branch']:.0f} us/branch (device {dev_us:.0f} us total)")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -411,7 +459,8 @@ def main() -> int:
                     help="plain workload only (no batch, grad and serving keys)")
     ap.add_argument("--protocol-n", type=int, default=100,
                     help="timed calls per span for the reference's statistic")
-    ap.add_argument("--workload", default="plain", choices=("plain", "batch", "grad", "serving", "long"),
+    ap.add_argument("--workload", default="plain",
+                    choices=("plain", "batch", "grad", "serving", "long", "adaptive"),
                     help="plain: the headline bb12 compile; the others print their own line")
     ap.add_argument("--variants", type=int, default=256, help="batch workload size")
     ap.add_argument("--sweep-rpt", type=int, default=0,
@@ -455,6 +504,10 @@ def main() -> int:
         return 0
     if args.workload == "long":
         return bench_long(args)
+    if args.workload == "adaptive":
+        print(json.dumps({"metric": METRIC, "workload": "adaptive", "results": [
+            measure_adaptive(4, 64), measure_adaptive(6, 64)]}), flush=True)
+        return 0
 
     out = measure_plain(args)
     locations = out["config"]["error_locations"]
@@ -462,6 +515,7 @@ def main() -> int:
         out["batch"] = measure_batch(args.variants, 2)
         out["grad"] = measure_grad(50, 5)
         out["serving"] = measure_serving((100, 1000))
+        out["adaptive"] = measure_adaptive(6, 64)
     cpu = None if args.no_cpu else cpu_baseline(read_circuit("bb12"), locations)
     if cpu:
         log(f"cpu baseline {cpu['value']:.3e} error-locations/s on 1 core "
