@@ -37,7 +37,7 @@ constexpr int kClassBlock = 256;
 constexpr int kScanBlock = 1024;
 constexpr int kScanTile = 4 * kScanBlock;
 constexpr int kBucketItemsPerWave = 256;  // wave-register sort capacity (4 per lane)
-std::atomic<uint32_t> gBucketFill{64};    // mean items per bucket the digit width aims at
+std::atomic<uint32_t> gBucketFill{0};     // mean items per bucket the digit width aims at (0: by load size)
 std::atomic<uint32_t> gBucketCap{256};    // slots per sort bucket (64, 128 or 256)
 std::atomic<uint32_t> gStash{1};          // target rows on (1) or off (0)
 std::atomic<uint32_t> gMaskTable{1};      // classify's LDS mask tables allowed (1) or not (0)
@@ -1424,39 +1424,9 @@ struct BucketIo {
 // workspace, bitonic-sorted there on (hash, rank), and merged chunk by chunk
 // by wave 0 with the items' node records; the sorted items and runs go to an
 // arena range taken with one atomic.
-__device__ void bigBucket(uint32_t b, uint32_t cnt, const BucketIo &io, const RunOut &o, uint32_t *tot) {
-  __shared__ uint32_t shOff, shFill;
-  __shared__ double hsSlots[32];
-  const uint32_t lane = threadIdx.x & 63;
-  if (threadIdx.x == 0) {
-    shOff = atomicAdd(&o.flags[5], cnt);  // total items never exceed the arena
-    shFill = io.cap;
-  }
-  __syncthreads();
-  const uint32_t off = shOff;
-  uint32_t P = 1;
-  while (P < cnt) P <<= 1;
-  uint64_t *K = io.scratchKey + 2ull * off;  // P < 2 * cnt
-  uint32_t *Rk = io.scratchRank + 2ull * off;
-  const uint32_t slot0 = b * io.cap;
-  for (uint32_t i = threadIdx.x; i < io.cap; i += blockDim.x) {
-    const uint4 x = io.pslot[slot0 + i];
-    K[i] = (static_cast<uint64_t>(x.y) << 32) | x.x;
-    Rk[i] = x.z;
-  }
-  const uint32_t nov = o.flags[3];
-  for (uint32_t k = threadIdx.x; k < nov; k += blockDim.x)
-    if (io.ovBucket[k] == b) {
-      const uint32_t at = atomicAdd(&shFill, 1u);
-      const uint4 x = io.ovSlot[k];
-      K[at] = (static_cast<uint64_t>(x.y) << 32) | x.x;
-      Rk[at] = x.z;
-    }
-  for (uint32_t i = cnt + threadIdx.x; i < P; i += blockDim.x) {
-    K[i] = ~0ull;
-    Rk[i] = NONE;
-  }
-  __syncthreads();
+// Bitonic sort of P (a power of two) (hash, rank) pairs by the workgroup.
+template <typename KP, typename RP>
+__device__ __forceinline__ void bitonicWG(KP K, RP Rk, uint32_t P) {
   for (uint32_t k = 2; k <= P; k <<= 1)
     for (uint32_t j = k >> 1; j > 0; j >>= 1) {
       for (uint32_t i = threadIdx.x; i < P; i += blockDim.x) {
@@ -1476,6 +1446,64 @@ __device__ void bigBucket(uint32_t b, uint32_t cnt, const BucketIo &io, const Ru
       }
       __syncthreads();
     }
+}
+
+// Items of an oversized bucket sorted in LDS when they fit this many
+// (`lds`: 12 bytes each), else in the global workspace.
+constexpr uint32_t kBigLdsItems = 2048;
+
+__device__ void bigBucket(uint32_t b, uint32_t cnt, const BucketIo &io, const RunOut &o, uint32_t *tot,
+                          uint64_t *ldsKeys, uint32_t *ldsRanks) {
+  __shared__ uint32_t shOff, shFill;
+  __shared__ double hsSlots[32];
+  const uint32_t lane = threadIdx.x & 63;
+  if (threadIdx.x == 0) {
+    shOff = atomicAdd(&o.flags[5], cnt);  // total items never exceed the arena
+    shFill = io.cap;
+  }
+  __syncthreads();
+  const uint32_t off = shOff;
+  uint32_t P = 1;
+  while (P < cnt) P <<= 1;
+  uint64_t *K = io.scratchKey + 2ull * off;  // P < 2 * cnt
+  uint32_t *Rk = io.scratchRank + 2ull * off;
+  // the items: its slots, then its entries of the overflow list, padded
+  // to P; sorted in LDS when they fit, else in the global workspace
+  const bool inLds = P <= kBigLdsItems;
+  auto put = [&](uint32_t at, uint64_t key, uint32_t rank) {
+    if (inLds) {
+      ldsKeys[at] = key;
+      ldsRanks[at] = rank;
+    } else {
+      K[at] = key;
+      Rk[at] = rank;
+    }
+  };
+  const uint32_t slot0 = b * io.cap;
+  for (uint32_t i = threadIdx.x; i < io.cap; i += blockDim.x) {
+    const uint4 x = io.pslot[slot0 + i];
+    put(i, (static_cast<uint64_t>(x.y) << 32) | x.x, x.z);
+  }
+  const uint32_t nov = o.flags[3];
+  for (uint32_t k = threadIdx.x; k < nov; k += blockDim.x)
+    if (io.ovBucket[k] == b) {
+      const uint32_t at = atomicAdd(&shFill, 1u);
+      const uint4 x = io.ovSlot[k];
+      put(at, (static_cast<uint64_t>(x.y) << 32) | x.x, x.z);
+    }
+  for (uint32_t i = cnt + threadIdx.x; i < P; i += blockDim.x) put(i, ~0ull, NONE);
+  __syncthreads();
+  if (inLds) {
+    bitonicWG(ldsKeys, ldsRanks, P);
+    // the sorted items to the workspace, where the merge reads them
+    for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) {
+      K[i] = ldsKeys[i];
+      Rk[i] = ldsRanks[i];
+    }
+    __syncthreads();
+  } else {
+    bitonicWG(K, Rk, P);
+  }
   const uint32_t base = io.arenaBase + off;
   if (threadIdx.x < 64) {
     RunCarry cy{0.0, 0ull, 0u, 0u, 0u, 0u, 0u};
@@ -1554,7 +1582,12 @@ __global__ __launch_bounds__(256) void bucketKernel(uint32_t numBuckets, BucketI
   }
   __syncthreads();
   const uint32_t n = nBig;
-  for (uint32_t k = 0; k < n; ++k) bigBucket(bigB[k], bigCnt[k], io, o, tot[bigB[k] - blockIdx.x * 4]);
+  // (the waves' LDS records are done with: the oversized buckets sort there)
+  static_assert(sizeof(pay) >= kBigLdsItems * 12, "LDS for oversized buckets");
+  uint64_t *ldsKeys = reinterpret_cast<uint64_t *>(&pay[0][0]);
+  uint32_t *ldsRanks = reinterpret_cast<uint32_t *>(ldsKeys + kBigLdsItems);
+  for (uint32_t k = 0; k < n; ++k)
+    bigBucket(bigB[k], bigCnt[k], io, o, tot[bigB[k] - blockIdx.x * 4], ldsKeys, ldsRanks);
   // the workgroup's totals, from which the emit stage forms the offsets of
   // its buckets
   __syncthreads();
@@ -2404,12 +2437,24 @@ void Pipeline::setStream(void *stream) {
   m.stream = stream ? static_cast<hipStream_t>(stream) : m.ownStream;
 }
 
+// Default items per bucket for a load of `loadRanks` error nodes: 64 while
+// the bucket stage is latency-bound (a few thousand buckets, one wave each:
+// shorter per-wave sorts win; bb12: 79.4 us per compile against 101.7 at
+// 128), 128 once there are millions (throughput-bound: fewer, fuller
+// buckets halve the emit stage; the 256-variant batch: 7.26 ms against
+// 7.90).
+constexpr uint64_t kLargeLoadRanks = uint64_t{1} << 22;
+static uint32_t defaultFill(uint64_t loadRanks) {
+  const uint32_t knob = gBucketFill.load(std::memory_order_relaxed);
+  return knob ? knob : loadRanks > kLargeLoadRanks ? 128u : 64u;
+}
+
 // First radix digit width of a circuit: the smallest b with fill * 2^b >=
-// its error nodes (fill: its size class's tuned value, else bucket_fill).
-static uint32_t bucketBitsFor(const StepgImage &g) {
+// its error nodes (fill: its size class's tuned value, else `fallback`).
+static uint32_t bucketBitsFor(const StepgImage &g, uint32_t fallback) {
   uint32_t b = 0;
   const uint64_t nodes = static_cast<uint64_t>(g.numLayers) * g.npl;
-  const uint64_t fill = classTuning("bucket_fill", g.level, nodes, gBucketFill.load(std::memory_order_relaxed));
+  const uint64_t fill = classTuning("bucket_fill", g.level, nodes, fallback);
   while (b < 24 && (fill << b) < g.numRanks) b++;
   return b;
 }
@@ -2598,6 +2643,9 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   std::vector<std::pair<int32_t, uint32_t>> order;  // (band estimate, item index)
   m.cols.clear();
   m.uniformCols = !growable_;
+  uint64_t loadRanks = 0;
+  for (const StepgImage *g : imgs) loadRanks += g->numRanks;
+  const uint32_t fill = defaultFill(loadRanks);
   for (uint32_t v = 0; v < V; ++v) {
     const StepgImage &g = *imgs[v];
     VDesc &d = m.descs[v];
@@ -2608,7 +2656,7 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
     d.numObservables = g.numObservables;
     d.numRanks = g.numRanks;
     d.numSeeds = g.numSeeds;
-    d.bucketBits = bucketBitsFor(g);
+    d.bucketBits = bucketBitsFor(g, fill);
     d.numBuckets = 1u << d.bucketBits;
     d.bucketBase = bucketBase;
     d.maskWords = (g.numCols + 31) / 32;
@@ -2774,9 +2822,11 @@ struct LoadTotals {
 
 LoadTotals loadTotals(const std::vector<const StepgImage *> &imgs, bool banded) {
   LoadTotals t;
-  uint64_t actOff = 0, seedOff = 0, spill = 0, probOff = 0, nTiles = 0;
+  uint64_t actOff = 0, seedOff = 0, spill = 0, probOff = 0, nTiles = 0, loadRanks = 0;
+  for (const StepgImage *g : imgs) loadRanks += g->numRanks;
+  const uint32_t fill = defaultFill(loadRanks);
   for (const StepgImage *g : imgs) {
-    t.B += uint64_t{1} << bucketBitsFor(*g);
+    t.B += uint64_t{1} << bucketBitsFor(*g, fill);
     nTiles += (g->numRanks + kClassBlock - 1) / kClassBlock;
     actOff += static_cast<uint64_t>(kSweepLeadPad + g->numPairs) * (g->layerStride + 1);
     seedOff += static_cast<uint64_t>(g->numCols) * g->numSeeds;
@@ -3441,7 +3491,7 @@ double Pipeline::benchDevice(uint32_t iters) {
   return ms * 1e3 / iters;
 }
 
-void setBucketFill(uint32_t fill) { gBucketFill.store(std::max<uint32_t>(fill, 1)); }
+void setBucketFill(uint32_t fill) { gBucketFill.store(fill); }  // 0: by load size
 void setStash(bool on) { gStash.store(on ? 1u : 0u); }
 bool stash() { return gStash.load() != 0; }
 void setSweepSplit(uint32_t mode) { gSweepSplit.store(mode); }

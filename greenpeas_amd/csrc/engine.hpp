@@ -149,7 +149,8 @@ class Pipeline {
 };
 
 // Autotuning knob: mean number of items per sort bucket the first radix
-// digit aims at (the digit width is chosen per circuit from it).
+// digit aims at (the digit width is chosen per circuit from it); 0 (the
+// default): 64, or 128 for loads of more than 2^22 error nodes.
 void setBucketFill(uint32_t fill);
 uint32_t bucketFill();
 // Whether classify stores each node's first targets for emit (on by

@@ -291,7 +291,8 @@ int gp_device_memory(int device, uint64_t *free_bytes, uint64_t *total_bytes);
 /* Autotuning knobs (process-wide):
  *   "bucket_fill": mean number of (hash, node) items per sort bucket the
  *   first radix digit aims at; the digit width per circuit is the smallest b
- *   with bucket_fill * 2^b >= its p>0 node count (default 64; read at the
+ *   with bucket_fill * 2^b >= its p>0 node count (default 0: 64, or 128
+ *   in loads of more than 2^22 error nodes; read at the
  *   next load).
  *   "sweep_rpt": fewest STEPG records per thread of the sensitivity sweep
  *   (1, 2, 4 or 8; default 1, raised as needed to fit a layer in 1024
