@@ -520,6 +520,8 @@ struct SweepState {
   uint32_t rowBytes;          // 4 * layerStride
   int32_t base;               // first layer the column's region stores
   uint32_t trash;             // its row for the even layers below base (the odd ones: the next)
+  int32_t snapPair;           // the pair whose even layer is copied to snapB (a split chunk's bottom)
+  uint32_t snapB;
   uint32_t off[RPT];          // byte offsets of this thread's records in a block
   uint32_t pos4[RPT];         // byte offsets of this thread's positions in a row or slot
 };
@@ -572,6 +574,7 @@ __device__ __forceinline__ void sweepStep(const SweepState<RPT> &st, uint4 (&r)[
 #pragma unroll
   for (int k = 0; k < RPT; ++k) {
     *ldsAt(slotB + st.pos4[k]) = b[k];
+    if (m == st.snapPair) *ldsAt(st.snapB + st.pos4[k]) = b[k];
     // write-through (system scope): S leaves no dirty lines for the end of
     // the kernel to write back (measured: 1 us less per compile on bb12)
     __builtin_amdgcn_raw_buffer_store_b32(a[k], st.sc, st.pos4[k], oddB, kStoreThrough);
@@ -584,59 +587,80 @@ __device__ __forceinline__ void sweepStep(const SweepState<RPT> &st, uint4 (&r)[
 
 // Time-chunked sweep of a long column.  The sweep is linear over GF(2): the
 // values of a run of pairs are a linear function of the ring state entering
-// it (the even layer above) plus the contribution of its own seeds.  A
-// column split into chunks 0..K-1 (chunk k = pairs bound[k] .. bound[k+1]-1)
-// is swept in two launches:
-//   launch 1  the top chunk, from the column's top seed (its state out goes
-//             to the column's state area), and for each middle chunk k
-//             (1 <= k <= K-2) its transfer map T_k -- ceil(P/32) sweeps with
-//             one 32-bit group of unit vectors injected as the state, no
-//             seeds, no S stores -- and its particular part p_k (zero state,
-//             its seeds, no S stores);
+// it (the even layer above) plus the contribution of its own seeds, and of
+// that state only the ring words its first pair reads count (the "live"
+// positions of the boundary: where a memory round resets its ancillas, only
+// the data qubits' frame nodes).  A column split into chunks 0..K-1 (chunk
+// k = pairs bound[k] .. bound[k+1]-1) is swept in two launches:
+//   launch 1  the top chunk, from the column's top seed (its state out: the
+//             even layer of pair bound[K-1] at the live positions of that
+//             boundary), and for each middle chunk k (1 <= k <= K-2) its
+//             transfer map T_k -- one sweep per 32 live positions of its top
+//             boundary, with unit vectors injected there, no seeds, no S
+//             stores -- and its particular part p_k (zero state, its seeds,
+//             no S stores), both read out at the live positions of its
+//             bottom boundary;
 //   launch 2  chunks 0..K-2, each from its entering state, formed as
-//             s = T_j s ^ p_j for j = K-2 .. k+1 starting from the top
-//             chunk's state out, storing S as the plain sweep does (seeds
-//             read from global memory, leaving LDS to the chain).
-// The chunks' lengths are multiples of 4 pairs (whole trips).  State area
-// of a split column (words, P = layerStride, G = ceil(P/32)): top state out
-// [P], then per middle chunk T_k [G][P] (word g*P + i: bit b = bit of
-// output node i for input node 32g + b) and p_k [P].
-constexpr int kSplitMaxChunks = 6;
+//             s = T_j s ^ p_j for j = K-2 .. k+1 from the top chunk's state
+//             out, storing S as the plain sweep does.
+// A chunk runs whole trips from its top pair.  A trip that passes the
+// chunk's bottom pair goes on below it: with the column's true values in
+// the top and resumed chunks (they store the very words the chunk below
+// stores), or with values nothing reads (maps and parts, which store no S).
+// The state out is a snapshot of the ring taken at the bottom pair's step.
+// n_k = live positions at boundary bound[k], listed ascending in
+// SplitIo::live from liveOff[k].  State area of a split column (words, each
+// part starting on a multiple of 4): the top chunk's state out [n_{K-1}] at
+// stateOff; then, for each middle chunk k in ascending order, at tOff[k],
+// T_k [ceil(n_{k+1} / 32)][n_k] (word g * n_k + o: bit b = bit of output o
+// for input 32g + b) followed by p_k [n_k]; stateEnd = one past the last.
+constexpr int kSplitMaxChunks = 8;
 constexpr uint32_t kSplitMaxColumns = 4;  // split columns per load
 struct SplitDesc {
   uint32_t v, c, K, stateOff;
   int32_t bound[kSplitMaxChunks + 1];  // first pair of chunk k; bound[K] = top pair + 1
-  uint32_t pad;
+  uint32_t nLive[kSplitMaxChunks + 1];
+  uint32_t liveOff[kSplitMaxChunks + 1];
+  uint32_t tOff[kSplitMaxChunks + 1];
+  uint32_t stateEnd;
+};
+static_assert(sizeof(SplitDesc) == 4 * (5 + 4 * (kSplitMaxChunks + 1)), "SplitDesc: 32-bit words, no padding");
+// A split column's descriptor held in one register of each wave (lane l =
+// word l; one load per wave instead of a memory round trip per field read)
+struct SplitView {
+  uint32_t w;
+  __device__ uint32_t at(uint32_t i) const { return static_cast<uint32_t>(__builtin_amdgcn_readlane(w, i)); }
+  __device__ uint32_t K() const { return at(2); }
+  __device__ uint32_t stateOff() const { return at(3); }
+  __device__ int32_t bound(uint32_t k) const { return static_cast<int32_t>(at(4 + k)); }
+  __device__ uint32_t nLive(uint32_t k) const { return at(5 + kSplitMaxChunks + k); }
+  __device__ uint32_t liveOff(uint32_t k) const { return at(6 + 2 * kSplitMaxChunks + k); }
+  __device__ uint32_t tOff(uint32_t k) const { return at(7 + 3 * kSplitMaxChunks + k); }
+  __device__ uint32_t stateEnd() const { return at(8 + 4 * kSplitMaxChunks); }
 };
 enum : uint32_t { kSplitNone = 0, kSplitTop = 1, kSplitBasis = 2, kSplitPart = 3, kSplitResume = 4 };
 struct SplitIo {
   const uint4 *items;  // after the launch's plain items: {split, mode | chunk << 8 | group << 16, circuit, column}
   const SplitDesc *descs;
   uint32_t *state;
+  const uint32_t *live;
   uint32_t numItems;
 };
 // A plain sweep item is {circuit, column, top seed layer, lowest seed
 // layer}; a split column's top chunk is {circuit, column | kItemSplit, top
 // seed layer, split}.
 constexpr uint32_t kItemSplit = 0x80000000u;
-__host__ __device__ inline uint64_t splitTOff(const SplitDesc &s, uint32_t P, uint32_t k) {
-  return s.stateOff + P + static_cast<uint64_t>(k - 1) * ((P + 31) / 32 + 1) * P;
-}
-__host__ __device__ inline uint64_t splitPOff(const SplitDesc &s, uint32_t P, uint32_t k) {
-  return splitTOff(s, P, k) + static_cast<uint64_t>((P + 31) / 32) * P;
-}
-__host__ __device__ inline uint64_t splitStateWords(uint32_t P, uint32_t K) {
-  return P + static_cast<uint64_t>(K - 2) * ((P + 31) / 32 + 1) * P;
-}
-// LDS words a resumed chunk's entering-state chain adds to the sweep's own,
-// with n matvecs: the state, the n transfer maps and particular parts it
-// applies (copied in at once), a scratch vector and a 4-bit lookup table
-// (16 entries per 4 input nodes, for a multiple of 32 words of T's rows),
-// plus alignment.
-__host__ __device__ inline uint32_t splitTableGroups(uint32_t P) { return ((P + 31) / 32 + 7) / 8 * 8; }
-__host__ __device__ inline uint32_t splitChainWords(uint32_t P, uint32_t n) {
-  const uint32_t G = (P + 31) / 32;
-  return 2 * P + 128 * splitTableGroups(P) + n * (G + 1) * P + 4;
+__host__ __device__ inline uint32_t quad(uint32_t n) { return (n + 3) & ~3u; }
+// words of T_k for n_{k+1} inputs and n_k outputs (p_k follows)
+__host__ __device__ inline uint32_t splitTWords(uint32_t nIn, uint32_t nOut) { return quad((nIn + 31) / 32 * nOut); }
+// LDS words a resumed chunk's entering-state chain adds to the sweep's own
+// (P = layer stride): two state vectors, the live positions it injects at,
+// a zero word (and three spare), a 4-bit lookup table (16 entries per 4
+// inputs, P inputs at most), the column's state area (mapWords: the top
+// chunk's state out and every map and part, copied in at once), a flag word
+// per map and the maps' row lists (listWords).
+__host__ __device__ inline uint32_t splitChainWords(uint32_t P, uint32_t mapWords, uint32_t listWords) {
+  return 3 * quad(P) + 4 + 128 * ((P + 31) / 32) + mapWords + kSplitMaxChunks + listWords + 4;
 }
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
@@ -645,77 +669,161 @@ __device__ __forceinline__ lds_u4 *lds4At(uint32_t byteAddr) {
   return reinterpret_cast<lds_u4 *>(static_cast<uintptr_t>(byteAddr));
 }
 
-// s = T s ^ p over GF(2), all in LDS (s at sB, written back there; T at TB
-// as [G][P] words, p at pB; scratch at wB, table at tabB): the table holds,
-// for each group of 4 input nodes, the XOR of every subset of their words;
-// output node i then XORs one table entry per 4 bits of its row of T.
-// Branch-free: the table covers whole groups of 8 words of T's rows (its
-// entries past the P input nodes are 0) and a row word past G reads as 0.
-__device__ __forceinline__ void splitMatvec(uint32_t P, uint32_t sB, uint32_t wB, uint32_t tabB, uint32_t TB,
-                                            uint32_t pB) {
-  const uint32_t G = (P + 31) / 32, G8 = splitTableGroups(P), tid = threadIdx.x;
-  for (uint32_t e = tid; e < 128 * G8; e += blockDim.x) {
-    const uint32_t q = e >> 4, x = e & 15;
+// y = T x ^ p over GF(2), all in LDS (x [nIn] at xB; T [ceil(nIn/32)][nOut]
+// at TB, p [nOut] at pB; y [nOut] to yB), with a 4-bit lookup table at tabB
+// (for each group of 4 inputs, the XOR of every subset of their words;
+// entry 0 of a group is 0, so a zero word of T looks up nothing).  L
+// adjacent lanes (a power of two, at most 8) share an output, each taking
+// every L-th word of its row of T, four words and their 32 lookups at a
+// time; shuffles combine them.  All lanes run the same number of passes.
+__device__ __forceinline__ void splitMatvec(uint32_t nIn, uint32_t nOut, uint32_t xB, uint32_t TB, uint32_t pB,
+                                            uint32_t yB, uint32_t tabB) {
+  const uint32_t G = (nIn + 31) / 32, tid = threadIdx.x, nt = blockDim.x;
+  for (uint32_t e = tid; e < 128 * G; e += nt) {
+    const uint32_t q = e >> 4, xs = e & 15;
     uint32_t v = 0;
 #pragma unroll
     for (uint32_t b = 0; b < 4; ++b) {
       const uint32_t j = 4 * q + b;
-      const uint32_t w = *ldsAt(sB + 4 * (j < P ? j : 0u));
-      v ^= (x >> b & 1u) && j < P ? w : 0u;
+      const uint32_t x = *ldsAt(xB + 4 * (j < nIn ? j : 0u));
+      v ^= (xs >> b & 1u) && j < nIn ? x : 0u;
     }
     *ldsAt(tabB + 4 * e) = v;
   }
   __syncthreads();
-  for (uint32_t i = tid; i < P; i += blockDim.x) {
-    uint32_t v = *ldsAt(pB + 4 * i);
-    // eight words of the row, then their 64 lookups: independent reads, so
-    // the LDS round trips overlap
-    for (uint32_t g0 = 0; g0 < G8; g0 += 8) {
-      uint32_t t[8];
+  uint32_t lg = 0;
+  while (lg < 3 && (2u << lg) * nOut <= nt) ++lg;
+  const uint32_t L = 1u << lg, lane = tid & (L - 1);
+  for (uint32_t base = 0; base < (nOut << lg); base += nt) {
+    const uint32_t o = (base + tid) >> lg;
+    uint32_t v = 0;
+    if (o < nOut) {
+      for (uint32_t g0 = lane; g0 < G; g0 += 4 * L) {
+        uint32_t w[4];
 #pragma unroll
-      for (uint32_t u = 0; u < 8; ++u) {
-        const uint32_t g = g0 + u;
-        const uint32_t w = *ldsAt(TB + 4 * ((g < G ? g : 0u) * P + i));
-        t[u] = g < G ? w : 0u;
+        for (uint32_t u = 0; u < 4; ++u) {
+          const uint32_t g = g0 + u * L;
+          const uint32_t x = *ldsAt(TB + 4 * ((g < G ? g : 0u) * nOut + o));
+          w[u] = g < G ? x : 0u;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) {
+          const uint32_t g = g0 + u * L, gg = g < G ? g : 0u;
+          uint32_t a = 0;
+#pragma unroll
+          for (uint32_t n = 0; n < 8; ++n) a ^= *ldsAt(tabB + 4 * (128 * gg + 16 * n + ((w[u] >> (4 * n)) & 15u)));
+          v ^= a;
+        }
       }
-      uint32_t a[8];
-#pragma unroll
-      for (uint32_t u = 0; u < 8; ++u) {
-        a[u] = 0u;
-#pragma unroll
-        for (uint32_t n = 0; n < 8; ++n)
-          a[u] ^= *ldsAt(tabB + 4 * (128 * (g0 + u) + 16 * n + ((t[u] >> (4 * n)) & 15u)));
-      }
-      v ^= ((a[0] ^ a[1]) ^ (a[2] ^ a[3])) ^ ((a[4] ^ a[5]) ^ (a[6] ^ a[7]));
     }
-    *ldsAt(wB + 4 * i) = v;
+    for (uint32_t sh = 1; sh < L; sh <<= 1) v ^= __shfl_xor(v, static_cast<int>(sh));
+    if (o < nOut && lane == 0) *ldsAt(yB + 4 * o) = v ^ *ldsAt(pB + 4 * o);
   }
-  __syncthreads();
-  for (uint32_t i = tid; i < P; i += blockDim.x) *ldsAt(sB + 4 * i) = *ldsAt(wB + 4 * i);
   __syncthreads();
 }
 
-// Copies a resumed chunk's inputs to LDS in one pass of wide loads (U per
-// thread in flight): the top chunk's state out (P words) to sB, then the
-// transfer maps and particular parts of chunks k+1 .. K-2 (contiguous in
-// the state area) right after it.  Every count and offset is a multiple of
-// 4 words.
-template <uint32_t U>
-__device__ void splitStage(const SplitDesc &sd, uint32_t P, uint32_t k, const uint32_t *state, uint32_t sB) {
-  const uint32_t q0 = P / 4, nq = q0 + (sd.K - 2 - k) * ((P + 31) / 32 + 1) * P / 4;
-  const u32x4 *src0 = reinterpret_cast<const u32x4 *>(state + sd.stateOff);
-  const u32x4 *src1 = reinterpret_cast<const u32x4 *>(state + splitTOff(sd, P, k + 1));
-  for (uint32_t base = threadIdx.x; base < nq; base += U * blockDim.x) {
-    u32x4 x[U];
+// Sparse rows of a map: per output, up to kRowBits input indices (16-bit,
+// two per word, 4 words) and their count (kRowDense: more).  A round of a
+// memory circuit maps the data qubits' frames nearly one to one, so most
+// rows hold one or two bits and a matvec from the lists costs each output
+// a few reads instead of a scan of its row.
+constexpr uint32_t kRowBits = 8, kRowDense = 0xFFFFu;
+__host__ __device__ inline uint32_t rowListWords(uint32_t nOut) { return 5 * quad(nOut); }
+
+// Scans the rows of one map (T [G][nOut] at TB) into lists (counts at cB,
+// indices at iB); a row with more than kRowBits bits sets the map's flag
+// word (flagB) to 1.  Output o is scanned by thread o (and o + nt, ...),
+// eight words of its row read at a time.
+__device__ __forceinline__ void buildRowLists(uint32_t G, uint32_t nOut, uint32_t TB, uint32_t cB, uint32_t iB,
+                                              uint32_t flagB) {
+  for (uint32_t o = threadIdx.x; o < nOut; o += blockDim.x) {
+    uint32_t c = 0, pk[4] = {0u, 0u, 0u, 0u};
+    for (uint32_t g0 = 0; g0 < G; g0 += 8) {
+      uint32_t w[8];
 #pragma unroll
-    for (uint32_t u = 0; u < U; ++u) {
-      const uint32_t q = base + u * blockDim.x;
-      if (q < nq) x[u] = q < q0 ? src0[q] : src1[q - q0];
+      for (uint32_t u = 0; u < 8; ++u) w[u] = *ldsAt(TB + 4 * ((g0 + u < G ? g0 + u : 0u) * nOut + o));
+#pragma unroll
+      for (uint32_t u = 0; u < 8; ++u) {
+        uint32_t x = g0 + u < G ? w[u] : 0u;
+        while (x) {
+          const uint32_t in = 32 * (g0 + u) + __builtin_ctz(x);
+          x &= x - 1;
+          if (c < kRowBits) {
+#pragma unroll
+            for (uint32_t k = 0; k < 4; ++k)
+              if (c >> 1 == k) pk[k] |= in << (16 * (c & 1));
+          }
+          ++c;
+        }
+      }
+    }
+    if (c > kRowBits) {
+      c = kRowDense;
+      *ldsAt(flagB) = 1u;
+    }
+    *ldsAt(cB + 4 * o) = c;
+    *lds4At(iB + 16 * o) = u32x4{pk[0], pk[1], pk[2], pk[3]};
+  }
+}
+
+// y = T x ^ p from a map's row lists (every count <= kRowBits): one lane
+// per output; the lists' reads, then the inputs', each in one batch (empty
+// entries read the zero word zB).  Waves with no output skip it.
+__device__ __forceinline__ void listMatvec(uint32_t nOut, uint32_t xB, uint32_t cB, uint32_t iB, uint32_t pB,
+                                           uint32_t yB, uint32_t zB) {
+  const uint32_t nt = blockDim.x;
+  for (uint32_t base = 0; base < nOut; base += nt) {
+    if (base + (threadIdx.x & ~63u) >= nOut) continue;  // (the whole wave)
+    const uint32_t o = base + threadIdx.x, oc = o < nOut ? o : 0u;
+    const uint32_t c = *ldsAt(cB + 4 * oc), pv = *ldsAt(pB + 4 * oc);
+    const u32x4 pk = *lds4At(iB + 16 * oc);
+    const uint32_t pw[4] = {pk.x, pk.y, pk.z, pk.w};
+    uint32_t v = pv;
+#pragma unroll
+    for (uint32_t k = 0; k < kRowBits; ++k) {
+      const uint32_t in = (pw[k >> 1] >> (16 * (k & 1))) & 0xFFFFu;
+      v ^= *ldsAt(k < c ? xB + 4 * in : zB);
+    }
+    if (o < nOut) *ldsAt(yB + 4 * o) = v;
+  }
+  __syncthreads();
+}
+
+// A resumed chunk's inputs to LDS: the column's state area (nq 16-byte
+// words from `st`: the top chunk's state out, then every middle chunk's map
+// and part) to stB, the live positions its state goes to (nlq 16-byte words,
+// at most one per thread) to lvB, and the column's seeds (ns words) to
+// seedB.  Each pass issues all of a thread's loads -- U state words, its
+// live word, US seeds, each range from one uniform base -- before its
+// stores, so one pass costs one memory round trip.
+template <uint32_t U, uint32_t US>
+__device__ void stageChain(const u32x4 *st, uint32_t nq, uint32_t stB, const u32x4 *lv, uint32_t nlq, uint32_t lvB,
+                           const uint32_t *seeds, uint32_t ns, uint32_t seedB) {
+  const uint32_t tid = threadIdx.x, nt = blockDim.x;
+  // (loads from clamped indices, so they issue back to back with no
+  // branches; only the stores check the range)
+  for (uint32_t r = 0; r * U * nt < nq || r * US * nt < ns || (r == 0 && nlq); ++r) {
+    u32x4 x[U], l = u32x4{0u, 0u, 0u, 0u};
+    uint32_t y[US];
+    if (nq) {
+#pragma unroll
+      for (uint32_t u = 0; u < U; ++u) x[u] = st[min((r * U + u) * nt + tid, nq - 1)];
+    }
+    if (r == 0 && nlq) l = lv[min(tid, nlq - 1)];
+    if (ns) {
+#pragma unroll
+      for (uint32_t u = 0; u < US; ++u) y[u] = seeds[min((r * US + u) * nt + tid, ns - 1)];
     }
 #pragma unroll
     for (uint32_t u = 0; u < U; ++u) {
-      const uint32_t q = base + u * blockDim.x;
-      if (q < nq) *lds4At(sB + 16 * q) = x[u];
+      const uint32_t q = (r * U + u) * nt + tid;
+      if (q < nq) *lds4At(stB + 16 * q) = x[u];
+    }
+    if (r == 0 && tid < nlq) *lds4At(lvB + 16 * tid) = l;
+#pragma unroll
+    for (uint32_t u = 0; u < US; ++u) {
+      const uint32_t i = (r * US + u) * nt + tid;
+      if (i < ns) *ldsAt(seedB + 4 * i) = y[u];
     }
   }
   __syncthreads();
@@ -754,8 +862,9 @@ __global__ __launch_bounds__(kSweepMaxThreads) void sweepKernel(
     c = __builtin_amdgcn_readfirstlane(it.w);
   }
   mode = __builtin_amdgcn_readfirstlane(mode);
+  SplitView sd{0u};  // loaded in split modes only
+  if (mode != kSplitNone) sd.w = (tid & 63) < sizeof(SplitDesc) / 4 ? reinterpret_cast<const uint32_t *>(sp.descs + si)[tid & 63] : 0u;
   const VDesc d = desc[v];
-  const SplitDesc *sd = sp.descs + si;  // read only in split modes
   if (mode == kSplitNone && hi < 0) {
     if (tid == 0) band[d.colOff + c] = make_int2(0x7FFFFFFF, -1);
     return;
@@ -763,12 +872,11 @@ __global__ __launch_bounds__(kSweepMaxThreads) void sweepKernel(
   const uint32_t P = d.layerStride, T = d.sweepThreads;
   int32_t start = hi / 2, stop = 0;
   if (mode != kSplitNone) {
-    const uint32_t K = __builtin_amdgcn_readfirstlane(sd->K);
     if (mode == kSplitTop) {
-      stop = __builtin_amdgcn_readfirstlane(sd->bound[K - 1]);
+      stop = sd.bound(sd.K() - 1);
     } else {
-      start = __builtin_amdgcn_readfirstlane(sd->bound[chunk + 1]) - 1;
-      stop = __builtin_amdgcn_readfirstlane(sd->bound[chunk]);
+      start = sd.bound(chunk + 1) - 1;
+      stop = sd.bound(chunk);
     }
   }
   const bool seeds = mode != kSplitBasis, stores = mode != kSplitBasis && mode != kSplitPart;
@@ -797,15 +905,19 @@ __global__ __launch_bounds__(kSweepMaxThreads) void sweepKernel(
   // seeds to LDS, several loads in flight per thread (fewer beside many
   // prefetched records)
   constexpr uint32_t U = RPT >= 4 ? 2 : 8;
-  constexpr uint32_t US = RPT >= 4 ? 2 : 16;  // wide loads of a resumed chunk's state inputs
-  if constexpr (!GSEEDS) {
+  // a resumed chunk's state inputs (16-byte words) and seeds per pass
+  constexpr uint32_t US = RPT >= 4 ? 2 : 8, USS = RPT >= 4 ? 4 : 16;
+  // (a resumed chunk loads its seeds with its chain's inputs, below)
+  if (!GSEEDS && mode != kSplitResume) {
     const uint32_t n = d.numSeeds;
     for (uint32_t base = tid; base < n; base += U * blockDim.x) {
+      // (clamped indices: the loads issue back to back, with no branches)
       uint32_t x[U];
 #pragma unroll
       for (uint32_t u = 0; u < U; ++u) {
         const uint32_t i = base + u * blockDim.x;
-        x[u] = i < n && seeds ? sv[i] : 0u;
+        x[u] = sv[min(i, n - 1)];
+        x[u] = seeds ? x[u] : 0u;
       }
 #pragma unroll
       for (uint32_t u = 0; u < U; ++u) {
@@ -814,30 +926,60 @@ __global__ __launch_bounds__(kSweepMaxThreads) void sweepKernel(
       }
     }
   }
-  // a resumed chunk's entering state (chain from the top chunk's state
-  // out): LDS state | maps and parts of chunks chunk+1 .. K-2 | scratch |
-  // table (16-byte aligned)
+  // split chunks: LDS from sB (16-byte aligned) holds the snapshot of the
+  // bottom pair's even layer (top chunks, maps, parts) or a resumed chunk's
+  // entering-state chain: vectors A and B | live positions of its top
+  // boundary | lookup table | maps and parts of chunks chunk+1 .. K-2
   const uint32_t sB = (voteB + 8 + 15) & ~15u;
   const uint32_t injSlot = mode == kSplitNone || mode == kSplitTop ? 0u : static_cast<uint32_t>((start + 1) & 1) * P;
+  uint32_t entB = 0, lvB = 0;  // a resumed chunk's entering state, its positions
   if (mode == kSplitResume) {
-    const uint32_t K = __builtin_amdgcn_readfirstlane(sd->K), G = (P + 31) / 32;
-    const uint32_t mB = sB + 4 * P, wB = mB + 4 * (K - 2 - chunk) * (G + 1) * P, tabB = wB + 4 * P;
-    splitStage<US>(*sd, P, chunk, sp.state, sB);
-    for (uint32_t j = K - 2; j > chunk; --j) {
-      const uint32_t TB = mB + 4 * (j - chunk - 1) * (G + 1) * P;
-      splitMatvec(P, sB, wB, tabB, TB, TB + 4 * G * P);
+    const uint32_t K = sd.K(), P4 = quad(P);
+    const uint32_t aB = sB, bB = aB + 4 * P4, miscB = bB + 8 * P4, tabB = miscB + 16;
+    const uint32_t mB = tabB + 4 * 128 * ((P + 31) / 32);
+    lvB = bB + 4 * P4;
+    const uint32_t s0 = sd.stateOff(), mapWords = sd.stateEnd() - s0;
+    // after the state area: per map of the chain, its dense flag, then its
+    // row lists (counts, indices)
+    const uint32_t fB = mB + 4 * mapWords, lB = fB + 4 * kSplitMaxChunks;
+    // misc: a zero word; the maps' flags (behind the staging's barrier)
+    if (tid == 0) *ldsAt(miscB) = 0u;
+    if (tid < kSplitMaxChunks) *ldsAt(fB + 4 * tid) = 0u;
+    stageChain<US, USS>(reinterpret_cast<const u32x4 *>(sp.state + s0), mapWords / 4, mB,
+                        reinterpret_cast<const u32x4 *>(sp.live + sd.liveOff(chunk + 1)), quad(sd.nLive(chunk + 1)) / 4,
+                        lvB, sv, GSEEDS ? 0u : d.numSeeds, 4 * (2 * P + 1));
+    // row lists of every map the chain applies (map j: inputs n_{j+1},
+    // outputs n_j), then the chain from the top chunk's state out (staged
+    // first)
+    for (uint32_t j = K - 2, lo = lB; j > chunk; lo += 4 * rowListWords(sd.nLive(j)), --j)
+      buildRowLists((sd.nLive(j + 1) + 31) / 32, sd.nLive(j), mB + 4 * (sd.tOff(j) - s0), lo,
+                    lo + 4 * quad(sd.nLive(j)), fB + 4 * j);
+    __syncthreads();
+    uint32_t cur = mB, nxt = aB, n = sd.nLive(K - 1);
+    for (uint32_t j = K - 2, lo = lB; j > chunk; lo += 4 * rowListWords(sd.nLive(j)), --j) {
+      const uint32_t nOut = sd.nLive(j);
+      const uint32_t TB = mB + 4 * (sd.tOff(j) - s0), pB = TB + 4 * splitTWords(n, nOut);
+      if (*ldsAt(fB + 4 * j)) splitMatvec(n, nOut, cur, TB, pB, nxt, tabB);
+      else listMatvec(nOut, cur, lo, lo + 4 * quad(nOut), pB, nxt, miscB);
+      cur = nxt;
+      nxt = cur == aB ? bB : aB;
+      n = nOut;
     }
+    entB = cur;
   }
-  // ring (with the injected state in the slot the chunk's first pair
-  // reads) and zero word
-  for (uint32_t i = tid; i <= 2 * P; i += blockDim.x) {
-    uint32_t x = 0u;
-    if (i >= injSlot && i < injSlot + P) {
-      const uint32_t j = i - injSlot;
-      if (mode == kSplitBasis) x = (j >> 5) == group ? 1u << (j & 31) : 0u;
-      else if (mode == kSplitResume) x = *ldsAt(sB + 4 * j);
-    }
-    *ldsAt(4 * i) = x;
+  // ring and zero word; then the injected state at the live positions of
+  // the slot the chunk's first pair reads
+  for (uint32_t i = tid; i <= 2 * P; i += blockDim.x) *ldsAt(4 * i) = 0u;
+  if (mode == kSplitResume) {
+    __syncthreads();
+    const uint32_t n = sd.nLive(chunk + 1);
+    for (uint32_t j = tid; j < n; j += blockDim.x) *ldsAt(4 * (injSlot + *ldsAt(lvB + 4 * j))) = *ldsAt(entB + 4 * j);
+  } else if (mode == kSplitBasis) {
+    const uint32_t *lv = sp.live + sd.liveOff(chunk + 1);
+    const uint32_t n = sd.nLive(chunk + 1), j = 32 * group + tid;
+    const uint32_t pos = tid < 32 && j < n ? lv[j] : 0u;
+    __syncthreads();
+    if (tid < 32 && j < n) *ldsAt(4 * (injSlot + pos)) = 1u << tid;
   }
   if (tid < 2) *ldsAt(voteB + 4 * tid) = 0u;
   __syncthreads();
@@ -860,6 +1002,8 @@ __global__ __launch_bounds__(kSweepMaxThreads) void sweepKernel(
     st.rowBytes = 4 * P;
     st.base = static_cast<int32_t>(__builtin_amdgcn_readfirstlane(reg.z));
     st.trash = rows - 2;
+    st.snapPair = mode == kSplitTop || mode == kSplitBasis || mode == kSplitPart ? stop : INT32_MIN;
+    st.snapB = sB;
     // ring slot of each trip position (pair mt - u; D is even)
 #pragma unroll
     for (int u = 0; u < D; ++u) slotB[u] = static_cast<uint32_t>((mt - u) & 1) * P * 4;
@@ -932,13 +1076,18 @@ __global__ __launch_bounds__(kSweepMaxThreads) void sweepKernel(
   }
   if (lo < 0) lo = 0;
   if (mode == kSplitTop || mode == kSplitBasis || mode == kSplitPart) {
-    // state out: the even layer of pair `stop` (written by the last step,
-    // behind its barrier)
-    uint32_t *out = sp.state + (mode == kSplitTop    ? static_cast<uint64_t>(sd->stateOff)
-                                : mode == kSplitPart ? splitPOff(*sd, P, chunk)
-                                                     : splitTOff(*sd, P, chunk) + static_cast<uint64_t>(group) * P);
-    const uint32_t slot = static_cast<uint32_t>(stop & 1) * P;
-    for (uint32_t i = tid; i < P; i += blockDim.x) out[i] = *ldsAt(4 * (slot + i));
+    // state out: the snapshot of pair `stop`'s even layer (taken in its
+    // step, behind that step's barrier) at the live positions of boundary
+    // `stop`
+    const uint32_t kb = mode == kSplitTop ? sd.K() - 1 : chunk;
+    const uint32_t n = sd.nLive(kb);
+    const uint32_t *lv = sp.live + sd.liveOff(kb);
+    uint32_t o = sd.stateOff();
+    if (mode != kSplitTop) {
+      o = sd.tOff(chunk);
+      o += mode == kSplitPart ? splitTWords(sd.nLive(chunk + 1), n) : group * n;
+    }
+    for (uint32_t i = tid; i < n; i += blockDim.x) sp.state[o + i] = *ldsAt(sB + 4 * lv[i]);
   }
   if (tid == 0 && (mode == kSplitNone || mode == kSplitTop)) {
     // a band below the stored rows is cut at them (readers stay inside the
@@ -2257,6 +2406,26 @@ size_t align256(size_t x) { return (x + 255) & ~static_cast<size_t>(255); }
 
 }  // namespace
 
+// Raises the sweep kernels' dynamic LDS limit to `bytes` (never lowers it:
+// other pipelines may launch with more).
+static void setSweepLds(size_t bytes) {
+  static std::mutex mu;
+  static size_t done = 64 * 1024;
+  std::lock_guard<std::mutex> lock(mu);
+  if (bytes <= done) return;
+  done = bytes;
+  const void *fns[] = {reinterpret_cast<const void *>(sweepKernel<1, false>),
+                       reinterpret_cast<const void *>(sweepKernel<2, false>),
+                       reinterpret_cast<const void *>(sweepKernel<4, false>),
+                       reinterpret_cast<const void *>(sweepKernel<8, false>),
+                       reinterpret_cast<const void *>(sweepKernel<1, true>),
+                       reinterpret_cast<const void *>(sweepKernel<2, true>),
+                       reinterpret_cast<const void *>(sweepKernel<4, true>),
+                       reinterpret_cast<const void *>(sweepKernel<8, true>)};
+  for (const void *f : fns)
+    GPA_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)));
+}
+
 struct Pipeline::Impl {
   int device;
   // the stream the pipeline's work is queued on: its own, or a caller's
@@ -2371,13 +2540,18 @@ struct Pipeline::Impl {
   // nodes see more than 12 columns on average, in loads of few workgroups)
   uint32_t classBatch = 4;
   uint32_t nSplit = 0, nSplit1 = 0, nSplit3 = 0;
-  size_t splitLdsBytes = 0, oSplitItems1 = 0, oSplitItems3 = 0;
+  // plain items of each launch: launch 1 takes the first nPlain1 of the
+  // items (the split columns' top chunks first), launch 2 the rest
+  uint32_t nPlain1 = 0;
+  bool split2Gseeds = true;  // launch 2 reads seeds from global memory (they do not fit in LDS beside the chain)
+  size_t split1LdsBytes = 0, splitLdsBytes = 0, oSplitItems1 = 0, oSplitItems3 = 0, oSplitLive = 0;
   DevBuf splitBuf, splitState;
   SplitIo splitIo(bool second) const {
     const char *b = splitBuf.as<char>();
-    if (!nSplit) return SplitIo{nullptr, nullptr, nullptr, 0};
+    if (!nSplit) return SplitIo{nullptr, nullptr, nullptr, nullptr, 0};
     return SplitIo{reinterpret_cast<const uint4 *>(b + (second ? oSplitItems3 : oSplitItems1)),
-                   reinterpret_cast<const SplitDesc *>(b), splitState.as<uint32_t>(), second ? nSplit3 : nSplit1};
+                   reinterpret_cast<const SplitDesc *>(b), splitState.as<uint32_t>(),
+                   reinterpret_cast<const uint32_t *>(b + oSplitLive), second ? nSplit3 : nSplit1};
   }
 
   uint32_t *flags() const { return counters.as<uint32_t>(); }
@@ -2626,6 +2800,7 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   m.splitPlanned = false;
   m.classBatch = 4;
   m.nSplit = m.nSplit1 = m.nSplit3 = 0;
+  m.nPlain1 = 0;
   const bool frozen = m.frozen;
   const uint32_t V = static_cast<uint32_t>(imgs.size());
   if (V == 0) throw InvalidArgument("no circuits to compile");
@@ -2790,19 +2965,7 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   }
   m.hVB.ensure(4 * (2 * V + 8));
   m.hFlags.ensure(4 * kFlagWords);
-  if (m.sweepLdsBytes > 64 * 1024) {
-    const void *fns[] = {reinterpret_cast<const void *>(sweepKernel<1, false>),
-                         reinterpret_cast<const void *>(sweepKernel<2, false>),
-                         reinterpret_cast<const void *>(sweepKernel<4, false>),
-                         reinterpret_cast<const void *>(sweepKernel<8, false>),
-                         reinterpret_cast<const void *>(sweepKernel<1, true>),
-                         reinterpret_cast<const void *>(sweepKernel<2, true>),
-                         reinterpret_cast<const void *>(sweepKernel<4, true>),
-                         reinterpret_cast<const void *>(sweepKernel<8, true>)};
-    for (const void *f : fns)
-      GPA_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  static_cast<int>(m.sweepLdsBytes)));
-  }
+  setSweepLds(m.sweepLdsBytes);
 
   // bucket counters start at zero; every run's bucket stage leaves them
   // zeroed; so does the band-overflow flag
@@ -2885,10 +3048,16 @@ uint64_t footprint(const LoadTotals &t, const HeldCaps &h, double headroom) {
   total += sBytes <= h.S ? h.S : grownBytes(sBytes, h.S ? headroom : 1.0);
   total += h.capTargets == 0 || h.capTargets < 16 * R ? afterEnsure(h.targets, 4 * 16 * R) : h.targets;
   total += h.members;
-  const uint64_t G = (t.maxP + 31) / 32;
-  const uint64_t splitItems = 4096 + 16ull * kSplitMaxColumns * (kSplitMaxChunks * (G + 1) + kSplitMaxChunks);
+  const uint64_t G = (t.maxP + 31) / 32, P4 = quad(static_cast<uint32_t>(t.maxP));
+  // descriptors, items and live lists; state areas (every boundary with
+  // all positions live)
+  const uint64_t splitItems = 4096 + sizeof(SplitDesc) * kSplitMaxColumns +
+                              16ull * kSplitMaxColumns * (kSplitMaxChunks * (G + 1) + kSplitMaxChunks) +
+                              4ull * kSplitMaxColumns * kSplitMaxChunks * P4;
+  const uint64_t splitState =
+      4ull * kSplitMaxColumns * (P4 + (kSplitMaxChunks - 2) * (splitTWords(P4, P4) + P4)) + 16;
   total += afterEnsure(h.splitBuf, splitItems);
-  total += afterEnsure(h.splitState, 4ull * kSplitMaxColumns * splitStateWords(t.maxP, kSplitMaxChunks));
+  total += afterEnsure(h.splitState, splitState);
   return total;
 }
 
@@ -2972,16 +3141,23 @@ void Pipeline::run(bool timed) {
   // captured in a graph.
   mark(0);
   {
-    // launch 1: every column (split columns: their top chunk) and the
-    // middle chunks' transfer maps; launch 2: the split columns' lower chunks
+    // launch 1: the split columns' top chunks, the middle chunks' transfer
+    // maps and the first nPlain1 plain items (every column when nothing is
+    // split); launch 2: the other plain items and the split columns' lower
+    // chunks
     const SplitIo sp1 = m.splitIo(false), sp2 = m.splitIo(true);
-#define GPA_SWEEP(RPT, G)                                                                             \
-  sweepKernel<RPT, G><<<std::max<uint32_t>(m.nItems + sp1.numItems, 1), m.sweepThreads, m.sweepLdsBytes, \
-                        st>>>(desc, items, act, seed, colRegion, m.band.as<int2>(), m.S.as<uint32_t>(), \
-                              m.nItems, m.flags(), sp1);                                                \
-  if (sp2.numItems)                                                                                     \
-    sweepKernel<RPT, true><<<sp2.numItems, m.sweepThreads, m.splitLdsBytes, st>>>(                      \
-        desc, items, act, seed, colRegion, m.band.as<int2>(), m.S.as<uint32_t>(), 0u, m.flags(), sp2)
+    const uint32_t n1 = m.nSplit ? m.nPlain1 : m.nItems, n2 = m.nItems - n1;
+    const size_t lds1 = m.nSplit ? m.split1LdsBytes : m.sweepLdsBytes;
+#define GPA_SWEEP2(RPT, G2)                                                                                \
+  sweepKernel<RPT, G2><<<n2 + sp2.numItems, m.sweepThreads, m.splitLdsBytes, st>>>(                       \
+      desc, items + n1, act, seed, colRegion, m.band.as<int2>(), m.S.as<uint32_t>(), n2, m.flags(), sp2)
+#define GPA_SWEEP(RPT, G)                                                                                  \
+  sweepKernel<RPT, G><<<std::max<uint32_t>(n1 + sp1.numItems, 1), m.sweepThreads, lds1, st>>>(             \
+      desc, items, act, seed, colRegion, m.band.as<int2>(), m.S.as<uint32_t>(), n1, m.flags(), sp1);       \
+  if (sp2.numItems) {                                                                                      \
+    if (m.split2Gseeds) GPA_SWEEP2(RPT, true);                                                             \
+    else GPA_SWEEP2(RPT, false);                                                                           \
+  }
     if (m.seedsInLds) {
       switch (m.rpt) {
       case 1: GPA_SWEEP(1, false); break;
@@ -2997,6 +3173,7 @@ void Pipeline::run(bool timed) {
       default: GPA_SWEEP(8, true); break;
       }
     }
+#undef GPA_SWEEP2
 #undef GPA_SWEEP
   }
   mark(1);
@@ -3057,34 +3234,59 @@ void Pipeline::run(bool timed) {
   GPA_HIP(hipGetLastError());
 }
 
-// Critical-path model of the sweep, in steps (pairs) of a lower chunk,
-// measured on MI355X (bb12): a step of the first launch (the top of a
-// column, or beside it) costs 1.7 steps -- the top chunk's 24 pairs took
-// 14 us, the lower chunks' ~0.3 us per pair (per-pair work, not contention:
-// the observables' column swept alone takes as long as with every other
-// column beside it); a matvec of the entering-state chain costs 22 (a
-// 576-node layer: VALU-bound on the one compute unit of the chunk); staging
-// the maps costs 10; the second launch's set-up 3.
-constexpr int32_t kSplitMatvecPairs = 22, kSplitStagePairs = 10, kSplitLaunchPairs = 3;
-constexpr int32_t kSplitFirstLaunchPct = 170;
+// Critical-path model of the sweep, in steps (pairs) of one column,
+// measured on MI355X (bb12, clock stamps per trip and per chain step): a
+// step costs ~0.29 us at every height of a column, whole or in chunks; a
+// chunk's first trip costs kSplitTrip steps more; each map of a resumed
+// chunk's entering-state chain (its row lists, then the product) costs
+// kSplitMvFixed plus kSplitMvPer288 per 288 x 288 inputs x outputs;
+// staging a resumed chunk's inputs costs kSplitStage, and the second
+// launch's start (the gap behind the first launch and its own set-up)
+// kSplitLaunch.
+constexpr double kSplitTrip = 1.0, kSplitMvFixed = 4.5, kSplitMvPer288 = 0.5, kSplitStage = 4.5,
+                 kSplitLaunch = 13.0;
 constexpr uint32_t kClassWideMaxTiles = 2048;  // class workgroups up to which the 16-column walk is used
+// boundaries a split plan chooses among (evenly thinned to this many)
+constexpr size_t kSplitMaxCandidates = 16;
 
-static int32_t firstLaunchSteps(int32_t n) { return n * kSplitFirstLaunchPct / 100; }
+// The live positions of boundary q of circuit d (the ring words that pair
+// q - 1 reads: the slot of layer 2q), ascending, from its staged sweep
+// records.
+static std::vector<uint32_t> livePositions(const Pipeline::Impl &m, const VDesc &d, int32_t q) {
+  const uint32_t P = d.layerStride, slot = static_cast<uint32_t>(q & 1) * P;
+  const uint4 *blk = reinterpret_cast<const uint4 *>(m.inHost.as<char>() + m.oAct) + d.actOff +
+                     static_cast<uint64_t>(static_cast<int64_t>(kSweepLeadPad) + q - 1) * (P + 1);
+  std::vector<uint8_t> seen(P, 0);
+  for (uint32_t j = 0; j < P; ++j) {
+    const uint4 r = blk[j];
+    for (const uint32_t w : {r.x, r.y, r.z})
+      for (const uint32_t x : {w & 0xFFFFu, w >> 16})
+        if (x >= slot && x < slot + P) seen[x - slot] = 1;
+  }
+  std::vector<uint32_t> out;
+  for (uint32_t j = 0; j < P; ++j)
+    if (seen[j]) out.push_back(j);
+  return out;
+}
 
 // Runs after a load's first run, from the column bands it found.  The
 // plain items are put in the order of their measured bands, longest first
 // (the launch order of the sweep's workgroups; the first run orders them by
 // seed range, which misses the observables' long band).  Then the
 // time-chunked sweep: the longest columns, while their band reaches the
-// bottom (layer <= 3) and K chunks are estimated to beat sweeping the
-// column whole by 10 % (the whole column: its first `rest` steps at the
-// first launch's rate, rest = the next longest band; split: the top chunk
-// or a middle chunk's maps at that rate, the second launch, then the
-// longest lower chunk with its staging and matvecs).  Writes the
-// items and descriptors to the device; the results do not depend on the
-// plan.
+// bottom (layer <= 3) and a plan is estimated to beat sweeping the column
+// whole by 10 %.  A plan's boundaries are chosen among the pairs whose
+// boundary has the fewest live positions (the kSplitMaxCandidates / 2
+// fewest and every boundary with as few), every combination of K - 1 of
+// them for K = 3 .. kSplitMaxChunks; its estimate is the first launch (the top chunk or the longest middle
+// chunk's maps) plus the second launch's start plus the second launch (the
+// longest of: a lower chunk with its staging and matvecs), with the other
+// columns (bounded by the next longest band) in the launch where they cost
+// least.  Writes the items and descriptors to the device; the results do
+// not depend on the plan.
 static void planSweep(Pipeline::Impl &m) {
   m.nSplit = m.nSplit1 = m.nSplit3 = 0;
+  m.nPlain1 = m.nItems;
   if (!m.Ccols) return;
   std::vector<int2> bands(m.Ccols);
   GPA_HIP(hipMemcpy(bands.data(), m.band.p, 8ull * m.Ccols, hipMemcpyDeviceToHost));
@@ -3141,89 +3343,178 @@ static void planSweep(Pipeline::Impl &m) {
     }
   }
   std::stable_sort(cols.begin(), cols.end(), [](const Col &a, const Col &b) { return a.len > b.len; });
-  std::vector<SplitDesc> descs;
-  std::vector<uint4> items1, items3;
-  uint64_t stateWords = 0;
-  size_t lds = 0;
-  for (size_t i = 0; i < cols.size() && descs.size() < kSplitMaxColumns; ++i) {
+  struct Plan {
+    SplitDesc sd{};
+    std::vector<std::vector<uint32_t>> live;  // [k] for boundaries 1 .. K-1
+    double est = 0;
+    bool otherFirst = true;  // the other columns in the first launch
+    bool seeds2 = false;     // launch 2 holds the seeds in LDS
+    size_t lds1 = 0, lds2 = 0;
+  };
+  std::vector<Plan> plans;
+  int detsFirst = -1;  // where the plain columns go (decided by the first split column)
+  for (size_t i = 0; i < cols.size() && plans.size() < kSplitMaxColumns; ++i) {
     const Col &col = cols[i];
     if (col.lo > 3) {
       if (force) continue;
       break;  // the longest column left bounds the sweep anyway
     }
     const VDesc &d = m.descs[col.v];
-    const uint32_t P = d.layerStride, G = (P + 31) / 32;
-    const int32_t rest = i + 1 < cols.size() ? cols[i + 1].len : 0;
+    const uint32_t P = d.layerStride;
+    const double rest = i + 1 < cols.size() ? cols[i + 1].len : 0;
     const int32_t top = col.hi / 2, total = top + 1;
-    SplitDesc best{};
-    // must beat the whole sweep by 10 % (unless forced)
-    const int32_t whole = firstLaunchSteps(std::min(rest, total)) + total - std::min(rest, total);
-    int32_t bestEst = force ? INT32_MAX : whole - whole / 10;
+    if (total < 3) continue;
+    // candidate boundaries: the pairs with the fewest live positions
+    std::vector<uint32_t> nl(total + 1, 0);
+    for (int32_t q = 1; q <= top; ++q) nl[q] = static_cast<uint32_t>(livePositions(m, d, q).size());
+    // (the boundaries with the fewest live positions, at least
+    // kSplitMaxCandidates / 2 of them)
+    std::vector<uint32_t> sorted(nl.begin() + 1, nl.begin() + top + 1);
+    std::sort(sorted.begin(), sorted.end());
+    const uint32_t limit = sorted[std::min<size_t>(sorted.size(), kSplitMaxCandidates / 2) - 1];
+    std::vector<int32_t> Q;
+    for (int32_t q = 1; q <= top; ++q)
+      if (nl[q] <= limit) Q.push_back(q);
+    if (Q.size() > kSplitMaxCandidates) {
+      std::vector<int32_t> t;
+      for (size_t k = 0; k < kSplitMaxCandidates; ++k) t.push_back(Q[k * (Q.size() - 1) / (kSplitMaxCandidates - 1)]);
+      Q = t;
+    }
+    const double whole = std::max<double>(total, rest) + kSplitTrip;
+    const size_t ring = 4ull * (2 * P + 1), seedB = 4ull * d.numSeeds;
+    auto al16 = [](size_t x) { return (x + 15) & ~static_cast<size_t>(15); };
+    Plan best;
+    best.est = force ? -1.0 : whole * 0.9;
+    int bestK = 0;
     for (int K = 3; K <= kSplitMaxChunks; ++K) {
-      SplitDesc s{};
-      s.v = col.v;
-      s.c = col.c;
-      s.K = static_cast<uint32_t>(K);
-      // lower chunk k pays its staging and K-2-k matvecs first: lengths
-      // shortened by that so the chunks of the second launch end together
-      // (forced plans: equal lengths)
-      auto over = [&](int k) {
-        const int32_t n = K - 2 - k;
-        return force ? 0 : n * kSplitMatvecPairs + (n ? kSplitStagePairs : 0);
-      };
-      int32_t sumOver = 0;
-      for (int k = 0; k <= K - 2; ++k) sumOver += over(k);
-      const int32_t x = std::max<int32_t>(4, (total + sumOver) / K / 4 * 4);
-      s.bound[K] = top + 1;
-      s.bound[K - 1] = s.bound[K] - x;
-      int32_t phase1 = std::max(x, rest), phase2 = 0;
-      for (int k = K - 2; k >= 1; --k) {
-        const int32_t len = std::max<int32_t>(4, (x - over(k)) / 4 * 4);
-        s.bound[k] = s.bound[k + 1] - len;
-        phase1 = std::max(phase1, len);
-        phase2 = std::max(phase2, len + over(k));
-      }
-      s.bound[0] = 0;
-      if (s.bound[1] < 4) continue;
-      phase2 = std::max(phase2, s.bound[1] + over(0));
-      const int32_t est = force ? -K : firstLaunchSteps(phase1) + kSplitLaunchPairs + phase2;
-      // (the second launch reads seeds from global memory: its LDS holds the
-      // ring and the entering-state chain)
-      const size_t need = 4ull * (2 * P + 1 + 2) + 4ull * splitChainWords(P, K - 2);
-      if (need > kLdsBudget) continue;
-      if (est < bestEst) {
-        bestEst = est;
-        best = s;
+      if (Q.size() < static_cast<size_t>(K - 1)) break;
+      std::vector<size_t> ix(K - 1);
+      for (int k = 0; k < K - 1; ++k) ix[k] = k;
+      for (;;) {
+        int32_t bd[kSplitMaxChunks + 1];
+        bd[0] = 0;
+        bd[K] = top + 1;
+        for (int k = 1; k < K; ++k) bd[k] = Q[ix[k - 1]];
+        double l1 = bd[K] - bd[K - 1], l2 = 0;
+        for (int k = 1; k <= K - 2; ++k) l1 = std::max<double>(l1, bd[k + 1] - bd[k]);
+        l1 += kSplitTrip;
+        double chain = 0;
+        size_t mapWords = 0, mapWords0 = 0;
+        for (int k = K - 2; k >= 0; --k) {
+          l2 = std::max(l2, kSplitStage + chain + (bd[k + 1] - bd[k]) + kSplitTrip);
+          if (k >= 1) {
+            chain += kSplitMvFixed + kSplitMvPer288 * nl[bd[k + 1]] * nl[bd[k]] / (288.0 * 288.0);
+            mapWords += splitTWords(nl[bd[k + 1]], nl[bd[k]]) + quad(nl[bd[k]]);
+          }
+        }
+        mapWords0 = mapWords + quad(nl[bd[K - 1]]);
+        uint32_t listWords0 = 0;
+        for (int k = 1; k <= K - 2; ++k) listWords0 += rowListWords(nl[bd[k]]);
+        const double other = rest + kSplitTrip;
+        const double estA = std::max(l1, other) + kSplitLaunch + l2;  // others in launch 1
+        const double estB = l1 + kSplitLaunch + std::max(l2, other);  // others in launch 2
+        const size_t chainB = 4ull * splitChainWords(P, static_cast<uint32_t>(mapWords0), listWords0);
+        const size_t lds2s = al16(ring + seedB + 8) + chainB, lds2g = al16(ring + 8) + chainB;
+        const bool seeds2 = m.seedsInLds && lds2s <= kLdsBudget;
+        bool ok = seeds2 || lds2g <= kLdsBudget;
+        // the other columns go to launch 2 only with their seeds in LDS
+        const bool firstOk = detsFirst != 0, secondOk = detsFirst != 1 && seeds2;
+        double est = 1e300;
+        bool otherFirst = true;
+        if (firstOk) est = estA;
+        if (secondOk && estB < est) {
+          est = estB;
+          otherFirst = false;
+        }
+        if (!firstOk && !secondOk) ok = false;
+        if (force) {
+          // most chunks, then the shortest longest chunk
+          int32_t longest = 0;
+          for (int k = 0; k < K; ++k) longest = std::max(longest, bd[k + 1] - bd[k]);
+          est = K * 1e6 - longest;
+        }
+        const bool better = force ? est > best.est : est < best.est;
+        if (ok && better) {
+          best.est = est;
+          best.otherFirst = otherFirst;
+          best.seeds2 = seeds2;
+          best.sd = SplitDesc{};
+          best.sd.v = col.v;
+          best.sd.c = col.c;
+          best.sd.K = static_cast<uint32_t>(K);
+          for (int k = 0; k <= K; ++k) best.sd.bound[k] = bd[k];
+          best.lds1 = al16(ring + (m.seedsInLds ? seedB : 0) + 8) + 4ull * quad(P);
+          best.lds2 = seeds2 ? lds2s : lds2g;
+          bestK = K;
+        }
+        // next combination
+        int k = K - 2;
+        while (k >= 0 && ix[k] == Q.size() - (K - 1) + k) --k;
+        if (k < 0) break;
+        ++ix[k];
+        for (int j = k + 1; j < K - 1; ++j) ix[j] = ix[j - 1] + 1;
       }
     }
-    if (!best.K) {
+    if (!bestK) {
       if (force) continue;
       break;
     }
-    lds = std::max<size_t>(lds, 4ull * (2 * P + 1 + 2) + 4ull * splitChainWords(P, best.K - 2));
-    const uint32_t si = static_cast<uint32_t>(descs.size());
-    best.stateOff = static_cast<uint32_t>(stateWords);
-    stateWords += splitStateWords(P, best.K);
-    if (stateWords >= 0xFFFFFFFFull) break;
-    for (uint32_t k = 1; k + 2 <= best.K; ++k) {
-      for (uint32_t g = 0; g < G; ++g) items1.push_back(make_uint4(si, kSplitBasis | k << 8 | g << 16, col.v, col.c));
-      items1.push_back(make_uint4(si, kSplitPart | k << 8, col.v, col.c));
+    if (detsFirst < 0) detsFirst = best.otherFirst ? 1 : 0;
+    best.live.resize(bestK);
+    for (int k = 1; k < bestK; ++k) {
+      best.live[k] = livePositions(m, d, best.sd.bound[k]);
+      best.sd.nLive[k] = static_cast<uint32_t>(best.live[k].size());
     }
-    for (uint32_t k = 0; k + 2 <= best.K; ++k) items3.push_back(make_uint4(si, kSplitResume | k << 8, col.v, col.c));
-    descs.push_back(best);
+    plans.push_back(std::move(best));
   }
-  if (descs.empty()) {
+  if (plans.empty()) {
     writeItems();
     return;
+  }
+  // state areas, live lists and items
+  std::vector<SplitDesc> descs;
+  std::vector<uint4> items1, items3;
+  std::vector<uint32_t> live;
+  uint64_t stateWords = 0;
+  size_t lds1 = m.sweepLdsBytes, lds2 = detsFirst ? 0 : m.sweepLdsBytes;
+  bool gseeds2 = false;
+  for (Plan &pl : plans) {
+    SplitDesc &sd = pl.sd;
+    const uint32_t K = sd.K, si = static_cast<uint32_t>(descs.size());
+    for (uint32_t k = 1; k < K; ++k) {
+      sd.liveOff[k] = static_cast<uint32_t>(live.size());
+      live.insert(live.end(), pl.live[k].begin(), pl.live[k].end());
+      live.resize(quad(static_cast<uint32_t>(live.size())), 0u);
+    }
+    sd.stateOff = static_cast<uint32_t>(stateWords);
+    stateWords += quad(sd.nLive[K - 1]);
+    for (uint32_t k = 1; k + 2 <= K; ++k) {
+      sd.tOff[k] = static_cast<uint32_t>(stateWords);
+      stateWords += splitTWords(sd.nLive[k + 1], sd.nLive[k]) + quad(sd.nLive[k]);
+    }
+    sd.tOff[K - 1] = sd.stateEnd = static_cast<uint32_t>(stateWords);
+    if (stateWords >= 0xFFFFFFF0ull) break;
+    for (uint32_t k = 1; k + 2 <= K; ++k) {
+      for (uint32_t g = 0; g < (sd.nLive[k + 1] + 31) / 32; ++g)
+        items1.push_back(make_uint4(si, kSplitBasis | k << 8 | g << 16, sd.v, sd.c));
+      items1.push_back(make_uint4(si, kSplitPart | k << 8, sd.v, sd.c));
+    }
+    for (uint32_t k = 0; k + 2 <= K; ++k) items3.push_back(make_uint4(si, kSplitResume | k << 8, sd.v, sd.c));
+    lds1 = std::max(lds1, pl.lds1);
+    lds2 = std::max(lds2, pl.lds2);
+    gseeds2 = gseeds2 || !pl.seeds2;
+    descs.push_back(sd);
   }
   auto al = [](size_t x) { return (x + 255) & ~static_cast<size_t>(255); };
   m.oSplitItems1 = al(sizeof(SplitDesc) * descs.size());
   m.oSplitItems3 = al(m.oSplitItems1 + 16 * items1.size());
-  const size_t bytes = m.oSplitItems3 + 16 * items3.size();
+  m.oSplitLive = al(m.oSplitItems3 + 16 * items3.size());
+  const size_t bytes = m.oSplitLive + 4 * live.size() + 16;
   std::vector<char> h(bytes, 0);
   std::memcpy(h.data(), descs.data(), sizeof(SplitDesc) * descs.size());
   std::memcpy(h.data() + m.oSplitItems1, items1.data(), 16 * items1.size());
   std::memcpy(h.data() + m.oSplitItems3, items3.data(), 16 * items3.size());
+  std::memcpy(h.data() + m.oSplitLive, live.data(), 4 * live.size());
   m.dropGraph();  // before the buffers it may name are replaced
   // a split column's band is reported from layer 0 (its top chunk does not
   // reach the bottom): it stores every layer
@@ -3237,18 +3528,25 @@ static void planSweep(Pipeline::Impl &m) {
   }
   if (relayout) layoutS(m, m.frozen, kGrowHeadroom, true);
   m.splitBuf.ensure(bytes, false, "split sweep");
-  m.splitState.ensure(4 * stateWords, false, "split sweep");
+  m.splitState.ensure(4 * stateWords + 16, false, "split sweep");
   GPA_HIP(hipMemcpy(m.splitBuf.p, h.data(), bytes, hipMemcpyHostToDevice));
   // the split columns' own items become their top chunks (in the staged
-  // inputs, which later compiles upload, and on the device now)
+  // inputs, which later compiles upload, and on the device now), launched
+  // first
   for (uint32_t si = 0; si < descs.size(); ++si)
     for (uint4 &it : m.hostItems)
       if (it.x == descs[si].v && it.y == descs[si].c) {
         it.y |= kItemSplit;
         it.w = si;
       }
+  std::stable_partition(m.hostItems.begin(), m.hostItems.end(),
+                        [](const uint4 &it) { return (it.y & kItemSplit) != 0; });
   writeItems();
-  m.splitLdsBytes = lds;
+  m.nPlain1 = detsFirst ? m.nItems : static_cast<uint32_t>(descs.size());
+  m.split2Gseeds = gseeds2;
+  m.split1LdsBytes = lds1;
+  m.splitLdsBytes = lds2;
+  setSweepLds(std::max(lds1, lds2));
   m.nSplit = static_cast<uint32_t>(descs.size());
   m.nSplit1 = static_cast<uint32_t>(items1.size());
   m.nSplit3 = static_cast<uint32_t>(items3.size());
