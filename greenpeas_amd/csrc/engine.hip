@@ -80,11 +80,13 @@ __device__ __forceinline__ double rankProbability(const VDesc &d, const double *
 
 // Counter block zeroed at the start of every run: [flags | bucket counts].
 // Per-run flag words, zeroed by the sweep kernel: [0] hash collision,
-// [1] target overflow, [3] overflow-list length, [5] oversized-bucket arena fill;
+// [1] target overflow, [3] overflow-list length, [5] oversized-bucket arena fill,
+// [kFlagSplitWait] a time-chunked sweep's wait for its inputs timed out;
 // and [kFlagBandOverflow], zeroed by the host: a column's band reached below
 // the rows its region of S stores
 constexpr uint32_t kFlagWords = 16;
 constexpr uint32_t kFlagBandOverflow = kFlagWords - 1;
+constexpr uint32_t kFlagSplitWait = 7;
 
 // S holds, per detector column, the rows of layers base .. top (the rows
 // its band can reach) at a region of its own, then two rows that the
@@ -528,6 +530,11 @@ struct SweepState {
 
 // cache policy of a buffer store: sc0 | sc1 (system scope, written through)
 constexpr int kStoreThrough = 1 | 16;
+// and of a load that must see what other compute units wrote through in the
+// same launch (system scope: not served from this XCD's L2)
+constexpr int kCoherent = 1 | 16;
+// s_waitcnt vmcnt(0) (expcnt and lgkmcnt not waited for)
+constexpr int kWaitVm0 = 0x0F70;
 
 // LDS byte address of the ring word named by a 16-bit index.
 __device__ __forceinline__ uint32_t ringLo(uint32_t x) { return (x & 0xFFFFu) << 2; }
@@ -591,18 +598,28 @@ __device__ __forceinline__ void sweepStep(const SweepState<RPT> &st, uint4 (&r)[
 // that state only the ring words its first pair reads count (the "live"
 // positions of the boundary: where a memory round resets its ancillas, only
 // the data qubits' frame nodes).  A column split into chunks 0..K-1 (chunk
-// k = pairs bound[k] .. bound[k+1]-1) is swept in two launches:
-//   launch 1  the top chunk, from the column's top seed (its state out: the
-//             even layer of pair bound[K-1] at the live positions of that
-//             boundary), and for each middle chunk k (1 <= k <= K-2) its
-//             transfer map T_k -- one sweep per 32 live positions of its top
-//             boundary, with unit vectors injected there, no seeds, no S
-//             stores -- and its particular part p_k (zero state, its seeds,
-//             no S stores), both read out at the live positions of its
-//             bottom boundary;
-//   launch 2  chunks 0..K-2, each from its entering state, formed as
-//             s = T_j s ^ p_j for j = K-2 .. k+1 from the top chunk's state
-//             out, storing S as the plain sweep does.
+// k = pairs bound[k] .. bound[k+1]-1) is swept by one launch:
+//   producers  the top chunk, from the column's top seed (its state out: the
+//              even layer of pair bound[K-1] at the live positions of that
+//              boundary), and for each middle chunk k (1 <= k <= K-2) its
+//              transfer map T_k -- one sweep per 32 live positions of its
+//              top boundary, with unit vectors injected there, no seeds, no
+//              S stores -- and its particular part p_k (zero state, its
+//              seeds, no S stores), read out at the live positions of its
+//              bottom boundary.  A map's sweeps also list each output's set
+//              bits (atomics on per-output counts); a row of more than
+//              kRowBits bits marks the map dense.  Each producer then counts
+//              itself done (release);
+//   resumed    chunks 0..K-2, launched after every producer (the hardware
+//              dispatches workgroups in order, so a producer never waits
+//              for a slot a waiting chunk holds), wait until the column's
+//              producers are all done (acquire; a bounded wait: past it the
+//              run is flagged and the host raises), form their entering
+//              state as s = T_j s ^ p_j for j = K-2 .. k+1 from the top
+//              chunk's state out (from the row lists, or a 4-bit lookup
+//              table for a dense map), and sweep, storing S as the plain
+//              sweep does.  The last of them to stage its inputs clears the
+//              column's counts for the next run.
 // A chunk runs whole trips from its top pair.  A trip that passes the
 // chunk's bottom pair goes on below it: with the column's true values in
 // the top and resumed chunks (they store the very words the chunk below
@@ -611,11 +628,15 @@ __device__ __forceinline__ void sweepStep(const SweepState<RPT> &st, uint4 (&r)[
 // n_k = live positions at boundary bound[k], listed ascending in
 // SplitIo::live from liveOff[k].  State area of a split column (words, each
 // part starting on a multiple of 4): the top chunk's state out [n_{K-1}] at
-// stateOff; then, for each middle chunk k in ascending order, at tOff[k],
+// stateOff; then, for each middle chunk k in ascending order, at tOff[k]:
 // T_k [ceil(n_{k+1} / 32)][n_k] (word g * n_k + o: bit b = bit of output o
-// for input 32g + b) followed by p_k [n_k]; stateEnd = one past the last.
+// for input 32g + b), p_k [n_k], the map's dense flag (4 words), its row
+// counts [n_k] and row lists (kRowBits 16-bit inputs per output); stateEnd
+// = one past the last.
 constexpr int kSplitMaxChunks = 8;
 constexpr uint32_t kSplitMaxColumns = 4;  // split columns per load
+constexpr uint32_t kRowBits = 8;          // set bits a row list holds
+constexpr uint32_t kSplitSpinLimit = 1u << 21;  // polls of a resumed chunk's wait (~2 s)
 struct SplitDesc {
   uint32_t v, c, K, stateOff;
   int32_t bound[kSplitMaxChunks + 1];  // first pair of chunk k; bound[K] = top pair + 1
@@ -623,8 +644,9 @@ struct SplitDesc {
   uint32_t liveOff[kSplitMaxChunks + 1];
   uint32_t tOff[kSplitMaxChunks + 1];
   uint32_t stateEnd;
+  uint32_t producers;  // workgroups that count themselves done per run
 };
-static_assert(sizeof(SplitDesc) == 4 * (5 + 4 * (kSplitMaxChunks + 1)), "SplitDesc: 32-bit words, no padding");
+static_assert(sizeof(SplitDesc) == 4 * (6 + 4 * (kSplitMaxChunks + 1)), "SplitDesc: 32-bit words, no padding");
 // A split column's descriptor held in one register of each wave (lane l =
 // word l; one load per wave instead of a memory round trip per field read)
 struct SplitView {
@@ -637,13 +659,17 @@ struct SplitView {
   __device__ uint32_t liveOff(uint32_t k) const { return at(6 + 2 * kSplitMaxChunks + k); }
   __device__ uint32_t tOff(uint32_t k) const { return at(7 + 3 * kSplitMaxChunks + k); }
   __device__ uint32_t stateEnd() const { return at(8 + 4 * kSplitMaxChunks); }
+  __device__ uint32_t producers() const { return at(9 + 4 * kSplitMaxChunks); }
 };
 enum : uint32_t { kSplitNone = 0, kSplitTop = 1, kSplitBasis = 2, kSplitPart = 3, kSplitResume = 4 };
 struct SplitIo {
-  const uint4 *items;  // after the launch's plain items: {split, mode | chunk << 8 | group << 16, circuit, column}
+  // after the launch's plain items: the maps' and parts' items, then the
+  // resumed chunks' {split, mode | chunk << 8 | group << 16, circuit, column}
+  const uint4 *items;
   const SplitDesc *descs;
   uint32_t *state;
   const uint32_t *live;
+  uint32_t *sync;  // per split column: producers done, resumed chunks staged
   uint32_t numItems;
 };
 // A plain sweep item is {circuit, column, top seed layer, lowest seed
@@ -651,16 +677,24 @@ struct SplitIo {
 // seed layer, split}.
 constexpr uint32_t kItemSplit = 0x80000000u;
 __host__ __device__ inline uint32_t quad(uint32_t n) { return (n + 3) & ~3u; }
-// words of T_k for n_{k+1} inputs and n_k outputs (p_k follows)
+// words of T_k for n_{k+1} inputs and n_k outputs
 __host__ __device__ inline uint32_t splitTWords(uint32_t nIn, uint32_t nOut) { return quad((nIn + 31) / 32 * nOut); }
+// a middle chunk's block of the state area: T_k, p_k, dense flag, row
+// counts, row lists; and where its flag starts
+__host__ __device__ inline uint32_t splitListOff(uint32_t nIn, uint32_t nOut) {
+  return splitTWords(nIn, nOut) + quad(nOut);
+}
+__host__ __device__ inline uint32_t splitMapWords(uint32_t nIn, uint32_t nOut) {
+  return splitListOff(nIn, nOut) + 4 + quad(nOut) + 4 * quad(nOut);
+}
 // LDS words a resumed chunk's entering-state chain adds to the sweep's own
 // (P = layer stride): two state vectors, the live positions it injects at,
-// a zero word (and three spare), a 4-bit lookup table (16 entries per 4
-// inputs, P inputs at most), the column's state area (mapWords: the top
-// chunk's state out and every map and part, copied in at once), a flag word
-// per map and the maps' row lists (listWords).
-__host__ __device__ inline uint32_t splitChainWords(uint32_t P, uint32_t mapWords, uint32_t listWords) {
-  return 3 * quad(P) + 4 + 128 * ((P + 31) / 32) + mapWords + kSplitMaxChunks + listWords + 4;
+// a zero word and a flag word (and two spare), a 4-bit lookup table (16
+// entries per 4 inputs, P inputs at most) and the column's state area
+// (mapWords: the top chunk's state out and every map's block, copied in at
+// once).
+__host__ __device__ inline uint32_t splitChainWords(uint32_t P, uint32_t mapWords) {
+  return 3 * quad(P) + 4 + 128 * ((P + 31) / 32) + mapWords + 4;
 }
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
@@ -722,53 +756,11 @@ __device__ __forceinline__ void splitMatvec(uint32_t nIn, uint32_t nOut, uint32_
   __syncthreads();
 }
 
-// Sparse rows of a map: per output, up to kRowBits input indices (16-bit,
-// two per word, 4 words) and their count (kRowDense: more).  A round of a
-// memory circuit maps the data qubits' frames nearly one to one, so most
-// rows hold one or two bits and a matvec from the lists costs each output
-// a few reads instead of a scan of its row.
-constexpr uint32_t kRowBits = 8, kRowDense = 0xFFFFu;
-__host__ __device__ inline uint32_t rowListWords(uint32_t nOut) { return 5 * quad(nOut); }
-
-// Scans the rows of one map (T [G][nOut] at TB) into lists (counts at cB,
-// indices at iB); a row with more than kRowBits bits sets the map's flag
-// word (flagB) to 1.  Output o is scanned by thread o (and o + nt, ...),
-// eight words of its row read at a time.
-__device__ __forceinline__ void buildRowLists(uint32_t G, uint32_t nOut, uint32_t TB, uint32_t cB, uint32_t iB,
-                                              uint32_t flagB) {
-  for (uint32_t o = threadIdx.x; o < nOut; o += blockDim.x) {
-    uint32_t c = 0, pk[4] = {0u, 0u, 0u, 0u};
-    for (uint32_t g0 = 0; g0 < G; g0 += 8) {
-      uint32_t w[8];
-#pragma unroll
-      for (uint32_t u = 0; u < 8; ++u) w[u] = *ldsAt(TB + 4 * ((g0 + u < G ? g0 + u : 0u) * nOut + o));
-#pragma unroll
-      for (uint32_t u = 0; u < 8; ++u) {
-        uint32_t x = g0 + u < G ? w[u] : 0u;
-        while (x) {
-          const uint32_t in = 32 * (g0 + u) + __builtin_ctz(x);
-          x &= x - 1;
-          if (c < kRowBits) {
-#pragma unroll
-            for (uint32_t k = 0; k < 4; ++k)
-              if (c >> 1 == k) pk[k] |= in << (16 * (c & 1));
-          }
-          ++c;
-        }
-      }
-    }
-    if (c > kRowBits) {
-      c = kRowDense;
-      *ldsAt(flagB) = 1u;
-    }
-    *ldsAt(cB + 4 * o) = c;
-    *lds4At(iB + 16 * o) = u32x4{pk[0], pk[1], pk[2], pk[3]};
-  }
-}
-
-// y = T x ^ p from a map's row lists (every count <= kRowBits): one lane
-// per output; the lists' reads, then the inputs', each in one batch (empty
-// entries read the zero word zB).  Waves with no output skip it.
+// y = T x ^ p from a map's row lists (cB: counts, iB: kRowBits 16-bit
+// inputs per output; every count <= kRowBits): one lane per output; the
+// lists' reads, then the inputs', each in one batch (empty entries read the
+// zero word zB).  A row of a memory circuit's round holds one or two bits,
+// so this replaces a scan of every row.  Waves with no output skip it.
 __device__ __forceinline__ void listMatvec(uint32_t nOut, uint32_t xB, uint32_t cB, uint32_t iB, uint32_t pB,
                                            uint32_t yB, uint32_t zB) {
   const uint32_t nt = blockDim.x;
@@ -789,41 +781,26 @@ __device__ __forceinline__ void listMatvec(uint32_t nOut, uint32_t xB, uint32_t 
   __syncthreads();
 }
 
-// A resumed chunk's inputs to LDS: the column's state area (nq 16-byte
-// words from `st`: the top chunk's state out, then every middle chunk's map
-// and part) to stB, the live positions its state goes to (nlq 16-byte words,
-// at most one per thread) to lvB, and the column's seeds (ns words) to
-// seedB.  Each pass issues all of a thread's loads -- U state words, its
-// live word, US seeds, each range from one uniform base -- before its
-// stores, so one pass costs one memory round trip.
-template <uint32_t U, uint32_t US>
-__device__ void stageChain(const u32x4 *st, uint32_t nq, uint32_t stB, const u32x4 *lv, uint32_t nlq, uint32_t lvB,
-                           const uint32_t *seeds, uint32_t ns, uint32_t seedB) {
+// A resumed chunk's state inputs to LDS: the column's state area (nq
+// 16-byte words through `st`, loaded system-coherent: other compute units
+// wrote them through in this launch; the top chunk's state out, then every
+// middle chunk's block) to stB.  Each pass issues all of a thread's U loads
+// (clamped indices: no branches) before its stores, so one pass costs one
+// memory round trip.
+template <uint32_t U>
+__device__ void stageChain(__amdgpu_buffer_rsrc_t st, uint32_t nq, uint32_t stB) {
   const uint32_t tid = threadIdx.x, nt = blockDim.x;
-  // (loads from clamped indices, so they issue back to back with no
-  // branches; only the stores check the range)
-  for (uint32_t r = 0; r * U * nt < nq || r * US * nt < ns || (r == 0 && nlq); ++r) {
-    u32x4 x[U], l = u32x4{0u, 0u, 0u, 0u};
-    uint32_t y[US];
-    if (nq) {
+  for (uint32_t r = 0; r * U * nt < nq; ++r) {
+    u32x4 x[U];
 #pragma unroll
-      for (uint32_t u = 0; u < U; ++u) x[u] = st[min((r * U + u) * nt + tid, nq - 1)];
-    }
-    if (r == 0 && nlq) l = lv[min(tid, nlq - 1)];
-    if (ns) {
-#pragma unroll
-      for (uint32_t u = 0; u < US; ++u) y[u] = seeds[min((r * US + u) * nt + tid, ns - 1)];
+    for (uint32_t u = 0; u < U; ++u) {
+      const auto v = __builtin_amdgcn_raw_buffer_load_b128(st, 16 * min((r * U + u) * nt + tid, nq - 1), 0, kCoherent);
+      x[u] = u32x4{v[0], v[1], v[2], v[3]};
     }
 #pragma unroll
     for (uint32_t u = 0; u < U; ++u) {
       const uint32_t q = (r * U + u) * nt + tid;
       if (q < nq) *lds4At(stB + 16 * q) = x[u];
-    }
-    if (r == 0 && tid < nlq) *lds4At(lvB + 16 * tid) = l;
-#pragma unroll
-    for (uint32_t u = 0; u < US; ++u) {
-      const uint32_t i = (r * US + u) * nt + tid;
-      if (i < ns) *ldsAt(seedB + 4 * i) = y[u];
     }
   }
   __syncthreads();
@@ -880,6 +857,18 @@ __global__ __launch_bounds__(kSweepMaxThreads) void sweepKernel(
     }
   }
   const bool seeds = mode != kSplitBasis, stores = mode != kSplitBasis && mode != kSplitPart;
+  // a producer's live positions of its bottom boundary (where its state out
+  // is read from the snapshot), loaded now, off the path to its signal:
+  // position tid + k * blockDim in lvOut[k] (at most RPT per thread, as
+  // n <= layerStride)
+  const bool producer = mode == kSplitTop || mode == kSplitBasis || mode == kSplitPart;
+  uint32_t lvOut[RPT] = {};
+  if (producer) {
+    const uint32_t kb = mode == kSplitTop ? sd.K() - 1 : chunk, n = sd.nLive(kb);
+    const uint32_t *lv = sp.live + sd.liveOff(kb);
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) lvOut[k] = n ? lv[min(tid + k * blockDim.x, n - 1)] : 0u;
+  }
   const uint32_t *sv = seedVals + d.seedOff + static_cast<uint64_t>(c) * d.numSeeds;
   uint32_t voteB = 4 * (2 * P + 1);
   if constexpr (!GSEEDS) voteB += 4 * d.numSeeds;
@@ -905,10 +894,9 @@ __global__ __launch_bounds__(kSweepMaxThreads) void sweepKernel(
   // seeds to LDS, several loads in flight per thread (fewer beside many
   // prefetched records)
   constexpr uint32_t U = RPT >= 4 ? 2 : 8;
-  // a resumed chunk's state inputs (16-byte words) and seeds per pass
-  constexpr uint32_t US = RPT >= 4 ? 2 : 8, USS = RPT >= 4 ? 4 : 16;
-  // (a resumed chunk loads its seeds with its chain's inputs, below)
-  if (!GSEEDS && mode != kSplitResume) {
+  // a resumed chunk's state inputs (16-byte words) per pass
+  constexpr uint32_t US = RPT >= 4 ? 2 : 8;
+  if (!GSEEDS) {
     const uint32_t n = d.numSeeds;
     for (uint32_t base = tid; base < n; base += U * blockDim.x) {
       // (clamped indices: the loads issue back to back, with no branches)
@@ -939,28 +927,37 @@ __global__ __launch_bounds__(kSweepMaxThreads) void sweepKernel(
     const uint32_t mB = tabB + 4 * 128 * ((P + 31) / 32);
     lvB = bB + 4 * P4;
     const uint32_t s0 = sd.stateOff(), mapWords = sd.stateEnd() - s0;
-    // after the state area: per map of the chain, its dense flag, then its
-    // row lists (counts, indices)
-    const uint32_t fB = mB + 4 * mapWords, lB = fB + 4 * kSplitMaxChunks;
-    // misc: a zero word; the maps' flags (behind the staging's barrier)
-    if (tid == 0) *ldsAt(miscB) = 0u;
-    if (tid < kSplitMaxChunks) *ldsAt(fB + 4 * tid) = 0u;
-    stageChain<US, USS>(reinterpret_cast<const u32x4 *>(sp.state + s0), mapWords / 4, mB,
-                        reinterpret_cast<const u32x4 *>(sp.live + sd.liveOff(chunk + 1)), quad(sd.nLive(chunk + 1)) / 4,
-                        lvB, sv, GSEEDS ? 0u : d.numSeeds, 4 * (2 * P + 1));
-    // row lists of every map the chain applies (map j: inputs n_{j+1},
-    // outputs n_j), then the chain from the top chunk's state out (staged
-    // first)
-    for (uint32_t j = K - 2, lo = lB; j > chunk; lo += 4 * rowListWords(sd.nLive(j)), --j)
-      buildRowLists((sd.nLive(j + 1) + 31) / 32, sd.nLive(j), mB + 4 * (sd.tOff(j) - s0), lo,
-                    lo + 4 * quad(sd.nLive(j)), fB + 4 * j);
+    uint32_t *sync = sp.sync + 2 * si;
+    // the live positions it injects at (they do not change from run to run)
+    {
+      const u32x4 *lv4 = reinterpret_cast<const u32x4 *>(sp.live + sd.liveOff(chunk + 1));
+      for (uint32_t q = tid; q < quad(sd.nLive(chunk + 1)) / 4; q += blockDim.x) *lds4At(lvB + 16 * q) = lv4[q];
+    }
+    // wait until the column's producers are done (thread 0 polls; the
+    // producers wrote the state area through to memory before counting
+    // themselves done, and it is read here past this XCD's L2)
+    if (tid == 0) {
+      *ldsAt(miscB) = 0u;  // misc: a zero word
+      const uint32_t need = sd.producers();
+      uint32_t spins = 0;
+      while (__hip_atomic_load(sync, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
+        __builtin_amdgcn_s_sleep(2);
+        if (++spins >= kSplitSpinLimit) {
+          atomicOr(&flags[kFlagSplitWait], 1u);
+          break;
+        }
+      }
+    }
     __syncthreads();
+    stageChain<US>(__builtin_amdgcn_make_buffer_rsrc(sp.state + s0, 0, 4 * mapWords, 0x00020000), mapWords / 4, mB);
+    // the chain, from the top chunk's state out (staged first)
     uint32_t cur = mB, nxt = aB, n = sd.nLive(K - 1);
-    for (uint32_t j = K - 2, lo = lB; j > chunk; lo += 4 * rowListWords(sd.nLive(j)), --j) {
+    for (uint32_t j = K - 2; j > chunk; --j) {
       const uint32_t nOut = sd.nLive(j);
       const uint32_t TB = mB + 4 * (sd.tOff(j) - s0), pB = TB + 4 * splitTWords(n, nOut);
-      if (*ldsAt(fB + 4 * j)) splitMatvec(n, nOut, cur, TB, pB, nxt, tabB);
-      else listMatvec(nOut, cur, lo, lo + 4 * quad(nOut), pB, nxt, miscB);
+      const uint32_t fB = TB + 4 * splitListOff(n, nOut);
+      if (*ldsAt(fB)) splitMatvec(n, nOut, cur, TB, pB, nxt, tabB);
+      else listMatvec(nOut, cur, fB + 16, fB + 16 + 4 * quad(nOut), pB, nxt, miscB);
       cur = nxt;
       nxt = cur == aB ? bB : aB;
       n = nOut;
@@ -1075,19 +1072,71 @@ __global__ __launch_bounds__(kSweepMaxThreads) void sweepKernel(
     }
   }
   if (lo < 0) lo = 0;
-  if (mode == kSplitTop || mode == kSplitBasis || mode == kSplitPart) {
+  if (mode == kSplitResume) {
+    // the last resumed chunk of the column to finish clears its row counts,
+    // dense flags and sync words for the next run (every resumed chunk has
+    // read the state area by then)
+    uint32_t *sync = sp.sync + 2 * si;
+    const uint32_t K = sd.K();
+    __syncthreads();
+    if (tid == 0) {
+      const uint32_t fin = __hip_atomic_fetch_add(sync + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      *ldsAt(sB) = fin + 2 == K ? 1u : 0u;
+    }
+    __syncthreads();
+    if (*ldsAt(sB)) {
+      const auto rsState = __builtin_amdgcn_make_buffer_rsrc(sp.state, 0, 4 * sd.stateEnd(), 0x00020000);
+      for (uint32_t j = 1; j + 2 <= K; ++j) {
+        const uint32_t n = sd.nLive(j), lz = sd.tOff(j) + splitListOff(sd.nLive(j + 1), n);
+        for (uint32_t i = tid; i < 4 + quad(n); i += blockDim.x)
+          __builtin_amdgcn_raw_buffer_store_b32(0u, rsState, 4 * (lz + i), 0, kStoreThrough);
+      }
+      if (tid < 2) sync[tid] = 0u;
+    }
+  }
+  if (producer) {
     // state out: the snapshot of pair `stop`'s even layer (taken in its
     // step, behind that step's barrier) at the live positions of boundary
-    // `stop`
+    // `stop` (lvOut)
     const uint32_t kb = mode == kSplitTop ? sd.K() - 1 : chunk;
     const uint32_t n = sd.nLive(kb);
-    const uint32_t *lv = sp.live + sd.liveOff(kb);
     uint32_t o = sd.stateOff();
     if (mode != kSplitTop) {
       o = sd.tOff(chunk);
       o += mode == kSplitPart ? splitTWords(sd.nLive(chunk + 1), n) : group * n;
     }
-    for (uint32_t i = tid; i < n; i += blockDim.x) sp.state[o + i] = *ldsAt(sB + 4 * lv[i]);
+    // written through to memory: the resumed chunks read them from there
+    const auto rs = __builtin_amdgcn_make_buffer_rsrc(sp.state, 0, 4 * sd.stateEnd(), 0x00020000);
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+      const uint32_t i = tid + k * blockDim.x;
+      if (i < n) __builtin_amdgcn_raw_buffer_store_b32(*ldsAt(sB + 4 * lvOut[k]), rs, 4 * (o + i), 0, kStoreThrough);
+    }
+    if (mode == kSplitBasis) {
+      // the map's row lists: each set bit appended to its output's list
+      const uint32_t lo = sd.tOff(chunk) + splitListOff(sd.nLive(chunk + 1), n);
+      uint32_t *cnt = sp.state + lo + 4;
+#pragma unroll
+      for (int k = 0; k < RPT; ++k) {
+        const uint32_t i = tid + k * blockDim.x;
+        uint32_t w = i < n ? *ldsAt(sB + 4 * lvOut[k]) : 0u;
+        while (w) {
+          const uint32_t in = 32 * group + __builtin_ctz(w);
+          w &= w - 1;
+          const uint32_t at = atomicAdd(&cnt[i], 1u);
+          if (at < kRowBits)
+            __builtin_amdgcn_raw_buffer_store_b16(static_cast<uint16_t>(in), rs,
+                                                  4 * (lo + 4 + quad(n)) + 2 * (kRowBits * i + at), 0, kStoreThrough);
+          else
+            __builtin_amdgcn_raw_buffer_store_b32(1u, rs, 4 * lo, 0, kStoreThrough);  // a row too dense
+        }
+      }
+    }
+    // counted done once every thread's stores are acknowledged (no release
+    // fence: it would write back the whole L2 of this XCD)
+    __builtin_amdgcn_s_waitcnt(kWaitVm0);
+    __syncthreads();
+    if (tid == 0) __hip_atomic_fetch_add(sp.sync + 2 * si, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   if (tid == 0 && (mode == kSplitNone || mode == kSplitTop)) {
     // a band below the stored rows is cut at them (readers stay inside the
@@ -2540,18 +2589,14 @@ struct Pipeline::Impl {
   // nodes see more than 12 columns on average, in loads of few workgroups)
   uint32_t classBatch = 4;
   uint32_t nSplit = 0, nSplit1 = 0, nSplit3 = 0;
-  // plain items of each launch: launch 1 takes the first nPlain1 of the
-  // items (the split columns' top chunks first), launch 2 the rest
-  uint32_t nPlain1 = 0;
-  bool split2Gseeds = true;  // launch 2 reads seeds from global memory (they do not fit in LDS beside the chain)
-  size_t split1LdsBytes = 0, splitLdsBytes = 0, oSplitItems1 = 0, oSplitItems3 = 0, oSplitLive = 0;
+  size_t splitLdsBytes = 0, oSplitItems = 0, oSplitLive = 0, oSplitSync = 0;
   DevBuf splitBuf, splitState;
-  SplitIo splitIo(bool second) const {
-    const char *b = splitBuf.as<char>();
-    if (!nSplit) return SplitIo{nullptr, nullptr, nullptr, nullptr, 0};
-    return SplitIo{reinterpret_cast<const uint4 *>(b + (second ? oSplitItems3 : oSplitItems1)),
-                   reinterpret_cast<const SplitDesc *>(b), splitState.as<uint32_t>(),
-                   reinterpret_cast<const uint32_t *>(b + oSplitLive), second ? nSplit3 : nSplit1};
+  SplitIo splitIo() const {
+    char *b = splitBuf.as<char>();
+    if (!nSplit) return SplitIo{nullptr, nullptr, nullptr, nullptr, nullptr, 0};
+    return SplitIo{reinterpret_cast<const uint4 *>(b + oSplitItems), reinterpret_cast<const SplitDesc *>(b),
+                   splitState.as<uint32_t>(), reinterpret_cast<const uint32_t *>(b + oSplitLive),
+                   reinterpret_cast<uint32_t *>(b + oSplitSync), nSplit1 + nSplit3};
   }
 
   uint32_t *flags() const { return counters.as<uint32_t>(); }
@@ -2800,7 +2845,6 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   m.splitPlanned = false;
   m.classBatch = 4;
   m.nSplit = m.nSplit1 = m.nSplit3 = 0;
-  m.nPlain1 = 0;
   const bool frozen = m.frozen;
   const uint32_t V = static_cast<uint32_t>(imgs.size());
   if (V == 0) throw InvalidArgument("no circuits to compile");
@@ -3053,9 +3097,9 @@ uint64_t footprint(const LoadTotals &t, const HeldCaps &h, double headroom) {
   // all positions live)
   const uint64_t splitItems = 4096 + sizeof(SplitDesc) * kSplitMaxColumns +
                               16ull * kSplitMaxColumns * (kSplitMaxChunks * (G + 1) + kSplitMaxChunks) +
-                              4ull * kSplitMaxColumns * kSplitMaxChunks * P4;
+                              4ull * kSplitMaxColumns * kSplitMaxChunks * P4 + 8ull * kSplitMaxColumns;
   const uint64_t splitState =
-      4ull * kSplitMaxColumns * (P4 + (kSplitMaxChunks - 2) * (splitTWords(P4, P4) + P4)) + 16;
+      4ull * kSplitMaxColumns * (P4 + (kSplitMaxChunks - 2) * splitMapWords(P4, P4)) + 16;
   total += afterEnsure(h.splitBuf, splitItems);
   total += afterEnsure(h.splitState, splitState);
   return total;
@@ -3141,23 +3185,13 @@ void Pipeline::run(bool timed) {
   // captured in a graph.
   mark(0);
   {
-    // launch 1: the split columns' top chunks, the middle chunks' transfer
-    // maps and the first nPlain1 plain items (every column when nothing is
-    // split); launch 2: the other plain items and the split columns' lower
-    // chunks
-    const SplitIo sp1 = m.splitIo(false), sp2 = m.splitIo(true);
-    const uint32_t n1 = m.nSplit ? m.nPlain1 : m.nItems, n2 = m.nItems - n1;
-    const size_t lds1 = m.nSplit ? m.split1LdsBytes : m.sweepLdsBytes;
-#define GPA_SWEEP2(RPT, G2)                                                                                \
-  sweepKernel<RPT, G2><<<n2 + sp2.numItems, m.sweepThreads, m.splitLdsBytes, st>>>(                       \
-      desc, items + n1, act, seed, colRegion, m.band.as<int2>(), m.S.as<uint32_t>(), n2, m.flags(), sp2)
+    // one launch: every column (split columns: their top chunk), then the
+    // middle chunks' maps and parts, then the lower chunks of split columns
+    const SplitIo sp = m.splitIo();
+    const size_t lds = m.nSplit ? m.splitLdsBytes : m.sweepLdsBytes;
 #define GPA_SWEEP(RPT, G)                                                                                  \
-  sweepKernel<RPT, G><<<std::max<uint32_t>(n1 + sp1.numItems, 1), m.sweepThreads, lds1, st>>>(             \
-      desc, items, act, seed, colRegion, m.band.as<int2>(), m.S.as<uint32_t>(), n1, m.flags(), sp1);       \
-  if (sp2.numItems) {                                                                                      \
-    if (m.split2Gseeds) GPA_SWEEP2(RPT, true);                                                             \
-    else GPA_SWEEP2(RPT, false);                                                                           \
-  }
+  sweepKernel<RPT, G><<<std::max<uint32_t>(m.nItems + sp.numItems, 1), m.sweepThreads, lds, st>>>(          \
+      desc, items, act, seed, colRegion, m.band.as<int2>(), m.S.as<uint32_t>(), m.nItems, m.flags(), sp)
     if (m.seedsInLds) {
       switch (m.rpt) {
       case 1: GPA_SWEEP(1, false); break;
@@ -3173,7 +3207,6 @@ void Pipeline::run(bool timed) {
       default: GPA_SWEEP(8, true); break;
       }
     }
-#undef GPA_SWEEP2
 #undef GPA_SWEEP
   }
   mark(1);
@@ -3237,14 +3270,11 @@ void Pipeline::run(bool timed) {
 // Critical-path model of the sweep, in steps (pairs) of one column,
 // measured on MI355X (bb12, clock stamps per trip and per chain step): a
 // step costs ~0.29 us at every height of a column, whole or in chunks; a
-// chunk's first trip costs kSplitTrip steps more; each map of a resumed
-// chunk's entering-state chain (its row lists, then the product) costs
-// kSplitMvFixed plus kSplitMvPer288 per 288 x 288 inputs x outputs;
-// staging a resumed chunk's inputs costs kSplitStage, and the second
-// launch's start (the gap behind the first launch and its own set-up)
-// kSplitLaunch.
-constexpr double kSplitTrip = 1.0, kSplitMvFixed = 4.5, kSplitMvPer288 = 0.5, kSplitStage = 4.5,
-                 kSplitLaunch = 13.0;
+// chunk's first trip costs kSplitTrip steps more; a resumed chunk starts
+// kSplitWake steps after the last producer of its column ends (the done
+// count, its wait, the staging of its inputs), and each map of its chain
+// costs kSplitMvFixed plus kSplitMvPer288 per 288 x 288 inputs x outputs.
+constexpr double kSplitTrip = 1.0, kSplitWake = 18.0, kSplitMvFixed = 1.7, kSplitMvPer288 = 0.2;
 constexpr uint32_t kClassWideMaxTiles = 2048;  // class workgroups up to which the 16-column walk is used
 // boundaries a split plan chooses among (evenly thinned to this many)
 constexpr size_t kSplitMaxCandidates = 16;
@@ -3278,15 +3308,13 @@ static std::vector<uint32_t> livePositions(const Pipeline::Impl &m, const VDesc 
 // whole by 10 %.  A plan's boundaries are chosen among the pairs whose
 // boundary has the fewest live positions (the kSplitMaxCandidates / 2
 // fewest and every boundary with as few), every combination of K - 1 of
-// them for K = 3 .. kSplitMaxChunks; its estimate is the first launch (the top chunk or the longest middle
-// chunk's maps) plus the second launch's start plus the second launch (the
-// longest of: a lower chunk with its staging and matvecs), with the other
-// columns (bounded by the next longest band) in the launch where they cost
-// least.  Writes the items and descriptors to the device; the results do
-// not depend on the plan.
+// them for K = 3 .. kSplitMaxChunks; its estimate is the longest of the
+// producers (the top chunk, a middle chunk's maps), then the longest of the
+// resumed chunks with their wake-up and chain, against the next longest
+// band.  Writes the items and descriptors to the device; the results do not
+// depend on the plan.
 static void planSweep(Pipeline::Impl &m) {
   m.nSplit = m.nSplit1 = m.nSplit3 = 0;
-  m.nPlain1 = m.nItems;
   if (!m.Ccols) return;
   std::vector<int2> bands(m.Ccols);
   GPA_HIP(hipMemcpy(bands.data(), m.band.p, 8ull * m.Ccols, hipMemcpyDeviceToHost));
@@ -3347,12 +3375,9 @@ static void planSweep(Pipeline::Impl &m) {
     SplitDesc sd{};
     std::vector<std::vector<uint32_t>> live;  // [k] for boundaries 1 .. K-1
     double est = 0;
-    bool otherFirst = true;  // the other columns in the first launch
-    bool seeds2 = false;     // launch 2 holds the seeds in LDS
-    size_t lds1 = 0, lds2 = 0;
+    size_t lds = 0;
   };
   std::vector<Plan> plans;
-  int detsFirst = -1;  // where the plain columns go (decided by the first split column)
   for (size_t i = 0; i < cols.size() && plans.size() < kSplitMaxColumns; ++i) {
     const Col &col = cols[i];
     if (col.lo > 3) {
@@ -3395,38 +3420,23 @@ static void planSweep(Pipeline::Impl &m) {
         bd[0] = 0;
         bd[K] = top + 1;
         for (int k = 1; k < K; ++k) bd[k] = Q[ix[k - 1]];
-        double l1 = bd[K] - bd[K - 1], l2 = 0;
-        for (int k = 1; k <= K - 2; ++k) l1 = std::max<double>(l1, bd[k + 1] - bd[k]);
-        l1 += kSplitTrip;
+        double prod = bd[K] - bd[K - 1], res = 0;
+        for (int k = 1; k <= K - 2; ++k) prod = std::max<double>(prod, bd[k + 1] - bd[k]);
+        prod += kSplitTrip;
         double chain = 0;
-        size_t mapWords = 0, mapWords0 = 0;
+        uint32_t mapWords = quad(nl[bd[K - 1]]);
         for (int k = K - 2; k >= 0; --k) {
-          l2 = std::max(l2, kSplitStage + chain + (bd[k + 1] - bd[k]) + kSplitTrip);
+          res = std::max(res, chain + (bd[k + 1] - bd[k]) + kSplitTrip);
           if (k >= 1) {
             chain += kSplitMvFixed + kSplitMvPer288 * nl[bd[k + 1]] * nl[bd[k]] / (288.0 * 288.0);
-            mapWords += splitTWords(nl[bd[k + 1]], nl[bd[k]]) + quad(nl[bd[k]]);
+            mapWords += splitMapWords(nl[bd[k + 1]], nl[bd[k]]);
           }
         }
-        mapWords0 = mapWords + quad(nl[bd[K - 1]]);
-        uint32_t listWords0 = 0;
-        for (int k = 1; k <= K - 2; ++k) listWords0 += rowListWords(nl[bd[k]]);
-        const double other = rest + kSplitTrip;
-        const double estA = std::max(l1, other) + kSplitLaunch + l2;  // others in launch 1
-        const double estB = l1 + kSplitLaunch + std::max(l2, other);  // others in launch 2
-        const size_t chainB = 4ull * splitChainWords(P, static_cast<uint32_t>(mapWords0), listWords0);
-        const size_t lds2s = al16(ring + seedB + 8) + chainB, lds2g = al16(ring + 8) + chainB;
-        const bool seeds2 = m.seedsInLds && lds2s <= kLdsBudget;
-        bool ok = seeds2 || lds2g <= kLdsBudget;
-        // the other columns go to launch 2 only with their seeds in LDS
-        const bool firstOk = detsFirst != 0, secondOk = detsFirst != 1 && seeds2;
-        double est = 1e300;
-        bool otherFirst = true;
-        if (firstOk) est = estA;
-        if (secondOk && estB < est) {
-          est = estB;
-          otherFirst = false;
-        }
-        if (!firstOk && !secondOk) ok = false;
+        // (one launch: a resumed chunk holds the ring, the seeds when the
+        // launch keeps them in LDS, and the chain)
+        const size_t lds = al16(ring + (m.seedsInLds ? seedB : 0) + 8) + 4ull * splitChainWords(P, mapWords);
+        const bool ok = lds <= kLdsBudget;
+        double est = std::max(prod + kSplitWake + res, rest + kSplitTrip);
         if (force) {
           // most chunks, then the shortest longest chunk
           int32_t longest = 0;
@@ -3436,15 +3446,12 @@ static void planSweep(Pipeline::Impl &m) {
         const bool better = force ? est > best.est : est < best.est;
         if (ok && better) {
           best.est = est;
-          best.otherFirst = otherFirst;
-          best.seeds2 = seeds2;
           best.sd = SplitDesc{};
           best.sd.v = col.v;
           best.sd.c = col.c;
           best.sd.K = static_cast<uint32_t>(K);
           for (int k = 0; k <= K; ++k) best.sd.bound[k] = bd[k];
-          best.lds1 = al16(ring + (m.seedsInLds ? seedB : 0) + 8) + 4ull * quad(P);
-          best.lds2 = seeds2 ? lds2s : lds2g;
+          best.lds = lds;
           bestK = K;
         }
         // next combination
@@ -3459,7 +3466,6 @@ static void planSweep(Pipeline::Impl &m) {
       if (force) continue;
       break;
     }
-    if (detsFirst < 0) detsFirst = best.otherFirst ? 1 : 0;
     best.live.resize(bestK);
     for (int k = 1; k < bestK; ++k) {
       best.live[k] = livePositions(m, d, best.sd.bound[k]);
@@ -3476,8 +3482,7 @@ static void planSweep(Pipeline::Impl &m) {
   std::vector<uint4> items1, items3;
   std::vector<uint32_t> live;
   uint64_t stateWords = 0;
-  size_t lds1 = m.sweepLdsBytes, lds2 = detsFirst ? 0 : m.sweepLdsBytes;
-  bool gseeds2 = false;
+  size_t lds = m.sweepLdsBytes;
   for (Plan &pl : plans) {
     SplitDesc &sd = pl.sd;
     const uint32_t K = sd.K, si = static_cast<uint32_t>(descs.size());
@@ -3488,9 +3493,11 @@ static void planSweep(Pipeline::Impl &m) {
     }
     sd.stateOff = static_cast<uint32_t>(stateWords);
     stateWords += quad(sd.nLive[K - 1]);
+    sd.producers = 1;
     for (uint32_t k = 1; k + 2 <= K; ++k) {
       sd.tOff[k] = static_cast<uint32_t>(stateWords);
-      stateWords += splitTWords(sd.nLive[k + 1], sd.nLive[k]) + quad(sd.nLive[k]);
+      stateWords += splitMapWords(sd.nLive[k + 1], sd.nLive[k]);
+      sd.producers += (sd.nLive[k + 1] + 31) / 32 + 1;
     }
     sd.tOff[K - 1] = sd.stateEnd = static_cast<uint32_t>(stateWords);
     if (stateWords >= 0xFFFFFFF0ull) break;
@@ -3500,20 +3507,20 @@ static void planSweep(Pipeline::Impl &m) {
       items1.push_back(make_uint4(si, kSplitPart | k << 8, sd.v, sd.c));
     }
     for (uint32_t k = 0; k + 2 <= K; ++k) items3.push_back(make_uint4(si, kSplitResume | k << 8, sd.v, sd.c));
-    lds1 = std::max(lds1, pl.lds1);
-    lds2 = std::max(lds2, pl.lds2);
-    gseeds2 = gseeds2 || !pl.seeds2;
+    lds = std::max(lds, pl.lds);
     descs.push_back(sd);
   }
   auto al = [](size_t x) { return (x + 255) & ~static_cast<size_t>(255); };
-  m.oSplitItems1 = al(sizeof(SplitDesc) * descs.size());
-  m.oSplitItems3 = al(m.oSplitItems1 + 16 * items1.size());
-  m.oSplitLive = al(m.oSplitItems3 + 16 * items3.size());
-  const size_t bytes = m.oSplitLive + 4 * live.size() + 16;
+  // descriptors | maps' and parts' items, then the resumed chunks' | live
+  // positions | sync words (zero: the runs leave them zero)
+  m.oSplitItems = al(sizeof(SplitDesc) * descs.size());
+  m.oSplitLive = al(m.oSplitItems + 16 * (items1.size() + items3.size()));
+  m.oSplitSync = al(m.oSplitLive + 4 * live.size());
+  const size_t bytes = m.oSplitSync + 8 * kSplitMaxColumns;
   std::vector<char> h(bytes, 0);
   std::memcpy(h.data(), descs.data(), sizeof(SplitDesc) * descs.size());
-  std::memcpy(h.data() + m.oSplitItems1, items1.data(), 16 * items1.size());
-  std::memcpy(h.data() + m.oSplitItems3, items3.data(), 16 * items3.size());
+  std::memcpy(h.data() + m.oSplitItems, items1.data(), 16 * items1.size());
+  std::memcpy(h.data() + m.oSplitItems + 16 * items1.size(), items3.data(), 16 * items3.size());
   std::memcpy(h.data() + m.oSplitLive, live.data(), 4 * live.size());
   m.dropGraph();  // before the buffers it may name are replaced
   // a split column's band is reported from layer 0 (its top chunk does not
@@ -3530,6 +3537,9 @@ static void planSweep(Pipeline::Impl &m) {
   m.splitBuf.ensure(bytes, false, "split sweep");
   m.splitState.ensure(4 * stateWords + 16, false, "split sweep");
   GPA_HIP(hipMemcpy(m.splitBuf.p, h.data(), bytes, hipMemcpyHostToDevice));
+  // the maps' dense flags and row counts start at zero (each run's last
+  // resumed chunk clears them again)
+  GPA_HIP(hipMemset(m.splitState.p, 0, 4 * stateWords + 16));
   // the split columns' own items become their top chunks (in the staged
   // inputs, which later compiles upload, and on the device now), launched
   // first
@@ -3542,11 +3552,8 @@ static void planSweep(Pipeline::Impl &m) {
   std::stable_partition(m.hostItems.begin(), m.hostItems.end(),
                         [](const uint4 &it) { return (it.y & kItemSplit) != 0; });
   writeItems();
-  m.nPlain1 = detsFirst ? m.nItems : static_cast<uint32_t>(descs.size());
-  m.split2Gseeds = gseeds2;
-  m.split1LdsBytes = lds1;
-  m.splitLdsBytes = lds2;
-  setSweepLds(std::max(lds1, lds2));
+  m.splitLdsBytes = lds;
+  setSweepLds(lds);
   m.nSplit = static_cast<uint32_t>(descs.size());
   m.nSplit1 = static_cast<uint32_t>(items1.size());
   m.nSplit3 = static_cast<uint32_t>(items3.size());
@@ -3601,6 +3608,7 @@ void Pipeline::download() {
   }
   const uint32_t *vb = m.hVB.as<uint32_t>();
   const uint32_t *fl = m.hFlags.as<uint32_t>();
+  if (fl[kFlagSplitWait]) throw RuntimeError("time-chunked sweep: a chunk's wait for its inputs timed out");
   if (fl[kFlagBandOverflow]) {
     // a column's band reached below its stored rows (cut there, so this
     // run's classes are wrong): the columns cut at their base store every
