@@ -2585,6 +2585,10 @@ struct Pipeline::Impl {
   // (splitBuf: descriptors | launch-1 items | launch-2 items | per-column
   // split index; splitState: the split columns' state areas)
   bool splitPlanned = false;
+  // planned from the first run's bands at the start of the load's second
+  // run (outside any graph capture): a load compiled once never pays for it
+  bool planPending = false;
+  bool capturing = false;
   // columns the class stage loads per round trip (4; 16 once a run has shown
   // nodes see more than 12 columns on average, in loads of few workgroups)
   uint32_t classBatch = 4;
@@ -2843,6 +2847,7 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   m.launchMode = -1;
   m.stashShift = 4;
   m.splitPlanned = false;
+  m.planPending = false;
   m.classBatch = 4;
   m.nSplit = m.nSplit1 = m.nSplit3 = 0;
   const bool frozen = m.frozen;
@@ -3156,9 +3161,15 @@ void Pipeline::upload() {
   m.uploadPending = true;
 }
 
+static void planSweep(Pipeline::Impl &m);
+
 void Pipeline::run(bool timed) {
   Impl &m = *impl_;
   GPA_HIP(hipSetDevice(device_));
+  if (m.planPending && !m.capturing) {
+    m.planPending = false;
+    planSweep(m);
+  }
   char *ib = m.in.as<char>();
   const VDesc *desc = reinterpret_cast<const VDesc *>(ib + m.oDesc);
   const uint4 *items = reinterpret_cast<const uint4 *>(ib + m.oItems);
@@ -3299,7 +3310,8 @@ static std::vector<uint32_t> livePositions(const Pipeline::Impl &m, const VDesc 
   return out;
 }
 
-// Runs after a load's first run, from the column bands it found.  The
+// Runs at the start of a load's second run (outside graph capture), from
+// the column bands its first run found.  The
 // plain items are put in the order of their measured bands, longest first
 // (the launch order of the sweep's workgroups; the first run orders them by
 // seed range, which misses the observables' long band).  Then the
@@ -3665,7 +3677,7 @@ void Pipeline::download() {
   }
   if (!m.splitPlanned) {
     m.splitPlanned = true;
-    planSweep(m);
+    m.planPending = true;
   }
   m.lastC = C;
   m.lastT = T;
@@ -3719,11 +3731,18 @@ void Pipeline::runGraph() {
   Impl &m = *impl_;
   GPA_HIP(hipSetDevice(device_));
   m.joinSide();
+  if (m.planPending) {
+    m.planPending = false;
+    planSweep(m);
+  }
   if (!m.graphExec) {
     GPA_HIP(hipStreamBeginCapture(m.stream, hipStreamCaptureModeThreadLocal));
+    m.capturing = true;
     try {
       run(false);
+      m.capturing = false;
     } catch (...) {
+      m.capturing = false;
       hipGraph_t g = nullptr;
       (void)hipStreamEndCapture(m.stream, &g);
       if (g) (void)hipGraphDestroy(g);

@@ -369,11 +369,13 @@ def sweep_split():
 
 
 def _compile_twice(text):
-    """A driver's DEM and sensitivities after its second compile (the first
-    compile of a load plans the split sweep; the second runs it)."""
+    """A driver's DEM and sensitivities after its second compile of one load
+    (the first compile finds the column bands; the second plans the split
+    sweep from them and runs it).  The compiles pass no circuit: a compile
+    given circuit text loads it anew, and a new load starts unplanned."""
     drv = get_driver(text)
-    drv.compile_detector_error_model(text)
-    dem = drv.compile_detector_error_model(text)
+    drv.compile_detector_error_model()
+    dem = drv.compile_detector_error_model()
     idx, sens = drv.debug_sensitivities()
     return dem, idx, sens, drv.last_hashes(), drv.debug_sweep_plan()
 
@@ -419,20 +421,51 @@ def test_split_sweep_planned_for_a_long_memory(sweep_split, d, rounds):
     assert split[0] == whole[0]
 
 
+def test_split_sweep_default_plan_bb12(oracle):
+    """As planned by default, bb12's observables column is swept in chunks
+    whose lower chunks wait for the maps of the chunks above them in the same
+    launch; the DEM, its order (the oracle's class hashes) and every
+    sensitivity equal those of the whole sweep."""
+    from greenpeas_amd import autotune
+
+    text = read_circuit("bb12")
+    split = _compile_twice(text)
+    assert split[4][0] == 1 and split[4][2] >= 2
+    autotune.set_tuning("sweep_split", 0)
+    try:
+        whole = _compile_twice(text)
+    finally:
+        autotune.set_tuning("sweep_split", 1)
+    assert whole[4] == (0, 0, 0)
+    assert np.array_equal(split[2], whole[2]) and np.array_equal(split[3], whole[3])
+    assert split[0] == whole[0]
+    ref = oracle.compile_stepg(text, 2)
+    assert np.array_equal(split[3], ref.hashes)
+    assert np.max(np.abs(split[0].probs - ref.probs) / ref.probs) <= REL
+
+
 def test_split_sweep_repeated_compiles_and_reload(sweep_split):
-    """A driver keeps its split plan over many compiles and re-plans after a
-    load of another circuit; every result equals the first."""
+    """A driver keeps its split plan over many compiles of a load (the
+    producers' counts and row lists are cleared by each run for the next) and
+    plans anew after a load of another circuit; every result equals the first
+    (the load's first compile sweeps every column whole)."""
     a, b = read_circuit("bb12"), read_circuit("surface5")
     sweep_split(2)
     drv = get_driver(a)
-    first = drv.compile_detector_error_model(a)
+    first = drv.compile_detector_error_model()
     for _ in range(4):
-        assert drv.compile_detector_error_model(a) == first
+        assert drv.compile_detector_error_model() == first
     assert drv.debug_sweep_plan()[0] >= 1
+    drv.bench_device(3)  # graph replays and direct launches of the split plan
+    assert drv.compile_detector_error_model() == first
     small = drv.compile_detector_error_model(b)
-    assert drv.compile_detector_error_model(b) == small
+    assert drv.debug_sweep_plan() == (0, 0, 0)  # a new load: not planned yet
+    assert drv.compile_detector_error_model() == small
+    assert drv.debug_sweep_plan()[0] >= 1
+    assert drv.compile_detector_error_model() == small
     assert drv.compile_detector_error_model(a) == first
-    assert drv.compile_detector_error_model(a) == first
+    assert drv.compile_detector_error_model() == first
+    assert drv.compile_detector_error_model() == first
 
 
 @pytest.mark.parametrize("name", ["iceberg_surface_4_adaptive_circl", "iceberg_surface_6_adaptive_pheno"])
