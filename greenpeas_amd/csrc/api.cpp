@@ -966,6 +966,7 @@ int gp_set_tuning(const char *name, uint32_t value) {
     else if (std::string(name) == "packed_sort") gpa::setPackedSort(value != 0);
     else if (std::string(name) == "s_bands") gpa::setSBands(value);
     else if (std::string(name) == "bucket_cap") gpa::setBucketCap(value);
+    else if (std::string(name) == "target_fill") gpa::setTargetFill(value);
     else if (std::string(name) == "sweep_split") {
       if (value > 2) throw InvalidArgument("sweep_split must be 0, 1 or 2");
       gpa::setSweepSplit(value);
@@ -985,6 +986,7 @@ int gp_get_tuning(const char *name, uint32_t *value) {
     else if (std::string(name) == "packed_sort") *value = gpa::packedSort() ? 1u : 0u;
     else if (std::string(name) == "s_bands") *value = gpa::sBands();
     else if (std::string(name) == "bucket_cap") *value = gpa::bucketCap();
+    else if (std::string(name) == "target_fill") *value = gpa::targetFill();
     else if (std::string(name) == "sweep_split") *value = gpa::sweepSplit();
     else throw InvalidArgument(std::string("unknown tuning parameter: ") + name);
   });

@@ -39,6 +39,7 @@ constexpr int kScanTile = 4 * kScanBlock;
 constexpr int kBucketItemsPerWave = 256;  // wave-register sort capacity (4 per lane)
 std::atomic<uint32_t> gBucketFill{0};     // mean items per bucket the digit width aims at (0: by load size)
 std::atomic<uint32_t> gBucketCap{256};    // slots per sort bucket (64, 128 or 256)
+std::atomic<uint32_t> gTargetFill{4};     // target slots per error node a growable load starts with
 std::atomic<uint32_t> gStash{1};          // target rows on (1) or off (0)
 std::atomic<uint32_t> gMaskTable{1};      // classify's LDS mask tables allowed (1) or not (0)
 std::atomic<uint32_t> gSweepSplit{1};     // long columns swept in time chunks (1; 2: always) or whole (0)
@@ -2683,7 +2684,10 @@ static uint32_t bucketBitsFor(const StepgImage &g, uint32_t fallback) {
 }
 
 // Slots per sort bucket of a load: the largest tuned value among its
-// circuits' size classes (bucket_cap where a class has none).
+// circuits' size classes (bucket_cap where a class has none).  (Measured on
+// the 256-variant batch: 128 slots at a mean fill of <= 96 leave the
+// buckets of big classes -- many nodes with one hash -- to the oversized
+// path, bucket stage 1.45 -> 6.5 ms; every load keeps kBucketCap.)
 static uint32_t bucketCapFor(const std::vector<const StepgImage *> &imgs) {
   uint32_t cap = 0;
   for (const StepgImage *g : imgs)
@@ -3008,8 +3012,12 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   workBufferSizes(R, B, V, m.Ccols, m.countStride, m.bucketCap,
                   [&](uint32_t k, uint64_t bytes, const char *what) { wa.request(*bufs[k], bytes, what); });
   wa.commit(frozen, growable_ ? kGrowHeadroom : 1.0);
-  if (m.capTargets == 0 || (!frozen && m.capTargets < 16 * R)) {
-    m.outTargets.ensure(4 * 16 * R, frozen, "targets");
+  // targets: 16 per error node for a driver (its capacity is fixed), a
+  // growable load starts with target_fill per node and grows (running again)
+  // if its DEM holds more
+  const uint64_t tgtFill = frozen ? 16u : gTargetFill.load();
+  if (m.capTargets == 0 || (!frozen && m.capTargets < tgtFill * R)) {
+    m.outTargets.ensure(4 * tgtFill * R, frozen, "targets");
     m.capTargets = m.outTargets.cap / 4;
   }
   m.hVB.ensure(4 * (2 * V + 8));
@@ -3837,6 +3845,11 @@ void setBucketCap(uint32_t cap) {
   gBucketCap.store(cap);
 }
 uint32_t bucketCap() { return gBucketCap.load(); }
+void setTargetFill(uint32_t n) {
+  if (n < 1 || n > 16) throw InvalidArgument("target_fill must be between 1 and 16");
+  gTargetFill.store(n);
+}
+uint32_t targetFill() { return gTargetFill.load(); }
 uint64_t Pipeline::sWords() const { return impl_->Swords; }
 bool packedSort() { return gPackedSort.load() != 0; }
 bool maskTable() { return gMaskTable.load() != 0; }

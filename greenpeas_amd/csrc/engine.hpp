@@ -183,10 +183,15 @@ uint32_t sBands();
 // first).
 void setBucketCap(uint32_t cap);
 uint32_t bucketCap();
-// Whether a load's long columns may be swept in time chunks after its first
-// run: 1 (default) where the estimated critical path gains, 2 every column
-// whose band reaches the bottom with the most chunks that fit (for tests), 0
-// never (every column swept whole).  Read at that run.
+// Target slots per error node a growable load's output buffer starts with
+// (1..16, default 4); a DEM with more targets grows it and runs again.
+void setTargetFill(uint32_t n);
+uint32_t targetFill();
+// Whether a load's long columns may be swept in time chunks from its second
+// run on (planned from the bands of its first): 1 (default) where the
+// estimated critical path gains, 2 every column whose band reaches the
+// bottom with the most chunks that fit (for tests), 0 never (every column
+// swept whole).  Read when the plan is made.
 void setSweepSplit(uint32_t mode);
 uint32_t sweepSplit();
 

@@ -323,17 +323,21 @@ int gp_device_memory(int device, uint64_t *free_bytes, uint64_t *total_bytes);
  *   setting); same results; read at each load.  Drivers always store every
  *   layer (their capacity is fixed at creation).
  *   "bucket_cap": slots per sort bucket, 64, 128 or 256 (default 256):
- *   classify writes a bucket's first bucket_cap nodes to its slots and the
- *   rest to an overflow list that the bucket's workgroup sorts; slot
- *   storage is buckets x bucket_cap + nodes; with bucket_fill it sets the
- *   shape of the hash table; same results; read at each load (a driver
- *   keeps the value it was created with).
- *   "sweep_split": 1 (default) lets a load's first compile plan a
- *   time-chunked sweep of columns much longer than the rest (transfer maps
- *   of the middle chunks, then the lower chunks in parallel; identical
- *   results); 2 splits every column whose band reaches the first layers
- *   into as many chunks as fit (a test setting); 0 sweeps every column
- *   whole; read at a load's first compile. */
+ *   classify writes a bucket's first bucket_cap nodes to
+ *   its slots and the rest to an overflow list that the bucket's workgroup
+ *   sorts; slot storage is buckets x bucket_cap + nodes; with bucket_fill it
+ *   sets the shape of the hash table; same results; read at each load (a
+ *   driver keeps the value it was created with).
+ *   "target_fill": target slots per error node that a batch's (growable)
+ *   output buffer starts with, 1..16 (default 4); a DEM with more targets
+ *   grows the buffer and compiles again; same results.  Drivers hold 16.
+ *   "sweep_split": 1 (default) lets a load's second compile plan, from the
+ *   bands its first compile found, a time-chunked sweep of a column much
+ *   longer than the rest (cut where few nodes are live; the middle chunks'
+ *   transfer maps and the top chunk first, the lower chunks waiting for
+ *   them in the same launch; identical results); 2 splits every column whose
+ *   band reaches the first layers into as many chunks as fit (a test
+ *   setting); 0 sweeps every column whole; read when the plan is made. */
 int gp_set_tuning(const char *name, uint32_t value);
 int gp_get_tuning(const char *name, uint32_t *value);
 /* Tuned knobs per size class (what an autotuner installs): circuits at

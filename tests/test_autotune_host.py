@@ -60,6 +60,20 @@ def test_bucket_cap_knob():
         autotune.set_tuning("bucket_cap", old)
 
 
+def test_target_fill_knob():
+    old = autotune.get_tuning("target_fill")
+    assert old == 4
+    try:
+        for v in (1, 16, 4):
+            autotune.set_tuning("target_fill", v)
+            assert autotune.get_tuning("target_fill") == v
+        for bad in (0, 17):
+            with pytest.raises(ValueError):
+                autotune.set_tuning("target_fill", bad)
+    finally:
+        autotune.set_tuning("target_fill", old)
+
+
 def test_install_cached_hands_the_cache_to_the_library(tmp_path, monkeypatch):
     cache = tmp_path / "tune.json"
     cache.write_text(json.dumps({

@@ -183,3 +183,29 @@ def test_sensitivity_layouts_give_identical_dems():
     finally:
         autotune.set_tuning("s_bands", 1)
     assert words[1] < words[0]
+
+
+def test_target_buffer_grows_from_its_start_size():
+    """A batch's target buffer starts with `target_fill` slots per error
+    node and grows (compiling again) when the DEM holds more: with 1 slot
+    per node every circuit below needs the growth; the DEMs equal those of
+    the default start."""
+    from greenpeas_amd import autotune
+
+    texts = [read_circuit(n) for n in ("bb12", "surface5", "bb6")]
+    bc = BatchCompiler()
+    bc.set_circuits(texts)
+    ref = bc.compile()
+    old = autotune.get_tuning("target_fill")
+    try:
+        autotune.set_tuning("target_fill", 1)
+        small = BatchCompiler()
+        small.set_circuits(texts)
+        grown = small.compile()
+        again = small.compile()
+        small.close()
+    finally:
+        autotune.set_tuning("target_fill", old)
+    bc.close()
+    for a, b, c in zip(ref, grown, again):
+        assert a == b and a == c
