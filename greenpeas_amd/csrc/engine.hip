@@ -40,6 +40,7 @@ constexpr int kBucketItemsPerWave = 256;  // wave-register sort capacity (4 per 
 std::atomic<uint32_t> gBucketFill{0};     // mean items per bucket the digit width aims at (0: by load size)
 std::atomic<uint32_t> gBucketCap{256};    // slots per sort bucket (64, 128 or 256)
 std::atomic<uint32_t> gTargetFill{4};     // target slots per error node a growable load starts with
+std::atomic<uint32_t> gCompactItems{1};   // growable loads' sorted items and runs packed by bucket: 1 large loads, 2 all, 0 none
 std::atomic<uint32_t> gStash{1};          // target rows on (1) or off (0)
 std::atomic<uint32_t> gMaskTable{1};      // classify's LDS mask tables allowed (1) or not (0)
 std::atomic<uint32_t> gSweepSplit{1};     // long columns swept in time chunks (1; 2: always) or whole (0)
@@ -1485,7 +1486,7 @@ __device__ __forceinline__ T prevElement(const T (&v)[R], int s, uint32_t lane, 
 // latency passes during the sort; slots and records then meet in the wave's
 // LDS by slot position, where the merge reads them in sorted order.
 template <int R>
-__device__ void bucketWave(uint32_t b, uint32_t bs, uint32_t cnt, uint32_t lane, const uint4 first,
+__device__ void bucketWave(uint32_t b, uint32_t bs, uint32_t ob, uint32_t cnt, uint32_t lane, const uint4 first,
                            const uint4 *__restrict__ pslot, const uint4 *__restrict__ info, SlotRec *pay,
                            const RunOut &o, uint32_t *__restrict__ runCount, uint32_t *__restrict__ runNnz,
                            uint32_t *tot) {
@@ -1593,7 +1594,7 @@ __device__ void bucketWave(uint32_t b, uint32_t bs, uint32_t cnt, uint32_t lane,
     waveLdsSync();
   }
   RunCarry cy{0.0, 0ull, 0u, 0u, 0u, 0u, 0u};
-  scanRuns<R>(h, r, pv, h2v, nzv, cnt, 0u, bs, cy, lane, o, LdsRunValues{vals, vals + 64 * R});
+  scanRuns<R>(h, r, pv, h2v, nzv, cnt, 0u, ob, cy, lane, o, LdsRunValues{vals, vals + 64 * R});
   if (lane == 0) {
     runCount[b] = cy.runs;
     runNnz[b] = cy.nnzSum;
@@ -1610,12 +1611,14 @@ struct BucketIo {
   const uint4 *ovSlot;    // overflow list (count in flags[3]) and each entry's bucket
   const uint32_t *ovBucket;
   uint32_t *runCount, *runNnz, *itemCount;  // per bucket
-  uint32_t *slotBase;     // per bucket: where its sorted items and runs start
+  uint32_t *slotBase;     // per bucket: where its sorted items and runs start (compact: set by the
+                          // scan of the bucket counts before this stage)
   uint32_t *wgSum;        // per workgroup: its buckets' runs, targets, items
   uint64_t *scratchKey;   // sort workspace for oversized buckets (2 * arenaSize)
   uint32_t *scratchRank;
   uint32_t arenaBase;     // numBuckets * cap: oversized buckets' items and runs follow
   uint32_t cap;           // slots per bucket (at most kBucketCap)
+  uint32_t compact;       // items and runs packed by bucket at slotBase (no arena)
 };
 
 // An oversized bucket (more than kBucketCap items), by the whole workgroup:
@@ -1657,7 +1660,9 @@ __device__ void bigBucket(uint32_t b, uint32_t cnt, const BucketIo &io, const Ru
   __shared__ double hsSlots[32];
   const uint32_t lane = threadIdx.x & 63;
   if (threadIdx.x == 0) {
-    shOff = atomicAdd(&o.flags[5], cnt);  // total items never exceed the arena
+    // its range: packed by bucket, or taken from the arena (total items
+    // never exceed it)
+    shOff = io.compact ? io.slotBase[b] : atomicAdd(&o.flags[5], cnt);
     shFill = io.cap;
   }
   __syncthreads();
@@ -1703,7 +1708,7 @@ __device__ void bigBucket(uint32_t b, uint32_t cnt, const BucketIo &io, const Ru
   } else {
     bitonicWG(K, Rk, P);
   }
-  const uint32_t base = io.arenaBase + off;
+  const uint32_t base = io.compact ? off : io.arenaBase + off;
   if (threadIdx.x < 64) {
     RunCarry cy{0.0, 0ull, 0u, 0u, 0u, 0u, 0u};
     // members' values for runs across chunks: from the sorted ranks' records
@@ -1750,28 +1755,29 @@ __global__ __launch_bounds__(256) void bucketKernel(uint32_t numBuckets, BucketI
   if (b < numBuckets) {
     const uint32_t bs = b * io.cap;
     // the bucket's first 64 slots travel with its counter (slots past the
-    // count hold stale records and are never used)
+    // count hold stale records and are never used), and its packed range
     const uint4 first = io.pslot[bs + lane];
     uint32_t *ctr = io.bucketCount + static_cast<uint64_t>(b) * io.countStride;
     const uint32_t cnt = __builtin_amdgcn_readfirstlane(*ctr);
+    const uint32_t ob = io.compact ? __builtin_amdgcn_readfirstlane(io.slotBase[b]) : bs;
     if (lane == 0) {
       *ctr = 0u;
       io.itemCount[b] = cnt;
       tot[wave][2] = cnt;
     }
     if (cnt <= io.cap) {
-      if (lane == 0) io.slotBase[b] = bs;
+      if (lane == 0 && !io.compact) io.slotBase[b] = bs;
       if (cnt == 0) {
         if (lane == 0) {
           io.runCount[b] = 0;
           io.runNnz[b] = 0;
         }
       } else if (cnt <= 64) {
-        bucketWave<1>(b, bs, cnt, lane, first, io.pslot, io.info, pay[wave], o, io.runCount, io.runNnz, tot[wave]);
+        bucketWave<1>(b, bs, ob, cnt, lane, first, io.pslot, io.info, pay[wave], o, io.runCount, io.runNnz, tot[wave]);
       } else if (cnt <= 128) {
-        bucketWave<2>(b, bs, cnt, lane, first, io.pslot, io.info, pay[wave], o, io.runCount, io.runNnz, tot[wave]);
+        bucketWave<2>(b, bs, ob, cnt, lane, first, io.pslot, io.info, pay[wave], o, io.runCount, io.runNnz, tot[wave]);
       } else {
-        bucketWave<4>(b, bs, cnt, lane, first, io.pslot, io.info, pay[wave], o, io.runCount, io.runNnz, tot[wave]);
+        bucketWave<4>(b, bs, ob, cnt, lane, first, io.pslot, io.info, pay[wave], o, io.runCount, io.runNnz, tot[wave]);
       }
     } else if (lane == 0) {
       const uint32_t k = atomicAdd(&nBig, 1u);
@@ -1822,7 +1828,7 @@ struct EmitIn {
   uint32_t stashShift;  // row words = 1 << stashShift
   const int2 *band;
   const int32_t *colAdj;
-  uint32_t bucketCap;  // slots per bucket
+  uint32_t bucketCap;  // slots per bucket (0: runs packed by bucket, read ahead from entry 0)
 };
 struct EmitOut {
   double *p;
@@ -2060,6 +2066,8 @@ constexpr int kScanArrays = 3;
 struct ScanArrays {
   const uint32_t *in[kScanArrays];
   uint32_t *out[kScanArrays];
+  uint32_t stride = 1;              // words between consecutive entries of the inputs
+  uint32_t count = kScanArrays;     // arrays in use (the first `count`)
 };
 
 __global__ __launch_bounds__(kScanBlock) void scanTileKernel(ScanArrays a, uint32_t n,
@@ -2072,7 +2080,7 @@ __global__ __launch_bounds__(kScanBlock) void scanTileKernel(ScanArrays a, uint3
 #pragma unroll
   for (int q = 0; q < kScanArrays; ++q)
 #pragma unroll
-    for (int k = 0; k < 4; ++k) v[q][k] = a.in[q][base + k < n ? base + k : 0u];
+    for (int k = 0; k < 4; ++k) v[q][k] = q < a.count ? a.in[q][(base + k < n ? base + k : 0u) * a.stride] : 0u;
 #pragma unroll
   for (int q = 0; q < kScanArrays; ++q) {
     t[q] = 0;
@@ -2093,14 +2101,15 @@ __global__ __launch_bounds__(kScanBlock) void scanTileKernel(ScanArrays a, uint3
       const uint32_t y = waveInclusiveSum(x, lane);
       if (lane < W) waveSums[q][lane] = y - x;
       if (lane == W - 1) {
-        if (gridDim.x == 1) a.out[q][n] = y;
-        else tileSums[kScanArrays * blockIdx.x + q] = y;
+        if (gridDim.x > 1) tileSums[kScanArrays * blockIdx.x + q] = y;
+        else if (q < a.count) a.out[q][n] = y;
       }
     }
   }
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < kScanArrays; ++q) {
+    if (q >= a.count) break;
     uint32_t r = waveSums[q][wave] + incl[q] - t[q];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -2162,6 +2171,7 @@ __global__ __launch_bounds__(kScanBlock) void scanAddKernel(ScanArrays a, uint32
   const uint32_t base = tile * kScanTile;
 #pragma unroll
   for (int q = 0; q < kScanArrays; ++q) {
+    if (q >= a.count) break;
     const uint32_t add = tileOff[kScanArrays * tile + q];
     for (uint32_t k = threadIdx.x; k < kScanTile; k += kScanBlock)
       if (base + k < n) a.out[q][base + k] += add;
@@ -2531,17 +2541,35 @@ struct Pipeline::Impl {
   // serialize) and one counter per 128-byte line (four times the lines)
   uint32_t countStride = 1;
 
-  // per node: stash, info (probability, second hash, class size), overflow
-  // list (ovSlot/ovBucket);
-  // per bucket slot (numBuckets * kBucketCap, plus a numRanks arena for
-  // oversized buckets): pslot (classify's records), prank (the sorted
-  // ranks) and the run arrays;
+  // per node: info (probability, second hash, class size), overflow list
+  // (ovSlot/ovBucket); stash (target rows, its own allocation: its width
+  // grows with a load's largest class);
+  // per bucket slot (numBuckets * bucketCap): pslot (classify's records);
+  // per sorted item, at the bucket's slots plus an arena of numRanks for
+  // oversized buckets or, in a compact load, packed by bucket (numRanks in
+  // all): prank (the sorted ranks) and the run arrays;
   // per bucket: counters, slotBase, item/run/target counts and their scans
   DevBuf band, stash, info, counters, bucketCnt, ovSlot, ovBucket, pslot, prank,
       runRank, runP, runHash, runTgtPre, runMember, runCount, runNnz, itemCount, slotBase, wgSum, classStart,
       tgtStart, memberStart, outP, outOff, outHash, outMember, outTargets, scratchKey, scratchRank,
       scanTmp, vBounds, members;
-  WorkArena work;  // holds the buffers above except in, outTargets, members and the split buffers
+  WorkArena work;  // holds the buffers above except stash, in, outTargets, members and the split buffers
+  // A compact load places each bucket's sorted items and runs at the
+  // exclusive prefix sum of the bucket counts (a scan before the bucket
+  // stage) instead of at its slots, and its outputs (p, hash, offsets,
+  // members) in pslot's space, which is dead once the bucket stage has run.
+  bool compact = false;
+  uint64_t aliasHash = 0, aliasOff = 0, aliasMember = 0;
+  double *oP() const { return compact ? pslot.as<double>() : outP.as<double>(); }
+  uint64_t *oHash() const {
+    return compact ? reinterpret_cast<uint64_t *>(pslot.as<char>() + aliasHash) : outHash.as<uint64_t>();
+  }
+  uint32_t *oOff() const {
+    return compact ? reinterpret_cast<uint32_t *>(pslot.as<char>() + aliasOff) : outOff.as<uint32_t>();
+  }
+  uint32_t *oMember() const {
+    return compact ? reinterpret_cast<uint32_t *>(pslot.as<char>() + aliasMember) : outMember.as<uint32_t>();
+  }
   uint64_t capTargets = 0;
   // S (its own allocation: it grows when a band needs more rows) and its
   // per-column layout (see colRegion)
@@ -2557,8 +2585,8 @@ struct Pipeline::Impl {
   uint32_t bucketCap = kBucketCap;  // slots per sort bucket of the load
   bool uniformCols = false;  // every column of a circuit stores all its layers in regions of one size
   // the arena's buffers, in the order workBufferSizes() sizes them
-  std::array<DevBuf *, 30> workBufs() {
-    return {&band,                    &info,        &stash,      &ovSlot,      &ovBucket,  &counters, &bucketCnt,
+  std::array<DevBuf *, 29> workBufs() {
+    return {&band,                    &info,                     &ovSlot,      &ovBucket,  &counters, &bucketCnt,
             &pslot,     &prank,      &runRank,     &runP,       &runHash,     &runTgtPre, &runMember, &runCount,
             &runNnz,    &itemCount,  &slotBase,    &wgSum,      &classStart,  &tgtStart,  &memberStart, &outP,
             &outOff,    &outHash,    &outMember,   &scratchKey, &scratchRank, &scanTmp,   &vBounds};
@@ -2668,6 +2696,15 @@ void Pipeline::setStream(void *stream) {
 // buckets halve the emit stage; the 256-variant batch: 7.26 ms against
 // 7.90).
 constexpr uint64_t kLargeLoadRanks = uint64_t{1} << 22;
+// Whether a load packs its sorted items and runs by bucket (Impl::compact):
+// growable loads above kLargeLoadRanks error nodes (the packing scan costs a
+// few microseconds; the slot arrays it replaces hold about 3.9 entries per
+// node there), or every growable load when compact_items is 2.
+static bool compactLoad(bool growable, uint64_t R) {
+  const uint32_t mode = gCompactItems.load(std::memory_order_relaxed);
+  return growable && (mode == 2 || (mode == 1 && R > kLargeLoadRanks));
+}
+
 static uint32_t defaultFill(uint64_t loadRanks) {
   const uint32_t knob = gBucketFill.load(std::memory_order_relaxed);
   return knob ? knob : loadRanks > kLargeLoadRanks ? 128u : 64u;
@@ -2720,25 +2757,41 @@ static uint32_t bucketCountStride(uint64_t B) {
 // Every work buffer of a load, in the arena's order, by its totals: R error
 // nodes, B sort buckets of `cap` slots, V circuits, Ccols detector columns.
 // req(index, bytes, what) is called once per buffer.
-constexpr uint32_t kWorkBuffers = 30;
+constexpr uint32_t kWorkBuffers = 29;
+// `compact`: the load's sorted items and runs packed by bucket, its outputs
+// in pslot's space (see Impl::compact).
+struct OutAlias {
+  uint64_t hash, off, member, bytes;  // byte offsets in pslot; bytes in all
+};
+static OutAlias outAlias(uint64_t R) {
+  OutAlias a;
+  a.hash = align256(8 * R);
+  a.off = a.hash + align256(8 * R);
+  a.member = a.off + align256(4 * (R + 1));
+  a.bytes = a.member + align256(4 * (R + 1));
+  return a;
+}
 template <typename F>
 static void workBufferSizes(uint64_t R, uint64_t B, uint64_t V, uint64_t Ccols, uint32_t countStride, uint32_t cap,
-                            F &&req) {
+                            bool compact, F &&req) {
   R = std::max<uint64_t>(R, 1);
-  // bucket slots, then an arena of R entries for oversized buckets
-  const uint64_t slots = B * static_cast<uint64_t>(cap) + R;
-  if (slots >= 0xFFFFFFFFull) throw RuntimeError("too many sort buckets in one load");
+  const uint64_t bucketSlots = B * static_cast<uint64_t>(cap);
+  if (bucketSlots + R >= 0xFFFFFFFFull) throw RuntimeError("too many sort buckets in one load");
+  // sorted items and runs: at the bucket slots, then an arena of R entries
+  // for oversized buckets; or packed by bucket (R, and the 65 entries the
+  // emit stage reads ahead from the start)
+  const uint64_t slots = compact ? R + 64 : bucketSlots + R;
+  const uint64_t outs = compact ? 0 : R;  // outputs in their own buffers (else in pslot's space)
   uint32_t k = 0;
   req(k++, 8 * std::max<uint64_t>(Ccols, 1), "columns");                // band
   req(k++, 16 * R, "nodes");                                             // info
-  req(k++, 4ull * kStashWords * R, "nodes");                             // stash
   req(k++, 16 * R, "nodes");                                             // ovSlot
   req(k++, 4 * R, "nodes");                                              // ovBucket
   req(k++, 4 * kFlagWords, "buckets");                                   // counters
   // (B + 1) * countStride <= 2^17 + 32 words whatever B is, so a reserved
   // pipeline fits every later load
   req(k++, 4 * std::max<uint64_t>((B + 1) * countStride, (1u << 17) + 32), "buckets");  // bucketCnt
-  req(k++, 16 * slots, "nodes");                                         // pslot
+  req(k++, std::max<uint64_t>(16 * bucketSlots, compact ? outAlias(R).bytes : 0), "nodes");  // pslot
   req(k++, 4 * slots, "nodes");                                          // prank
   req(k++, 4 * slots, "nodes");                                          // runRank
   req(k++, 8 * slots, "nodes");                                          // runP
@@ -2748,10 +2801,10 @@ static void workBufferSizes(uint64_t R, uint64_t B, uint64_t V, uint64_t Ccols, 
   for (int q = 0; q < 4; ++q) req(k++, 4 * (B + 1), "buckets");          // runCount, runNnz, itemCount, slotBase
   req(k++, 4 * 3 * ((B + 3) / 4 + 1), "buckets");                        // wgSum
   for (int q = 0; q < 3; ++q) req(k++, 4 * (B + 1), "buckets");          // classStart, tgtStart, memberStart
-  req(k++, 8 * R, "nodes");                                              // outP
-  req(k++, 4 * (R + 1), "nodes");                                        // outOff
-  req(k++, 8 * R, "nodes");                                              // outHash
-  req(k++, 4 * (R + 1), "nodes");                                        // outMember
+  req(k++, 8 * outs, "nodes");                                           // outP
+  req(k++, 4 * (outs + 1), "nodes");                                     // outOff
+  req(k++, 8 * outs, "nodes");                                           // outHash
+  req(k++, 4 * (outs + 1), "nodes");                                     // outMember
   req(k++, 16 * R, "nodes");                                             // scratchKey
   req(k++, 8 * R, "nodes");                                              // scratchRank
   req(k++, 4 * kScanArrays * ((B + kScanTile - 1) / kScanTile + 2), "buckets");  // scanTmp
@@ -2849,7 +2902,6 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   GPA_HIP(hipStreamSynchronize(m.side));
   m.sizesKnown = false;
   m.launchMode = -1;
-  m.stashShift = 4;
   m.splitPlanned = false;
   m.planPending = false;
   m.classBatch = 4;
@@ -3009,9 +3061,27 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
   static_assert(std::tuple_size<decltype(bufs)>::value == kWorkBuffers, "one DevBuf per work buffer");
   // (a frozen pipeline keeps the slots per bucket it was sized with)
   if (!frozen) m.bucketCap = bucketCapFor(imgs);
-  workBufferSizes(R, B, V, m.Ccols, m.countStride, m.bucketCap,
+  m.compact = compactLoad(growable_, R);
+  if (m.compact) {
+    const OutAlias a = outAlias(R);
+    m.aliasHash = a.hash;
+    m.aliasOff = a.off;
+    m.aliasMember = a.member;
+  }
+  workBufferSizes(R, B, V, m.Ccols, m.countStride, m.bucketCap, m.compact,
                   [&](uint32_t k, uint64_t bytes, const char *what) { wa.request(*bufs[k], bytes, what); });
   wa.commit(frozen, growable_ ? kGrowHeadroom : 1.0);
+  // target rows: a driver's 16 words per node (its capacity is fixed); a
+  // growable load starts at the widest rows its stash already holds (at
+  // least 4 words) and widens them, growing the stash, when a run shows a
+  // larger class (rows that do not fit are read from S meanwhile)
+  if (!growable_) {
+    m.stashShift = 4;
+  } else {
+    m.stashShift = 2;
+    while (m.stashShift < 4 && (4ull << (m.stashShift + 1)) * R <= m.stash.cap) ++m.stashShift;
+  }
+  m.stash.ensure((4ull << m.stashShift) * R, frozen, "nodes");
   // targets: 16 per error node for a driver (its capacity is fixed), a
   // growable load starts with target_fill per node and grows (running again)
   // if its DEM holds more
@@ -3077,23 +3147,25 @@ uint64_t afterEnsure(size_t cap, uint64_t bytes) { return bytes <= cap ? cap : r
 struct HeldCaps {
   std::array<size_t, kWorkBuffers> work{};
   bool arena = false;  // the arena is allocated
-  size_t arenaCap = 0, in = 0, S = 0, targets = 0, members = 0, splitBuf = 0, splitState = 0;
+  size_t arenaCap = 0, in = 0, S = 0, targets = 0, members = 0, splitBuf = 0, splitState = 0, stash = 0;
   uint64_t capTargets = 0;
 };
 
 // Device bytes a growable pipeline holding `h` holds after loading a load
 // with totals `t` and compiling it: the work arena as WorkArena::commit
-// lays it out, the input arena and targets as load() sizes them, members as
-// held, and the time-chunked sweep's buffers at their largest (at most
-// kSplitMaxColumns columns of kSplitMaxChunks chunks of the widest circuit).
-// Every buffer is freed before its replacement is allocated, so this also
-// bounds the peak during the load and its compiles -- unless the DEM has
-// more than 16 targets per error node, when the target buffer grows.
-uint64_t footprint(const LoadTotals &t, const HeldCaps &h, double headroom) {
+// lays it out, the input arena and targets as load() sizes them, the
+// target rows at their widest, members as held, and the time-chunked
+// sweep's buffers at their largest (at most kSplitMaxColumns columns of
+// kSplitMaxChunks chunks of the widest circuit).  Every buffer is freed
+// before its replacement is allocated, so this also bounds the peak during
+// the load and its compiles -- unless the DEM has more than 16 targets per
+// error node, when the target buffer grows.
+uint64_t footprint(const LoadTotals &t, const HeldCaps &h, double headroom, bool growable) {
   const uint64_t R = std::max<uint64_t>(t.R, 1);
   std::vector<size_t> caps;
   bool fits = h.arena;
-  workBufferSizes(R, t.B, t.V, t.Ccols, bucketCountStride(t.B), t.cap, [&](uint32_t k, uint64_t bytes, const char *) {
+  workBufferSizes(R, t.B, t.V, t.Ccols, bucketCountStride(t.B), t.cap, compactLoad(growable, R),
+                  [&](uint32_t k, uint64_t bytes, const char *) {
     const size_t b = std::max<size_t>(bytes, 256), cap = h.work[k];
     const size_t grown = h.arena && b > cap ? static_cast<size_t>(b * headroom) : 0;
     if (b > cap) fits = false;
@@ -3104,6 +3176,7 @@ uint64_t footprint(const LoadTotals &t, const HeldCaps &h, double headroom) {
   const uint64_t sBytes = 4 * std::max<uint64_t>(t.Swords, 1);
   total += sBytes <= h.S ? h.S : grownBytes(sBytes, h.S ? headroom : 1.0);
   total += h.capTargets == 0 || h.capTargets < 16 * R ? afterEnsure(h.targets, 4 * 16 * R) : h.targets;
+  total += afterEnsure(h.stash, 4ull * kStashWords * R);
   total += h.members;
   const uint64_t G = (t.maxP + 31) / 32, P4 = quad(static_cast<uint32_t>(t.maxP));
   // descriptors, items and live lists; state areas (every boundary with
@@ -3121,7 +3194,7 @@ uint64_t footprint(const LoadTotals &t, const HeldCaps &h, double headroom) {
 }  // namespace
 
 uint64_t Pipeline::deviceBytesFor(const std::vector<const StepgImage *> &imgs, bool growable) {
-  return footprint(loadTotals(imgs, growable), HeldCaps{}, growable ? kGrowHeadroom : 1.0);
+  return footprint(loadTotals(imgs, growable), HeldCaps{}, growable ? kGrowHeadroom : 1.0, growable);
 }
 
 uint64_t Pipeline::deviceBytesAfter(const std::vector<const StepgImage *> &imgs) const {
@@ -3137,9 +3210,10 @@ uint64_t Pipeline::deviceBytesAfter(const std::vector<const StepgImage *> &imgs)
   h.targets = m.outTargets.cap;
   h.capTargets = m.capTargets;
   h.members = m.members.cap;
+  h.stash = m.stash.cap;
   h.splitBuf = m.splitBuf.cap;
   h.splitState = m.splitState.cap;
-  return footprint(loadTotals(imgs, true), h, kGrowHeadroom);
+  return footprint(loadTotals(imgs, true), h, kGrowHeadroom, true);
 }
 
 void Pipeline::upload() {
@@ -3260,7 +3334,14 @@ void Pipeline::run(bool timed) {
                m.runNnz.as<uint32_t>(),     m.itemCount.as<uint32_t>(),
                m.slotBase.as<uint32_t>(),   m.wgSum.as<uint32_t>(),
                m.scratchKey.as<uint64_t>(), m.scratchRank.as<uint32_t>(),
-               B * m.bucketCap, m.bucketCap};
+               B * m.bucketCap, m.bucketCap, m.compact ? 1u : 0u};
+  if (B && m.compact) {
+    // each bucket's packed range: the exclusive prefix sum of the counts
+    ScanArrays sa{{m.bucketCount(), nullptr, nullptr}, {m.slotBase.as<uint32_t>(), nullptr, nullptr}};
+    sa.stride = m.countStride;
+    sa.count = 1;
+    m.scan(sa, B);
+  }
   if (B) bucketKernel<<<(B + 3) / 4, 256, 0, st>>>(B, bio, o);
   mark(3);
   // bucket offsets: inside emit from the workgroup totals, or by the scans
@@ -3277,8 +3358,8 @@ void Pipeline::run(bool timed) {
              m.runTgtPre.as<uint32_t>(), m.runMember.as<uint32_t>(), m.runP.as<double>(),
              m.runHash.as<uint64_t>(), m.runNnz.as<uint32_t>(), m.S.as<uint32_t>(),
              m.stash.as<uint32_t>(), rin.code, rin.spill, stashMaxCols, m.stashShift, m.band.as<int2>(), colAdj,
-             m.bucketCap};
-  EmitOut eout{m.outP.as<double>(), m.outOff.as<uint32_t>(), m.outHash.as<uint64_t>(), m.outMember.as<uint32_t>(),
+             m.compact ? 0u : m.bucketCap};
+  EmitOut eout{m.oP(), m.oOff(), m.oHash(), m.oMember(),
                m.outTargets.as<uint32_t>(), m.targets16 ? m.outTargets.as<uint16_t>() : nullptr, m.capTargets,
                m.flags(), m.vBounds.as<uint32_t>()};
   emitKernel<<<std::max<uint32_t>((B + 3) / 4, 1), 256, 0, st>>>(desc, ein, eout);
@@ -3594,10 +3675,10 @@ void Pipeline::download() {
     rb.targets.ensure(4 * T + 4);
     const uint64_t tb = m.targets16 ? 2 : 4;
     if (C) {
-      GPA_HIP(hipMemcpyAsync(rb.p.p, m.outP.p, 8 * C, hipMemcpyDeviceToHost, m.stream));
-      if (m.wantHashes) GPA_HIP(hipMemcpyAsync(rb.hash.p, m.outHash.p, 8 * C, hipMemcpyDeviceToHost, m.stream));
+      GPA_HIP(hipMemcpyAsync(rb.p.p, m.oP(), 8 * C, hipMemcpyDeviceToHost, m.stream));
+      if (m.wantHashes) GPA_HIP(hipMemcpyAsync(rb.hash.p, m.oHash(), 8 * C, hipMemcpyDeviceToHost, m.stream));
     }
-    GPA_HIP(hipMemcpyAsync(rb.off.p, m.outOff.p, 4 * (C + 1), hipMemcpyDeviceToHost, m.stream));
+    GPA_HIP(hipMemcpyAsync(rb.off.p, m.oOff(), 4 * (C + 1), hipMemcpyDeviceToHost, m.stream));
     if (T) GPA_HIP(hipMemcpyAsync(rb.targets.p, m.outTargets.p, tb * T, hipMemcpyDeviceToHost, m.stream));
   };
   GPA_HIP(hipMemcpyAsync(m.hVB.p, m.vBounds.p, 4 * (2 * V + 2), hipMemcpyDeviceToHost, m.stream));
@@ -3680,6 +3761,7 @@ void Pipeline::download() {
     while (shift < 4 && (2u << shift) < most) ++shift;
     if (shift != m.stashShift) {
       m.stashShift = shift;
+      m.stash.ensure((4ull << shift) * std::max<uint64_t>(m.R, 1), m.frozen, "nodes");
       m.dropGraph();
     }
   }
@@ -3732,7 +3814,7 @@ void Pipeline::copyHashes(uint32_t v, uint64_t *dst) const {
   const uint32_t c0 = m.vClass[v], n = m.vClass[v + 1] - c0;
   GPA_HIP(hipSetDevice(device_));
   GPA_HIP(hipStreamSynchronize(m.stream));
-  if (n) GPA_HIP(hipMemcpy(dst, m.outHash.as<uint64_t>() + c0, 8ull * n, hipMemcpyDeviceToHost));
+  if (n) GPA_HIP(hipMemcpy(dst, m.oHash() + c0, 8ull * n, hipMemcpyDeviceToHost));
 }
 
 void Pipeline::runGraph() {
@@ -3850,6 +3932,11 @@ void setTargetFill(uint32_t n) {
   gTargetFill.store(n);
 }
 uint32_t targetFill() { return gTargetFill.load(); }
+void setCompactItems(uint32_t mode) {
+  if (mode > 2) throw InvalidArgument("compact_items must be 0, 1 or 2");
+  gCompactItems.store(mode);
+}
+uint32_t compactItems() { return gCompactItems.load(); }
 uint64_t Pipeline::sWords() const { return impl_->Swords; }
 bool packedSort() { return gPackedSort.load() != 0; }
 bool maskTable() { return gMaskTable.load() != 0; }
@@ -3859,7 +3946,7 @@ uint64_t Pipeline::deviceBytes() const {
   const Impl &m = *impl_;
   // the work arena and the buffers with their own allocations
   uint64_t b = m.work.mem.cap;
-  for (const DevBuf *x : {&m.in, &m.S, &m.outTargets, &m.members, &m.splitBuf, &m.splitState}) b += x->cap;
+  for (const DevBuf *x : {&m.in, &m.S, &m.stash, &m.outTargets, &m.members, &m.splitBuf, &m.splitState}) b += x->cap;
   return b;
 }
 
@@ -3931,7 +4018,7 @@ const uint32_t *Pipeline::devSortedRanks() {
   GPA_HIP(hipStreamSynchronize(m.stream));
   return m.members.as<uint32_t>();
 }
-const uint32_t *Pipeline::devClassMember() const { return impl_->outMember.as<uint32_t>(); }
+const uint32_t *Pipeline::devClassMember() const { return impl_->oMember(); }
 
 void xorMergeForward(const double *values, const uint32_t *index, const uint32_t *segStart,
                      uint32_t numSegments, double *out, void *stream) {

@@ -178,7 +178,7 @@ void setSBands(uint32_t mode);
 uint32_t sBands();
 // Slots per sort bucket (64, 128 or 256; default 256): classify writes a
 // bucket's first `cap` nodes to its slots (the rest to an overflow list
-// sorted by the workgroup); slot storage is numBuckets * cap + nodes.
+// sorted by the workgroup); slot storage is numBuckets * cap.
 // Same results for every value.  Read at load (a size class's tuned value
 // first).
 void setBucketCap(uint32_t cap);
@@ -187,6 +187,15 @@ uint32_t bucketCap();
 // (1..16, default 4); a DEM with more targets grows it and runs again.
 void setTargetFill(uint32_t n);
 uint32_t targetFill();
+// How growable loads store the bucket stage's sorted items and runs: 1
+// (default) packed by bucket -- at the prefix sums of the bucket counts,
+// numRanks entries in all, with the outputs in the space of classify's slot
+// records -- when a load has more than 2^22 error nodes, else at each
+// bucket's slots (numBuckets * cap entries plus an arena of numRanks); 2
+// packed for every growable load (for tests); 0 never.  Same results in
+// every mode.  Read at load.
+void setCompactItems(uint32_t mode);
+uint32_t compactItems();
 // Whether a load's long columns may be swept in time chunks from its second
 // run on (planned from the bands of its first): 1 (default) where the
 // estimated critical path gains, 2 every column whose band reaches the

@@ -325,12 +325,19 @@ int gp_device_memory(int device, uint64_t *free_bytes, uint64_t *total_bytes);
  *   "bucket_cap": slots per sort bucket, 64, 128 or 256 (default 256):
  *   classify writes a bucket's first bucket_cap nodes to
  *   its slots and the rest to an overflow list that the bucket's workgroup
- *   sorts; slot storage is buckets x bucket_cap + nodes; with bucket_fill it
+ *   sorts; slot storage is buckets x bucket_cap; with bucket_fill it
  *   sets the shape of the hash table; same results; read at each load (a
  *   driver keeps the value it was created with).
  *   "target_fill": target slots per error node that a batch's (growable)
  *   output buffer starts with, 1..16 (default 4); a DEM with more targets
  *   grows the buffer and compiles again; same results.  Drivers hold 16.
+ *   "compact_items": how a batch stores the bucket stage's sorted nodes and
+ *   runs: 1 (default) packed by bucket (at the prefix sums of the bucket
+ *   counts: one entry per error node, the outputs in the space of the
+ *   bucket slots) for loads of more than 2^22 error nodes, else at each
+ *   bucket's slots (buckets x bucket_cap entries plus one per error node);
+ *   2 packed for every batch (a test setting); 0 never; same results; read
+ *   at each load.  Drivers always use the slots.
  *   "sweep_split": 1 (default) lets a load's second compile plan, from the
  *   bands its first compile found, a time-chunked sweep of a column much
  *   longer than the rest (cut where few nodes are live; the middle chunks'

@@ -74,6 +74,19 @@ def test_target_fill_knob():
         autotune.set_tuning("target_fill", old)
 
 
+def test_compact_items_knob():
+    old = autotune.get_tuning("compact_items")
+    assert old == 1
+    try:
+        for v in (0, 2, 1):
+            autotune.set_tuning("compact_items", v)
+            assert autotune.get_tuning("compact_items") == v
+        with pytest.raises(ValueError):
+            autotune.set_tuning("compact_items", 3)
+    finally:
+        autotune.set_tuning("compact_items", old)
+
+
 def test_install_cached_hands_the_cache_to_the_library(tmp_path, monkeypatch):
     cache = tmp_path / "tune.json"
     cache.write_text(json.dumps({

@@ -209,3 +209,37 @@ def test_target_buffer_grows_from_its_start_size():
     bc.close()
     for a, b, c in zip(ref, grown, again):
         assert a == b and a == c
+
+
+def test_compact_items_give_identical_dems():
+    """Sorted nodes and runs packed by bucket (compact_items 2: placed at the
+    prefix sums of the bucket counts, the outputs in the bucket slots'
+    space) give the DEMs of each circuit compiled alone: with default
+    buckets, with 64-slot buckets that leave most nodes to oversized buckets
+    (overflow list, workgroup sort in LDS), and with buckets of thousands
+    of nodes (sorted in the global workspace); again on a graph replay and
+    after a smaller load."""
+    from greenpeas_amd import autotune
+
+    texts = variant_circuits(6, seed=5, distances=(5, 7)) + [
+        read_circuit(n) for n in ("bb12", "surface5", "iceberg_surface_6_adaptive_circl")]
+    want = [solo(t) for t in texts]
+    old = {k: autotune.get_tuning(k) for k in ("compact_items", "bucket_fill", "bucket_cap")}
+    try:
+        autotune.set_tuning("compact_items", 2)
+        for fill, cap in ((0, 256), (256, 64), (4096, 64)):
+            print(f"bucket_fill {fill} bucket_cap {cap}", flush=True)
+            autotune.set_tuning("bucket_fill", fill)
+            autotune.set_tuning("bucket_cap", cap)
+            bc = BatchCompiler()
+            bc.set_circuits(texts)
+            got = bc.compile()
+            for i, (a, b) in enumerate(zip(got, want)):
+                assert a == b, f"circuit {i} differs from its solo compile"
+            assert bc.compile() == got
+            bc.set_circuits(texts[-3:])
+            assert bc.compile() == want[-3:]
+            bc.close()
+    finally:
+        for k, v in old.items():
+            autotune.set_tuning(k, v)
