@@ -968,6 +968,7 @@ int gp_set_tuning(const char *name, uint32_t value) {
     else if (std::string(name) == "bucket_cap") gpa::setBucketCap(value);
     else if (std::string(name) == "target_fill") gpa::setTargetFill(value);
     else if (std::string(name) == "compact_items") gpa::setCompactItems(value);
+    else if (std::string(name) == "overflow_start") gpa::setOverflowStart(value);
     else if (std::string(name) == "sweep_split") {
       if (value > 2) throw InvalidArgument("sweep_split must be 0, 1 or 2");
       gpa::setSweepSplit(value);
@@ -989,6 +990,7 @@ int gp_get_tuning(const char *name, uint32_t *value) {
     else if (std::string(name) == "bucket_cap") *value = gpa::bucketCap();
     else if (std::string(name) == "target_fill") *value = gpa::targetFill();
     else if (std::string(name) == "compact_items") *value = gpa::compactItems();
+    else if (std::string(name) == "overflow_start") *value = gpa::overflowStart();
     else if (std::string(name) == "sweep_split") *value = gpa::sweepSplit();
     else throw InvalidArgument(std::string("unknown tuning parameter: ") + name);
   });

@@ -40,6 +40,7 @@ constexpr int kBucketItemsPerWave = 256;  // wave-register sort capacity (4 per 
 std::atomic<uint32_t> gBucketFill{0};     // mean items per bucket the digit width aims at (0: by load size)
 std::atomic<uint32_t> gBucketCap{256};    // slots per sort bucket (64, 128 or 256)
 std::atomic<uint32_t> gTargetFill{4};     // target slots per error node a growable load starts with
+std::atomic<uint32_t> gOverflowStart{1u << 20};  // start entries of a growable load's overflow list and workspace
 std::atomic<uint32_t> gCompactItems{1};   // growable loads' sorted items and runs packed by bucket: 1 large loads, 2 all, 0 none
 std::atomic<uint32_t> gStash{1};          // target rows on (1) or off (0)
 std::atomic<uint32_t> gMaskTable{1};      // classify's LDS mask tables allowed (1) or not (0)
@@ -89,6 +90,7 @@ __device__ __forceinline__ double rankProbability(const VDesc &d, const double *
 constexpr uint32_t kFlagWords = 16;
 constexpr uint32_t kFlagBandOverflow = kFlagWords - 1;
 constexpr uint32_t kFlagSplitWait = 7;
+constexpr uint32_t kFlagScratchNeed = 6;  // workspace entries the oversized buckets asked for
 
 // S holds, per detector column, the rows of layers base .. top (the rows
 // its band can reach) at a region of its own, then two rows that the
@@ -1336,6 +1338,7 @@ struct ClassifyOut {
   uint32_t stashShift;    // a row holds 1 << stashShift words (2..4)
   uint32_t packedKey;     // 0: slots carry kLocalRankNone (the bucket stage sorts exactly)
   uint32_t bucketCap;     // slots per bucket
+  uint32_t ovCap;         // overflow list entries held (past them the count goes on, nothing is written)
 };
 
 // Ranks' inputs: compact frame rows, spill entries, probabilities.
@@ -1450,8 +1453,10 @@ __global__ __launch_bounds__(kClassBlock) void classifyKernel(
       o.pslot[static_cast<uint64_t>(bkt) * o.bucketCap + pos] = slot;
     } else {
       const uint32_t k = atomicAdd(&o.flags[3], 1u);
-      o.ovSlot[k] = slot;
-      o.ovBucket[k] = bkt;
+      if (k < o.ovCap) {
+        o.ovSlot[k] = slot;
+        o.ovBucket[k] = bkt;
+      }
     }
   }
 }
@@ -1614,18 +1619,21 @@ struct BucketIo {
   uint32_t *slotBase;     // per bucket: where its sorted items and runs start (compact: set by the
                           // scan of the bucket counts before this stage)
   uint32_t *wgSum;        // per workgroup: its buckets' runs, targets, items
-  uint64_t *scratchKey;   // sort workspace for oversized buckets (2 * arenaSize)
+  uint64_t *scratchKey;   // sort workspace for buckets too large for LDS
   uint32_t *scratchRank;
   uint32_t arenaBase;     // numBuckets * cap: oversized buckets' items and runs follow
   uint32_t cap;           // slots per bucket (at most kBucketCap)
   uint32_t compact;       // items and runs packed by bucket at slotBase (no arena)
+  uint32_t ovCap;         // overflow list entries held
+  uint64_t scratchCap;    // workspace entries held
 };
 
-// An oversized bucket (more than kBucketCap items), by the whole workgroup:
-// its items (slots, then its entries of the overflow list) are copied to a
-// workspace, bitonic-sorted there on (hash, rank), and merged chunk by chunk
-// by wave 0 with the items' node records; the sorted items and runs go to an
-// arena range taken with one atomic.
+// An oversized bucket (more than its slots' worth of items), by the whole
+// workgroup: its items (slots, then its entries of the overflow list) are
+// gathered in LDS (up to kBigLdsItems) or a global workspace range taken
+// with one atomic, bitonic-sorted there on (hash, rank), and merged chunk by
+// chunk by wave 0 with the items' node records; the sorted items and runs
+// go to the bucket's packed range or an arena range taken with one atomic.
 // Bitonic sort of P (a power of two) (hash, rank) pairs by the workgroup.
 template <typename KP, typename RP>
 __device__ __forceinline__ void bitonicWG(KP K, RP Rk, uint32_t P) {
@@ -1656,77 +1664,82 @@ constexpr uint32_t kBigLdsItems = 2048;
 
 __device__ void bigBucket(uint32_t b, uint32_t cnt, const BucketIo &io, const RunOut &o, uint32_t *tot,
                           uint64_t *ldsKeys, uint32_t *ldsRanks) {
-  __shared__ uint32_t shOff, shFill;
+  __shared__ uint32_t shOff, shFill, shWs;
   __shared__ double hsSlots[32];
   const uint32_t lane = threadIdx.x & 63;
-  if (threadIdx.x == 0) {
-    // its range: packed by bucket, or taken from the arena (total items
-    // never exceed it)
-    shOff = io.compact ? io.slotBase[b] : atomicAdd(&o.flags[5], cnt);
-    shFill = io.cap;
-  }
-  __syncthreads();
-  const uint32_t off = shOff;
   uint32_t P = 1;
   while (P < cnt) P <<= 1;
-  uint64_t *K = io.scratchKey + 2ull * off;  // P < 2 * cnt
-  uint32_t *Rk = io.scratchRank + 2ull * off;
-  // the items: its slots, then its entries of the overflow list, padded
-  // to P; sorted in LDS when they fit, else in the global workspace
+  // sorted in LDS when the items fit, else in the global workspace
   const bool inLds = P <= kBigLdsItems;
-  auto put = [&](uint32_t at, uint64_t key, uint32_t rank) {
-    if (inLds) {
-      ldsKeys[at] = key;
-      ldsRanks[at] = rank;
-    } else {
-      K[at] = key;
-      Rk[at] = rank;
+  if (threadIdx.x == 0) {
+    // its range: packed by bucket, or taken from the arena (total items
+    // never exceed it); its workspace range, if it needs one
+    shOff = io.compact ? io.slotBase[b] : atomicAdd(&o.flags[5], cnt);
+    shFill = io.cap;
+    shWs = inLds ? 0u : atomicAdd(&o.flags[kFlagScratchNeed], P);
+  }
+  __syncthreads();
+  const uint32_t off = shOff, ws = shWs;
+  if (!inLds && static_cast<uint64_t>(ws) + P > io.scratchCap) {
+    // the workspace is full: the bucket is left empty, and the host grows
+    // the workspace to what the run asked for and runs again
+    if (threadIdx.x == 0) {
+      io.runCount[b] = 0;
+      io.runNnz[b] = 0;
+      io.slotBase[b] = io.compact ? off : io.arenaBase + off;
+      tot[0] = 0;
+      tot[1] = 0;
     }
-  };
+    __syncthreads();
+    return;
+  }
+  uint64_t *K = inLds ? ldsKeys : io.scratchKey + ws;
+  uint32_t *Rk = inLds ? ldsRanks : io.scratchRank + ws;
+  // the items: its slots, then its entries of the overflow list, padded
+  // to P
   const uint32_t slot0 = b * io.cap;
   for (uint32_t i = threadIdx.x; i < io.cap; i += blockDim.x) {
     const uint4 x = io.pslot[slot0 + i];
-    put(i, (static_cast<uint64_t>(x.y) << 32) | x.x, x.z);
+    K[i] = (static_cast<uint64_t>(x.y) << 32) | x.x;
+    Rk[i] = x.z;
   }
-  const uint32_t nov = o.flags[3];
+  // (a list cut at its capacity holds fewer: the run is repeated)
+  const uint32_t nov = min(o.flags[3], io.ovCap);
   for (uint32_t k = threadIdx.x; k < nov; k += blockDim.x)
     if (io.ovBucket[k] == b) {
       const uint32_t at = atomicAdd(&shFill, 1u);
       const uint4 x = io.ovSlot[k];
-      put(at, (static_cast<uint64_t>(x.y) << 32) | x.x, x.z);
+      K[at] = (static_cast<uint64_t>(x.y) << 32) | x.x;
+      Rk[at] = x.z;
     }
-  for (uint32_t i = cnt + threadIdx.x; i < P; i += blockDim.x) put(i, ~0ull, NONE);
   __syncthreads();
-  if (inLds) {
-    bitonicWG(ldsKeys, ldsRanks, P);
-    // the sorted items to the workspace, where the merge reads them
-    for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) {
-      K[i] = ldsKeys[i];
-      Rk[i] = ldsRanks[i];
-    }
-    __syncthreads();
-  } else {
-    bitonicWG(K, Rk, P);
+  const uint32_t got = shFill;  // == cnt unless the list was cut
+  for (uint32_t i = got + threadIdx.x; i < P; i += blockDim.x) {
+    K[i] = ~0ull;
+    Rk[i] = NONE;
   }
+  __syncthreads();
+  if (inLds) bitonicWG(ldsKeys, ldsRanks, P);
+  else bitonicWG(K, Rk, P);
   const uint32_t base = io.compact ? off : io.arenaBase + off;
   if (threadIdx.x < 64) {
     RunCarry cy{0.0, 0ull, 0u, 0u, 0u, 0u, 0u};
     // members' values for runs across chunks: from the sorted ranks' records
     const NodeRunValues members{Rk, io.info, hsSlots};
-    for (uint32_t c0 = 0; c0 < cnt; c0 += 64) {
+    for (uint32_t c0 = 0; c0 < got; c0 += 64) {
       uint64_t h[1];
       uint32_t r[1], h2v[1] = {0u}, nzv[1] = {0u};
       double pv[1] = {0.0};
       const uint32_t e = c0 + lane;
-      h[0] = e < cnt ? K[e] : ~0ull;
-      r[0] = e < cnt ? Rk[e] : NONE;
-      if (e < cnt) {
+      h[0] = e < got ? K[e] : ~0ull;
+      r[0] = e < got ? Rk[e] : NONE;
+      if (e < got) {
         const uint4 x = io.info[r[0]];
         pv[0] = infoProb(x);
         h2v[0] = x.z;
         nzv[0] = x.w;
       }
-      const uint32_t n = cnt - c0 < 64 ? cnt - c0 : 64;
+      const uint32_t n = got - c0 < 64 ? got - c0 : 64;
       scanRuns<1>(h, r, pv, h2v, nzv, n, c0, base, cy, lane, o, members);
     }
     if (lane == 0) {
@@ -2541,9 +2554,12 @@ struct Pipeline::Impl {
   // serialize) and one counter per 128-byte line (four times the lines)
   uint32_t countStride = 1;
 
-  // per node: info (probability, second hash, class size), overflow list
-  // (ovSlot/ovBucket); stash (target rows, its own allocation: its width
-  // grows with a load's largest class);
+  // per node: info (probability, second hash, class size); stash (target
+  // rows, its own allocation: its width grows with a load's largest class);
+  // the overflow list (ovSlot/ovBucket) and the oversized buckets'
+  // workspace (scratchKey/scratchRank), own allocations sized for the worst
+  // case in a driver and grown on demand in a growable load (ovCap and
+  // scratchCap entries);
   // per bucket slot (numBuckets * bucketCap): pslot (classify's records);
   // per sorted item, at the bucket's slots plus an arena of numRanks for
   // oversized buckets or, in a compact load, packed by bucket (numRanks in
@@ -2553,7 +2569,19 @@ struct Pipeline::Impl {
       runRank, runP, runHash, runTgtPre, runMember, runCount, runNnz, itemCount, slotBase, wgSum, classStart,
       tgtStart, memberStart, outP, outOff, outHash, outMember, outTargets, scratchKey, scratchRank,
       scanTmp, vBounds, members;
-  WorkArena work;  // holds the buffers above except stash, in, outTargets, members and the split buffers
+  WorkArena work;  // holds the buffers above except stash, the overflow list, the workspace, in,
+                   // outTargets, members and the split buffers
+  uint64_t ovCap = 0, scratchCap = 0;
+  void ensureOverflow(uint64_t n, bool fz) {
+    ovSlot.ensure(16 * n, fz, "nodes");
+    ovBucket.ensure(4 * n, fz, "nodes");
+    ovCap = std::min<uint64_t>({ovSlot.cap / 16, ovBucket.cap / 4, 0xFFFFFFFFull});
+  }
+  void ensureWorkspace(uint64_t n, bool fz) {
+    scratchKey.ensure(8 * n, fz, "nodes");
+    scratchRank.ensure(4 * n, fz, "nodes");
+    scratchCap = std::min(scratchKey.cap / 8, scratchRank.cap / 4);
+  }
   // A compact load places each bucket's sorted items and runs at the
   // exclusive prefix sum of the bucket counts (a scan before the bucket
   // stage) instead of at its slots, and its outputs (p, hash, offsets,
@@ -2585,11 +2613,11 @@ struct Pipeline::Impl {
   uint32_t bucketCap = kBucketCap;  // slots per sort bucket of the load
   bool uniformCols = false;  // every column of a circuit stores all its layers in regions of one size
   // the arena's buffers, in the order workBufferSizes() sizes them
-  std::array<DevBuf *, 29> workBufs() {
-    return {&band,                    &info,                     &ovSlot,      &ovBucket,  &counters, &bucketCnt,
-            &pslot,     &prank,      &runRank,     &runP,       &runHash,     &runTgtPre, &runMember, &runCount,
-            &runNnz,    &itemCount,  &slotBase,    &wgSum,      &classStart,  &tgtStart,  &memberStart, &outP,
-            &outOff,    &outHash,    &outMember,   &scratchKey, &scratchRank, &scanTmp,   &vBounds};
+  std::array<DevBuf *, 25> workBufs() {
+    return {&band,      &info,       &counters,    &bucketCnt,  &pslot,       &prank,     &runRank,
+            &runP,      &runHash,    &runTgtPre,   &runMember,  &runCount,    &runNnz,    &itemCount,
+            &slotBase,  &wgSum,      &classStart,  &tgtStart,   &memberStart, &outP,      &outOff,
+            &outHash,   &outMember,  &scanTmp,     &vBounds};
   }
 
   // host results: the block of the last download, blocks released by
@@ -2692,9 +2720,10 @@ void Pipeline::setStream(void *stream) {
 // Default items per bucket for a load of `loadRanks` error nodes: 64 while
 // the bucket stage is latency-bound (a few thousand buckets, one wave each:
 // shorter per-wave sorts win; bb12: 79.4 us per compile against 101.7 at
-// 128), 128 once there are millions (throughput-bound: fewer, fuller
-// buckets halve the emit stage; the 256-variant batch: 7.26 ms against
-// 7.90).
+// 128), 192 once there are millions (throughput-bound: fewer, fuller
+// buckets halve the emit stage and need fewer slots; the 256-variant
+// batch: 7.24 ms and 9.34 GB against 7.34 ms and 10.13 GB at 128, 7.90 ms
+// at 64; at 256 buckets overflow their 256 slots and it takes 13.3 ms).
 constexpr uint64_t kLargeLoadRanks = uint64_t{1} << 22;
 // Whether a load packs its sorted items and runs by bucket (Impl::compact):
 // growable loads above kLargeLoadRanks error nodes (the packing scan costs a
@@ -2707,7 +2736,7 @@ static bool compactLoad(bool growable, uint64_t R) {
 
 static uint32_t defaultFill(uint64_t loadRanks) {
   const uint32_t knob = gBucketFill.load(std::memory_order_relaxed);
-  return knob ? knob : loadRanks > kLargeLoadRanks ? 128u : 64u;
+  return knob ? knob : loadRanks > kLargeLoadRanks ? 192u : 64u;
 }
 
 // First radix digit width of a circuit: the smallest b with fill * 2^b >=
@@ -2757,7 +2786,7 @@ static uint32_t bucketCountStride(uint64_t B) {
 // Every work buffer of a load, in the arena's order, by its totals: R error
 // nodes, B sort buckets of `cap` slots, V circuits, Ccols detector columns.
 // req(index, bytes, what) is called once per buffer.
-constexpr uint32_t kWorkBuffers = 29;
+constexpr uint32_t kWorkBuffers = 25;
 // `compact`: the load's sorted items and runs packed by bucket, its outputs
 // in pslot's space (see Impl::compact).
 struct OutAlias {
@@ -2785,8 +2814,6 @@ static void workBufferSizes(uint64_t R, uint64_t B, uint64_t V, uint64_t Ccols, 
   uint32_t k = 0;
   req(k++, 8 * std::max<uint64_t>(Ccols, 1), "columns");                // band
   req(k++, 16 * R, "nodes");                                             // info
-  req(k++, 16 * R, "nodes");                                             // ovSlot
-  req(k++, 4 * R, "nodes");                                              // ovBucket
   req(k++, 4 * kFlagWords, "buckets");                                   // counters
   // (B + 1) * countStride <= 2^17 + 32 words whatever B is, so a reserved
   // pipeline fits every later load
@@ -2805,8 +2832,6 @@ static void workBufferSizes(uint64_t R, uint64_t B, uint64_t V, uint64_t Ccols, 
   req(k++, 4 * (outs + 1), "nodes");                                     // outOff
   req(k++, 8 * outs, "nodes");                                           // outHash
   req(k++, 4 * (outs + 1), "nodes");                                     // outMember
-  req(k++, 16 * R, "nodes");                                             // scratchKey
-  req(k++, 8 * R, "nodes");                                              // scratchRank
   req(k++, 4 * kScanArrays * ((B + kScanTile - 1) / kScanTile + 2), "buckets");  // scanTmp
   req(k++, 4 * (2 * V + 8), "circuits");                                 // vBounds
 }
@@ -3082,6 +3107,18 @@ void Pipeline::load(const std::vector<const StepgImage *> &imgs) {
     while (m.stashShift < 4 && (4ull << (m.stashShift + 1)) * R <= m.stash.cap) ++m.stashShift;
   }
   m.stash.ensure((4ull << m.stashShift) * R, frozen, "nodes");
+  // the overflow list and the oversized buckets' workspace: a driver's for
+  // the worst case (every node past its bucket's slots; twice the nodes), a
+  // growable load's from a start size (what it holds, at least R / 16 and
+  // overflow_start entries), grown when a run asks for more
+  if (!growable_) {
+    m.ensureOverflow(R, frozen);
+    m.ensureWorkspace(2 * R, frozen);
+  } else {
+    const uint64_t start = gOverflowStart.load();
+    m.ensureOverflow(std::min<uint64_t>(R, std::max<uint64_t>(R / 16, start)), false);
+    m.ensureWorkspace(std::min<uint64_t>(2 * R, start), false);
+  }
   // targets: 16 per error node for a driver (its capacity is fixed), a
   // growable load starts with target_fill per node and grows (running again)
   // if its DEM holds more
@@ -3148,6 +3185,7 @@ struct HeldCaps {
   std::array<size_t, kWorkBuffers> work{};
   bool arena = false;  // the arena is allocated
   size_t arenaCap = 0, in = 0, S = 0, targets = 0, members = 0, splitBuf = 0, splitState = 0, stash = 0;
+  size_t ovSlot = 0, ovBucket = 0, scratchKey = 0, scratchRank = 0;
   uint64_t capTargets = 0;
 };
 
@@ -3177,6 +3215,9 @@ uint64_t footprint(const LoadTotals &t, const HeldCaps &h, double headroom, bool
   total += sBytes <= h.S ? h.S : grownBytes(sBytes, h.S ? headroom : 1.0);
   total += h.capTargets == 0 || h.capTargets < 16 * R ? afterEnsure(h.targets, 4 * 16 * R) : h.targets;
   total += afterEnsure(h.stash, 4ull * kStashWords * R);
+  // the overflow list and workspace at their largest (they grow on demand)
+  total += afterEnsure(h.ovSlot, 16 * R) + afterEnsure(h.ovBucket, 4 * R);
+  total += afterEnsure(h.scratchKey, 16 * R) + afterEnsure(h.scratchRank, 8 * R);
   total += h.members;
   const uint64_t G = (t.maxP + 31) / 32, P4 = quad(static_cast<uint32_t>(t.maxP));
   // descriptors, items and live lists; state areas (every boundary with
@@ -3211,6 +3252,10 @@ uint64_t Pipeline::deviceBytesAfter(const std::vector<const StepgImage *> &imgs)
   h.capTargets = m.capTargets;
   h.members = m.members.cap;
   h.stash = m.stash.cap;
+  h.ovSlot = m.ovSlot.cap;
+  h.ovBucket = m.ovBucket.cap;
+  h.scratchKey = m.scratchKey.cap;
+  h.scratchRank = m.scratchRank.cap;
   h.splitBuf = m.splitBuf.cap;
   h.splitState = m.splitState.cap;
   return footprint(loadTotals(imgs, true), h, kGrowHeadroom, true);
@@ -3308,7 +3353,8 @@ void Pipeline::run(bool timed) {
     ClassifyOut co{m.info.as<uint4>(),     m.bucketCount(),          m.countStride,
                    m.pslot.as<uint4>(),    m.ovSlot.as<uint4>(),     m.ovBucket.as<uint32_t>(),
                    m.flags(),              m.stash.as<uint32_t>(),   stashMaxCols,
-                   m.stashShift,           gPackedSort.load(),       m.bucketCap};
+                   m.stashShift,           gPackedSort.load(),       m.bucketCap,
+                   static_cast<uint32_t>(m.ovCap)};
     if (m.classTable && gMaskTable.load() && m.classBatch == 16)
       classifyKernel<true, 16><<<m.nTiles, kClassBlock, 0, st>>>(desc, tiles, m.band.as<int2>(), colAdj,
                                                                  m.S.as<uint32_t>(), rin, co);
@@ -3334,7 +3380,8 @@ void Pipeline::run(bool timed) {
                m.runNnz.as<uint32_t>(),     m.itemCount.as<uint32_t>(),
                m.slotBase.as<uint32_t>(),   m.wgSum.as<uint32_t>(),
                m.scratchKey.as<uint64_t>(), m.scratchRank.as<uint32_t>(),
-               B * m.bucketCap, m.bucketCap, m.compact ? 1u : 0u};
+               B * m.bucketCap, m.bucketCap, m.compact ? 1u : 0u,
+               static_cast<uint32_t>(m.ovCap), m.scratchCap};
   if (B && m.compact) {
     // each bucket's packed range: the exclusive prefix sum of the counts
     ScanArrays sa{{m.bucketCount(), nullptr, nullptr}, {m.slotBase.as<uint32_t>(), nullptr, nullptr}};
@@ -3727,6 +3774,20 @@ void Pipeline::download() {
     download();
     return;
   }
+  if (fl[3] > m.ovCap || fl[kFlagScratchNeed] > m.scratchCap) {
+    // the overflow list or the oversized buckets' workspace was too small
+    // (cut there, so this run's classes are wrong): grow them to what the
+    // run asked for, and run again.  A driver's hold the worst case.
+    if (m.frozen) throw RuntimeError("sort overflow exceeded the driver's capacity");
+    if (fl[3] > m.ovCap) m.ensureOverflow(std::min<uint64_t>(m.R, fl[3] + fl[3] / 4), false);
+    if (fl[kFlagScratchNeed] > m.scratchCap)
+      m.ensureWorkspace(fl[kFlagScratchNeed] + fl[kFlagScratchNeed] / 4, false);
+    m.dropGraph();
+    m.sizesKnown = false;
+    run(false);
+    download();
+    return;
+  }
   if (fl[0]) throw RuntimeError("64-bit class hash collision between different error classes");
   m.totalClasses = vb[2 * V];
   m.totalTargets = vb[2 * V + 1];
@@ -3736,6 +3797,7 @@ void Pipeline::download() {
       throw RuntimeError("circuit exceeds the driver's capacity (targets)");
     m.outTargets.ensure(4 * (m.totalTargets + m.totalTargets / 4 + 64), false, "targets");
     m.capTargets = m.outTargets.cap / 4;
+    m.dropGraph();
     m.sizesKnown = false;
     run(false);
     download();
@@ -3937,6 +3999,11 @@ void setCompactItems(uint32_t mode) {
   gCompactItems.store(mode);
 }
 uint32_t compactItems() { return gCompactItems.load(); }
+void setOverflowStart(uint32_t n) {
+  if (n < 1) throw InvalidArgument("overflow_start must be at least 1");
+  gOverflowStart.store(n);
+}
+uint32_t overflowStart() { return gOverflowStart.load(); }
 uint64_t Pipeline::sWords() const { return impl_->Swords; }
 bool packedSort() { return gPackedSort.load() != 0; }
 bool maskTable() { return gMaskTable.load() != 0; }
@@ -3946,7 +4013,9 @@ uint64_t Pipeline::deviceBytes() const {
   const Impl &m = *impl_;
   // the work arena and the buffers with their own allocations
   uint64_t b = m.work.mem.cap;
-  for (const DevBuf *x : {&m.in, &m.S, &m.stash, &m.outTargets, &m.members, &m.splitBuf, &m.splitState}) b += x->cap;
+  for (const DevBuf *x : {&m.in, &m.S, &m.stash, &m.ovSlot, &m.ovBucket, &m.scratchKey, &m.scratchRank,
+                          &m.outTargets, &m.members, &m.splitBuf, &m.splitState})
+    b += x->cap;
   return b;
 }
 

@@ -150,7 +150,7 @@ class Pipeline {
 
 // Autotuning knob: mean number of items per sort bucket the first radix
 // digit aims at (the digit width is chosen per circuit from it); 0 (the
-// default): 64, or 128 for loads of more than 2^22 error nodes.
+// default): 64, or 192 for loads of more than 2^22 error nodes.
 void setBucketFill(uint32_t fill);
 uint32_t bucketFill();
 // Whether classify stores each node's first targets for emit (on by
@@ -196,6 +196,12 @@ uint32_t targetFill();
 // every mode.  Read at load.
 void setCompactItems(uint32_t mode);
 uint32_t compactItems();
+// Entries a growable load's sort overflow list (nodes past their bucket's
+// slots; at least numRanks / 16) and oversized-bucket workspace start with
+// (default 2^20); a run that needs more grows them and runs again.  Drivers
+// hold the worst case.  Same results for every value.  Read at load.
+void setOverflowStart(uint32_t n);
+uint32_t overflowStart();
 // Whether a load's long columns may be swept in time chunks from its second
 // run on (planned from the bands of its first): 1 (default) where the
 // estimated critical path gains, 2 every column whose band reaches the

@@ -291,7 +291,7 @@ int gp_device_memory(int device, uint64_t *free_bytes, uint64_t *total_bytes);
 /* Autotuning knobs (process-wide):
  *   "bucket_fill": mean number of (hash, node) items per sort bucket the
  *   first radix digit aims at; the digit width per circuit is the smallest b
- *   with bucket_fill * 2^b >= its p>0 node count (default 0: 64, or 128
+ *   with bucket_fill * 2^b >= its p>0 node count (default 0: 64, or 192
  *   in loads of more than 2^22 error nodes; read at the
  *   next load).
  *   "sweep_rpt": fewest STEPG records per thread of the sensitivity sweep
@@ -338,6 +338,11 @@ int gp_device_memory(int device, uint64_t *free_bytes, uint64_t *total_bytes);
  *   bucket's slots (buckets x bucket_cap entries plus one per error node);
  *   2 packed for every batch (a test setting); 0 never; same results; read
  *   at each load.  Drivers always use the slots.
+ *   "overflow_start": entries a batch's sort overflow list (nodes past
+ *   their bucket's slots; at least one in 16 error nodes) and its workspace
+ *   for buckets too large for LDS start with, at least 1 (default 2^20); a
+ *   compile that needs more grows them and runs again; same results; read
+ *   at each load.  Drivers hold the worst case.
  *   "sweep_split": 1 (default) lets a load's second compile plan, from the
  *   bands its first compile found, a time-chunked sweep of a column much
  *   longer than the rest (cut where few nodes are live; the middle chunks'

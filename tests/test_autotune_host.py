@@ -87,6 +87,19 @@ def test_compact_items_knob():
         autotune.set_tuning("compact_items", old)
 
 
+def test_overflow_start_knob():
+    old = autotune.get_tuning("overflow_start")
+    assert old == 1 << 20
+    try:
+        for v in (1, 64, 1 << 24):
+            autotune.set_tuning("overflow_start", v)
+            assert autotune.get_tuning("overflow_start") == v
+        with pytest.raises(ValueError):
+            autotune.set_tuning("overflow_start", 0)
+    finally:
+        autotune.set_tuning("overflow_start", old)
+
+
 def test_install_cached_hands_the_cache_to_the_library(tmp_path, monkeypatch):
     cache = tmp_path / "tune.json"
     cache.write_text(json.dumps({

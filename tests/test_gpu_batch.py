@@ -243,3 +243,39 @@ def test_compact_items_give_identical_dems():
     finally:
         for k, v in old.items():
             autotune.set_tuning(k, v)
+
+
+def test_overflow_list_and_workspace_grow_on_demand():
+    """A batch's sort overflow list and oversized-bucket workspace start
+    small (overflow_start 64) and grow, compiling again, when 64-slot
+    buckets of thousands of nodes need more: the DEMs equal each circuit
+    compiled alone, in both item layouts, on the first compile and on the
+    next ones (which need no growth)."""
+    from greenpeas_amd import autotune
+
+    texts = [read_circuit(n) for n in ("bb12", "surface5", "bb6")]
+    want = [solo(t) for t in texts]
+    keys = ("overflow_start", "bucket_fill", "bucket_cap", "compact_items")
+    old = {k: autotune.get_tuning(k) for k in keys}
+    try:
+        autotune.set_tuning("overflow_start", 64)
+        autotune.set_tuning("bucket_fill", 4096)
+        autotune.set_tuning("bucket_cap", 64)
+        for compact in (0, 2):
+            print(f"compact_items {compact}", flush=True)
+            autotune.set_tuning("compact_items", compact)
+            # (nearly every node lies past its bucket's 64 slots, far more
+            # than the list's start of one in 16, and buckets of ~4k nodes
+            # need the workspace: the first compile is right only if both grew)
+            bc = BatchCompiler()
+            bc.set_circuits(texts)
+            got = bc.compile()
+            assert got == want
+            assert bc.compile() == want  # (plans the split sweep: its buffers)
+            held = bc.device_bytes()
+            assert bc.compile() == want
+            assert bc.device_bytes() == held
+            bc.close()
+    finally:
+        for k, v in old.items():
+            autotune.set_tuning(k, v)
