@@ -199,45 +199,6 @@ __device__ __forceinline__ void waveBitonicSort(uint64_t (&h)[R], uint32_t (&r)[
   }
 }
 
-// Bitonic sort of 64*R packed 64-bit keys, R per lane: one 64-bit compare
-// and one 64-bit exchange per step (the bucket stage's common case, where
-// the key packs the hash bits below the bucket digit, the circuit-local
-// rank and the slot index).  Keys are distinct except identical padding.
-template <int R>
-__device__ __forceinline__ void waveBitonicSort64(uint64_t (&key)[R], uint32_t lane) {
-  constexpr uint32_t N = 64 * R;
-#pragma unroll
-  for (uint32_t k = 2; k <= N; k <<= 1) {
-#pragma unroll
-    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-      if (j >= 64) {
-        const uint32_t js = j >> 6;
-#pragma unroll
-        for (int s = 0; s < R; ++s) {
-          const int sp = s ^ static_cast<int>(js);
-          if (sp > s) {
-            const uint32_t e = static_cast<uint32_t>(s) * 64 + lane;
-            const bool asc = (e & k) == 0;
-            if ((key[sp] < key[s]) == asc) {
-              const uint64_t t = key[s];
-              key[s] = key[sp];
-              key[sp] = t;
-            }
-          }
-        }
-      } else {
-#pragma unroll
-        for (int s = 0; s < R; ++s) {
-          const uint32_t e = static_cast<uint32_t>(s) * 64 + lane;
-          const uint64_t ko = __shfl_xor(key[s], static_cast<int>(j));
-          const bool keepMin = ((lane & j) == 0) == ((e & k) == 0);
-          if ((ko < key[s]) == keepMin) key[s] = ko;
-        }
-      }
-    }
-  }
-}
-
 // State carried from one 64-element chunk of a bucket to the next: the open
 // run's merged probability and second hash, the last key, run and target
 // counts so far.
@@ -1490,6 +1451,42 @@ __device__ __forceinline__ T prevElement(const T (&v)[R], int s, uint32_t lane, 
 // records are requested as soon as their slots arrive, so that gather's
 // latency passes during the sort; slots and records then meet in the wave's
 // LDS by slot position, where the merge reads them in sorted order.
+// Sorts a wave's 64*R packed 64-bit keys (key[s] is entry s * 64 + lane;
+// the bucket stage's common case, where the key packs the hash bits below
+// the bucket digit, the circuit-local rank and the slot index, so the cnt
+// real keys are distinct and entries past them hold ~0, above all of them):
+// each key's sorted position is the number of real keys below it, counted
+// against every real key read back from LDS by broadcast -- independent
+// reads and two VALU operations per comparison, no exchange chain -- then
+// the keys are written to their positions and read back in order.  Uses
+// the .b halves of the wave's SlotRec entries (rewritten by the caller).
+template <int R>
+__device__ __forceinline__ void waveRankSort64(uint64_t (&key)[R], uint32_t lane, uint32_t cnt, SlotRec *pay) {
+  typedef uint64_t __attribute__((may_alias)) u64_alias;
+#pragma unroll
+  for (int s = 0; s < R; ++s)
+    *reinterpret_cast<u64_alias *>(&pay[static_cast<uint32_t>(s) * 64 + lane].b.x) = key[s];
+  waveLdsSync();
+  uint32_t rank[R];
+#pragma unroll
+  for (int s = 0; s < R; ++s) rank[s] = 0;
+#pragma unroll 8
+  for (uint32_t j = 0; j < cnt; ++j) {
+    const uint64_t kj = *reinterpret_cast<const u64_alias *>(&pay[j].b.x);
+#pragma unroll
+    for (int s = 0; s < R; ++s) rank[s] += kj < key[s] ? 1u : 0u;
+  }
+#pragma unroll
+  for (int s = 0; s < R; ++s)
+    if (static_cast<uint32_t>(s) * 64 + lane < cnt) *reinterpret_cast<u64_alias *>(&pay[rank[s]].b.z) = key[s];
+  waveLdsSync();
+#pragma unroll
+  for (int s = 0; s < R; ++s) {
+    const uint32_t e = static_cast<uint32_t>(s) * 64 + lane;
+    key[s] = e < cnt ? *reinterpret_cast<const u64_alias *>(&pay[e].b.z) : ~0ull;
+  }
+}
+
 template <int R>
 __device__ void bucketWave(uint32_t b, uint32_t bs, uint32_t ob, uint32_t cnt, uint32_t lane, const uint4 first,
                            const uint4 *__restrict__ pslot, const uint4 *__restrict__ info, SlotRec *pay,
@@ -1525,7 +1522,10 @@ __device__ void bucketWave(uint32_t b, uint32_t bs, uint32_t ob, uint32_t cnt, u
   }
   // every item of a bucket comes from one circuit (lane 0's item exists)
   const bool packed = (__builtin_amdgcn_readfirstlane(meta) & kLocalRankNone) != kLocalRankNone;
-  if (packed) waveBitonicSort64<R>(key, lane);
+  if (packed) {
+    waveRankSort64<R>(key, lane, cnt, pay);
+    waveLdsSync();  // (the sorted keys are read before the records overwrite them)
+  }
 #pragma unroll
   for (int s = 0; s < R; ++s) {
     const uint32_t e = static_cast<uint32_t>(s) * 64 + lane;
