@@ -1269,14 +1269,16 @@ __device__ __forceinline__ void forEachTargetScan(uint32_t numCols, const int2 *
 // bits, FNV-1a-64 over it padded with 0xFFFFFFFF to 32 slots (lists longer
 // than 32 hash all entries, unpadded), a second hash and the class size.
 // The node then takes a slot of its sort bucket (first radix digit = top
-// bucketBits hash bits): slot pos of bucket b is pslot[b * kBucketCap +
-// pos], one 16-byte {hash, rank} record (one write per node), pos from an
-// atomic counter per bucket; nodes beyond a bucket's kBucketCap slots go to
-// the overflow list.  In circuits whose targets fit 16 bits (at most
-// kStashMaxCols columns), the first kStashTargets targets of every node are
-// also stored as one 64-byte row per node (`stash`, 16-bit entries; 32
-// targets cover every class of the reference's circuits, 16 would leave
-// 1.1 % of bb12 L2's to the much slower gather from S), so the emit stage
+// bucketBits hash bits): slot pos of bucket b is pslot[b * cap + pos], one
+// 16-byte {hash, rank} record (one write per node), pos from an atomic
+// counter per bucket; nodes beyond a bucket's cap slots go to the overflow
+// list.  In circuits whose targets fit 16 bits (at most kStashMaxCols
+// columns), the first 2 W targets of every node are also stored as one
+// row of W words per node (`stash`, 16-bit entries; W = 1 << stashShift,
+// at most kStashWords: 32 targets cover every class of the reference's
+// circuits, 16 would leave 1.1 % of bb12 L2's to the much slower gather
+// from S; a load's rows take the narrowest width that holds its largest
+// class once a run has shown it), so the emit stage
 // copies a class's targets from its
 // representative's row instead of gathering them from S one column at a
 // time: each wave assembles its 64 rows in LDS and writes them out
