@@ -775,7 +775,7 @@ template <int RPT, bool GSEEDS>
 __global__ __launch_bounds__(kSweepMaxThreads) void sweepKernel(
     const VDesc *__restrict__ desc, const uint4 *__restrict__ items, const uint4 *__restrict__ act,
     const uint32_t *__restrict__ seedVals, const uint4 *__restrict__ colRegion, int2 *__restrict__ band,
-    uint32_t *__restrict__ S, uint32_t numItems, uint32_t *__restrict__ flags, SplitIo sp) {
+    uint32_t *__restrict__ S, uint32_t numItems, uint32_t *__restrict__ flags, SplitIo sp, uint32_t oneCircuit) {
   // dynamic LDS only, addressed from 0:
   // ring[2P] | zero word | seeds (unless GSEEDS) | vote[2] | (resumed
   // chunks: the entering-state chain's words)
@@ -783,7 +783,9 @@ __global__ __launch_bounds__(kSweepMaxThreads) void sweepKernel(
   if (blockIdx.x == 0 && tid < kFlagBandOverflow) flags[tid] = 0u;
   if (blockIdx.x >= numItems + sp.numItems) return;
   // the item, then the circuit's descriptor (and the split's): two round
-  // trips before the seed, state and record loads
+  // trips before the seed, state and record loads -- one in a load of one
+  // circuit, whose descriptor is requested with the item
+  const VDesc d0 = desc[0];
   const bool plain = blockIdx.x < numItems;
   const uint4 it = plain ? items[blockIdx.x] : sp.items[blockIdx.x - numItems];
   uint32_t v, c, mode, chunk = 0, group = 0, si;
@@ -806,7 +808,8 @@ __global__ __launch_bounds__(kSweepMaxThreads) void sweepKernel(
   mode = __builtin_amdgcn_readfirstlane(mode);
   SplitView sd{0u};  // loaded in split modes only
   if (mode != kSplitNone) sd.w = (tid & 63) < sizeof(SplitDesc) / 4 ? reinterpret_cast<const uint32_t *>(sp.descs + si)[tid & 63] : 0u;
-  const VDesc d = desc[v];
+  VDesc d = d0;
+  if (!oneCircuit) d = desc[v];
   if (mode == kSplitNone && hi < 0) {
     if (tid == 0) band[d.colOff + c] = make_int2(0x7FFFFFFF, -1);
     return;
@@ -1315,7 +1318,8 @@ struct RankIn {
 template <bool TABLE, int BATCH>
 __global__ __launch_bounds__(kClassBlock) void classifyKernel(
     const VDesc *__restrict__ desc, const uint4 *__restrict__ tiles, const int2 *__restrict__ band,
-    const int32_t *__restrict__ colAdj, const uint32_t *__restrict__ S, RankIn rin, ClassifyOut o) {
+    const int32_t *__restrict__ colAdj, const uint32_t *__restrict__ S, RankIn rin, ClassifyOut o,
+    uint32_t oneCircuit) {
   // per wave: 64 rows of kStashWords words, padded to kStashWords + 1 so
   // that a lane's writes and the row-major read-out avoid bank conflicts
   __shared__ uint32_t rows[kClassBlock / 64][64 * (kStashWords + 1)];
@@ -1325,8 +1329,11 @@ __global__ __launch_bounds__(kClassBlock) void classifyKernel(
   // tile: {circuit, first rank, lowest layer of its ranks' rows, layers
   // spanned by them}; the host picks TABLE only when every workgroup's
   // table fits kMaskTableWords
+  // (a load of one circuit: its descriptor requested with the tile)
+  const VDesc d0 = desc[0];
   const uint4 t = tiles[blockIdx.x];
-  const VDesc d = desc[t.x];
+  VDesc d = d0;
+  if (!oneCircuit) d = desc[t.x];
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t i = t.y + threadIdx.x;
   // this node's frame rows and probability, loaded before the tables are built
@@ -3331,7 +3338,8 @@ void Pipeline::run(bool timed) {
     const size_t lds = m.nSplit ? m.splitLdsBytes : m.sweepLdsBytes;
 #define GPA_SWEEP(RPT, G)                                                                                  \
   sweepKernel<RPT, G><<<std::max<uint32_t>(m.nItems + sp.numItems, 1), m.sweepThreads, lds, st>>>(          \
-      desc, items, act, seed, colRegion, m.band.as<int2>(), m.S.as<uint32_t>(), m.nItems, m.flags(), sp)
+      desc, items, act, seed, colRegion, m.band.as<int2>(), m.S.as<uint32_t>(), m.nItems, m.flags(), sp, \
+      m.V == 1 ? 1u : 0u)
     if (m.seedsInLds) {
       switch (m.rpt) {
       case 1: GPA_SWEEP(1, false); break;
@@ -3359,13 +3367,13 @@ void Pipeline::run(bool timed) {
                    static_cast<uint32_t>(m.ovCap)};
     if (m.classTable && gMaskTable.load() && m.classBatch == 16)
       classifyKernel<true, 16><<<m.nTiles, kClassBlock, 0, st>>>(desc, tiles, m.band.as<int2>(), colAdj,
-                                                                 m.S.as<uint32_t>(), rin, co);
+                                                                 m.S.as<uint32_t>(), rin, co, m.V == 1 ? 1u : 0u);
     else if (m.classTable && gMaskTable.load())
       classifyKernel<true, 4><<<m.nTiles, kClassBlock, 0, st>>>(desc, tiles, m.band.as<int2>(), colAdj,
-                                                                m.S.as<uint32_t>(), rin, co);
+                                                                m.S.as<uint32_t>(), rin, co, m.V == 1 ? 1u : 0u);
     else
       classifyKernel<false, 4><<<m.nTiles, kClassBlock, 0, st>>>(desc, tiles, m.band.as<int2>(), colAdj,
-                                                                 m.S.as<uint32_t>(), rin, co);
+                                                                 m.S.as<uint32_t>(), rin, co, m.V == 1 ? 1u : 0u);
   }
   mark(2);
   RunOut o{m.prank.as<uint32_t>(),
